@@ -1,0 +1,209 @@
+/*
+ * sfx.h -- C ABI of the MI355X-native SFINCS explicit time step.
+ *
+ * The library replaces the hot path of SFINCS (momentum flux update on uv-points,
+ * continuity water-level update on cells, CFL minimum-dt reduction) together with
+ * the per-step forcing it needs, behind plain C entry points: no C++ types, no torch
+ * types, plain pointers and sizes. Every entry point returns an int status:
+ *   0  = ok,
+ *   1  = instability stop (the reference's error=1, sfincs_lib.f90:634-648),
+ *  <0  = error; the message is available from sfx_last_error().
+ *
+ * Index tables exported by sfx_model_get_table() carry the reference's 1-based
+ * values (dummy uv index = npuv+ncuv+1), stored in 0-based C arrays.
+ * State fields are exchanged in the reference's element order (active cells
+ * level-major then column-major; uv-points per cell right-u then above-v,
+ * followed by the ncuv combined points).
+ *
+ * Threading: one host thread per handle; handles are independent.
+ */
+#ifndef SFX_H
+#define SFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFX_OK            0
+#define SFX_UNSTABLE      1
+#define SFX_ERR_ARG      -1
+#define SFX_ERR_STATE    -2
+#define SFX_ERR_HIP      -3
+#define SFX_ERR_MEMORY   -4
+#define SFX_ERR_IO       -5
+
+/* Physics switches and constants, named after the sfincs.inp keywords.
+ * Defaults (sfx_params_default) are the reference's: sfincs_input.f90:50-343,
+ * derived flags :347-745, constants g=9.81, pi=3.14159 (:372-373). */
+typedef struct sfx_params {
+  float t0, t1;              /* simulation start/stop in seconds (tstart/tstop - tref) */
+  float tspinup;             /* absolute; default t0 (sfincs_input.f90:369) */
+  float alpha;               /* CFL factor, default 0.5 */
+  float theta;               /* 1.0 = no theta smoothing */
+  float hmin_cfl;            /* 0.1 */
+  float manning;             /* 0.04 */
+  float manning_land;        /* -999 = use manning */
+  float manning_sea;         /* -999 = use manning */
+  float rgh_lev_land;        /* 0.0 */
+  float zsini;               /* 0.0 */
+  float dtmax;               /* 60.0 */
+  float huthresh;            /* 0.05 (forced to 0 for subgrid tables, sfincs_subgrid.F90:40) */
+  float huvmin;              /* 0.0 */
+  float rhoa, rhow;          /* 1.25, 1024 */
+  float advlim;              /* 1.0 */
+  float slopelim;            /* 9999.9 */
+  float btfilter, btrelax;   /* 60, 3600 */
+  float wiggle_factor;       /* 0.1 */
+  float wiggle_threshold;    /* 0.1 */
+  float uvlim, uvmax;        /* 10, 1000 */
+  float nuvisc;              /* nuviscdim 0.01 */
+  float nuviscfac;           /* 100 */
+  float latitude;            /* 0 -> no Coriolis */
+  float g, pi;               /* 9.81, 3.14159 */
+  int32_t advection;         /* 1 */
+  int32_t advection_scheme;  /* 1 = upw1, 0 = original */
+  int32_t viscosity;         /* 0 (code default, sfincs_input.f90:111) */
+  int32_t coriolis;          /* 1, switched off at |latitude| < 0.01 */
+  int32_t friction2d;        /* 1 */
+  int32_t advection_mask;    /* 1 */
+  int32_t wiggle_suppression;/* 1 (used by subgrid only) */
+  int32_t store_zsmax;       /* 1 (dtmaxout > 0 by default) */
+  int32_t bndtype;           /* 1 = weakly reflective */
+  int32_t h73table;          /* 0 */
+} sfx_params;
+
+void sfx_params_default(sfx_params* p);
+
+/* Quadtree mesh in the reference's quadtree-file semantics (sfincs_quadtree.F90:210-292,
+ * 295-442): points sorted level-major then by nm; md/mu/nd/nu flags (-1 coarser, 0 same,
+ * 1 finer); md1..nu2 1-based point indices (0 = none); mask per point. */
+typedef struct sfx_quadtree_input {
+  int32_t nr_points, nr_levels;
+  float x0, y0, dx, dy, rotation_deg;
+  const int8_t  *level, *md, *mu, *nd, *nu, *mask;
+  const int32_t *n, *m, *md1, *md2, *mu1, *mu2, *nd1, *nd2, *nu1, *nu2;
+} sfx_quadtree_input;
+
+/* Subgrid tables in the reference's NetCDF semantics (sfincs_subgrid.F90:57-455), given
+ * per active cell / active uv-point in sfincs order; level-fastest 2-D arrays. */
+typedef struct sfx_subgrid_input {
+  int32_t nlevels;
+  const float *z_zmin, *z_zmax, *z_volmax, *z_level;               /* np, np, np, np*nl */
+  const float *uv_zmin, *uv_zmax, *uv_navg, *uv_ffit;              /* npuv each */
+  const float *uv_havg, *uv_nrep, *uv_pwet;                        /* npuv*nl each */
+} sfx_subgrid_input;
+
+typedef struct sfx_sizes {
+  int32_t np, npuv, ncuv, nref, nlevels, nmax, mmax;
+  int32_t ngbnd, nkcuv2, nsrc, nbnd;
+  int32_t subgrid, quadtree, dense;   /* dense = full rectangle, implicit neighbours on device */
+  float dtmax, dtmin;
+} sfx_sizes;
+
+/* ---- host-side model (CPU only; no GPU needed) -------------------------------------
+ * Replaces the table/mask builders of sfincs_initialize: initialize_mesh
+ * (sfincs_domain.f90:143-1586), make_quadtree_from_indices (sfincs_quadtree.F90:445-669),
+ * initialize_bathymetry (:1588-1744), initialize_boundaries (:1765-1958),
+ * initialize_roughness (:1960-2061), set_advection_mask (:2114-2180),
+ * initialize_hydro (:2183-2410), read_subgrid_file sanitize (sfincs_subgrid.F90:57-455),
+ * compute_initial_subgrid_volumes (:907-962), find_boundary_indices
+ * (sfincs_boundaries.f90:455-679), read_discharges cell lookup (sfincs_discharges.f90:159-171),
+ * the Cd table (sfincs_meteo.f90:316-328). */
+typedef struct sfx_model sfx_model;
+
+/* Regular grid: msk is nmax*mmax, n fastest (the reference's column-major kcsg(n,m)). */
+int  sfx_model_create_regular(const sfx_params* p, int32_t nmax, int32_t mmax,
+                              float x0, float y0, float dx, float dy, float rotation_deg,
+                              const int8_t* msk, sfx_model** out);
+int  sfx_model_create_quadtree(const sfx_params* p, const sfx_quadtree_input* qt, sfx_model** out);
+int  sfx_model_sizes(const sfx_model* m, sfx_sizes* s);
+/* Copy a table by its reference name (e.g. "uv_index_z_nm", "kcuv", "mask_adv", "zbuvmx",
+ * "z_index_uv_md", "cuv_index_uv1", "dxrinv"); nbytes must equal the table size. */
+int  sfx_model_get_table(const sfx_model* m, const char* name, void* dst, size_t nbytes);
+int  sfx_model_table_bytes(const sfx_model* m, const char* name, size_t* nbytes);
+int  sfx_model_set_bathymetry(sfx_model* m, const float* zb_active);           /* np values */
+int  sfx_model_set_subgrid(sfx_model* m, const sfx_subgrid_input* sg);
+/* bzs: water level boundaries (bnd/bzs files): zs is nt*nbnd, point fastest. */
+int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const float* y,
+                            int32_t nt, const float* t, const float* zs);
+/* src/dis: point discharges; q is nt*nsrc, source fastest. */
+int  sfx_model_set_sources(sfx_model* m, int32_t nsrc, const float* x, const float* y,
+                           int32_t nt, const float* t, const float* q);
+/* prcfile: uniform rain series in mm/h. */
+int  sfx_model_set_rain(sfx_model* m, int32_t nt, const float* t, const float* mmh);
+/* wndfile: uniform wind series, speed m/s and nautical direction in degrees. */
+int  sfx_model_set_wind(sfx_model* m, int32_t nt, const float* t, const float* mag, const float* dir);
+/* Finish initialisation (kcuv, roughness, initial conditions, volumes, advection mask). */
+int  sfx_model_initialize(sfx_model* m);
+/* Initial state by name ("zs" f64, "q" f32, "uv" f32, "kfuv" i8, "z_volume" f64, ...). */
+int  sfx_model_get_state(const sfx_model* m, const char* name, void* dst, size_t nbytes);
+void sfx_model_destroy(sfx_model* m);
+
+/* ---- device instance ---------------------------------------------------------------
+ * sfx_create replaces initialize_openacc (sfincs_openacc.f90:9-45): all tables and state
+ * move to HBM once; sfx_destroy replaces finalize_openacc (:47-79). */
+typedef struct sfx_handle sfx_handle;
+
+typedef struct sfx_step_stats {
+  double t;          /* model time after the call */
+  float  dt;         /* last dt */
+  float  min_dt;     /* CFL minimum (without alpha) reduced in the last momentum sweep */
+  int64_t nt;        /* steps taken since creation */
+  int32_t error;     /* 1 = instability stop */
+  double cell_updates; /* sum over steps of np */
+  double seconds;      /* wall time of the call */
+} sfx_step_stats;
+
+int  sfx_create(const sfx_model* m, int32_t device, sfx_handle** out);
+int  sfx_destroy(sfx_handle* h);
+/* State exchange in reference order; the explicit form of `acc update host/device`. */
+int  sfx_get_state(sfx_handle* h, const char* name, void* dst, size_t nbytes);
+int  sfx_set_state(sfx_handle* h, const char* name, const void* src, size_t nbytes);
+/* compute_fluxes(dt) (sfincs_momentum.f90:9-784): one momentum sweep with the given dt,
+ * including the combined-uv averaging; *min_dt receives the CFL reduction (without alpha). */
+int  sfx_compute_fluxes(sfx_handle* h, float dt, float* min_dt);
+/* compute_water_levels(t, dt) (sfincs_continuity.f90:5-43), point sources included. */
+int  sfx_compute_water_levels(sfx_handle* h, double t, float dt);
+/* The per-step companions of sfincs_update (sfincs_lib.f90:517-550): meteo series,
+ * boundary points/conditions/fluxes, discharges, at time t with step dt. */
+int  sfx_update_forcing(sfx_handle* h, double t, float dt);
+/* sfincs_update(dtrange) (sfincs_lib.f90:337-683): dtrange < -999 runs to t1, dtrange <= 0
+ * one step, dtrange > 0 runs to t + dtrange. Device-resident loop, graph-captured. */
+int  sfx_update(sfx_handle* h, double dtrange, sfx_step_stats* st);
+int  sfx_get_time(sfx_handle* h, double* t, float* dt, int64_t* nt);
+/* Per-branch uv-point counters of the last sweeps (dry, wet-regular, subgrid-full,
+ * subgrid-partial), enabled by sfx_set_counting(h,1); used for the byte model. */
+int  sfx_set_counting(sfx_handle* h, int32_t on);
+int  sfx_get_counters(sfx_handle* h, int64_t counts[4]);
+/* FP64 volume accumulator: total water volume (sum of zs*A or V) in double. */
+int  sfx_total_volume(sfx_handle* h, double* vol);
+
+/* ---- batched ensembles ---------------------------------------------------------------
+ * No counterpart in the reference (it runs scenarios one at a time, docs/output.rst:19-20).
+ * Members of uneven size advance together: each launch covers an (instance, tile) work
+ * list, each member keeps its own adaptive dt and end time. */
+typedef struct sfx_batch sfx_batch;
+int  sfx_batch_create(int32_t device, sfx_batch** out);
+int  sfx_batch_destroy(sfx_batch* b);
+int  sfx_batch_add(sfx_batch* b, sfx_handle* h, double t_end);
+int  sfx_batch_remove(sfx_batch* b, sfx_handle* h);
+/* Advance every active member by up to nsteps steps (graph-captured). *n_active gets the
+ * number of members still short of their end time. */
+int  sfx_batch_advance(sfx_batch* b, int32_t nsteps, int32_t* n_active);
+int  sfx_batch_done(sfx_batch* b, sfx_handle* h, int32_t* done);
+/* HBM footprint of a model once on the device, for admission control. */
+int  sfx_model_footprint(const sfx_model* m, size_t* bytes);
+
+/* Autotuner: times candidate block/tile shapes on this handle's mesh and keeps the best. */
+int  sfx_autotune(sfx_handle* h, int32_t trials);
+int  sfx_get_tuning(sfx_handle* h, int32_t out[4]);
+
+int  sfx_last_error(char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,323 @@
+"""ORACLE -- test infrastructure only.
+
+Python glue for the two CPU checkers:
+  * OracleRun: the C restatement (oracle/liboracle.so) of the whole step
+    (sfincs_lib.f90:337-683 with its forcing companions and hot path).
+  * RefRun: the reference's own compute_fluxes / compute_water_levels
+    (oracle/_ref/libsfincs_ref.so, compiled from /root/reference by oracle/Makefile),
+    driven step by step with the restated forcing of OracleRun ("hybrid" loop).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this.
+Both runs take their tables from a host model built by libsfx (sfincs_amd._lib.Model);
+those tables are pinned separately (closed forms and invariants in tests/).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libsfincs_ref.so")
+
+CELL_I8 = ("kcs", "z_flags_iref")
+TABLES = (
+    "kcs", "z_flags_iref", "z_index_uv_md", "z_index_uv_mu", "z_index_uv_nd", "z_index_uv_nu",
+    "cell_area", "dxrm", "dyrm", "dxrinv", "dyrinv", "dxr2inv", "dyr2inv", "dxrinvc", "dyrinvc", "nuvisc",
+    "uv_index_z_nm", "uv_index_z_nmu", "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm", "uv_index_u_num",
+    "uv_index_v_ndm", "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
+    "cuv_index_uv2", "kcuv", "mask_adv", "uv_flags_iref", "uv_flags_type", "uv_flags_dir",
+    "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp", "ind2_bnd_gbp", "nmi_gbp",
+    "ibuvdir", "fac_bnd_gbp", "t_bnd", "zs_bnd", "nmindsrc", "tsrc", "qsrc", "tprcpt", "tprcpv", "twnd",
+    "wndmag", "wnddir", "cdval",
+)
+REGULAR_TABLES = ("zb", "zbuv", "zbuvmx", "gn2uv")
+SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax", "uv_navg_w", "uv_fnfit",
+                  "uv_havg", "uv_nrep", "uv_pwet", "uv_havg_zmax", "uv_nrep_zmax")
+
+
+def build():
+    """Compile liboracle.so (and oracle/_ref when /root/reference is present)."""
+    subprocess.check_call(["make", "-C", HERE, "oracle"])
+    if os.path.isdir("/root/reference/source/src"):
+        subprocess.check_call(["make", "-C", HERE, "ref"])
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (make -C oracle)")
+    return C.CDLL(path)
+
+
+_olib = None
+
+
+def olib():
+    global _olib
+    if _olib is None:
+        _olib = _load(ORACLE_LIB)
+        _olib.oracle_new.restype = C.c_void_p
+        _olib.oracle_free.argtypes = [C.c_void_p]
+        _olib.oracle_bind.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        _olib.oracle_set.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        _olib.oracle_get.argtypes = [C.c_void_p, C.c_char_p]
+        _olib.oracle_get.restype = C.c_double
+        _olib.oracle_compute_fluxes.argtypes = [C.c_void_p, C.c_float]
+        _olib.oracle_compute_fluxes.restype = C.c_float
+        _olib.oracle_compute_water_levels.argtypes = [C.c_void_p, C.c_double, C.c_float]
+        for f in ("oracle_update_meteo", "oracle_update_boundaries", "oracle_update_discharges"):
+            getattr(_olib, f).argtypes = [C.c_void_p, C.c_double, C.c_float]
+        _olib.oracle_update.argtypes = [C.c_void_p, C.c_double, C.c_int32]
+        _olib.oracle_update.restype = C.c_int32
+    return _olib
+
+
+def x73_table():
+    """h**(7/3) table of fill_h73_tables (sfincs_data.f90:858-881), restated in float32."""
+    x = np.empty((10, 1000), np.float32)
+    for k in range(1, 11):
+        p = np.float32(10.0) ** np.float32(k - 6) if k >= 6 else np.float32(1.0) / (np.float32(10.0) ** np.float32(6 - k))
+        for j in range(1, 1001):
+            hh = np.float32(np.float32(0.001) * np.float32(j)) * np.float32(p)
+            x[k - 1, j - 1] = np.float32(hh * hh) * np.float32(np.power(hh, np.float32(1.0) / np.float32(3.0)))
+    return x.reshape(-1)
+
+
+class OracleRun:
+    """CPU restatement of the step on the tables of a libsfx host model."""
+
+    def __init__(self, model):
+        L = olib()
+        self.L = L
+        self.m = C.c_void_p(L.oracle_new())
+        s = model.sizes()
+        self.sizes = s
+        self.keep = {}
+        names = list(TABLES) + list(SUBGRID_TABLES if s["subgrid"] else REGULAR_TABLES)
+        for n in names:
+            a = model.table(n)
+            self._bind(n, a)
+        # state (copies of the initial state)
+        for n in ("zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv"):
+            self._bind(n, model.table(n).copy())
+        if s["subgrid"]:
+            self._bind("z_volume", model.table("z_volume").copy())
+        nuv = s["npuv"] + s["ncuv"] + 1
+        self._bind("q0", np.zeros(nuv, np.float32))
+        self._bind("uv0", np.zeros(nuv, np.float32))
+        self._bind("tauwu", np.zeros(s["np"], np.float32))
+        self._bind("tauwv", np.zeros(s["np"], np.float32))
+        self._bind("netprcp", np.zeros(s["np"], np.float32))
+        self._bind("zsb", model.table("zsb").copy())
+        self._bind("zsb0", model.table("zsb0").copy())
+        self._bind("uvmean", model.table("uvmean").copy())
+        self._bind("zst_bnd", np.zeros(max(1, s["nbnd"]), np.float32))
+        self._bind("qtsrc", np.zeros(max(1, s["nsrc"]), np.float32))
+        p = model.params_dict if hasattr(model, "params_dict") else None
+        self.params = p
+        self._bind("x73", x73_table())
+
+    def _bind(self, name, a):
+        a = np.ascontiguousarray(a)
+        self.keep[name] = a
+        rc = self.L.oracle_bind(self.m, name.encode(), a.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise KeyError(name)
+
+    def set(self, name, v):
+        if self.L.oracle_set(self.m, name.encode(), float(v)) != 0:
+            raise KeyError(name)
+
+    def get(self, name):
+        return self.L.oracle_get(self.m, name.encode())
+
+    def configure(self, scenario_params, model):
+        """Scalars from sfx_params + derived switches (mirrors sfincs_input.f90:347-745)."""
+        from sfincs_amd import _lib
+        p = _lib.default_params(**scenario_params)
+        s = self.sizes
+        tb = model.table
+        lat = p.latitude
+        coriolis = bool(p.coriolis) and not (-0.01 < lat < 0.01)
+        fcorio = np.float32(2 * np.float32(7.2921e-05)) * np.float32(np.sin(np.float32(np.float32(lat * np.float32(p.pi)) / np.float32(180))))
+        vals = dict(np=s["np"], npuv=s["npuv"], ncuv=s["ncuv"], nref=s["nref"], nl=s["nlevels"],
+                    subgrid=s["subgrid"], use_quadtree=s["quadtree"], advection=p.advection,
+                    advection_scheme=p.advection_scheme, viscosity=p.viscosity, coriolis=int(coriolis),
+                    friction2d=p.friction2d, thetasmoothing=int(p.theta < np.float32(0.9999)),
+                    wiggle_suppression=p.wiggle_suppression, store_zsmax=p.store_zsmax,
+                    wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
+                    ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
+                    nsrc=s["nsrc"], ntsrc=len(tb("tsrc")), ntprcp=len(tb("tprcpt")), ntwnd=len(tb("twnd")),
+                    g=p.g, pi=p.pi, huthresh=(0.0 if s["subgrid"] else p.huthresh), huvmin=p.huvmin,
+                    advlim=p.advlim, slopelim=p.slopelim, uvlim=p.uvlim, nuviscfac=p.nuviscfac, fcorio=fcorio,
+                    theta=p.theta, wiggle_factor=p.wiggle_factor, wiggle_threshold=p.wiggle_threshold,
+                    dtmax=s["dtmax"], dtmin=s["dtmin"], alpha=p.alpha, rhoa=p.rhoa, rhow=p.rhow,
+                    btfilter=p.btfilter, btrelax=p.btrelax, t0=p.t0, t1=p.t1,
+                    tspinup=np.float32(p.t0) + np.float32(p.tspinup), zini=p.zsini,
+                    dx=tb("dxrm")[0], dy=tb("dyrm")[0], t=float(p.t0))
+        for k, v in vals.items():
+            self.set(k, v)
+        self.vals = vals
+        return self
+
+    # forcing + hot path pieces
+    def compute_fluxes(self, dt):
+        return self.L.oracle_compute_fluxes(self.m, dt)
+
+    def compute_water_levels(self, t, dt):
+        self.L.oracle_compute_water_levels(self.m, t, dt)
+
+    def update_forcing(self, t, dt):
+        if self.vals["wind"] or self.vals["precip"]:
+            self.L.oracle_update_meteo(self.m, t, dt)
+        self.L.oracle_update_boundaries(self.m, t, dt)
+        self.L.oracle_update_discharges(self.m, t, dt)
+
+    def update(self, tend, max_steps=0):
+        return self.L.oracle_update(self.m, tend, max_steps)
+
+    def state(self, name):
+        return self.keep[name]
+
+    def close(self):
+        if self.m:
+            self.L.oracle_free(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ reference library
+_rlib = None
+
+
+def rlib():
+    global _rlib
+    if _rlib is None:
+        _rlib = _load(REF_LIB)
+        _rlib.ref_set_sizes.argtypes = [C.c_int32] * 5
+        _rlib.ref_set_scalar.argtypes = [C.c_char_p, C.c_double]
+        for f in ("ref_set_i32", "ref_set_i8", "ref_set_f32", "ref_set_f64"):
+            getattr(_rlib, f).argtypes = [C.c_char_p, C.c_void_p, C.c_int32]
+        _rlib.ref_get.argtypes = [C.c_char_p, C.c_void_p, C.c_int64]
+        _rlib.ref_compute_fluxes.argtypes = [C.c_float]
+        _rlib.ref_compute_fluxes.restype = C.c_float
+        _rlib.ref_compute_water_levels.argtypes = [C.c_double, C.c_float]
+        _rlib.ref_timers.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _rlib.ref_read_input.argtypes = [C.c_void_p, C.c_int32]
+    return _rlib
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+REF_I32 = ("uv_index_z_nm", "uv_index_z_nmu", "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm",
+           "uv_index_u_num", "uv_index_v_ndm", "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu",
+           "cuv_index_uv", "cuv_index_uv1", "cuv_index_uv2", "z_index_uv_md", "z_index_uv_mu", "z_index_uv_nd",
+           "z_index_uv_nu", "nmindsrc")
+REF_I8 = ("kcs", "kcuv", "mask_adv", "z_flags_iref", "uv_flags_iref", "uv_flags_type", "uv_flags_dir")
+REF_F32_TABLES = ("dxrinv", "dyrinv", "dxr2inv", "dyr2inv", "dxrinvc", "dyrinvc", "dxrm", "dyrm", "cell_area",
+                  "nuvisc")
+REF_SCALARS = ("subgrid", "use_quadtree", "advection", "advection_scheme", "viscosity", "coriolis", "friction2d",
+               "thetasmoothing", "wiggle_suppression", "store_zsmax", "wind", "precip", "h73table", "nsrc", "g",
+               "huthresh", "huvmin", "advlim", "slopelim", "uvlim", "nuviscfac", "fcorio", "theta",
+               "wiggle_factor", "wiggle_threshold", "dtmax", "rhow", "dx", "dy")
+
+
+class RefRun:
+    """The reference's compute_fluxes/compute_water_levels, driven by an OracleRun's state.
+
+    Each step: the oracle applies the restated forcing to its arrays, the arrays are
+    copied into the reference's module variables, the reference's routines run, and
+    the results are copied back. The reference library holds one model at a time."""
+
+    def __init__(self, orc):
+        R = rlib()
+        self.R = R
+        self.o = orc
+        v = orc.vals
+        R.ref_set_sizes(v["np"], v["npuv"], v["ncuv"], v["nref"], max(v["nl"], 1))
+        for k in REF_SCALARS:
+            R.ref_set_scalar(k.encode(), float(v[k]))
+        k = orc.keep
+        for n in REF_I32:
+            if n in k and k[n].size:
+                R.ref_set_i32(n.encode(), k[n].ctypes.data, k[n].size)
+        for n in REF_I8:
+            R.ref_set_i8(n.encode(), k[n].ctypes.data, k[n].size)
+        names = list(REF_F32_TABLES)
+        names += ["z_zmin", "z_zmax", "z_volmax", "uv_zmin", "uv_zmax", "uv_navg_w", "uv_fnfit", "uv_havg_zmax",
+                  "uv_nrep_zmax", "z_dep", "uv_havg", "uv_nrep", "uv_pwet"] if v["subgrid"] else ["zb", "zbuvmx",
+                                                                                                  "gn2uv"]
+        for n in names:
+            R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
+        if v["h73table"]:
+            R.ref_fill_h73()
+        for n in ("q0", "uv0"):
+            R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
+        self.push_state()
+
+    def push_state(self):
+        R, k = self.R, self.o.keep
+        R.ref_set_f64(b"zs", k["zs"].ctypes.data, k["zs"].size)
+        if self.o.vals["subgrid"]:
+            R.ref_set_f64(b"z_volume", k["z_volume"].ctypes.data, k["z_volume"].size)
+        for n in ("q", "uv", "zs0", "zsderv", "zsmax", "tauwu", "tauwv", "netprcp", "qtsrc"):
+            R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
+        R.ref_set_i8(b"kfuv", k["kfuv"].ctypes.data, k["kfuv"].size)
+
+    def pull_state(self):
+        R, k = self.R, self.o.keep
+        names = ["zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv"]
+        if self.o.vals["subgrid"]:
+            names.append("z_volume")
+        for n in names:
+            R.ref_get(n.encode(), k[n].ctypes.data, k[n].nbytes)
+
+    def step(self):
+        """One sfincs_update iteration with the reference hot path (sfincs_lib.f90:379-648)."""
+        o = self.o
+        mdt = np.float32(o.get("min_dt"))
+        alpha = np.float32(o.vals["alpha"])
+        dt = np.float32(alpha * mdt)
+        t = o.get("t") + float(dt)
+        nt = int(o.get("nt")) + 1
+        o.set("nt", nt)
+        o.set("dt", dt)
+        o.set("t", t)
+        o.update_forcing(t, dt)
+        self.push_state()
+        mdt_new = self.R.ref_compute_fluxes(dt)
+        self.R.ref_compute_water_levels(t, dt)
+        self.pull_state()
+        o.set("min_dt", mdt_new)
+        if dt < np.float32(o.vals["dtmin"]) and nt > 1:
+            o.set("error", 1)
+        return t, dt
+
+    def timers(self):
+        a, b = C.c_double(), C.c_double()
+        self.R.ref_timers(C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+def ref_read_input(workdir, text):
+    """Run the reference's read_sfincs_input on `text` in `workdir`; returns 41 values."""
+    R = rlib()
+    cwd = os.getcwd()
+    os.makedirs(workdir, exist_ok=True)
+    with open(os.path.join(workdir, "sfincs.inp"), "w") as f:
+        f.write(text)
+    out = np.zeros(41, np.float64)
+    try:
+        os.chdir(workdir)
+        R.ref_read_input(out.ctypes.data, 41)
+    finally:
+        os.chdir(cwd)
+    return out

@@ -1,0 +1,305 @@
+! ORACLE -- test infrastructure only (see oracle/sfx_oracle.c header).
+!
+! Thin C-callable driver around the reference's own hot-path modules, compiled from
+! /root/reference/source/src by oracle/Makefile into oracle/_ref/libsfincs_ref.so:
+!   sfincs_data        (module state the kernels read and write)
+!   sfincs_momentum    compute_fluxes            (sfincs_momentum.f90:9-784)
+!   sfincs_continuity  compute_water_levels      (sfincs_continuity.f90:5-43)
+!   sfincs_input       read_sfincs_input         (sfincs_input.f90:5-747, defaults)
+! The driver fills the module arrays from caller buffers, runs one routine and copies
+! results back. It contains no physics of its own.
+module ref_driver
+   use iso_c_binding
+   use sfincs_data
+   use sfincs_momentum
+   use sfincs_continuity
+   use sfincs_input
+   use sfincs_log
+   implicit none
+   real :: tloopflux = 0.0
+   real :: tloopcont = 0.0
+contains
+
+   function cstr(s) result(f)
+      character(kind=c_char), intent(in) :: s(*)
+      character(len=64) :: f
+      integer :: i
+      f = ' '
+      do i = 1, 64
+         if (s(i) == c_null_char) exit
+         f(i:i) = s(i)
+      enddo
+   end function
+
+   subroutine ref_set_sizes(np_, npuv_, ncuv_, nref_, nl_) bind(C, name='ref_set_sizes')
+      integer(c_int32_t), value :: np_, npuv_, ncuv_, nref_, nl_
+      np = np_; npuv = npuv_; ncuv = ncuv_; nref = int(nref_, 1); subgrid_nlevels = nl_
+      ! switches the hot path reads that this harness never turns on
+      crsgeo = .false.; patmos = .false.; snapwave = .false.; timestep_analysis = .false.
+      wave_enhanced_roughness = .false.; use_qext = .false.; wavemaker = .false.
+      store_t_zsmax = .false.; store_maximum_velocity = .false.; store_maximum_flux = .false.
+      store_twet = .false.; use_storage_volume = .false.; h73table = .false.
+      nsrcdrn = 0
+   end subroutine
+
+   subroutine ref_set_scalar(name, v) bind(C, name='ref_set_scalar')
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), value :: v
+      select case (trim(cstr(name)))
+      case ('subgrid');            subgrid = (v /= 0.0d0)
+      case ('use_quadtree');       use_quadtree = (v /= 0.0d0)
+      case ('advection');          advection = (v /= 0.0d0)
+      case ('advection_scheme');   advection_scheme = int(v)
+      case ('viscosity');          viscosity = (v /= 0.0d0)
+      case ('coriolis');           coriolis = (v /= 0.0d0)
+      case ('friction2d');         friction2d = (v /= 0.0d0)
+      case ('thetasmoothing');     thetasmoothing = (v /= 0.0d0)
+      case ('wiggle_suppression'); wiggle_suppression = (v /= 0.0d0)
+      case ('store_zsmax');        store_maximum_waterlevel = (v /= 0.0d0)
+      case ('wind');               wind = (v /= 0.0d0)
+      case ('precip');             precip = (v /= 0.0d0)
+      case ('h73table');           h73table = (v /= 0.0d0)
+      case ('nsrc');               nsrcdrn = int(v)
+      case ('g');                  g = real(v, 4)
+      case ('huthresh');           huthresh = real(v, 4)
+      case ('huvmin');             huvmin = real(v, 4)
+      case ('advlim');             advlim = real(v, 4)
+      case ('slopelim');           slopelim = real(v, 4)
+      case ('uvlim');              uvlim = real(v, 4)
+      case ('nuviscfac');          nuviscfac = real(v, 4)
+      case ('fcorio');             fcorio = real(v, 4)
+      case ('theta');              theta = real(v, 4)
+      case ('wiggle_factor');      wiggle_factor = real(v, 4)
+      case ('wiggle_threshold');   wiggle_threshold = real(v, 4)
+      case ('dtmax');              dtmax = real(v, 4)
+      case ('rhow');               rhow = real(v, 4)
+      case ('dx');                 dx = real(v, 4)
+      case ('dy');                 dy = real(v, 4)
+      case default
+      end select
+   end subroutine
+
+   subroutine ref_set_i32(name, p, n) bind(C, name='ref_set_i32')
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: p(n)
+      select case (trim(cstr(name)))
+      case ('uv_index_z_nm');   call put4(uv_index_z_nm)
+      case ('uv_index_z_nmu');  call put4(uv_index_z_nmu)
+      case ('uv_index_u_nmd');  call put4(uv_index_u_nmd)
+      case ('uv_index_u_nmu');  call put4(uv_index_u_nmu)
+      case ('uv_index_u_ndm');  call put4(uv_index_u_ndm)
+      case ('uv_index_u_num');  call put4(uv_index_u_num)
+      case ('uv_index_v_ndm');  call put4(uv_index_v_ndm)
+      case ('uv_index_v_nm');   call put4(uv_index_v_nm)
+      case ('uv_index_v_ndmu'); call put4(uv_index_v_ndmu)
+      case ('uv_index_v_nmu');  call put4(uv_index_v_nmu)
+      case ('cuv_index_uv');    call put4(cuv_index_uv)
+      case ('cuv_index_uv1');   call put4(cuv_index_uv1)
+      case ('cuv_index_uv2');   call put4(cuv_index_uv2)
+      case ('z_index_uv_md');   call put4(z_index_uv_md)
+      case ('z_index_uv_mu');   call put4(z_index_uv_mu)
+      case ('z_index_uv_nd');   call put4(z_index_uv_nd)
+      case ('z_index_uv_nu');   call put4(z_index_uv_nu)
+      case ('nmindsrc');        call put4(nmindsrc)
+      case default
+      end select
+   contains
+      subroutine put4(a)
+         integer*4, allocatable :: a(:)
+         if (allocated(a)) deallocate(a)
+         allocate(a(n))
+         a = p
+      end subroutine
+   end subroutine
+
+   subroutine ref_set_i8(name, p, n) bind(C, name='ref_set_i8')
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int32_t), value :: n
+      integer(c_int8_t), intent(in) :: p(n)
+      select case (trim(cstr(name)))
+      case ('kcs');           call put1(kcs)
+      case ('kcuv');          call put1(kcuv)
+      case ('kfuv');          call put1(kfuv)
+      case ('mask_adv');      call put1(mask_adv)
+      case ('z_flags_iref');  call put1(z_flags_iref)
+      case ('uv_flags_iref'); call put1(uv_flags_iref)
+      case ('uv_flags_type'); call put1(uv_flags_type)
+      case ('uv_flags_dir');  call put1(uv_flags_dir)
+      case default
+      end select
+   contains
+      subroutine put1(a)
+         integer*1, allocatable :: a(:)
+         if (allocated(a)) deallocate(a)
+         allocate(a(n))
+         a = p
+      end subroutine
+   end subroutine
+
+   subroutine ref_set_f32(name, p, n) bind(C, name='ref_set_f32')
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int32_t), value :: n
+      real(c_float), intent(in) :: p(n)
+      select case (trim(cstr(name)))
+      case ('zb');           call putr(zb)
+      case ('zbuvmx');       call putr(zbuvmx)
+      case ('gn2uv');        call putr(gn2uv)
+      case ('dxrinv');       call putr(dxrinv)
+      case ('dyrinv');       call putr(dyrinv)
+      case ('dxr2inv');      call putr(dxr2inv)
+      case ('dyr2inv');      call putr(dyr2inv)
+      case ('dxrinvc');      call putr(dxrinvc)
+      case ('dyrinvc');      call putr(dyrinvc)
+      case ('dxrm');         call putr(dxrm)
+      case ('dyrm');         call putr(dyrm)
+      case ('cell_area');    call putr(cell_area)
+      case ('nuvisc');       call putr(nuvisc)
+      case ('q');            call putr(q)
+      case ('q0');           call putr(q0)
+      case ('uv0');          call putr(uv0)
+      case ('uv');           call putr(uv)
+      case ('zs0');          call putr(zs0)
+      case ('zsderv');       call putr(zsderv)
+      case ('zsmax');        call putr(zsmax)
+      case ('tauwu');        call putr(tauwu)
+      case ('tauwv');        call putr(tauwv)
+      case ('netprcp');      call putr(netprcp)
+      case ('qtsrc');        call putr(qtsrc)
+      case ('z_zmin');       call putr(subgrid_z_zmin)
+      case ('z_zmax');       call putr(subgrid_z_zmax)
+      case ('z_volmax');     call putr(subgrid_z_volmax)
+      case ('uv_zmin');      call putr(subgrid_uv_zmin)
+      case ('uv_zmax');      call putr(subgrid_uv_zmax)
+      case ('uv_navg_w');    call putr(subgrid_uv_navg_w)
+      case ('uv_fnfit');     call putr(subgrid_uv_fnfit)
+      case ('uv_havg_zmax'); call putr(subgrid_uv_havg_zmax)
+      case ('uv_nrep_zmax'); call putr(subgrid_uv_nrep_zmax)
+      case ('z_dep');        call putr2(subgrid_z_dep)
+      case ('uv_havg');      call putr2(subgrid_uv_havg)
+      case ('uv_nrep');      call putr2(subgrid_uv_nrep)
+      case ('uv_pwet');      call putr2(subgrid_uv_pwet)
+      case default
+      end select
+   contains
+      subroutine putr(a)
+         real*4, allocatable :: a(:)
+         if (allocated(a)) then
+            if (size(a) /= n) deallocate(a)
+         endif
+         if (.not. allocated(a)) allocate(a(n))
+         a = p
+      end subroutine
+      subroutine putr2(a)
+         real*4, allocatable :: a(:,:)
+         integer :: nlv
+         nlv = subgrid_nlevels
+         if (allocated(a)) deallocate(a)
+         allocate(a(nlv, n / nlv))
+         a = reshape(p, (/ nlv, n / nlv /))
+      end subroutine
+   end subroutine
+
+   subroutine ref_set_f64(name, p, n) bind(C, name='ref_set_f64')
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int32_t), value :: n
+      real(c_double), intent(in) :: p(n)
+      select case (trim(cstr(name)))
+      case ('zs');       call putd(zs)
+      case ('z_volume'); call putd(z_volume)
+      case default
+      end select
+   contains
+      subroutine putd(a)
+         real*8, allocatable :: a(:)
+         if (allocated(a)) then
+            if (size(a) /= n) deallocate(a)
+         endif
+         if (.not. allocated(a)) allocate(a(n))
+         a = p
+      end subroutine
+   end subroutine
+
+   subroutine ref_get(name, p, nbytes) bind(C, name='ref_get')
+      character(kind=c_char), intent(in) :: name(*)
+      type(c_ptr), value :: p
+      integer(c_int64_t), value :: nbytes
+      real(c_float), pointer :: f(:)
+      real(c_double), pointer :: d(:)
+      integer(c_int8_t), pointer :: b(:)
+      select case (trim(cstr(name)))
+      case ('q');        call c_f_pointer(p, f, [nbytes / 4]); f = q
+      case ('uv');       call c_f_pointer(p, f, [nbytes / 4]); f = uv
+      case ('zs0');      call c_f_pointer(p, f, [nbytes / 4]); f = zs0
+      case ('zsderv');   call c_f_pointer(p, f, [nbytes / 4]); f = zsderv
+      case ('zsmax');    call c_f_pointer(p, f, [nbytes / 4]); f = zsmax
+      case ('zs');       call c_f_pointer(p, d, [nbytes / 8]); d = zs
+      case ('z_volume'); call c_f_pointer(p, d, [nbytes / 8]); d = z_volume
+      case ('kfuv');     call c_f_pointer(p, b, [nbytes]);     b = kfuv
+      case ('x73');      call c_f_pointer(p, f, [nbytes / 4]); f = reshape(x73, [10000])
+      case default
+      end select
+   end subroutine
+
+   ! compute_fluxes(dt, tloop) (sfincs_momentum.f90:9); returns min_dt
+   function ref_compute_fluxes(dt_) result(mdt) bind(C, name='ref_compute_fluxes')
+      real(c_float), value :: dt_
+      real(c_float) :: mdt
+      real*4 :: dtl
+      dtl = dt_
+      call compute_fluxes(dtl, tloopflux)
+      mdt = min_dt
+   end function
+
+   ! compute_water_levels(t, dt, tloop) (sfincs_continuity.f90:5)
+   subroutine ref_compute_water_levels(t_, dt_) bind(C, name='ref_compute_water_levels')
+      real(c_double), value :: t_
+      real(c_float), value :: dt_
+      real*8 :: tl
+      real*4 :: dtl
+      tl = t_; dtl = dt_
+      call compute_water_levels(tl, dtl, tloopcont)
+   end subroutine
+
+   ! the reference's own per-phase timers (sfincs_lib.f90:743,756)
+   subroutine ref_timers(tf, tc) bind(C, name='ref_timers')
+      real(c_double), intent(out) :: tf, tc
+      tf = tloopflux; tc = tloopcont
+   end subroutine
+
+   subroutine ref_reset_timers() bind(C, name='ref_reset_timers')
+      tloopflux = 0.0; tloopcont = 0.0
+   end subroutine
+
+   ! fill_h73_tables (sfincs_data.f90:858-881)
+   subroutine ref_fill_h73() bind(C, name='ref_fill_h73')
+      h73table = .true.
+      if (allocated(x73)) deallocate(x73)
+      call fill_h73_tables()
+   end subroutine
+
+   ! read_sfincs_input (sfincs_input.f90:5) from ./sfincs.inp; returns the values the hot
+   ! path depends on, in the order of ref_input_names.
+   subroutine ref_read_input(out, nout) bind(C, name='ref_read_input')
+      integer(c_int32_t), value :: nout
+      real(c_double), intent(out) :: out(nout)
+      call open_log()
+      call read_sfincs_input()
+      call close_log()
+      out = 0.0d0
+      out(1) = alfa; out(2) = theta; out(3) = hmin_cfl; out(4) = manning; out(5) = manning_land
+      out(6) = manning_sea; out(7) = rghlevland; out(8) = zini; out(9) = dtmax; out(10) = huthresh
+      out(11) = huvmin; out(12) = rhoa; out(13) = rhow; out(14) = advlim; out(15) = slopelim
+      out(16) = btfilter; out(17) = btrelax; out(18) = wiggle_factor; out(19) = wiggle_threshold
+      out(20) = uvlim; out(21) = uvmax; out(22) = nuviscdim; out(23) = nuviscfac; out(24) = latitude
+      out(25) = g; out(26) = pi
+      out(27) = merge(1.0d0, 0.0d0, advection); out(28) = advection_scheme
+      out(29) = merge(1.0d0, 0.0d0, viscosity); out(30) = merge(1.0d0, 0.0d0, coriolis)
+      out(31) = merge(1.0d0, 0.0d0, friction2d); out(32) = merge(1.0d0, 0.0d0, advection_mask)
+      out(33) = merge(1.0d0, 0.0d0, wiggle_suppression); out(34) = merge(1.0d0, 0.0d0, store_maximum_waterlevel)
+      out(35) = bndtype; out(36) = merge(1.0d0, 0.0d0, h73table)
+      out(37) = t0; out(38) = t1; out(39) = tspinup; out(40) = fcorio
+      out(41) = merge(1.0d0, 0.0d0, thetasmoothing)
+   end subroutine
+
+end module ref_driver

@@ -1,0 +1,597 @@
+/*
+ * ORACLE -- test infrastructure only. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker.
+ *
+ * A scalar C restatement of the SFINCS explicit time step, operation for operation and
+ * type for type (real*4 arithmetic, real*8 zs / z_volume / t), so that it reproduces
+ * the reference's own Fortran bit for bit on x86-64 (pinned against oracle/_ref, the
+ * reference's sfincs_momentum/sfincs_continuity compiled from source):
+ *   oracle_compute_fluxes        sfincs_momentum.f90:9-784
+ *   oracle_compute_water_levels  sfincs_continuity.f90:5-624
+ *   oracle_update_meteo          sfincs_meteo.f90:1246-1558 (time-series branch)
+ *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
+ *   oracle_update_discharges     sfincs_discharges.f90:325-369
+ *   oracle_update                sfincs_lib.f90:337-683 (step order, lagged dt)
+ * Arrays use 0-based storage; index tables hold the reference's 1-based values.
+ * Parity status per function is stated in DESIGN.md and the test files.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct OM {
+  /* sizes and switches */
+  int32_t np, npuv, ncuv, nref, nl;
+  int32_t subgrid, use_quadtree, advection, advection_scheme, viscosity, coriolis, friction2d;
+  int32_t thetasmoothing, wiggle_suppression, store_zsmax, wind, precip, h73table;
+  int32_t ngbnd, nkcuv2, nbnd, ntbnd, nsrc, ntsrc, ntprcp, ntwnd;
+  float g, pi, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
+  float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
+  float t0, t1, tspinup, zini, dx, dy;
+  /* cell tables */
+  const int8_t *kcs, *z_flags_iref;
+  const int32_t *z_index_uv_md, *z_index_uv_mu, *z_index_uv_nd, *z_index_uv_nu;
+  const float *zb, *cell_area, *dxrm, *dyrm, *dxrinv, *dyrinv, *dxr2inv, *dyr2inv, *dxrinvc, *dyrinvc, *nuvisc;
+  /* uv tables */
+  const int32_t *uv_index_z_nm, *uv_index_z_nmu, *uv_index_u_nmd, *uv_index_u_nmu, *uv_index_u_ndm, *uv_index_u_num;
+  const int32_t *uv_index_v_ndm, *uv_index_v_nm, *uv_index_v_ndmu, *uv_index_v_nmu;
+  const int32_t *cuv_index_uv, *cuv_index_uv1, *cuv_index_uv2;
+  const int8_t *kcuv, *mask_adv, *uv_flags_iref, *uv_flags_type, *uv_flags_dir;
+  const float *zbuv, *zbuvmx, *gn2uv;
+  /* subgrid */
+  const float *z_zmin, *z_zmax, *z_volmax, *z_dep;
+  const float *uv_zmin, *uv_zmax, *uv_navg_w, *uv_fnfit, *uv_havg, *uv_nrep, *uv_pwet, *uv_havg_zmax, *uv_nrep_zmax;
+  /* state */
+  double *zs, *z_volume;
+  float *q, *uv, *q0, *uv0, *zs0, *zsderv, *zsmax;
+  int8_t *kfuv;
+  float *tauwu, *tauwv, *netprcp;   /* np each */
+  /* boundaries */
+  const int32_t *nmindbnd, *index_kcuv2, *nmbkcuv2, *nmikcuv2, *ibkcuv2, *ind1_bnd_gbp, *ind2_bnd_gbp, *nmi_gbp;
+  const int8_t *ibuvdir;
+  const float *fac_bnd_gbp, *t_bnd, *zs_bnd;
+  float *zsb, *zsb0, *uvmean, *zst_bnd;
+  /* sources, meteo */
+  const int32_t *nmindsrc;
+  const float *tsrc, *qsrc, *tprcpt, *tprcpv, *twnd, *wndmag, *wnddir, *cdval;
+  const float *x73;   /* h**(7/3) table, 1000 x 10, column-major (sfincs_data.f90:858-881) */
+  float *qtsrc;
+  /* time state */
+  double t;
+  float dt, min_dt;
+  int32_t nt, error, itbndlast, itsrclast, itwndlast, itprcplast;
+} OM;
+
+OM* oracle_new(void) {
+  OM* m = (OM*)calloc(1, sizeof(OM));
+  m->min_dt = 1.0e-6f;   /* sfincs_lib.f90:281-282 */
+  m->dt = 1.0e-6f;
+  m->itbndlast = 1; m->itsrclast = 1; m->itwndlast = 1; m->itprcplast = 1;
+  return m;
+}
+void oracle_free(OM* m) { free(m); }
+
+#define BIND(field) if (!strcmp(name, #field)) { *(void**)&m->field = ptr; return 0; }
+int oracle_bind(OM* m, const char* name, void* ptr) {
+  BIND(kcs) BIND(z_flags_iref) BIND(z_index_uv_md) BIND(z_index_uv_mu) BIND(z_index_uv_nd) BIND(z_index_uv_nu)
+  BIND(zb) BIND(cell_area) BIND(dxrm) BIND(dyrm) BIND(dxrinv) BIND(dyrinv) BIND(dxr2inv) BIND(dyr2inv)
+  BIND(dxrinvc) BIND(dyrinvc) BIND(nuvisc)
+  BIND(uv_index_z_nm) BIND(uv_index_z_nmu) BIND(uv_index_u_nmd) BIND(uv_index_u_nmu) BIND(uv_index_u_ndm)
+  BIND(uv_index_u_num) BIND(uv_index_v_ndm) BIND(uv_index_v_nm) BIND(uv_index_v_ndmu) BIND(uv_index_v_nmu)
+  BIND(cuv_index_uv) BIND(cuv_index_uv1) BIND(cuv_index_uv2)
+  BIND(kcuv) BIND(mask_adv) BIND(uv_flags_iref) BIND(uv_flags_type) BIND(uv_flags_dir) BIND(zbuv) BIND(zbuvmx) BIND(gn2uv)
+  BIND(z_zmin) BIND(z_zmax) BIND(z_volmax) BIND(z_dep) BIND(uv_zmin) BIND(uv_zmax) BIND(uv_navg_w) BIND(uv_fnfit)
+  BIND(uv_havg) BIND(uv_nrep) BIND(uv_pwet) BIND(uv_havg_zmax) BIND(uv_nrep_zmax)
+  BIND(zs) BIND(z_volume) BIND(q) BIND(uv) BIND(q0) BIND(uv0) BIND(zs0) BIND(zsderv) BIND(zsmax) BIND(kfuv)
+  BIND(tauwu) BIND(tauwv) BIND(netprcp)
+  BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
+  BIND(nmi_gbp) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
+  BIND(nmindsrc) BIND(tsrc) BIND(qsrc) BIND(tprcpt) BIND(tprcpv) BIND(twnd) BIND(wndmag) BIND(wnddir) BIND(cdval) BIND(qtsrc) BIND(x73)
+  return -1;
+}
+#undef BIND
+
+#define SETI(field) if (!strcmp(name, #field)) { m->field = (int32_t)v; return 0; }
+#define SETF(field) if (!strcmp(name, #field)) { m->field = (float)v; return 0; }
+int oracle_set(OM* m, const char* name, double v) {
+  SETI(np) SETI(npuv) SETI(ncuv) SETI(nref) SETI(nl) SETI(subgrid) SETI(use_quadtree) SETI(advection)
+  SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
+  SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
+  SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error)
+  SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
+  SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
+  SETF(btfilter) SETF(btrelax) SETF(t0) SETF(t1) SETF(tspinup) SETF(zini) SETF(dx) SETF(dy) SETF(dt) SETF(min_dt)
+  if (!strcmp(name, "t")) { m->t = v; return 0; }
+  return -1;
+}
+#undef SETI
+#undef SETF
+
+double oracle_get(const OM* m, const char* name) {
+  if (!strcmp(name, "t")) return m->t;
+  if (!strcmp(name, "dt")) return m->dt;
+  if (!strcmp(name, "min_dt")) return m->min_dt;
+  if (!strcmp(name, "nt")) return m->nt;
+  if (!strcmp(name, "error")) return m->error;
+  return NAN;
+}
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline float fmx(float a, float b) { return a > b ? a : b; }
+static inline float fmn(float a, float b) { return a < b ? a : b; }
+
+/* power7over3 (sfincs_momentum.f90:787-820) */
+static float power7over3(const float* x73, float hu) {
+  if (hu < 0.00001f) return x73[0 * 1000 + (int)(1e8f * hu) - 1];
+  if (hu < 0.0001f) return x73[1 * 1000 + (int)(1e7f * hu) - 1];
+  if (hu < 0.001f) return x73[2 * 1000 + (int)(1e6f * hu) - 1];
+  if (hu < 0.01f) return x73[3 * 1000 + (int)(1e5f * hu) - 1];
+  if (hu < 0.1f) return x73[4 * 1000 + (int)(1e4f * hu) - 1];
+  if (hu < 1.0f) return x73[5 * 1000 + (int)(1e3f * hu) - 1];
+  if (hu < 10.0f) return x73[6 * 1000 + (int)(100 * hu) - 1];
+  if (hu < 100.0f) return x73[7 * 1000 + (int)(10 * hu) - 1];
+  if (hu < 1000.0f) return x73[8 * 1000 + (int)(1 * hu) - 1];
+  if (hu < 10000.0f) return x73[9 * 1000 + (int)(0.1f * hu) - 1];
+  return powf(10000.0f, 7.0f / 3.0f);
+}
+
+/* ---- momentum: sfincs_momentum.f90:9-784 ------------------------------------------- */
+float oracle_compute_fluxes(OM* m, float dt) {
+  const float expo = 1.0f / 3.0f;   /* :92 */
+  float min_dt = m->dtmax;           /* :99 */
+  const int32_t nuv = m->npuv + m->ncuv;
+  memcpy(m->q0, m->q, sizeof(float) * nuv);   /* :125-138 */
+  memcpy(m->uv0, m->uv, sizeof(float) * nuv);
+  const int nbrs = m->advection || m->coriolis || m->viscosity || m->friction2d || m->thetasmoothing;
+  for (int32_t i = 0; i < m->npuv; ++i) {
+    const int kc = m->kcuv[i];
+    if (!(kc == 1 || kc == 6)) continue;
+    const int32_t nm = m->uv_index_z_nm[i] - 1, nmu = m->uv_index_z_nmu[i] - 1;
+    int iok = 0;
+    const float zsu = (float)dmax(m->zs[nm], m->zs[nmu]);
+    float zmin = 0.f, zmax = 0.f;
+    if (m->subgrid) {
+      zmin = m->uv_zmin[i]; zmax = m->uv_zmax[i];
+      if (zsu > zmin + m->huthresh) iok = 1;
+    } else {
+      if (zsu > m->zbuvmx[i]) iok = 1;
+    }
+    if (!iok) { m->q[i] = 0.0f; m->uv[i] = 0.0f; m->kfuv[i] = 0; continue; }
+    int iref = 1, itype = 0;
+    if (m->use_quadtree) { iref = m->uv_flags_iref[i]; itype = m->uv_flags_type[i]; }
+    const int idir = m->uv_flags_dir[i];
+    float dxuvinv, dyuvinv, dxuv2inv, dyuv2inv;
+    if (itype == 0) {
+      if (idir == 0) { dxuvinv = m->dxrinv[iref - 1]; dyuvinv = m->dyrinv[iref - 1]; dxuv2inv = m->dxr2inv[iref - 1]; dyuv2inv = m->dyr2inv[iref - 1]; }
+      else { dxuvinv = m->dyrinv[iref - 1]; dyuvinv = m->dxrinv[iref - 1]; dxuv2inv = m->dyr2inv[iref - 1]; dyuv2inv = m->dxr2inv[iref - 1]; }
+    } else {
+      if (idir == 0) { dxuvinv = m->dxrinvc[iref - 1]; dyuvinv = m->dyrinv[iref - 1]; dxuv2inv = 0.0f; dyuv2inv = m->dyr2inv[iref - 1]; }
+      else { dxuvinv = m->dyrinvc[iref - 1]; dyuvinv = m->dxrinv[iref - 1]; dxuv2inv = 0.0f; dyuv2inv = m->dxr2inv[iref - 1]; }
+    }
+    (void)dyuvinv;
+    const float fcoriouv = m->fcorio;
+    const float qx_nm = m->q0[i];
+    float qx_nmd = 0, qx_nmu = 0, qy_nm = 0, qy_nmu = 0, qy_ndm = 0, qy_ndmu = 0;
+    float uu_nm = 0, uu_nmd = 0, uu_nmu = 0, uu_ndm = 0, uu_num = 0, vu = 0;
+    if (nbrs) {
+      qx_nmd = m->q0[m->uv_index_u_nmd[i] - 1];
+      qx_nmu = m->q0[m->uv_index_u_nmu[i] - 1];
+      qy_nm = m->q0[m->uv_index_v_nm[i] - 1];
+      qy_nmu = m->q0[m->uv_index_v_nmu[i] - 1];
+      qy_ndm = m->q0[m->uv_index_v_ndm[i] - 1];
+      qy_ndmu = m->q0[m->uv_index_v_ndmu[i] - 1];
+      uu_nm = m->uv0[i];
+      uu_nmd = m->uv0[m->uv_index_u_nmd[i] - 1];
+      uu_nmu = m->uv0[m->uv_index_u_nmu[i] - 1];
+      uu_ndm = m->uv0[m->uv_index_u_ndm[i] - 1];
+      uu_num = m->uv0[m->uv_index_u_num[i] - 1];
+      vu = (m->uv0[m->uv_index_v_ndm[i] - 1] + m->uv0[m->uv_index_v_ndmu[i] - 1] + m->uv0[m->uv_index_v_nm[i] - 1] +
+            m->uv0[m->uv_index_v_nmu[i] - 1]) / 4;
+    }
+    float phi = 1.0f, hu, gnavg2;
+    if (m->subgrid) {
+      const int nl = m->nl;
+      if (zsu > zmax) {
+        hu = m->uv_havg_zmax[i] + zsu;
+        gnavg2 = m->uv_navg_w[i] - (m->uv_navg_w[i] - m->uv_nrep_zmax[i]) / (m->uv_fnfit[i] * (zsu - zmax) + 1.0f);
+      } else {
+        const float dzuv = (zmax - zmin) / (float)(nl - 1);
+        int iuv = (int)((zsu - zmin) / dzuv) + 1;
+        if (iuv > nl - 1) iuv = nl - 1;
+        const float facint = (zsu - (zmin + (float)(iuv - 1) * dzuv)) / dzuv;
+        const float* hv = m->uv_havg + (size_t)i * nl;
+        const float* nr = m->uv_nrep + (size_t)i * nl;
+        const float* pw = m->uv_pwet + (size_t)i * nl;
+        hu = hv[iuv - 1] + (hv[iuv] - hv[iuv - 1]) * facint;
+        gnavg2 = nr[iuv - 1] + (nr[iuv] - nr[iuv - 1]) * facint;
+        phi = pw[iuv - 1] + (pw[iuv] - pw[iuv - 1]) * facint;
+      }
+    } else {
+      hu = fmx(zsu - m->zbuvmx[i], m->huthresh);
+      gnavg2 = m->gn2uv[i];
+    }
+    const float hwet = hu / phi;
+    float dzdx;
+    if (m->slopelim < 9999.0f) {
+      double d = (m->zs[nmu] - m->zs[nm]) * (double)dxuvinv;
+      if (d < (double)(-m->slopelim)) d = (double)(-m->slopelim);
+      if (d > (double)m->slopelim) d = (double)m->slopelim;
+      dzdx = (float)d;
+    } else {
+      dzdx = (float)((m->zs[nmu] - m->zs[nm]) * (double)dxuvinv);
+    }
+    float frc = -(m->g * hu * dzdx);
+    if (m->advection && m->mask_adv[i] == 1) {
+      float dqxudx = 0.0f, dqyudy = 0.0f;
+      if (m->advection_scheme == 0) {
+        if (qx_nm > 1.0e-6f) dqxudx = (qx_nm * uu_nm - qx_nmd * uu_nmd) * dxuvinv;
+        else if (qx_nm < -1.0e-6f) dqxudx = (qx_nmu * uu_nmu - qx_nm * uu_nm) * dxuvinv;
+        const float qy = qy_ndm + qy_ndmu + qy_nm + qy_nmu;
+        if (qy > 1.0e-6f) dqyudy = (qy_ndm + qy_ndmu) * (uu_nm - uu_ndm) * dyuvinv / 2;
+        else if (qy < -1.0e-6f) dqyudy = (qy_nm + qy_nmu) * (uu_num - uu_nm) * dyuvinv / 2;
+      } else if (m->advection_scheme == 1) {
+        float qd = (qx_nmd + qx_nm) / 2;
+        float qu = (qx_nm + qx_nmu) / 2;
+        if (qd > 1.0e-6f) {
+          const float ud = (uu_nmd + uu_nm) / 2;
+          dqxudx = (qd * (uu_nm - uu_nmd) + ud * (qx_nm - qx_nmd)) * dxuvinv;
+        }
+        if (qu < -1.0e-6f) {
+          const float uu = (uu_nm + uu_nmu) / 2;
+          dqxudx = dqxudx + (qu * (uu_nmu - uu_nm) + uu * (qx_nmu - qx_nm)) * dxuvinv;
+        }
+        qu = (qy_nm + qy_nmu) / 2;
+        qd = (qy_ndm + qy_ndmu) / 2;
+        if (qd > 1.0e-6f) dqyudy = qd * (uu_nm - uu_ndm) * dyuvinv;
+        if (qu < -1.0e-6f) dqyudy = dqyudy + qu * (uu_num - uu_nm) * dyuvinv;
+        const float ud = (uu_nmd + uu_nm) / 2;
+        const float uu = (uu_nm + uu_nmu) / 2;
+        if (ud > 1.0e-6f) dqyudy = dqyudy + ud * (qy_nm - qy_ndm) * dyuvinv;
+        if (uu < -1.0e-6f) dqyudy = dqyudy + uu * (qy_nmu - qy_ndmu) * dyuvinv;
+      }
+      float adv = -(phi * (dqxudx + dqyudy));
+      adv = fmn(fmx(adv, -(m->advlim * hu)), m->advlim * hu);
+      frc = frc + adv;
+    }
+    if (m->viscosity) {
+      const float lap = (uu_nmu - 2 * uu_nm + uu_nmd) * dxuv2inv + (uu_num - 2 * uu_nm + uu_ndm) * dyuv2inv;
+      if (itype == 0) frc = frc + m->nuvisc[iref - 1] * hu * lap;
+      else frc = frc + m->nuviscfac * m->nuvisc[iref - 1] * hu * lap;
+    }
+    if (m->coriolis) {
+      if (idir == 0) frc = frc + fcoriouv * hu * vu;
+      else frc = frc - fcoriouv * hu * vu;
+    }
+    if (m->wind) {
+      if (hwet > 0.25f) frc = frc + phi * (idir == 0 ? m->tauwu[nm] : m->tauwv[nm]);
+      else frc = frc + (idir == 0 ? m->tauwu[nm] : m->tauwv[nm]) * hu * 4;
+    }
+    float qfr;
+    if (m->kfuv[i] == 0) {
+      qfr = sqrtf(fabsf(dzdx) / (fmx(gnavg2, 1.0e-5f) / 10)) * powf(hu, 5.0f / 3.0f);
+    } else if (m->friction2d) {
+      qfr = sqrtf(qx_nm * qx_nm + (hu * vu) * (hu * vu));
+    } else {
+      qfr = fabsf(qx_nm);
+    }
+    float qsm = qx_nm;
+    if (m->thetasmoothing) {
+      if (m->kcuv[m->uv_index_u_nmd[i] - 1] == 1 && m->kcuv[m->uv_index_u_nmu[i] - 1] == 1)
+        if (fabsf(qx_nmd) > 1.0e-6f && fabsf(qx_nmu) > 1.0e-6f)
+          qsm = m->theta * qx_nm + 0.5f * (1.0f - m->theta) * (qx_nmu + qx_nmd);
+    }
+    const float hu73 = m->h73table ? power7over3(m->x73, fmx(hu, 1.0e-6f)) : hu * hu * powf(hu, expo);
+    float qn = (qsm + frc * dt) / (1.0f + gnavg2 * dt * qfr / hu73);
+    if (m->subgrid && m->wiggle_suppression) {
+      const float mdrv = fabsf(m->zsderv[nm] - m->zsderv[nmu]) - m->wiggle_threshold;
+      if (mdrv > 0.0f) qn = qn * m->wiggle_threshold / (m->wiggle_factor * mdrv + m->wiggle_threshold);
+    }
+    if (m->subgrid) {
+      if (m->z_volume[nm] < 0.0) qn = fmn(qn, 0.0f);
+      if (m->z_volume[nmu] < 0.0) qn = fmx(qn, 0.0f);
+    } else {
+      if (m->zs[nm] < (double)m->zb[nm]) qn = fmn(qn, 0.0f);
+      if (m->zs[nmu] < (double)m->zb[nmu]) qn = fmx(qn, 0.0f);
+    }
+    qn = fmn(fmx(qn, -(hu * m->uvlim)), hu * m->uvlim);
+    m->q[i] = qn;
+    const float uvn = qn / fmx(hu, m->huvmin);
+    m->uv[i] = uvn;
+    m->kfuv[i] = 1;
+    const float min_dt_ip = 1.0f / (fmx(sqrtf(m->g * hu), fabsf(uvn)) * dxuvinv);
+    min_dt = fmn(min_dt, min_dt_ip);
+  }
+  for (int32_t j = 0; j < m->ncuv; ++j) {   /* :757-779 */
+    const int32_t c = m->cuv_index_uv[j] - 1, a = m->cuv_index_uv1[j] - 1, b = m->cuv_index_uv2[j] - 1;
+    m->q[c] = (m->q[a] + m->q[b]) / 2;
+    m->uv[c] = (m->uv[a] + m->uv[b]) / 2;
+  }
+  m->min_dt = min_dt;
+  return min_dt;
+}
+
+/* ---- continuity: sfincs_continuity.f90:46-262 and :265-624 -------------------------- */
+void oracle_compute_water_levels(OM* m, double t, float dt) {
+  (void)t;
+  if (!m->subgrid) {
+    for (int32_t s = 0; s < m->nsrc; ++s) {   /* :87-106 */
+      const int32_t nm = m->nmindsrc[s] - 1;
+      const float inc = m->qtsrc[s] * dt / m->cell_area[m->z_flags_iref[nm] - 1];
+      m->zs[nm] = dmax(m->zs[nm] + (double)inc, (double)m->zb[nm]);
+    }
+    for (int32_t nm = 0; nm < m->np; ++nm) {
+      if (m->kcs[nm] == 1) {
+        if (m->precip) m->zs[nm] = m->zs[nm] + (double)(m->netprcp[nm] * dt);
+        const int32_t nmd = m->z_index_uv_md[nm] - 1, nmu = m->z_index_uv_mu[nm] - 1;
+        const int32_t ndm = m->z_index_uv_nd[nm] - 1, num = m->z_index_uv_nu[nm] - 1;
+        const int ir = m->z_flags_iref[nm] - 1;
+        const float d = ((m->q[nmd] - m->q[nmu]) * m->dxrinv[ir] + (m->q[ndm] - m->q[num]) * m->dyrinv[ir]) * dt;
+        m->zs[nm] = m->zs[nm] + (double)d;
+      }
+      if (m->store_zsmax) m->zsmax[nm] = (float)dmax((double)m->zsmax[nm], m->zs[nm]);
+    }
+    return;
+  }
+  for (int32_t s = 0; s < m->nsrc; ++s) {   /* :311-325 */
+    const int32_t nm = m->nmindsrc[s] - 1;
+    if ((m->z_volume[nm] >= 0) || ((m->qtsrc[s] < 0.0f) && (m->z_volume[nm] >= 0)))
+      m->z_volume[nm] = m->z_volume[nm] + (double)(m->qtsrc[s] * dt);
+  }
+  const int nl = m->nl;
+  for (int32_t nm = 0; nm < m->np; ++nm) {
+    float dvol = 0.0f;
+    const int k = m->kcs[nm];
+    const int ir = m->z_flags_iref[nm] - 1;
+    if (k == 1) {
+      const int32_t nmd = m->z_index_uv_md[nm] - 1, nmu = m->z_index_uv_mu[nm] - 1;
+      const int32_t ndm = m->z_index_uv_nd[nm] - 1, num = m->z_index_uv_nu[nm] - 1;
+      if (m->use_quadtree)
+        dvol = dvol + ((m->q[nmd] - m->q[nmu]) * m->dyrm[ir] + (m->q[ndm] - m->q[num]) * m->dxrm[ir]) * dt;
+      else
+        dvol = dvol + ((m->q[nmd] - m->q[nmu]) * m->dy + (m->q[ndm] - m->q[num]) * m->dx) * dt;
+    }
+    if (k == 1 || k == 4) {
+      const float a = m->cell_area[ir];
+      if (m->precip) {
+        float dzsdt = 0.0f;
+        dzsdt = dzsdt + m->netprcp[nm];
+        dvol = dvol + dzsdt * a * dt;
+      }
+      m->z_volume[nm] = m->z_volume[nm] + (double)dvol;
+      float zs00 = 0.f, zs11 = 0.f;
+      if (m->wiggle_suppression) {
+        zs00 = m->zs0[nm];
+        zs11 = (float)m->zs[nm];
+        m->zs0[nm] = zs11;
+      }
+      const float vmax = m->z_volmax[nm];
+      const double V = m->z_volume[nm];
+      if (V >= (double)(vmax * 0.999f)) {
+        m->zs[nm] = (double)fmx(m->z_zmax[nm], -20.0f) + (V - (double)vmax) / (double)a;
+      } else if (V <= (double)1.0e-6f) {
+        m->zs[nm] = (double)fmx(m->z_zmin[nm], -20.0f);
+      } else {
+        const float dzvol = vmax / (float)(nl - 1);
+        const int iuv = (int)(V / (double)dzvol) + 1;
+        const float facint = (float)((V - (double)((float)(iuv - 1) * dzvol)) / (double)dzvol);
+        const float* dep = m->z_dep + (size_t)nm * nl;
+        m->zs[nm] = (double)(dep[iuv - 1] + (dep[iuv] - dep[iuv - 1]) * facint);
+      }
+      if (m->wiggle_suppression)
+        m->zsderv[nm] = (float)((m->zs[nm] - (double)(2 * zs11)) + (double)zs00);
+    }
+    if (m->store_zsmax) m->zsmax[nm] = (float)dmax((double)m->zsmax[nm], m->zs[nm]);
+  }
+}
+
+/* ---- meteo time series: sfincs_meteo.f90:1423-1437, 1445-1558 ----------------------- */
+static float f_modulo(float a, float p) {
+  float r = fmodf(a, p);
+  if (r != 0.0f && ((r < 0.0f) != (p < 0.0f))) r += p;
+  return r;
+}
+
+void oracle_update_meteo(OM* m, double t, float dt) {
+  (void)dt;
+  if (m->wind) {
+    for (int32_t itw = m->itwndlast; itw <= m->ntwnd; ++itw) {
+      if ((double)m->twnd[itw - 1] > t) {
+        const float twfac = (float)((t - (double)m->twnd[itw - 2]) / (double)(m->twnd[itw - 1] - m->twnd[itw - 2]));
+        const float vmag = m->wndmag[itw - 2] * (1.0f - twfac) + m->wndmag[itw - 1] * twfac;
+        float dr0 = f_modulo(m->wnddir[itw - 2], 2 * m->pi);
+        float dr1 = f_modulo(m->wnddir[itw - 1], 2 * m->pi);
+        if (dr1 > dr0 + m->pi) dr0 = dr0 + 2 * m->pi;
+        else if (dr0 > dr1 + m->pi) dr1 = dr1 + 2 * m->pi;
+        const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
+        const float cd = m->cdval[(int)(vmag * 10) + 1 - 1];
+        const double twu = (double)(vmag * vmag * cosf(vdir) * m->rhoa * cd / m->rhow);
+        const double twv = (double)(vmag * vmag * sinf(vdir) * m->rhoa * cd / m->rhow);
+        for (int32_t nm = 0; nm < m->np; ++nm) { m->tauwu[nm] = (float)twu; m->tauwv[nm] = (float)twv; }
+        m->itwndlast = itw - 1;
+        break;
+      }
+    }
+  }
+  if (m->precip) {
+    int32_t itp;
+    for (itp = m->itprcplast; itp <= m->ntprcp; ++itp)
+      if ((double)m->tprcpt[itp - 1] > t) break;
+    if (itp > m->ntprcp) itp = m->ntprcp;
+    if (itp < 1) itp = 1;
+    const float twfac = (float)((t - (double)m->tprcpt[itp - 2]) / (double)(m->tprcpt[itp - 1] - m->tprcpt[itp - 2]));
+    const float ptmp = (m->tprcpv[itp - 2] * (1.0f - twfac) + m->tprcpv[itp - 1] * twfac) / (float)(1000 * 3600);
+    for (int32_t nm = 0; nm < m->np; ++nm) m->netprcp[nm] = ptmp;
+    m->itprcplast = itp - 1;
+  }
+}
+
+/* ---- boundaries: sfincs_boundaries.f90:682-1175 ------------------------------------- */
+static void update_boundary_points(OM* m, double t) {
+  int32_t itb0 = 1, itb1 = 1;
+  float tb = 0.f;
+  const int32_t nt = m->ntbnd;
+  if ((double)m->t_bnd[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = m->t_bnd[0]; }
+  else if ((double)m->t_bnd[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = m->t_bnd[nt - 1]; }
+  else {
+    for (int32_t itb = m->itbndlast; itb <= nt; ++itb) {
+      if ((double)m->t_bnd[itb - 1] > t + (double)1.0e-6f) {
+        itb0 = itb - 1; itb1 = itb; tb = (float)t; m->itbndlast = itb - 1;
+        break;
+      }
+    }
+  }
+  const float tbfac = (tb - m->t_bnd[itb0 - 1]) / fmx(m->t_bnd[itb1 - 1] - m->t_bnd[itb0 - 1], 1.0e-6f);
+  for (int32_t ib = 0; ib < m->nbnd; ++ib) {
+    const float a = m->zs_bnd[(size_t)(itb0 - 1) * m->nbnd + ib], b = m->zs_bnd[(size_t)(itb1 - 1) * m->nbnd + ib];
+    m->zst_bnd[ib] = a + (b - a) * tbfac;
+  }
+}
+
+static void update_boundary_conditions(OM* m, double t, float dt) {
+  (void)dt;
+  for (int32_t ib = 0; ib < m->ngbnd; ++ib) {
+    const int32_t nmb = m->nmindbnd[ib] - 1;
+    const int k = m->kcs[nmb];
+    if (k == 2) {
+      double zst;
+      if (m->nbnd > 1) {
+        const float f = m->fac_bnd_gbp[ib];
+        zst = (double)(m->zst_bnd[m->ind1_bnd_gbp[ib] - 1] * f + m->zst_bnd[m->ind2_bnd_gbp[ib] - 1] * (1.0f - f));
+      } else {
+        zst = (double)m->zst_bnd[0];
+      }
+      const float zsetup = 0.0f, zig = 0.0f;
+      if (t < (double)(m->tspinup - 1.0e-3f)) {
+        const float smfac = (float)(1.0 - (t - (double)m->t0) / (double)(m->tspinup - m->t0));
+        float zs0act = (float)(zst + (double)zsetup);
+        m->zsb0[ib] = m->zini * smfac + zs0act * (1.0f - smfac);
+        zs0act = (float)(zst + (double)zsetup + (double)zig);
+        m->zsb[ib] = m->zini * smfac + zs0act * (1.0f - smfac);
+      } else {
+        m->zsb0[ib] = (float)(zst + (double)zsetup);
+        m->zsb[ib] = (float)(zst + (double)zsetup + (double)zig);
+      }
+      if (m->subgrid) {
+        m->zsb0[ib] = fmx(m->zsb0[ib], m->z_zmin[nmb]);
+        m->zsb[ib] = fmx(m->zsb[ib], m->z_zmin[nmb]);
+      } else {
+        m->zsb0[ib] = fmx(m->zsb0[ib], m->zb[nmb]);
+        m->zsb[ib] = fmx(m->zsb[ib], m->zb[nmb]);
+      }
+    } else if (k == 6) {
+      m->zs[nmb] = m->zs[m->nmi_gbp[ib] - 1];
+    }
+  }
+}
+
+static void update_boundary_fluxes(OM* m, float dt, double t) {
+  (void)t;
+  const float factime = fmn(dt / m->btfilter, 1.0f);
+  const float one_minus_factime = 1.0f - factime;
+  const float facrel = 1.0f - fmn(dt / m->btrelax, 1.0f);
+  const int nl = m->nl;
+  for (int32_t ib = 0; ib < m->nkcuv2; ++ib) {
+    const int32_t indb = m->ibkcuv2[ib] - 1;
+    const int32_t nmb = m->nmbkcuv2[ib] - 1;
+    if (m->kcs[nmb] == 6) continue;
+    const int32_t nmi = m->nmikcuv2[ib] - 1;
+    const int32_t ip = m->index_kcuv2[ib] - 1;
+    const float zsnmi = (float)m->zs[nmi];
+    float zsnmb = m->zsb[indb];
+    float zs0nmb = m->zsb0[indb];
+    float hnmb;
+    if (m->subgrid) {
+      const float zsuv = fmx(zsnmb, zsnmi);
+      float depthuv;
+      if (zsuv >= m->uv_zmax[ip] - 1.0e-4f) {
+        depthuv = m->uv_havg_zmax[ip] + zsuv;
+      } else if (zsuv > m->uv_zmin[ip]) {
+        const float dzuv = (m->uv_zmax[ip] - m->uv_zmin[ip]) / (float)(nl - 1);
+        const int iuv = (int)((zsuv - m->uv_zmin[ip]) / dzuv) + 1;
+        const float facint = (zsuv - (m->uv_zmin[ip] + (float)(iuv - 1) * dzuv)) / dzuv;
+        const float* hv = m->uv_havg + (size_t)ip * nl;
+        depthuv = hv[iuv - 1] + (hv[iuv] - hv[iuv - 1]) * facint;
+      } else {
+        depthuv = 0.0f;
+      }
+      hnmb = fmx(depthuv, m->huthresh);
+      zsnmb = fmx(zsnmb, m->z_zmin[nmb]);
+      zs0nmb = fmx(zs0nmb, m->z_zmin[nmb]);
+    } else {
+      hnmb = fmx(0.5f * (zsnmb + zsnmi) - m->zbuv[ip], m->huthresh);
+      zsnmb = fmx(zsnmb, m->zb[nmb]);
+      zs0nmb = fmx(zs0nmb, m->zb[nmb]);
+    }
+    if (hnmb < m->huthresh + 1.0e-6f || m->kcuv[ip] == 3) {
+      m->q[ip] = 0.0f; m->uv[ip] = 0.0f; m->uvmean[ib] = 0.0f;
+    } else {
+      const float ui = sqrtf(m->g / hnmb) * (zsnmb - zs0nmb);
+      const float ub = (float)m->ibuvdir[ib] * (2 * ui - sqrtf(m->g / hnmb) * (zsnmi - zs0nmb));
+      float qb = ub * hnmb + m->uvmean[ib];
+      const int dir = m->ibuvdir[ib];
+      if (m->subgrid) {
+        if (m->z_volume[nmi] <= 0.0) qb = (dir == 1) ? fmx(qb, 0.0f) : fmn(qb, 0.0f);
+        if (zsnmb - m->z_zmin[nmb] < m->huthresh) qb = (dir == 1) ? fmn(qb, 0.0f) : fmx(qb, 0.0f);
+      } else {
+        if (zsnmi - m->zb[nmi] <= m->huthresh) qb = (dir == 1) ? fmx(qb, 0.0f) : fmn(qb, 0.0f);
+        if (zsnmb - m->zb[nmb] < m->huthresh) qb = (dir == 1) ? fmn(qb, 0.0f) : fmx(qb, 0.0f);
+      }
+      m->q[ip] = qb;
+      m->uv[ip] = fmx(fmn(qb / hnmb, 4.0f), -4.0f);
+    }
+    if (m->btfilter >= -1.0e-6f) m->uvmean[ib] = factime * m->q[ip] + facrel * one_minus_factime * m->uvmean[ib];
+    else m->uvmean[ib] = 0.0f;
+    m->zs[nmb] = (double)m->zsb[indb];
+    if (m->store_zsmax) m->zsmax[nmb] = (float)dmax((double)m->zsmax[nmb], m->zs[nmb]);
+  }
+}
+
+void oracle_update_boundaries(OM* m, double t, float dt) {
+  if (m->ngbnd == 0) return;   /* boundaries_in_mask */
+  if (m->nbnd > 0) update_boundary_points(m, t);
+  update_boundary_conditions(m, t, dt);
+  update_boundary_fluxes(m, dt, t);
+}
+
+/* ---- discharges: sfincs_discharges.f90:353-369 ---------------------------------------- */
+void oracle_update_discharges(OM* m, double t, float dt) {
+  (void)dt;
+  if (m->nsrc <= 0) return;
+  for (int32_t it = m->itsrclast; it <= m->ntsrc; ++it) {
+    if ((double)m->tsrc[it - 1] > t) {
+      for (int32_t s = 0; s < m->nsrc; ++s) {
+        const float q0v = m->qsrc[(size_t)(it - 2) * m->nsrc + s], q1v = m->qsrc[(size_t)(it - 1) * m->nsrc + s];
+        const double v = (double)q0v + (double)(q1v - q0v) * (t - (double)m->tsrc[it - 2]) /
+                                          (double)(m->tsrc[it - 1] - m->tsrc[it - 2]);
+        m->qtsrc[s] = (float)v;
+      }
+      m->itsrclast = it - 1;
+      break;
+    }
+  }
+}
+
+/* ---- step driver: sfincs_lib.f90:337-683 ---------------------------------------------- */
+/* Runs while t < tend; single step when max_steps == 1. Returns the number of steps. */
+int32_t oracle_update(OM* m, double tend, int32_t max_steps) {
+  int32_t steps = 0;
+  while (m->t < tend) {
+    if (max_steps > 0 && steps >= max_steps) break;
+    m->nt = m->nt + 1;
+    m->dt = m->alpha * m->min_dt;
+    const float dtchk = m->alpha * m->min_dt;
+    m->t = m->t + (double)m->dt;
+    if (m->wind || m->precip) oracle_update_meteo(m, m->t, m->dt);
+    oracle_update_boundaries(m, m->t, m->dt);
+    oracle_update_discharges(m, m->t, m->dt);
+    oracle_compute_fluxes(m, m->dt);
+    oracle_compute_water_levels(m, m->t, m->dt);
+    ++steps;
+    if (dtchk < m->dtmin && m->nt > 1) {
+      m->error = 1;
+      m->t = (double)m->t1 + 1.0;
+    }
+  }
+  return steps;
+}

@@ -1,0 +1,400 @@
+"""ctypes bindings of libsfx.so (declared in include/sfx.h).
+
+The library is loaded from the package directory. There is no fallback: if the native
+library is missing, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsfx.so")
+
+_lib = None
+
+
+class SfxError(RuntimeError):
+    pass
+
+
+class sfx_params(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "t0", "t1", "tspinup", "alpha", "theta", "hmin_cfl", "manning", "manning_land",
+        "manning_sea", "rgh_lev_land", "zsini", "dtmax", "huthresh", "huvmin", "rhoa", "rhow",
+        "advlim", "slopelim", "btfilter", "btrelax", "wiggle_factor", "wiggle_threshold", "uvlim",
+        "uvmax", "nuvisc", "nuviscfac", "latitude", "g", "pi")] + [(n, C.c_int32) for n in (
+        "advection", "advection_scheme", "viscosity", "coriolis", "friction2d", "advection_mask",
+        "wiggle_suppression", "store_zsmax", "bndtype", "h73table")]
+
+
+class sfx_quadtree_input(C.Structure):
+    _fields_ = [("nr_points", C.c_int32), ("nr_levels", C.c_int32),
+                ("x0", C.c_float), ("y0", C.c_float), ("dx", C.c_float), ("dy", C.c_float),
+                ("rotation_deg", C.c_float)] + \
+               [(n, C.c_void_p) for n in ("level", "md", "mu", "nd", "nu", "mask")] + \
+               [(n, C.c_void_p) for n in ("n", "m", "md1", "md2", "mu1", "mu2", "nd1", "nd2", "nu1", "nu2")]
+
+
+class sfx_subgrid_input(C.Structure):
+    _fields_ = [("nlevels", C.c_int32)] + [(n, C.c_void_p) for n in (
+        "z_zmin", "z_zmax", "z_volmax", "z_level", "uv_zmin", "uv_zmax", "uv_navg", "uv_ffit",
+        "uv_havg", "uv_nrep", "uv_pwet")]
+
+
+class sfx_sizes(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "np", "npuv", "ncuv", "nref", "nlevels", "nmax", "mmax", "ngbnd", "nkcuv2", "nsrc", "nbnd",
+        "subgrid", "quadtree", "dense")] + [("dtmax", C.c_float), ("dtmin", C.c_float)]
+
+
+class sfx_step_stats(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt", C.c_float), ("min_dt", C.c_float), ("nt", C.c_int64),
+                ("error", C.c_int32), ("cell_updates", C.c_double), ("seconds", C.c_double)]
+
+
+def _sig(lib):
+    P = C.c_void_p
+    I = C.c_int
+    f = C.c_float
+    d = C.c_double
+    sz = C.c_size_t
+    s = C.c_char_p
+    i32 = C.c_int32
+    table = {
+        "sfx_params_default": (None, [C.POINTER(sfx_params)]),
+        "sfx_model_create_regular": (I, [C.POINTER(sfx_params), i32, i32, f, f, f, f, f, P, C.POINTER(P)]),
+        "sfx_model_create_quadtree": (I, [C.POINTER(sfx_params), C.POINTER(sfx_quadtree_input), C.POINTER(P)]),
+        "sfx_model_sizes": (I, [P, C.POINTER(sfx_sizes)]),
+        "sfx_model_get_table": (I, [P, s, P, sz]),
+        "sfx_model_table_bytes": (I, [P, s, C.POINTER(sz)]),
+        "sfx_model_set_bathymetry": (I, [P, P]),
+        "sfx_model_set_subgrid": (I, [P, C.POINTER(sfx_subgrid_input)]),
+        "sfx_model_set_boundary": (I, [P, i32, P, P, i32, P, P]),
+        "sfx_model_set_sources": (I, [P, i32, P, P, i32, P, P]),
+        "sfx_model_set_rain": (I, [P, i32, P, P]),
+        "sfx_model_set_wind": (I, [P, i32, P, P, P]),
+        "sfx_model_initialize": (I, [P]),
+        "sfx_model_get_state": (I, [P, s, P, sz]),
+        "sfx_model_destroy": (None, [P]),
+        "sfx_model_footprint": (I, [P, C.POINTER(sz)]),
+        "sfx_create": (I, [P, i32, C.POINTER(P)]),
+        "sfx_destroy": (I, [P]),
+        "sfx_get_state": (I, [P, s, P, sz]),
+        "sfx_set_state": (I, [P, s, P, sz]),
+        "sfx_compute_fluxes": (I, [P, f, C.POINTER(f)]),
+        "sfx_compute_water_levels": (I, [P, d, f]),
+        "sfx_update_forcing": (I, [P, d, f]),
+        "sfx_update": (I, [P, d, C.POINTER(sfx_step_stats)]),
+        "sfx_get_time": (I, [P, C.POINTER(d), C.POINTER(f), C.POINTER(C.c_int64)]),
+        "sfx_set_counting": (I, [P, i32]),
+        "sfx_get_counters": (I, [P, C.POINTER(C.c_int64)]),
+        "sfx_total_volume": (I, [P, C.POINTER(d)]),
+        "sfx_batch_create": (I, [i32, C.POINTER(P)]),
+        "sfx_batch_destroy": (I, [P]),
+        "sfx_batch_add": (I, [P, P, d]),
+        "sfx_batch_remove": (I, [P, P]),
+        "sfx_batch_advance": (I, [P, i32, C.POINTER(i32)]),
+        "sfx_batch_done": (I, [P, P, C.POINTER(i32)]),
+        "sfx_autotune": (I, [P, i32]),
+        "sfx_get_tuning": (I, [P, C.POINTER(i32)]),
+        "sfx_last_error": (I, [C.c_char_p, sz]),
+    }
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Load libsfx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SfxError(f"{LIB_PATH} not built; run `python -m sfincs_amd.build`")
+        _lib = C.CDLL(LIB_PATH)
+        _sig(_lib)
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    lib().sfx_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def check(rc, what=""):
+    if rc < 0:
+        raise SfxError(f"{what}: {last_error()} (rc={rc})")
+    return rc
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def default_params(**over):
+    p = sfx_params()
+    lib().sfx_params_default(C.byref(p))
+    for k, v in over.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+TABLE_DTYPES = {
+    "i8": ("kcs", "z_flags_iref", "uv_flags_iref", "uv_flags_type", "uv_flags_dir", "kcuv", "mask_adv",
+           "kfuv", "ibuvdir"),
+    "f64": ("zs", "z_volume"),
+    "i32": ("z_index_z_n", "z_index_z_m", "z_index_uv_md", "z_index_uv_mu", "z_index_uv_nd", "z_index_uv_nu",
+            "z_index_uv_md1", "z_index_uv_md2", "z_index_uv_mu1", "z_index_uv_mu2", "z_index_uv_nd1",
+            "z_index_uv_nd2", "z_index_uv_nu1", "z_index_uv_nu2", "uv_index_z_nm", "uv_index_z_nmu",
+            "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm", "uv_index_u_num", "uv_index_v_ndm",
+            "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
+            "cuv_index_uv2", "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp",
+            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc"),
+}
+
+
+def dtype_of(name):
+    for dt, names in TABLE_DTYPES.items():
+        if name in names:
+            return {"i8": np.int8, "f64": np.float64, "i32": np.int32}[dt]
+    return np.float32
+
+
+class Model:
+    """Host-side model (tables + initial state); wraps sfx_model."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def regular(cls, params, nmax, mmax, dx, dy, msk, x0=0.0, y0=0.0, rotation=0.0):
+        msk = np.ascontiguousarray(msk, dtype=np.int8).reshape(-1, order="C")
+        if msk.size != nmax * mmax:
+            raise ValueError("msk must have nmax*mmax entries (n fastest)")
+        out = C.c_void_p()
+        check(lib().sfx_model_create_regular(C.byref(params), nmax, mmax, x0, y0, dx, dy, rotation,
+                                             ptr(msk), C.byref(out)), "create_regular")
+        m = cls(out)
+        m._keep = [msk]
+        return m
+
+    @classmethod
+    def quadtree(cls, params, qt):
+        """qt: dict with quadtree-file arrays (level, n, m, md, md1, ..., mask) and x0,y0,dx,dy."""
+        arrs = {}
+        s = sfx_quadtree_input()
+        s.nr_points = len(qt["n"])
+        s.nr_levels = int(qt["nr_levels"])
+        s.x0, s.y0, s.dx, s.dy = qt["x0"], qt["y0"], qt["dx"], qt["dy"]
+        s.rotation_deg = qt.get("rotation", 0.0)
+        for k in ("level", "md", "mu", "nd", "nu", "mask"):
+            arrs[k] = np.ascontiguousarray(qt[k], dtype=np.int8)
+            setattr(s, k, arrs[k].ctypes.data)
+        for k in ("n", "m", "md1", "md2", "mu1", "mu2", "nd1", "nd2", "nu1", "nu2"):
+            arrs[k] = np.ascontiguousarray(qt[k], dtype=np.int32)
+            setattr(s, k, arrs[k].ctypes.data)
+        out = C.c_void_p()
+        check(lib().sfx_model_create_quadtree(C.byref(params), C.byref(s), C.byref(out)), "create_quadtree")
+        return cls(out)
+
+    def sizes(self):
+        s = sfx_sizes()
+        check(lib().sfx_model_sizes(self.h, C.byref(s)), "sizes")
+        return {k: getattr(s, k) for k, _ in sfx_sizes._fields_}
+
+    def table(self, name):
+        n = C.c_size_t()
+        check(lib().sfx_model_table_bytes(self.h, name.encode(), C.byref(n)), name)
+        dt = dtype_of(name)
+        a = np.empty(n.value // np.dtype(dt).itemsize, dtype=dt)
+        check(lib().sfx_model_get_table(self.h, name.encode(), ptr(a), n.value), name)
+        return a
+
+    def set_bathymetry(self, zb):
+        zb = np.ascontiguousarray(zb, dtype=np.float32)
+        check(lib().sfx_model_set_bathymetry(self.h, ptr(zb)), "set_bathymetry")
+
+    def set_subgrid(self, sg):
+        s = sfx_subgrid_input()
+        s.nlevels = int(sg["nlevels"])
+        keep = []
+        for k in ("z_zmin", "z_zmax", "z_volmax", "z_level", "uv_zmin", "uv_zmax", "uv_navg", "uv_ffit",
+                  "uv_havg", "uv_nrep", "uv_pwet"):
+            a = np.ascontiguousarray(sg[k], dtype=np.float32)
+            keep.append(a)
+            setattr(s, k, a.ctypes.data)
+        check(lib().sfx_model_set_subgrid(self.h, C.byref(s)), "set_subgrid")
+
+    def set_boundary(self, x, y, t=None, zs=None):
+        x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
+        if t is None:
+            check(lib().sfx_model_set_boundary(self.h, len(x), ptr(x), ptr(y), 0, None, None), "set_boundary")
+            return
+        t = np.ascontiguousarray(t, np.float32); zs = np.ascontiguousarray(zs, np.float32)
+        check(lib().sfx_model_set_boundary(self.h, len(x), ptr(x), ptr(y), len(t), ptr(t), ptr(zs)), "set_boundary")
+
+    def set_sources(self, x, y, t, q):
+        x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
+        t = np.ascontiguousarray(t, np.float32); q = np.ascontiguousarray(q, np.float32)
+        check(lib().sfx_model_set_sources(self.h, len(x), ptr(x), ptr(y), len(t), ptr(t), ptr(q)), "set_sources")
+
+    def set_rain(self, t, mmh):
+        t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)
+        check(lib().sfx_model_set_rain(self.h, len(t), ptr(t), ptr(v)), "set_rain")
+
+    def set_wind(self, t, mag, direction):
+        t = np.ascontiguousarray(t, np.float32); a = np.ascontiguousarray(mag, np.float32)
+        b = np.ascontiguousarray(direction, np.float32)
+        check(lib().sfx_model_set_wind(self.h, len(t), ptr(t), ptr(a), ptr(b)), "set_wind")
+
+    def initialize(self):
+        check(lib().sfx_model_initialize(self.h), "initialize")
+        return self
+
+    def footprint(self):
+        n = C.c_size_t()
+        check(lib().sfx_model_footprint(self.h, C.byref(n)), "footprint")
+        return n.value
+
+    def close(self):
+        if self.h:
+            lib().sfx_model_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv": np.float32,
+                "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
+                "uvmean": np.float32}
+
+
+class Instance:
+    """A model resident in HBM (sfx_handle). One instance per scenario."""
+
+    def __init__(self, model, device=0):
+        out = C.c_void_p()
+        check(lib().sfx_create(model.h, device, C.byref(out)), "sfx_create")
+        self.h = out
+        self.sizes = model.sizes()
+
+    def _n(self, name):
+        s = self.sizes
+        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax"):
+            return s["np"]
+        if name in ("q", "uv"):
+            return s["npuv"] + s["ncuv"] + 1
+        if name == "kfuv":
+            return s["npuv"]
+        if name == "uvmean":
+            return s["nkcuv2"]
+        raise KeyError(name)
+
+    def get(self, name):
+        a = np.empty(self._n(name), dtype=STATE_FIELDS[name])
+        check(lib().sfx_get_state(self.h, name.encode(), ptr(a), a.nbytes), f"get {name}")
+        return a
+
+    def set(self, name, a):
+        a = np.ascontiguousarray(a, dtype=STATE_FIELDS[name])
+        check(lib().sfx_set_state(self.h, name.encode(), ptr(a), a.nbytes), f"set {name}")
+
+    def compute_fluxes(self, dt):
+        out = C.c_float()
+        check(lib().sfx_compute_fluxes(self.h, dt, C.byref(out)), "compute_fluxes")
+        return out.value
+
+    def compute_water_levels(self, t, dt):
+        check(lib().sfx_compute_water_levels(self.h, t, dt), "compute_water_levels")
+
+    def update_forcing(self, t, dt):
+        check(lib().sfx_update_forcing(self.h, t, dt), "update_forcing")
+
+    def update(self, dtrange=-1000.0):
+        st = sfx_step_stats()
+        rc = check(lib().sfx_update(self.h, dtrange, C.byref(st)), "update")
+        return rc, {k: getattr(st, k) for k, _ in sfx_step_stats._fields_}
+
+    def time(self):
+        t = C.c_double(); dt = C.c_float(); nt = C.c_int64()
+        check(lib().sfx_get_time(self.h, C.byref(t), C.byref(dt), C.byref(nt)), "get_time")
+        return t.value, dt.value, nt.value
+
+    def set_counting(self, on):
+        check(lib().sfx_set_counting(self.h, int(on)), "set_counting")
+
+    def counters(self):
+        a = (C.c_int64 * 4)()
+        check(lib().sfx_get_counters(self.h, a), "get_counters")
+        return list(a)
+
+    def total_volume(self):
+        v = C.c_double()
+        check(lib().sfx_total_volume(self.h, C.byref(v)), "total_volume")
+        return v.value
+
+    def autotune(self, trials=3):
+        check(lib().sfx_autotune(self.h, trials), "autotune")
+        a = (C.c_int32 * 4)()
+        check(lib().sfx_get_tuning(self.h, a), "get_tuning")
+        return list(a)
+
+    def close(self):
+        if self.h:
+            lib().sfx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Members of uneven size advanced together (sfx_batch)."""
+
+    def __init__(self, device=0):
+        out = C.c_void_p()
+        check(lib().sfx_batch_create(device, C.byref(out)), "batch_create")
+        self.h = out
+        self.members = []
+
+    def add(self, inst, t_end):
+        check(lib().sfx_batch_add(self.h, inst.h, t_end), "batch_add")
+        self.members.append(inst)
+
+    def remove(self, inst):
+        check(lib().sfx_batch_remove(self.h, inst.h), "batch_remove")
+        self.members.remove(inst)
+
+    def advance(self, nsteps):
+        n = C.c_int32()
+        check(lib().sfx_batch_advance(self.h, nsteps, C.byref(n)), "batch_advance")
+        return n.value
+
+    def done(self, inst):
+        d = C.c_int32()
+        check(lib().sfx_batch_done(self.h, inst.h, C.byref(d)), "batch_done")
+        return bool(d.value)
+
+    def close(self):
+        if self.h:
+            lib().sfx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

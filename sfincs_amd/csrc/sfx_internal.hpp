@@ -1,0 +1,96 @@
+// Internal declarations shared by the host model builder and the device runtime.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sfx.h"
+
+namespace sfx {
+
+void set_error(const std::string& msg);
+
+// 1-based reference index -> 0-based storage slot.
+inline int32_t z0(int32_t one_based) { return one_based - 1; }
+
+}  // namespace sfx
+
+// Host model: the reference's sfincs_data arrays after sfincs_initialize, in the
+// reference's element order. Index tables hold 1-based values; storage is 0-based.
+struct sfx_model {
+  sfx_params p{};
+  // derived switches (sfincs_input.f90:347-745)
+  bool subgrid = false, use_quadtree = false, dense = false;
+  bool coriolis = false, thetasmoothing = false, precip = false, wind = false;
+  bool mesh_built = false, initialized = false;
+  float fcorio = 0.f, rotation = 0.f, cosrot = 1.f, sinrot = 0.f;
+  float x0 = 0.f, y0 = 0.f, dx = 0.f, dy = 0.f;
+  int32_t nmax = 0, mmax = 0;        // regular: grid size; quadtree: level-1 extent
+  int32_t np = 0, npuv = 0, ncuv = 0, nref = 1, nl = 0;
+  int32_t qt_nmax = 0;                // quadtree_nmax used by point lookup
+
+  // quadtree point list (for point lookup: level-major, nm sorted)
+  std::vector<int8_t> qt_level;
+  std::vector<int32_t> qt_nm, qt_first, qt_last, index_sfincs_in_quadtree;
+
+  // cells
+  std::vector<int8_t> kcs, z_flags_iref;
+  std::vector<int32_t> z_index_z_n, z_index_z_m;
+  std::vector<int32_t> z_index_uv_md, z_index_uv_mu, z_index_uv_nd, z_index_uv_nu;
+  std::vector<int32_t> z_index_uv_md1, z_index_uv_md2, z_index_uv_mu1, z_index_uv_mu2;
+  std::vector<int32_t> z_index_uv_nd1, z_index_uv_nd2, z_index_uv_nu1, z_index_uv_nu2;
+  std::vector<float> z_xz, z_yz, zb;
+
+  // uv-points
+  std::vector<int32_t> uv_index_z_nm, uv_index_z_nmu;
+  std::vector<int32_t> uv_index_u_nmd, uv_index_u_nmu, uv_index_u_ndm, uv_index_u_num;
+  std::vector<int32_t> uv_index_v_ndm, uv_index_v_nm, uv_index_v_ndmu, uv_index_v_nmu;
+  std::vector<int8_t> uv_flags_iref, uv_flags_type, uv_flags_dir;
+  std::vector<int8_t> kcuv;       // npuv + ncuv + 1
+  std::vector<int8_t> mask_adv;   // npuv
+  std::vector<int32_t> cuv_index_uv, cuv_index_uv1, cuv_index_uv2;
+  std::vector<float> zbuv, zbuvmx, gn2uv;
+
+  // refinement levels
+  std::vector<float> dxr, dyr, dxyr, dxrm, dyrm, dxrinv, dyrinv, dxr2inv, dyr2inv;
+  std::vector<float> dxrinvc, dyrinvc, cell_area, nuvisc;
+  float dxymin = 0.f, dtmax = 0.f, dtmin = 0.f;
+
+  // subgrid tables after sanitising (level fastest)
+  std::vector<float> z_zmin, z_zmax, z_volmax, z_dep;
+  std::vector<float> uv_zmin, uv_zmax, uv_navg_w, uv_fnfit, uv_havg, uv_nrep, uv_pwet;
+  std::vector<float> uv_havg_zmax, uv_nrep_zmax;
+
+  // initial state
+  std::vector<double> zs, z_volume;
+  std::vector<float> q, uv, zs0, zsderv, zsmax;
+  std::vector<int8_t> kfuv;
+
+  // boundaries
+  int32_t ngbnd = 0, nkcuv2 = 0, nbnd = 0, ntbnd = 0;
+  std::vector<int32_t> nmindbnd;
+  std::vector<float> zsb, zsb0;
+  std::vector<int32_t> index_kcuv2, nmbkcuv2, nmikcuv2, ibkcuv2;
+  std::vector<int8_t> ibuvdir;
+  std::vector<float> uvmean;
+  std::vector<float> x_bnd, y_bnd, t_bnd, zs_bnd;   // zs_bnd[it*nbnd + ib]
+  std::vector<int32_t> ind1_bnd_gbp, ind2_bnd_gbp, nmi_gbp;
+  std::vector<float> fac_bnd_gbp;
+
+  // point sources
+  int32_t nsrc = 0, ntsrc = 0;
+  std::vector<float> x_src, y_src;
+  std::vector<int32_t> nmindsrc;
+  std::vector<float> tsrc, qsrc;   // qsrc[it*nsrc + isrc]
+
+  // meteo series
+  int32_t ntprcp = 0, ntwnd = 0;
+  std::vector<float> tprcpt, tprcpv, twnd, wndmag, wnddir;
+  std::vector<float> cdval;   // 1000 entries
+
+  // helpers
+  int32_t dummy_uv() const { return npuv + ncuv + 1; }
+  int32_t find_cell_by_index(int32_t n, int32_t m, int32_t iref) const;   // sfincs cell (1-based) or 0
+  int32_t find_cell_xy(float x, float y) const;                          // sfincs cell (1-based) or 0
+};

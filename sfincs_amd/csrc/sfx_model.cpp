@@ -1,0 +1,879 @@
+// Host-side model builder: the init-time part of sfincs_initialize that produces the
+// tables the time step reads. Everything here runs once on the CPU; the resulting arrays
+// are moved to HBM by sfx_create. Element orders and integer conventions are the
+// reference's so that every index table and mask is bit-exact with it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "sfx_internal.hpp"
+
+namespace sfx {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace sfx
+
+extern "C" int sfx_last_error(char* buf, size_t n) {
+  if (!buf || n == 0) return SFX_ERR_ARG;
+  std::strncpy(buf, sfx::g_last_error.c_str(), n - 1);
+  buf[n - 1] = 0;
+  return SFX_OK;
+}
+
+// Defaults of read_sfincs_input (sfincs_input.f90:50-343) and constants (:372-373).
+extern "C" void sfx_params_default(sfx_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->t0 = 0.f; p->t1 = 0.f; p->tspinup = 0.f;
+  p->alpha = 0.5f; p->theta = 1.0f; p->hmin_cfl = 0.1f; p->manning = 0.04f;
+  p->manning_land = -999.0f; p->manning_sea = -999.0f; p->rgh_lev_land = 0.0f;
+  p->zsini = 0.0f; p->dtmax = 60.0f; p->huthresh = 0.05f; p->huvmin = 0.0f;
+  p->rhoa = 1.25f; p->rhow = 1024.0f; p->advlim = 1.0f; p->slopelim = 9999.9f;
+  p->btfilter = 60.0f; p->btrelax = 3600.0f; p->wiggle_factor = 0.1f; p->wiggle_threshold = 0.1f;
+  p->uvlim = 10.0f; p->uvmax = 1000.0f; p->nuvisc = 0.01f; p->nuviscfac = 100.0f;
+  p->latitude = 0.0f; p->g = 9.81f; p->pi = 3.14159f;
+  p->advection = 1; p->advection_scheme = 1; p->viscosity = 0; p->coriolis = 1;
+  p->friction2d = 1; p->advection_mask = 1; p->wiggle_suppression = 1; p->store_zsmax = 1;
+  p->bndtype = 1; p->h73table = 0;
+}
+
+namespace {
+
+using sfx::set_error;
+
+// Derived switches of read_sfincs_input (sfincs_input.f90:369-419, 611-619).
+void derive_switches(sfx_model* M) {
+  const sfx_params& p = M->p;
+  M->rotation = M->rotation;  // set by the mesh constructor (radians)
+  M->coriolis = p.coriolis != 0;
+  M->fcorio = 2 * 7.2921e-05f * std::sin(p.latitude * p.pi / 180);
+  if (p.latitude < 0.01f && p.latitude > -0.01f) M->coriolis = false;
+  M->thetasmoothing = p.theta < 0.9999f;
+}
+
+bool need_uv_neighbours(const sfx_model* M) {
+  const sfx_params& p = M->p;
+  return p.advection || M->coriolis || p.viscosity || p.friction2d || M->thetasmoothing;
+}
+
+// Quadtree description in file semantics (what quadtree_read_file or
+// make_quadtree_from_indices leave behind).
+struct QT {
+  int32_t npts = 0, nlev = 1, nmax = 0, mmax = 0;
+  float x0 = 0, y0 = 0, dx = 0, dy = 0, rot = 0;
+  std::vector<int8_t> level, md, mu, nd, nu, mask;
+  std::vector<int32_t> n, m, md1, md2, mu1, mu2, nd1, nd2, nu1, nu2;
+};
+
+// make_quadtree_from_indices (sfincs_quadtree.F90:445-669), non-global branch.
+void regular_to_qt(int32_t nmax, int32_t mmax, const int8_t* msk, QT& q) {
+  std::vector<int32_t> indices;
+  for (int32_t m = 1; m <= mmax; ++m)
+    for (int32_t n = 1; n <= nmax; ++n)
+      if (msk[(size_t)(m - 1) * nmax + (n - 1)] > 0) indices.push_back((m - 1) * nmax + n);
+  const int32_t np = (int32_t)indices.size();
+  std::vector<int32_t> g((size_t)nmax * mmax, 0);   // index_g_nm(n, m)
+  q.npts = np; q.nlev = 1; q.nmax = nmax; q.mmax = mmax;
+  q.level.assign(np, 1); q.md.assign(np, 0); q.mu.assign(np, 0); q.nd.assign(np, 0); q.nu.assign(np, 0);
+  q.n.resize(np); q.m.resize(np);
+  q.md1.assign(np, 0); q.md2.assign(np, 0); q.mu1.assign(np, 0); q.mu2.assign(np, 0);
+  q.nd1.assign(np, 0); q.nd2.assign(np, 0); q.nu1.assign(np, 0); q.nu2.assign(np, 0);
+  q.mask.resize(np);
+  for (int32_t k = 0; k < np; ++k) {
+    int32_t m = (indices[k] - 1) / nmax + 1;
+    int32_t n = indices[k] - (m - 1) * nmax;
+    q.n[k] = n; q.m[k] = m;
+    g[(size_t)(m - 1) * nmax + (n - 1)] = k + 1;
+    q.mask[k] = msk[(size_t)(m - 1) * nmax + (n - 1)];
+  }
+  auto G = [&](int32_t n, int32_t m) { return g[(size_t)(m - 1) * nmax + (n - 1)]; };
+  for (int32_t k = 0; k < np; ++k) {
+    const int32_t n = q.n[k], m = q.m[k];
+    if (m < mmax) q.mu1[k] = G(n, m + 1);
+    if (m > 1) q.md1[k] = G(n, m - 1);
+    if (n < nmax) q.nu1[k] = G(n + 1, m);
+    if (n > 1) q.nd1[k] = G(n - 1, m);
+  }
+}
+
+// initialize_mesh (sfincs_domain.f90:143-1586) for projected coordinates.
+int build_mesh(sfx_model* M, const QT& Q) {
+  const int32_t nq = Q.npts;
+  M->nref = Q.nlev;
+  // point lookup arrays (sfincs_quadtree.F90:118-186)
+  M->qt_level = Q.level;
+  M->qt_nmax = 0;
+  for (int32_t k = 0; k < nq; ++k) {
+    int32_t iref = Q.level[k];
+    int32_t p2 = 1 << (iref - 1);
+    M->qt_nmax = std::max(M->qt_nmax, (int32_t)((1.0f * (Q.n[k] - 1) + 0.01f) / p2) + 1);
+  }
+  if (!M->use_quadtree) M->qt_nmax = Q.nmax;
+  M->qt_nm.resize(nq);
+  M->qt_first.assign(M->nref + 1, 0);
+  M->qt_last.assign(M->nref + 1, 0);
+  int32_t ireflast = 0;
+  for (int32_t k = 0; k < nq; ++k) {
+    int32_t iref = Q.level[k];
+    int32_t nmx = M->qt_nmax * (1 << (iref - 1));
+    M->qt_nm[k] = (Q.m[k] - 1) * nmx + Q.n[k];
+    if (iref > ireflast) { M->qt_first[iref] = k + 1; ireflast = iref; }
+    M->qt_last[iref] = k + 1;
+  }
+
+  // compaction of msk > 0 points (:327-482)
+  M->index_sfincs_in_quadtree.assign(nq, 0);
+  std::vector<int32_t> q_of_s;
+  for (int32_t k = 0; k < nq; ++k)
+    if (Q.mask[k] > 0) { q_of_s.push_back(k + 1); M->index_sfincs_in_quadtree[k] = (int32_t)q_of_s.size(); }
+  const int32_t np = (int32_t)q_of_s.size();
+  if (np == 0) { set_error("mesh has no active cells"); return SFX_ERR_ARG; }
+  M->np = np;
+  auto S = [&](int32_t qidx) { return qidx > 0 ? M->index_sfincs_in_quadtree[qidx - 1] : 0; };
+  std::vector<int8_t> fmd(np), fmu(np), fnd(np), fnu(np);
+  std::vector<int32_t> md1(np), md2(np), mu1(np), mu2(np), nd1(np), nd2(np), nu1(np), nu2(np);
+  M->kcs.resize(np); M->z_flags_iref.resize(np); M->z_index_z_n.resize(np); M->z_index_z_m.resize(np);
+  for (int32_t ip = 0; ip < np; ++ip) {
+    const int32_t iq = q_of_s[ip] - 1;
+    M->kcs[ip] = Q.mask[iq];
+    M->z_flags_iref[ip] = Q.level[iq];
+    M->z_index_z_n[ip] = Q.n[iq];
+    M->z_index_z_m[ip] = Q.m[iq];
+    fnu[ip] = Q.nu[iq]; fnd[ip] = Q.nd[iq]; fmu[ip] = Q.mu[iq]; fmd[ip] = Q.md[iq];
+    nu1[ip] = S(Q.nu1[iq]); nu2[ip] = S(Q.nu2[iq]); nd1[ip] = S(Q.nd1[iq]); nd2[ip] = S(Q.nd2[iq]);
+    mu1[ip] = S(Q.mu1[iq]); mu2[ip] = S(Q.mu2[iq]); md1[ip] = S(Q.md1[iq]); md2[ip] = S(Q.md2[iq]);
+  }
+
+  // count u and v points (:486-533)
+  int32_t npu = 0, npv = 0;
+  for (int32_t nm = 0; nm < np; ++nm) {
+    if (fmu[nm] < 1) { if (mu1[nm] > 0) ++npu; }
+    else { if (mu1[nm] > 0) ++npu; if (mu2[nm] > 0) ++npu; }
+    if (fnu[nm] < 1) { if (nu1[nm] > 0) ++npv; }
+    else { if (nu1[nm] > 0) ++npv; if (nu2[nm] > 0) ++npv; }
+  }
+  const int32_t npuv = npu + npv;
+  // combined uv points at refinement transitions (:535-596)
+  int32_t ncuv = 0;
+  if (Q.nlev > 1) {
+    for (int32_t nm = 0; nm < np; ++nm) {
+      if (fmd[nm] == 1 && (md1[nm] > 0 || md2[nm] > 0)) ++ncuv;
+      if (fmu[nm] == 1 && (mu1[nm] > 0 || mu2[nm] > 0)) ++ncuv;
+      if (fnd[nm] == 1 && (nd1[nm] > 0 || nd2[nm] > 0)) ++ncuv;
+      if (fnu[nm] == 1 && (nu1[nm] > 0 || nu2[nm] > 0)) ++ncuv;
+    }
+  }
+  M->npuv = npuv; M->ncuv = ncuv;
+  const int32_t dummy = npuv + ncuv + 1;
+  M->z_index_uv_md.assign(np, dummy); M->z_index_uv_mu.assign(np, dummy);
+  M->z_index_uv_nd.assign(np, dummy); M->z_index_uv_nu.assign(np, dummy);
+  M->z_index_uv_md1.assign(np, 0); M->z_index_uv_md2.assign(np, 0);
+  M->z_index_uv_mu1.assign(np, 0); M->z_index_uv_mu2.assign(np, 0);
+  M->z_index_uv_nd1.assign(np, 0); M->z_index_uv_nd2.assign(np, 0);
+  M->z_index_uv_nu1.assign(np, 0); M->z_index_uv_nu2.assign(np, 0);
+  M->cuv_index_uv.assign(ncuv, 0);
+  M->cuv_index_uv1.assign(ncuv, dummy); M->cuv_index_uv2.assign(ncuv, dummy);
+  std::vector<int32_t> cmd(np, 0), cmu(np, 0), cnd(np, 0), cnu(np, 0);
+  {
+    int32_t irefuv = 0;
+    for (int32_t nm = 0; nm < np; ++nm) {
+      if (fmd[nm] == 1 && (md1[nm] > 0 || md2[nm] > 0)) { ++irefuv; M->cuv_index_uv[irefuv - 1] = npuv + irefuv; cmd[nm] = irefuv; }
+      if (fmu[nm] == 1 && (mu1[nm] > 0 || mu2[nm] > 0)) { ++irefuv; M->cuv_index_uv[irefuv - 1] = npuv + irefuv; cmu[nm] = irefuv; }
+      if (fnd[nm] == 1 && (nd1[nm] > 0 || nd2[nm] > 0)) { ++irefuv; M->cuv_index_uv[irefuv - 1] = npuv + irefuv; cnd[nm] = irefuv; }
+      if (fnu[nm] == 1 && (nu1[nm] > 0 || nu2[nm] > 0)) { ++irefuv; M->cuv_index_uv[irefuv - 1] = npuv + irefuv; cnu[nm] = irefuv; }
+    }
+  }
+  // uv points (:680-1003); all values below are 1-based reference indices
+  M->kcuv.assign(dummy, 0);
+  M->uv_index_z_nm.assign(npuv, 0); M->uv_index_z_nmu.assign(npuv, 0);
+  M->uv_flags_iref.assign(npuv, 0); M->uv_flags_dir.assign(npuv, 0); M->uv_flags_type.assign(npuv, 0);
+  auto& zmd = M->z_index_uv_md; auto& zmu = M->z_index_uv_mu; auto& znd = M->z_index_uv_nd; auto& znu = M->z_index_uv_nu;
+  auto& zmd1 = M->z_index_uv_md1; auto& zmd2 = M->z_index_uv_md2; auto& zmu1 = M->z_index_uv_mu1; auto& zmu2 = M->z_index_uv_mu2;
+  auto& znd1 = M->z_index_uv_nd1; auto& znd2 = M->z_index_uv_nd2; auto& znu1 = M->z_index_uv_nu1; auto& znu2 = M->z_index_uv_nu2;
+  auto& cuv1 = M->cuv_index_uv1; auto& cuv2 = M->cuv_index_uv2; auto& cuv = M->cuv_index_uv;
+  int32_t ip = 0;
+  auto set_uv = [&](int32_t nmc, int32_t nbr, int dir, int type, int iref) {
+    ++ip;
+    M->uv_index_z_nm[ip - 1] = nmc; M->uv_index_z_nmu[ip - 1] = nbr;
+    M->uv_flags_iref[ip - 1] = (int8_t)iref; M->uv_flags_dir[ip - 1] = (int8_t)dir; M->uv_flags_type[ip - 1] = (int8_t)type;
+  };
+  for (int32_t c = 0; c < np; ++c) {
+    const int32_t nm = c + 1;
+    const int iref = M->z_flags_iref[c];
+    // right
+    if (fmu[c] == 0) {
+      if (mu1[c] > 0) {
+        const int32_t nmu = mu1[c];
+        set_uv(nm, nmu, 0, 0, iref);
+        zmu[c] = ip; zmd[nmu - 1] = ip; zmu1[c] = ip; zmd1[nmu - 1] = ip;
+      }
+    } else if (fmu[c] == -1) {
+      if (mu1[c] > 0) {
+        const int32_t nmu = mu1[c];
+        set_uv(nm, nmu, 0, -1, iref);
+        zmu[c] = ip; zmu1[c] = ip;
+        const int32_t icuv = cmd[nmu - 1];
+        if (md1[mu1[c] - 1] == nm) { cuv1[icuv - 1] = ip; zmd1[nmu - 1] = ip; }
+        else { cuv2[icuv - 1] = ip; zmd2[nmu - 1] = ip; }
+        zmd[nmu - 1] = cuv[icuv - 1];
+      }
+    } else {
+      if (mu1[c] > 0) {
+        const int32_t nmu = mu1[c];
+        set_uv(nm, nmu, 0, 1, iref + 1);
+        zmu1[c] = ip; zmd[nmu - 1] = ip; zmd1[nmu - 1] = ip;
+        const int32_t icuv = cmu[c];
+        zmu[c] = cuv[icuv - 1];
+        cuv1[icuv - 1] = ip;
+      }
+      if (mu2[c] > 0) {
+        const int32_t nmu = mu2[c];
+        set_uv(nm, nmu, 0, 1, iref + 1);
+        zmu2[c] = ip; zmd[nmu - 1] = ip; zmd1[nmu - 1] = ip;
+        const int32_t icuv = cmu[c];
+        zmu[c] = cuv[icuv - 1];
+        cuv2[icuv - 1] = ip;
+      }
+    }
+    // above
+    if (fnu[c] == 0) {
+      if (nu1[c] > 0) {
+        const int32_t num = nu1[c];
+        set_uv(nm, num, 1, 0, iref);
+        znu[c] = ip; znd[num - 1] = ip; znu1[c] = ip; znd1[num - 1] = ip;
+      }
+    } else if (fnu[c] == -1) {
+      if (nu1[c] > 0) {
+        const int32_t num = nu1[c];
+        set_uv(nm, num, 1, -1, iref);
+        znu[c] = ip; znu1[c] = ip;
+        const int32_t icuv = cnd[num - 1];
+        if (nd1[nu1[c] - 1] == nm) { cuv1[icuv - 1] = ip; znd1[num - 1] = ip; }
+        else { cuv2[icuv - 1] = ip; znd2[num - 1] = ip; }
+        znd[num - 1] = cuv[icuv - 1];
+      }
+    } else {
+      if (nu1[c] > 0) {
+        const int32_t num = nu1[c];
+        set_uv(nm, num, 1, 1, iref + 1);
+        znd[num - 1] = ip; znu1[c] = ip; znd1[num - 1] = ip;
+        const int32_t icuv = cnu[c];
+        znu[c] = cuv[icuv - 1];
+        cuv1[icuv - 1] = ip;
+      }
+      if (nu2[c] > 0) {
+        const int32_t num = nu2[c];
+        set_uv(nm, num, 1, 1, iref + 1);
+        znd[num - 1] = ip; znu2[c] = ip; znd1[num - 1] = ip;
+        const int32_t icuv = cnu[c];
+        znu[c] = cuv[icuv - 1];
+        cuv2[icuv - 1] = ip;
+      }
+    }
+  }
+  if (ip != npuv) { set_error("inconsistent quadtree neighbour description"); return SFX_ERR_ARG; }
+
+  // eight uv neighbours of every uv point (:1005-1261)
+  M->uv_index_u_nmd.assign(npuv, 0); M->uv_index_u_nmu.assign(npuv, 0);
+  M->uv_index_u_ndm.assign(npuv, 0); M->uv_index_u_num.assign(npuv, 0);
+  M->uv_index_v_ndm.assign(npuv, 0); M->uv_index_v_nm.assign(npuv, 0);
+  M->uv_index_v_ndmu.assign(npuv, 0); M->uv_index_v_nmu.assign(npuv, 0);
+  if (need_uv_neighbours(M)) {
+    for (int32_t i = 0; i < npuv; ++i) {
+      const int32_t ipu = i + 1;
+      if (M->uv_flags_dir[i] == 0) {
+        const int32_t nm = M->uv_index_z_nm[i], nmu = M->uv_index_z_nmu[i];
+        M->uv_index_u_nmd[i] = zmd[nm - 1];
+        M->uv_index_u_nmu[i] = zmu[nmu - 1];
+        if (fnd[nm - 1] > -1) {
+          const int32_t ndm = nd1[nm - 1];
+          if (ndm > 0) M->uv_index_u_ndm[i] = zmu[ndm - 1];
+        } else {
+          const int32_t ndm = nd2[nm - 1];
+          if (ndm > 0) {
+            if (fmu[ndm - 1] == 0) M->uv_index_u_ndm[i] = zmu[ndm - 1];
+            else {
+              const int32_t ndmu = mu1[ndm - 1];
+              if (ndmu > 0) M->uv_index_u_ndm[i] = zmd[ndmu - 1];
+            }
+          }
+        }
+        if (fnu[nm - 1] > -1) {
+          const int32_t num = nu1[nm - 1];
+          if (num > 0) M->uv_index_u_num[i] = zmu[num - 1];
+        } else {
+          const int32_t num = nu2[nm - 1];
+          if (num > 0) {
+            if (fmu[num - 1] == 0) M->uv_index_u_num[i] = zmu[num - 1];
+            else {
+              const int32_t numu = mu1[num - 1];
+              if (numu > 0) M->uv_index_u_num[i] = zmd[numu - 1];
+            }
+          }
+        }
+        M->uv_index_v_nm[i] = znu[nm - 1];
+        M->uv_index_v_ndm[i] = znd[nm - 1];
+        M->uv_index_v_nmu[i] = znu[nmu - 1];
+        M->uv_index_v_ndmu[i] = znd[nmu - 1];
+      } else {
+        const int32_t nm = M->uv_index_z_nm[i], num = M->uv_index_z_nmu[i];
+        M->uv_index_u_nmd[i] = znd[nm - 1];
+        M->uv_index_u_nmu[i] = znu[num - 1];
+        if (fmd[nm - 1] > -1) {
+          const int32_t nmd = md1[nm - 1];
+          if (nmd > 0) M->uv_index_u_ndm[i] = znu[nmd - 1];
+        } else {
+          const int32_t nmd = md2[nm - 1];
+          if (nmd > 0) {
+            if (fnu[nmd - 1] == 0) M->uv_index_u_ndm[i] = znu[nmd - 1];
+            else {
+              const int32_t nmdu = nu1[nmd - 1];
+              if (nmdu > 0) M->uv_index_u_ndm[i] = znd[nmdu - 1];
+            }
+          }
+        }
+        if (fmu[nm - 1] > -1) {
+          const int32_t nmu = mu1[nm - 1];
+          if (nmu > 0) M->uv_index_u_num[i] = znu[nmu - 1];
+        } else {
+          const int32_t nmu = mu2[nm - 1];
+          if (nmu > 0) {
+            if (fnu[nmu - 1] == 0) M->uv_index_u_num[i] = znu[nmu - 1];
+            else {
+              const int32_t numu = nu1[nmu - 1];
+              if (numu > 0) M->uv_index_u_num[i] = znd[numu - 1];
+            }
+          }
+        }
+        M->uv_index_v_nm[i] = zmu[nm - 1];
+        M->uv_index_v_ndm[i] = zmd[nm - 1];
+        M->uv_index_v_nmu[i] = zmu[num - 1];
+        M->uv_index_v_ndmu[i] = zmd[num - 1];
+      }
+      if (M->uv_index_u_nmd[i] == 0) M->uv_index_u_nmd[i] = ipu;
+      if (M->uv_index_u_nmu[i] == 0) M->uv_index_u_nmu[i] = ipu;
+      if (M->uv_index_u_ndm[i] == 0) M->uv_index_u_ndm[i] = ipu;
+      if (M->uv_index_u_num[i] == 0) M->uv_index_u_num[i] = ipu;
+      if (M->uv_index_v_ndm[i] == 0) M->uv_index_v_ndm[i] = dummy;
+      if (M->uv_index_v_ndmu[i] == 0) M->uv_index_v_ndmu[i] = dummy;
+      if (M->uv_index_v_nm[i] == 0) M->uv_index_v_nm[i] = dummy;
+      if (M->uv_index_v_nmu[i] == 0) M->uv_index_v_nmu[i] = dummy;
+    }
+  }
+
+  // geometry per refinement level, projected coordinates (:1369-1541)
+  const int nref = M->nref;
+  M->dxr.resize(nref); M->dyr.resize(nref); M->dxyr.resize(nref);
+  for (int iref = 1; iref <= nref; ++iref) {
+    M->dxr[iref - 1] = M->dx / (float)(1 << (iref - 1));
+    M->dyr[iref - 1] = M->dy / (float)(1 << (iref - 1));
+    M->dxyr[iref - 1] = std::min(M->dxr[iref - 1], M->dyr[iref - 1]);
+  }
+  M->z_xz.resize(np); M->z_yz.resize(np);
+  for (int32_t c = 0; c < np; ++c) {
+    const int32_t n = M->z_index_z_n[c], m = M->z_index_z_m[c];
+    const int iref = M->z_flags_iref[c];
+    M->z_xz[c] = M->x0 + M->cosrot * (1.0f * ((float)m - 0.5f)) * M->dxr[iref - 1]
+               - M->sinrot * (1.0f * ((float)n - 0.5f)) * M->dyr[iref - 1];
+    M->z_yz[c] = M->y0 + M->sinrot * (1.0f * ((float)m - 0.5f)) * M->dxr[iref - 1]
+               + M->cosrot * (1.0f * ((float)n - 0.5f)) * M->dyr[iref - 1];
+  }
+  M->dxymin = 1.0e6f;
+  M->dxrm.resize(nref); M->dxrinv.resize(nref); M->dxr2inv.resize(nref); M->dxrinvc.assign(nref, 0.f);
+  M->dyrm.resize(nref); M->dyrinv.resize(nref); M->dyr2inv.resize(nref); M->dyrinvc.assign(nref, 0.f);
+  M->cell_area.resize(nref);
+  for (int i = 0; i < nref; ++i) {
+    M->dxrm[i] = M->dxr[i];
+    M->dxrinv[i] = 1.0f / M->dxrm[i];
+    M->dxr2inv[i] = M->dxrinv[i] * M->dxrinv[i];
+    M->dyrm[i] = M->dyr[i];
+    M->dyrinv[i] = 1.0f / M->dyrm[i];
+    M->dyr2inv[i] = M->dyrinv[i] * M->dyrinv[i];
+    M->cell_area[i] = M->dxrm[i] * M->dyrm[i];
+    if (i + 1 > 1) {
+      M->dxrinvc[i] = 1.0f / ((3 * M->dxrm[i]) / 2);
+      M->dyrinvc[i] = 1.0f / ((3 * M->dyrm[i]) / 2);
+    }
+    M->dxymin = std::min(M->dxymin, M->dxr[i]);
+    M->dxymin = std::min(M->dxymin, M->dyr[i]);
+  }
+  M->dtmax = std::min(M->p.dtmax, M->p.alpha * M->dxymin / std::sqrt(9.81f * M->p.hmin_cfl));
+  M->dtmin = M->p.alpha * M->dxymin / (1.25f * std::fabs(M->p.uvmax));
+  M->nuvisc.assign(nref, 0.f);
+  if (M->p.viscosity)
+    for (int i = 0; i < nref; ++i) M->nuvisc[i] = std::max(M->p.nuvisc * M->dxyr[i], 0.0f);
+  M->mesh_built = true;
+  return SFX_OK;
+}
+
+// initialize_mesh tail (:42-50): a single-level quadtree reverts to the regular path.
+void finish_mesh_flags(sfx_model* M) {
+  if (M->nref == 1) M->use_quadtree = false;
+  // Full rectangle with no holes: the device may use implicit neighbours.
+  M->dense = (!M->use_quadtree && M->nmax > 0 && (int64_t)M->np == (int64_t)M->nmax * M->mmax);
+}
+
+}  // namespace
+
+int32_t sfx_model::find_cell_by_index(int32_t n, int32_t m, int32_t iref) const {
+  // find_quadtree_cell_by_index (sfincs_quadtree.F90:822-845) + remap to sfincs cells
+  if (iref < 1 || iref > nref || qt_first[iref] == 0) return 0;
+  const int32_t nmx = qt_nmax * (1 << (iref - 1));
+  const int32_t nm = (m - 1) * nmx + n;
+  if (nm <= 0) return 0;
+  const int32_t i1 = qt_first[iref], i2 = qt_last[iref];
+  auto b = qt_nm.begin() + (i1 - 1), e = qt_nm.begin() + i2;
+  auto it = std::lower_bound(b, e, nm);
+  if (it == e || *it != nm) return 0;
+  const int32_t k = (int32_t)(it - qt_nm.begin());
+  if (qt_level[k] != iref) return 0;
+  return index_sfincs_in_quadtree[k];
+}
+
+int32_t sfx_model::find_cell_xy(float x, float y) const {
+  // find_quadtree_cell (sfincs_quadtree.F90:764-820)
+  const float xt = cosrot * (x - x0) + sinrot * (y - y0);
+  const float yt = -sinrot * (x - x0) + cosrot * (y - y0);
+  for (int iref = nref; iref >= 1; --iref) {
+    const int32_t n = (int32_t)(yt / dyr[iref - 1]) + 1;
+    const int32_t m = (int32_t)(xt / dxr[iref - 1]) + 1;
+    if (n < 1 || m < 1) continue;
+    const int32_t c = find_cell_by_index(n, m, iref);
+    if (c > 0) return c;
+  }
+  return 0;
+}
+
+extern "C" int sfx_model_create_regular(const sfx_params* p, int32_t nmax, int32_t mmax, float x0,
+                                        float y0, float dx, float dy, float rotation_deg,
+                                        const int8_t* msk, sfx_model** out) {
+  if (!p || !msk || !out || nmax < 1 || mmax < 1 || !(dx > 0) || !(dy > 0)) {
+    set_error("sfx_model_create_regular: invalid arguments");
+    return SFX_ERR_ARG;
+  }
+  sfx_model* M = new sfx_model();
+  M->p = *p;
+  M->nmax = nmax; M->mmax = mmax; M->x0 = x0; M->y0 = y0; M->dx = dx; M->dy = dy;
+  M->rotation = rotation_deg * p->pi / 180;   // sfincs_input.f90:378
+  M->cosrot = std::cos(M->rotation); M->sinrot = std::sin(M->rotation);
+  derive_switches(M);
+  M->use_quadtree = false;
+  QT Q;
+  regular_to_qt(nmax, mmax, msk, Q);
+  int rc = build_mesh(M, Q);
+  if (rc != SFX_OK) { delete M; return rc; }
+  finish_mesh_flags(M);
+  *out = M;
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_create_quadtree(const sfx_params* p, const sfx_quadtree_input* qt, sfx_model** out) {
+  if (!p || !qt || !out || qt->nr_points < 1 || qt->nr_levels < 1 || !(qt->dx > 0) || !(qt->dy > 0)) {
+    set_error("sfx_model_create_quadtree: invalid arguments");
+    return SFX_ERR_ARG;
+  }
+  const int32_t n = qt->nr_points;
+  QT Q;
+  Q.npts = n; Q.nlev = qt->nr_levels; Q.x0 = qt->x0; Q.y0 = qt->y0; Q.dx = qt->dx; Q.dy = qt->dy;
+  auto cp8 = [&](const int8_t* s, std::vector<int8_t>& d) { d.assign(s, s + n); };
+  auto cp32 = [&](const int32_t* s, std::vector<int32_t>& d) { d.assign(s, s + n); };
+  cp8(qt->level, Q.level); cp8(qt->md, Q.md); cp8(qt->mu, Q.mu); cp8(qt->nd, Q.nd); cp8(qt->nu, Q.nu);
+  cp8(qt->mask, Q.mask);
+  cp32(qt->n, Q.n); cp32(qt->m, Q.m); cp32(qt->md1, Q.md1); cp32(qt->md2, Q.md2); cp32(qt->mu1, Q.mu1);
+  cp32(qt->mu2, Q.mu2); cp32(qt->nd1, Q.nd1); cp32(qt->nd2, Q.nd2); cp32(qt->nu1, Q.nu1); cp32(qt->nu2, Q.nu2);
+  for (int32_t k = 0; k < n; ++k) {
+    if (Q.level[k] < 1 || Q.level[k] > Q.nlev || (k > 0 && Q.level[k] < Q.level[k - 1])) {
+      set_error("quadtree points must be sorted by level with levels in 1..nr_levels");
+      return SFX_ERR_ARG;
+    }
+    const int32_t nb[8] = {Q.md1[k], Q.md2[k], Q.mu1[k], Q.mu2[k], Q.nd1[k], Q.nd2[k], Q.nu1[k], Q.nu2[k]};
+    for (int j = 0; j < 8; ++j)
+      if (nb[j] < 0 || nb[j] > n) { set_error("quadtree neighbour index out of range"); return SFX_ERR_ARG; }
+  }
+  sfx_model* M = new sfx_model();
+  M->p = *p;
+  M->x0 = qt->x0; M->y0 = qt->y0; M->dx = qt->dx; M->dy = qt->dy;
+  // quadtree_read_file converts the rotation with its own pi (sfincs_quadtree.F90:81,99)
+  M->rotation = qt->rotation_deg * 3.141592653589793f / 180;
+  M->cosrot = std::cos(M->rotation); M->sinrot = std::sin(M->rotation);
+  derive_switches(M);
+  M->use_quadtree = true;
+  int rc = build_mesh(M, Q);
+  if (rc != SFX_OK) { delete M; return rc; }
+  M->nmax = 0; M->mmax = 0;
+  for (int32_t k = 0; k < n; ++k) {
+    if (Q.level[k] != 1) continue;
+    M->nmax = std::max(M->nmax, Q.n[k]); M->mmax = std::max(M->mmax, Q.m[k]);
+  }
+  finish_mesh_flags(M);
+  *out = M;
+  return SFX_OK;
+}
+
+extern "C" void sfx_model_destroy(sfx_model* m) { delete m; }
+
+extern "C" int sfx_model_sizes(const sfx_model* M, sfx_sizes* s) {
+  if (!M || !s) { set_error("sfx_model_sizes: null"); return SFX_ERR_ARG; }
+  std::memset(s, 0, sizeof(*s));
+  s->np = M->np; s->npuv = M->npuv; s->ncuv = M->ncuv; s->nref = M->nref; s->nlevels = M->nl;
+  s->nmax = M->nmax; s->mmax = M->mmax; s->ngbnd = M->ngbnd; s->nkcuv2 = M->nkcuv2;
+  s->nsrc = M->nsrc; s->nbnd = M->nbnd;
+  s->subgrid = M->subgrid; s->quadtree = M->use_quadtree; s->dense = M->dense;
+  s->dtmax = M->dtmax; s->dtmin = M->dtmin;
+  return SFX_OK;
+}
+
+// initialize_bathymetry, regular branch (sfincs_domain.f90:1608-1710)
+extern "C" int sfx_model_set_bathymetry(sfx_model* M, const float* zb) {
+  if (!M || !zb || !M->mesh_built || M->initialized) { set_error("sfx_model_set_bathymetry: bad state"); return SFX_ERR_STATE; }
+  M->subgrid = false;
+  M->zb.assign(zb, zb + M->np);
+  M->zbuv.resize(M->npuv); M->zbuvmx.resize(M->npuv);
+  for (int32_t i = 0; i < M->npuv; ++i) {
+    const float a = M->zb[M->uv_index_z_nm[i] - 1], b = M->zb[M->uv_index_z_nmu[i] - 1];
+    if (M->uv_flags_type[i] == 0) M->zbuv[i] = 0.5f * (a + b);
+    else if (M->uv_flags_type[i] == -1) M->zbuv[i] = 0.3333f * a + 0.6667f * b;
+    else M->zbuv[i] = 0.6667f * a + 0.3333f * b;
+    M->zbuvmx[i] = std::max(a, b) + M->p.huthresh;
+  }
+  return SFX_OK;
+}
+
+// read_subgrid_file_netcdf sanitising (sfincs_subgrid.F90:316-446) with huthresh = 0 (:40).
+extern "C" int sfx_model_set_subgrid(sfx_model* M, const sfx_subgrid_input* s) {
+  if (!M || !s || !M->mesh_built || M->initialized) { set_error("sfx_model_set_subgrid: bad state"); return SFX_ERR_STATE; }
+  if (s->nlevels < 2) { set_error("subgrid tables need at least 2 levels"); return SFX_ERR_ARG; }
+  const int32_t np = M->np, npuv = M->npuv, nl = s->nlevels;
+  const float g = M->p.g;
+  M->subgrid = true;
+  M->p.huthresh = 0.0f;
+  M->nl = nl;
+  M->z_zmin.assign(s->z_zmin, s->z_zmin + np);
+  M->z_zmax.assign(s->z_zmax, s->z_zmax + np);
+  M->z_volmax.assign(s->z_volmax, s->z_volmax + np);
+  M->z_dep.assign(s->z_level, s->z_level + (size_t)np * nl);
+  M->uv_zmin.assign(s->uv_zmin, s->uv_zmin + npuv);
+  M->uv_zmax.resize(npuv);
+  for (int32_t i = 0; i < npuv; ++i) M->uv_zmax[i] = std::max(s->uv_zmax[i], M->uv_zmin[i] + 0.01f);
+  M->uv_fnfit.assign(s->uv_ffit, s->uv_ffit + npuv);
+  M->uv_navg_w.resize(npuv);
+  for (int32_t i = 0; i < npuv; ++i) { float v = std::max(s->uv_navg[i], 0.0001f); M->uv_navg_w[i] = g * (v * v); }
+  M->uv_havg.assign(s->uv_havg, s->uv_havg + (size_t)npuv * nl);
+  M->uv_nrep.resize((size_t)npuv * nl);
+  for (size_t k = 0; k < (size_t)npuv * nl; ++k) { float v = std::max(s->uv_nrep[k], 0.0001f); M->uv_nrep[k] = g * (v * v); }
+  M->uv_pwet.assign(s->uv_pwet, s->uv_pwet + (size_t)npuv * nl);
+  for (int32_t i = 0; i < npuv; ++i) {
+    const int32_t nm = M->uv_index_z_nm[i] - 1, nmu = M->uv_index_z_nmu[i] - 1;
+    if (M->z_zmin[nm] > M->uv_zmin[i]) {
+      M->uv_zmin[i] = std::max(M->uv_zmin[i], M->z_zmin[nm]);
+      M->uv_zmax[i] = std::max(M->uv_zmax[i], M->uv_zmin[i] + 0.01f);
+    }
+    if (M->z_zmin[nmu] > M->uv_zmin[i]) {
+      M->uv_zmin[i] = std::max(M->uv_zmin[i], M->z_zmin[nmu]);
+      M->uv_zmax[i] = std::max(M->uv_zmax[i], M->uv_zmin[i] + 0.01f);
+    }
+  }
+  for (int32_t c = 0; c < np; ++c)
+    if (M->z_zmax[c] - M->z_zmin[c] <= 0.0f) M->z_zmax[c] = M->z_zmax[c] + 0.01f;
+  for (int32_t i = 0; i < npuv; ++i)
+    if (M->uv_zmax[i] - M->uv_zmin[i] < 1.0e-7f) M->uv_zmax[i] = M->uv_zmax[i] + 0.01f;
+  M->uv_havg_zmax.resize(npuv); M->uv_nrep_zmax.resize(npuv);
+  for (int32_t i = 0; i < npuv; ++i) {
+    M->uv_havg_zmax[i] = M->uv_havg[(size_t)i * nl + (nl - 1)] - M->uv_zmax[i];
+    M->uv_nrep_zmax[i] = M->uv_nrep[(size_t)i * nl + (nl - 1)];
+  }
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_set_boundary(sfx_model* M, int32_t nbnd, const float* x, const float* y,
+                                      int32_t nt, const float* t, const float* zs) {
+  if (!M || M->initialized || nbnd < 0 || (nbnd > 0 && (!x || !y))) { set_error("sfx_model_set_boundary: bad args"); return SFX_ERR_ARG; }
+  M->nbnd = nbnd;
+  M->x_bnd.assign(x, x + nbnd); M->y_bnd.assign(y, y + nbnd);
+  if (nt > 0 && t && zs) {
+    M->ntbnd = nt;
+    M->t_bnd.assign(t, t + nt);
+    M->zs_bnd.assign(zs, zs + (size_t)nt * nbnd);
+  } else {
+    // bnd without bzs: zero levels over [t0, t1] (sfincs_boundaries.f90:94-112)
+    M->ntbnd = 2;
+    M->t_bnd = {M->p.t0, M->p.t1};
+    M->zs_bnd.assign((size_t)2 * nbnd, 0.0f);
+  }
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_set_sources(sfx_model* M, int32_t nsrc, const float* x, const float* y,
+                                     int32_t nt, const float* t, const float* q) {
+  if (!M || M->initialized || nsrc < 0 || (nsrc > 0 && (!x || !y || !t || !q || nt < 2))) {
+    set_error("sfx_model_set_sources: bad args (need >= 2 times)");
+    return SFX_ERR_ARG;
+  }
+  M->nsrc = nsrc; M->ntsrc = nt;
+  M->x_src.assign(x, x + nsrc); M->y_src.assign(y, y + nsrc);
+  M->tsrc.assign(t, t + nt);
+  M->qsrc.assign(q, q + (size_t)nt * nsrc);
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_set_rain(sfx_model* M, int32_t nt, const float* t, const float* mmh) {
+  if (!M || M->initialized || nt < 2 || !t || !mmh) { set_error("sfx_model_set_rain: need >= 2 times"); return SFX_ERR_ARG; }
+  M->ntprcp = nt; M->tprcpt.assign(t, t + nt); M->tprcpv.assign(mmh, mmh + nt);
+  M->precip = true;
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_set_wind(sfx_model* M, int32_t nt, const float* t, const float* mag, const float* dir) {
+  if (!M || M->initialized || nt < 2 || !t || !mag || !dir) { set_error("sfx_model_set_wind: need >= 2 times"); return SFX_ERR_ARG; }
+  M->ntwnd = nt; M->twnd.assign(t, t + nt); M->wndmag.assign(mag, mag + nt);
+  M->wnddir.resize(nt);
+  // Cartesian direction the wind blows to, with grid rotation (sfincs_meteo.f90:275)
+  for (int32_t i = 0; i < nt; ++i) M->wnddir[i] = M->p.pi * (270.0f - dir[i]) / 180 - M->rotation;
+  M->wind = true;
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_initialize(sfx_model* M) {
+  if (!M || !M->mesh_built || M->initialized) { set_error("sfx_model_initialize: bad state"); return SFX_ERR_STATE; }
+  if (!M->subgrid && M->zb.empty()) { set_error("bathymetry or subgrid tables must be set first"); return SFX_ERR_STATE; }
+  const int32_t np = M->np, npuv = M->npuv, ncuv = M->ncuv;
+  const sfx_params& p = M->p;
+
+  // initialize_boundaries (sfincs_domain.f90:1786-1956)
+  M->ngbnd = 0; M->nmindbnd.clear();
+  bool has2 = false;
+  for (int32_t c = 0; c < np; ++c) {
+    const int k = M->kcs[c];
+    if (k == 5) { set_error("downstream river boundaries (msk=5) are not supported"); return SFX_ERR_ARG; }
+    if (k == 2 || k == 3 || k == 6) { M->nmindbnd.push_back(c + 1); if (k == 2) has2 = true; }
+    if (k == 4) { set_error("wave maker cells (msk=4) are not supported"); return SFX_ERR_ARG; }
+  }
+  M->ngbnd = (int32_t)M->nmindbnd.size();
+  M->zsb.assign(M->ngbnd, 0.f); M->zsb0.assign(M->ngbnd, 0.f);
+  M->nkcuv2 = 0;
+  for (int32_t i = 0; i < npuv; ++i) {
+    const int a = M->kcs[M->uv_index_z_nm[i] - 1], b = M->kcs[M->uv_index_z_nmu[i] - 1];
+    int8_t kv;
+    if (a > 1 && b > 1) kv = 0;
+    else if (a * b == 0) kv = 0;
+    else if ((a == 1 && b == 2) || (a == 2 && b == 1)) { kv = 2; ++M->nkcuv2; }
+    else if ((a == 1 && b == 3) || (a == 3 && b == 1)) { kv = 2; ++M->nkcuv2; }
+    else if ((a == 1 && b == 5) || (a == 5 && b == 1)) { kv = 2; ++M->nkcuv2; }
+    else if ((a == 1 && b == 6) || (a == 6 && b == 1)) { kv = 6; ++M->nkcuv2; }
+    else kv = 1;
+    M->kcuv[i] = kv;
+  }
+  M->index_kcuv2.clear(); M->nmbkcuv2.clear(); M->nmikcuv2.clear(); M->ibuvdir.clear(); M->ibkcuv2.clear();
+  for (int32_t i = 0; i < npuv; ++i) {
+    if (M->kcuv[i] != 2 && M->kcuv[i] != 6) continue;
+    const int32_t nm = M->uv_index_z_nm[i], nmu = M->uv_index_z_nmu[i];
+    M->index_kcuv2.push_back(i + 1);
+    int32_t nmb, nmi; int8_t dir;
+    if (M->kcs[nm - 1] == 1) { nmb = nmu; nmi = nm; dir = -1; } else { nmb = nm; nmi = nmu; dir = 1; }
+    M->nmbkcuv2.push_back(nmb); M->nmikcuv2.push_back(nmi); M->ibuvdir.push_back(dir);
+    int32_t ibk = 0;
+    for (int32_t ib = 0; ib < M->ngbnd; ++ib) {
+      if (M->nmindbnd[ib] == nmb) {
+        ibk = ib + 1;
+        if (M->kcs[nmb - 1] == 3) {
+          M->zsb0[ib] = M->subgrid ? M->z_zmin[nmb - 1] : M->zb[nmb - 1];
+          M->zsb[ib] = M->zsb0[ib];
+        }
+        break;
+      }
+    }
+    M->ibkcuv2.push_back(ibk);
+  }
+  M->uvmean.assign(M->nkcuv2, 0.f);
+
+  // initialize_roughness (:1978-2047): uniform manning (land/sea split by rgh_lev_land)
+  if (!M->subgrid) {
+    M->gn2uv.assign(npuv, 9.81f * 0.02f * 0.02f);
+    const float gn2land = (p.manning_land < 0.0f) ? 9.81f * (p.manning * p.manning) : 9.81f * (p.manning_land * p.manning_land);
+    const float gn2sea = (p.manning_sea < 0.0f) ? 9.81f * (p.manning * p.manning) : 9.81f * (p.manning_sea * p.manning_sea);
+    for (int32_t i = 0; i < npuv; ++i)
+      if (M->kcuv[i] > 0) M->gn2uv[i] = (M->zbuv[i] > p.rgh_lev_land) ? gn2land : gn2sea;
+  }
+
+  // initialize_hydro + set_initial_conditions (:2195-2398; sfincs_initial_conditions.F90:98-174)
+  M->zs.resize(np);
+  const double zini = (double)p.zsini;
+  for (int32_t c = 0; c < np; ++c)
+    M->zs[c] = M->subgrid ? std::max((double)M->z_zmin[c], zini) : std::max((double)M->zb[c], zini);
+  M->q.assign(npuv + ncuv + 1, 0.f);
+  M->uv.assign(npuv + ncuv + 1, 0.f);
+  M->kfuv.assign(npuv, 0);
+  if (p.wiggle_suppression) {
+    M->zs0.resize(np); M->zsderv.assign(np, 0.f);
+    for (int32_t c = 0; c < np; ++c) M->zs0[c] = (float)M->zs[c];
+  } else {
+    M->zs0.assign(np, 0.f); M->zsderv.assign(np, 0.f);
+  }
+  M->zsmax.assign(np, -999.0f);
+  if (M->subgrid) {
+    // compute_initial_subgrid_volumes (sfincs_subgrid.F90:907-962)
+    const int nl = M->nl;
+    M->z_volume.resize(np);
+    for (int32_t c = 0; c < np; ++c) {
+      const double zsc = M->zs[c];
+      const float area = M->cell_area[M->z_flags_iref[c] - 1];
+      if (zsc >= (double)M->z_zmax[c]) {
+        M->z_volume[c] = (double)M->z_volmax[c] + (double)area * (zsc - (double)std::max(M->z_zmax[c], -20.0f));
+      } else {
+        int ivol = 1;
+        for (int il = 2; il <= nl; ++il)
+          if ((double)M->z_dep[(size_t)c * nl + il - 1] > zsc) { ivol = il - 1; break; }
+        const float dzvol = M->z_volmax[c] / (float)(nl - 1);
+        const float d0 = M->z_dep[(size_t)c * nl + ivol - 1], d1 = M->z_dep[(size_t)c * nl + ivol];
+        const float facint = (float)((zsc - (double)d0) / (double)std::max(d1 - d0, 0.001f));
+        M->z_volume[c] = (double)((float)(ivol - 1) * dzvol + facint * dzvol);
+      }
+    }
+  } else {
+    M->z_volume.clear();
+  }
+
+  // find_boundary_indices (sfincs_boundaries.f90:473-679)
+  M->ind1_bnd_gbp.assign(M->ngbnd, 0); M->ind2_bnd_gbp.assign(M->ngbnd, 0);
+  M->fac_bnd_gbp.assign(M->ngbnd, 0.f); M->nmi_gbp.assign(M->ngbnd, 0);
+  if (has2 && M->nbnd == 0) { set_error("water level boundary cells (msk=2) need boundary points"); return SFX_ERR_ARG; }
+  for (int32_t ib = 0; ib < M->ngbnd; ++ib) {
+    const int32_t nm = M->nmindbnd[ib];
+    const float x = M->z_xz[nm - 1], y = M->z_yz[nm - 1];
+    const int k = M->kcs[nm - 1];
+    if (k == 2) {
+      if (M->nbnd > 1) {
+        float dst1 = 1.0e10f, dst2 = 1.0e10f; int32_t ib1 = 0, ib2 = 0;
+        for (int32_t j = 0; j < M->nbnd; ++j) {
+          const float ddx = M->x_bnd[j] - x, ddy = M->y_bnd[j] - y;
+          const float dst = std::sqrt(ddx * ddx + ddy * ddy);
+          if (dst < dst1) { dst2 = dst1; ib2 = ib1; dst1 = dst; ib1 = j + 1; }
+          else if (dst < dst2) { dst2 = dst; ib2 = j + 1; }
+        }
+        M->ind1_bnd_gbp[ib] = ib1; M->ind2_bnd_gbp[ib] = ib2;
+        M->fac_bnd_gbp[ib] = dst2 / std::max(dst1 + dst2, 1.0e-9f);
+      } else {
+        M->ind1_bnd_gbp[ib] = 1; M->ind2_bnd_gbp[ib] = 1; M->fac_bnd_gbp[ib] = 1.0f;
+      }
+    } else if (k == 6) {
+      const int32_t n = M->z_index_z_n[nm - 1], m = M->z_index_z_m[nm - 1];
+      const int iref = M->z_flags_iref[nm - 1];
+      const int32_t cand[4][2] = {{n, m + 1}, {n + 1, m}, {n, m - 1}, {n - 1, m}};
+      for (int j = 0; j < 4; ++j) {
+        const int32_t nmi = M->find_cell_by_index(cand[j][0], cand[j][1], iref);
+        if (nmi > 0 && M->kcs[nmi - 1] == 1) M->nmi_gbp[ib] = nmi;
+      }
+      if (M->nmi_gbp[ib] == 0) M->kcs[nm - 1] = 0;
+    }
+  }
+
+  // read_discharges cell lookup (sfincs_discharges.f90:159-171)
+  M->nmindsrc.assign(M->nsrc, 0);
+  for (int32_t i = 0; i < M->nsrc; ++i) {
+    M->nmindsrc[i] = M->find_cell_xy(M->x_src[i], M->y_src[i]);
+    if (M->nmindsrc[i] == 0) { set_error("point source outside the active mesh"); return SFX_ERR_ARG; }
+  }
+  // discharge times must cover [t0, t1] (sfincs_discharges.f90:134-155)
+  if (M->nsrc > 0) {
+    if (M->tsrc[0] > M->p.t0 + 1.0f || M->tsrc[M->ntsrc - 1] < M->p.t1 - 1.0f) {
+      if (M->tsrc[0] > M->p.t0 + 1.0f) M->tsrc[0] = M->p.t0 - 1.0f;
+      else M->tsrc[M->ntsrc - 1] = M->p.t1 + 1.0f;
+    }
+  }
+  if (M->precip && M->tprcpt[0] > M->p.t0) { set_error("rain series must start at or before t0"); return SFX_ERR_ARG; }
+  if (M->wind && M->twnd[0] > M->p.t0) { set_error("wind series must start at or before t0"); return SFX_ERR_ARG; }
+
+  // Cd table (sfincs_meteo.f90:316-328), default cd_nr = 3 (sfincs_input.f90:314-328)
+  {
+    const float cd_wnd[3] = {0.0f, 28.0f, 50.0f}, cd_val[3] = {0.0010f, 0.0025f, 0.0015f};
+    const int cd_nr = 3;
+    M->cdval.assign(1000, 0.f);
+    for (int iw = 1; iw <= 1000; ++iw) {
+      const float wnd = (float)iw * 0.1f;
+      for (int icd = 1; icd <= cd_nr - 1; ++icd) {
+        M->cdval[iw - 1] = cd_val[icd - 1];
+        if (wnd >= cd_wnd[icd - 1] && wnd < cd_wnd[icd]) {
+          M->cdval[iw - 1] = cd_val[icd - 1] + (wnd - cd_wnd[icd - 1]) * (cd_val[icd] - cd_val[icd - 1]) / (cd_wnd[icd] - cd_wnd[icd - 1]);
+          break;
+        } else if (wnd >= cd_wnd[cd_nr - 1]) {
+          M->cdval[iw - 1] = cd_val[cd_nr - 1];
+          break;
+        }
+      }
+    }
+  }
+
+  // set_advection_mask (sfincs_domain.f90:2114-2180)
+  for (int32_t j = 0; j < ncuv; ++j) {
+    const int a = M->kcuv[M->cuv_index_uv1[j] - 1], b = M->kcuv[M->cuv_index_uv2[j] - 1];
+    int8_t kv;
+    if (a == 0 || b == 0) kv = 0;
+    else if (a == 2 || b == 2) kv = 2;
+    else if (a == 4 || b == 4) kv = 4;
+    else kv = 1;
+    M->kcuv[M->cuv_index_uv[j] - 1] = kv;
+  }
+  M->mask_adv.assign(npuv, 1);
+  if (p.advection && p.advection_mask && (M->use_quadtree || M->nmax > 1) && need_uv_neighbours(M)) {
+    for (int32_t i = 0; i < npuv; ++i) {
+      const int32_t nb[8] = {M->uv_index_u_nmu[i], M->uv_index_u_nmd[i], M->uv_index_v_nm[i], M->uv_index_v_nmu[i],
+                             M->uv_index_v_ndm[i], M->uv_index_v_ndmu[i], M->uv_index_u_ndm[i], M->uv_index_u_num[i]};
+      for (int j = 0; j < 8; ++j)
+        if (M->kcuv[nb[j] - 1] != 1) { M->mask_adv[i] = 0; break; }
+    }
+  }
+  M->initialized = true;
+  return SFX_OK;
+}
+
+// ---- table / state export ----------------------------------------------------------
+namespace {
+struct View { const void* ptr; size_t bytes; };
+
+template <typename T> View V(const std::vector<T>& v) { return View{v.data(), v.size() * sizeof(T)}; }
+
+bool lookup(const sfx_model* M, const std::string& k, View& out) {
+  static const char* dummy = "";
+  (void)dummy;
+#define T(name) if (k == #name) { out = V(M->name); return true; }
+  T(kcs) T(z_flags_iref) T(z_index_z_n) T(z_index_z_m)
+  T(z_index_uv_md) T(z_index_uv_mu) T(z_index_uv_nd) T(z_index_uv_nu)
+  T(z_index_uv_md1) T(z_index_uv_md2) T(z_index_uv_mu1) T(z_index_uv_mu2)
+  T(z_index_uv_nd1) T(z_index_uv_nd2) T(z_index_uv_nu1) T(z_index_uv_nu2)
+  T(z_xz) T(z_yz) T(zb)
+  T(uv_index_z_nm) T(uv_index_z_nmu) T(uv_index_u_nmd) T(uv_index_u_nmu) T(uv_index_u_ndm) T(uv_index_u_num)
+  T(uv_index_v_ndm) T(uv_index_v_nm) T(uv_index_v_ndmu) T(uv_index_v_nmu)
+  T(uv_flags_iref) T(uv_flags_type) T(uv_flags_dir) T(kcuv) T(mask_adv)
+  T(cuv_index_uv) T(cuv_index_uv1) T(cuv_index_uv2) T(zbuv) T(zbuvmx) T(gn2uv)
+  T(dxr) T(dyr) T(dxrm) T(dyrm) T(dxrinv) T(dyrinv) T(dxr2inv) T(dyr2inv) T(dxrinvc) T(dyrinvc) T(cell_area) T(nuvisc)
+  T(z_zmin) T(z_zmax) T(z_volmax) T(z_dep) T(uv_zmin) T(uv_zmax) T(uv_navg_w) T(uv_fnfit)
+  T(uv_havg) T(uv_nrep) T(uv_pwet) T(uv_havg_zmax) T(uv_nrep_zmax)
+  T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv)
+  T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
+  T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval)
+  T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)
+#undef T
+  return false;
+}
+}  // namespace
+
+extern "C" int sfx_model_table_bytes(const sfx_model* M, const char* name, size_t* nbytes) {
+  View v{};
+  if (!M || !name || !nbytes || !lookup(M, name, v)) { set_error(std::string("unknown table ") + (name ? name : "")); return SFX_ERR_ARG; }
+  *nbytes = v.bytes;
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_get_table(const sfx_model* M, const char* name, void* dst, size_t nbytes) {
+  View v{};
+  if (!M || !name || !dst || !lookup(M, name, v)) { set_error(std::string("unknown table ") + (name ? name : "")); return SFX_ERR_ARG; }
+  if (nbytes != v.bytes) { set_error(std::string("size mismatch for ") + name); return SFX_ERR_ARG; }
+  if (nbytes) std::memcpy(dst, v.ptr, nbytes);
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_get_state(const sfx_model* M, const char* name, void* dst, size_t nbytes) {
+  return sfx_model_get_table(M, name, dst, nbytes);
+}

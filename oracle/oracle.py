@@ -74,13 +74,18 @@ def olib():
 
 
 def x73_table():
-    """h**(7/3) table of fill_h73_tables (sfincs_data.f90:858-881), restated in float32."""
+    """h**(7/3) table of fill_h73_tables (sfincs_data.f90:858-881), restated in float32
+    with the C library's powf (the function the reference's hh**(1.0/3.0) calls)."""
+    libm = C.CDLL("libm.so.6")
+    libm.powf.restype = C.c_float
+    libm.powf.argtypes = [C.c_float, C.c_float]
+    third = float(np.float32(1.0) / np.float32(3.0))
     x = np.empty((10, 1000), np.float32)
     for k in range(1, 11):
-        p = np.float32(10.0) ** np.float32(k - 6) if k >= 6 else np.float32(1.0) / (np.float32(10.0) ** np.float32(6 - k))
+        p = np.float32(10.0) ** np.float32(k - 6) if k >= 6 else np.float32(1.0) / np.float32(10.0 ** (6 - k))
         for j in range(1, 1001):
             hh = np.float32(np.float32(0.001) * np.float32(j)) * np.float32(p)
-            x[k - 1, j - 1] = np.float32(hh * hh) * np.float32(np.power(hh, np.float32(1.0) / np.float32(3.0)))
+            x[k - 1, j - 1] = np.float32(hh * hh) * np.float32(libm.powf(float(hh), third))
     return x.reshape(-1)
 
 

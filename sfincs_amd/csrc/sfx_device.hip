@@ -1,0 +1,1562 @@
+// Device runtime of the MI355X-native explicit step: HBM-resident scenario state, the
+// HIP kernels of one step (forcing, momentum + CFL reduction, combined-uv averaging,
+// continuity), graph-captured step loops, batched multi-scenario launches and the
+// C ABI entry points of include/sfx.h that need a GPU.
+//
+// Layouts in HBM (structure of arrays, one allocation per field):
+//  * cells: reference order (level-major, then column-major n fastest); zs and z_volume
+//    in double, all other cell fields in R (float, or double for the FP64 twin).
+//  * uv-points, "dense" scenarios (full rectangles): two planes of nmax*mmax slots,
+//    slot c = u-point right of cell c, slot np+c = v-point above cell c, plus one dummy
+//    slot (always zero) -- neighbours are implicit (+-1, +-nmax), no index tables.
+//  * uv-points, "indexed" scenarios (quadtree, masked grids): reference order with the
+//    ncuv combined points and the dummy, neighbour tables converted to 0-based slots.
+//  * q and uv are double-buffered: a momentum sweep reads buffer `cur` and writes
+//    `cur^1`, which replaces the reference's q0/uv0 copy (sfincs_momentum.f90:125-138).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sfx_internal.hpp"
+#include "sfx_physics.hpp"
+
+using sfx::Consts;
+using sfx::set_error;
+
+namespace {
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                           \
+      return SFX_ERR_HIP;                                                                  \
+    }                                                                                      \
+  } while (0)
+
+constexpr int BLOCK = 256;
+constexpr int WAVE = 64;
+
+// Device-resident step scalars of one scenario (t, dt, CFL reduction, forcing scalars).
+struct Scalars {
+  double t, tend;
+  float dt, min_dt, dtchk;
+  uint32_t min_dt_bits;          // atomicMin target of the momentum sweep (positive float bits)
+  int32_t nt, active, error, error_pending, cur, mom_done;
+  int32_t itbndlast, itsrclast, itwndlast, itprcplast;
+  float tauwu, tauwv, netprcp;   // uniform meteo forcing of this step
+  float factime, facrel, one_minus_factime;
+  int32_t pad0;
+  unsigned long long cnt[4];     // uv-points: copied (kcuv not 1/6), dry, wet/full, partial
+};
+
+struct DevInst {
+  Consts c;
+  int32_t dense, subgrid, fp64;
+  int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
+  int32_t tn, tm, ntn;           // dense tile shape and tiles along n
+  int32_t count;                 // per-branch counting on
+  Scalars* sc;
+  // cells
+  double* zs;
+  double* vol;
+  void* zs0;
+  void* zsderv;
+  void* zsmax;
+  const uint8_t* kcs;            // kcs | 0x80 when the cell holds a point source
+  const float* zb;
+  const int8_t* z_iref;
+  const int32_t* zmd; const int32_t* zmu; const int32_t* znd; const int32_t* znu;
+  const float* z_zmin; const float* z_zmax; const float* z_volmax; const float* z_dep;
+  // uv slots
+  void* q[2];
+  void* uv[2];
+  int8_t* kfuv;
+  const int8_t* kcuv;
+  const int8_t* mask_adv;
+  const float* zbuvmx;
+  const float* gn2uv;
+  const uint8_t* uvf;            // indexed: iref | (type+1)<<4 | dir<<6
+  const int32_t* nb;             // indexed: 10 tables of npuv (nm,nmu,u_nmd,u_nmu,u_ndm,u_num,v_ndm,v_nm,v_ndmu,v_nmu)
+  const int32_t* cuv;            // indexed: 3 tables of ncuv (cuv, uv1, uv2)
+  const float* uv_zmin;
+  const float4* uv_s4;           // {zmax, havg_zmax, nrep_zmax, navg_w}
+  const float* uv_fnfit;
+  const float4* uv_lev;          // nslot*nl {havg, nrep, pwet, 0}
+  // boundaries
+  int32_t ngbnd, nkcuv2, nbnd, ntbnd;
+  const int32_t* gb_cell; const int32_t* gb_ind1; const int32_t* gb_ind2; const float* gb_fac; const int32_t* gb_nmi;
+  float* zsb; float* zsb0;
+  const int32_t* k2_slot; const int32_t* k2_nmb; const int32_t* k2_nmi; const int32_t* k2_ib; const int8_t* k2_dir;
+  const float* k2_zbuv;
+  void* uvmean;
+  const float* t_bnd; const float* zs_bnd; float* zst_bnd;
+  // sources and meteo
+  int32_t nsrc, ntsrc, ntprcp, ntwnd;
+  const int32_t* src_cell; const float* tsrc; const float* qsrc; float* qtsrc;
+  const float* tprcpt; const float* tprcpv; const float* twnd; const float* wndmag; const float* wnddir;
+  const float* cdval; const float* x73;
+};
+
+struct Item { int32_t inst, tile; };
+
+__device__ __forceinline__ float wave_min(float v) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+__device__ __forceinline__ void block_min_to(float v, uint32_t* target) {
+  __shared__ float red[BLOCK / WAVE];
+  v = wave_min(v);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int i = 1; i < BLOCK / WAVE; ++i) m = fminf(m, red[i]);
+    if (m == m && m > 0.f) atomicMin(target, __float_as_uint(m));
+  }
+}
+
+// ----------------------------------------------------------------------------- forcing
+// One block per scenario: step scalars, meteo series, boundary points/conditions/fluxes,
+// discharge series (sfincs_lib.f90:379-550 with sfincs_meteo.f90:1423-1558,
+// sfincs_boundaries.f90:682-1122, sfincs_discharges.f90:353-369).
+__device__ float f_modulo(float a, float p) {
+  float r = fmodf(a, p);
+  if (r != 0.0f && ((r < 0.0f) != (p < 0.0f))) r += p;
+  return r;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids,
+                                                   int mode) {
+  const DevInst& I = insts[ids[blockIdx.x]];
+  Scalars* S = I.sc;
+  const Consts& c = I.c;
+  __shared__ int sh_go;
+  __shared__ int sh_itb0, sh_itb1;
+  __shared__ float sh_tbfac;
+  if (threadIdx.x == 0) {
+    if (S->mom_done) { S->cur ^= 1; S->mom_done = 0; S->min_dt = __uint_as_float(S->min_dt_bits); }
+    int go = 1;
+    if (mode == 0) {
+      if (S->error_pending) { S->error = 1; S->error_pending = 0; S->t = (double)c.t1 + 1.0; }
+      go = (S->t < S->tend) && !S->error;
+      S->active = go;
+      if (go) {
+        S->nt += 1;
+        S->dt = c.alpha * S->min_dt;
+        S->dtchk = c.alpha * S->min_dt;
+        S->t = S->t + (double)S->dt;
+        if (S->dtchk < c.dtmin && S->nt > 1) S->error_pending = 1;
+      }
+    } else {
+      S->active = 1;
+    }
+    if (go) {
+      const double t = S->t;
+      const float dt = S->dt;
+      if (c.flags & sfx::F_WIND) {
+        for (int itw = max(S->itwndlast, 2); itw <= I.ntwnd; ++itw) {
+          if ((double)I.twnd[itw - 1] > t) {
+            const float twfac = (float)((t - (double)I.twnd[itw - 2]) / (double)(I.twnd[itw - 1] - I.twnd[itw - 2]));
+            const float vmag = I.wndmag[itw - 2] * (1.0f - twfac) + I.wndmag[itw - 1] * twfac;
+            float dr0 = f_modulo(I.wnddir[itw - 2], 2 * c.pi);
+            float dr1 = f_modulo(I.wnddir[itw - 1], 2 * c.pi);
+            if (dr1 > dr0 + c.pi) dr0 = dr0 + 2 * c.pi;
+            else if (dr0 > dr1 + c.pi) dr1 = dr1 + 2 * c.pi;
+            const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
+            const float cd = I.cdval[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
+            S->tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
+            S->tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
+            S->itwndlast = itw - 1;
+            break;
+          }
+        }
+      }
+      if (c.flags & sfx::F_PRECIP) {
+        int itp;
+        for (itp = S->itprcplast; itp <= I.ntprcp; ++itp)
+          if ((double)I.tprcpt[itp - 1] > t) break;
+        if (itp > I.ntprcp) itp = I.ntprcp;
+        if (itp < 2) itp = 2;   // series start at or before t0 (checked at initialisation)
+        const float twfac = (float)((t - (double)I.tprcpt[itp - 2]) / (double)(I.tprcpt[itp - 1] - I.tprcpt[itp - 2]));
+        S->netprcp = (I.tprcpv[itp - 2] * (1.0f - twfac) + I.tprcpv[itp - 1] * twfac) / (float)(1000 * 3600);
+        S->itprcplast = itp - 1;
+      }
+      if (I.nbnd > 0) {
+        int itb0 = 1, itb1 = 1;
+        float tb = 0.f;
+        const int nt = I.ntbnd;
+        if ((double)I.t_bnd[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = I.t_bnd[0]; }
+        else if ((double)I.t_bnd[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = I.t_bnd[nt - 1]; }
+        else {
+          for (int itb = S->itbndlast; itb <= nt; ++itb)
+            if ((double)I.t_bnd[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; S->itbndlast = itb - 1; break; }
+        }
+        sh_itb0 = itb0; sh_itb1 = itb1;
+        sh_tbfac = (tb - I.t_bnd[itb0 - 1]) / sfx::vmax<float>(I.t_bnd[itb1 - 1] - I.t_bnd[itb0 - 1], 1.0e-6f);
+      }
+      if (I.nsrc > 0) {
+        for (int it = max(S->itsrclast, 2); it <= I.ntsrc; ++it) {
+          if ((double)I.tsrc[it - 1] > t) {
+            for (int s = 0; s < I.nsrc; ++s) {
+              const float q0v = I.qsrc[(it - 2) * I.nsrc + s], q1v = I.qsrc[(it - 1) * I.nsrc + s];
+              I.qtsrc[s] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)I.tsrc[it - 2]) /
+                                                     (double)(I.tsrc[it - 1] - I.tsrc[it - 2]));
+            }
+            S->itsrclast = it - 1;
+            break;
+          }
+        }
+      }
+      S->min_dt_bits = __float_as_uint(c.dtmax);
+      S->factime = sfx::vmin<float>(dt / c.btfilter, 1.0f);
+      S->one_minus_factime = 1.0f - S->factime;
+      S->facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
+    }
+    sh_go = go;
+  }
+  __syncthreads();
+  if (!sh_go) return;
+  // water level at boundary points (update_boundary_points, :728-756)
+  for (int ib = threadIdx.x; ib < I.nbnd; ib += blockDim.x) {
+    const float a = I.zs_bnd[(sh_itb0 - 1) * I.nbnd + ib], b = I.zs_bnd[(sh_itb1 - 1) * I.nbnd + ib];
+    I.zst_bnd[ib] = a + (b - a) * sh_tbfac;
+  }
+  __syncthreads();
+  const double t = S->t;
+  // update_boundary_conditions (:789-917)
+  for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
+    const int nmb = I.gb_cell[ib];
+    const int k = I.kcs[nmb] & 0x7f;
+    if (k == 2) {
+      double zst;
+      if (I.nbnd > 1) {
+        const float f = I.gb_fac[ib];
+        zst = (double)(I.zst_bnd[I.gb_ind1[ib] - 1] * f + I.zst_bnd[I.gb_ind2[ib] - 1] * (1.0f - f));
+      } else {
+        zst = (double)I.zst_bnd[0];
+      }
+      float zb0, zb1;
+      if (t < (double)(c.tspinup - 1.0e-3f)) {
+        const float smfac = (float)(1.0 - (t - (double)c.t0) / (double)(c.tspinup - c.t0));
+        const float zs0act = (float)(zst + 0.0);
+        zb0 = c.zini * smfac + zs0act * (1.0f - smfac);
+        zb1 = zb0;
+      } else {
+        zb0 = (float)zst;
+        zb1 = (float)zst;
+      }
+      const float floorz = I.subgrid ? I.z_zmin[nmb] : I.zb[nmb];
+      I.zsb0[ib] = sfx::vmax<float>(zb0, floorz);
+      I.zsb[ib] = sfx::vmax<float>(zb1, floorz);
+    } else if (k == 6) {
+      I.zs[nmb] = I.zs[I.gb_nmi[ib]];
+    }
+  }
+  __syncthreads();
+  // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
+  R* q = (R*)I.q[S->cur];
+  R* uv = (R*)I.uv[S->cur];
+  R* uvmean = (R*)I.uvmean;
+  R* zsmax = (R*)I.zsmax;
+  const int nl = c.nl;
+  for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
+    const int nmb = I.k2_nmb[ib];
+    if ((I.kcs[nmb] & 0x7f) == 6) continue;
+    const int indb = I.k2_ib[ib];
+    const int nmi = I.k2_nmi[ib];
+    const int ip = I.k2_slot[ib];
+    const R zsnmi = (R)I.zs[nmi];
+    R zsnmb = I.zsb[indb];
+    R zs0nmb = I.zsb0[indb];
+    R hnmb;
+    const R huth = c.huthresh;
+    if (I.subgrid) {
+      const R zsuv = sfx::vmax<R>(zsnmb, zsnmi);
+      const float4 s4 = I.uv_s4[ip];
+      const R zmaxu = s4.x, zminu = I.uv_zmin[ip];
+      R depthuv;
+      if (zsuv >= zmaxu - R(1.0e-4f)) {
+        depthuv = (R)s4.y + zsuv;
+      } else if (zsuv > zminu) {
+        const R dzuv = (zmaxu - zminu) / (R)(nl - 1);
+        const int iuv = (int)((zsuv - zminu) / dzuv) + 1;
+        const R facint = (zsuv - (zminu + (R)(iuv - 1) * dzuv)) / dzuv;
+        const float4* lv = I.uv_lev + (size_t)ip * nl;
+        depthuv = (R)lv[iuv - 1].x + ((R)lv[iuv].x - (R)lv[iuv - 1].x) * facint;
+      } else {
+        depthuv = R(0);
+      }
+      hnmb = sfx::vmax<R>(depthuv, huth);
+      zsnmb = sfx::vmax<R>(zsnmb, (R)I.z_zmin[nmb]);
+      zs0nmb = sfx::vmax<R>(zs0nmb, (R)I.z_zmin[nmb]);
+    } else {
+      hnmb = sfx::vmax<R>(R(0.5f) * (zsnmb + zsnmi) - (R)I.k2_zbuv[ib], huth);
+      zsnmb = sfx::vmax<R>(zsnmb, (R)I.zb[nmb]);
+      zs0nmb = sfx::vmax<R>(zs0nmb, (R)I.zb[nmb]);
+    }
+    R qb;
+    if (hnmb < huth + R(1.0e-6f) || I.kcuv[ip] == 3) {
+      qb = R(0);
+      q[ip] = R(0); uv[ip] = R(0); uvmean[ib] = R(0);
+    } else {
+      const R g = c.g;
+      const R ui = sfx::rsqrt_<R>(g / hnmb) * (zsnmb - zs0nmb);
+      const int dir = I.k2_dir[ib];
+      const R ub = (R)dir * (R(2) * ui - sfx::rsqrt_<R>(g / hnmb) * (zsnmi - zs0nmb));
+      qb = ub * hnmb + uvmean[ib];
+      if (I.subgrid) {
+        if (I.vol[nmi] <= 0.0) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
+        if (zsnmb - (R)I.z_zmin[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
+      } else {
+        if (zsnmi - (R)I.zb[nmi] <= huth) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
+        if (zsnmb - (R)I.zb[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
+      }
+      q[ip] = qb;
+      uv[ip] = sfx::vmax<R>(sfx::vmin<R>(qb / hnmb, R(4)), R(-4));
+    }
+    if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)S->factime * q[ip] + (R)S->facrel * (R)S->one_minus_factime * uvmean[ib];
+    else uvmean[ib] = R(0);
+    I.zs[nmb] = (double)I.zsb[indb];
+    if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], I.zs[nmb]);
+  }
+}
+
+// ----------------------------------------------------------------------------- momentum
+template <typename R, bool SUBGRID>
+__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, R dt, R* q1, R* uv1,
+                                          float& mdt, int& br) {
+  sfx::UvTables tb;
+  if (SUBGRID) {
+    tb.zmin = I.uv_zmin[slot];
+  } else {
+    tb.zbuvmx = I.zbuvmx[slot];
+  }
+  R zsu;
+  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
+    q1[slot] = R(0); uv1[slot] = R(0); I.kfuv[slot] = 0;
+    br = 1;
+    return;
+  }
+  if (SUBGRID) {
+    const float4 s4 = I.uv_s4[slot];
+    tb.zmax = s4.x; tb.havg_zmax = s4.y; tb.nrep_zmax = s4.z; tb.navg_w = s4.w; tb.fnfit = I.uv_fnfit[slot];
+  } else {
+    tb.gn2uv = I.gn2uv[slot];
+  }
+  const sfx::UvOut<R> o = sfx::uv_update<R, SUBGRID>(in, tb, I.c, zsu, SUBGRID ? I.uv_lev + (size_t)slot * I.c.nl : nullptr,
+                                                     I.x73, dt);
+  q1[slot] = o.q; uv1[slot] = o.uv; I.kfuv[slot] = 1;
+  mdt = fminf(mdt, (float)o.min_dt);
+  br = o.branch;
+}
+
+// Dense scenarios: one thread per cell handles the u-point on its right and the v-point
+// above it. Neighbours follow the reference's tables for a full rectangle
+// (sfincs_domain.f90:1005-1261): missing left/right/below/above faces read the dummy
+// point (q = uv = 0), missing cross neighbours of u_ndm/u_num fall back to the point itself.
+template <typename R, bool SUBGRID>
+__device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc) {
+  Scalars* S = I.sc;
+  const int cur = S->cur;
+  const R dt = (R)S->dt;
+  const R* __restrict__ q0 = (const R*)I.q[cur];
+  const R* __restrict__ u0 = (const R*)I.uv[cur];
+  R* q1 = (R*)I.q[cur ^ 1];
+  R* u1 = (R*)I.uv[cur ^ 1];
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
+  const int tn = tile % I.ntn, tmi = tile / I.ntn;
+  const int n = tn * I.tn + (threadIdx.x % I.tn);
+  const int m = tmi * I.tm + (threadIdx.x / I.tn);
+  float mdt = I.c.dtmax;
+  int br_u = 0, br_v = 0;
+  if (n < nmax && m < mmax) {
+    const int c = m * nmax + n;
+    const R tauwu = (R)S->tauwu, tauwv = (R)S->tauwv;
+    const double zs_c = I.zs[c];
+    // u-point between c and c+nmax
+    if (m < mmax - 1) {
+      const int s = c;
+      const int kc = I.kcuv[s];
+      if (kc == 1 || kc == 6) {
+        sfx::UvIn<R> in;
+        in.zs_nm = zs_c; in.zs_nmu = I.zs[c + nmax];
+        const int s_nmd = m >= 1 ? s - nmax : dm;
+        const int s_nmu = m + 1 < mmax - 1 ? s + nmax : dm;
+        const int s_ndm = n >= 1 ? s - 1 : s;
+        const int s_num = n < nmax - 1 ? s + 1 : s;
+        const int v_nm = n < nmax - 1 ? np + c : dm;
+        const int v_ndm = n >= 1 ? np + c - 1 : dm;
+        const int v_nmu = n < nmax - 1 ? np + c + nmax : dm;
+        const int v_ndmu = n >= 1 ? np + c + nmax - 1 : dm;
+        in.qx_nm = q0[s]; in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
+        in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
+        in.uu_nm = u0[s]; in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
+        in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
+        in.kcuv_nmd = I.kcuv[s_nmd]; in.kcuv_nmu = I.kcuv[s_nmu];
+        in.kfuv = I.kfuv[s]; in.mask_adv = I.mask_adv[s]; in.idir = 0; in.iref = 1; in.itype = 0;
+        in.tauwu = tauwu; in.tauwv = tauwv;
+        if (SUBGRID) {
+          in.vol_nm = I.vol[c]; in.vol_nmu = I.vol[c + nmax];
+          in.zsderv_nm = ((const R*)I.zsderv)[c]; in.zsderv_nmu = ((const R*)I.zsderv)[c + nmax];
+        } else {
+          in.zb_nm = I.zb[c]; in.zb_nmu = I.zb[c + nmax];
+        }
+        finish_uv<R, SUBGRID>(I, s, in, dt, q1, u1, mdt, br_u);
+      } else {
+        q1[s] = q0[s]; u1[s] = u0[s];
+      }
+    }
+    // v-point between c and c+1
+    if (n < nmax - 1) {
+      const int s = np + c;
+      const int kc = I.kcuv[s];
+      if (kc == 1 || kc == 6) {
+        sfx::UvIn<R> in;
+        in.zs_nm = zs_c; in.zs_nmu = I.zs[c + 1];
+        const int s_nmd = n >= 1 ? s - 1 : dm;
+        const int s_nmu = n + 1 < nmax - 1 ? s + 1 : dm;
+        const int s_ndm = m >= 1 ? s - nmax : s;
+        const int s_num = m < mmax - 1 ? s + nmax : s;
+        const int v_nm = m < mmax - 1 ? c : dm;
+        const int v_ndm = m >= 1 ? c - nmax : dm;
+        const int v_nmu = m < mmax - 1 ? c + 1 : dm;
+        const int v_ndmu = m >= 1 ? c + 1 - nmax : dm;
+        in.qx_nm = q0[s]; in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
+        in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
+        in.uu_nm = u0[s]; in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
+        in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
+        in.kcuv_nmd = I.kcuv[s_nmd]; in.kcuv_nmu = I.kcuv[s_nmu];
+        in.kfuv = I.kfuv[s]; in.mask_adv = I.mask_adv[s]; in.idir = 1; in.iref = 1; in.itype = 0;
+        in.tauwu = tauwu; in.tauwv = tauwv;
+        if (SUBGRID) {
+          in.vol_nm = I.vol[c]; in.vol_nmu = I.vol[c + 1];
+          in.zsderv_nm = ((const R*)I.zsderv)[c]; in.zsderv_nmu = ((const R*)I.zsderv)[c + 1];
+        } else {
+          in.zb_nm = I.zb[c]; in.zb_nmu = I.zb[c + 1];
+        }
+        finish_uv<R, SUBGRID>(I, s, in, dt, q1, u1, mdt, br_v);
+      } else {
+        q1[s] = q0[s]; u1[s] = u0[s];
+      }
+    }
+  }
+  block_min_to(mdt, &S->min_dt_bits);
+  if (I.count) {
+    atomicAdd(&bc[br_u], 1u);
+    atomicAdd(&bc[br_v], 1u);
+  }
+}
+
+// Indexed scenarios: one thread per uv-point, neighbours through the reference's tables.
+template <typename R, bool SUBGRID>
+__device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc) {
+  Scalars* S = I.sc;
+  const int cur = S->cur;
+  const R dt = (R)S->dt;
+  const R* __restrict__ q0 = (const R*)I.q[cur];
+  const R* __restrict__ u0 = (const R*)I.uv[cur];
+  R* q1 = (R*)I.q[cur ^ 1];
+  R* u1 = (R*)I.uv[cur ^ 1];
+  const int ip = tile * BLOCK + threadIdx.x;
+  float mdt = I.c.dtmax;
+  int br = 0;
+  const int npuv = I.npuv;
+  if (ip < npuv) {
+    const int kc = I.kcuv[ip];
+    if (kc == 1 || kc == 6) {
+      const int32_t* nb = I.nb;
+      const int nm = nb[ip], nmu = nb[npuv + ip];
+      sfx::UvIn<R> in;
+      in.zs_nm = I.zs[nm]; in.zs_nmu = I.zs[nmu];
+      const uint8_t f = I.uvf[ip];
+      in.iref = (I.c.flags & sfx::F_QUADTREE) ? (f & 15) : 1;
+      in.itype = (I.c.flags & sfx::F_QUADTREE) ? ((f >> 4) & 3) - 1 : 0;
+      in.idir = (f >> 6) & 1;
+      in.qx_nm = q0[ip]; in.uu_nm = u0[ip];
+      if (I.c.flags & sfx::F_NBRS) {
+        const int a = nb[2 * npuv + ip], b = nb[3 * npuv + ip], cdn = nb[4 * npuv + ip], d = nb[5 * npuv + ip];
+        const int e = nb[6 * npuv + ip], f2 = nb[7 * npuv + ip], g = nb[8 * npuv + ip], h = nb[9 * npuv + ip];
+        in.qx_nmd = q0[a]; in.qx_nmu = q0[b];
+        in.qy_nm = q0[f2]; in.qy_nmu = q0[h]; in.qy_ndm = q0[e]; in.qy_ndmu = q0[g];
+        in.uu_nmd = u0[a]; in.uu_nmu = u0[b]; in.uu_ndm = u0[cdn]; in.uu_num = u0[d];
+        in.v_ndm = u0[e]; in.v_ndmu = u0[g]; in.v_nm = u0[f2]; in.v_nmu = u0[h];
+        in.kcuv_nmd = I.kcuv[a]; in.kcuv_nmu = I.kcuv[b];
+      } else {
+        in.qx_nmd = in.qx_nmu = in.qy_nm = in.qy_nmu = in.qy_ndm = in.qy_ndmu = R(0);
+        in.uu_nmd = in.uu_nmu = in.uu_ndm = in.uu_num = in.v_ndm = in.v_ndmu = in.v_nm = in.v_nmu = R(0);
+        in.kcuv_nmd = in.kcuv_nmu = 0;
+      }
+      in.kfuv = I.kfuv[ip]; in.mask_adv = I.mask_adv[ip];
+      in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+      if (SUBGRID) {
+        in.vol_nm = I.vol[nm]; in.vol_nmu = I.vol[nmu];
+        in.zsderv_nm = ((const R*)I.zsderv)[nm]; in.zsderv_nmu = ((const R*)I.zsderv)[nmu];
+      } else {
+        in.zb_nm = I.zb[nm]; in.zb_nmu = I.zb[nmu];
+      }
+      finish_uv<R, SUBGRID>(I, ip, in, dt, q1, u1, mdt, br);
+    } else {
+      q1[ip] = q0[ip]; u1[ip] = u0[ip];
+    }
+  }
+  block_min_to(mdt, &S->min_dt_bits);
+  if (I.count) atomicAdd(&bc[br], 1u);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BLOCK) k_momentum(const DevInst* __restrict__ insts, const Item* __restrict__ items,
+                                                    int indexed) {
+  const Item it = items[blockIdx.x];
+  if (it.inst < 0) return;
+  const DevInst& I = insts[it.inst];
+  if (!I.sc->active) return;
+  __shared__ unsigned int bc[4];
+  if (threadIdx.x < 4) bc[threadIdx.x] = 0;
+  __syncthreads();
+  if (indexed) {
+    if (I.subgrid) mom_idx_tile<R, true>(I, it.tile, bc);
+    else mom_idx_tile<R, false>(I, it.tile, bc);
+  } else {
+    if (I.subgrid) mom_dense_tile<R, true>(I, it.tile, bc);
+    else mom_dense_tile<R, false>(I, it.tile, bc);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    I.sc->mom_done = 1;
+    if (I.count)
+      for (int k = 0; k < 4; ++k)
+        if (bc[k]) atomicAdd(&I.sc->cnt[k], (unsigned long long)bc[k]);
+  }
+}
+
+// Combined uv-points at refinement transitions (sfincs_momentum.f90:757-779).
+template <typename R>
+__global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  if (it.inst < 0) return;
+  const DevInst& I = insts[it.inst];
+  if (!I.sc->active) return;
+  const int j = it.tile * BLOCK + threadIdx.x;
+  if (j >= I.ncuv) return;
+  const int cur1 = I.sc->cur ^ 1;
+  R* q = (R*)I.q[cur1];
+  R* uv = (R*)I.uv[cur1];
+  const int c = I.cuv[j], a = I.cuv[I.ncuv + j], b = I.cuv[2 * I.ncuv + j];
+  q[c] = (q[a] + q[b]) / R(2);
+  uv[c] = (uv[a] + uv[b]) / R(2);
+}
+
+// ----------------------------------------------------------------------------- continuity
+template <typename R, bool SUBGRID>
+__device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref, R dt,
+                                          R netprcp) {
+  const uint8_t kraw = I.kcs[c];
+  const int kcs = kraw & 0x7f;
+  double zs = I.zs[c];
+  R* zsmaxp = (R*)I.zsmax;
+  R zsmax = zsmaxp[c];
+  if (SUBGRID) {
+    double vol = I.vol[c];
+    if (kraw & 0x80) {   // point sources of this cell, in source order (:311-325)
+      for (int s = 0; s < I.nsrc; ++s)
+        if (I.src_cell[s] == c) {
+          const float qs = I.qtsrc[s];
+          if ((vol >= 0) || ((qs < 0.0f) && (vol >= 0))) vol = vol + (double)((R)qs * dt);
+        }
+    }
+    R* zs0p = (R*)I.zs0;
+    R* zsdp = (R*)I.zsderv;
+    R zs0 = zs0p[c], zsd = zsdp[c];
+    const int nl = I.c.nl;
+    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, I.z_zmin[c],
+                         I.z_zmax[c], I.z_volmax[c], I.z_dep + (size_t)c * nl, I.c, dt);
+    I.vol[c] = vol;
+    if (kcs == 1 || kcs == 4) {
+      zs0p[c] = zs0;
+      zsdp[c] = zsd;
+    }
+  } else {
+    if (kraw & 0x80) {   // (:87-106)
+      for (int s = 0; s < I.nsrc; ++s)
+        if (I.src_cell[s] == c) {
+          const R inc = (R)I.qtsrc[s] * dt / (R)I.c.cell_area[iref - 1];
+          zs = sfx::vmax<double>(zs + (double)inc, (double)I.zb[c]);
+        }
+    }
+    sfx::cell_regular<R>(zs, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, I.c, dt);
+  }
+  I.zs[c] = zs;
+  if (I.c.flags & sfx::F_ZSMAX) zsmaxp[c] = zsmax;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts,
+                                                      const Item* __restrict__ items, int indexed) {
+  const Item it = items[blockIdx.x];
+  if (it.inst < 0) return;
+  const DevInst& I = insts[it.inst];
+  Scalars* S = I.sc;
+  if (!S->active) return;
+  const int qb = S->mom_done ? (S->cur ^ 1) : S->cur;
+  const R* __restrict__ q = (const R*)I.q[qb];
+  const R dt = (R)S->dt;
+  const R netprcp = (R)S->netprcp;
+  if (!indexed) {
+    const int nmax = I.nmax, mmax = I.mmax, np = I.np;
+    const int tn = it.tile % I.ntn, tmi = it.tile / I.ntn;
+    const int n = tn * I.tn + (threadIdx.x % I.tn);
+    const int m = tmi * I.tm + (threadIdx.x / I.tn);
+    if (n >= nmax || m >= mmax) return;
+    const int c = m * nmax + n;
+    const R q_md = m >= 1 ? q[c - nmax] : R(0);
+    const R q_mu = m < mmax - 1 ? q[c] : R(0);
+    const R q_nd = n >= 1 ? q[np + c - 1] : R(0);
+    const R q_nu = n < nmax - 1 ? q[np + c] : R(0);
+    if (I.subgrid) cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
+    else cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
+  } else {
+    const int c = it.tile * BLOCK + threadIdx.x;
+    if (c >= I.np) return;
+    const R q_md = q[I.zmd[c]], q_mu = q[I.zmu[c]], q_nd = q[I.znd[c]], q_nu = q[I.znu[c]];
+    const int iref = I.z_iref[c];
+    if (I.subgrid) cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+    else cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+  }
+}
+
+// FP64 volume accumulator: sum over active cells of V (subgrid) or zs*A (regular).
+__global__ void __launch_bounds__(BLOCK) k_volume(const DevInst* __restrict__ I0, double* out) {
+  const DevInst& I = *I0;
+  double acc = 0.0;
+  for (int c = blockIdx.x * BLOCK + threadIdx.x; c < I.np; c += gridDim.x * BLOCK) {
+    const int k = I.kcs[c] & 0x7f;
+    if (k == 0) continue;
+    const int iref = I.dense ? 1 : I.z_iref[c];
+    if (I.subgrid) acc += I.vol[c];
+    else acc += I.zs[c] * (double)I.c.cell_area[iref - 1];
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
+  __shared__ double red[BLOCK / WAVE];
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < BLOCK / WAVE; ++i) s += red[i];
+    atomicAdd(out, s);
+  }
+}
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+}  // namespace
+
+// =============================================================================== host side
+struct sfx_handle {
+  int device = 0;
+  int fp64 = 0;
+  sfx_sizes sz{};
+  DevInst hI{};
+  DevInst* dI = nullptr;
+  Scalars* dS = nullptr;
+  std::vector<void*> allocs;
+  std::vector<int32_t> ref2slot;   // reference uv index (0-based, incl. cuv and dummy) -> slot
+  int32_t nslot = 0;
+  hipStream_t stream = nullptr;
+  struct sfx_batch* solo = nullptr;
+  int tile_tn = 64, tile_tm = 4;
+  double* dvol = nullptr;
+  template <typename T> T* upload(const T* src, size_t n);
+  template <typename T> T* alloc(size_t n);
+};
+
+template <typename T> T* sfx_handle::alloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+  (void)hipMemset(p, 0, n * sizeof(T));
+  allocs.push_back(p);
+  return (T*)p;
+}
+
+template <typename T> T* sfx_handle::upload(const T* src, size_t n) {
+  T* p = alloc<T>(n);
+  if (p && n && src) (void)hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice);
+  return p;
+}
+
+struct sfx_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  std::vector<sfx_handle*> members;
+  std::vector<double> tend;
+  DevInst* dInsts = nullptr;
+  size_t dInsts_cap = 0;
+  Item* dItems = nullptr;
+  size_t items_cap = 0;
+  int32_t* dIds = nullptr;
+  size_t ids_cap = 0;
+  // launch descriptors (offsets into dItems)
+  size_t off_md = 0, n_md = 0, off_mi = 0, n_mi = 0, off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
+  bool dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+  int fp64 = 0;
+};
+
+namespace {
+
+uint32_t build_flags(const sfx_model* M) {
+  const sfx_params& p = M->p;
+  uint32_t f = 0;
+  if (p.advection) f |= sfx::F_ADVECTION;
+  if (p.advection_scheme == 1) f |= sfx::F_UPW1;
+  if (p.viscosity) f |= sfx::F_VISCOSITY;
+  if (M->coriolis) f |= sfx::F_CORIOLIS;
+  if (p.friction2d) f |= sfx::F_FRICTION2D;
+  if (M->thetasmoothing) f |= sfx::F_THETA;
+  if (p.wiggle_suppression) f |= sfx::F_WIGGLE;
+  if (p.store_zsmax) f |= sfx::F_ZSMAX;
+  if (M->wind) f |= sfx::F_WIND;
+  if (M->precip) f |= sfx::F_PRECIP;
+  if (p.h73table) f |= sfx::F_H73;
+  if (M->use_quadtree) f |= sfx::F_QUADTREE;
+  if (M->subgrid) f |= sfx::F_SUBGRID;
+  if (p.advection || M->coriolis || p.viscosity || p.friction2d || M->thetasmoothing) f |= sfx::F_NBRS;
+  return f;
+}
+
+int fill_consts(const sfx_model* M, Consts& c) {
+  const sfx_params& p = M->p;
+  if (M->nref > sfx::MAXREF) { set_error("more than 8 refinement levels"); return SFX_ERR_ARG; }
+  std::memset(&c, 0, sizeof(c));
+  c.g = p.g; c.huthresh = p.huthresh; c.huvmin = p.huvmin; c.advlim = p.advlim; c.slopelim = p.slopelim;
+  c.uvlim = p.uvlim; c.nuviscfac = p.nuviscfac; c.fcorio = M->fcorio; c.theta = p.theta;
+  c.wiggle_factor = p.wiggle_factor; c.wiggle_threshold = p.wiggle_threshold; c.dtmax = M->dtmax; c.dtmin = M->dtmin;
+  c.alpha = p.alpha; c.rhoa = p.rhoa; c.rhow = p.rhow; c.btfilter = p.btfilter; c.btrelax = p.btrelax;
+  c.t0 = p.t0; c.t1 = p.t1; c.tspinup = p.t0 + p.tspinup; c.zini = p.zsini; c.dx = M->dxrm[0]; c.dy = M->dyrm[0];
+  c.pi = p.pi;
+  for (int i = 0; i < M->nref; ++i) {
+    c.dxrinv[i] = M->dxrinv[i]; c.dyrinv[i] = M->dyrinv[i]; c.dxr2inv[i] = M->dxr2inv[i]; c.dyr2inv[i] = M->dyr2inv[i];
+    c.dxrinvc[i] = M->dxrinvc[i]; c.dyrinvc[i] = M->dyrinvc[i]; c.cell_area[i] = M->cell_area[i];
+    c.dxrm[i] = M->dxrm[i]; c.dyrm[i] = M->dyrm[i]; c.nuvisc[i] = M->nuvisc[i];
+  }
+  c.flags = build_flags(M);
+  c.nl = M->nl;
+  return SFX_OK;
+}
+
+template <typename R> std::vector<R> to_r(const std::vector<float>& v) { return std::vector<R>(v.begin(), v.end()); }
+
+template <typename R>
+int upload_state(sfx_handle* H, const sfx_model* M) {
+  DevInst& I = H->hI;
+  const int np = M->np;
+  // q / uv double buffers in slot layout
+  std::vector<R> qs(H->nslot + 1, R(0)), uvs(H->nslot + 1, R(0));
+  for (size_t k = 0; k < H->ref2slot.size(); ++k) {
+    const int s = H->ref2slot[k];
+    if (s >= 0 && k < M->q.size()) { qs[s] = M->q[k]; uvs[s] = M->uv[k]; }
+  }
+  for (int b = 0; b < 2; ++b) { I.q[b] = H->upload(qs.data(), qs.size()); I.uv[b] = H->upload(uvs.data(), uvs.size()); }
+  auto zs0 = to_r<R>(M->zs0); auto zsd = to_r<R>(M->zsderv); auto zsm = to_r<R>(M->zsmax);
+  I.zs0 = H->upload(zs0.data(), np); I.zsderv = H->upload(zsd.data(), np); I.zsmax = H->upload(zsm.data(), np);
+  auto um = to_r<R>(M->uvmean);
+  I.uvmean = H->upload(um.data(), um.size());
+  return SFX_OK;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------- create / destroy
+extern "C" int sfx_model_footprint(const sfx_model* M, size_t* bytes) {
+  if (!M || !bytes) { set_error("sfx_model_footprint: null"); return SFX_ERR_ARG; }
+  const size_t np = M->np, nslot = M->dense ? 2 * (size_t)M->np : (size_t)(M->npuv + M->ncuv);
+  size_t b = np * (8 + 4 * 3 + 1) + nslot * (4 * 4 + 3);
+  if (M->subgrid) b += np * (8 + 12 + 4 * (size_t)M->nl) + nslot * (4 + 16 + 4 + 16 * (size_t)M->nl);
+  else b += np * 4 + nslot * 8;
+  if (!M->dense) b += (size_t)M->npuv * (40 + 1) + np * 17 + (size_t)M->ncuv * 12;
+  *bytes = b + (1 << 20);
+  return SFX_OK;
+}
+
+extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, sfx_handle** out);
+
+extern "C" int sfx_create(const sfx_model* M, int32_t device, sfx_handle** out) {
+  return sfx_create_ex(M, device, 0, out);
+}
+
+extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, sfx_handle** out) {
+  if (!M || !out || !M->initialized) { set_error("sfx_create: model not initialized"); return SFX_ERR_STATE; }
+  if (M->p.bndtype != 1) { set_error("only bndtype=1 (weakly reflective) boundaries are supported"); return SFX_ERR_ARG; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("no HIP device available"); return SFX_ERR_HIP; }
+  if (device < 0 || device >= ndev) { set_error("bad device index"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(device));
+  sfx_handle* H = new sfx_handle();
+  H->device = device;
+  H->fp64 = fp64 ? 1 : 0;
+  sfx_model_sizes(M, &H->sz);
+  DevInst& I = H->hI;
+  std::memset(&I, 0, sizeof(I));
+  int rc = fill_consts(M, I.c);
+  if (rc) { delete H; return rc; }
+  const int np = M->np, npuv = M->npuv, ncuv = M->ncuv, nl = M->nl;
+  I.dense = M->dense; I.subgrid = M->subgrid; I.fp64 = H->fp64;
+  I.np = np; I.npuv = npuv; I.ncuv = ncuv; I.nmax = M->nmax; I.mmax = M->mmax; I.nl = nl;
+  // slot mapping
+  H->ref2slot.assign(npuv + ncuv + 1, -1);
+  if (M->dense) {
+    H->nslot = 2 * np;
+    for (int ip = 0; ip < npuv; ++ip) {
+      const int c = M->uv_index_z_nm[ip] - 1;
+      H->ref2slot[ip] = M->uv_flags_dir[ip] == 0 ? c : np + c;
+    }
+    H->ref2slot[npuv + ncuv] = H->nslot;
+  } else {
+    H->nslot = npuv + ncuv;
+    for (int k = 0; k < npuv + ncuv + 1; ++k) H->ref2slot[k] = k;
+  }
+  I.nslot = H->nslot;
+  const int ns = H->nslot + 1;
+  auto perm8 = [&](const std::vector<int8_t>& src, int8_t fill) {
+    std::vector<int8_t> d(ns, fill);
+    for (size_t k = 0; k < src.size() && k < H->ref2slot.size(); ++k) if (H->ref2slot[k] >= 0) d[H->ref2slot[k]] = src[k];
+    return d;
+  };
+  auto permf = [&](const std::vector<float>& src) {
+    std::vector<float> d(ns, 0.f);
+    for (size_t k = 0; k < src.size() && k < H->ref2slot.size(); ++k) if (H->ref2slot[k] >= 0) d[H->ref2slot[k]] = src[k];
+    return d;
+  };
+  // cells
+  I.zs = H->upload(M->zs.data(), np);
+  if (M->subgrid) I.vol = H->upload(M->z_volume.data(), np);
+  {
+    std::vector<uint8_t> k(np);
+    for (int c = 0; c < np; ++c) k[c] = (uint8_t)M->kcs[c];
+    for (int s = 0; s < M->nsrc; ++s) k[M->nmindsrc[s] - 1] |= 0x80;
+    I.kcs = H->upload(k.data(), np);
+  }
+  if (!M->subgrid) I.zb = H->upload(M->zb.data(), np);
+  if (M->subgrid) {
+    I.z_zmin = H->upload(M->z_zmin.data(), np); I.z_zmax = H->upload(M->z_zmax.data(), np);
+    I.z_volmax = H->upload(M->z_volmax.data(), np); I.z_dep = H->upload(M->z_dep.data(), (size_t)np * nl);
+  }
+  if (!M->dense) {
+    I.z_iref = H->upload(M->z_flags_iref.data(), np);
+    auto conv = [&](const std::vector<int32_t>& v) { std::vector<int32_t> d(v.size()); for (size_t i = 0; i < v.size(); ++i) d[i] = v[i] - 1; return d; };
+    auto a = conv(M->z_index_uv_md), b = conv(M->z_index_uv_mu), c2 = conv(M->z_index_uv_nd), d = conv(M->z_index_uv_nu);
+    I.zmd = H->upload(a.data(), np); I.zmu = H->upload(b.data(), np); I.znd = H->upload(c2.data(), np); I.znu = H->upload(d.data(), np);
+    std::vector<int32_t> nb((size_t)10 * npuv);
+    const std::vector<int32_t>* tabs[10] = {&M->uv_index_z_nm, &M->uv_index_z_nmu, &M->uv_index_u_nmd, &M->uv_index_u_nmu,
+                                             &M->uv_index_u_ndm, &M->uv_index_u_num, &M->uv_index_v_ndm, &M->uv_index_v_nm,
+                                             &M->uv_index_v_ndmu, &M->uv_index_v_nmu};
+    for (int t = 0; t < 10; ++t)
+      for (int i = 0; i < npuv; ++i) nb[(size_t)t * npuv + i] = (*tabs[t])[i] - 1;
+    I.nb = H->upload(nb.data(), nb.size());
+    std::vector<uint8_t> f(npuv);
+    for (int i = 0; i < npuv; ++i)
+      f[i] = (uint8_t)((M->uv_flags_iref[i] & 15) | (((M->uv_flags_type[i] + 1) & 3) << 4) | ((M->uv_flags_dir[i] & 1) << 6));
+    I.uvf = H->upload(f.data(), npuv);
+    std::vector<int32_t> cv((size_t)3 * ncuv);
+    for (int j = 0; j < ncuv; ++j) {
+      cv[j] = M->cuv_index_uv[j] - 1; cv[ncuv + j] = M->cuv_index_uv1[j] - 1; cv[2 * ncuv + j] = M->cuv_index_uv2[j] - 1;
+    }
+    I.cuv = H->upload(cv.data(), cv.size());
+  }
+  // uv slots
+  {
+    auto kc = perm8(M->kcuv, 0);
+    I.kcuv = H->upload(kc.data(), ns);
+    auto ma = perm8(M->mask_adv, 0);
+    I.mask_adv = H->upload(ma.data(), ns);
+    auto kf = perm8(M->kfuv, 0);
+    I.kfuv = H->upload(kf.data(), ns);
+  }
+  if (!M->subgrid) {
+    auto a = permf(M->zbuvmx), b = permf(M->gn2uv);
+    I.zbuvmx = H->upload(a.data(), ns); I.gn2uv = H->upload(b.data(), ns);
+  } else {
+    auto zmn = permf(M->uv_zmin);
+    I.uv_zmin = H->upload(zmn.data(), ns);
+    std::vector<float4> s4(ns, make_float4(0, 0, 0, 0));
+    std::vector<float> ff(ns, 0.f);
+    std::vector<float4> lev((size_t)ns * nl, make_float4(0, 0, 0, 0));
+    for (int ip = 0; ip < npuv; ++ip) {
+      const int s = H->ref2slot[ip];
+      s4[s] = make_float4(M->uv_zmax[ip], M->uv_havg_zmax[ip], M->uv_nrep_zmax[ip], M->uv_navg_w[ip]);
+      ff[s] = M->uv_fnfit[ip];
+      for (int l = 0; l < nl; ++l)
+        lev[(size_t)s * nl + l] = make_float4(M->uv_havg[(size_t)ip * nl + l], M->uv_nrep[(size_t)ip * nl + l],
+                                              M->uv_pwet[(size_t)ip * nl + l], 0.f);
+    }
+    I.uv_s4 = H->upload(s4.data(), ns); I.uv_fnfit = H->upload(ff.data(), ns);
+    I.uv_lev = H->upload(lev.data(), lev.size());
+  }
+  // boundaries
+  I.ngbnd = M->ngbnd; I.nkcuv2 = M->nkcuv2; I.nbnd = M->nbnd; I.ntbnd = M->ntbnd;
+  {
+    std::vector<int32_t> cell(M->ngbnd), nmi(M->ngbnd);
+    for (int i = 0; i < M->ngbnd; ++i) { cell[i] = M->nmindbnd[i] - 1; nmi[i] = M->nmi_gbp[i] - 1; }
+    I.gb_cell = H->upload(cell.data(), cell.size()); I.gb_nmi = H->upload(nmi.data(), nmi.size());
+    I.gb_ind1 = H->upload(M->ind1_bnd_gbp.data(), M->ngbnd); I.gb_ind2 = H->upload(M->ind2_bnd_gbp.data(), M->ngbnd);
+    I.gb_fac = H->upload(M->fac_bnd_gbp.data(), M->ngbnd);
+    I.zsb = H->upload(M->zsb.data(), M->ngbnd); I.zsb0 = H->upload(M->zsb0.data(), M->ngbnd);
+    std::vector<int32_t> slot(M->nkcuv2), nmb(M->nkcuv2), nmi2(M->nkcuv2), ib(M->nkcuv2);
+    std::vector<float> zbuv(M->nkcuv2, 0.f);
+    for (int i = 0; i < M->nkcuv2; ++i) {
+      slot[i] = H->ref2slot[M->index_kcuv2[i] - 1]; nmb[i] = M->nmbkcuv2[i] - 1; nmi2[i] = M->nmikcuv2[i] - 1;
+      ib[i] = M->ibkcuv2[i] - 1;
+      if (!M->subgrid) zbuv[i] = M->zbuv[M->index_kcuv2[i] - 1];
+    }
+    I.k2_slot = H->upload(slot.data(), slot.size()); I.k2_nmb = H->upload(nmb.data(), nmb.size());
+    I.k2_nmi = H->upload(nmi2.data(), nmi2.size()); I.k2_ib = H->upload(ib.data(), ib.size());
+    I.k2_dir = H->upload(M->ibuvdir.data(), M->ibuvdir.size()); I.k2_zbuv = H->upload(zbuv.data(), zbuv.size());
+    I.t_bnd = H->upload(M->t_bnd.data(), M->t_bnd.size()); I.zs_bnd = H->upload(M->zs_bnd.data(), M->zs_bnd.size());
+    I.zst_bnd = H->alloc<float>(std::max(1, M->nbnd));
+  }
+  I.nsrc = M->nsrc; I.ntsrc = M->ntsrc; I.ntprcp = M->ntprcp; I.ntwnd = M->ntwnd;
+  {
+    std::vector<int32_t> sc(M->nsrc);
+    for (int s = 0; s < M->nsrc; ++s) sc[s] = M->nmindsrc[s] - 1;
+    I.src_cell = H->upload(sc.data(), sc.size()); I.tsrc = H->upload(M->tsrc.data(), M->tsrc.size());
+    I.qsrc = H->upload(M->qsrc.data(), M->qsrc.size()); I.qtsrc = H->alloc<float>(std::max(1, M->nsrc));
+    I.tprcpt = H->upload(M->tprcpt.data(), M->tprcpt.size()); I.tprcpv = H->upload(M->tprcpv.data(), M->tprcpv.size());
+    I.twnd = H->upload(M->twnd.data(), M->twnd.size()); I.wndmag = H->upload(M->wndmag.data(), M->wndmag.size());
+    I.wnddir = H->upload(M->wnddir.data(), M->wnddir.size()); I.cdval = H->upload(M->cdval.data(), M->cdval.size());
+    I.x73 = H->upload(M->x73.data(), M->x73.size());
+  }
+  rc = H->fp64 ? upload_state<double>(H, M) : upload_state<float>(H, M);
+  // scalars
+  Scalars s{};
+  s.t = (double)M->p.t0; s.tend = (double)M->p.t1;
+  s.dt = 1.0e-6f; s.min_dt = 1.0e-6f;   // sfincs_lib.f90:281-282
+  s.itbndlast = 1; s.itsrclast = 1; s.itwndlast = 1; s.itprcplast = 1;
+  H->dS = H->upload(&s, 1);
+  I.sc = H->dS;
+  // tiles
+  I.tn = H->tile_tn; I.tm = H->tile_tm;
+  I.ntn = M->dense ? (M->nmax + I.tn - 1) / I.tn : 0;
+  H->dvol = H->alloc<double>(1);
+  H->dI = H->upload(&I, 1);
+  for (void* p : H->allocs) if (!p) { set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
+  *out = H;
+  return rc;
+}
+
+extern "C" int sfx_batch_destroy(sfx_batch* b);
+
+extern "C" int sfx_destroy(sfx_handle* H) {
+  if (!H) return SFX_OK;
+  hipSetDevice(H->device);
+  if (H->solo) { sfx_batch_destroy(H->solo); H->solo = nullptr; }
+  hipDeviceSynchronize();
+  for (void* p : H->allocs) hipFree(p);
+  if (H->stream) hipStreamDestroy(H->stream);
+  delete H;
+  return SFX_OK;
+}
+
+// -------------------------------------------------------------------------- state exchange
+namespace {
+
+int read_scalars(sfx_handle* H, Scalars& s) {
+  HIPCHK(hipStreamSynchronize(H->stream));
+  HIPCHK(hipMemcpy(&s, H->dS, sizeof(Scalars), hipMemcpyDeviceToHost));
+  return SFX_OK;
+}
+int write_scalars(sfx_handle* H, const Scalars& s) {
+  HIPCHK(hipStreamSynchronize(H->stream));
+  HIPCHK(hipMemcpy(H->dS, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+  return SFX_OK;
+}
+// Normalise the double buffer so that `cur` holds the latest q/uv.
+int normalise(sfx_handle* H, Scalars& s) {
+  int rc = read_scalars(H, s);
+  if (rc) return rc;
+  if (s.mom_done) {
+    s.cur ^= 1; s.mom_done = 0;
+    std::memcpy(&s.min_dt, &s.min_dt_bits, 4);
+    rc = write_scalars(H, s);
+  }
+  return rc;
+}
+
+template <typename R>
+int get_uv_field(sfx_handle* H, void* src, void* dst, size_t nbytes, bool is_q) {
+  (void)is_q;
+  const size_t n = H->ref2slot.size();
+  if (nbytes != n * sizeof(R) && nbytes != n * sizeof(float)) { set_error("size mismatch"); return SFX_ERR_ARG; }
+  std::vector<R> buf(H->nslot + 1);
+  HIPCHK(hipMemcpy(buf.data(), src, buf.size() * sizeof(R), hipMemcpyDeviceToHost));
+  if (nbytes == n * sizeof(R)) {
+    R* d = (R*)dst;
+    for (size_t k = 0; k < n; ++k) d[k] = H->ref2slot[k] >= 0 ? buf[H->ref2slot[k]] : R(0);
+  } else {
+    float* d = (float*)dst;
+    for (size_t k = 0; k < n; ++k) d[k] = H->ref2slot[k] >= 0 ? (float)buf[H->ref2slot[k]] : 0.f;
+  }
+  return SFX_OK;
+}
+
+template <typename R>
+int set_uv_field(sfx_handle* H, void* dst, const void* src, size_t nbytes) {
+  const size_t n = H->ref2slot.size();
+  if (nbytes != n * sizeof(float) && nbytes != n * sizeof(R)) { set_error("size mismatch"); return SFX_ERR_ARG; }
+  std::vector<R> buf(H->nslot + 1, R(0));
+  HIPCHK(hipMemcpy(buf.data(), dst, buf.size() * sizeof(R), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < n; ++k) {
+    if (H->ref2slot[k] < 0) continue;
+    buf[H->ref2slot[k]] = nbytes == n * sizeof(R) ? ((const R*)src)[k] : (R)((const float*)src)[k];
+  }
+  buf[H->nslot] = R(0);
+  HIPCHK(hipMemcpy(dst, buf.data(), buf.size() * sizeof(R), hipMemcpyHostToDevice));
+  return SFX_OK;
+}
+
+template <typename R>
+int get_cell_r(sfx_handle* H, const void* src, void* dst, size_t nbytes, size_t n) {
+  if (nbytes == n * sizeof(R)) { HIPCHK(hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost)); return SFX_OK; }
+  if (nbytes != n * sizeof(float)) { set_error("size mismatch"); return SFX_ERR_ARG; }
+  std::vector<R> b(n);
+  HIPCHK(hipMemcpy(b.data(), src, n * sizeof(R), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) ((float*)dst)[i] = (float)b[i];
+  return SFX_OK;
+}
+
+template <typename R>
+int set_cell_r(sfx_handle* H, void* dst, const void* src, size_t nbytes, size_t n) {
+  (void)H;
+  if (nbytes == n * sizeof(R)) { HIPCHK(hipMemcpy(dst, src, nbytes, hipMemcpyHostToDevice)); return SFX_OK; }
+  if (nbytes != n * sizeof(float)) { set_error("size mismatch"); return SFX_ERR_ARG; }
+  std::vector<R> b(n);
+  for (size_t i = 0; i < n; ++i) b[i] = (R)((const float*)src)[i];
+  HIPCHK(hipMemcpy(dst, b.data(), n * sizeof(R), hipMemcpyHostToDevice));
+  return SFX_OK;
+}
+
+}  // namespace
+
+extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t nbytes) {
+  if (!H || !name || !dst) { set_error("sfx_get_state: null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = normalise(H, s);
+  if (rc) return rc;
+  const std::string k = name;
+  const size_t np = H->hI.np;
+  DevInst& I = H->hI;
+  if (k == "zs") { if (nbytes != np * 8) { set_error("size mismatch"); return SFX_ERR_ARG; } HIPCHK(hipMemcpy(dst, I.zs, nbytes, hipMemcpyDeviceToHost)); return SFX_OK; }
+  if (k == "z_volume") {
+    if (!I.vol || nbytes != np * 8) { set_error("no z_volume / size mismatch"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(dst, I.vol, nbytes, hipMemcpyDeviceToHost)); return SFX_OK;
+  }
+  if (k == "q" || k == "uv") {
+    void* src = k == "q" ? I.q[s.cur] : I.uv[s.cur];
+    return H->fp64 ? get_uv_field<double>(H, src, dst, nbytes, k == "q") : get_uv_field<float>(H, src, dst, nbytes, k == "q");
+  }
+  if (k == "kfuv") {
+    const size_t n = H->sz.npuv;
+    if (nbytes != n) { set_error("size mismatch"); return SFX_ERR_ARG; }
+    std::vector<int8_t> b(H->nslot + 1);
+    HIPCHK(hipMemcpy(b.data(), I.kfuv, b.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) ((int8_t*)dst)[i] = b[H->ref2slot[i]];
+    return SFX_OK;
+  }
+  if (k == "zs0" || k == "zsderv" || k == "zsmax") {
+    void* src = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
+    return H->fp64 ? get_cell_r<double>(H, src, dst, nbytes, np) : get_cell_r<float>(H, src, dst, nbytes, np);
+  }
+  if (k == "uvmean") {
+    const size_t n = H->sz.nkcuv2;
+    return H->fp64 ? get_cell_r<double>(H, I.uvmean, dst, nbytes, n) : get_cell_r<float>(H, I.uvmean, dst, nbytes, n);
+  }
+  set_error("unknown state field " + k);
+  return SFX_ERR_ARG;
+}
+
+extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, size_t nbytes) {
+  if (!H || !name || !src) { set_error("sfx_set_state: null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = normalise(H, s);
+  if (rc) return rc;
+  const std::string k = name;
+  const size_t np = H->hI.np;
+  DevInst& I = H->hI;
+  if (k == "zs") { if (nbytes != np * 8) { set_error("size mismatch"); return SFX_ERR_ARG; } HIPCHK(hipMemcpy(I.zs, src, nbytes, hipMemcpyHostToDevice)); return SFX_OK; }
+  if (k == "z_volume") {
+    if (!I.vol || nbytes != np * 8) { set_error("no z_volume / size mismatch"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(I.vol, src, nbytes, hipMemcpyHostToDevice)); return SFX_OK;
+  }
+  if (k == "q" || k == "uv") {
+    void* dst = k == "q" ? I.q[s.cur] : I.uv[s.cur];
+    return H->fp64 ? set_uv_field<double>(H, dst, src, nbytes) : set_uv_field<float>(H, dst, src, nbytes);
+  }
+  if (k == "kfuv") {
+    const size_t n = H->sz.npuv;
+    if (nbytes != n) { set_error("size mismatch"); return SFX_ERR_ARG; }
+    std::vector<int8_t> b(H->nslot + 1, 0);
+    HIPCHK(hipMemcpy(b.data(), I.kfuv, b.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) b[H->ref2slot[i]] = ((const int8_t*)src)[i];
+    HIPCHK(hipMemcpy(I.kfuv, b.data(), b.size(), hipMemcpyHostToDevice));
+    return SFX_OK;
+  }
+  if (k == "zs0" || k == "zsderv" || k == "zsmax") {
+    void* dst = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
+    return H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
+  }
+  set_error("unknown state field " + k);
+  return SFX_ERR_ARG;
+}
+
+extern "C" int sfx_get_time(sfx_handle* H, double* t, float* dt, int64_t* nt) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = read_scalars(H, s);
+  if (rc) return rc;
+  if (s.error_pending) { s.error = 1; s.t = (double)H->hI.c.t1 + 1.0; }
+  if (t) *t = s.t;
+  if (dt) *dt = s.dt;
+  if (nt) *nt = s.nt;
+  return SFX_OK;
+}
+
+// -------------------------------------------------------------------------- batches / launches
+namespace {
+
+size_t dense_tiles(const DevInst& I) {
+  const int ntm = (I.mmax + I.tm - 1) / I.tm;
+  return (size_t)I.ntn * ntm;
+}
+
+// XCD-aware ordering: the hardware hands workgroup b to XCD b % 8, so item b is the
+// (b%8)-th eighth of the natural tile order; each XCD sweeps a contiguous strip and
+// the neighbours a tile reads stay in that XCD's L2.
+void xcd_order(std::vector<Item>& v) {
+  const size_t n = v.size();
+  size_t padded = (n + 7) / 8 * 8;
+  std::vector<Item> nat(v);
+  nat.resize(padded, Item{-1, 0});
+  const size_t per = padded / 8;
+  v.assign(padded, Item{-1, 0});
+  for (size_t b = 0; b < padded; ++b) v[b] = nat[(b % 8) * per + b / 8];
+}
+
+int rebuild(sfx_batch* B) {
+  HIPCHK(hipSetDevice(B->device));
+  const size_t nm = B->members.size();
+  std::vector<DevInst> insts(nm);
+  std::vector<Item> md, mi, cu, cd, ci;
+  std::vector<int32_t> ids(nm);
+  int fp64 = -1;
+  for (size_t i = 0; i < nm; ++i) {
+    sfx_handle* H = B->members[i];
+    insts[i] = H->hI;
+    ids[i] = (int32_t)i;
+    if (fp64 < 0) fp64 = H->fp64;
+    if (fp64 != H->fp64) { set_error("a batch cannot mix FP32 and FP64 members"); return SFX_ERR_ARG; }
+    const DevInst& I = H->hI;
+    if (I.dense) {
+      const size_t nt = dense_tiles(I);
+      for (size_t t = 0; t < nt; ++t) { md.push_back(Item{(int)i, (int)t}); cd.push_back(Item{(int)i, (int)t}); }
+    } else {
+      const int tu = (I.npuv + BLOCK - 1) / BLOCK, tc = (I.np + BLOCK - 1) / BLOCK, tq = (I.ncuv + BLOCK - 1) / BLOCK;
+      for (int t = 0; t < tu; ++t) mi.push_back(Item{(int)i, t});
+      for (int t = 0; t < tc; ++t) ci.push_back(Item{(int)i, t});
+      for (int t = 0; t < tq; ++t) cu.push_back(Item{(int)i, t});
+    }
+  }
+  B->fp64 = fp64 < 0 ? 0 : fp64;
+  xcd_order(md); xcd_order(cd); xcd_order(mi); xcd_order(ci);
+  if (cu.size()) xcd_order(cu);
+  std::vector<Item> all;
+  B->off_md = all.size(); all.insert(all.end(), md.begin(), md.end()); B->n_md = md.size();
+  B->off_mi = all.size(); all.insert(all.end(), mi.begin(), mi.end()); B->n_mi = mi.size();
+  B->off_cuv = all.size(); all.insert(all.end(), cu.begin(), cu.end()); B->n_cuv = cu.size();
+  B->off_cd = all.size(); all.insert(all.end(), cd.begin(), cd.end()); B->n_cd = cd.size();
+  B->off_ci = all.size(); all.insert(all.end(), ci.begin(), ci.end()); B->n_ci = ci.size();
+  if (B->dInsts_cap < nm) {
+    if (B->dInsts) hipFree(B->dInsts);
+    B->dInsts_cap = std::max<size_t>(nm, 8);
+    HIPCHK(hipMalloc(&B->dInsts, B->dInsts_cap * sizeof(DevInst)));
+  }
+  if (B->items_cap < all.size()) {
+    if (B->dItems) hipFree(B->dItems);
+    B->items_cap = std::max<size_t>(all.size(), 64);
+    HIPCHK(hipMalloc(&B->dItems, B->items_cap * sizeof(Item)));
+  }
+  if (B->ids_cap < nm) {
+    if (B->dIds) hipFree(B->dIds);
+    B->ids_cap = std::max<size_t>(nm, 8);
+    HIPCHK(hipMalloc(&B->dIds, B->ids_cap * sizeof(int32_t)));
+  }
+  if (nm) HIPCHK(hipMemcpy(B->dInsts, insts.data(), nm * sizeof(DevInst), hipMemcpyHostToDevice));
+  if (all.size()) HIPCHK(hipMemcpy(B->dItems, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice));
+  if (nm) HIPCHK(hipMemcpy(B->dIds, ids.data(), nm * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+  if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
+  B->graph_steps = 0;
+  B->dirty = false;
+  return SFX_OK;
+}
+
+// Launch phases of one step. what: bit 0 forcing, bit 1 momentum (+cuv), bit 2 continuity.
+template <typename R>
+void launch_phases(sfx_batch* B, int what, int forcing_mode) {
+  hipStream_t st = B->stream;
+  const int nm = (int)B->members.size();
+  if ((what & 1) && nm) hipLaunchKernelGGL(k_forcing<R>, dim3(nm), dim3(BLOCK), 0, st, B->dInsts, B->dIds, forcing_mode);
+  if (what & 2) {
+    if (B->n_md) hipLaunchKernelGGL(k_momentum<R>, dim3(B->n_md), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_md, 0);
+    if (B->n_mi) hipLaunchKernelGGL(k_momentum<R>, dim3(B->n_mi), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_mi, 1);
+    if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cuv);
+  }
+  if (what & 4) {
+    if (B->n_cd) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cd, 0);
+    if (B->n_ci) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_ci, 1);
+  }
+}
+
+void launch(sfx_batch* B, int what, int mode) {
+  if (B->fp64) launch_phases<double>(B, what, mode);
+  else launch_phases<float>(B, what, mode);
+}
+
+int ensure_graph(sfx_batch* B, int steps) {
+  if (B->dirty) { int rc = rebuild(B); if (rc) return rc; }
+  if (B->exec && B->graph_steps == steps) return SFX_OK;
+  if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+  if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
+  HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < steps; ++s) launch(B, 7, 0);
+  HIPCHK(hipStreamEndCapture(B->stream, &B->graph));
+  HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  B->graph_steps = steps;
+  return SFX_OK;
+}
+
+sfx_batch* solo(sfx_handle* H) {
+  if (!H->solo) {
+    H->solo = new sfx_batch();
+    H->solo->device = H->device;
+    H->solo->stream = H->stream;
+    H->solo->own_stream = false;
+    H->solo->members.push_back(H);
+    H->solo->tend.push_back(0.0);
+    H->solo->dirty = true;
+  }
+  return H->solo;
+}
+
+}  // namespace
+
+extern "C" int sfx_compute_fluxes(sfx_handle* H, float dt, float* min_dt) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = normalise(H, s);
+  if (rc) return rc;
+  s.dt = dt; s.active = 1; s.min_dt_bits = 0;
+  std::memcpy(&s.min_dt_bits, &H->hI.c.dtmax, 4);
+  if ((rc = write_scalars(H, s))) return rc;
+  sfx_batch* B = solo(H);
+  if (B->dirty && (rc = rebuild(B))) return rc;
+  launch(B, 2, 0);
+  HIPCHK(hipGetLastError());
+  if ((rc = read_scalars(H, s))) return rc;
+  float m;
+  std::memcpy(&m, &s.min_dt_bits, 4);
+  s.min_dt = m;
+  if ((rc = write_scalars(H, s))) return rc;
+  if (min_dt) *min_dt = m;
+  return SFX_OK;
+}
+
+extern "C" int sfx_compute_water_levels(sfx_handle* H, double t, float dt) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = read_scalars(H, s);
+  if (rc) return rc;
+  s.t = t; s.dt = dt; s.active = 1;
+  if ((rc = write_scalars(H, s))) return rc;
+  sfx_batch* B = solo(H);
+  if (B->dirty && (rc = rebuild(B))) return rc;
+  launch(B, 4, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(H->stream));
+  return SFX_OK;
+}
+
+extern "C" int sfx_update_forcing(sfx_handle* H, double t, float dt) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = normalise(H, s);
+  if (rc) return rc;
+  s.t = t; s.dt = dt;
+  if ((rc = write_scalars(H, s))) return rc;
+  sfx_batch* B = solo(H);
+  if (B->dirty && (rc = rebuild(B))) return rc;
+  launch(B, 1, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(H->stream));
+  return SFX_OK;
+}
+
+extern "C" int sfx_update(sfx_handle* H, double dtrange, sfx_step_stats* st) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  auto t_start = std::chrono::steady_clock::now();
+  Scalars s;
+  int rc = read_scalars(H, s);
+  if (rc) return rc;
+  if (s.error_pending) { s.error = 1; s.error_pending = 0; s.t = (double)H->hI.c.t1 + 1.0; }
+  const int64_t nt0 = s.nt;
+  bool single = false;
+  if (dtrange < -999.0) s.tend = (double)H->hI.c.t1;
+  else if (dtrange <= 0.0) { s.tend = (double)H->hI.c.t1; single = true; }
+  else s.tend = s.t + dtrange;
+  if ((rc = write_scalars(H, s))) return rc;
+  sfx_batch* B = solo(H);
+  if (B->dirty && (rc = rebuild(B))) return rc;
+  if (single) {
+    launch(B, 7, 0);
+    HIPCHK(hipGetLastError());
+  } else {
+    const int K = 32;
+    if ((rc = ensure_graph(B, K))) return rc;
+    for (;;) {
+      HIPCHK(hipGraphLaunch(B->exec, B->stream));
+      if ((rc = read_scalars(H, s))) return rc;
+      if (!s.active || s.error || s.error_pending || !(s.t < s.tend)) break;
+    }
+  }
+  if ((rc = read_scalars(H, s))) return rc;
+  if (s.error_pending) { s.error = 1; s.error_pending = 0; s.t = (double)H->hI.c.t1 + 1.0; if ((rc = write_scalars(H, s))) return rc; }
+  float m;
+  std::memcpy(&m, &s.min_dt_bits, 4);
+  if (st) {
+    st->t = s.t; st->dt = s.dt; st->min_dt = m; st->nt = s.nt; st->error = s.error;
+    st->cell_updates = (double)(s.nt - nt0) * (double)H->hI.np;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return s.error ? SFX_UNSTABLE : SFX_OK;
+}
+
+// min_dt of the last sweep must feed the next step's dt: the forcing kernel reads
+// S->min_dt, so it is refreshed from the reduction bits at the end of every momentum
+// launch by the next forcing kernel (see k_forcing: dt = alpha * min_dt).
+
+extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  H->hI.count = on ? 1 : 0;
+  HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
+  Scalars s;
+  int rc = read_scalars(H, s);
+  if (rc) return rc;
+  for (int k = 0; k < 4; ++k) s.cnt[k] = 0;
+  if ((rc = write_scalars(H, s))) return rc;
+  if (H->solo) H->solo->dirty = true;
+  return SFX_OK;
+}
+
+extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[4]) {
+  if (!H || !counts) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = read_scalars(H, s);
+  if (rc) return rc;
+  for (int k = 0; k < 4; ++k) counts[k] = (int64_t)s.cnt[k];
+  return SFX_OK;
+}
+
+extern "C" int sfx_total_volume(sfx_handle* H, double* vol) {
+  if (!H || !vol) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, H->stream, H->dvol);
+  const int nb = std::min(1024, (H->hI.np + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_volume, dim3(nb), dim3(BLOCK), 0, H->stream, H->dI, H->dvol);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(H->stream));
+  HIPCHK(hipMemcpy(vol, H->dvol, sizeof(double), hipMemcpyDeviceToHost));
+  return SFX_OK;
+}
+
+// -------------------------------------------------------------------------- batch API
+extern "C" int sfx_batch_create(int32_t device, sfx_batch** out) {
+  if (!out) { set_error("null"); return SFX_ERR_ARG; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("no HIP device available"); return SFX_ERR_HIP; }
+  HIPCHK(hipSetDevice(device));
+  sfx_batch* B = new sfx_batch();
+  B->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+  *out = B;
+  return SFX_OK;
+}
+
+extern "C" int sfx_batch_destroy(sfx_batch* B) {
+  if (!B) return SFX_OK;
+  hipSetDevice(B->device);
+  if (B->stream) hipStreamSynchronize(B->stream);
+  if (B->exec) hipGraphExecDestroy(B->exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  if (B->dInsts) hipFree(B->dInsts);
+  if (B->dItems) hipFree(B->dItems);
+  if (B->dIds) hipFree(B->dIds);
+  if (B->own_stream && B->stream) hipStreamDestroy(B->stream);
+  delete B;
+  return SFX_OK;
+}
+
+extern "C" int sfx_batch_add(sfx_batch* B, sfx_handle* H, double t_end) {
+  if (!B || !H) { set_error("null"); return SFX_ERR_ARG; }
+  if (H->device != B->device) { set_error("member lives on another device"); return SFX_ERR_ARG; }
+  for (auto* m : B->members) if (m == H) { set_error("already a member"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(B->device));
+  Scalars s;
+  int rc = normalise(H, s);
+  if (rc) return rc;
+  s.tend = t_end;
+  if ((rc = write_scalars(H, s))) return rc;
+  B->members.push_back(H);
+  B->tend.push_back(t_end);
+  B->dirty = true;
+  return SFX_OK;
+}
+
+extern "C" int sfx_batch_remove(sfx_batch* B, sfx_handle* H) {
+  if (!B || !H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(B->device));
+  HIPCHK(hipStreamSynchronize(B->stream));
+  for (size_t i = 0; i < B->members.size(); ++i)
+    if (B->members[i] == H) {
+      B->members.erase(B->members.begin() + i);
+      B->tend.erase(B->tend.begin() + i);
+      B->dirty = true;
+      return SFX_OK;
+    }
+  set_error("not a member");
+  return SFX_ERR_ARG;
+}
+
+extern "C" int sfx_batch_advance(sfx_batch* B, int32_t nsteps, int32_t* n_active) {
+  if (!B) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(B->device));
+  int rc;
+  if (B->members.empty()) { if (n_active) *n_active = 0; return SFX_OK; }
+  const int K = std::max(1, std::min(nsteps, 64));
+  if ((rc = ensure_graph(B, K))) return rc;
+  int done = 0;
+  while (done < nsteps) {
+    HIPCHK(hipGraphLaunch(B->exec, B->stream));
+    done += K;
+  }
+  HIPCHK(hipStreamSynchronize(B->stream));
+  int act = 0;
+  for (auto* H : B->members) {
+    Scalars s;
+    HIPCHK(hipMemcpy(&s, H->dS, sizeof(Scalars), hipMemcpyDeviceToHost));
+    if (!s.error && !s.error_pending && s.t < s.tend) ++act;
+  }
+  if (n_active) *n_active = act;
+  return SFX_OK;
+}
+
+extern "C" int sfx_batch_done(sfx_batch* B, sfx_handle* H, int32_t* done) {
+  if (!B || !H || !done) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(B->device));
+  HIPCHK(hipStreamSynchronize(B->stream));
+  Scalars s;
+  HIPCHK(hipMemcpy(&s, H->dS, sizeof(Scalars), hipMemcpyDeviceToHost));
+  *done = (s.error || s.error_pending || !(s.t < s.tend)) ? 1 : 0;
+  return SFX_OK;
+}
+
+// -------------------------------------------------------------------------- autotuner
+extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[4]) {
+  if (!H || !out) { set_error("null"); return SFX_ERR_ARG; }
+  out[0] = H->hI.tn; out[1] = H->hI.tm; out[2] = BLOCK; out[3] = H->hI.dense;
+  return SFX_OK;
+}
+
+extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  if (!H->hI.dense) return SFX_OK;   // indexed sweeps have one shape (a uv-point per lane)
+  const int cand[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {16, 16}};
+  Scalars s0;
+  int rc = normalise(H, s0);
+  if (rc) return rc;
+  // momentum and continuity write state: time on a snapshot and restore it afterwards
+  std::vector<std::string> fields = {"zs", "q", "uv", "kfuv", "zs0", "zsderv", "zsmax"};
+  if (H->hI.vol) fields.push_back("z_volume");
+  std::map<std::string, std::vector<char>> snap;
+  for (auto& f : fields) {
+    size_t n = 0;
+    if (f == "zs" || f == "z_volume") n = (size_t)H->hI.np * 8;
+    else if (f == "q" || f == "uv") n = H->ref2slot.size() * (H->fp64 ? 8 : 4);
+    else if (f == "kfuv") n = H->sz.npuv;
+    else n = (size_t)H->hI.np * (H->fp64 ? 8 : 4);
+    snap[f].resize(n);
+    if ((rc = sfx_get_state(H, f.c_str(), snap[f].data(), n))) return rc;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bi = 0;
+  for (int ci = 0; ci < (int)(sizeof(cand) / sizeof(cand[0])); ++ci) {
+    H->hI.tn = cand[ci][0]; H->hI.tm = cand[ci][1];
+    H->hI.ntn = (H->hI.nmax + H->hI.tn - 1) / H->hI.tn;
+    HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
+    sfx_batch* B = solo(H);
+    B->dirty = true;
+    if ((rc = rebuild(B))) return rc;
+    Scalars s = s0;
+    s.active = 1; s.dt = std::max(s0.dt, 1.0e-3f);
+    if ((rc = write_scalars(H, s))) return rc;
+    launch(B, 6, 0);   // warm-up
+    HIPCHK(hipStreamSynchronize(H->stream));
+    float tot = 0.f;
+    for (int t = 0; t < std::max(1, trials); ++t) {
+      HIPCHK(hipEventRecord(e0, H->stream));
+      launch(B, 6, 0);
+      HIPCHK(hipEventRecord(e1, H->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      tot += ms;
+    }
+    if (tot < best) { best = tot; bi = ci; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  H->hI.tn = cand[bi][0]; H->hI.tm = cand[bi][1];
+  H->hI.ntn = (H->hI.nmax + H->hI.tn - 1) / H->hI.tn;
+  HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
+  H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm;
+  if ((rc = write_scalars(H, s0))) return rc;
+  for (auto& f : fields)
+    if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;
+  if (H->solo) H->solo->dirty = true;
+  return SFX_OK;
+}

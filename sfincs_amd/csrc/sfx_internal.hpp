@@ -88,6 +88,7 @@ struct sfx_model {
   int32_t ntprcp = 0, ntwnd = 0;
   std::vector<float> tprcpt, tprcpv, twnd, wndmag, wnddir;
   std::vector<float> cdval;   // 1000 entries
+  std::vector<float> x73;     // h**(7/3) table, 1000 x 10 column-major
 
   // helpers
   int32_t dummy_uv() const { return npuv + ncuv + 1; }

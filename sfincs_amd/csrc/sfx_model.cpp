@@ -805,6 +805,18 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
     }
   }
 
+  // fill_h73_tables (sfincs_data.f90:858-881): hh = 0.001*j*10**(k-6), x73 = hh**2*hh**(1/3)
+  M->x73.assign(10000, 0.f);
+  for (int k = 1; k <= 10; ++k) {
+    float p10 = 1.0f;
+    if (k >= 6) { for (int i = 0; i < k - 6; ++i) p10 *= 10.0f; }
+    else { float d = 1.0f; for (int i = 0; i < 6 - k; ++i) d *= 10.0f; p10 = 1.0f / d; }
+    for (int j = 1; j <= 1000; ++j) {
+      const float hh = 0.001f * (float)j * p10;
+      M->x73[(k - 1) * 1000 + (j - 1)] = hh * hh * std::pow(hh, 1.0f / 3.0f);
+    }
+  }
+
   // set_advection_mask (sfincs_domain.f90:2114-2180)
   for (int32_t j = 0; j < ncuv; ++j) {
     const int a = M->kcuv[M->cuv_index_uv1[j] - 1], b = M->kcuv[M->cuv_index_uv2[j] - 1];
@@ -852,7 +864,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(uv_havg) T(uv_nrep) T(uv_pwet) T(uv_havg_zmax) T(uv_nrep_zmax)
   T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv)
   T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
-  T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval)
+  T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval) T(x73)
   T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)
 #undef T
   return false;

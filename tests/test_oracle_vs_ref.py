@@ -1,0 +1,75 @@
+"""The oracle (oracle/sfx_oracle.c) against the reference's own compute_fluxes and
+compute_water_levels (oracle/_ref, compiled from the reference sources).
+
+Both run on the same tables and the same forcing; after every step every state array
+must be bit-identical (float equality). This pins oracle_compute_fluxes and
+oracle_compute_water_levels for each physics switch exercised below."""
+import numpy as np
+import pytest
+
+import oracle as O
+from sfincs_amd import workloads as W
+
+pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (needs /root/reference)")
+
+FIELDS = ("zs", "q", "uv", "kfuv", "zsmax")
+SG_FIELDS = ("z_volume", "zsderv", "zs0")
+
+
+def run_pair(sc, steps):
+    m = sc.build_model()
+    a = O.OracleRun(m).configure(sc.params, m)
+    b = O.OracleRun(m).configure(sc.params, m)
+    r = O.RefRun(b)
+    fields = FIELDS + (SG_FIELDS if a.vals["subgrid"] else ())
+    for i in range(steps):
+        a.update(1e30, 1)
+        r.step()
+        for n in fields:
+            x, y = a.state(n), b.state(n)
+            assert np.array_equal(x, y), f"step {i} field {n}: max diff {np.max(np.abs(x.astype(float) - y)):g}"
+        assert a.get("min_dt") == b.get("min_dt")
+        assert a.get("t") == b.get("t")
+    return a
+
+
+VARIANTS = {
+    "default": {},
+    "original_advection": dict(advection_scheme=0),
+    "viscosity": dict(viscosity=1),
+    "coriolis": dict(latitude=52.0),
+    "theta": dict(theta=0.9),
+    "slopelim": dict(slopelim=0.01),
+    "no_friction2d": dict(friction2d=0),
+    "h73table": dict(h73table=1),
+    "no_advection": dict(advection=0),
+    "huvmin": dict(huvmin=0.1),
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_regular_variants_bit_exact(variant):
+    sc = W.regular_case(40, 52, dx=50.0, seed=21, t_end=3600.0, tide_amp=1.5, **VARIANTS[variant])
+    o = run_pair(sc, 120)
+    assert o.get("t") > 0
+
+
+@pytest.mark.parametrize("variant", ["default", "original_advection", "viscosity", "coriolis", "theta",
+                                     "no_friction2d"])
+def test_subgrid_variants_bit_exact(variant):
+    sc = W.subgrid_case(32, 44, dx=25.0, seed=22, t_end=3600.0, nsrc=3, rain_mmh=40.0, **VARIANTS[variant])
+    o = run_pair(sc, 120)
+    assert o.get("t") > 0
+
+
+def test_wind_bit_exact():
+    t = np.array([0.0, 7200.0], np.float32)
+    sc = W.regular_case(36, 48, dx=50.0, seed=23, t_end=3600.0,
+                        wind=(t, np.array([20.0, 24.0], np.float32), np.array([225.0, 240.0], np.float32)))
+    run_pair(sc, 100)
+
+
+def test_wetting_front_long_run_bit_exact():
+    """A longer run through a rising tide: wetting/drying fronts cross many uv-points."""
+    sc = W.regular_case(24, 40, dx=50.0, seed=24, t_end=7200.0, tide_amp=3.0, rain_mmh=60.0)
+    run_pair(sc, 400)

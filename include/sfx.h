@@ -197,6 +197,13 @@ int  sfx_batch_done(sfx_batch* b, sfx_handle* h, int32_t* done);
 /* HBM footprint of a model once on the device, for admission control. */
 int  sfx_model_footprint(const sfx_model* m, size_t* bytes);
 
+/* Profiling: nsteps ordinary (non-graph) steps with HIP events around each phase on the
+ * launch stream; ms[0..2] = mean forcing, momentum, continuity launch durations. */
+int  sfx_profile_steps(sfx_handle* h, int32_t nsteps, double ms[3]);
+/* FP64 twin: the same kernels instantiated with double arithmetic and state (q, uv, zs0,
+ * zsderv, zsmax), used to state the FP32 path's rounding error. */
+int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle** out);
+
 /* Autotuner: times candidate block/tile shapes on this handle's mesh and keeps the best. */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
 int  sfx_get_tuning(sfx_handle* h, int32_t out[4]);

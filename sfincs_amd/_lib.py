@@ -99,6 +99,8 @@ def _sig(lib):
         "sfx_autotune": (I, [P, i32]),
         "sfx_get_tuning": (I, [P, C.POINTER(i32)]),
         "sfx_last_error": (I, [C.c_char_p, sz]),
+        "sfx_profile_steps": (I, [P, i32, C.POINTER(d)]),
+        "sfx_create_ex": (I, [P, i32, i32, C.POINTER(P)]),
     }
     for name, (res, args) in table.items():
         fn = getattr(lib, name, None)
@@ -282,9 +284,9 @@ STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv":
 class Instance:
     """A model resident in HBM (sfx_handle). One instance per scenario."""
 
-    def __init__(self, model, device=0):
+    def __init__(self, model, device=0, fp64=False):
         out = C.c_void_p()
-        check(lib().sfx_create(model.h, device, C.byref(out)), "sfx_create")
+        check(lib().sfx_create_ex(model.h, device, 1 if fp64 else 0, C.byref(out)), "sfx_create")
         self.h = out
         self.sizes = model.sizes()
 
@@ -348,6 +350,11 @@ class Instance:
         a = (C.c_int32 * 4)()
         check(lib().sfx_get_tuning(self.h, a), "get_tuning")
         return list(a)
+
+    def profile(self, nsteps):
+        ms = (C.c_double * 3)()
+        check(lib().sfx_profile_steps(self.h, nsteps, ms), "profile_steps")
+        return list(ms)
 
     def close(self):
         if self.h:

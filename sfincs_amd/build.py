@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsfx.so")
 
-SOURCES = ["sfx_model.cpp", "sfx_device.hip"]
+SOURCES = ["sfx_model.cpp", "sfx_device.hip", "sfx_libm.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -39,7 +39,20 @@ def build(verbose=True):
     cmd = ["hipcc"] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs + ["-lpthread"]
     _run(cmd)
     os.replace(tmp, LIB)
+    tables()
     return LIB
+
+
+def tables():
+    """Enumerate (or load and re-verify) the C library's powf tables next to the library."""
+    import ctypes
+    lib = ctypes.CDLL(LIB)
+    counts = (ctypes.c_int64 * 2)()
+    lib.sfx_libm_tables.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
+    rc = lib.sfx_libm_tables(None, counts)
+    if rc != 0:
+        raise RuntimeError("powf table generation failed")
+    print(f"powf tables: {counts[0]} + {counts[1]} entries", flush=True)
 
 
 def build_fortran_host():

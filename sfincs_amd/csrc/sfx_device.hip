@@ -29,6 +29,10 @@
 using sfx::Consts;
 using sfx::set_error;
 
+namespace sfx {
+bool pow_tables_device(int device, PowTables& out, std::string& err);
+}
+
 namespace {
 
 #define HIPCHK(x)                                                                          \
@@ -102,6 +106,7 @@ struct DevInst {
   const int32_t* src_cell; const float* tsrc; const float* qsrc; float* qtsrc;
   const float* tprcpt; const float* tprcpv; const float* twnd; const float* wndmag; const float* wnddir;
   const float* cdval; const float* x73;
+  sfx::PowTables pw;
 };
 
 struct Item { int32_t inst, tile; };
@@ -354,7 +359,7 @@ __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx:
     tb.gn2uv = I.gn2uv[slot];
   }
   const sfx::UvOut<R> o = sfx::uv_update<R, SUBGRID>(in, tb, I.c, zsu, SUBGRID ? I.uv_lev + (size_t)slot * I.c.nl : nullptr,
-                                                     I.x73, dt);
+                                                     I.x73, I.pw, dt);
   q1[slot] = o.q; uv1[slot] = o.uv; I.kfuv[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
   br = o.branch;
@@ -937,6 +942,8 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.twnd = H->upload(M->twnd.data(), M->twnd.size()); I.wndmag = H->upload(M->wndmag.data(), M->wndmag.size());
     I.wnddir = H->upload(M->wnddir.data(), M->wnddir.size()); I.cdval = H->upload(M->cdval.data(), M->cdval.size());
     I.x73 = H->upload(M->x73.data(), M->x73.size());
+    std::string perr;
+    if (!sfx::pow_tables_device(device, I.pw, perr)) { set_error(perr); delete H; return SFX_ERR_STATE; }
   }
   rc = H->fp64 ? upload_state<double>(H, M) : upload_state<float>(H, M);
   // scalars
@@ -1465,11 +1472,9 @@ extern "C" int sfx_batch_advance(sfx_batch* B, int32_t nsteps, int32_t* n_active
   if (B->members.empty()) { if (n_active) *n_active = 0; return SFX_OK; }
   const int K = std::max(1, std::min(nsteps, 64));
   if ((rc = ensure_graph(B, K))) return rc;
-  int done = 0;
-  while (done < nsteps) {
-    HIPCHK(hipGraphLaunch(B->exec, B->stream));
-    done += K;
-  }
+  for (int r = 0; r < nsteps / K; ++r) HIPCHK(hipGraphLaunch(B->exec, B->stream));
+  for (int r = 0; r < nsteps % K; ++r) launch(B, 7, 0);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(B->stream));
   int act = 0;
   for (auto* H : B->members) {
@@ -1558,5 +1563,42 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;
   if (H->solo) H->solo->dirty = true;
+  return SFX_OK;
+}
+
+// -------------------------------------------------------------------------- profiling
+// Runs nsteps ordinary (non-graph) steps of one handle with HIP events around each phase
+// on the stream the kernels are launched on; ms[0..2] receive the mean duration of the
+// forcing, momentum (+combined points) and continuity launches.
+extern "C" int sfx_profile_steps(sfx_handle* H, int32_t nsteps, double ms[3]) {
+  if (!H || !ms || nsteps < 1) { set_error("sfx_profile_steps: bad args"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  sfx_batch* B = solo(H);
+  int rc;
+  if (B->dirty && (rc = rebuild(B))) return rc;
+  Scalars s;
+  if ((rc = read_scalars(H, s))) return rc;
+  s.tend = 1.0e30;
+  if ((rc = write_scalars(H, s))) return rc;
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  double acc[3] = {0, 0, 0};
+  for (int i = 0; i < nsteps; ++i) {
+    HIPCHK(hipEventRecord(ev[0], B->stream));
+    launch(B, 1, 0);
+    HIPCHK(hipEventRecord(ev[1], B->stream));
+    launch(B, 2, 0);
+    HIPCHK(hipEventRecord(ev[2], B->stream));
+    launch(B, 4, 0);
+    HIPCHK(hipEventRecord(ev[3], B->stream));
+    HIPCHK(hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; ++k) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  for (int k = 0; k < 3; ++k) ms[k] = acc[k] / nsteps;
   return SFX_OK;
 }

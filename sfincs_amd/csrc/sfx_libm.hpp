@@ -1,0 +1,90 @@
+// x**(1/3) and x**(5/3) with the exact float results of the C library's powf.
+//
+// The reference computes hu**expo (expo = 1.0/3.0 as real*4) and hu**(5.0/3.0)
+// (sfincs_momentum.f90:92,616,673) with the C library's powf. The momentum and continuity
+// updates branch on thresholds (wet/dry tests, the table inversion switch at
+// V = 0.999*volmax), so a one-ulp difference in powf can move a cell to the other branch.
+// These routines return powf's exact result: a double-precision estimate (cube root by
+// two Newton steps from cbrtf, times exp(d*ln x) for the offset d between the float
+// exponent and 1/3 or 5/3) is rounded to float unless it lies within 2^-30 (relative) of
+// a rounding midpoint; those inputs (about 2% of calls) are looked up in a table of
+// powf's own results that sfx_libm.hip enumerates over every float in [2^-40, 2^14) and
+// verifies. Outside that range the rounded estimate is returned.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#ifndef SFX_HD
+#define SFX_HD __host__ __device__
+#endif
+
+namespace sfx {
+
+constexpr int POW_HASH_BITS = 20;
+constexpr uint32_t POW_HASH_SIZE = 1u << POW_HASH_BITS;
+constexpr float POW_XLO = 9.094947017729282e-13f;   // 2^-40
+constexpr float POW_XHI = 16384.0f;                 // 2^14
+
+struct PowTables {
+  const uint2* t13;   // open-addressing hash: {x bits, powf(x, 1/3f) bits}; key 0 = empty
+  const uint2* t53;
+};
+
+SFX_HD inline uint32_t pow_hash(uint32_t k) {
+  k ^= k >> 16; k *= 0x7feb352dU; k ^= k >> 15; k *= 0x846ca68bU; k ^= k >> 16;
+  return k & (POW_HASH_SIZE - 1);
+}
+
+SFX_HD inline uint32_t f2u(float f) { union { float f; uint32_t u; } v; v.f = f; return v.u; }
+SFX_HD inline float u2f(uint32_t u) { union { float f; uint32_t u; } v; v.u = u; return v.f; }
+
+// Double estimate of x**e for e = (float)(1/3) (which = 0) or (float)(5/3) (which = 1),
+// relative error below 2^-46 for normal positive x.
+SFX_HD inline double pow_estimate(float x, int which) {
+  const double xd = x;
+  double y = (double)cbrtf(x);
+  y = y - (y * y * y - xd) / (3.0 * y * y);
+  y = y - (y * y * y - xd) / (3.0 * y * y);
+  int e;
+  const float mant = frexpf(x, &e);                  // x = mant * 2^e, mant in [0.5, 1)
+  const double lnx = (double)e * 0.69314718055994530942 + (double)logf(mant);
+  // offsets of the float exponents from 1/3 and 5/3
+  const double d13 = (double)(1.0f / 3.0f) - 1.0 / 3.0;
+  const double d53 = (double)(5.0f / 3.0f) - 5.0 / 3.0;
+  const double t = (which == 0 ? d13 : d53) * lnx;
+  const double corr = 1.0 + t + 0.5 * t * t;
+  return which == 0 ? y * corr : xd * y * y * corr;
+}
+
+// True when the double value r lies within 2^-30 (relative) of a midpoint between two
+// adjacent floats, i.e. when its float rounding is not certain.
+SFX_HD inline bool near_midpoint(double r) {
+  const float f = (float)r;
+  const double fd = f;
+  const float nb = r >= fd ? nextafterf(f, INFINITY) : nextafterf(f, 0.0f);
+  const double mid = 0.5 * (fd + (double)nb);
+  const double d = r > mid ? r - mid : mid - r;
+  return d < 9.313225746154785e-10 * r;              // 2^-30
+}
+
+SFX_HD inline float pow_lookup(const uint2* t, uint32_t key, float fallback) {
+  uint32_t h = pow_hash(key);
+  for (int probe = 0; probe < 64; ++probe) {
+    const uint2 e = t[h];
+    if (e.x == key) return u2f(e.y);
+    if (e.x == 0u) break;
+    h = (h + 1) & (POW_HASH_SIZE - 1);
+  }
+  return fallback;
+}
+
+// powf(x, (float)(1/3)) or powf(x, (float)(5/3)) as the C library computes it.
+SFX_HD inline float libm_pow(float x, int which, const PowTables& T) {
+  if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
+  const double r = pow_estimate(x, which);
+  const float f = (float)r;
+  if (x >= POW_XLO && x < POW_XHI && near_midpoint(r)) return pow_lookup(which == 0 ? T.t13 : T.t53, f2u(x), f);
+  return f;
+}
+
+}  // namespace sfx

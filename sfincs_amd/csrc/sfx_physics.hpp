@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "sfx_libm.hpp"
+
 namespace sfx {
 
 enum : uint32_t {
@@ -34,6 +36,16 @@ template <typename T> __device__ __forceinline__ T vmin(T a, T b) { return a < b
 template <typename R> __device__ __forceinline__ R rpow(R x, R y);
 template <> __device__ __forceinline__ float rpow<float>(float x, float y) { return powf(x, y); }
 template <> __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
+// x**(float)(1/3) and x**(float)(5/3): the C library's exact powf results in FP32,
+// ordinary double pow in the FP64 twin.
+template <typename R> __device__ __forceinline__ R pow_third(R x, int which, const PowTables& T);
+template <> __device__ __forceinline__ float pow_third<float>(float x, int which, const PowTables& T) {
+  return libm_pow(x, which, T);
+}
+template <> __device__ __forceinline__ double pow_third<double>(double x, int which, const PowTables& T) {
+  (void)T;
+  return pow(x, which == 0 ? (double)(1.0f / 3.0f) : (double)(5.0f / 3.0f));
+}
 template <typename R> __device__ __forceinline__ R rsqrt_(R x);
 template <> __device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }
 template <> __device__ __forceinline__ double rsqrt_<double>(double x) { return sqrt(x); }
@@ -95,7 +107,7 @@ __device__ __forceinline__ R power7over3(const float* __restrict__ x73, R hu) {
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables& tb, const Consts& c, R zsu,
                                               const float4* __restrict__ lev_row, const float* __restrict__ x73,
-                                              R dt) {
+                                              const PowTables& pw, R dt) {
   const uint32_t F = c.flags;
   UvOut<R> o;
   const int iref = in.iref - 1;
@@ -200,7 +212,7 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
   }
   R qfr;
   if (in.kfuv == 0) {
-    qfr = rsqrt_<R>(rabs<R>(dzdx) / (vmax<R>(gnavg2, R(1.0e-5f)) / R(10))) * rpow<R>(hu, R(5.0f / 3.0f));
+    qfr = rsqrt_<R>(rabs<R>(dzdx) / (vmax<R>(gnavg2, R(1.0e-5f)) / R(10))) * pow_third<R>(hu, 1, pw);
   } else if (F & F_FRICTION2D) {
     qfr = rsqrt_<R>(qx_nm * qx_nm + (hu * vu) * (hu * vu));
   } else {
@@ -212,7 +224,7 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
       if (rabs<R>(in.qx_nmd) > R(1.0e-6f) && rabs<R>(in.qx_nmu) > R(1.0e-6f))
         qsm = (R)c.theta * qx_nm + R(0.5f) * (R(1) - (R)c.theta) * (in.qx_nmu + in.qx_nmd);
   }
-  const R hu73 = (F & F_H73) ? power7over3<R>(x73, vmax<R>(hu, R(1.0e-6f))) : hu * hu * rpow<R>(hu, R(1.0f / 3.0f));
+  const R hu73 = (F & F_H73) ? power7over3<R>(x73, vmax<R>(hu, R(1.0e-6f))) : hu * hu * pow_third<R>(hu, 0, pw);
   R qn = (qsm + frc * dt) / (R(1) + gnavg2 * dt * qfr / hu73);
   if (SUBGRID) {
     if (F & F_WIGGLE) {

@@ -1,0 +1,118 @@
+"""Write a scenario as a SFINCS input deck in the reference's file formats.
+
+sfincs.inp (key = value), inputformat = bin: indexfile (int32 np, then the column-major
+indices (m-1)*nmax+n of the active cells), mskfile (int8 per index), depfile (float32 per
+index) -- sfincs_domain.f90:244-260, 1670-1676; quadtree meshes as the legacy binary
+qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile; ASCII bnd/bzs, src/dis,
+prc and wnd files (sfincs_boundaries.f90:8-140, sfincs_discharges.f90:36-126,
+sfincs_meteo.f90:247-312). Values are written with 9 significant digits so that
+float32 round-trips exactly.
+"""
+import datetime
+import os
+
+import numpy as np
+
+_DEFAULTS_SKIP = {"t0", "t1"}
+
+
+def _fmt(v):
+    return f"{np.float32(v):.9g}"
+
+
+def _date(seconds):
+    d = datetime.datetime(2000, 1, 1) + datetime.timedelta(seconds=float(seconds))
+    return d.strftime("%Y%m%d %H%M%S")
+
+
+def write(sc, folder):
+    os.makedirs(folder, exist_ok=True)
+    p = dict(sc.params)
+    lines = []
+    t0 = float(p.get("t0", 0.0))
+    t1 = float(p.get("t1", sc.t_end))
+    lines += [f"tref = {_date(0)}", f"tstart = {_date(t0)}", f"tstop = {_date(t1)}"]
+    for k, v in p.items():
+        if k in _DEFAULTS_SKIP:
+            continue
+        if k == "zsini":
+            lines.append(f"zsini = {_fmt(v)}")
+        elif isinstance(v, (int, np.integer)) and k in ("viscosity", "coriolis", "friction2d", "advection_mask",
+                                                         "wiggle_suppression", "h73table"):
+            lines.append(f"{k} = {int(v)}")
+        elif k == "advection_scheme":
+            lines.append(f"advection_scheme = {'original' if int(v) == 0 else 'upw1'}")
+        elif k == "advection":
+            lines.append(f"advection = {int(v)}")
+        else:
+            lines.append(f"{k} = {_fmt(v)}")
+    if sc.kind == "regular":
+        lines += [f"nmax = {sc.nmax}", f"mmax = {sc.mmax}", f"dx = {_fmt(sc.dx)}", f"dy = {_fmt(sc.dy)}",
+                  "x0 = 0", "y0 = 0", "rotation = 0", "inputformat = bin", "indexfile = sfincs.ind",
+                  "mskfile = sfincs.msk", "depfile = sfincs.dep"]
+        msk = sc.msk.reshape(sc.mmax, sc.nmax)          # [m][n]
+        mm, nn = np.nonzero(msk > 0)
+        order = np.lexsort((nn, mm))
+        mm, nn = mm[order], nn[order]
+        ind = (mm * sc.nmax + nn + 1).astype(np.int32)
+        with open(os.path.join(folder, "sfincs.ind"), "wb") as f:
+            f.write(np.int32(len(ind)).tobytes())
+            f.write(ind.tobytes())
+        with open(os.path.join(folder, "sfincs.msk"), "wb") as f:
+            f.write(msk[mm, nn].astype(np.int8).tobytes())
+        with open(os.path.join(folder, "sfincs.dep"), "wb") as f:
+            f.write(sc.zb_grid[mm, nn].astype(np.float32).tobytes())
+    else:
+        qt = sc.qt
+        lines += ["qtrfile = sfincs.qtr", "mskfile = sfincs.msk"]
+        with open(os.path.join(folder, "sfincs.qtr"), "wb") as f:
+            f.write(np.int8(1).tobytes())
+            f.write(np.int32(0).tobytes())
+            f.write(np.int32(len(qt["n"])).tobytes())
+            f.write(np.int8(qt["nr_levels"]).tobytes())
+            for k in ("x0", "y0", "dx", "dy"):
+                f.write(np.float32(qt[k]).tobytes())
+            f.write(np.float32(qt.get("rotation", 0.0)).tobytes())
+            f.write(qt["level"].astype(np.int8).tobytes())
+            f.write(qt["n"].astype(np.int32).tobytes())
+            f.write(qt["m"].astype(np.int32).tobytes())
+            for side in ("nu", "mu", "nd", "md"):
+                f.write(qt[side].astype(np.int8).tobytes())
+                f.write(qt[side + "1"].astype(np.int32).tobytes())
+                f.write(qt[side + "2"].astype(np.int32).tobytes())
+            f.write(sc.zb_qt.astype(np.float32).tobytes())
+        with open(os.path.join(folder, "sfincs.msk"), "wb") as f:
+            f.write(qt["mask"].astype(np.int8).tobytes())
+    if sc.bnd is not None:
+        x, y, t, zs = sc.bnd
+        lines += ["bndfile = sfincs.bnd", "bzsfile = sfincs.bzs"]
+        with open(os.path.join(folder, "sfincs.bnd"), "w") as f:
+            for a, b in zip(x, y):
+                f.write(f"{_fmt(a)} {_fmt(b)}\n")
+        with open(os.path.join(folder, "sfincs.bzs"), "w") as f:
+            for it in range(len(t)):
+                f.write(_fmt(t[it]) + " " + " ".join(_fmt(v) for v in zs[it]) + "\n")
+    if sc.src is not None:
+        x, y, t, q = sc.src
+        lines += ["srcfile = sfincs.src", "disfile = sfincs.dis"]
+        with open(os.path.join(folder, "sfincs.src"), "w") as f:
+            for a, b in zip(x, y):
+                f.write(f"{_fmt(a)} {_fmt(b)}\n")
+        with open(os.path.join(folder, "sfincs.dis"), "w") as f:
+            for it in range(len(t)):
+                f.write(_fmt(t[it]) + " " + " ".join(_fmt(v) for v in q[it]) + "\n")
+    if sc.rain is not None:
+        t, v = sc.rain
+        lines.append("prcfile = sfincs.prc")
+        with open(os.path.join(folder, "sfincs.prc"), "w") as f:
+            for a, b in zip(t, v):
+                f.write(f"{_fmt(a)} {_fmt(b)}\n")
+    if sc.wind is not None:
+        t, mag, d = sc.wind
+        lines.append("wndfile = sfincs.wnd")
+        with open(os.path.join(folder, "sfincs.wnd"), "w") as f:
+            for a, b, c in zip(t, mag, d):
+                f.write(f"{_fmt(a)} {_fmt(b)} {_fmt(c)}\n")
+    with open(os.path.join(folder, "sfincs.inp"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return folder

@@ -1,0 +1,684 @@
+! Fortran host of the MI355X-native SFINCS step.
+!
+! Keeps the reference's model-driver entry points (sfincs_lib.f90:85-819:
+! sfincs_initialize / sfincs_update(dtrange) / sfincs_finalize, public t and dt) and the
+! BMI C names (sfincs_bmi.f90:56-387), and calls the HIP kernels through the C ABI of
+! include/sfx.h with iso_c_binding. The host reads the reference's input deck:
+!   sfincs.inp (key = value, reference defaults), inputformat bin (indexfile/mskfile/
+!   depfile, stream binary) or asc (mskfile/depfile, nmax rows of mmax values), or a
+!   legacy binary qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile;
+!   bndfile/bzsfile, srcfile/disfile, prcfile, wndfile (ASCII time series).
+module sfx_capi
+   use iso_c_binding
+   implicit none
+   type, bind(C) :: sfx_params
+      real(c_float) :: t0, t1, tspinup, alpha, theta, hmin_cfl, manning, manning_land, manning_sea
+      real(c_float) :: rgh_lev_land, zsini, dtmax, huthresh, huvmin, rhoa, rhow, advlim, slopelim
+      real(c_float) :: btfilter, btrelax, wiggle_factor, wiggle_threshold, uvlim, uvmax, nuvisc
+      real(c_float) :: nuviscfac, latitude, g, pi
+      integer(c_int32_t) :: advection, advection_scheme, viscosity, coriolis, friction2d
+      integer(c_int32_t) :: advection_mask, wiggle_suppression, store_zsmax, bndtype, h73table
+   end type
+   type, bind(C) :: sfx_quadtree_input
+      integer(c_int32_t) :: nr_points, nr_levels
+      real(c_float) :: x0, y0, dx, dy, rotation_deg
+      type(c_ptr) :: level, md, mu, nd, nu, mask
+      type(c_ptr) :: n, m, md1, md2, mu1, mu2, nd1, nd2, nu1, nu2
+   end type
+   type, bind(C) :: sfx_step_stats
+      real(c_double) :: t
+      real(c_float) :: dt, min_dt
+      integer(c_int64_t) :: nt
+      integer(c_int32_t) :: error
+      real(c_double) :: cell_updates, seconds
+   end type
+   interface
+      subroutine sfx_params_default(p) bind(C, name='sfx_params_default')
+         import :: sfx_params
+         type(sfx_params), intent(out) :: p
+      end subroutine
+      integer(c_int) function sfx_model_create_regular(p, nmax, mmax, x0, y0, dx, dy, rot, msk, out) &
+            bind(C, name='sfx_model_create_regular')
+         import :: sfx_params, c_int, c_int32_t, c_float, c_ptr
+         type(sfx_params), intent(in) :: p
+         integer(c_int32_t), value :: nmax, mmax
+         real(c_float), value :: x0, y0, dx, dy, rot
+         type(c_ptr), value :: msk
+         type(c_ptr), intent(out) :: out
+      end function
+      integer(c_int) function sfx_model_create_quadtree(p, qt, out) bind(C, name='sfx_model_create_quadtree')
+         import :: sfx_params, sfx_quadtree_input, c_int, c_ptr
+         type(sfx_params), intent(in) :: p
+         type(sfx_quadtree_input), intent(in) :: qt
+         type(c_ptr), intent(out) :: out
+      end function
+      integer(c_int) function sfx_model_table_bytes(m, name, nbytes) bind(C, name='sfx_model_table_bytes')
+         import :: c_int, c_ptr, c_char, c_size_t
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: name(*)
+         integer(c_size_t), intent(out) :: nbytes
+      end function
+      integer(c_int) function sfx_model_get_table(m, name, dst, nbytes) bind(C, name='sfx_model_get_table')
+         import :: c_int, c_ptr, c_char, c_size_t
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: name(*)
+         type(c_ptr), value :: dst
+         integer(c_size_t), value :: nbytes
+      end function
+      integer(c_int) function sfx_model_set_bathymetry(m, zb) bind(C, name='sfx_model_set_bathymetry')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m, zb
+      end function
+      integer(c_int) function sfx_model_set_boundary(m, nbnd, x, y, nt, t, zs) bind(C, name='sfx_model_set_boundary')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: nbnd, nt
+         type(c_ptr), value :: x, y, t, zs
+      end function
+      integer(c_int) function sfx_model_set_sources(m, nsrc, x, y, nt, t, q) bind(C, name='sfx_model_set_sources')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: nsrc, nt
+         type(c_ptr), value :: x, y, t, q
+      end function
+      integer(c_int) function sfx_model_set_rain(m, nt, t, v) bind(C, name='sfx_model_set_rain')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: nt
+         type(c_ptr), value :: t, v
+      end function
+      integer(c_int) function sfx_model_set_wind(m, nt, t, mag, dir) bind(C, name='sfx_model_set_wind')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: nt
+         type(c_ptr), value :: t, mag, dir
+      end function
+      integer(c_int) function sfx_model_initialize(m) bind(C, name='sfx_model_initialize')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m
+      end function
+      subroutine sfx_model_destroy(m) bind(C, name='sfx_model_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: m
+      end subroutine
+      integer(c_int) function sfx_create(m, device, out) bind(C, name='sfx_create')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: device
+         type(c_ptr), intent(out) :: out
+      end function
+      integer(c_int) function sfx_destroy(h) bind(C, name='sfx_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
+      integer(c_int) function sfx_update(h, dtrange, st) bind(C, name='sfx_update')
+         import :: c_int, c_ptr, c_double, sfx_step_stats
+         type(c_ptr), value :: h
+         real(c_double), value :: dtrange
+         type(sfx_step_stats), intent(out) :: st
+      end function
+      integer(c_int) function sfx_get_state(h, name, dst, nbytes) bind(C, name='sfx_get_state')
+         import :: c_int, c_ptr, c_char, c_size_t
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: name(*)
+         type(c_ptr), value :: dst
+         integer(c_size_t), value :: nbytes
+      end function
+      integer(c_int) function sfx_get_time(h, t, dt, nt) bind(C, name='sfx_get_time')
+         import :: c_int, c_ptr, c_double, c_float, c_int64_t
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: t
+         real(c_float), intent(out) :: dt
+         integer(c_int64_t), intent(out) :: nt
+      end function
+      integer(c_int) function sfx_last_error(buf, n) bind(C, name='sfx_last_error')
+         import :: c_int, c_char, c_size_t
+         character(kind=c_char), intent(out) :: buf(*)
+         integer(c_size_t), value :: n
+      end function
+   end interface
+end module sfx_capi
+
+
+module sfincs_lib
+   use iso_c_binding
+   use sfx_capi
+   implicit none
+   private
+   public :: sfincs_initialize, sfincs_update, sfincs_finalize, t, dt
+   public :: host_np, host_npuv, host_ncuv, host_zs, host_zsmax, host_zb, host_sync, host_t0, host_t1
+
+   real*8, target :: t = 0.0d0
+   real*4, target :: dt = 0.0
+   real*8 :: host_t0 = 0.0d0, host_t1 = 0.0d0
+   integer :: host_np = 0, host_npuv = 0, host_ncuv = 0
+   real*8, allocatable, target :: host_zs(:)
+   real*4, allocatable, target :: host_zsmax(:), host_zb(:)
+   type(c_ptr) :: model = c_null_ptr
+   type(c_ptr) :: handle = c_null_ptr
+   integer :: logu = 0
+
+contains
+
+   subroutine logmsg(s)
+      character(*), intent(in) :: s
+      write(*, '(a)') trim(s)
+      if (logu /= 0) write(logu, '(a)') trim(s)
+   end subroutine
+
+   subroutine fail(where)
+      character(*), intent(in) :: where
+      character(kind=c_char) :: buf(512)
+      character(512) :: msg
+      integer :: i, rc
+      rc = sfx_last_error(buf, int(512, c_size_t))
+      msg = ' '
+      do i = 1, 512
+         if (buf(i) == c_null_char) exit
+         msg(i:i) = buf(i)
+      enddo
+      call logmsg('Error   : '//trim(where)//': '//trim(msg))
+   end subroutine
+
+   ! ---- sfincs.inp ------------------------------------------------------------------
+   subroutine inp_get(key, val, found)
+      character(*), intent(in) :: key
+      character(256), intent(out) :: val
+      logical, intent(out) :: found
+      character(512) :: line
+      character(128) :: k
+      integer :: u, ios, ieq
+      found = .false.
+      val = ' '
+      open(newunit=u, file='sfincs.inp', status='old', action='read', iostat=ios)
+      if (ios /= 0) return
+      do
+         read(u, '(a)', iostat=ios) line
+         if (ios /= 0) exit
+         ieq = index(line, '=')
+         if (ieq == 0) cycle
+         k = adjustl(line(1:ieq - 1))
+         if (trim(k) == key) then
+            val = adjustl(line(ieq + 1:))
+            found = .true.
+            exit
+         endif
+      enddo
+      close(u)
+   end subroutine
+
+   subroutine inp_real(key, v)
+      character(*), intent(in) :: key
+      real(c_float), intent(inout) :: v
+      character(256) :: s
+      logical :: f
+      call inp_get(key, s, f)
+      if (f) read(s, *) v
+   end subroutine
+
+   subroutine inp_int(key, v)
+      character(*), intent(in) :: key
+      integer, intent(inout) :: v
+      character(256) :: s
+      logical :: f
+      call inp_get(key, s, f)
+      if (f) read(s, *) v
+   end subroutine
+
+   subroutine inp_str(key, v, default)
+      character(*), intent(in) :: key, default
+      character(256), intent(out) :: v
+      logical :: f
+      call inp_get(key, v, f)
+      if (.not. f) v = default
+   end subroutine
+
+   subroutine inp_flag(key, v)
+      character(*), intent(in) :: key
+      integer(c_int32_t), intent(inout) :: v
+      character(256) :: s
+      logical :: f
+      call inp_get(key, s, f)
+      if (f) then
+         select case (s(1:1))
+         case ('t', 'T', '1', 'y', 'Y')
+            v = 1
+         case default
+            v = 0
+         end select
+      endif
+   end subroutine
+
+   ! seconds since 0001-01-01 for 'yyyymmdd hhmmss' (proleptic Gregorian)
+   real*8 function date_seconds(s)
+      character(*), intent(in) :: s
+      integer :: y, mo, d, h, mi, se, a, yy, mm, jdn
+      read(s(1:4), *) y; read(s(5:6), *) mo; read(s(7:8), *) d
+      read(s(10:11), *) h; read(s(12:13), *) mi; read(s(14:15), *) se
+      a = (14 - mo) / 12
+      yy = y + 4800 - a
+      mm = mo + 12 * a - 3
+      jdn = d + (153 * mm + 2) / 5 + 365 * yy + yy / 4 - yy / 100 + yy / 400 - 32045
+      date_seconds = dble(jdn) * 86400.0d0 + dble(h) * 3600.0d0 + dble(mi) * 60.0d0 + dble(se)
+   end function
+
+   integer function count_lines(fname)
+      character(*), intent(in) :: fname
+      character(1024) :: line
+      integer :: u, ios
+      count_lines = 0
+      open(newunit=u, file=trim(fname), status='old', action='read', iostat=ios)
+      if (ios /= 0) then
+         count_lines = -1
+         return
+      endif
+      do
+         read(u, '(a)', iostat=ios) line
+         if (ios /= 0) exit
+         if (len_trim(line) > 0) count_lines = count_lines + 1
+      enddo
+      close(u)
+   end function
+
+   ! ---- sfincs_initialize (sfincs_lib.f90:85-333) -------------------------------------
+   function sfincs_initialize() result(ierr)
+      integer :: ierr
+      type(sfx_params), target :: p
+      integer :: nmax, mmax, np, ios, u, i, j, n, m, nt, nb, rc
+      real(c_float) :: x0, y0, dx, dy, rot
+      character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
+      character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, s
+      integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
+      integer(c_int32_t), allocatable :: indices(:)
+      real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:)
+      real(c_float), allocatable, target :: tt(:), v1(:), v2(:)
+      integer(c_size_t) :: nbytes
+      logical :: f
+      type(sfx_step_stats) :: st
+      ierr = 0
+      open(newunit=logu, file='sfincs_mi355x.log', status='replace')
+      call logmsg('------------ SFINCS (MI355X-native time step) ------------')
+      call sfx_params_default(p)
+      call inp_str('tref', tref, '20000101 000000')
+      call inp_str('tstart', tstart, '20000101 000000')
+      call inp_str('tstop', tstop, '20000101 000000')
+      call inp_get('tref', s, f)
+      if (.not. f) tref = tstart
+      p%t0 = real(date_seconds(tstart) - date_seconds(tref), c_float)
+      p%t1 = real(date_seconds(tstop) - date_seconds(tref), c_float)
+      call inp_real('tspinup', p%tspinup)
+      call inp_real('alpha', p%alpha); call inp_real('theta', p%theta); call inp_real('hmin_cfl', p%hmin_cfl)
+      call inp_real('manning', p%manning); call inp_real('manning_land', p%manning_land)
+      call inp_real('manning_sea', p%manning_sea); call inp_real('rgh_lev_land', p%rgh_lev_land)
+      call inp_real('zsini', p%zsini); call inp_real('dtmax', p%dtmax); call inp_real('huthresh', p%huthresh)
+      call inp_real('huvmin', p%huvmin); call inp_real('rhoa', p%rhoa); call inp_real('rhow', p%rhow)
+      call inp_real('advlim', p%advlim); call inp_real('slopelim', p%slopelim); call inp_real('btfilter', p%btfilter)
+      call inp_real('btrelax', p%btrelax); call inp_real('wiggle_factor', p%wiggle_factor)
+      call inp_real('wiggle_threshold', p%wiggle_threshold); call inp_real('uvlim', p%uvlim)
+      call inp_real('uvmax', p%uvmax); call inp_real('nuvisc', p%nuvisc); call inp_real('nuviscfac', p%nuviscfac)
+      call inp_real('latitude', p%latitude)
+      call inp_flag('viscosity', p%viscosity); call inp_flag('coriolis', p%coriolis)
+      call inp_flag('friction2d', p%friction2d); call inp_flag('advection_mask', p%advection_mask)
+      call inp_flag('wiggle_suppression', p%wiggle_suppression); call inp_flag('h73table', p%h73table)
+      i = p%advection
+      call inp_int('advection', i); p%advection = merge(1, 0, i > 0)
+      call inp_str('advection_scheme', s, 'upw1')
+      if (trim(s) == 'original') p%advection_scheme = 0
+      host_t0 = p%t0; host_t1 = p%t1
+      x0 = 0.0; y0 = 0.0; dx = 0.0; dy = 0.0; rot = 0.0; nmax = 0; mmax = 0
+      call inp_int('nmax', nmax); call inp_int('mmax', mmax)
+      call inp_real('x0', x0); call inp_real('y0', y0); call inp_real('dx', dx); call inp_real('dy', dy)
+      call inp_real('rotation', rot)
+      call inp_str('inputformat', inputformat, 'bin')
+      call inp_str('depfile', depfile, 'none'); call inp_str('mskfile', mskfile, 'none')
+      call inp_str('indexfile', indexfile, 'none'); call inp_str('qtrfile', qtrfile, 'none')
+      call inp_str('bndfile', bndfile, 'none'); call inp_str('bzsfile', bzsfile, 'none')
+      call inp_str('srcfile', srcfile, 'none'); call inp_str('disfile', disfile, 'none')
+      call inp_str('prcfile', prcfile, 'none'); call inp_str('wndfile', wndfile, 'none')
+
+      if (qtrfile(1:4) /= 'none') then
+         call read_quadtree(qtrfile, mskfile, p, rc)
+         if (rc /= 0) then
+            ierr = -1
+            return
+         endif
+      else
+         ! regular grid: mask on the full nmax x mmax grid (n fastest)
+         allocate(mskg(nmax, mmax))
+         mskg = 0
+         if (inputformat(1:3) == 'asc') then
+            open(newunit=u, file=trim(mskfile), status='old', action='read')
+            allocate(tmp(mmax))
+            do n = 1, nmax
+               read(u, *) tmp
+               do m = 1, mmax
+                  mskg(n, m) = int(tmp(m), c_int8_t)
+               enddo
+            enddo
+            close(u)
+            deallocate(tmp)
+         else
+            open(newunit=u, file=trim(indexfile), form='unformatted', access='stream', status='old')
+            read(u) np
+            allocate(indices(np), msk(np))
+            read(u) indices
+            close(u)
+            open(newunit=u, file=trim(mskfile), form='unformatted', access='stream', status='old')
+            read(u) msk
+            close(u)
+            do i = 1, np
+               m = (indices(i) - 1) / nmax + 1
+               n = indices(i) - (m - 1) * nmax
+               mskg(n, m) = msk(i)
+            enddo
+            deallocate(msk)
+         endif
+         rc = sfx_model_create_regular(p, int(nmax, c_int32_t), int(mmax, c_int32_t), x0, y0, dx, dy, rot, &
+                                       c_loc(mskg), model)
+         if (rc < 0) then
+            call fail('mesh')
+            ierr = -1
+            return
+         endif
+         call table_size('zs', 8, host_np)
+         allocate(zb(host_np))
+         if (inputformat(1:3) == 'asc') then
+            allocate(zbg(nmax, mmax))
+            open(newunit=u, file=trim(depfile), status='old', action='read')
+            do n = 1, nmax
+               read(u, *) (zbg(n, m), m = 1, mmax)
+            enddo
+            close(u)
+            j = 0
+            do m = 1, mmax
+               do n = 1, nmax
+                  if (mskg(n, m) > 0) then
+                     j = j + 1
+                     zb(j) = zbg(n, m)
+                  endif
+               enddo
+            enddo
+         else
+            open(newunit=u, file=trim(depfile), form='unformatted', access='stream', status='old')
+            read(u) zb
+            close(u)
+         endif
+         rc = sfx_model_set_bathymetry(model, c_loc(zb))
+         if (rc < 0) then
+            call fail('bathymetry')
+            ierr = -1
+            return
+         endif
+      endif
+
+      ! water level boundaries (sfincs_boundaries.f90:8-452)
+      if (bndfile(1:4) /= 'none') then
+         nb = count_lines(bndfile)
+         allocate(xb(nb), yb(nb))
+         open(newunit=u, file=trim(bndfile), status='old', action='read')
+         do i = 1, nb
+            read(u, *) xb(i), yb(i)
+         enddo
+         close(u)
+         if (bzsfile(1:4) /= 'none') then
+            nt = count_lines(bzsfile)
+            allocate(tb(nt), zsb(nb, nt))
+            open(newunit=u, file=trim(bzsfile), status='old', action='read')
+            do i = 1, nt
+               read(u, *) tb(i), (zsb(j, i), j = 1, nb)
+            enddo
+            close(u)
+            rc = sfx_model_set_boundary(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), int(nt, c_int32_t), &
+                                        c_loc(tb), c_loc(zsb))
+         else
+            rc = sfx_model_set_boundary(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), 0_c_int32_t, &
+                                        c_null_ptr, c_null_ptr)
+         endif
+         if (rc < 0) call fail('boundaries')
+      endif
+      ! point discharges (sfincs_discharges.f90:8-320)
+      if (srcfile(1:4) /= 'none') then
+         nb = count_lines(srcfile)
+         nt = count_lines(disfile)
+         if (allocated(xb)) deallocate(xb, yb)
+         allocate(xb(nb), yb(nb), tt(nt))
+         if (allocated(zsb)) deallocate(zsb)
+         allocate(zsb(nb, nt))
+         open(newunit=u, file=trim(srcfile), status='old', action='read')
+         do i = 1, nb
+            read(u, *) xb(i), yb(i)
+         enddo
+         close(u)
+         open(newunit=u, file=trim(disfile), status='old', action='read')
+         do i = 1, nt
+            read(u, *) tt(i), (zsb(j, i), j = 1, nb)
+         enddo
+         close(u)
+         rc = sfx_model_set_sources(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), int(nt, c_int32_t), &
+                                    c_loc(tt), c_loc(zsb))
+         if (rc < 0) call fail('discharges')
+         deallocate(tt)
+      endif
+      ! uniform rain and wind series (sfincs_meteo.f90:247-312)
+      if (prcfile(1:4) /= 'none') then
+         nt = count_lines(prcfile)
+         allocate(tt(nt), v1(nt))
+         open(newunit=u, file=trim(prcfile), status='old', action='read')
+         do i = 1, nt
+            read(u, *) tt(i), v1(i)
+         enddo
+         close(u)
+         rc = sfx_model_set_rain(model, int(nt, c_int32_t), c_loc(tt), c_loc(v1))
+         if (rc < 0) call fail('rain')
+         deallocate(tt, v1)
+      endif
+      if (wndfile(1:4) /= 'none') then
+         nt = count_lines(wndfile)
+         allocate(tt(nt), v1(nt), v2(nt))
+         open(newunit=u, file=trim(wndfile), status='old', action='read')
+         do i = 1, nt
+            read(u, *) tt(i), v1(i), v2(i)
+         enddo
+         close(u)
+         rc = sfx_model_set_wind(model, int(nt, c_int32_t), c_loc(tt), c_loc(v1), c_loc(v2))
+         if (rc < 0) call fail('wind')
+         deallocate(tt, v1, v2)
+      endif
+      rc = sfx_model_initialize(model)
+      if (rc < 0) then
+         call fail('initialize')
+         ierr = -1
+         return
+      endif
+      call table_size('zs', 8, host_np)
+      call table_size('uv_index_z_nm', 4, host_npuv)
+      allocate(host_zs(host_np), host_zsmax(host_np))
+      ! the replacement of initialize_openacc: tables and state to HBM once
+      rc = sfx_create(model, 0_c_int32_t, handle)
+      if (rc < 0) then
+         call fail('sfx_create')
+         ierr = -1
+         return
+      endif
+      t = host_t0
+      dt = 1.0e-6
+      call logmsg('---------- Starting simulation -----------')
+      ierr = 0
+   end function
+
+   subroutine table_size(name, esize, n)
+      character(*), intent(in) :: name
+      integer, intent(in) :: esize
+      integer, intent(out) :: n
+      integer(c_size_t) :: nb
+      integer :: rc
+      rc = sfx_model_table_bytes(model, trim(name)//c_null_char, nb)
+      n = int(nb / esize)
+   end subroutine
+
+   subroutine read_quadtree(qtrfile, mskfile, p, rc)
+      ! legacy binary quadtree file (sfincs_quadtree.F90:210-292) + binary mask file
+      character(256), intent(in) :: qtrfile, mskfile
+      type(sfx_params), intent(in) :: p
+      integer, intent(out) :: rc
+      type(sfx_quadtree_input) :: q
+      integer(c_int8_t) :: iversion, nlev
+      integer(c_int32_t) :: iepsg, np
+      real(c_float) :: x0, y0, dx, dy, rot
+      integer(c_int8_t), allocatable, target, save :: lev(:), fnu(:), fmu(:), fnd(:), fmd(:), msk(:)
+      integer(c_int32_t), allocatable, target, save :: n(:), m(:), nu1(:), nu2(:), mu1(:), mu2(:)
+      integer(c_int32_t), allocatable, target, save :: nd1(:), nd2(:), md1(:), md2(:)
+      real(c_float), allocatable, target, save :: zz(:), zb(:)
+      integer :: u, i, j
+      open(newunit=u, file=trim(qtrfile), form='unformatted', access='stream', status='old')
+      read(u) iversion, iepsg, np, nlev, x0, y0, dx, dy, rot
+      allocate(lev(np), n(np), m(np), fnu(np), nu1(np), nu2(np), fmu(np), mu1(np), mu2(np))
+      allocate(fnd(np), nd1(np), nd2(np), fmd(np), md1(np), md2(np), zz(np), msk(np))
+      read(u) lev, n, m, fnu, nu1, nu2, fmu, mu1, mu2, fnd, nd1, nd2, fmd, md1, md2, zz
+      close(u)
+      open(newunit=u, file=trim(mskfile), form='unformatted', access='stream', status='old')
+      read(u) msk
+      close(u)
+      q%nr_points = np; q%nr_levels = nlev; q%x0 = x0; q%y0 = y0; q%dx = dx; q%dy = dy; q%rotation_deg = rot
+      q%level = c_loc(lev); q%md = c_loc(fmd); q%mu = c_loc(fmu); q%nd = c_loc(fnd); q%nu = c_loc(fnu)
+      q%mask = c_loc(msk); q%n = c_loc(n); q%m = c_loc(m); q%md1 = c_loc(md1); q%md2 = c_loc(md2)
+      q%mu1 = c_loc(mu1); q%mu2 = c_loc(mu2); q%nd1 = c_loc(nd1); q%nd2 = c_loc(nd2)
+      q%nu1 = c_loc(nu1); q%nu2 = c_loc(nu2)
+      rc = sfx_model_create_quadtree(p, q, model)
+      if (rc < 0) then
+         call fail('quadtree mesh')
+         return
+      endif
+      ! bed level from the quadtree file for active points (sfincs_domain.f90:1635-1637)
+      allocate(zb(count(msk > 0)))
+      j = 0
+      do i = 1, np
+         if (msk(i) > 0) then
+            j = j + 1
+            zb(j) = zz(i)
+         endif
+      enddo
+      rc = sfx_model_set_bathymetry(model, c_loc(zb))
+      if (rc < 0) call fail('bathymetry')
+      rc = min(rc, 0)
+   end subroutine
+
+   ! ---- sfincs_update (sfincs_lib.f90:337-683) ----------------------------------------
+   function sfincs_update(dtrange) result(ierr)
+      double precision, intent(in) :: dtrange
+      integer :: ierr
+      type(sfx_step_stats) :: st
+      integer(c_int) :: rc
+      rc = sfx_update(handle, real(dtrange, c_double), st)
+      t = st%t
+      dt = st%dt
+      ierr = 0
+      if (rc < 0) then
+         call fail('update')
+         ierr = -1
+      else if (rc == 1 .or. st%error /= 0) then
+         call logmsg('Error! Minimum time step reached. Simulation stopped.')
+         ierr = 1
+      endif
+   end function
+
+   ! explicit device-to-host copy of zs and zsmax (the form of `acc update host`)
+   subroutine host_sync()
+      integer(c_int) :: rc
+      rc = sfx_get_state(handle, 'zs'//c_null_char, c_loc(host_zs), int(8 * host_np, c_size_t))
+      rc = sfx_get_state(handle, 'zsmax'//c_null_char, c_loc(host_zsmax), int(4 * host_np, c_size_t))
+   end subroutine
+
+   ! ---- sfincs_finalize (sfincs_lib.f90:687-819) ---------------------------------------
+   function sfincs_finalize() result(ierr)
+      integer :: ierr, u
+      integer(c_int) :: rc
+      ierr = 0
+      if (c_associated(handle)) then
+         call host_sync()
+         open(newunit=u, file='zs_final.dat', form='unformatted', access='stream', status='replace')
+         write(u) host_zs
+         close(u)
+         open(newunit=u, file='zsmax.dat', form='unformatted', access='stream', status='replace')
+         write(u) host_zsmax
+         close(u)
+         rc = sfx_destroy(handle)
+         handle = c_null_ptr
+      endif
+      if (c_associated(model)) then
+         call sfx_model_destroy(model)
+         model = c_null_ptr
+      endif
+      call logmsg('---------- Simulation finished -----------')
+      if (logu /= 0) close(logu)
+      logu = 0
+   end function
+
+end module sfincs_lib
+
+
+! BMI / XMI names of the reference's shared library (sfincs_bmi.f90:56-387, subset).
+module sfincs_bmi
+   use iso_c_binding
+   use sfincs_lib
+   implicit none
+contains
+   integer(c_int) function initialize(c_config_file) result(ierr) bind(C, name='initialize')
+      type(c_ptr), value :: c_config_file
+      ierr = sfincs_initialize()
+   end function
+   integer(c_int) function update() result(ierr) bind(C, name='update')
+      ierr = sfincs_update(-1.0d0)
+   end function
+   integer(c_int) function update_until(tt) result(ierr) bind(C, name='update_until')
+      real(c_double), value :: tt
+      ierr = sfincs_update(tt - t)
+   end function
+   integer(c_int) function finalize() result(ierr) bind(C, name='finalize')
+      ierr = sfincs_finalize()
+   end function
+   integer(c_int) function get_start_time(tt) result(ierr) bind(C, name='get_start_time')
+      real(c_double), intent(out) :: tt
+      tt = host_t0
+      ierr = 0
+   end function
+   integer(c_int) function get_end_time(tt) result(ierr) bind(C, name='get_end_time')
+      real(c_double), intent(out) :: tt
+      tt = host_t1
+      ierr = 0
+   end function
+   integer(c_int) function get_current_time(tt) result(ierr) bind(C, name='get_current_time')
+      real(c_double), intent(out) :: tt
+      tt = t
+      ierr = 0
+   end function
+   integer(c_int) function get_time_step(tt) result(ierr) bind(C, name='get_time_step')
+      real(c_double), intent(out) :: tt
+      tt = dt
+      ierr = 0
+   end function
+   ! Host pointers to zs / zsmax / zb after an explicit device-to-host copy
+   ! (the reference hands out c_loc of its arrays, sfincs_bmi.f90:95-129).
+   integer(c_int) function get_value_ptr(c_var_name, x) result(ierr) bind(C, name='get_value_ptr')
+      character(kind=c_char), intent(in) :: c_var_name(*)
+      type(c_ptr), intent(out) :: x
+      character(64) :: nm
+      integer :: i
+      nm = ' '
+      do i = 1, 64
+         if (c_var_name(i) == c_null_char) exit
+         nm(i:i) = c_var_name(i)
+      enddo
+      call host_sync()
+      ierr = 0
+      select case (trim(nm))
+      case ('zs')
+         x = c_loc(host_zs)
+      case ('zsmax')
+         x = c_loc(host_zsmax)
+      case default
+         x = c_null_ptr
+         ierr = -1
+      end select
+   end function
+end module sfincs_bmi

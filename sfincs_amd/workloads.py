@@ -89,140 +89,156 @@ def beach_with_bumps(r, nmax, mmax, dx, dy, nbumps=20):
     return zb.astype(np.float32)          # (mmax, nmax)
 
 
+def _device(npix):
+    """Torch device for input generation: the GPU for large inputs when present."""
+    import torch
+    if npix >= (1 << 20) and torch.cuda.is_available():
+        return torch.device("cuda")
+    return torch.device("cpu")
+
+
 def fbm(r, n, m, hurst=0.7, lo=-15.0, hi=20.0):
-    """2-D fractional Brownian surface by spectral synthesis, P(k) ~ k^-(2H+2), rescaled."""
-    from scipy import fft as sfft
-    ky = np.fft.fftfreq(n).astype(np.float32)
-    kx = np.fft.rfftfreq(m).astype(np.float32)
+    """2-D fractional Brownian surface by spectral synthesis, P(k) ~ k^-(2H+2), rescaled
+    affinely to [lo, hi]. Random phases come from the numpy generator `r`."""
+    import torch
+    dev = _device(n * m)
+    phase = torch.from_numpy(r.uniform(0.0, 2.0 * np.pi, size=(n, m // 2 + 1)).astype(np.float32)).to(dev)
+    ky = torch.fft.fftfreq(n, device=dev, dtype=torch.float32)
+    kx = torch.fft.rfftfreq(m, device=dev, dtype=torch.float32)
     k2 = ky[:, None] ** 2 + kx[None, :] ** 2
     k2[0, 0] = 1.0
     amp = k2 ** (-(2.0 * hurst + 2.0) / 4.0)
     amp[0, 0] = 0.0
-    phase = r.uniform(0.0, 2.0 * np.pi, size=amp.shape).astype(np.float32)
-    spec = (amp * (np.cos(phase) + 1j * np.sin(phase))).astype(np.complex64)
-    del phase, k2
-    z = sfft.irfft2(spec, s=(n, m), workers=8).astype(np.float32)
+    spec = torch.polar(amp, phase)
+    del phase, k2, amp
+    z = torch.fft.irfft2(spec, s=(n, m))
     del spec
-    zmin, zmax = float(z.min()), float(z.max())
+    zmin, zmax = z.min(), z.max()
     z = lo + (z - zmin) * ((hi - lo) / (zmax - zmin))
-    return z.astype(np.float32)           # (n rows, m cols)
+    return z.float().cpu().numpy()          # (n rows, m cols)
 
 
-def landuse_manning(r, n, m, values=(0.02, 0.035, 0.06)):
-    """Seeded land-use: smoothed noise split into three quantile classes."""
-    from scipy import ndimage
-    noise = r.standard_normal((n, m)).astype(np.float32)
-    sm = ndimage.gaussian_filter(noise, sigma=max(2.0, min(n, m) / 64.0))
+def landuse_manning(r, n, m, values=(0.02, 0.035, 0.06), refi=1):
+    """Seeded land-use: smoothed noise on the cell grid split into three quantile classes,
+    repeated onto the refi x refi pixels of each cell."""
+    import torch
+    nc, mc = n // refi, m // refi
+    noise = torch.from_numpy(r.standard_normal((nc, mc)).astype(np.float32))
+    sig = max(2.0, min(nc, mc) / 64.0)
+    ky = torch.fft.fftfreq(nc)[:, None]
+    kx = torch.fft.rfftfreq(mc)[None, :]
+    filt = torch.exp(-2.0 * (np.pi ** 2) * sig * sig * (ky ** 2 + kx ** 2))
+    sm = torch.fft.irfft2(torch.fft.rfft2(noise) * filt, s=(nc, mc)).numpy()
     q1, q2 = np.quantile(sm, [1.0 / 3.0, 2.0 / 3.0])
-    out = np.where(sm < q1, values[0], np.where(sm < q2, values[1], values[2])).astype(np.float32)
-    return out
+    cls = np.where(sm < q1, values[0], np.where(sm < q2, values[1], values[2])).astype(np.float32)
+    return np.repeat(np.repeat(cls, refi, axis=0), refi, axis=1)
 
 
 # ---------------------------------------------------------------- subgrid tables
 def _volume_levels(zsorted, nl, pix_area):
-    """Cell volume table: z at nl equally spaced volumes between 0 and volmax."""
-    npx = zsorted.shape[1]
-    zmin = zsorted[:, 0]
-    zmax = zsorted[:, -1]
-    # cumulative volume at each sorted pixel elevation: V(z_k) = sum_{j<k} (z_k - z_j) * A
-    csum = np.cumsum(zsorted, axis=1, dtype=np.float64)
-    j = np.arange(npx)
-    vk = (zsorted.astype(np.float64) * j[None, :] - np.concatenate(
-        [np.zeros((zsorted.shape[0], 1)), csum[:, :-1]], axis=1)) * pix_area
+    """Cell volume table: z at nl equally spaced volumes between 0 and volmax.
+    zsorted: (ncell, npx) torch tensor of sorted pixel elevations."""
+    import torch
+    z = zsorted.double()
+    npx = z.shape[1]
+    csum = torch.cumsum(z, dim=1)
+    j = torch.arange(npx, device=z.device, dtype=torch.float64)
+    prev = torch.cat([torch.zeros_like(csum[:, :1]), csum[:, :-1]], dim=1)
+    vk = (z * j[None, :] - prev) * pix_area          # volume below pixel k's elevation
     volmax = vk[:, -1]
-    zl = np.empty((zsorted.shape[0], nl), dtype=np.float64)
+    zl = torch.empty((z.shape[0], nl), dtype=torch.float64, device=z.device)
     for lev in range(nl):
         target = volmax * lev / (nl - 1)
-        # find segment k with vk[k] <= target < vk[k+1]
-        k = np.clip((vk <= target[:, None]).sum(axis=1) - 1, 0, npx - 1)
-        kk = np.minimum(k + 1, npx - 1)
-        v0 = np.take_along_axis(vk, k[:, None], 1)[:, 0]
-        z0 = np.take_along_axis(zsorted, k[:, None], 1)[:, 0].astype(np.float64)
-        z1 = np.take_along_axis(zsorted, kk[:, None], 1)[:, 0].astype(np.float64)
-        wet = (k + 1).astype(np.float64) * pix_area          # wetted area in segment
+        k = ((vk <= target[:, None]).sum(dim=1) - 1).clamp(0, npx - 1)
+        kk = (k + 1).clamp(max=npx - 1)
+        v0 = vk.gather(1, k[:, None])[:, 0]
+        z0 = z.gather(1, k[:, None])[:, 0]
+        z1 = z.gather(1, kk[:, None])[:, 0]
+        wet = (k + 1).double() * pix_area
         dz = (target - v0) / wet
-        zl[:, lev] = np.where(kk > k, np.minimum(z0 + dz, z1), z0 + dz)
-    zl[:, 0] = zmin
-    zl[:, -1] = zmax
-    return zmin, zmax, volmax, zl
+        zl[:, lev] = torch.where(kk > k, torch.minimum(z0 + dz, z1), z0 + dz)
+    zl[:, 0] = z[:, 0]
+    zl[:, -1] = z[:, -1]
+    return z[:, 0], z[:, -1], volmax, zl
 
 
 def _uv_tables(zpix, npix, nl):
-    """Face tables from the pixels around a uv-point: zmin/zmax, havg/nrep/pwet per level."""
-    zmin = zpix.min(axis=1)
-    zmax = zpix.max(axis=1)
-    zmax = np.maximum(zmax, zmin + 0.01)
-    levels = zmin[:, None] + (zmax - zmin)[:, None] * (np.arange(nl)[None, :] / (nl - 1))
-    havg = np.empty((zpix.shape[0], nl), np.float32)
-    nrep = np.empty_like(havg)
-    pwet = np.empty_like(havg)
-    navg = npix.mean(axis=1)
+    """Face tables from the pixels around a uv-point: zmin/zmax, havg/nrep/pwet per level
+    (build-owned recipe; torch tensors in, float32 numpy arrays out)."""
+    import torch
+    zmin = zpix.min(dim=1).values
+    zmax = torch.maximum(zpix.max(dim=1).values, zmin + 0.01)
+    navg = npix.mean(dim=1)
+    P = zpix.shape[0]
+    havg = torch.empty((P, nl), dtype=torch.float32, device=zpix.device)
+    nrep = torch.empty_like(havg)
+    pwet = torch.empty_like(havg)
     for lev in range(nl):
-        h = np.maximum(levels[:, lev][:, None] - zpix, 0.0)
-        ha = h.mean(axis=1)
+        level = zmin + (zmax - zmin) * (lev / (nl - 1))
+        h = (level[:, None] - zpix).clamp(min=0.0)
+        ha = h.mean(dim=1)
         havg[:, lev] = ha
-        pwet[:, lev] = (h > 0).mean(axis=1)
-        conv = (h ** (5.0 / 3.0) / npix).mean(axis=1)
-        with np.errstate(divide="ignore", invalid="ignore"):
-            nr = np.where(conv > 0, ha ** (5.0 / 3.0) / conv, navg)
-        nrep[:, lev] = np.where(np.isfinite(nr) & (nr > 0), nr, navg)
-    # fit parameter of the above-zmax roughness transition (build-owned recipe)
-    ffit = (1.0 / np.maximum(zmax - zmin, 0.1)).astype(np.float32)
-    return zmin.astype(np.float32), zmax.astype(np.float32), havg, nrep, pwet, navg.astype(np.float32), ffit
+        pwet[:, lev] = (h > 0).float().mean(dim=1)
+        conv = (h.pow(5.0 / 3.0) / npix).mean(dim=1)
+        nr = torch.where(conv > 0, ha.pow(5.0 / 3.0) / conv.clamp(min=1e-30), navg)
+        nrep[:, lev] = torch.where(torch.isfinite(nr) & (nr > 0), nr, navg)
+    ffit = 1.0 / (zmax - zmin).clamp(min=0.1)
+    f = lambda t: t.float().cpu().numpy()
+    return f(zmin), f(zmax), f(havg), f(nrep), f(pwet), f(navg), f(ffit)
 
 
 def regular_subgrid_tables(dem, nman, refi, dx, nl, model):
     """Tables for a dense regular grid from a DEM with refi x refi pixels per cell.
 
     dem/nman: (nmax*refi, mmax*refi) arrays, rows = n (y), cols = m (x)."""
+    import torch
     ny, nx = dem.shape
     nmax, mmax = ny // refi, nx // refi
+    dev = _device(ny * nx)
     pix_area = (dx / refi) ** 2
+    D = torch.from_numpy(np.ascontiguousarray(dem)).to(dev)
+    N = torch.from_numpy(np.ascontiguousarray(nman)).to(dev)
     # cells in sfincs order (column-major: n fastest)
-    blocks = dem.reshape(nmax, refi, mmax, refi).transpose(2, 0, 1, 3).reshape(nmax * mmax, refi * refi)
-    zs_sorted = np.sort(blocks, axis=1)
-    zmin, zmax, volmax, zl = _volume_levels(zs_sorted, nl, pix_area)
-    del blocks, zs_sorted
-    nm = model.table("uv_index_z_nm") - 1
-    nmu = model.table("uv_index_z_nmu") - 1
-    dirs = model.table("uv_flags_dir")
-    nsrc_n = model.table("z_index_z_n") - 1
-    nsrc_m = model.table("z_index_z_m") - 1
-    npuv = len(nm)
+    blocks = D.reshape(nmax, refi, mmax, refi).permute(2, 0, 1, 3).reshape(nmax * mmax, refi * refi)
+    zmin, zmax, volmax, zl = _volume_levels(torch.sort(blocks, dim=1).values, nl, pix_area)
+    del blocks
+    nm = torch.from_numpy(model.table("uv_index_z_nm").astype(np.int64) - 1).to(dev)
+    dirs = torch.from_numpy(model.table("uv_flags_dir").astype(np.int64)).to(dev)
+    zn = torch.from_numpy(model.table("z_index_z_n").astype(np.int64) - 1).to(dev)
+    zm = torch.from_numpy(model.table("z_index_z_m").astype(np.int64) - 1).to(dev)
+    npuv = nm.shape[0]
     half = refi // 2 + 1
     out = {k: np.empty(npuv, np.float32) for k in ("uv_zmin", "uv_zmax", "uv_navg", "uv_ffit")}
     for k in ("uv_havg", "uv_nrep", "uv_pwet"):
         out[k] = np.empty((npuv, nl), np.float32)
-    # window of 2*half pixel columns (u) or rows (v) centred on the face, refi wide across
+    ar_r = torch.arange(refi, device=dev)
+    ar_h = torch.arange(2 * half, device=dev)
     chunk = 1 << 20
     for d in (0, 1):
-        sel = np.nonzero(dirs == d)[0]
-        for s0 in range(0, len(sel), chunk):
+        sel = torch.nonzero(dirs == d)[:, 0]
+        for s0 in range(0, sel.shape[0], chunk):
             idx = sel[s0:s0 + chunk]
-            n0 = nsrc_n[nm[idx]]
-            m0 = nsrc_m[nm[idx]]
-            if d == 0:
-                rows = n0[:, None] * refi + np.arange(refi)[None, :]
-                cols = (m0[:, None] + 1) * refi - half + np.arange(2 * half)[None, :]
-                cols = np.clip(cols, 0, nx - 1)
-                zp = dem[rows[:, :, None], cols[:, None, :]].reshape(len(idx), -1)
-                npx = nman[rows[:, :, None], cols[:, None, :]].reshape(len(idx), -1)
-            else:
-                rows = (n0[:, None] + 1) * refi - half + np.arange(2 * half)[None, :]
-                rows = np.clip(rows, 0, ny - 1)
-                cols = m0[:, None] * refi + np.arange(refi)[None, :]
-                zp = dem[rows[:, :, None], cols[:, None, :]].reshape(len(idx), -1)
-                npx = nman[rows[:, :, None], cols[:, None, :]].reshape(len(idx), -1)
+            n0 = zn[nm[idx]]
+            m0 = zm[nm[idx]]
+            if d == 0:      # window across the face between columns m0 and m0+1
+                rows = n0[:, None] * refi + ar_r[None, :]
+                cols = ((m0[:, None] + 1) * refi - half + ar_h[None, :]).clamp(0, nx - 1)
+            else:           # window across the face between rows n0 and n0+1
+                rows = ((n0[:, None] + 1) * refi - half + ar_h[None, :]).clamp(0, ny - 1)
+                cols = m0[:, None] * refi + ar_r[None, :]
+            zp = D[rows[:, :, None], cols[:, None, :]].reshape(idx.shape[0], -1)
+            npx = N[rows[:, :, None], cols[:, None, :]].reshape(idx.shape[0], -1)
             zmn, zmx, ha, nr, pw, na, ff = _uv_tables(zp, npx, nl)
-            out["uv_zmin"][idx] = zmn
-            out["uv_zmax"][idx] = zmx
-            out["uv_havg"][idx] = ha
-            out["uv_nrep"][idx] = nr
-            out["uv_pwet"][idx] = pw
-            out["uv_navg"][idx] = na
-            out["uv_ffit"][idx] = ff
-    sg = dict(nlevels=nl, z_zmin=zmin.astype(np.float32), z_zmax=zmax.astype(np.float32),
-              z_volmax=volmax.astype(np.float32), z_level=zl.astype(np.float32))
+            ii = idx.cpu().numpy()
+            out["uv_zmin"][ii] = zmn
+            out["uv_zmax"][ii] = zmx
+            out["uv_havg"][ii] = ha
+            out["uv_nrep"][ii] = nr
+            out["uv_pwet"][ii] = pw
+            out["uv_navg"][ii] = na
+            out["uv_ffit"][ii] = ff
+    f = lambda t: t.float().cpu().numpy()
+    sg = dict(nlevels=nl, z_zmin=f(zmin), z_zmax=f(zmax), z_volmax=f(volmax), z_level=f(zl))
     sg.update(out)
     return sg
 
@@ -258,7 +274,7 @@ def subgrid_case(nmax, mmax, dx=25.0, refi=5, nl=10, seed=1001, t_end=21600.0, t
     """Configuration-1 recipe at any size (regular mesh with subgrid tables from an fBm DEM)."""
     r = rng(seed)
     dem = fbm(r, nmax * refi, mmax * refi)
-    nman = landuse_manning(r, nmax * refi, mmax * refi)
+    nman = landuse_manning(r, nmax * refi, mmax * refi, refi=refi)
     zcell_min = dem.reshape(nmax, refi, mmax, refi).min(axis=(1, 3))     # (nmax, mmax)
     msk = np.ones((mmax, nmax), np.int8)
     msk[0, :] = np.where(zcell_min[:, 0] < 0.0, 2, 1)
@@ -316,3 +332,36 @@ def ensemble_members(count=64, seed_base=1003, t_end=10800.0, nmin=256, nmax=204
         sc.name = f"member{j}_{sc.name}"
         out.append(sc)
     return out
+
+
+def quadtree_case(nb, mb, nlev=4, dx0=100.0, seed=1002, t_end=43200.0, refine=(-5.0, 5.0), surge_amp=3.0,
+                  wind=True, **params):
+    """Configuration-2 recipe at any size: quadtree mesh refined where the seeded fBm bed
+    level lies within `refine`, 2:1 balanced; Gaussian surge (peak at 6 h, sigma 3 h) on
+    the west edge, uniform 20 m/s wind from the south-west."""
+    from . import quadtree as Q
+    r = rng(seed)
+    F = 2 ** (nlev - 1)
+    dem = fbm(r, nb * F, mb * F)
+    _, leaf, _, _ = Q.build(dem, nlev, refine)
+    qt = Q.quadtree_arrays(leaf, nlev, 0.0, 0.0, dx0, dx0)
+    zb = Q.leaf_bed_level(qt, dem, nlev)
+    mask = np.ones(len(zb), np.int8)
+    mask[(qt["m"] == 1) & (zb < 0.0)] = 2
+    qt["mask"] = mask
+    t = np.arange(0.0, t_end + 1200.0, 600.0, dtype=np.float64)
+    zs = (surge_amp * np.exp(-((t - 21600.0) ** 2) / (2.0 * 10800.0 ** 2))).astype(np.float32)[:, None]
+    p = dict(t0=0.0, t1=float(t_end))
+    p.update(params)
+    wnd = None
+    if wind:
+        tw = np.array([0.0, t_end + 3600.0], np.float32)
+        wnd = (tw, np.array([20.0, 20.0], np.float32), np.array([225.0, 225.0], np.float32))
+    return Scenario(name=f"quadtree_{nb}x{mb}_L{nlev}", params=p, kind="quadtree", qt=qt, zb_qt=zb,
+                    bnd=(np.array([-dx0 / 2], np.float32), np.array([nb * dx0 / 2], np.float32),
+                         t.astype(np.float32), zs),
+                    wind=wnd, t_end=float(t_end))
+
+
+def config2(nb=700, mb=700):
+    return quadtree_case(nb, mb, nlev=4, dx0=100.0, seed=1002, t_end=43200.0)

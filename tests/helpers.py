@@ -16,19 +16,7 @@ def oracle_for(sc, model=None):
 
 def gpu_for(model, fp64=False):
     from sfincs_amd import _lib
-    if fp64:
-        import ctypes as C
-        L = _lib.lib()
-        out = C.c_void_p()
-        f = L.sfx_create_ex
-        f.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
-        f.restype = C.c_int
-        _lib.check(f(model.h, 0, 1, C.byref(out)), "sfx_create_ex")
-        inst = _lib.Instance.__new__(_lib.Instance)
-        inst.h = out
-        inst.sizes = model.sizes()
-        return inst
-    return _lib.Instance(model)
+    return _lib.Instance(model, fp64=fp64)
 
 
 def maxdiff(a, b):

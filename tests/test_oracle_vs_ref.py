@@ -73,3 +73,15 @@ def test_wetting_front_long_run_bit_exact():
     """A longer run through a rising tide: wetting/drying fronts cross many uv-points."""
     sc = W.regular_case(24, 40, dx=50.0, seed=24, t_end=7200.0, tide_amp=3.0, rain_mmh=60.0)
     run_pair(sc, 400)
+
+
+def test_quadtree_bit_exact():
+    """Quadtree branches of compute_fluxes (transition spacings, flags) and the
+    combined-uv averaging (sfincs_momentum.f90:193-313, 757-779)."""
+    sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=5, t_end=3600.0, surge_amp=4.0)
+    run_pair(sc, 250)
+
+
+def test_quadtree_viscosity_bit_exact():
+    sc = W.quadtree_case(8, 12, nlev=3, dx0=100.0, seed=6, t_end=3600.0, surge_amp=4.0, viscosity=1)
+    run_pair(sc, 150)

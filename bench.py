@@ -5,7 +5,9 @@ subgrid tables from a seeded 5 m fBm DEM, 8 point discharges, 1.5 m tidal bounda
 A step is one model time step (forcing + momentum with CFL reduction + continuity) of
 the whole grid with all state resident in HBM; value = cell-updates/s.
 
-Prints one JSON line. See DESIGN.md for the byte model behind `roofline`.
+Prints one JSON line. `roofline.achieved` is the momentum kernel's algorithmic bytes per
+launch (per-branch byte counts below times the branch counters of the device) divided by
+its mean launch time, measured with HIP events on the launch stream.
 """
 import argparse
 import json
@@ -95,7 +97,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(ensembles shard by member across GPUs; see DESIGN.md)", file=sys.stderr)
+              f"(ensembles shard by member across GPUs; each GPU runs its own batch)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

@@ -13,7 +13,7 @@
  *   oracle_update_discharges     sfincs_discharges.f90:325-369
  *   oracle_update                sfincs_lib.f90:337-683 (step order, lagged dt)
  * Arrays use 0-based storage; index tables hold the reference's 1-based values.
- * Parity status per function is stated in DESIGN.md and the test files.
+ * Each function's parity check is named in the test files under tests/.
  */
 #include <math.h>
 #include <stdint.h>

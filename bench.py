@@ -85,6 +85,87 @@ def cpu_baseline(sc, model, inst, nsteps):
                 seconds=secs)
 
 
+def ensemble(args):
+    """Configuration 3: 64 members of log-uniform size (256^2..2048^2, regular or subgrid),
+    each with its own surge amplitude, rain and adaptive dt, 3 h simulated. The same
+    members run one by one (sfx_update per member) and together in one batch; the JSON
+    line reports both aggregate rates, the speed-up, time-to-result percentiles and the
+    largest zs difference of any member between the two runs."""
+    import torch
+    from sfincs_amd import ensemble as E
+    from sfincs_amd import workloads as W
+    t0 = time.perf_counter()
+    scs = W.ensemble_members(args.members, t_end=args.t_end, scale=args.scale)
+    models = [sc.build_model() for sc in scs]
+    t_ends = [sc.t_end for sc in scs]
+    cells = [m.sizes()["np"] for m in models]
+    print(f"# generated {len(models)} members, {sum(cells)} cells in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    seq = E.run_sequential(models, t_ends, keep_state=True)
+    print(f"# sequential {seq['wall_s']:.2f}s {seq['cell_updates_per_s']:.3e}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    bat = E.run_batched(models, t_ends, round_steps=args.round_steps, keep_state=True)
+    print(f"# batched {bat['wall_s']:.2f}s {bat['cell_updates_per_s']:.3e}", file=sys.stderr, flush=True)
+    dzs = max(float(np.max(np.abs(a - b))) for a, b in zip(seq["zs"], bat["zs"]))
+    same_steps = seq["steps"] == bat["steps"]
+    out = {
+        "metric": "ensemble aggregate cell-updates/s (batched) and p50/p99 time-to-result",
+        "value": bat["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32 (zs, z_volume f64)", "data": "synthetic (seeded, PCG64((1000+3)*100000+j))",
+        "config": {"workload": f"config 3: {len(models)} members, log-uniform 256..2048 per side x {args.scale}, "
+                               f"regular/subgrid mix, {args.t_end / 3600:.2f} h simulated each",
+                   "cells_total": sum(cells), "round_steps": args.round_steps},
+        "batched": {k: bat[k] for k in ("wall_s", "cell_updates", "cell_updates_per_s", "p50_s", "p99_s")},
+        "sequential": {k: seq[k] for k in ("wall_s", "cell_updates", "cell_updates_per_s", "p50_s", "p99_s")},
+        "speedup": bat["cell_updates_per_s"] / seq["cell_updates_per_s"],
+        "max_abs_dzs_batched_vs_alone": dzs, "same_step_counts": same_steps,
+    }
+    print(json.dumps(out))
+
+
+def serve(args):
+    """Configuration 4 (scaled by the flags): Poisson arrivals of requests drawn from the
+    configuration-3 distribution plus 10% quadtree meshes, served by continuous batching
+    with admission by HBM footprint. Reports p50/p99 time-to-result and aggregate rate."""
+    import torch
+    from sfincs_amd import ensemble as E
+    from sfincs_amd import workloads as W
+    r = W.rng(1004)
+    t = 0.0
+    reqs = []
+    j = 0
+    while True:
+        t += float(r.exponential(1.0 / args.rate))
+        if t > args.window:
+            break
+        dur = float(r.uniform(3600.0, 43200.0)) * args.duration_scale
+        if r.uniform() < 0.1:
+            nb = int(r.integers(40, 120) * args.scale) + 4
+            sc = W.quadtree_case(nb, nb, nlev=4, dx0=100.0, seed=1004 * 100000 + j, t_end=dur, wind=False)
+        else:
+            sc = W.member(1004 * 100000 + j, t_end=dur, scale=args.scale, name=f"request{j}")
+        reqs.append((t, sc))
+        j += 1
+    total, free = torch.cuda.mem_get_info()
+    budget = int(min(args.hbm_fraction * free, free - (8 << 30)))
+    srv = E.Server(budget, round_steps=args.round_steps, workers=args.workers)
+    for a, sc in reqs:
+        srv.submit(a, sc)
+    res = srv.run()
+    out = {
+        "metric": "served stream: p50/p99 time-to-result and aggregate cell-updates/s",
+        "value": res["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32 (zs, z_volume f64)", "data": "synthetic (seeded, PCG64(1004))",
+        "config": {"workload": f"config 4 scaled: Poisson {args.rate} req/s for {args.window} s, sizes x {args.scale}, "
+                               f"durations U(1,12) h x {args.duration_scale}, 10% quadtree",
+                   "requests": len(reqs), "hbm_budget_bytes": budget, "round_steps": args.round_steps},
+        "p50_s": res["p50_s"], "p99_s": res["p99_s"], "wall_s": res["wall_s"],
+        "cell_updates": res["cell_updates"], "max_concurrent": res["max_concurrent"],
+        "max_footprint_bytes": res["max_footprint_bytes"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +175,21 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--mode", choices=("step", "ensemble", "serve"), default="step",
+                    help="step: configuration 1 (the default line); ensemble: configuration 3; "
+                         "serve: configuration 4 (scaled by --window/--scale/--duration-scale)")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--t-end", type=float, default=10800.0)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--round-steps", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--window", type=float, default=600.0)
+    ap.add_argument("--duration-scale", type=float, default=1.0)
+    ap.add_argument("--hbm-fraction", type=float, default=0.9)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed region, run a 1 GiB elementwise copy (torch.mul by 1) three "
+                         "times: a kernel of known HBM traffic for scaling PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -103,6 +199,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "ensemble":
+        return ensemble(args)
+    if args.mode == "serve":
+        return serve(args)
     from sfincs_amd import _lib
     from sfincs_amd import workloads as W
 
@@ -142,6 +242,15 @@ def main():
     t_model, dt_model, nt_model = inst.time()
     zs = inst.get("zs")
     assert np.all(np.isfinite(zs)), "non-finite water levels"
+
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: 3 x {src.numel() * 4} bytes read and written", file=sys.stderr, flush=True)
+        del src, dst
 
     value = sizes["np"] * args.steps / secs
     mom_gbs = mom_b / (ms[1] * 1e-3) / 1e9

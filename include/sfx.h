@@ -121,7 +121,8 @@ int  sfx_model_create_regular(const sfx_params* p, int32_t nmax, int32_t mmax,
 int  sfx_model_create_quadtree(const sfx_params* p, const sfx_quadtree_input* qt, sfx_model** out);
 int  sfx_model_sizes(const sfx_model* m, sfx_sizes* s);
 /* Copy a table by its reference name (e.g. "uv_index_z_nm", "kcuv", "mask_adv", "zbuvmx",
- * "z_index_uv_md", "cuv_index_uv1", "dxrinv"); nbytes must equal the table size. */
+ * "z_index_uv_md", "cuv_index_uv1", "dxrinv"); nbytes must equal the table size.
+ * "params" copies the model's sfx_params. */
 int  sfx_model_get_table(const sfx_model* m, const char* name, void* dst, size_t nbytes);
 int  sfx_model_table_bytes(const sfx_model* m, const char* name, size_t* nbytes);
 int  sfx_model_set_bathymetry(sfx_model* m, const float* zb_active);           /* np values */
@@ -184,7 +185,8 @@ int  sfx_total_volume(sfx_handle* h, double* vol);
 /* ---- batched ensembles ---------------------------------------------------------------
  * No counterpart in the reference (it runs scenarios one at a time, docs/output.rst:19-20).
  * Members of uneven size advance together: each launch covers an (instance, tile) work
- * list, each member keeps its own adaptive dt and end time. */
+ * list, each member keeps its own adaptive dt and end time. A member must outlive the
+ * batches that hold it (remove it, or destroy the batch, before sfx_destroy). */
 typedef struct sfx_batch sfx_batch;
 int  sfx_batch_create(int32_t device, sfx_batch** out);
 int  sfx_batch_destroy(sfx_batch* b);
@@ -204,7 +206,8 @@ int  sfx_profile_steps(sfx_handle* h, int32_t nsteps, double ms[3]);
  * zsderv, zsmax), used to state the FP32 path's rounding error. */
 int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle** out);
 
-/* Autotuner: times candidate block/tile shapes on this handle's mesh and keeps the best. */
+/* Autotuner: times candidate block/tile shapes on this handle's mesh and keeps the best.
+ * Call it before the handle joins a batch (SFX_ERR_STATE otherwise). */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
 int  sfx_get_tuning(sfx_handle* h, int32_t out[4]);
 

@@ -289,6 +289,7 @@ class Instance:
         check(lib().sfx_create_ex(model.h, device, 1 if fp64 else 0, C.byref(out)), "sfx_create")
         self.h = out
         self.sizes = model.sizes()
+        self._nbatch = 0         # batches holding this instance (they keep a reference)
 
     def _n(self, name):
         s = self.sizes
@@ -358,6 +359,8 @@ class Instance:
 
     def close(self):
         if self.h:
+            if self._nbatch:
+                raise SfxError("close: the instance is still a member of a batch")
             lib().sfx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -380,10 +383,12 @@ class Batch:
     def add(self, inst, t_end):
         check(lib().sfx_batch_add(self.h, inst.h, t_end), "batch_add")
         self.members.append(inst)
+        inst._nbatch += 1
 
     def remove(self, inst):
         check(lib().sfx_batch_remove(self.h, inst.h), "batch_remove")
         self.members.remove(inst)
+        inst._nbatch -= 1
 
     def advance(self, nsteps):
         n = C.c_int32()
@@ -399,6 +404,9 @@ class Batch:
         if self.h:
             lib().sfx_batch_destroy(self.h)
             self.h = C.c_void_p()
+            for m in self.members:
+                m._nbatch -= 1
+            self.members = []
 
     def __del__(self):
         try:

@@ -51,13 +51,11 @@ constexpr int WAVE = 64;
 struct Scalars {
   double t, tend;
   float dt, min_dt, dtchk;
-  uint32_t min_dt_bits;          // atomicMin target of the momentum sweep (positive float bits)
-  int32_t nt, active, error, error_pending, cur, mom_done;
+  int32_t nt, active, error, error_pending, cur;
+  int32_t mom_done;              // a momentum sweep wrote buffer cur^1 and its per-tile minima
   int32_t itbndlast, itsrclast, itwndlast, itprcplast;
   float tauwu, tauwv, netprcp;   // uniform meteo forcing of this step
   float factime, facrel, one_minus_factime;
-  int32_t pad0;
-  unsigned long long cnt[4];     // uv-points: copied (kcuv not 1/6), dry, wet/full, partial
 };
 
 struct DevInst {
@@ -66,7 +64,12 @@ struct DevInst {
   int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
   int32_t tn, tm, ntn;           // dense tile shape and tiles along n
   int32_t count;                 // per-branch counting on
+  int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
+  // CFL reduction: each momentum tile stores the minimum of its points' time steps in
+  // blkmin[tile] (no atomics); the fold at the start of the next step reduces them.
+  float* blkmin;
+  uint32_t* blkcnt;              // per tile: uv-points copied (kcuv not 1/6), dry, wet/full, partial
   // cells
   double* zs;
   double* vol;
@@ -111,22 +114,59 @@ struct DevInst {
 
 struct Item { int32_t inst, tile; };
 
+// Dense tile shapes (cells along n x along m, BLOCK threads) the autotuner tries; the
+// first is the default.
+constexpr int TILE_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {16, 16}};
+constexpr int N_TILE_SHAPES = sizeof(TILE_SHAPES) / sizeof(TILE_SHAPES[0]);
+
+size_t dense_tiles(const DevInst& I, int tn, int tm) {
+  return (size_t)((I.nmax + tn - 1) / tn) * (size_t)((I.mmax + tm - 1) / tm);
+}
+
+// Momentum tiles of one scenario: dense tiles of the current shape, or BLOCK uv-points.
+size_t mom_tiles(const DevInst& I) {
+  return I.dense ? dense_tiles(I, I.tn, I.tm) : (size_t)((I.npuv + BLOCK - 1) / BLOCK);
+}
+
 __device__ __forceinline__ float wave_min(float v) {
   for (int o = WAVE / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, WAVE));
   return v;
 }
 
-__device__ __forceinline__ void block_min_to(float v, uint32_t* target) {
+// Block-wide minimum; the result is valid in thread 0.
+__device__ __forceinline__ float block_min(float v) {
   __shared__ float red[BLOCK / WAVE];
   v = wave_min(v);
   const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
   if (lane == 0) red[w] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    float m = red[0];
-    for (int i = 1; i < BLOCK / WAVE; ++i) m = fminf(m, red[i]);
-    if (m == m && m > 0.f) atomicMin(target, __float_as_uint(m));
-  }
+  float m = red[0];
+  for (int i = 1; i < BLOCK / WAVE; ++i) m = fminf(m, red[i]);
+  return m;
+}
+
+// Completes a momentum sweep (run by one whole block): min_dt = min(dtmax, per-tile
+// minima) as in sfincs_momentum.f90:121,542-545, and the q/uv double buffer flips.
+__device__ void fold_sweep(const DevInst& I) {
+  Scalars* S = I.sc;
+  __shared__ int sh_done;
+  if (threadIdx.x == 0) sh_done = S->mom_done;
+  __syncthreads();
+  if (!sh_done) return;
+  float v = I.c.dtmax;
+  const float* __restrict__ p = I.blkmin;
+  const int n = I.nmt;
+  int i = threadIdx.x;
+  for (; i + 3 * BLOCK < n; i += 4 * BLOCK)
+    v = fminf(fminf(v, fminf(p[i], p[i + BLOCK])), fminf(p[i + 2 * BLOCK], p[i + 3 * BLOCK]));
+  for (; i < n; i += BLOCK) v = fminf(v, p[i]);
+  v = block_min(v);
+  if (threadIdx.x == 0) { S->cur ^= 1; S->mom_done = 0; S->min_dt = v; }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fold(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids) {
+  fold_sweep(insts[ids ? ids[blockIdx.x] : blockIdx.x]);
 }
 
 // ----------------------------------------------------------------------------- forcing
@@ -148,8 +188,8 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
   __shared__ int sh_go;
   __shared__ int sh_itb0, sh_itb1;
   __shared__ float sh_tbfac;
+  fold_sweep(I);
   if (threadIdx.x == 0) {
-    if (S->mom_done) { S->cur ^= 1; S->mom_done = 0; S->min_dt = __uint_as_float(S->min_dt_bits); }
     int go = 1;
     if (mode == 0) {
       if (S->error_pending) { S->error = 1; S->error_pending = 0; S->t = (double)c.t1 + 1.0; }
@@ -222,7 +262,6 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
           }
         }
       }
-      S->min_dt_bits = __float_as_uint(c.dtmax);
       S->factime = sfx::vmin<float>(dt / c.btfilter, 1.0f);
       S->one_minus_factime = 1.0f - S->factime;
       S->facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
@@ -370,7 +409,7 @@ __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx:
 // (sfincs_domain.f90:1005-1261): missing left/right/below/above faces read the dummy
 // point (q = uv = 0), missing cross neighbours of u_ndm/u_num fall back to the point itself.
 template <typename R, bool SUBGRID>
-__device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc) {
+__device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc, float& mdt) {
   Scalars* S = I.sc;
   const int cur = S->cur;
   const R dt = (R)S->dt;
@@ -382,7 +421,6 @@ __device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc) {
   const int tn = tile % I.ntn, tmi = tile / I.ntn;
   const int n = tn * I.tn + (threadIdx.x % I.tn);
   const int m = tmi * I.tm + (threadIdx.x / I.tn);
-  float mdt = I.c.dtmax;
   int br_u = 0, br_v = 0;
   if (n < nmax && m < mmax) {
     const int c = m * nmax + n;
@@ -455,8 +493,7 @@ __device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc) {
       }
     }
   }
-  block_min_to(mdt, &S->min_dt_bits);
-  if (I.count) {
+  if (I.count && n < nmax && m < mmax) {
     atomicAdd(&bc[br_u], 1u);
     atomicAdd(&bc[br_v], 1u);
   }
@@ -464,7 +501,7 @@ __device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc) {
 
 // Indexed scenarios: one thread per uv-point, neighbours through the reference's tables.
 template <typename R, bool SUBGRID>
-__device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc) {
+__device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc, float& mdt) {
   Scalars* S = I.sc;
   const int cur = S->cur;
   const R dt = (R)S->dt;
@@ -473,7 +510,6 @@ __device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc) {
   R* q1 = (R*)I.q[cur ^ 1];
   R* u1 = (R*)I.uv[cur ^ 1];
   const int ip = tile * BLOCK + threadIdx.x;
-  float mdt = I.c.dtmax;
   int br = 0;
   const int npuv = I.npuv;
   if (ip < npuv) {
@@ -514,8 +550,7 @@ __device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc) {
       q1[ip] = q0[ip]; u1[ip] = u0[ip];
     }
   }
-  block_min_to(mdt, &S->min_dt_bits);
-  if (I.count) atomicAdd(&bc[br], 1u);
+  if (I.count && ip < npuv) atomicAdd(&bc[br], 1u);
 }
 
 template <typename R>
@@ -528,19 +563,23 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(const DevInst* __restrict__ 
   __shared__ unsigned int bc[4];
   if (threadIdx.x < 4) bc[threadIdx.x] = 0;
   __syncthreads();
+  float mdt = I.c.dtmax;
   if (indexed) {
-    if (I.subgrid) mom_idx_tile<R, true>(I, it.tile, bc);
-    else mom_idx_tile<R, false>(I, it.tile, bc);
+    if (I.subgrid) mom_idx_tile<R, true>(I, it.tile, bc, mdt);
+    else mom_idx_tile<R, false>(I, it.tile, bc, mdt);
   } else {
-    if (I.subgrid) mom_dense_tile<R, true>(I, it.tile, bc);
-    else mom_dense_tile<R, false>(I, it.tile, bc);
+    if (I.subgrid) mom_dense_tile<R, true>(I, it.tile, bc, mdt);
+    else mom_dense_tile<R, false>(I, it.tile, bc, mdt);
   }
-  __syncthreads();
+  // NaN or non-positive time steps do not take part (the reduction starts at dtmax).
+  const float m = block_min(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);
   if (threadIdx.x == 0) {
-    I.sc->mom_done = 1;
-    if (I.count)
-      for (int k = 0; k < 4; ++k)
-        if (bc[k]) atomicAdd(&I.sc->cnt[k], (unsigned long long)bc[k]);
+    I.blkmin[it.tile] = m;
+    if (it.tile == 0) I.sc->mom_done = 1;
+    if (I.count) {
+      uint32_t* cnt = I.blkcnt + 4 * (size_t)it.tile;
+      for (int k = 0; k < 4; ++k) cnt[k] += bc[k];
+    }
   }
 }
 
@@ -678,7 +717,9 @@ struct sfx_handle {
   int32_t nslot = 0;
   hipStream_t stream = nullptr;
   struct sfx_batch* solo = nullptr;
-  int tile_tn = 64, tile_tm = 4;
+  int tile_tn = TILE_SHAPES[0][0], tile_tm = TILE_SHAPES[0][1];
+  size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
+  int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;
   template <typename T> T* upload(const T* src, size_t n);
   template <typename T> T* alloc(size_t n);
@@ -956,6 +997,13 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   // tiles
   I.tn = H->tile_tn; I.tm = H->tile_tm;
   I.ntn = M->dense ? (M->nmax + I.tn - 1) / I.tn : 0;
+  I.nmt = (int32_t)mom_tiles(I);
+  H->nmt_cap = I.nmt;
+  if (M->dense)
+    for (int k = 0; k < N_TILE_SHAPES; ++k)
+      H->nmt_cap = std::max(H->nmt_cap, dense_tiles(I, TILE_SHAPES[k][0], TILE_SHAPES[k][1]));
+  I.blkmin = H->alloc<float>(H->nmt_cap);
+  I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
   H->dI = H->upload(&I, 1);
   for (void* p : H->allocs) if (!p) { set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
@@ -991,16 +1039,14 @@ int write_scalars(sfx_handle* H, const Scalars& s) {
   HIPCHK(hipMemcpy(H->dS, &s, sizeof(Scalars), hipMemcpyHostToDevice));
   return SFX_OK;
 }
-// Normalise the double buffer so that `cur` holds the latest q/uv.
+// Completes a pending momentum sweep (fold of the tile minima, buffer flip) so that
+// `cur` holds the latest q/uv and min_dt the sweep's minimum.
 int normalise(sfx_handle* H, Scalars& s) {
   int rc = read_scalars(H, s);
-  if (rc) return rc;
-  if (s.mom_done) {
-    s.cur ^= 1; s.mom_done = 0;
-    std::memcpy(&s.min_dt, &s.min_dt_bits, 4);
-    rc = write_scalars(H, s);
-  }
-  return rc;
+  if (rc || !s.mom_done) return rc;
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(BLOCK), 0, H->stream, H->dI, (const int32_t*)nullptr);
+  HIPCHK(hipGetLastError());
+  return read_scalars(H, s);
 }
 
 template <typename R>
@@ -1147,10 +1193,6 @@ extern "C" int sfx_get_time(sfx_handle* H, double* t, float* dt, int64_t* nt) {
 // -------------------------------------------------------------------------- batches / launches
 namespace {
 
-size_t dense_tiles(const DevInst& I) {
-  const int ntm = (I.mmax + I.tm - 1) / I.tm;
-  return (size_t)I.ntn * ntm;
-}
 
 // XCD-aware ordering: the hardware hands workgroup b to XCD b % 8, so item b is the
 // (b%8)-th eighth of the natural tile order; each XCD sweeps a contiguous strip and
@@ -1180,7 +1222,7 @@ int rebuild(sfx_batch* B) {
     if (fp64 != H->fp64) { set_error("a batch cannot mix FP32 and FP64 members"); return SFX_ERR_ARG; }
     const DevInst& I = H->hI;
     if (I.dense) {
-      const size_t nt = dense_tiles(I);
+      const size_t nt = mom_tiles(I);
       for (size_t t = 0; t < nt; ++t) { md.push_back(Item{(int)i, (int)t}); cd.push_back(Item{(int)i, (int)t}); }
     } else {
       const int tu = (I.npuv + BLOCK - 1) / BLOCK, tc = (I.np + BLOCK - 1) / BLOCK, tq = (I.ncuv + BLOCK - 1) / BLOCK;
@@ -1279,19 +1321,14 @@ extern "C" int sfx_compute_fluxes(sfx_handle* H, float dt, float* min_dt) {
   Scalars s;
   int rc = normalise(H, s);
   if (rc) return rc;
-  s.dt = dt; s.active = 1; s.min_dt_bits = 0;
-  std::memcpy(&s.min_dt_bits, &H->hI.c.dtmax, 4);
+  s.dt = dt; s.active = 1;
   if ((rc = write_scalars(H, s))) return rc;
   sfx_batch* B = solo(H);
   if (B->dirty && (rc = rebuild(B))) return rc;
   launch(B, 2, 0);
   HIPCHK(hipGetLastError());
-  if ((rc = read_scalars(H, s))) return rc;
-  float m;
-  std::memcpy(&m, &s.min_dt_bits, 4);
-  s.min_dt = m;
-  if ((rc = write_scalars(H, s))) return rc;
-  if (min_dt) *min_dt = m;
+  if ((rc = normalise(H, s))) return rc;
+  if (min_dt) *min_dt = s.min_dt;
   return SFX_OK;
 }
 
@@ -1355,32 +1392,23 @@ extern "C" int sfx_update(sfx_handle* H, double dtrange, sfx_step_stats* st) {
       if (!s.active || s.error || s.error_pending || !(s.t < s.tend)) break;
     }
   }
-  if ((rc = read_scalars(H, s))) return rc;
+  if ((rc = normalise(H, s))) return rc;
   if (s.error_pending) { s.error = 1; s.error_pending = 0; s.t = (double)H->hI.c.t1 + 1.0; if ((rc = write_scalars(H, s))) return rc; }
-  float m;
-  std::memcpy(&m, &s.min_dt_bits, 4);
   if (st) {
-    st->t = s.t; st->dt = s.dt; st->min_dt = m; st->nt = s.nt; st->error = s.error;
+    st->t = s.t; st->dt = s.dt; st->min_dt = s.min_dt; st->nt = s.nt; st->error = s.error;
     st->cell_updates = (double)(s.nt - nt0) * (double)H->hI.np;
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   }
   return s.error ? SFX_UNSTABLE : SFX_OK;
 }
 
-// min_dt of the last sweep must feed the next step's dt: the forcing kernel reads
-// S->min_dt, so it is refreshed from the reduction bits at the end of every momentum
-// launch by the next forcing kernel (see k_forcing: dt = alpha * min_dt).
-
 extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
   H->hI.count = on ? 1 : 0;
+  HIPCHK(hipStreamSynchronize(H->stream));
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
-  Scalars s;
-  int rc = read_scalars(H, s);
-  if (rc) return rc;
-  for (int k = 0; k < 4; ++k) s.cnt[k] = 0;
-  if ((rc = write_scalars(H, s))) return rc;
+  HIPCHK(hipMemset(H->hI.blkcnt, 0, 4 * H->nmt_cap * sizeof(uint32_t)));
   if (H->solo) H->solo->dirty = true;
   return SFX_OK;
 }
@@ -1388,10 +1416,11 @@ extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {
 extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[4]) {
   if (!H || !counts) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
-  Scalars s;
-  int rc = read_scalars(H, s);
-  if (rc) return rc;
-  for (int k = 0; k < 4; ++k) counts[k] = (int64_t)s.cnt[k];
+  HIPCHK(hipStreamSynchronize(H->stream));
+  std::vector<uint32_t> c(4 * H->nmt_cap);
+  HIPCHK(hipMemcpy(c.data(), H->hI.blkcnt, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; ++k) counts[k] = 0;
+  for (size_t i = 0; i < c.size(); ++i) counts[i % 4] += c[i];
   return SFX_OK;
 }
 
@@ -1424,6 +1453,8 @@ extern "C" int sfx_batch_destroy(sfx_batch* B) {
   if (!B) return SFX_OK;
   hipSetDevice(B->device);
   if (B->stream) hipStreamSynchronize(B->stream);
+  if (B->own_stream)
+    for (auto* H : B->members) H->in_batches--;
   if (B->exec) hipGraphExecDestroy(B->exec);
   if (B->graph) hipGraphDestroy(B->graph);
   if (B->dInsts) hipFree(B->dInsts);
@@ -1447,6 +1478,7 @@ extern "C" int sfx_batch_add(sfx_batch* B, sfx_handle* H, double t_end) {
   B->members.push_back(H);
   B->tend.push_back(t_end);
   B->dirty = true;
+  H->in_batches++;
   return SFX_OK;
 }
 
@@ -1458,6 +1490,7 @@ extern "C" int sfx_batch_remove(sfx_batch* B, sfx_handle* H) {
     if (B->members[i] == H) {
       B->members.erase(B->members.begin() + i);
       B->tend.erase(B->tend.begin() + i);
+      H->in_batches--;
       B->dirty = true;
       return SFX_OK;
     }
@@ -1507,7 +1540,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
   if (!H->hI.dense) return SFX_OK;   // indexed sweeps have one shape (a uv-point per lane)
-  const int cand[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {16, 16}};
+  if (H->in_batches) { set_error("sfx_autotune: tune a scenario before adding it to a batch"); return SFX_ERR_STATE; }
   Scalars s0;
   int rc = normalise(H, s0);
   if (rc) return rc;
@@ -1529,9 +1562,10 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
   int bi = 0;
-  for (int ci = 0; ci < (int)(sizeof(cand) / sizeof(cand[0])); ++ci) {
-    H->hI.tn = cand[ci][0]; H->hI.tm = cand[ci][1];
+  for (int ci = 0; ci < N_TILE_SHAPES; ++ci) {
+    H->hI.tn = TILE_SHAPES[ci][0]; H->hI.tm = TILE_SHAPES[ci][1];
     H->hI.ntn = (H->hI.nmax + H->hI.tn - 1) / H->hI.tn;
+    H->hI.nmt = (int32_t)mom_tiles(H->hI);
     HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
     sfx_batch* B = solo(H);
     B->dirty = true;
@@ -1555,8 +1589,9 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  H->hI.tn = cand[bi][0]; H->hI.tm = cand[bi][1];
+  H->hI.tn = TILE_SHAPES[bi][0]; H->hI.tm = TILE_SHAPES[bi][1];
   H->hI.ntn = (H->hI.nmax + H->hI.tn - 1) / H->hI.tn;
+  H->hI.nmt = (int32_t)mom_tiles(H->hI);
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
   H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm;
   if ((rc = write_scalars(H, s0))) return rc;

@@ -847,8 +847,7 @@ struct View { const void* ptr; size_t bytes; };
 template <typename T> View V(const std::vector<T>& v) { return View{v.data(), v.size() * sizeof(T)}; }
 
 bool lookup(const sfx_model* M, const std::string& k, View& out) {
-  static const char* dummy = "";
-  (void)dummy;
+  if (k == "params") { out = View{&M->p, sizeof(sfx_params)}; return true; }
 #define T(name) if (k == #name) { out = V(M->name); return true; }
   T(kcs) T(z_flags_iref) T(z_index_z_n) T(z_index_z_m)
   T(z_index_uv_md) T(z_index_uv_mu) T(z_index_uv_nd) T(z_index_uv_nu)

@@ -8,6 +8,9 @@ run_sequential: the same members one after the other, each on its own (the refer
 Server:        continuous batching -- requests are admitted as they arrive while their HBM
                footprint fits the budget, advance in the shared batch while on the GPU,
                and retire (freeing their memory) when they reach their end time.
+
+Model building (table generation, mesh construction) happens on the host before a member
+is timed; both runners time only the GPU time stepping of members already in HBM.
 """
 import concurrent.futures as cf
 import heapq
@@ -22,15 +25,36 @@ def _np(model):
     return model.sizes()["np"]
 
 
-def run_batched(scenarios, round_steps=32, device=0):
-    models = [sc.build_model() for sc in scenarios]
-    insts = [_lib.Instance(m, device) for m in models]
+def _instances(models, device, tune_min_cells):
+    insts = []
+    for m in models:
+        inst = _lib.Instance(m, device)
+        if tune_min_cells is not None and _np(m) >= tune_min_cells:
+            inst.autotune(2)
+        insts.append(inst)
+    return insts
+
+
+def _summary(models, insts, wall, done_at, keep_state):
+    cells = sum(_np(m) * inst.time()[2] for m, inst in zip(models, insts))
+    out = dict(members=len(insts), wall_s=wall, cell_updates=cells, cell_updates_per_s=cells / wall,
+               time_to_result_s=done_at,
+               p50_s=float(np.percentile(done_at, 50)), p99_s=float(np.percentile(done_at, 99)),
+               steps=[inst.time()[2] for inst in insts])
+    if keep_state:
+        out["zs"] = [inst.get("zs") for inst in insts]
+    return out
+
+
+def run_batched(models, t_ends, round_steps=32, device=0, tune_min_cells=1 << 20, keep_state=False):
+    """All members in one batch until each reaches its end time."""
+    insts = _instances(models, device, tune_min_cells)
     batch = _lib.Batch(device)
-    for sc, inst in zip(scenarios, insts):
-        batch.add(inst, sc.t_end)
-    t0 = time.perf_counter()
+    for te, inst in zip(t_ends, insts):
+        batch.add(inst, te)
     done_at = [None] * len(insts)
     active = len(insts)
+    t0 = time.perf_counter()
     while active:
         active = batch.advance(round_steps)
         now = time.perf_counter() - t0
@@ -38,33 +62,26 @@ def run_batched(scenarios, round_steps=32, device=0):
             if done_at[j] is None and batch.done(inst):
                 done_at[j] = now
     wall = time.perf_counter() - t0
-    cells = sum(_np(m) * inst.time()[2] for m, inst in zip(models, insts))
-    out = dict(members=len(insts), wall_s=wall, cell_updates=cells, cell_updates_per_s=cells / wall,
-               time_to_result_s=done_at,
-               p50_s=float(np.percentile(done_at, 50)), p99_s=float(np.percentile(done_at, 99)))
+    out = _summary(models, insts, wall, done_at, keep_state)
     batch.close()
     for i in insts:
         i.close()
     return out
 
 
-def run_sequential(scenarios, device=0):
-    models = [sc.build_model() for sc in scenarios]
-    t0 = time.perf_counter()
-    cells = 0
+def run_sequential(models, t_ends, device=0, tune_min_cells=1 << 20, keep_state=False):
+    """The same members one after the other, each to its end time with sfx_update."""
+    insts = _instances(models, device, tune_min_cells)
     done_at = []
-    for sc, m in zip(scenarios, models):
-        inst = _lib.Instance(m, device)
-        m_t1 = sc.t_end
-        # run to the member's end time (its t1)
-        rc, st = inst.update(m_t1 - inst.time()[0])
-        cells += _np(m) * st["nt"]
+    t0 = time.perf_counter()
+    for te, inst in zip(t_ends, insts):
+        inst.update(te - inst.time()[0])
         done_at.append(time.perf_counter() - t0)
-        inst.close()
     wall = time.perf_counter() - t0
-    return dict(members=len(models), wall_s=wall, cell_updates=cells, cell_updates_per_s=cells / wall,
-                time_to_result_s=done_at,
-                p50_s=float(np.percentile(done_at, 50)), p99_s=float(np.percentile(done_at, 99)))
+    out = _summary(models, insts, wall, done_at, keep_state)
+    for i in insts:
+        i.close()
+    return out
 
 
 class Server:
@@ -75,13 +92,14 @@ class Server:
     at a time, and retires finished members. Host-side model building overlaps the GPU
     work in a thread pool."""
 
-    def __init__(self, hbm_budget, round_steps=32, device=0, workers=4):
+    def __init__(self, hbm_budget, round_steps=32, device=0, workers=4, keep_state=False):
         self.budget = hbm_budget
         self.round_steps = round_steps
         self.device = device
         self.pool = cf.ThreadPoolExecutor(max_workers=workers)
         self.queue = []          # (arrival, seq, scenario)
         self.seq = 0
+        self.keep_state = keep_state
 
     def submit(self, arrival, scenario):
         heapq.heappush(self.queue, (arrival, self.seq, scenario))
@@ -92,17 +110,17 @@ class Server:
         t0 = clock()
         batch = _lib.Batch(self.device)
         pending = []             # (arrival, seq, future-model, scenario) in arrival order
-        live = {}                # seq -> (inst, model, arrival, footprint, scenario)
+        live = {}                # seq -> (inst, model, arrival, footprint, scenario, admitted)
         used = 0
         results = []
         max_live = 0
+        max_used = 0
         cells = 0
         while self.queue or pending or live:
             now = clock() - t0
             while self.queue and self.queue[0][0] <= now:
                 arrival, seq, sc = heapq.heappop(self.queue)
                 pending.append((arrival, seq, self.pool.submit(sc.build_model), sc))
-            admitted = False
             while pending:
                 arrival, seq, fut, sc = pending[0]
                 if not fut.done():
@@ -114,12 +132,12 @@ class Server:
                 pending.pop(0)
                 inst = _lib.Instance(model, self.device)
                 batch.add(inst, sc.t_end)
-                live[seq] = (inst, model, arrival, need, sc)
+                live[seq] = (inst, model, arrival, need, sc, clock() - t0)
                 used += need
-                admitted = True
             max_live = max(max_live, len(live))
+            max_used = max(max_used, used)
             if not live:
-                # nothing on the GPU: wait for the next arrival or model
+                # nothing on the GPU: wait for the next model or arrival
                 if pending:
                     cf.wait([pending[0][2]], timeout=0.05)
                 elif self.queue:
@@ -128,13 +146,15 @@ class Server:
             batch.advance(self.round_steps)
             now = clock() - t0
             for seq in list(live):
-                inst, model, arrival, need, sc = live[seq]
+                inst, model, arrival, need, sc, admitted = live[seq]
                 if batch.done(inst):
                     nt = inst.time()[2]
-                    zsmax = inst.get("zsmax")
                     cells += model.sizes()["np"] * nt
-                    results.append(dict(seq=seq, arrival=arrival, done=now, ttr=now - arrival, steps=nt,
-                                        np=model.sizes()["np"], zsmax_max=float(np.max(zsmax))))
+                    r = dict(seq=seq, arrival=arrival, admitted=admitted, done=now, ttr=now - arrival,
+                             steps=nt, np=model.sizes()["np"])
+                    if self.keep_state:
+                        r["zs"] = inst.get("zs")
+                    results.append(r)
                     batch.remove(inst)
                     inst.close()
                     model.close()
@@ -146,4 +166,4 @@ class Server:
         return dict(requests=len(results), wall_s=wall, cell_updates=cells, cell_updates_per_s=cells / wall,
                     p50_s=float(np.percentile(ttr, 50)) if ttr else None,
                     p99_s=float(np.percentile(ttr, 99)) if ttr else None,
-                    max_concurrent=max_live, results=results)
+                    max_concurrent=max_live, max_footprint_bytes=max_used, results=results)

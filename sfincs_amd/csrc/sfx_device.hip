@@ -47,6 +47,15 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 
+// Device arrays are reached through pointers stored in DevInst; used as generic pointers
+// every access becomes a flat load, which the compiler must wait for one group at a time.
+// G() returns the pointer in the global address space, so the accesses become global
+// loads and stores that can all be in flight together. Pointers kept in local variables
+// are held as gptr<T> for the same reason.
+using sfx::gptr;
+using sfx::ldf4;
+template <typename T> __device__ __forceinline__ gptr<T> G(T* p) { return (gptr<T>)p; }
+
 // Device-resident step scalars of one scenario (t, dt, CFL reduction, forcing scalars).
 struct Scalars {
   double t, tend;
@@ -62,7 +71,8 @@ struct DevInst {
   Consts c;
   int32_t dense, subgrid, fp64;
   int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
-  int32_t tn, tm, ntn;           // dense tile shape and tiles along n
+  int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
+  int32_t sk, nstrip;            // dense momentum: rows per wave of the strip march, strips
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
@@ -114,18 +124,30 @@ struct DevInst {
 
 struct Item { int32_t inst, tile; };
 
-// Dense tile shapes (cells along n x along m, BLOCK threads) the autotuner tries; the
-// first is the default.
-constexpr int TILE_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {16, 16}};
-constexpr int N_TILE_SHAPES = sizeof(TILE_SHAPES) / sizeof(TILE_SHAPES[0]);
+// Dense continuity tile shapes (cells along n x along m, BLOCK threads) and strip-march
+// lengths (rows per wave) of the dense momentum kernel that the autotuner tries; the
+// first of each is the default.
+constexpr int CONT_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}};
+constexpr int N_CONT_SHAPES = sizeof(CONT_SHAPES) / sizeof(CONT_SHAPES[0]);
+constexpr int STRIP_ROWS[] = {8, 4, 16, 32};
+constexpr int N_STRIP_ROWS = sizeof(STRIP_ROWS) / sizeof(STRIP_ROWS[0]);
+constexpr int STRIP_COLS = WAVE - 2;             // columns owned by one wave
 
-size_t dense_tiles(const DevInst& I, int tn, int tm) {
-  return (size_t)((I.nmax + tn - 1) / tn) * (size_t)((I.mmax + tm - 1) / tm);
+size_t strip_tiles(const DevInst& I, int sk) {
+  const int nstrip = (I.nmax + STRIP_COLS - 1) / STRIP_COLS;
+  const int rows = sk * (BLOCK / WAVE);
+  return (size_t)nstrip * (size_t)((I.mmax + rows - 1) / rows);
 }
 
-// Momentum tiles of one scenario: dense tiles of the current shape, or BLOCK uv-points.
+// Momentum tiles of one scenario: strip-march blocks, or BLOCK uv-points (indexed).
 size_t mom_tiles(const DevInst& I) {
-  return I.dense ? dense_tiles(I, I.tn, I.tm) : (size_t)((I.npuv + BLOCK - 1) / BLOCK);
+  return I.dense ? strip_tiles(I, I.sk) : (size_t)((I.npuv + BLOCK - 1) / BLOCK);
+}
+
+// Continuity tiles: tn x tm cells, or BLOCK cells (indexed).
+size_t cont_tiles(const DevInst& I) {
+  return I.dense ? (size_t)((I.nmax + I.tn - 1) / I.tn) * (size_t)((I.mmax + I.tm - 1) / I.tm)
+                 : (size_t)((I.np + BLOCK - 1) / BLOCK);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -148,13 +170,13 @@ __device__ __forceinline__ float block_min(float v) {
 // Completes a momentum sweep (run by one whole block): min_dt = min(dtmax, per-tile
 // minima) as in sfincs_momentum.f90:121,542-545, and the q/uv double buffer flips.
 __device__ void fold_sweep(const DevInst& I) {
-  Scalars* S = I.sc;
+  const auto S = G(I.sc);
   __shared__ int sh_done;
   if (threadIdx.x == 0) sh_done = S->mom_done;
   __syncthreads();
   if (!sh_done) return;
   float v = I.c.dtmax;
-  const float* __restrict__ p = I.blkmin;
+  const auto p = G(I.blkmin);
   const int n = I.nmt;
   int i = threadIdx.x;
   for (; i + 3 * BLOCK < n; i += 4 * BLOCK)
@@ -183,7 +205,7 @@ template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids,
                                                    int mode) {
   const DevInst& I = insts[ids[blockIdx.x]];
-  Scalars* S = I.sc;
+  const auto S = G(I.sc);
   const Consts& c = I.c;
   __shared__ int sh_go;
   __shared__ int sh_itb0, sh_itb1;
@@ -210,15 +232,15 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
       const float dt = S->dt;
       if (c.flags & sfx::F_WIND) {
         for (int itw = max(S->itwndlast, 2); itw <= I.ntwnd; ++itw) {
-          if ((double)I.twnd[itw - 1] > t) {
-            const float twfac = (float)((t - (double)I.twnd[itw - 2]) / (double)(I.twnd[itw - 1] - I.twnd[itw - 2]));
-            const float vmag = I.wndmag[itw - 2] * (1.0f - twfac) + I.wndmag[itw - 1] * twfac;
-            float dr0 = f_modulo(I.wnddir[itw - 2], 2 * c.pi);
-            float dr1 = f_modulo(I.wnddir[itw - 1], 2 * c.pi);
+          if ((double)G(I.twnd)[itw - 1] > t) {
+            const float twfac = (float)((t - (double)G(I.twnd)[itw - 2]) / (double)(G(I.twnd)[itw - 1] - G(I.twnd)[itw - 2]));
+            const float vmag = G(I.wndmag)[itw - 2] * (1.0f - twfac) + G(I.wndmag)[itw - 1] * twfac;
+            float dr0 = f_modulo(G(I.wnddir)[itw - 2], 2 * c.pi);
+            float dr1 = f_modulo(G(I.wnddir)[itw - 1], 2 * c.pi);
             if (dr1 > dr0 + c.pi) dr0 = dr0 + 2 * c.pi;
             else if (dr0 > dr1 + c.pi) dr1 = dr1 + 2 * c.pi;
             const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
-            const float cd = I.cdval[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
+            const float cd = G(I.cdval)[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
             S->tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
             S->tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
             S->itwndlast = itw - 1;
@@ -229,33 +251,33 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
       if (c.flags & sfx::F_PRECIP) {
         int itp;
         for (itp = S->itprcplast; itp <= I.ntprcp; ++itp)
-          if ((double)I.tprcpt[itp - 1] > t) break;
+          if ((double)G(I.tprcpt)[itp - 1] > t) break;
         if (itp > I.ntprcp) itp = I.ntprcp;
         if (itp < 2) itp = 2;   // series start at or before t0 (checked at initialisation)
-        const float twfac = (float)((t - (double)I.tprcpt[itp - 2]) / (double)(I.tprcpt[itp - 1] - I.tprcpt[itp - 2]));
-        S->netprcp = (I.tprcpv[itp - 2] * (1.0f - twfac) + I.tprcpv[itp - 1] * twfac) / (float)(1000 * 3600);
+        const float twfac = (float)((t - (double)G(I.tprcpt)[itp - 2]) / (double)(G(I.tprcpt)[itp - 1] - G(I.tprcpt)[itp - 2]));
+        S->netprcp = (G(I.tprcpv)[itp - 2] * (1.0f - twfac) + G(I.tprcpv)[itp - 1] * twfac) / (float)(1000 * 3600);
         S->itprcplast = itp - 1;
       }
       if (I.nbnd > 0) {
         int itb0 = 1, itb1 = 1;
         float tb = 0.f;
         const int nt = I.ntbnd;
-        if ((double)I.t_bnd[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = I.t_bnd[0]; }
-        else if ((double)I.t_bnd[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = I.t_bnd[nt - 1]; }
+        if ((double)G(I.t_bnd)[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = G(I.t_bnd)[0]; }
+        else if ((double)G(I.t_bnd)[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = G(I.t_bnd)[nt - 1]; }
         else {
           for (int itb = S->itbndlast; itb <= nt; ++itb)
-            if ((double)I.t_bnd[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; S->itbndlast = itb - 1; break; }
+            if ((double)G(I.t_bnd)[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; S->itbndlast = itb - 1; break; }
         }
         sh_itb0 = itb0; sh_itb1 = itb1;
-        sh_tbfac = (tb - I.t_bnd[itb0 - 1]) / sfx::vmax<float>(I.t_bnd[itb1 - 1] - I.t_bnd[itb0 - 1], 1.0e-6f);
+        sh_tbfac = (tb - G(I.t_bnd)[itb0 - 1]) / sfx::vmax<float>(G(I.t_bnd)[itb1 - 1] - G(I.t_bnd)[itb0 - 1], 1.0e-6f);
       }
       if (I.nsrc > 0) {
         for (int it = max(S->itsrclast, 2); it <= I.ntsrc; ++it) {
-          if ((double)I.tsrc[it - 1] > t) {
+          if ((double)G(I.tsrc)[it - 1] > t) {
             for (int s = 0; s < I.nsrc; ++s) {
-              const float q0v = I.qsrc[(it - 2) * I.nsrc + s], q1v = I.qsrc[(it - 1) * I.nsrc + s];
-              I.qtsrc[s] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)I.tsrc[it - 2]) /
-                                                     (double)(I.tsrc[it - 1] - I.tsrc[it - 2]));
+              const float q0v = G(I.qsrc)[(it - 2) * I.nsrc + s], q1v = G(I.qsrc)[(it - 1) * I.nsrc + s];
+              G(I.qtsrc)[s] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)G(I.tsrc)[it - 2]) /
+                                                     (double)(G(I.tsrc)[it - 1] - G(I.tsrc)[it - 2]));
             }
             S->itsrclast = it - 1;
             break;
@@ -272,22 +294,22 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
   if (!sh_go) return;
   // water level at boundary points (update_boundary_points, :728-756)
   for (int ib = threadIdx.x; ib < I.nbnd; ib += blockDim.x) {
-    const float a = I.zs_bnd[(sh_itb0 - 1) * I.nbnd + ib], b = I.zs_bnd[(sh_itb1 - 1) * I.nbnd + ib];
-    I.zst_bnd[ib] = a + (b - a) * sh_tbfac;
+    const float a = G(I.zs_bnd)[(sh_itb0 - 1) * I.nbnd + ib], b = G(I.zs_bnd)[(sh_itb1 - 1) * I.nbnd + ib];
+    G(I.zst_bnd)[ib] = a + (b - a) * sh_tbfac;
   }
   __syncthreads();
   const double t = S->t;
   // update_boundary_conditions (:789-917)
   for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
-    const int nmb = I.gb_cell[ib];
-    const int k = I.kcs[nmb] & 0x7f;
+    const int nmb = G(I.gb_cell)[ib];
+    const int k = G(I.kcs)[nmb] & 0x7f;
     if (k == 2) {
       double zst;
       if (I.nbnd > 1) {
-        const float f = I.gb_fac[ib];
-        zst = (double)(I.zst_bnd[I.gb_ind1[ib] - 1] * f + I.zst_bnd[I.gb_ind2[ib] - 1] * (1.0f - f));
+        const float f = G(I.gb_fac)[ib];
+        zst = (double)(G(I.zst_bnd)[G(I.gb_ind1)[ib] - 1] * f + G(I.zst_bnd)[G(I.gb_ind2)[ib] - 1] * (1.0f - f));
       } else {
-        zst = (double)I.zst_bnd[0];
+        zst = (double)G(I.zst_bnd)[0];
       }
       float zb0, zb1;
       if (t < (double)(c.tspinup - 1.0e-3f)) {
@@ -299,35 +321,35 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
         zb0 = (float)zst;
         zb1 = (float)zst;
       }
-      const float floorz = I.subgrid ? I.z_zmin[nmb] : I.zb[nmb];
-      I.zsb0[ib] = sfx::vmax<float>(zb0, floorz);
-      I.zsb[ib] = sfx::vmax<float>(zb1, floorz);
+      const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
+      G(I.zsb0)[ib] = sfx::vmax<float>(zb0, floorz);
+      G(I.zsb)[ib] = sfx::vmax<float>(zb1, floorz);
     } else if (k == 6) {
-      I.zs[nmb] = I.zs[I.gb_nmi[ib]];
+      G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]];
     }
   }
   __syncthreads();
   // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
-  R* q = (R*)I.q[S->cur];
-  R* uv = (R*)I.uv[S->cur];
-  R* uvmean = (R*)I.uvmean;
-  R* zsmax = (R*)I.zsmax;
+  const auto q = G((R*)I.q[S->cur]);
+  const auto uv = G((R*)I.uv[S->cur]);
+  const auto uvmean = G((R*)I.uvmean);
+  const auto zsmax = G((R*)I.zsmax);
   const int nl = c.nl;
   for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
-    const int nmb = I.k2_nmb[ib];
-    if ((I.kcs[nmb] & 0x7f) == 6) continue;
-    const int indb = I.k2_ib[ib];
-    const int nmi = I.k2_nmi[ib];
-    const int ip = I.k2_slot[ib];
-    const R zsnmi = (R)I.zs[nmi];
-    R zsnmb = I.zsb[indb];
-    R zs0nmb = I.zsb0[indb];
+    const int nmb = G(I.k2_nmb)[ib];
+    if ((G(I.kcs)[nmb] & 0x7f) == 6) continue;
+    const int indb = G(I.k2_ib)[ib];
+    const int nmi = G(I.k2_nmi)[ib];
+    const int ip = G(I.k2_slot)[ib];
+    const R zsnmi = (R)G(I.zs)[nmi];
+    R zsnmb = G(I.zsb)[indb];
+    R zs0nmb = G(I.zsb0)[indb];
     R hnmb;
     const R huth = c.huthresh;
     if (I.subgrid) {
       const R zsuv = sfx::vmax<R>(zsnmb, zsnmi);
-      const float4 s4 = I.uv_s4[ip];
-      const R zmaxu = s4.x, zminu = I.uv_zmin[ip];
+      const float4 s4 = ldf4(G(I.uv_s4), ip);
+      const R zmaxu = s4.x, zminu = G(I.uv_zmin)[ip];
       R depthuv;
       if (zsuv >= zmaxu - R(1.0e-4f)) {
         depthuv = (R)s4.y + zsuv;
@@ -335,191 +357,248 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
         const R dzuv = (zmaxu - zminu) / (R)(nl - 1);
         const int iuv = (int)((zsuv - zminu) / dzuv) + 1;
         const R facint = (zsuv - (zminu + (R)(iuv - 1) * dzuv)) / dzuv;
-        const float4* lv = I.uv_lev + (size_t)ip * nl;
-        depthuv = (R)lv[iuv - 1].x + ((R)lv[iuv].x - (R)lv[iuv - 1].x) * facint;
+        const auto lv = G(I.uv_lev) + (size_t)ip * nl;
+        depthuv = (R)ldf4(lv, iuv - 1).x + ((R)ldf4(lv, iuv).x - (R)ldf4(lv, iuv - 1).x) * facint;
       } else {
         depthuv = R(0);
       }
       hnmb = sfx::vmax<R>(depthuv, huth);
-      zsnmb = sfx::vmax<R>(zsnmb, (R)I.z_zmin[nmb]);
-      zs0nmb = sfx::vmax<R>(zs0nmb, (R)I.z_zmin[nmb]);
+      zsnmb = sfx::vmax<R>(zsnmb, (R)G(I.z_zmin)[nmb]);
+      zs0nmb = sfx::vmax<R>(zs0nmb, (R)G(I.z_zmin)[nmb]);
     } else {
-      hnmb = sfx::vmax<R>(R(0.5f) * (zsnmb + zsnmi) - (R)I.k2_zbuv[ib], huth);
-      zsnmb = sfx::vmax<R>(zsnmb, (R)I.zb[nmb]);
-      zs0nmb = sfx::vmax<R>(zs0nmb, (R)I.zb[nmb]);
+      hnmb = sfx::vmax<R>(R(0.5f) * (zsnmb + zsnmi) - (R)G(I.k2_zbuv)[ib], huth);
+      zsnmb = sfx::vmax<R>(zsnmb, (R)G(I.zb)[nmb]);
+      zs0nmb = sfx::vmax<R>(zs0nmb, (R)G(I.zb)[nmb]);
     }
     R qb;
-    if (hnmb < huth + R(1.0e-6f) || I.kcuv[ip] == 3) {
+    if (hnmb < huth + R(1.0e-6f) || G(I.kcuv)[ip] == 3) {
       qb = R(0);
       q[ip] = R(0); uv[ip] = R(0); uvmean[ib] = R(0);
     } else {
       const R g = c.g;
       const R ui = sfx::rsqrt_<R>(g / hnmb) * (zsnmb - zs0nmb);
-      const int dir = I.k2_dir[ib];
+      const int dir = G(I.k2_dir)[ib];
       const R ub = (R)dir * (R(2) * ui - sfx::rsqrt_<R>(g / hnmb) * (zsnmi - zs0nmb));
       qb = ub * hnmb + uvmean[ib];
       if (I.subgrid) {
-        if (I.vol[nmi] <= 0.0) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
-        if (zsnmb - (R)I.z_zmin[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
+        if (G(I.vol)[nmi] <= 0.0) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
+        if (zsnmb - (R)G(I.z_zmin)[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
       } else {
-        if (zsnmi - (R)I.zb[nmi] <= huth) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
-        if (zsnmb - (R)I.zb[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
+        if (zsnmi - (R)G(I.zb)[nmi] <= huth) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
+        if (zsnmb - (R)G(I.zb)[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
       }
       q[ip] = qb;
       uv[ip] = sfx::vmax<R>(sfx::vmin<R>(qb / hnmb, R(4)), R(-4));
     }
     if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)S->factime * q[ip] + (R)S->facrel * (R)S->one_minus_factime * uvmean[ib];
     else uvmean[ib] = R(0);
-    I.zs[nmb] = (double)I.zsb[indb];
-    if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], I.zs[nmb]);
+    G(I.zs)[nmb] = (double)G(I.zsb)[indb];
+    if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], G(I.zs)[nmb]);
   }
 }
 
 // ----------------------------------------------------------------------------- momentum
-template <typename R, bool SUBGRID>
-__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, R dt, R* q1, R* uv1,
+template <typename R, bool SUBGRID, typename QP>
+__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, R dt, QP q1, QP uv1,
                                           float& mdt, int& br) {
   sfx::UvTables tb;
   if (SUBGRID) {
-    tb.zmin = I.uv_zmin[slot];
+    tb.zmin = G(I.uv_zmin)[slot];
   } else {
-    tb.zbuvmx = I.zbuvmx[slot];
+    tb.zbuvmx = G(I.zbuvmx)[slot];
   }
   R zsu;
   if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
-    q1[slot] = R(0); uv1[slot] = R(0); I.kfuv[slot] = 0;
+    q1[slot] = R(0); uv1[slot] = R(0); G(I.kfuv)[slot] = 0;
     br = 1;
     return;
   }
   if (SUBGRID) {
-    const float4 s4 = I.uv_s4[slot];
-    tb.zmax = s4.x; tb.havg_zmax = s4.y; tb.nrep_zmax = s4.z; tb.navg_w = s4.w; tb.fnfit = I.uv_fnfit[slot];
+    const float4 s4 = ldf4(G(I.uv_s4), slot);
+    tb.zmax = s4.x; tb.havg_zmax = s4.y; tb.nrep_zmax = s4.z; tb.navg_w = s4.w; tb.fnfit = G(I.uv_fnfit)[slot];
   } else {
-    tb.gn2uv = I.gn2uv[slot];
+    tb.gn2uv = G(I.gn2uv)[slot];
   }
-  const sfx::UvOut<R> o = sfx::uv_update<R, SUBGRID>(in, tb, I.c, zsu, SUBGRID ? I.uv_lev + (size_t)slot * I.c.nl : nullptr,
-                                                     I.x73, I.pw, dt);
-  q1[slot] = o.q; uv1[slot] = o.uv; I.kfuv[slot] = 1;
+  const sfx::UvOut<R> o = sfx::uv_update<R, SUBGRID>(in, tb, I.c, zsu,
+                                                     SUBGRID ? G(I.uv_lev) + (size_t)slot * I.c.nl : G(I.uv_lev),
+                                                     G(I.x73), sfx::pow_tables_g(I.pw), dt);
+  q1[slot] = o.q; uv1[slot] = o.uv; G(I.kfuv)[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
   br = o.branch;
 }
 
-// Dense scenarios: one thread per cell handles the u-point on its right and the v-point
-// above it. Neighbours follow the reference's tables for a full rectangle
-// (sfincs_domain.f90:1005-1261): missing left/right/below/above faces read the dummy
-// point (q = uv = 0), missing cross neighbours of u_ndm/u_num fall back to the point itself.
+// Dense scenarios, strip march. A wave covers columns n = n0-1 .. n0+62 of one strip
+// (lane l -> n0+l-1) and marches over K rows m0 .. m0+K-1; lanes 1..62 own their column,
+// lanes 0 and 63 only supply the neighbour values of the owned lanes. Each lane keeps rows
+// m-1, m, m+1 (and the prefetched row m+2) of its column in registers and takes the n-1 /
+// n+1 values from its neighbour lanes, so every element is read from memory once per
+// strip, plus the two halo lanes and the halo rows at the ends of the march.
+//
+// Neighbours follow the reference's tables for a full rectangle (sfincs_domain.f90:
+// 1005-1261): missing left/right/below/above faces read the dummy point (q = uv = 0),
+// missing cross neighbours of u_ndm/u_num fall back to the point itself.
+template <typename R>
+struct Row {
+  R uq, uu, vq, vu;     // q/uv of the u-point (n,m) and the v-point (n,m) (dummy where absent)
+  int ku, kv;           // kcuv of the two points
+  double z, vol;        // zs and z_volume of cell (n,m)
+  R zd;                 // zsderv (subgrid) or zb (regular) of cell (n,m)
+  R r_uq, r_uu, r_vq, r_vu, r_zd;   // from column n+1
+  int r_kv;
+  double r_z, r_vol;
+  R l_uu, l_vq, l_vu;               // from column n-1
+  int l_kv;
+};
+
+template <typename R, bool SUBGRID, typename QP>
+__device__ __forceinline__ void strip_raw(const DevInst& I, QP q0, QP u0, int n, int m, bool colok, Row<R>& w,
+                                          R dq, R du, int dk) {
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np;
+  const bool ok = colok && m >= 0 && m < mmax;
+  const int c = ok ? m * nmax + n : 0;
+  const bool hu = ok && m < mmax - 1;
+  const bool hv = ok && n < nmax - 1;
+  w.uq = hu ? q0[c] : dq; w.uu = hu ? u0[c] : du; w.ku = hu ? (int)G(I.kcuv)[c] : dk;
+  w.vq = hv ? q0[np + c] : dq; w.vu = hv ? u0[np + c] : du; w.kv = hv ? (int)G(I.kcuv)[np + c] : dk;
+  w.z = ok ? G(I.zs)[c] : 0.0;
+  if (SUBGRID) {
+    w.vol = ok ? G(I.vol)[c] : 0.0;
+    w.zd = ok ? G((const R*)I.zsderv)[c] : R(0);
+  } else {
+    w.vol = 0.0;
+    w.zd = ok ? (R)G(I.zb)[c] : R(0);
+  }
+}
+
+// Values of the neighbour columns; executed by every lane of the wave.
 template <typename R, bool SUBGRID>
-__device__ void mom_dense_tile(const DevInst& I, int tile, unsigned int* bc, float& mdt) {
-  Scalars* S = I.sc;
+__device__ __forceinline__ void strip_nbrs(Row<R>& w) {
+  w.r_uq = __shfl_down(w.uq, 1, WAVE); w.r_uu = __shfl_down(w.uu, 1, WAVE);
+  w.r_vq = __shfl_down(w.vq, 1, WAVE); w.r_vu = __shfl_down(w.vu, 1, WAVE);
+  w.r_kv = __shfl_down(w.kv, 1, WAVE);
+  w.r_z = __shfl_down(w.z, 1, WAVE);
+  w.r_vol = SUBGRID ? __shfl_down(w.vol, 1, WAVE) : 0.0;
+  w.r_zd = __shfl_down(w.zd, 1, WAVE);
+  w.l_uu = __shfl_up(w.uu, 1, WAVE); w.l_vq = __shfl_up(w.vq, 1, WAVE); w.l_vu = __shfl_up(w.vu, 1, WAVE);
+  w.l_kv = __shfl_up(w.kv, 1, WAVE);
+}
+
+template <typename R, bool SUBGRID>
+__device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, float& mdt) {
+  const auto S = G(I.sc);
   const int cur = S->cur;
   const R dt = (R)S->dt;
-  const R* __restrict__ q0 = (const R*)I.q[cur];
-  const R* __restrict__ u0 = (const R*)I.uv[cur];
-  R* q1 = (R*)I.q[cur ^ 1];
-  R* u1 = (R*)I.uv[cur ^ 1];
+  const auto q0 = G((const R*)I.q[cur]);
+  const auto u0 = G((const R*)I.uv[cur]);
+  const auto q1 = G((R*)I.q[cur ^ 1]);
+  const auto u1 = G((R*)I.uv[cur ^ 1]);
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
-  const int tn = tile % I.ntn, tmi = tile / I.ntn;
-  const int n = tn * I.tn + (threadIdx.x % I.tn);
-  const int m = tmi * I.tm + (threadIdx.x / I.tn);
-  int br_u = 0, br_v = 0;
-  if (n < nmax && m < mmax) {
-    const int c = m * nmax + n;
-    const R tauwu = (R)S->tauwu, tauwv = (R)S->tauwv;
-    const double zs_c = I.zs[c];
-    // u-point between c and c+nmax
-    if (m < mmax - 1) {
-      const int s = c;
-      const int kc = I.kcuv[s];
-      if (kc == 1 || kc == 6) {
-        sfx::UvIn<R> in;
-        in.zs_nm = zs_c; in.zs_nmu = I.zs[c + nmax];
-        const int s_nmd = m >= 1 ? s - nmax : dm;
-        const int s_nmu = m + 1 < mmax - 1 ? s + nmax : dm;
-        const int s_ndm = n >= 1 ? s - 1 : s;
-        const int s_num = n < nmax - 1 ? s + 1 : s;
-        const int v_nm = n < nmax - 1 ? np + c : dm;
-        const int v_ndm = n >= 1 ? np + c - 1 : dm;
-        const int v_nmu = n < nmax - 1 ? np + c + nmax : dm;
-        const int v_ndmu = n >= 1 ? np + c + nmax - 1 : dm;
-        in.qx_nm = q0[s]; in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
-        in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
-        in.uu_nm = u0[s]; in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
-        in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
-        in.kcuv_nmd = I.kcuv[s_nmd]; in.kcuv_nmu = I.kcuv[s_nmu];
-        in.kfuv = I.kfuv[s]; in.mask_adv = I.mask_adv[s]; in.idir = 0; in.iref = 1; in.itype = 0;
-        in.tauwu = tauwu; in.tauwv = tauwv;
-        if (SUBGRID) {
-          in.vol_nm = I.vol[c]; in.vol_nmu = I.vol[c + nmax];
-          in.zsderv_nm = ((const R*)I.zsderv)[c]; in.zsderv_nmu = ((const R*)I.zsderv)[c + nmax];
+  const int K = I.sk;
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int strip = tile % I.nstrip, seg = tile / I.nstrip;
+  const int n = strip * (WAVE - 2) + lane - 1;
+  const bool colok = n >= 0 && n < nmax;
+  const bool own = colok && lane >= 1 && lane <= WAVE - 2;
+  const int m0 = (seg * (BLOCK / WAVE) + wv) * K;
+  const int m1 = min(m0 + K, mmax);
+  if (m0 >= mmax) return;                              // uniform over the wave
+  const R dq = q0[dm], du = u0[dm];
+  const int dk = G(I.kcuv)[dm];
+  const R tauwu = (R)S->tauwu, tauwv = (R)S->tauwv;
+  Row<R> A, B, C, D;
+  strip_raw<R, SUBGRID>(I, q0, u0, n, m0 - 1, colok, A, dq, du, dk);
+  strip_raw<R, SUBGRID>(I, q0, u0, n, m0, colok, B, dq, du, dk);
+  strip_raw<R, SUBGRID>(I, q0, u0, n, m0 + 1, colok, C, dq, du, dk);
+  strip_nbrs<R, SUBGRID>(A);
+  strip_nbrs<R, SUBGRID>(B);
+  for (int m = m0; m < m1; ++m) {
+    strip_nbrs<R, SUBGRID>(C);
+    strip_raw<R, SUBGRID>(I, q0, u0, n, m + 2, colok, D, dq, du, dk);   // prefetch
+    if (own) {
+      const int c = m * nmax + n;
+      // u-point between cells (n,m) and (n,m+1)
+      if (m < mmax - 1) {
+        if (B.ku == 1 || B.ku == 6) {
+          sfx::UvIn<R> in;
+          in.zs_nm = B.z; in.zs_nmu = C.z;
+          in.qx_nm = B.uq; in.uu_nm = B.uu;
+          in.qx_nmd = A.uq; in.uu_nmd = A.uu; in.kcuv_nmd = A.ku;
+          in.qx_nmu = C.uq; in.uu_nmu = C.uu; in.kcuv_nmu = C.ku;
+          in.uu_ndm = n >= 1 ? B.l_uu : B.uu;
+          in.uu_num = n < nmax - 1 ? B.r_uu : B.uu;
+          in.qy_nm = B.vq; in.v_nm = B.vu;
+          in.qy_ndm = B.l_vq; in.v_ndm = B.l_vu;
+          in.qy_nmu = C.vq; in.v_nmu = C.vu;
+          in.qy_ndmu = C.l_vq; in.v_ndmu = C.l_vu;
+          in.kfuv = G(I.kfuv)[c]; in.mask_adv = G(I.mask_adv)[c]; in.idir = 0; in.iref = 1; in.itype = 0;
+          in.tauwu = tauwu; in.tauwv = tauwv;
+          if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = C.vol; in.zsderv_nm = B.zd; in.zsderv_nmu = C.zd; }
+          else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)C.zd; }
+          int br = 0;
+          finish_uv<R, SUBGRID>(I, c, in, dt, q1, u1, mdt, br);
+          cnt[br]++;
         } else {
-          in.zb_nm = I.zb[c]; in.zb_nmu = I.zb[c + nmax];
+          q1[c] = B.uq; u1[c] = B.uu;
+          cnt[0]++;
         }
-        finish_uv<R, SUBGRID>(I, s, in, dt, q1, u1, mdt, br_u);
       } else {
-        q1[s] = q0[s]; u1[s] = u0[s];
+        cnt[0]++;
+      }
+      // v-point between cells (n,m) and (n+1,m)
+      if (n < nmax - 1) {
+        const int s = np + c;
+        if (B.kv == 1 || B.kv == 6) {
+          sfx::UvIn<R> in;
+          in.zs_nm = B.z; in.zs_nmu = B.r_z;
+          in.qx_nm = B.vq; in.uu_nm = B.vu;
+          in.qx_nmd = B.l_vq; in.uu_nmd = B.l_vu; in.kcuv_nmd = B.l_kv;
+          in.qx_nmu = B.r_vq; in.uu_nmu = B.r_vu; in.kcuv_nmu = B.r_kv;
+          in.uu_ndm = m >= 1 ? A.vu : B.vu;
+          in.uu_num = m < mmax - 1 ? C.vu : B.vu;
+          in.qy_nm = B.uq; in.v_nm = B.uu;
+          in.qy_ndm = A.uq; in.v_ndm = A.uu;
+          in.qy_nmu = B.r_uq; in.v_nmu = B.r_uu;
+          in.qy_ndmu = A.r_uq; in.v_ndmu = A.r_uu;
+          in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s]; in.idir = 1; in.iref = 1; in.itype = 0;
+          in.tauwu = tauwu; in.tauwv = tauwv;
+          if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = B.r_vol; in.zsderv_nm = B.zd; in.zsderv_nmu = B.r_zd; }
+          else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)B.r_zd; }
+          int br = 0;
+          finish_uv<R, SUBGRID>(I, s, in, dt, q1, u1, mdt, br);
+          cnt[br]++;
+        } else {
+          q1[s] = B.vq; u1[s] = B.vu;
+          cnt[0]++;
+        }
+      } else {
+        cnt[0]++;
       }
     }
-    // v-point between c and c+1
-    if (n < nmax - 1) {
-      const int s = np + c;
-      const int kc = I.kcuv[s];
-      if (kc == 1 || kc == 6) {
-        sfx::UvIn<R> in;
-        in.zs_nm = zs_c; in.zs_nmu = I.zs[c + 1];
-        const int s_nmd = n >= 1 ? s - 1 : dm;
-        const int s_nmu = n + 1 < nmax - 1 ? s + 1 : dm;
-        const int s_ndm = m >= 1 ? s - nmax : s;
-        const int s_num = m < mmax - 1 ? s + nmax : s;
-        const int v_nm = m < mmax - 1 ? c : dm;
-        const int v_ndm = m >= 1 ? c - nmax : dm;
-        const int v_nmu = m < mmax - 1 ? c + 1 : dm;
-        const int v_ndmu = m >= 1 ? c + 1 - nmax : dm;
-        in.qx_nm = q0[s]; in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
-        in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
-        in.uu_nm = u0[s]; in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
-        in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
-        in.kcuv_nmd = I.kcuv[s_nmd]; in.kcuv_nmu = I.kcuv[s_nmu];
-        in.kfuv = I.kfuv[s]; in.mask_adv = I.mask_adv[s]; in.idir = 1; in.iref = 1; in.itype = 0;
-        in.tauwu = tauwu; in.tauwv = tauwv;
-        if (SUBGRID) {
-          in.vol_nm = I.vol[c]; in.vol_nmu = I.vol[c + 1];
-          in.zsderv_nm = ((const R*)I.zsderv)[c]; in.zsderv_nmu = ((const R*)I.zsderv)[c + 1];
-        } else {
-          in.zb_nm = I.zb[c]; in.zb_nmu = I.zb[c + 1];
-        }
-        finish_uv<R, SUBGRID>(I, s, in, dt, q1, u1, mdt, br_v);
-      } else {
-        q1[s] = q0[s]; u1[s] = u0[s];
-      }
-    }
-  }
-  if (I.count && n < nmax && m < mmax) {
-    atomicAdd(&bc[br_u], 1u);
-    atomicAdd(&bc[br_v], 1u);
+    A = B; B = C; C = D;
   }
 }
 
 // Indexed scenarios: one thread per uv-point, neighbours through the reference's tables.
 template <typename R, bool SUBGRID>
-__device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc, float& mdt) {
-  Scalars* S = I.sc;
+__device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, float& mdt) {
+  const auto S = G(I.sc);
   const int cur = S->cur;
   const R dt = (R)S->dt;
-  const R* __restrict__ q0 = (const R*)I.q[cur];
-  const R* __restrict__ u0 = (const R*)I.uv[cur];
-  R* q1 = (R*)I.q[cur ^ 1];
-  R* u1 = (R*)I.uv[cur ^ 1];
+  const auto q0 = G((const R*)I.q[cur]);
+  const auto u0 = G((const R*)I.uv[cur]);
+  const auto q1 = G((R*)I.q[cur ^ 1]);
+  const auto u1 = G((R*)I.uv[cur ^ 1]);
   const int ip = tile * BLOCK + threadIdx.x;
   int br = 0;
   const int npuv = I.npuv;
   if (ip < npuv) {
-    const int kc = I.kcuv[ip];
+    const int kc = G(I.kcuv)[ip];
     if (kc == 1 || kc == 6) {
-      const int32_t* nb = I.nb;
+      const auto nb = G(I.nb);
       const int nm = nb[ip], nmu = nb[npuv + ip];
       sfx::UvIn<R> in;
-      in.zs_nm = I.zs[nm]; in.zs_nmu = I.zs[nmu];
-      const uint8_t f = I.uvf[ip];
+      in.zs_nm = G(I.zs)[nm]; in.zs_nmu = G(I.zs)[nmu];
+      const uint8_t f = G(I.uvf)[ip];
       in.iref = (I.c.flags & sfx::F_QUADTREE) ? (f & 15) : 1;
       in.itype = (I.c.flags & sfx::F_QUADTREE) ? ((f >> 4) & 3) - 1 : 0;
       in.idir = (f >> 6) & 1;
@@ -531,56 +610,72 @@ __device__ void mom_idx_tile(const DevInst& I, int tile, unsigned int* bc, float
         in.qy_nm = q0[f2]; in.qy_nmu = q0[h]; in.qy_ndm = q0[e]; in.qy_ndmu = q0[g];
         in.uu_nmd = u0[a]; in.uu_nmu = u0[b]; in.uu_ndm = u0[cdn]; in.uu_num = u0[d];
         in.v_ndm = u0[e]; in.v_ndmu = u0[g]; in.v_nm = u0[f2]; in.v_nmu = u0[h];
-        in.kcuv_nmd = I.kcuv[a]; in.kcuv_nmu = I.kcuv[b];
+        in.kcuv_nmd = G(I.kcuv)[a]; in.kcuv_nmu = G(I.kcuv)[b];
       } else {
         in.qx_nmd = in.qx_nmu = in.qy_nm = in.qy_nmu = in.qy_ndm = in.qy_ndmu = R(0);
         in.uu_nmd = in.uu_nmu = in.uu_ndm = in.uu_num = in.v_ndm = in.v_ndmu = in.v_nm = in.v_nmu = R(0);
         in.kcuv_nmd = in.kcuv_nmu = 0;
       }
-      in.kfuv = I.kfuv[ip]; in.mask_adv = I.mask_adv[ip];
+      in.kfuv = G(I.kfuv)[ip]; in.mask_adv = G(I.mask_adv)[ip];
       in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
       if (SUBGRID) {
-        in.vol_nm = I.vol[nm]; in.vol_nmu = I.vol[nmu];
-        in.zsderv_nm = ((const R*)I.zsderv)[nm]; in.zsderv_nmu = ((const R*)I.zsderv)[nmu];
+        in.vol_nm = G(I.vol)[nm]; in.vol_nmu = G(I.vol)[nmu];
+        in.zsderv_nm = (G((const R*)I.zsderv))[nm]; in.zsderv_nmu = (G((const R*)I.zsderv))[nmu];
       } else {
-        in.zb_nm = I.zb[nm]; in.zb_nmu = I.zb[nmu];
+        in.zb_nm = G(I.zb)[nm]; in.zb_nmu = G(I.zb)[nmu];
       }
       finish_uv<R, SUBGRID>(I, ip, in, dt, q1, u1, mdt, br);
     } else {
       q1[ip] = q0[ip]; u1[ip] = u0[ip];
     }
   }
-  if (I.count && ip < npuv) atomicAdd(&bc[br], 1u);
+  if (ip < npuv) cnt[br]++;
 }
 
-template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_momentum(const DevInst* __restrict__ insts, const Item* __restrict__ items,
-                                                    int indexed) {
+// Block epilogue of a momentum tile: the tile's minimum time step into blkmin[tile] (NaN
+// or non-positive values do not take part; the reduction starts at dtmax), the
+// per-branch counters into the tile's own blkcnt slots, and the sweep-done flag.
+__device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float mdt, const int* cnt) {
+  __shared__ unsigned int bc[4];
+  if (threadIdx.x < 4) bc[threadIdx.x] = 0;
+  const float m = block_min(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);   // contains a barrier
+  if (I.count)
+    for (int k = 0; k < 4; ++k)
+      if (cnt[k]) atomicAdd(&bc[k], (unsigned int)cnt[k]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    G(I.blkmin)[tile] = m;
+    if (tile == 0) G(I.sc)->mom_done = 1;
+    if (I.count) {
+      const auto c = G(I.blkcnt) + 4 * (size_t)tile;
+      for (int k = 0; k < 4; ++k) c[k] += bc[k];
+    }
+  }
+}
+
+// One launch per (layout, mesh kind): every block handles one (scenario, tile) item.
+template <typename R, bool SUBGRID>
+__global__ void __launch_bounds__(BLOCK) k_mom_dense(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
   if (it.inst < 0) return;
   const DevInst& I = insts[it.inst];
-  if (!I.sc->active) return;
-  __shared__ unsigned int bc[4];
-  if (threadIdx.x < 4) bc[threadIdx.x] = 0;
-  __syncthreads();
+  if (!G(I.sc)->active) return;
   float mdt = I.c.dtmax;
-  if (indexed) {
-    if (I.subgrid) mom_idx_tile<R, true>(I, it.tile, bc, mdt);
-    else mom_idx_tile<R, false>(I, it.tile, bc, mdt);
-  } else {
-    if (I.subgrid) mom_dense_tile<R, true>(I, it.tile, bc, mdt);
-    else mom_dense_tile<R, false>(I, it.tile, bc, mdt);
-  }
-  // NaN or non-positive time steps do not take part (the reduction starts at dtmax).
-  const float m = block_min(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);
-  if (threadIdx.x == 0) {
-    I.blkmin[it.tile] = m;
-    if (it.tile == 0) I.sc->mom_done = 1;
-    if (I.count) {
-      uint32_t* cnt = I.blkcnt + 4 * (size_t)it.tile;
-      for (int k = 0; k < 4; ++k) cnt[k] += bc[k];
-    }
-  }
+  int cnt[4] = {0, 0, 0, 0};
+  mom_strip<R, SUBGRID>(I, it.tile, cnt, mdt);
+  mom_epilogue(I, it.tile, mdt, cnt);
+}
+
+template <typename R, bool SUBGRID>
+__global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  if (it.inst < 0) return;
+  const DevInst& I = insts[it.inst];
+  if (!G(I.sc)->active) return;
+  float mdt = I.c.dtmax;
+  int cnt[4] = {0, 0, 0, 0};
+  mom_idx<R, SUBGRID>(I, it.tile, cnt, mdt);
+  mom_epilogue(I, it.tile, mdt, cnt);
 }
 
 // Combined uv-points at refinement transitions (sfincs_momentum.f90:757-779).
@@ -589,13 +684,13 @@ __global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts
   const Item it = items[blockIdx.x];
   if (it.inst < 0) return;
   const DevInst& I = insts[it.inst];
-  if (!I.sc->active) return;
+  if (!G(I.sc)->active) return;
   const int j = it.tile * BLOCK + threadIdx.x;
   if (j >= I.ncuv) return;
-  const int cur1 = I.sc->cur ^ 1;
-  R* q = (R*)I.q[cur1];
-  R* uv = (R*)I.uv[cur1];
-  const int c = I.cuv[j], a = I.cuv[I.ncuv + j], b = I.cuv[2 * I.ncuv + j];
+  const int cur1 = G(I.sc)->cur ^ 1;
+  const auto q = G((R*)I.q[cur1]);
+  const auto uv = G((R*)I.uv[cur1]);
+  const int c = G(I.cuv)[j], a = G(I.cuv)[I.ncuv + j], b = G(I.cuv)[2 * I.ncuv + j];
   q[c] = (q[a] + q[b]) / R(2);
   uv[c] = (uv[a] + uv[b]) / R(2);
 }
@@ -604,27 +699,27 @@ __global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref, R dt,
                                           R netprcp) {
-  const uint8_t kraw = I.kcs[c];
+  const uint8_t kraw = G(I.kcs)[c];
   const int kcs = kraw & 0x7f;
-  double zs = I.zs[c];
-  R* zsmaxp = (R*)I.zsmax;
+  double zs = G(I.zs)[c];
+  const auto zsmaxp = G((R*)I.zsmax);
   R zsmax = zsmaxp[c];
   if (SUBGRID) {
-    double vol = I.vol[c];
+    double vol = G(I.vol)[c];
     if (kraw & 0x80) {   // point sources of this cell, in source order (:311-325)
       for (int s = 0; s < I.nsrc; ++s)
-        if (I.src_cell[s] == c) {
-          const float qs = I.qtsrc[s];
+        if (G(I.src_cell)[s] == c) {
+          const float qs = G(I.qtsrc)[s];
           if ((vol >= 0) || ((qs < 0.0f) && (vol >= 0))) vol = vol + (double)((R)qs * dt);
         }
     }
-    R* zs0p = (R*)I.zs0;
-    R* zsdp = (R*)I.zsderv;
+    const auto zs0p = G((R*)I.zs0);
+    const auto zsdp = G((R*)I.zsderv);
     R zs0 = zs0p[c], zsd = zsdp[c];
     const int nl = I.c.nl;
-    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, I.z_zmin[c],
-                         I.z_zmax[c], I.z_volmax[c], I.z_dep + (size_t)c * nl, I.c, dt);
-    I.vol[c] = vol;
+    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, G(I.z_zmin)[c],
+                         G(I.z_zmax)[c], G(I.z_volmax)[c], G(I.z_dep) + (size_t)c * nl, I.c, dt);
+    G(I.vol)[c] = vol;
     if (kcs == 1 || kcs == 4) {
       zs0p[c] = zs0;
       zsdp[c] = zsd;
@@ -632,14 +727,14 @@ __device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_m
   } else {
     if (kraw & 0x80) {   // (:87-106)
       for (int s = 0; s < I.nsrc; ++s)
-        if (I.src_cell[s] == c) {
-          const R inc = (R)I.qtsrc[s] * dt / (R)I.c.cell_area[iref - 1];
-          zs = sfx::vmax<double>(zs + (double)inc, (double)I.zb[c]);
+        if (G(I.src_cell)[s] == c) {
+          const R inc = (R)G(I.qtsrc)[s] * dt / (R)I.c.cell_area[iref - 1];
+          zs = sfx::vmax<double>(zs + (double)inc, (double)G(I.zb)[c]);
         }
     }
     sfx::cell_regular<R>(zs, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, I.c, dt);
   }
-  I.zs[c] = zs;
+  G(I.zs)[c] = zs;
   if (I.c.flags & sfx::F_ZSMAX) zsmaxp[c] = zsmax;
 }
 
@@ -649,10 +744,10 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   const Item it = items[blockIdx.x];
   if (it.inst < 0) return;
   const DevInst& I = insts[it.inst];
-  Scalars* S = I.sc;
+  const auto S = G(I.sc);
   if (!S->active) return;
   const int qb = S->mom_done ? (S->cur ^ 1) : S->cur;
-  const R* __restrict__ q = (const R*)I.q[qb];
+  const auto q = G((const R*)I.q[qb]);
   const R dt = (R)S->dt;
   const R netprcp = (R)S->netprcp;
   if (!indexed) {
@@ -671,8 +766,8 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   } else {
     const int c = it.tile * BLOCK + threadIdx.x;
     if (c >= I.np) return;
-    const R q_md = q[I.zmd[c]], q_mu = q[I.zmu[c]], q_nd = q[I.znd[c]], q_nu = q[I.znu[c]];
-    const int iref = I.z_iref[c];
+    const R q_md = q[G(I.zmd)[c]], q_mu = q[G(I.zmu)[c]], q_nd = q[G(I.znd)[c]], q_nu = q[G(I.znu)[c]];
+    const int iref = G(I.z_iref)[c];
     if (I.subgrid) cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
     else cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
   }
@@ -683,11 +778,11 @@ __global__ void __launch_bounds__(BLOCK) k_volume(const DevInst* __restrict__ I0
   const DevInst& I = *I0;
   double acc = 0.0;
   for (int c = blockIdx.x * BLOCK + threadIdx.x; c < I.np; c += gridDim.x * BLOCK) {
-    const int k = I.kcs[c] & 0x7f;
+    const int k = G(I.kcs)[c] & 0x7f;
     if (k == 0) continue;
-    const int iref = I.dense ? 1 : I.z_iref[c];
-    if (I.subgrid) acc += I.vol[c];
-    else acc += I.zs[c] * (double)I.c.cell_area[iref - 1];
+    const int iref = I.dense ? 1 : G(I.z_iref)[c];
+    if (I.subgrid) acc += G(I.vol)[c];
+    else acc += G(I.zs)[c] * (double)I.c.cell_area[iref - 1];
   }
   for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
   __shared__ double red[BLOCK / WAVE];
@@ -717,7 +812,7 @@ struct sfx_handle {
   int32_t nslot = 0;
   hipStream_t stream = nullptr;
   struct sfx_batch* solo = nullptr;
-  int tile_tn = TILE_SHAPES[0][0], tile_tm = TILE_SHAPES[0][1];
+  int tile_tn = CONT_SHAPES[0][0], tile_tm = CONT_SHAPES[0][1], strip_rows = STRIP_ROWS[0];
   size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
   int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;
@@ -753,7 +848,9 @@ struct sfx_batch {
   int32_t* dIds = nullptr;
   size_t ids_cap = 0;
   // launch descriptors (offsets into dItems)
-  size_t off_md = 0, n_md = 0, off_mi = 0, n_mi = 0, off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
+  // work lists: momentum dense/indexed x subgrid/regular, combined points, continuity
+  size_t off_m[4] = {0, 0, 0, 0}, n_m[4] = {0, 0, 0, 0};   // [dense sg, dense reg, idx sg, idx reg]
+  size_t off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
   bool dirty = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -997,11 +1094,12 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   // tiles
   I.tn = H->tile_tn; I.tm = H->tile_tm;
   I.ntn = M->dense ? (M->nmax + I.tn - 1) / I.tn : 0;
+  I.sk = H->strip_rows;
+  I.nstrip = M->dense ? (M->nmax + STRIP_COLS - 1) / STRIP_COLS : 0;
   I.nmt = (int32_t)mom_tiles(I);
   H->nmt_cap = I.nmt;
   if (M->dense)
-    for (int k = 0; k < N_TILE_SHAPES; ++k)
-      H->nmt_cap = std::max(H->nmt_cap, dense_tiles(I, TILE_SHAPES[k][0], TILE_SHAPES[k][1]));
+    for (int k = 0; k < N_STRIP_ROWS; ++k) H->nmt_cap = std::max(H->nmt_cap, strip_tiles(I, STRIP_ROWS[k]));
   I.blkmin = H->alloc<float>(H->nmt_cap);
   I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
@@ -1211,7 +1309,7 @@ int rebuild(sfx_batch* B) {
   HIPCHK(hipSetDevice(B->device));
   const size_t nm = B->members.size();
   std::vector<DevInst> insts(nm);
-  std::vector<Item> md, mi, cu, cd, ci;
+  std::vector<Item> mom[4], cu, cd, ci;
   std::vector<int32_t> ids(nm);
   int fp64 = -1;
   for (size_t i = 0; i < nm; ++i) {
@@ -1221,22 +1319,24 @@ int rebuild(sfx_batch* B) {
     if (fp64 < 0) fp64 = H->fp64;
     if (fp64 != H->fp64) { set_error("a batch cannot mix FP32 and FP64 members"); return SFX_ERR_ARG; }
     const DevInst& I = H->hI;
-    if (I.dense) {
-      const size_t nt = mom_tiles(I);
-      for (size_t t = 0; t < nt; ++t) { md.push_back(Item{(int)i, (int)t}); cd.push_back(Item{(int)i, (int)t}); }
-    } else {
-      const int tu = (I.npuv + BLOCK - 1) / BLOCK, tc = (I.np + BLOCK - 1) / BLOCK, tq = (I.ncuv + BLOCK - 1) / BLOCK;
-      for (int t = 0; t < tu; ++t) mi.push_back(Item{(int)i, t});
-      for (int t = 0; t < tc; ++t) ci.push_back(Item{(int)i, t});
+    std::vector<Item>& mv = mom[(I.dense ? 0 : 2) + (I.subgrid ? 0 : 1)];
+    const size_t tm = mom_tiles(I), tc = cont_tiles(I);
+    for (size_t t = 0; t < tm; ++t) mv.push_back(Item{(int)i, (int)t});
+    for (size_t t = 0; t < tc; ++t) (I.dense ? cd : ci).push_back(Item{(int)i, (int)t});
+    if (!I.dense) {
+      const int tq = (I.ncuv + BLOCK - 1) / BLOCK;
       for (int t = 0; t < tq; ++t) cu.push_back(Item{(int)i, t});
     }
   }
   B->fp64 = fp64 < 0 ? 0 : fp64;
-  xcd_order(md); xcd_order(cd); xcd_order(mi); xcd_order(ci);
-  if (cu.size()) xcd_order(cu);
   std::vector<Item> all;
-  B->off_md = all.size(); all.insert(all.end(), md.begin(), md.end()); B->n_md = md.size();
-  B->off_mi = all.size(); all.insert(all.end(), mi.begin(), mi.end()); B->n_mi = mi.size();
+  for (int k = 0; k < 4; ++k) {
+    if (mom[k].size()) xcd_order(mom[k]);
+    B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
+  }
+  if (cu.size()) xcd_order(cu);
+  if (cd.size()) xcd_order(cd);
+  if (ci.size()) xcd_order(ci);
   B->off_cuv = all.size(); all.insert(all.end(), cu.begin(), cu.end()); B->n_cuv = cu.size();
   B->off_cd = all.size(); all.insert(all.end(), cd.begin(), cd.end()); B->n_cd = cd.size();
   B->off_ci = all.size(); all.insert(all.end(), ci.begin(), ci.end()); B->n_ci = ci.size();
@@ -1272,8 +1372,11 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   const int nm = (int)B->members.size();
   if ((what & 1) && nm) hipLaunchKernelGGL(k_forcing<R>, dim3(nm), dim3(BLOCK), 0, st, B->dInsts, B->dIds, forcing_mode);
   if (what & 2) {
-    if (B->n_md) hipLaunchKernelGGL(k_momentum<R>, dim3(B->n_md), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_md, 0);
-    if (B->n_mi) hipLaunchKernelGGL(k_momentum<R>, dim3(B->n_mi), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_mi, 1);
+    const Item* it = B->dItems;
+    if (B->n_m[0]) hipLaunchKernelGGL((k_mom_dense<R, true>), dim3(B->n_m[0]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[0]);
+    if (B->n_m[1]) hipLaunchKernelGGL((k_mom_dense<R, false>), dim3(B->n_m[1]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[1]);
+    if (B->n_m[2]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[2]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[2]);
+    if (B->n_m[3]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[3]);
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cuv);
   }
   if (what & 4) {
@@ -1532,15 +1635,47 @@ extern "C" int sfx_batch_done(sfx_batch* B, sfx_handle* H, int32_t* done) {
 // -------------------------------------------------------------------------- autotuner
 extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[4]) {
   if (!H || !out) { set_error("null"); return SFX_ERR_ARG; }
-  out[0] = H->hI.tn; out[1] = H->hI.tm; out[2] = BLOCK; out[3] = H->hI.dense;
+  out[0] = H->hI.dense ? H->hI.sk : 0; out[1] = H->hI.tn; out[2] = H->hI.tm; out[3] = H->hI.dense;
   return SFX_OK;
 }
+
+namespace {
+// Mean duration (ms) of `trials` launches of the phases in `what` on the handle's stream.
+int time_phases(sfx_handle* H, int what, int trials, hipEvent_t e0, hipEvent_t e1, float& mean) {
+  sfx_batch* B = solo(H);
+  B->dirty = true;
+  int rc = rebuild(B);
+  if (rc) return rc;
+  launch(B, what, 0);   // warm-up
+  HIPCHK(hipStreamSynchronize(H->stream));
+  float tot = 0.f;
+  for (int t = 0; t < trials; ++t) {
+    HIPCHK(hipEventRecord(e0, H->stream));
+    launch(B, what, 0);
+    HIPCHK(hipEventRecord(e1, H->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    tot += ms;
+  }
+  mean = tot / trials;
+  return SFX_OK;
+}
+
+void set_shapes(sfx_handle* H, int sk, int tn, int tm) {
+  DevInst& I = H->hI;
+  I.sk = sk; I.tn = tn; I.tm = tm;
+  I.ntn = (I.nmax + tn - 1) / tn;
+  I.nmt = (int32_t)mom_tiles(I);
+}
+}  // namespace
 
 extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
   if (!H->hI.dense) return SFX_OK;   // indexed sweeps have one shape (a uv-point per lane)
   if (H->in_batches) { set_error("sfx_autotune: tune a scenario before adding it to a batch"); return SFX_ERR_STATE; }
+  trials = std::max(1, (int)trials);
   Scalars s0;
   int rc = normalise(H, s0);
   if (rc) return rc;
@@ -1557,43 +1692,35 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
     snap[f].resize(n);
     if ((rc = sfx_get_state(H, f.c_str(), snap[f].data(), n))) return rc;
   }
+  Scalars s = s0;
+  s.active = 1; s.dt = std::max(s0.dt, 1.0e-3f);
+  if ((rc = write_scalars(H, s))) return rc;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  const int tn0 = H->hI.tn, tm0 = H->hI.tm;
   float best = 1e30f;
-  int bi = 0;
-  for (int ci = 0; ci < N_TILE_SHAPES; ++ci) {
-    H->hI.tn = TILE_SHAPES[ci][0]; H->hI.tm = TILE_SHAPES[ci][1];
-    H->hI.ntn = (H->hI.nmax + H->hI.tn - 1) / H->hI.tn;
-    H->hI.nmt = (int32_t)mom_tiles(H->hI);
-    HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
-    sfx_batch* B = solo(H);
-    B->dirty = true;
-    if ((rc = rebuild(B))) return rc;
-    Scalars s = s0;
-    s.active = 1; s.dt = std::max(s0.dt, 1.0e-3f);
-    if ((rc = write_scalars(H, s))) return rc;
-    launch(B, 6, 0);   // warm-up
-    HIPCHK(hipStreamSynchronize(H->stream));
-    float tot = 0.f;
-    for (int t = 0; t < std::max(1, trials); ++t) {
-      HIPCHK(hipEventRecord(e0, H->stream));
-      launch(B, 6, 0);
-      HIPCHK(hipEventRecord(e1, H->stream));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      tot += ms;
-    }
-    if (tot < best) { best = tot; bi = ci; }
+  int bk = STRIP_ROWS[0];
+  for (int k = 0; k < N_STRIP_ROWS && !rc; ++k) {
+    set_shapes(H, STRIP_ROWS[k], tn0, tm0);
+    float ms = 0.f;
+    rc = time_phases(H, 2, trials, e0, e1, ms);
+    if (!rc && ms < best) { best = ms; bk = STRIP_ROWS[k]; }
+  }
+  best = 1e30f;
+  int bc = 0;
+  for (int k = 0; k < N_CONT_SHAPES && !rc; ++k) {
+    set_shapes(H, bk, CONT_SHAPES[k][0], CONT_SHAPES[k][1]);
+    float ms = 0.f;
+    rc = time_phases(H, 4, trials, e0, e1, ms);
+    if (!rc && ms < best) { best = ms; bc = k; }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  H->hI.tn = TILE_SHAPES[bi][0]; H->hI.tm = TILE_SHAPES[bi][1];
-  H->hI.ntn = (H->hI.nmax + H->hI.tn - 1) / H->hI.tn;
-  H->hI.nmt = (int32_t)mom_tiles(H->hI);
+  if (rc) return rc;
+  set_shapes(H, bk, CONT_SHAPES[bc][0], CONT_SHAPES[bc][1]);
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
-  H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm;
+  H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm; H->strip_rows = H->hI.sk;
   if ((rc = write_scalars(H, s0))) return rc;
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;

@@ -67,24 +67,32 @@ SFX_HD inline bool near_midpoint(double r) {
   return d < 9.313225746154785e-10 * r;              // 2^-30
 }
 
-SFX_HD inline float pow_lookup(const uint2* t, uint32_t key, float fallback) {
+// Table entries read as 64-bit words: key in the low half, powf bits in the high half
+// (the little-endian layout of uint2{key, value}).
+template <typename P>
+SFX_HD inline float pow_lookup(P t, uint32_t key, float fallback) {
   uint32_t h = pow_hash(key);
   for (int probe = 0; probe < 64; ++probe) {
-    const uint2 e = t[h];
-    if (e.x == key) return u2f(e.y);
-    if (e.x == 0u) break;
+    const uint64_t e = t[h];
+    const uint32_t k = (uint32_t)e;
+    if (k == key) return u2f((uint32_t)(e >> 32));
+    if (k == 0u) break;
     h = (h + 1) & (POW_HASH_SIZE - 1);
   }
   return fallback;
 }
 
 // powf(x, (float)(1/3)) or powf(x, (float)(5/3)) as the C library computes it.
-SFX_HD inline float libm_pow(float x, int which, const PowTables& T) {
+template <typename P>
+SFX_HD inline float libm_pow(float x, int which, P t13, P t53) {
   if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
   const double r = pow_estimate(x, which);
   const float f = (float)r;
-  if (x >= POW_XLO && x < POW_XHI && near_midpoint(r)) return pow_lookup(which == 0 ? T.t13 : T.t53, f2u(x), f);
+  if (x >= POW_XLO && x < POW_XHI && near_midpoint(r)) return pow_lookup(which == 0 ? t13 : t53, f2u(x), f);
   return f;
+}
+SFX_HD inline float libm_pow(float x, int which, const PowTables& T) {
+  return libm_pow(x, which, (const uint64_t*)T.t13, (const uint64_t*)T.t53);
 }
 
 }  // namespace sfx

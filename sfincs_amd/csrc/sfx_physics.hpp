@@ -19,6 +19,18 @@ enum : uint32_t {
 
 constexpr int MAXREF = 8;
 
+// Pointers into device memory held in the global address space (see G() in the kernels).
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+
+// 16-byte loads of float4 tables through generic or global pointers.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldf4(const float4* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float4 ldf4(gptr<const float4> p, size_t i) {
+  const f4v v = ((gptr<const f4v>)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ldf4(gptr<float4> p, size_t i) { return ldf4((gptr<const float4>)p, i); }
+
 // Physics constants of one scenario (sfincs_input.f90 defaults + derived values).
 struct Consts {
   float g, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
@@ -38,11 +50,16 @@ template <> __device__ __forceinline__ float rpow<float>(float x, float y) { ret
 template <> __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
 // x**(float)(1/3) and x**(float)(5/3): the C library's exact powf results in FP32,
 // ordinary double pow in the FP64 twin.
-template <typename R> __device__ __forceinline__ R pow_third(R x, int which, const PowTables& T);
-template <> __device__ __forceinline__ float pow_third<float>(float x, int which, const PowTables& T) {
-  return libm_pow(x, which, T);
+// Device view of the powf tables: 64-bit entries {key, value} in the global address space.
+struct PowTablesG { gptr<const uint64_t> t13, t53; };
+__device__ __forceinline__ PowTablesG pow_tables_g(const PowTables& T) {
+  return PowTablesG{(gptr<const uint64_t>)T.t13, (gptr<const uint64_t>)T.t53};
 }
-template <> __device__ __forceinline__ double pow_third<double>(double x, int which, const PowTables& T) {
+template <typename R> __device__ __forceinline__ R pow_third(R x, int which, const PowTablesG& T);
+template <> __device__ __forceinline__ float pow_third<float>(float x, int which, const PowTablesG& T) {
+  return libm_pow(x, which, T.t13, T.t53);
+}
+template <> __device__ __forceinline__ double pow_third<double>(double x, int which, const PowTablesG& T) {
   (void)T;
   return pow(x, which == 0 ? (double)(1.0f / 3.0f) : (double)(5.0f / 3.0f));
 }
@@ -86,8 +103,8 @@ __device__ __forceinline__ bool uv_wet(const UvIn<R>& in, const UvTables& tb, co
 }
 
 // h**(7/3) table lookup (power7over3, sfincs_momentum.f90:787-820)
-template <typename R>
-__device__ __forceinline__ R power7over3(const float* __restrict__ x73, R hu) {
+template <typename R, typename P>
+__device__ __forceinline__ R power7over3(P x73, R hu) {
   const float h = (float)hu;
   if (h < 0.00001f) return x73[0 * 1000 + (int)(1e8f * h) - 1];
   if (h < 0.0001f) return x73[1 * 1000 + (int)(1e7f * h) - 1];
@@ -104,10 +121,9 @@ __device__ __forceinline__ R power7over3(const float* __restrict__ x73, R hu) {
 
 // One wet uv-point (sfincs_momentum.f90:189-742). `lev` points at the 2 levels
 // {havg,nrep,pwet,-} bracketing zsu for partially wet subgrid points.
-template <typename R, bool SUBGRID>
+template <typename R, bool SUBGRID, typename LevP, typename X73P>
 __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables& tb, const Consts& c, R zsu,
-                                              const float4* __restrict__ lev_row, const float* __restrict__ x73,
-                                              const PowTables& pw, R dt) {
+                                              LevP lev_row, X73P x73, const PowTablesG& pw, R dt) {
   const uint32_t F = c.flags;
   UvOut<R> o;
   const int iref = in.iref - 1;
@@ -138,8 +154,8 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
       int iuv = (int)((zsu - zmin) / dzuv) + 1;
       if (iuv > nl - 1) iuv = nl - 1;
       const R facint = (zsu - (zmin + (R)(iuv - 1) * dzuv)) / dzuv;
-      const float4 a = lev_row[iuv - 1];
-      const float4 b = lev_row[iuv];
+      const float4 a = ldf4(lev_row, iuv - 1);
+      const float4 b = ldf4(lev_row, iuv);
       hu = (R)a.x + ((R)b.x - (R)a.x) * facint;
       gnavg2 = (R)a.y + ((R)b.y - (R)a.y) * facint;
       phi = (R)a.z + ((R)b.z - (R)a.z) * facint;
@@ -258,10 +274,10 @@ __device__ __forceinline__ void cell_regular(double& zs, R& zsmax, int kcs, R q_
 }
 
 // Cell update of the subgrid continuity (sfincs_continuity.f90:337-618).
-template <typename R>
+template <typename R, typename DepP>
 __device__ __forceinline__ void cell_subgrid(double& zs, double& vol, R& zs0, R& zsderv, R& zsmax, int kcs,
                                              R q_md, R q_mu, R q_nd, R q_nu, R netprcp, int iref,
-                                             float zmin, float zmax, float volmax, const float* __restrict__ dep,
+                                             float zmin, float zmax, float volmax, DepP dep,
                                              const Consts& c, R dt) {
   R dvol = R(0);
   if (kcs == 1) {

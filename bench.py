@@ -276,11 +276,11 @@ def main():
         "data": "synthetic (seeded fBm DEM, PCG64(1001))",
         "config": {"workload": f"config 1: regular {args.size}x{args.size} subgrid (10 levels), dx=25 m, "
                                f"8 discharges, 1.5 m tide", "np": sizes["np"], "npuv": sizes["npuv"],
-                   "tuning": {"momentum_strip_rows": tuning[0], "continuity_tile": tuning[1:3]},
+                   "tuning": tuning,
                    "model_time_s": t_model, "dt_s": dt_model},
         "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": mom_gbs / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_mom_dense<float, subgrid> (strip march)",
+                     "kernel": "k_mom_strip<float, subgrid>" if tuning["momentum_strip_rows"] else "k_mom_cell<float, subgrid>",
                      "bytes_per_launch": mom_b, "launch_ms": ms[1]},
         "continuity": {"achieved_GBs": cont_gbs, "frac": cont_gbs / PEAK_HBM_GBS, "bytes_per_launch": cont_b,
                        "launch_ms": ms[2]},

@@ -98,6 +98,7 @@ def _sig(lib):
         "sfx_batch_done": (I, [P, P, C.POINTER(i32)]),
         "sfx_autotune": (I, [P, i32]),
         "sfx_get_tuning": (I, [P, C.POINTER(i32)]),
+        "sfx_set_tuning": (I, [P, C.POINTER(i32)]),
         "sfx_last_error": (I, [C.c_char_p, sz]),
         "sfx_profile_steps": (I, [P, i32, C.POINTER(d)]),
         "sfx_create_ex": (I, [P, i32, i32, C.POINTER(P)]),
@@ -348,9 +349,13 @@ class Instance:
 
     def autotune(self, trials=3):
         check(lib().sfx_autotune(self.h, trials), "autotune")
-        a = (C.c_int32 * 4)()
+        a = (C.c_int32 * 6)()
         check(lib().sfx_get_tuning(self.h, a), "get_tuning")
-        return list(a)
+        return dict(dense=a[0], momentum_strip_rows=a[1], momentum_tile=[a[2], a[3]], continuity_tile=[a[4], a[5]])
+
+    def set_tuning(self, strip_rows=0, momentum_tile=(64, 4), continuity_tile=(64, 4)):
+        a = (C.c_int32 * 5)(strip_rows, momentum_tile[0], momentum_tile[1], continuity_tile[0], continuity_tile[1])
+        check(lib().sfx_set_tuning(self.h, a), "set_tuning")
 
     def profile(self, nsteps):
         ms = (C.c_double * 3)()

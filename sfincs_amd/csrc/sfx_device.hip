@@ -72,7 +72,8 @@ struct DevInst {
   int32_t dense, subgrid, fp64;
   int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
   int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
-  int32_t sk, nstrip;            // dense momentum: rows per wave of the strip march, strips
+  int32_t sk, nstrip;            // dense momentum: rows per wave of the strip march (0: per-cell kernel), strips
+  int32_t mtn, mtm, mntn;        // dense momentum, per-cell kernel: tile shape and tiles along n
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
@@ -124,12 +125,12 @@ struct DevInst {
 
 struct Item { int32_t inst, tile; };
 
-// Dense continuity tile shapes (cells along n x along m, BLOCK threads) and strip-march
-// lengths (rows per wave) of the dense momentum kernel that the autotuner tries; the
-// first of each is the default.
-constexpr int CONT_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}};
-constexpr int N_CONT_SHAPES = sizeof(CONT_SHAPES) / sizeof(CONT_SHAPES[0]);
-constexpr int STRIP_ROWS[] = {8, 4, 16, 32};
+// Dense tile shapes (cells along n x along m, BLOCK threads) of the per-cell momentum and
+// continuity kernels, and strip-march lengths (rows per wave) of the strip momentum
+// kernel, that the autotuner tries; the first of each is the default.
+constexpr int CELL_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}};
+constexpr int N_CELL_SHAPES = sizeof(CELL_SHAPES) / sizeof(CELL_SHAPES[0]);
+constexpr int STRIP_ROWS[] = {4, 8, 16};
 constexpr int N_STRIP_ROWS = sizeof(STRIP_ROWS) / sizeof(STRIP_ROWS[0]);
 constexpr int STRIP_COLS = WAVE - 2;             // columns owned by one wave
 
@@ -139,15 +140,20 @@ size_t strip_tiles(const DevInst& I, int sk) {
   return (size_t)nstrip * (size_t)((I.mmax + rows - 1) / rows);
 }
 
-// Momentum tiles of one scenario: strip-march blocks, or BLOCK uv-points (indexed).
+size_t cell_tiles(const DevInst& I, int tn, int tm) {
+  return (size_t)((I.nmax + tn - 1) / tn) * (size_t)((I.mmax + tm - 1) / tm);
+}
+
+// Momentum tiles of one scenario: per-cell tiles or strip-march blocks (dense), or BLOCK
+// uv-points (indexed).
 size_t mom_tiles(const DevInst& I) {
-  return I.dense ? strip_tiles(I, I.sk) : (size_t)((I.npuv + BLOCK - 1) / BLOCK);
+  if (!I.dense) return (size_t)((I.npuv + BLOCK - 1) / BLOCK);
+  return I.sk > 0 ? strip_tiles(I, I.sk) : cell_tiles(I, I.mtn, I.mtm);
 }
 
 // Continuity tiles: tn x tm cells, or BLOCK cells (indexed).
 size_t cont_tiles(const DevInst& I) {
-  return I.dense ? (size_t)((I.nmax + I.tn - 1) / I.tn) * (size_t)((I.mmax + I.tm - 1) / I.tm)
-                 : (size_t)((I.np + BLOCK - 1) / BLOCK);
+  return I.dense ? cell_tiles(I, I.tn, I.tm) : (size_t)((I.np + BLOCK - 1) / BLOCK);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -155,20 +161,25 @@ __device__ __forceinline__ float wave_min(float v) {
   return v;
 }
 
-// Block-wide minimum; the result is valid in thread 0.
+// Block-wide minimum over NT threads; the result is valid in every thread.
+template <int NT = BLOCK>
 __device__ __forceinline__ float block_min(float v) {
-  __shared__ float red[BLOCK / WAVE];
+  __shared__ float red[NT / WAVE];
   v = wave_min(v);
   const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
   if (lane == 0) red[w] = v;
   __syncthreads();
   float m = red[0];
-  for (int i = 1; i < BLOCK / WAVE; ++i) m = fminf(m, red[i]);
+  for (int i = 1; i < NT / WAVE; ++i) m = fminf(m, red[i]);
   return m;
 }
 
 // Completes a momentum sweep (run by one whole block): min_dt = min(dtmax, per-tile
 // minima) as in sfincs_momentum.f90:121,542-545, and the q/uv double buffer flips.
+// Blocks of the per-scenario kernels (forcing, fold): one block covers a scenario's
+// boundary points and tile minima.
+constexpr int FBLOCK = 1024;
+
 __device__ void fold_sweep(const DevInst& I) {
   const auto S = G(I.sc);
   __shared__ int sh_done;
@@ -179,15 +190,15 @@ __device__ void fold_sweep(const DevInst& I) {
   const auto p = G(I.blkmin);
   const int n = I.nmt;
   int i = threadIdx.x;
-  for (; i + 3 * BLOCK < n; i += 4 * BLOCK)
-    v = fminf(fminf(v, fminf(p[i], p[i + BLOCK])), fminf(p[i + 2 * BLOCK], p[i + 3 * BLOCK]));
-  for (; i < n; i += BLOCK) v = fminf(v, p[i]);
-  v = block_min(v);
+  for (; i + 3 * FBLOCK < n; i += 4 * FBLOCK)
+    v = fminf(fminf(v, fminf(p[i], p[i + FBLOCK])), fminf(p[i + 2 * FBLOCK], p[i + 3 * FBLOCK]));
+  for (; i < n; i += FBLOCK) v = fminf(v, p[i]);
+  v = block_min<FBLOCK>(v);
   if (threadIdx.x == 0) { S->cur ^= 1; S->mom_done = 0; S->min_dt = v; }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(BLOCK) k_fold(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids) {
+__global__ void __launch_bounds__(FBLOCK) k_fold(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids) {
   fold_sweep(insts[ids ? ids[blockIdx.x] : blockIdx.x]);
 }
 
@@ -202,7 +213,7 @@ __device__ float f_modulo(float a, float p) {
 }
 
 template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids,
+__global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids,
                                                    int mode) {
   const DevInst& I = insts[ids[blockIdx.x]];
   const auto S = G(I.sc);
@@ -398,15 +409,20 @@ __global__ void __launch_bounds__(BLOCK) k_forcing(const DevInst* __restrict__ i
 }
 
 // ----------------------------------------------------------------------------- momentum
+// Loads the wet/dry threshold of a uv-point: subgrid_uv_zmin or zbuvmx.
+template <bool SUBGRID>
+__device__ __forceinline__ float uv_floor(const DevInst& I, int slot) {
+  return SUBGRID ? G(I.uv_zmin)[slot] : G(I.zbuvmx)[slot];
+}
+
+// Completes one active uv-point from its gathered inputs and its preloaded wet/dry
+// threshold `floor` (uv_floor): dry points get q = uv = 0, wet points the full update.
 template <typename R, bool SUBGRID, typename QP>
-__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, R dt, QP q1, QP uv1,
-                                          float& mdt, int& br) {
+__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, float floor, R dt,
+                                          QP q1, QP uv1, float& mdt, int& br) {
   sfx::UvTables tb;
-  if (SUBGRID) {
-    tb.zmin = G(I.uv_zmin)[slot];
-  } else {
-    tb.zbuvmx = G(I.zbuvmx)[slot];
-  }
+  if (SUBGRID) tb.zmin = floor;
+  else tb.zbuvmx = floor;
   R zsu;
   if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
     q1[slot] = R(0); uv1[slot] = R(0); G(I.kfuv)[slot] = 0;
@@ -425,6 +441,100 @@ __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx:
   q1[slot] = o.q; uv1[slot] = o.uv; G(I.kfuv)[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
   br = o.branch;
+}
+
+// Dense scenarios, one thread per cell: the u-point on its right and the v-point above
+// it. Every load that does not depend on a branch is issued before the first test, so a
+// thread waits for memory once for its stencil and thresholds, once more for the tables
+// of wet points. Neighbours follow the reference's tables for a full rectangle
+// (sfincs_domain.f90:1005-1261): missing left/right/below/above faces read the dummy
+// point (q = uv = 0), missing cross neighbours of u_ndm/u_num fall back to the point.
+// Gathers the inputs of one uv-point. The neighbour values are read whether or not the
+// scenario's physics uses them (uv_update reads them only under the flags that need
+// them), so that the loads carry no branch.
+template <typename R, bool SUBGRID, typename QP>
+__device__ __forceinline__ void gather_uv(const DevInst& I, QP q0, QP u0, int s, int s_nmd, int s_nmu, int s_ndm,
+                                          int s_num, int v_nm, int v_ndm, int v_nmu, int v_ndmu, int c_nm,
+                                          int c_nmu, sfx::UvIn<R>& in) {
+  in.zs_nm = G(I.zs)[c_nm]; in.zs_nmu = G(I.zs)[c_nmu];
+  in.qx_nm = q0[s]; in.uu_nm = u0[s];
+  in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
+  in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
+  in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
+  in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
+  in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
+  in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s];
+  if (SUBGRID) {
+    in.vol_nm = G(I.vol)[c_nm]; in.vol_nmu = G(I.vol)[c_nmu];
+    in.zsderv_nm = G((const R*)I.zsderv)[c_nm]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
+  } else {
+    in.zb_nm = G(I.zb)[c_nm]; in.zb_nmu = G(I.zb)[c_nmu];
+  }
+}
+
+template <typename R, bool SUBGRID>
+__device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, float& mdt) {
+  const auto S = G(I.sc);
+  const int cur = S->cur;
+  const R dt = (R)S->dt;
+  const auto q0 = G((const R*)I.q[cur]);
+  const auto u0 = G((const R*)I.uv[cur]);
+  const auto q1 = G((R*)I.q[cur ^ 1]);
+  const auto u1 = G((R*)I.uv[cur ^ 1]);
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
+  const int tn = tile % I.mntn, tmi = tile / I.mntn;
+  const int n = tn * I.mtn + (threadIdx.x % I.mtn);
+  const int m = tmi * I.mtm + (threadIdx.x / I.mtn);
+  if (n >= nmax || m >= mmax) return;
+  const int c = m * nmax + n;
+  const bool hu = m < mmax - 1, hv = n < nmax - 1;
+  // A point that does not exist (u on the last row, v on the last column) reads the dummy
+  // slot and its own cell, and is not updated.
+  sfx::UvIn<R> iu, iv;
+  {   // u-point between c and c+nmax
+    const int s = hu ? c : dm;
+    gather_uv<R, SUBGRID>(I, q0, u0, s, hu && m >= 1 ? c - nmax : dm, m + 1 < mmax - 1 ? c + nmax : dm,
+                          hu ? (n >= 1 ? c - 1 : c) : dm, hu ? (n < nmax - 1 ? c + 1 : c) : dm,
+                          hu && hv ? np + c : dm, hu && n >= 1 ? np + c - 1 : dm, hu && hv ? np + c + nmax : dm,
+                          hu && n >= 1 ? np + c + nmax - 1 : dm, c, hu ? c + nmax : c, iu);
+  }
+  {   // v-point between c and c+1
+    const int s = hv ? np + c : dm;
+    gather_uv<R, SUBGRID>(I, q0, u0, s, hv && n >= 1 ? np + c - 1 : dm, n + 1 < nmax - 1 ? np + c + 1 : dm,
+                          hv ? (m >= 1 ? s - nmax : s) : dm, hv ? (m < mmax - 1 ? s + nmax : s) : dm,
+                          hv && hu ? c : dm, hv && m >= 1 ? c - nmax : dm, hv && hu ? c + 1 : dm,
+                          hv && m >= 1 ? c + 1 - nmax : dm, c, hv ? c + 1 : c, iv);
+  }
+  const int ku = G(I.kcuv)[hu ? c : dm], kv = G(I.kcuv)[hv ? np + c : dm];
+  const float fu = uv_floor<SUBGRID>(I, hu ? c : dm), fv = uv_floor<SUBGRID>(I, hv ? np + c : dm);
+  const R tauwu = (R)S->tauwu, tauwv = (R)S->tauwv;
+  if (hu) {
+    if (ku == 1 || ku == 6) {
+      iu.idir = 0; iu.iref = 1; iu.itype = 0; iu.tauwu = tauwu; iu.tauwv = tauwv;
+      int br = 0;
+      finish_uv<R, SUBGRID>(I, c, iu, fu, dt, q1, u1, mdt, br);
+      cnt[br]++;
+    } else {
+      q1[c] = iu.qx_nm; u1[c] = iu.uu_nm;
+      cnt[0]++;
+    }
+  } else {
+    cnt[0]++;
+  }
+  if (hv) {
+    const int s = np + c;
+    if (kv == 1 || kv == 6) {
+      iv.idir = 1; iv.iref = 1; iv.itype = 0; iv.tauwu = tauwu; iv.tauwv = tauwv;
+      int br = 0;
+      finish_uv<R, SUBGRID>(I, s, iv, fv, dt, q1, u1, mdt, br);
+      cnt[br]++;
+    } else {
+      q1[s] = iv.qx_nm; u1[s] = iv.uu_nm;
+      cnt[0]++;
+    }
+  } else {
+    cnt[0]++;
+  }
 }
 
 // Dense scenarios, strip march. A wave covers columns n = n0-1 .. n0+62 of one strip
@@ -535,7 +645,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = C.vol; in.zsderv_nm = B.zd; in.zsderv_nmu = C.zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)C.zd; }
           int br = 0;
-          finish_uv<R, SUBGRID>(I, c, in, dt, q1, u1, mdt, br);
+          finish_uv<R, SUBGRID>(I, c, in, uv_floor<SUBGRID>(I, c), dt, q1, u1, mdt, br);
           cnt[br]++;
         } else {
           q1[c] = B.uq; u1[c] = B.uu;
@@ -564,7 +674,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = B.r_vol; in.zsderv_nm = B.zd; in.zsderv_nmu = B.r_zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)B.r_zd; }
           int br = 0;
-          finish_uv<R, SUBGRID>(I, s, in, dt, q1, u1, mdt, br);
+          finish_uv<R, SUBGRID>(I, s, in, uv_floor<SUBGRID>(I, s), dt, q1, u1, mdt, br);
           cnt[br]++;
         } else {
           q1[s] = B.vq; u1[s] = B.vu;
@@ -593,6 +703,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
   const int npuv = I.npuv;
   if (ip < npuv) {
     const int kc = G(I.kcuv)[ip];
+    const float fl = uv_floor<SUBGRID>(I, ip);
     if (kc == 1 || kc == 6) {
       const auto nb = G(I.nb);
       const int nm = nb[ip], nmu = nb[npuv + ip];
@@ -624,7 +735,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
       } else {
         in.zb_nm = G(I.zb)[nm]; in.zb_nmu = G(I.zb)[nmu];
       }
-      finish_uv<R, SUBGRID>(I, ip, in, dt, q1, u1, mdt, br);
+      finish_uv<R, SUBGRID>(I, ip, in, fl, dt, q1, u1, mdt, br);
     } else {
       q1[ip] = q0[ip]; u1[ip] = u0[ip];
     }
@@ -638,7 +749,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
 __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float mdt, const int* cnt) {
   __shared__ unsigned int bc[4];
   if (threadIdx.x < 4) bc[threadIdx.x] = 0;
-  const float m = block_min(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);   // contains a barrier
+  const float m = block_min<BLOCK>(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);   // contains a barrier
   if (I.count)
     for (int k = 0; k < 4; ++k)
       if (cnt[k]) atomicAdd(&bc[k], (unsigned int)cnt[k]);
@@ -655,7 +766,19 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
 
 // One launch per (layout, mesh kind): every block handles one (scenario, tile) item.
 template <typename R, bool SUBGRID>
-__global__ void __launch_bounds__(BLOCK) k_mom_dense(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
+__global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  if (it.inst < 0) return;
+  const DevInst& I = insts[it.inst];
+  if (!G(I.sc)->active) return;
+  float mdt = I.c.dtmax;
+  int cnt[4] = {0, 0, 0, 0};
+  mom_cell<R, SUBGRID>(I, it.tile, cnt, mdt);
+  mom_epilogue(I, it.tile, mdt, cnt);
+}
+
+template <typename R, bool SUBGRID>
+__global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
   if (it.inst < 0) return;
   const DevInst& I = insts[it.inst];
@@ -812,7 +935,9 @@ struct sfx_handle {
   int32_t nslot = 0;
   hipStream_t stream = nullptr;
   struct sfx_batch* solo = nullptr;
-  int tile_tn = CONT_SHAPES[0][0], tile_tm = CONT_SHAPES[0][1], strip_rows = STRIP_ROWS[0];
+  // tuned shapes: continuity tile, momentum per-cell tile, momentum strip rows (0: per-cell)
+  int tile_tn = CELL_SHAPES[0][0], tile_tm = CELL_SHAPES[0][1];
+  int mtile_tn = CELL_SHAPES[0][0], mtile_tm = CELL_SHAPES[0][1], strip_rows = 0;
   size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
   int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;
@@ -849,7 +974,8 @@ struct sfx_batch {
   size_t ids_cap = 0;
   // launch descriptors (offsets into dItems)
   // work lists: momentum dense/indexed x subgrid/regular, combined points, continuity
-  size_t off_m[4] = {0, 0, 0, 0}, n_m[4] = {0, 0, 0, 0};   // [dense sg, dense reg, idx sg, idx reg]
+  // [cell sg, cell reg, strip sg, strip reg, idx sg, idx reg]
+  size_t off_m[6] = {0, 0, 0, 0, 0, 0}, n_m[6] = {0, 0, 0, 0, 0, 0};
   size_t off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
   bool dirty = true;
   hipGraph_t graph = nullptr;
@@ -1096,10 +1222,15 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   I.ntn = M->dense ? (M->nmax + I.tn - 1) / I.tn : 0;
   I.sk = H->strip_rows;
   I.nstrip = M->dense ? (M->nmax + STRIP_COLS - 1) / STRIP_COLS : 0;
+  I.mtn = H->mtile_tn; I.mtm = H->mtile_tm;
+  I.mntn = M->dense ? (M->nmax + I.mtn - 1) / I.mtn : 0;
   I.nmt = (int32_t)mom_tiles(I);
   H->nmt_cap = I.nmt;
-  if (M->dense)
+  if (M->dense) {
     for (int k = 0; k < N_STRIP_ROWS; ++k) H->nmt_cap = std::max(H->nmt_cap, strip_tiles(I, STRIP_ROWS[k]));
+    for (int k = 0; k < N_CELL_SHAPES; ++k)
+      H->nmt_cap = std::max(H->nmt_cap, cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
+  }
   I.blkmin = H->alloc<float>(H->nmt_cap);
   I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
@@ -1142,7 +1273,7 @@ int write_scalars(sfx_handle* H, const Scalars& s) {
 int normalise(sfx_handle* H, Scalars& s) {
   int rc = read_scalars(H, s);
   if (rc || !s.mom_done) return rc;
-  hipLaunchKernelGGL(k_fold, dim3(1), dim3(BLOCK), 0, H->stream, H->dI, (const int32_t*)nullptr);
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(FBLOCK), 0, H->stream, H->dI, (const int32_t*)nullptr);
   HIPCHK(hipGetLastError());
   return read_scalars(H, s);
 }
@@ -1309,7 +1440,7 @@ int rebuild(sfx_batch* B) {
   HIPCHK(hipSetDevice(B->device));
   const size_t nm = B->members.size();
   std::vector<DevInst> insts(nm);
-  std::vector<Item> mom[4], cu, cd, ci;
+  std::vector<Item> mom[6], cu, cd, ci;
   std::vector<int32_t> ids(nm);
   int fp64 = -1;
   for (size_t i = 0; i < nm; ++i) {
@@ -1319,7 +1450,7 @@ int rebuild(sfx_batch* B) {
     if (fp64 < 0) fp64 = H->fp64;
     if (fp64 != H->fp64) { set_error("a batch cannot mix FP32 and FP64 members"); return SFX_ERR_ARG; }
     const DevInst& I = H->hI;
-    std::vector<Item>& mv = mom[(I.dense ? 0 : 2) + (I.subgrid ? 0 : 1)];
+    std::vector<Item>& mv = mom[(I.dense ? (I.sk > 0 ? 2 : 0) : 4) + (I.subgrid ? 0 : 1)];
     const size_t tm = mom_tiles(I), tc = cont_tiles(I);
     for (size_t t = 0; t < tm; ++t) mv.push_back(Item{(int)i, (int)t});
     for (size_t t = 0; t < tc; ++t) (I.dense ? cd : ci).push_back(Item{(int)i, (int)t});
@@ -1330,7 +1461,7 @@ int rebuild(sfx_batch* B) {
   }
   B->fp64 = fp64 < 0 ? 0 : fp64;
   std::vector<Item> all;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 6; ++k) {
     if (mom[k].size()) xcd_order(mom[k]);
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
   }
@@ -1370,13 +1501,15 @@ template <typename R>
 void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   hipStream_t st = B->stream;
   const int nm = (int)B->members.size();
-  if ((what & 1) && nm) hipLaunchKernelGGL(k_forcing<R>, dim3(nm), dim3(BLOCK), 0, st, B->dInsts, B->dIds, forcing_mode);
+  if ((what & 1) && nm) hipLaunchKernelGGL(k_forcing<R>, dim3(nm), dim3(FBLOCK), 0, st, B->dInsts, B->dIds, forcing_mode);
   if (what & 2) {
     const Item* it = B->dItems;
-    if (B->n_m[0]) hipLaunchKernelGGL((k_mom_dense<R, true>), dim3(B->n_m[0]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[0]);
-    if (B->n_m[1]) hipLaunchKernelGGL((k_mom_dense<R, false>), dim3(B->n_m[1]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[1]);
-    if (B->n_m[2]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[2]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[2]);
-    if (B->n_m[3]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[3]);
+    if (B->n_m[0]) hipLaunchKernelGGL((k_mom_cell<R, true>), dim3(B->n_m[0]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[0]);
+    if (B->n_m[1]) hipLaunchKernelGGL((k_mom_cell<R, false>), dim3(B->n_m[1]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[1]);
+    if (B->n_m[2]) hipLaunchKernelGGL((k_mom_strip<R, true>), dim3(B->n_m[2]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[2]);
+    if (B->n_m[3]) hipLaunchKernelGGL((k_mom_strip<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[3]);
+    if (B->n_m[4]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[4]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[4]);
+    if (B->n_m[5]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[5]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[5]);
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cuv);
   }
   if (what & 4) {
@@ -1633,9 +1766,10 @@ extern "C" int sfx_batch_done(sfx_batch* B, sfx_handle* H, int32_t* done) {
 }
 
 // -------------------------------------------------------------------------- autotuner
-extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[4]) {
+extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[6]) {
   if (!H || !out) { set_error("null"); return SFX_ERR_ARG; }
-  out[0] = H->hI.dense ? H->hI.sk : 0; out[1] = H->hI.tn; out[2] = H->hI.tm; out[3] = H->hI.dense;
+  const DevInst& I = H->hI;
+  out[0] = I.dense; out[1] = I.sk; out[2] = I.mtn; out[3] = I.mtm; out[4] = I.tn; out[5] = I.tm;
   return SFX_OK;
 }
 
@@ -1662,13 +1796,40 @@ int time_phases(sfx_handle* H, int what, int trials, hipEvent_t e0, hipEvent_t e
   return SFX_OK;
 }
 
-void set_shapes(sfx_handle* H, int sk, int tn, int tm) {
+void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
   DevInst& I = H->hI;
-  I.sk = sk; I.tn = tn; I.tm = tm;
+  I.sk = sk; I.mtn = mtn; I.mtm = mtm; I.tn = tn; I.tm = tm;
+  I.mntn = (I.nmax + mtn - 1) / mtn;
   I.ntn = (I.nmax + tn - 1) / tn;
   I.nmt = (int32_t)mom_tiles(I);
 }
 }  // namespace
+
+extern "C" int sfx_set_tuning(sfx_handle* H, const int32_t in[5]) {
+  if (!H || !in) { set_error("null"); return SFX_ERR_ARG; }
+  const int sk = in[0], mtn = in[1], mtm = in[2], tn = in[3], tm = in[4];
+  if (sk < 0 || sk > 256 || mtn <= 0 || mtm <= 0 || mtn * mtm != BLOCK || tn <= 0 || tm <= 0 || tn * tm != BLOCK) {
+    set_error("sfx_set_tuning: strip rows must be 0..256 and tiles must hold 256 cells");
+    return SFX_ERR_ARG;
+  }
+  if (H->in_batches) { set_error("sfx_set_tuning: set shapes before adding the scenario to a batch"); return SFX_ERR_STATE; }
+  if (!H->hI.dense) return SFX_OK;
+  HIPCHK(hipSetDevice(H->device));
+  Scalars s;
+  int rc = normalise(H, s);
+  if (rc) return rc;
+  const DevInst keep = H->hI;
+  set_shapes(H, sk, mtn, mtm, tn, tm);
+  if (mom_tiles(H->hI) > H->nmt_cap) {
+    H->hI = keep;
+    set_error("sfx_set_tuning: more momentum tiles than the handle holds");
+    return SFX_ERR_ARG;
+  }
+  HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
+  H->tile_tn = tn; H->tile_tm = tm; H->strip_rows = sk; H->mtile_tn = mtn; H->mtile_tm = mtm;
+  if (H->solo) H->solo->dirty = true;
+  return SFX_OK;
+}
 
 extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
@@ -1699,18 +1860,22 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int tn0 = H->hI.tn, tm0 = H->hI.tm;
+  // momentum: per-cell tiles (sk = 0) and strip marches
   float best = 1e30f;
-  int bk = STRIP_ROWS[0];
-  for (int k = 0; k < N_STRIP_ROWS && !rc; ++k) {
-    set_shapes(H, STRIP_ROWS[k], tn0, tm0);
+  int bsk = 0, bmtn = CELL_SHAPES[0][0], bmtm = CELL_SHAPES[0][1];
+  for (int k = 0; k < N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
+    const int sk = k < N_CELL_SHAPES ? 0 : STRIP_ROWS[k - N_CELL_SHAPES];
+    const int mtn = k < N_CELL_SHAPES ? CELL_SHAPES[k][0] : bmtn, mtm = k < N_CELL_SHAPES ? CELL_SHAPES[k][1] : bmtm;
+    set_shapes(H, sk, mtn, mtm, tn0, tm0);
     float ms = 0.f;
     rc = time_phases(H, 2, trials, e0, e1, ms);
-    if (!rc && ms < best) { best = ms; bk = STRIP_ROWS[k]; }
+    if (!rc && ms < best) { best = ms; bsk = sk; bmtn = mtn; bmtm = mtm; }
   }
+  // continuity tiles
   best = 1e30f;
   int bc = 0;
-  for (int k = 0; k < N_CONT_SHAPES && !rc; ++k) {
-    set_shapes(H, bk, CONT_SHAPES[k][0], CONT_SHAPES[k][1]);
+  for (int k = 0; k < N_CELL_SHAPES && !rc; ++k) {
+    set_shapes(H, bsk, bmtn, bmtm, CELL_SHAPES[k][0], CELL_SHAPES[k][1]);
     float ms = 0.f;
     rc = time_phases(H, 4, trials, e0, e1, ms);
     if (!rc && ms < best) { best = ms; bc = k; }
@@ -1718,9 +1883,10 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc) return rc;
-  set_shapes(H, bk, CONT_SHAPES[bc][0], CONT_SHAPES[bc][1]);
+  set_shapes(H, bsk, bmtn, bmtm, CELL_SHAPES[bc][0], CELL_SHAPES[bc][1]);
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
   H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm; H->strip_rows = H->hI.sk;
+  H->mtile_tn = H->hI.mtn; H->mtile_tm = H->hI.mtm;
   if ((rc = write_scalars(H, s0))) return rc;
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;

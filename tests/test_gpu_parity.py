@@ -237,3 +237,26 @@ def test_branch_counters_cover_every_point():
         assert a.time() == b.time()
         assert np.array_equal(a.get("zs"), b.get("zs"))
 
+
+SHAPES = [(0, (64, 4), (64, 4)), (0, (256, 1), (128, 2)), (0, (32, 8), (256, 1)), (4, (64, 4), (32, 8)),
+          (8, (64, 4), (64, 4)), (16, (64, 4), (64, 4))]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"strip{a}_mom{b[0]}x{b[1]}_cont{c[0]}x{c[1]}" for a, b, c in SHAPES])
+def test_every_dense_kernel_shape_matches_oracle(shape):
+    """Each dense momentum variant (per-cell tiles, strip marches) and continuity tile the
+    autotuner can pick gives the oracle's result; the grid (70 x 83) is not a multiple of
+    any tile, so partial tiles and strips are exercised."""
+    strip, mt, ct = shape
+    print(f"case shape {shape}", flush=True)
+    sc = W.subgrid_case(70, 83, dx=25.0, seed=19, t_end=900.0, nsrc=2, rain_mmh=20.0)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    inst.set_tuning(strip, mt, ct)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    print(shape, st["nt"], d, flush=True)
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+

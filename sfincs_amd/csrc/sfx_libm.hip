@@ -25,7 +25,7 @@ namespace sfx {
 namespace {
 
 constexpr uint32_t MAGIC = 0x5346504Du;   // "SFPM"
-constexpr uint32_t VERSION = 3;
+constexpr uint32_t VERSION = 4;
 
 struct Entry { uint32_t x, g; };
 

@@ -4,12 +4,11 @@
 // (sfincs_momentum.f90:92,616,673) with the C library's powf. The momentum and continuity
 // updates branch on thresholds (wet/dry tests, the table inversion switch at
 // V = 0.999*volmax), so a one-ulp difference in powf can move a cell to the other branch.
-// These routines return powf's exact result: a double-precision estimate (cube root by
-// two Newton steps from cbrtf, times exp(d*ln x) for the offset d between the float
-// exponent and 1/3 or 5/3) is rounded to float unless it lies within 2^-30 (relative) of
-// a rounding midpoint; those inputs (about 2% of calls) are looked up in a table of
-// powf's own results that sfx_libm.hip enumerates over every float in [2^-40, 2^14) and
-// verifies. Outside that range the rounded estimate is returned.
+// These routines return powf's exact result: a double-precision estimate (pow_estimate)
+// is rounded to float unless it lies within 2^-30 (relative) of a rounding midpoint;
+// those inputs (about 2% of calls) are looked up in a table of powf's own results that
+// sfx_libm.hip enumerates over every float in [2^-40, 2^14) and verifies. Outside that
+// range the rounded estimate is returned.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -39,21 +38,49 @@ SFX_HD inline uint32_t f2u(float f) { union { float f; uint32_t u; } v; v.f = f;
 SFX_HD inline float u2f(uint32_t u) { union { float f; uint32_t u; } v; v.u = u; return v.f; }
 
 // Double estimate of x**e for e = (float)(1/3) (which = 0) or (float)(5/3) (which = 1),
-// relative error below 2^-46 for normal positive x.
+// relative error below 2^-46 for normal positive x (checked for every float of the table
+// range when the tables are built). Only IEEE-754 basic operations in double are used,
+// evaluated in the order written, so host and device compute bit-identical estimates.
+//   x = m * 2^e, m in [1, 2), e = 3q + r:  x^(1/3) = (m * 2^r)^(1/3) * 2^q, from a
+//   polynomial guess of (m 2^r)^(-1/3) refined by two Newton steps z += z (1 - a z^3) / 3
+//   (no division); the float exponents differ from 1/3 and 5/3 by d, applied as
+//   x^d = 1 + t + t^2/2 with t = d ln x, ln x = e ln 2 + ln m from a polynomial.
 SFX_HD inline double pow_estimate(float x, int which) {
-  const double xd = x;
-  double y = (double)cbrtf(x);
-  y = y - (y * y * y - xd) / (3.0 * y * y);
-  y = y - (y * y * y - xd) / (3.0 * y * y);
-  int e;
-  const float mant = frexpf(x, &e);                  // x = mant * 2^e, mant in [0.5, 1)
-  const double lnx = (double)e * 0.69314718055994530942 + (double)logf(mant);
+  const uint32_t b = f2u(x);
+  if (b < 0x00800000u)   // subnormal: outside the table range, no exactness claimed
+    return which == 0 ? (double)cbrtf(x) : (double)x * (double)cbrtf(x) * (double)cbrtf(x);
+  const int e = (int)(b >> 23) - 127;
+  const double m = (double)u2f((b & 0x7fffffu) | 0x3f800000u);
+  const int q = e >= 0 ? e / 3 : -((2 - e) / 3);
+  const int r = e - 3 * q;
+  const double a = m * (double)(1 << r);
+  const double cr = r == 0 ? 1.0 : (r == 1 ? 0.79370052598409979 : 0.62996052494743658);   // 2^(-r/3)
+  // m^(-1/3) on [1, 2], relative error 3.8e-5
+  double z = (((0.0285876759712902 * m - 0.22121268212268402) * m + 0.6954153443691581) * m -
+              1.1728037038550854) * m + 1.6699753569259383;
+  z = z * cr;
+  const double third = 1.0 / 3.0;
+  z = z + z * (1.0 - a * z * z * z) * third;
+  z = z + z * (1.0 - a * z * z * z) * third;
+  // 2^q as a double (exact power of two; q is within [-43, 42] for normal floats)
+  union { uint64_t u; double d; } p2;
+  p2.u = (uint64_t)(q + 1023) << 52;
+  const double y = a * z * z * p2.d;
+  // ln x, with m folded into [sqrt(1/2), sqrt(2)); ln(1+u) on that range, abs error 1.9e-7
+  double mm = m;
+  int ee = e;
+  if (mm > 1.4142135623730951) { mm = mm * 0.5; ee = ee + 1; }
+  const double u = mm - 1.0;
+  const double lnm = ((((((0.11860324847309349 * u - 0.1896650762497095) * u + 0.20616715842644762) * u -
+                         0.24870626475368562) * u + 0.333012775052759) * u - 0.5000253637544767) * u +
+                      1.0000038762934083) * u + 9.149265129190923e-08;
+  const double lnx = (double)ee * 0.69314718055994530942 + lnm;
   // offsets of the float exponents from 1/3 and 5/3
   const double d13 = (double)(1.0f / 3.0f) - 1.0 / 3.0;
   const double d53 = (double)(5.0f / 3.0f) - 5.0 / 3.0;
   const double t = (which == 0 ? d13 : d53) * lnx;
   const double corr = 1.0 + t + 0.5 * t * t;
-  return which == 0 ? y * corr : xd * y * y * corr;
+  return which == 0 ? y * corr : (double)x * y * y * corr;
 }
 
 // True when the double value r lies within 2^-30 (relative) of a midpoint between two

@@ -48,13 +48,13 @@ constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 
 // Device arrays are reached through pointers stored in DevInst; used as generic pointers
-// every access becomes a flat load, which the compiler must wait for one group at a time.
-// G() returns the pointer in the global address space, so the accesses become global
-// loads and stores that can all be in flight together. Pointers kept in local variables
-// are held as gptr<T> for the same reason.
-using sfx::gptr;
+// every access becomes a flat load with a 64-bit per-lane address, which the compiler
+// must wait for one group at a time. G() wraps the pointer as a global-address-space
+// array indexed by 32-bit element numbers (sfx::GA), so accesses become global loads and
+// stores with a scalar base that can all be in flight together.
+using sfx::GA;
 using sfx::ldf4;
-template <typename T> __device__ __forceinline__ gptr<T> G(T* p) { return (gptr<T>)p; }
+template <typename T> __device__ __forceinline__ GA<T> G(T* p) { return sfx::ga(p); }
 
 // Device-resident step scalars of one scenario (t, dt, CFL reduction, forcing scalars).
 struct Scalars {
@@ -1099,6 +1099,15 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   }
   I.nslot = H->nslot;
   const int ns = H->nslot + 1;
+  {   // kernels index every array with 32-bit byte offsets
+    const size_t big = std::max({(size_t)ns * (size_t)std::max(nl, 1) * 16, (size_t)np * (size_t)std::max(nl, 1) * 4,
+                                 (size_t)npuv * 40, (size_t)ns * 8, (size_t)np * 8});
+    if (big >= ((size_t)1 << 32)) {
+      set_error("sfx_create: a device array of this mesh would exceed 4 GiB; split the domain");
+      delete H;
+      return SFX_ERR_ARG;
+    }
+  }
   auto perm8 = [&](const std::vector<int8_t>& src, int8_t fill) {
     std::vector<int8_t> d(ns, fill);
     for (size_t k = 0; k < src.size() && k < H->ref2slot.size(); ++k) if (H->ref2slot[k] >= 0) d[H->ref2slot[k]] = src[k];

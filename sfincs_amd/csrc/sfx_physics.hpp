@@ -19,17 +19,34 @@ enum : uint32_t {
 
 constexpr int MAXREF = 8;
 
-// Pointers into device memory held in the global address space (see G() in the kernels).
+// Device arrays in the global address space, indexed by 32-bit unsigned element numbers:
+// an access compiles to a global load or store with the array base in scalar registers
+// and one 32-bit byte offset per lane (every array is smaller than 4 GiB; sfx_create
+// checks). Used for all device memory reached through pointers held in structures,
+// which the compiler would otherwise access with flat instructions and 64-bit per-lane
+// addresses.
 template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+struct GA {
+  gptr<T> p;
+  __device__ __forceinline__ __attribute__((address_space(1))) T& operator[](uint32_t i) const {
+    return *(gptr<T>)((gptr<char>)(gptr<void>)p + i * (uint32_t)sizeof(T));
+  }
+  __device__ __forceinline__ GA operator+(uint32_t i) const {
+    return GA{(gptr<T>)((gptr<char>)(gptr<void>)p + i * (uint32_t)sizeof(T))};
+  }
+  __device__ __forceinline__ gptr<T> operator->() const { return p; }
+};
+template <typename T> __device__ __forceinline__ GA<T> ga(T* p) { return GA<T>{(gptr<T>)p}; }
 
-// 16-byte loads of float4 tables through generic or global pointers.
+// 16-byte loads of float4 tables.
 typedef float f4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ldf4(const float4* p, size_t i) { return p[i]; }
-__device__ __forceinline__ float4 ldf4(gptr<const float4> p, size_t i) {
-  const f4v v = ((gptr<const f4v>)p)[i];
+template <typename T>
+__device__ __forceinline__ float4 ldf4(GA<T> p, uint32_t i) {
+  const f4v v = *(gptr<const f4v>)(gptr<const void>)(p + i).p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ float4 ldf4(gptr<float4> p, size_t i) { return ldf4((gptr<const float4>)p, i); }
 
 // Physics constants of one scenario (sfincs_input.f90 defaults + derived values).
 struct Consts {
@@ -51,9 +68,9 @@ template <> __device__ __forceinline__ double rpow<double>(double x, double y) {
 // x**(float)(1/3) and x**(float)(5/3): the C library's exact powf results in FP32,
 // ordinary double pow in the FP64 twin.
 // Device view of the powf tables: 64-bit entries {key, value} in the global address space.
-struct PowTablesG { gptr<const uint64_t> t13, t53; };
+struct PowTablesG { GA<const uint64_t> t13, t53; };
 __device__ __forceinline__ PowTablesG pow_tables_g(const PowTables& T) {
-  return PowTablesG{(gptr<const uint64_t>)T.t13, (gptr<const uint64_t>)T.t53};
+  return PowTablesG{ga((const uint64_t*)T.t13), ga((const uint64_t*)T.t53)};
 }
 template <typename R> __device__ __forceinline__ R pow_third(R x, int which, const PowTablesG& T);
 template <> __device__ __forceinline__ float pow_third<float>(float x, int which, const PowTablesG& T) {

@@ -85,6 +85,72 @@ def cpu_baseline(sc, model, inst, nsteps):
                 seconds=secs)
 
 
+# Indexed (quadtree / masked) regular meshes, per uv-point by branch: as the dense regular
+# figures plus the two cell indices (8 B) and the flags byte; wet points also read the
+# eight neighbour indices (32 B). Per cell: zs 8 + zb 4. Continuity per cell: kcs 1,
+# four face indices 16, iref 1, zs r/w 16, zsmax r/w 8, four faces 16.
+IDX_MOM_CELL = 12
+IDX_MOM_UV = {"copied": 17, "dry": 23, "wet": 69, "partial": 69}
+IDX_CONT_CELL = 58
+
+
+def quadtree(args):
+    """Configuration 2: quadtree mesh of about 7.7M active cells (4 levels, 100 m .. 12.5 m,
+    refined where the seeded fBm bed lies within -5..+5 m), Gaussian surge, 20 m/s SW wind.
+    value = cell-updates/s of the whole step with state resident in HBM."""
+    import torch
+    from sfincs_amd import _lib
+    from sfincs_amd import workloads as W
+    t0 = time.perf_counter()
+    sc = W.config2()
+    sc.params["t1"] = 1.0e7
+    model = sc.build_model()
+    sizes = model.sizes()
+    print(f"# generated quadtree: np {sizes['np']} npuv {sizes['npuv']} ncuv {sizes['ncuv']} in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    inst = _lib.Instance(model)
+    batch = _lib.Batch()
+    batch.add(inst, 1.0e9)
+    batch.advance(args.warmup)
+    torch.cuda.synchronize()
+    inst.set_counting(True)
+    ms = inst.profile(args.profile_steps)
+    counts = [c / args.profile_steps for c in inst.counters()]
+    inst.set_counting(False)
+    batch.close()
+    batch = _lib.Batch()
+    batch.add(inst, 1.0e9)
+    batch.advance(8)
+    copied, dry, wet, part = counts
+    mom_b = sizes["np"] * IDX_MOM_CELL + copied * IDX_MOM_UV["copied"] + dry * IDX_MOM_UV["dry"] + \
+        (wet + part) * IDX_MOM_UV["wet"] + sizes["ncuv"] * 36
+    cont_b = sizes["np"] * IDX_CONT_CELL
+    torch.cuda.synchronize()
+    t_a = time.perf_counter()
+    batch.advance(args.steps)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t_a
+    t_model, dt_model, _ = inst.time()
+    assert np.all(np.isfinite(inst.get("zs"))), "non-finite water levels"
+    mom_gbs = mom_b / (ms[1] * 1e-3) / 1e9
+    out = {
+        "metric": "cell-updates/s (mom+cont), quadtree configuration", "value": sizes["np"] * args.steps / secs,
+        "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True, "dtype": "f32 (zs f64)",
+        "data": "synthetic (seeded fBm DEM, PCG64(1002))",
+        "config": {"workload": "config 2: quadtree 4 levels 100..12.5 m, surge 3 m, wind 20 m/s SW",
+                   "np": sizes["np"], "npuv": sizes["npuv"], "ncuv": sizes["ncuv"], "model_time_s": t_model,
+                   "dt_s": dt_model},
+        "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": mom_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_mom_idx<float, regular> + k_cuv",
+                     "bytes_per_launch": mom_b, "launch_ms": ms[1]},
+        "continuity": {"achieved_GBs": cont_b / (ms[2] * 1e-3) / 1e9, "bytes_per_launch": cont_b, "launch_ms": ms[2]},
+        "forcing_launch_ms": ms[0],
+        "branches_per_step": dict(copied=copied, dry=dry, wet=wet, partial=part),
+    }
+    print(json.dumps(out))
+
+
 def ensemble(args):
     """Configuration 3: 64 members of log-uniform size (256^2..2048^2, regular or subgrid),
     each with its own surge amplitude, rain and adaptive dt, 3 h simulated. The same
@@ -175,9 +241,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
-    ap.add_argument("--mode", choices=("step", "ensemble", "serve"), default="step",
-                    help="step: configuration 1 (the default line); ensemble: configuration 3; "
-                         "serve: configuration 4 (scaled by --window/--scale/--duration-scale)")
+    ap.add_argument("--mode", choices=("step", "quadtree", "ensemble", "serve"), default="step",
+                    help="step: configuration 1 (the default line); quadtree: configuration 2; ensemble: "
+                         "configuration 3; serve: configuration 4 (scaled by --window/--scale/--duration-scale)")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--t-end", type=float, default=10800.0)
     ap.add_argument("--scale", type=float, default=1.0)
@@ -199,6 +265,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "quadtree":
+        return quadtree(args)
     if args.mode == "ensemble":
         return ensemble(args)
     if args.mode == "serve":

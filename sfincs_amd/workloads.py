@@ -364,5 +364,7 @@ def quadtree_case(nb, mb, nlev=4, dx0=100.0, seed=1002, t_end=43200.0, refine=(-
                     wind=wnd, t_end=float(t_end))
 
 
-def config2(nb=700, mb=700):
+def config2(nb=512, mb=512):
+    """Configuration 2: quadtree, 4 levels 100 m .. 12.5 m, about 7.7M active cells on a
+    512 x 512 base grid (51.2 km square)."""
     return quadtree_case(nb, mb, nlev=4, dx0=100.0, seed=1002, t_end=43200.0)

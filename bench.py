@@ -348,7 +348,8 @@ def main():
                    "model_time_s": t_model, "dt_s": dt_model},
         "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": mom_gbs / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_mom_strip<float, subgrid>" if tuning["momentum_strip_rows"] else "k_mom_cell<float, subgrid>",
+                     "kernel": {-1: "k_mom_point<float, subgrid>", 0: "k_mom_cell<float, subgrid>"}.get(
+                         tuning["momentum_kernel"], "k_mom_strip<float, subgrid>"),
                      "bytes_per_launch": mom_b, "launch_ms": ms[1]},
         "continuity": {"achieved_GBs": cont_gbs, "frac": cont_gbs / PEAK_HBM_GBS, "bytes_per_launch": cont_b,
                        "launch_ms": ms[2]},

@@ -206,17 +206,18 @@ int  sfx_profile_steps(sfx_handle* h, int32_t nsteps, double ms[3]);
  * zsderv, zsmax), used to state the FP32 path's rounding error. */
 int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle** out);
 
-/* Autotuner: on this handle's mesh, times the dense momentum kernels (per-cell tiles of
- * several shapes, strip marches of several lengths) and the tile shapes of the dense
- * continuity kernel, and keeps the fastest. Call it before the handle joins a batch
- * (SFX_ERR_STATE otherwise). */
+/* Autotuner: on this handle's mesh, times the dense momentum kernels (one thread per cell
+ * or per uv-point on tiles of several shapes, strip marches of several lengths) and the
+ * tile shapes of the dense continuity kernel, and keeps the fastest. Call it before the
+ * handle joins a batch (SFX_ERR_STATE otherwise). */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
-/* out = {dense, momentum strip rows per wave (0: per-cell kernel), momentum tile n,
- *        momentum tile m, continuity tile n, continuity tile m} */
+/* out = {dense, momentum kernel (-1: per uv-point, 0: per cell, k > 0: strip march of k
+ *        rows per wave), momentum tile n, momentum tile m, continuity tile n, continuity
+ *        tile m} */
 int  sfx_get_tuning(sfx_handle* h, int32_t out[6]);
-/* Pins the dense shapes without timing: in = {strip rows (0: per-cell kernel), momentum
- * tile n, momentum tile m, continuity tile n, continuity tile m}; tiles hold 256 cells.
- * No effect on indexed meshes. Call it before the handle joins a batch. */
+/* Pins the dense shapes without timing: in = {momentum kernel as above, momentum tile n,
+ * momentum tile m, continuity tile n, continuity tile m}; tiles hold 256 cells. No effect
+ * on indexed meshes. Call it before the handle joins a batch. */
 int  sfx_set_tuning(sfx_handle* h, const int32_t in[5]);
 
 int  sfx_last_error(char* buf, size_t n);

@@ -351,10 +351,13 @@ class Instance:
         check(lib().sfx_autotune(self.h, trials), "autotune")
         a = (C.c_int32 * 6)()
         check(lib().sfx_get_tuning(self.h, a), "get_tuning")
-        return dict(dense=a[0], momentum_strip_rows=a[1], momentum_tile=[a[2], a[3]], continuity_tile=[a[4], a[5]])
+        return dict(dense=a[0], momentum_kernel=a[1], momentum_tile=[a[2], a[3]], continuity_tile=[a[4], a[5]])
 
-    def set_tuning(self, strip_rows=0, momentum_tile=(64, 4), continuity_tile=(64, 4)):
-        a = (C.c_int32 * 5)(strip_rows, momentum_tile[0], momentum_tile[1], continuity_tile[0], continuity_tile[1])
+    def set_tuning(self, momentum_kernel=0, momentum_tile=(64, 4), continuity_tile=(64, 4)):
+        """momentum_kernel: -1 one thread per uv-point, 0 one thread per cell, k > 0 strip
+        march of k rows per wave."""
+        a = (C.c_int32 * 5)(momentum_kernel, momentum_tile[0], momentum_tile[1], continuity_tile[0],
+                            continuity_tile[1])
         check(lib().sfx_set_tuning(self.h, a), "set_tuning")
 
     def profile(self, nsteps):

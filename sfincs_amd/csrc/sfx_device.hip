@@ -72,7 +72,7 @@ struct DevInst {
   int32_t dense, subgrid, fp64;
   int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
   int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
-  int32_t sk, nstrip;            // dense momentum: rows per wave of the strip march (0: per-cell kernel), strips
+  int32_t sk, nstrip;            // dense momentum kernel: strip march with sk rows per wave (sk > 0), per-cell (0), per-point (-1)
   int32_t mtn, mtm, mntn;        // dense momentum, per-cell kernel: tile shape and tiles along n
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
@@ -148,7 +148,8 @@ size_t cell_tiles(const DevInst& I, int tn, int tm) {
 // uv-points (indexed).
 size_t mom_tiles(const DevInst& I) {
   if (!I.dense) return (size_t)((I.npuv + BLOCK - 1) / BLOCK);
-  return I.sk > 0 ? strip_tiles(I, I.sk) : cell_tiles(I, I.mtn, I.mtm);
+  if (I.sk > 0) return strip_tiles(I, I.sk);
+  return (I.sk < 0 ? 2 : 1) * cell_tiles(I, I.mtn, I.mtm);
 }
 
 // Continuity tiles: tn x tm cells, or BLOCK cells (indexed).
@@ -537,6 +538,60 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
   }
 }
 
+// Dense scenarios, one thread per uv-point: tiles 0 .. T-1 cover the u-points, tiles T ..
+// 2T-1 the v-points of the same cells (T tiles of mtn x mtm cells). Half the registers of
+// the per-cell kernel, twice the waves to hide memory latency.
+template <typename R, bool SUBGRID>
+__device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, float& mdt) {
+  const auto S = G(I.sc);
+  const int cur = S->cur;
+  const R dt = (R)S->dt;
+  const auto q0 = G((const R*)I.q[cur]);
+  const auto u0 = G((const R*)I.uv[cur]);
+  const auto q1 = G((R*)I.q[cur ^ 1]);
+  const auto u1 = G((R*)I.uv[cur ^ 1]);
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
+  const int ntile = I.mntn * ((mmax + I.mtm - 1) / I.mtm);
+  const int vplane = tile >= ntile;
+  const int t = vplane ? tile - ntile : tile;
+  const int tn = t % I.mntn, tmi = t / I.mntn;
+  const int n = tn * I.mtn + (threadIdx.x % I.mtn);
+  const int m = tmi * I.mtm + (threadIdx.x / I.mtn);
+  if (n >= nmax || m >= mmax) return;
+  const int c = m * nmax + n;
+  const bool hu = m < mmax - 1, hv = n < nmax - 1;
+  sfx::UvIn<R> in;
+  int s;
+  if (!vplane) {   // u-point between c and c+nmax
+    if (!hu) { cnt[0]++; return; }
+    s = c;
+    gather_uv<R, SUBGRID>(I, q0, u0, s, m >= 1 ? c - nmax : dm, m + 1 < mmax - 1 ? c + nmax : dm,
+                          n >= 1 ? c - 1 : c, n < nmax - 1 ? c + 1 : c,
+                          hv ? np + c : dm, n >= 1 ? np + c - 1 : dm, hv ? np + c + nmax : dm,
+                          n >= 1 ? np + c + nmax - 1 : dm, c, c + nmax, in);
+    in.idir = 0;
+  } else {         // v-point between c and c+1
+    if (!hv) { cnt[0]++; return; }
+    s = np + c;
+    gather_uv<R, SUBGRID>(I, q0, u0, s, n >= 1 ? s - 1 : dm, n + 1 < nmax - 1 ? s + 1 : dm,
+                          m >= 1 ? s - nmax : s, m < mmax - 1 ? s + nmax : s,
+                          hu ? c : dm, m >= 1 ? c - nmax : dm, hu ? c + 1 : dm, m >= 1 ? c + 1 - nmax : dm,
+                          c, c + 1, in);
+    in.idir = 1;
+  }
+  const int kc = G(I.kcuv)[s];
+  const float fl = uv_floor<SUBGRID>(I, s);
+  if (kc == 1 || kc == 6) {
+    in.iref = 1; in.itype = 0; in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+    int br = 0;
+    finish_uv<R, SUBGRID>(I, s, in, fl, dt, q1, u1, mdt, br);
+    cnt[br]++;
+  } else {
+    q1[s] = in.qx_nm; u1[s] = in.uu_nm;
+    cnt[0]++;
+  }
+}
+
 // Dense scenarios, strip march. A wave covers columns n = n0-1 .. n0+62 of one strip
 // (lane l -> n0+l-1) and marches over K rows m0 .. m0+K-1; lanes 1..62 own their column,
 // lanes 0 and 63 only supply the neighbour values of the owned lanes. Each lane keeps rows
@@ -778,6 +833,18 @@ __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ 
 }
 
 template <typename R, bool SUBGRID>
+__global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  if (it.inst < 0) return;
+  const DevInst& I = insts[it.inst];
+  if (!G(I.sc)->active) return;
+  float mdt = I.c.dtmax;
+  int cnt[4] = {0, 0, 0, 0};
+  mom_point<R, SUBGRID>(I, it.tile, cnt, mdt);
+  mom_epilogue(I, it.tile, mdt, cnt);
+}
+
+template <typename R, bool SUBGRID>
 __global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
   if (it.inst < 0) return;
@@ -974,8 +1041,8 @@ struct sfx_batch {
   size_t ids_cap = 0;
   // launch descriptors (offsets into dItems)
   // work lists: momentum dense/indexed x subgrid/regular, combined points, continuity
-  // [cell sg, cell reg, strip sg, strip reg, idx sg, idx reg]
-  size_t off_m[6] = {0, 0, 0, 0, 0, 0}, n_m[6] = {0, 0, 0, 0, 0, 0};
+  // [cell sg, cell reg, strip sg, strip reg, idx sg, idx reg, point sg, point reg]
+  size_t off_m[8] = {0, 0, 0, 0, 0, 0, 0, 0}, n_m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   size_t off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
   bool dirty = true;
   hipGraph_t graph = nullptr;
@@ -1238,7 +1305,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   if (M->dense) {
     for (int k = 0; k < N_STRIP_ROWS; ++k) H->nmt_cap = std::max(H->nmt_cap, strip_tiles(I, STRIP_ROWS[k]));
     for (int k = 0; k < N_CELL_SHAPES; ++k)
-      H->nmt_cap = std::max(H->nmt_cap, cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
+      H->nmt_cap = std::max(H->nmt_cap, 2 * cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
   }
   I.blkmin = H->alloc<float>(H->nmt_cap);
   I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
@@ -1449,7 +1516,7 @@ int rebuild(sfx_batch* B) {
   HIPCHK(hipSetDevice(B->device));
   const size_t nm = B->members.size();
   std::vector<DevInst> insts(nm);
-  std::vector<Item> mom[6], cu, cd, ci;
+  std::vector<Item> mom[8], cu, cd, ci;
   std::vector<int32_t> ids(nm);
   int fp64 = -1;
   for (size_t i = 0; i < nm; ++i) {
@@ -1459,7 +1526,7 @@ int rebuild(sfx_batch* B) {
     if (fp64 < 0) fp64 = H->fp64;
     if (fp64 != H->fp64) { set_error("a batch cannot mix FP32 and FP64 members"); return SFX_ERR_ARG; }
     const DevInst& I = H->hI;
-    std::vector<Item>& mv = mom[(I.dense ? (I.sk > 0 ? 2 : 0) : 4) + (I.subgrid ? 0 : 1)];
+    std::vector<Item>& mv = mom[(I.dense ? (I.sk > 0 ? 2 : (I.sk < 0 ? 6 : 0)) : 4) + (I.subgrid ? 0 : 1)];
     const size_t tm = mom_tiles(I), tc = cont_tiles(I);
     for (size_t t = 0; t < tm; ++t) mv.push_back(Item{(int)i, (int)t});
     for (size_t t = 0; t < tc; ++t) (I.dense ? cd : ci).push_back(Item{(int)i, (int)t});
@@ -1470,7 +1537,7 @@ int rebuild(sfx_batch* B) {
   }
   B->fp64 = fp64 < 0 ? 0 : fp64;
   std::vector<Item> all;
-  for (int k = 0; k < 6; ++k) {
+  for (int k = 0; k < 8; ++k) {
     if (mom[k].size()) xcd_order(mom[k]);
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
   }
@@ -1519,6 +1586,8 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
     if (B->n_m[3]) hipLaunchKernelGGL((k_mom_strip<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[3]);
     if (B->n_m[4]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[4]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[4]);
     if (B->n_m[5]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[5]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[5]);
+    if (B->n_m[6]) hipLaunchKernelGGL((k_mom_point<R, true>), dim3(B->n_m[6]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[6]);
+    if (B->n_m[7]) hipLaunchKernelGGL((k_mom_point<R, false>), dim3(B->n_m[7]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[7]);
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cuv);
   }
   if (what & 4) {
@@ -1817,8 +1886,8 @@ void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
 extern "C" int sfx_set_tuning(sfx_handle* H, const int32_t in[5]) {
   if (!H || !in) { set_error("null"); return SFX_ERR_ARG; }
   const int sk = in[0], mtn = in[1], mtm = in[2], tn = in[3], tm = in[4];
-  if (sk < 0 || sk > 256 || mtn <= 0 || mtm <= 0 || mtn * mtm != BLOCK || tn <= 0 || tm <= 0 || tn * tm != BLOCK) {
-    set_error("sfx_set_tuning: strip rows must be 0..256 and tiles must hold 256 cells");
+  if (sk < -1 || sk > 256 || mtn <= 0 || mtm <= 0 || mtn * mtm != BLOCK || tn <= 0 || tm <= 0 || tn * tm != BLOCK) {
+    set_error("sfx_set_tuning: the momentum kernel must be -1, 0 or 1..256 strip rows and tiles must hold 256 cells");
     return SFX_ERR_ARG;
   }
   if (H->in_batches) { set_error("sfx_set_tuning: set shapes before adding the scenario to a batch"); return SFX_ERR_STATE; }
@@ -1872,9 +1941,11 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   // momentum: per-cell tiles (sk = 0) and strip marches
   float best = 1e30f;
   int bsk = 0, bmtn = CELL_SHAPES[0][0], bmtm = CELL_SHAPES[0][1];
-  for (int k = 0; k < N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
-    const int sk = k < N_CELL_SHAPES ? 0 : STRIP_ROWS[k - N_CELL_SHAPES];
-    const int mtn = k < N_CELL_SHAPES ? CELL_SHAPES[k][0] : bmtn, mtm = k < N_CELL_SHAPES ? CELL_SHAPES[k][1] : bmtm;
+  for (int k = 0; k < 2 * N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
+    const bool tiled = k < 2 * N_CELL_SHAPES;
+    const int sk = tiled ? (k < N_CELL_SHAPES ? 0 : -1) : STRIP_ROWS[k - 2 * N_CELL_SHAPES];
+    const int mtn = tiled ? CELL_SHAPES[k % N_CELL_SHAPES][0] : bmtn;
+    const int mtm = tiled ? CELL_SHAPES[k % N_CELL_SHAPES][1] : bmtm;
     set_shapes(H, sk, mtn, mtm, tn0, tm0);
     float ms = 0.f;
     rc = time_phases(H, 2, trials, e0, e1, ms);

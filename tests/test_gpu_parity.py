@@ -239,12 +239,13 @@ def test_branch_counters_cover_every_point():
 
 
 SHAPES = [(0, (64, 4), (64, 4)), (0, (256, 1), (128, 2)), (0, (32, 8), (256, 1)), (4, (64, 4), (32, 8)),
-          (8, (64, 4), (64, 4)), (16, (64, 4), (64, 4))]
+          (8, (64, 4), (64, 4)), (16, (64, 4), (64, 4)), (-1, (64, 4), (64, 4)), (-1, (32, 8), (128, 2)),
+          (-1, (256, 1), (64, 4))]
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=[f"strip{a}_mom{b[0]}x{b[1]}_cont{c[0]}x{c[1]}" for a, b, c in SHAPES])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"kernel{a}_mom{b[0]}x{b[1]}_cont{c[0]}x{c[1]}" for a, b, c in SHAPES])
 def test_every_dense_kernel_shape_matches_oracle(shape):
-    """Each dense momentum variant (per-cell tiles, strip marches) and continuity tile the
+    """Each dense momentum variant (per-cell and per-point tiles, strip marches) and continuity tile the
     autotuner can pick gives the oracle's result; the grid (70 x 83) is not a multiple of
     any tile, so partial tiles and strips are exercised."""
     strip, mt, ct = shape

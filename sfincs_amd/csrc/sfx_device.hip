@@ -418,18 +418,10 @@ __device__ __forceinline__ float uv_floor(const DevInst& I, int slot) {
 
 // Completes one active uv-point from its gathered inputs and its preloaded wet/dry
 // threshold `floor` (uv_floor): dry points get q = uv = 0, wet points the full update.
+// The update of a wet uv-point (zsu from uv_wet): its tables, the physics, the stores.
 template <typename R, bool SUBGRID, typename QP>
-__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, float floor, R dt,
-                                          QP q1, QP uv1, float& mdt, int& br) {
-  sfx::UvTables tb;
-  if (SUBGRID) tb.zmin = floor;
-  else tb.zbuvmx = floor;
-  R zsu;
-  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
-    q1[slot] = R(0); uv1[slot] = R(0); G(I.kfuv)[slot] = 0;
-    br = 1;
-    return;
-  }
+__device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, sfx::UvTables& tb, R zsu,
+                                       R dt, QP q1, QP uv1, float& mdt, int& br) {
   if (SUBGRID) {
     const float4 s4 = ldf4(G(I.uv_s4), slot);
     tb.zmax = s4.x; tb.havg_zmax = s4.y; tb.nrep_zmax = s4.z; tb.navg_w = s4.w; tb.fnfit = G(I.uv_fnfit)[slot];
@@ -442,6 +434,21 @@ __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx:
   q1[slot] = o.q; uv1[slot] = o.uv; G(I.kfuv)[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
   br = o.branch;
+}
+
+template <typename R, bool SUBGRID, typename QP>
+__device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, float floor, R dt,
+                                          QP q1, QP uv1, float& mdt, int& br) {
+  sfx::UvTables tb;
+  if (SUBGRID) tb.zmin = floor;
+  else tb.zbuvmx = floor;
+  R zsu;
+  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
+    q1[slot] = R(0); uv1[slot] = R(0); G(I.kfuv)[slot] = 0;
+    br = 1;
+    return;
+  }
+  wet_uv<R, SUBGRID>(I, slot, in, tb, zsu, dt, q1, uv1, mdt, br);
 }
 
 // Dense scenarios, one thread per cell: the u-point on its right and the v-point above
@@ -539,8 +546,10 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 }
 
 // Dense scenarios, one thread per uv-point: tiles 0 .. T-1 cover the u-points, tiles T ..
-// 2T-1 the v-points of the same cells (T tiles of mtn x mtm cells). Half the registers of
-// the per-cell kernel, twice the waves to hide memory latency.
+// 2T-1 the v-points of the same cells (T tiles of mtn x mtm cells). A point first reads
+// what decides its branch (kcuv, the two water levels, its wet/dry threshold); only wet
+// points read the rest of their stencil, so waves without wet points finish after one
+// round trip to memory.
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
@@ -560,36 +569,57 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
   if (n >= nmax || m >= mmax) return;
   const int c = m * nmax + n;
   const bool hu = m < mmax - 1, hv = n < nmax - 1;
-  sfx::UvIn<R> in;
-  int s;
-  if (!vplane) {   // u-point between c and c+nmax
-    if (!hu) { cnt[0]++; return; }
-    s = c;
-    gather_uv<R, SUBGRID>(I, q0, u0, s, m >= 1 ? c - nmax : dm, m + 1 < mmax - 1 ? c + nmax : dm,
-                          n >= 1 ? c - 1 : c, n < nmax - 1 ? c + 1 : c,
-                          hv ? np + c : dm, n >= 1 ? np + c - 1 : dm, hv ? np + c + nmax : dm,
-                          n >= 1 ? np + c + nmax - 1 : dm, c, c + nmax, in);
-    in.idir = 0;
-  } else {         // v-point between c and c+1
-    if (!hv) { cnt[0]++; return; }
-    s = np + c;
-    gather_uv<R, SUBGRID>(I, q0, u0, s, n >= 1 ? s - 1 : dm, n + 1 < nmax - 1 ? s + 1 : dm,
-                          m >= 1 ? s - nmax : s, m < mmax - 1 ? s + nmax : s,
-                          hu ? c : dm, m >= 1 ? c - nmax : dm, hu ? c + 1 : dm, m >= 1 ? c + 1 - nmax : dm,
-                          c, c + 1, in);
-    in.idir = 1;
-  }
+  if (vplane ? !hv : !hu) { cnt[0]++; return; }
+  const int s = vplane ? np + c : c;
+  const int c_nmu = vplane ? c + 1 : c + nmax;
+  // branch inputs
   const int kc = G(I.kcuv)[s];
-  const float fl = uv_floor<SUBGRID>(I, s);
-  if (kc == 1 || kc == 6) {
-    in.iref = 1; in.itype = 0; in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
-    int br = 0;
-    finish_uv<R, SUBGRID>(I, s, in, fl, dt, q1, u1, mdt, br);
-    cnt[br]++;
-  } else {
-    q1[s] = in.qx_nm; u1[s] = in.uu_nm;
+  sfx::UvIn<R> in;
+  in.zs_nm = G(I.zs)[c]; in.zs_nmu = G(I.zs)[c_nmu];
+  sfx::UvTables tb;
+  if (SUBGRID) tb.zmin = G(I.uv_zmin)[s];
+  else tb.zbuvmx = G(I.zbuvmx)[s];
+  if (!(kc == 1 || kc == 6)) {
+    q1[s] = q0[s]; u1[s] = u0[s];
     cnt[0]++;
+    return;
   }
+  R zsu;
+  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
+    q1[s] = R(0); u1[s] = R(0); G(I.kfuv)[s] = 0;
+    cnt[1]++;
+    return;
+  }
+  // the rest of the stencil of a wet point
+  int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
+  if (!vplane) {
+    s_nmd = m >= 1 ? c - nmax : dm; s_nmu = m + 1 < mmax - 1 ? c + nmax : dm;
+    s_ndm = n >= 1 ? c - 1 : c; s_num = n < nmax - 1 ? c + 1 : c;
+    v_nm = hv ? np + c : dm; v_ndm = n >= 1 ? np + c - 1 : dm;
+    v_nmu = hv ? np + c + nmax : dm; v_ndmu = n >= 1 ? np + c + nmax - 1 : dm;
+  } else {
+    s_nmd = n >= 1 ? s - 1 : dm; s_nmu = n + 1 < nmax - 1 ? s + 1 : dm;
+    s_ndm = m >= 1 ? s - nmax : s; s_num = m < mmax - 1 ? s + nmax : s;
+    v_nm = hu ? c : dm; v_ndm = m >= 1 ? c - nmax : dm;
+    v_nmu = hu ? c + 1 : dm; v_ndmu = m >= 1 ? c + 1 - nmax : dm;
+  }
+  in.qx_nm = q0[s]; in.uu_nm = u0[s];
+  in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
+  in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
+  in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
+  in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
+  in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
+  in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s];
+  if (SUBGRID) {
+    in.vol_nm = G(I.vol)[c]; in.vol_nmu = G(I.vol)[c_nmu];
+    in.zsderv_nm = G((const R*)I.zsderv)[c]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
+  } else {
+    in.zb_nm = G(I.zb)[c]; in.zb_nmu = G(I.zb)[c_nmu];
+  }
+  in.idir = vplane; in.iref = 1; in.itype = 0; in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+  int br = 0;
+  wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br);
+  cnt[br]++;
 }
 
 // Dense scenarios, strip march. A wave covers columns n = n0-1 .. n0+62 of one strip
@@ -744,6 +774,8 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
 }
 
 // Indexed scenarios: one thread per uv-point, neighbours through the reference's tables.
+// As on dense meshes, the branch inputs come first and only wet points read the
+// neighbour tables and values.
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
@@ -754,48 +786,56 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
   const auto q1 = G((R*)I.q[cur ^ 1]);
   const auto u1 = G((R*)I.uv[cur ^ 1]);
   const int ip = tile * BLOCK + threadIdx.x;
-  int br = 0;
   const int npuv = I.npuv;
-  if (ip < npuv) {
-    const int kc = G(I.kcuv)[ip];
-    const float fl = uv_floor<SUBGRID>(I, ip);
-    if (kc == 1 || kc == 6) {
-      const auto nb = G(I.nb);
-      const int nm = nb[ip], nmu = nb[npuv + ip];
-      sfx::UvIn<R> in;
-      in.zs_nm = G(I.zs)[nm]; in.zs_nmu = G(I.zs)[nmu];
-      const uint8_t f = G(I.uvf)[ip];
-      in.iref = (I.c.flags & sfx::F_QUADTREE) ? (f & 15) : 1;
-      in.itype = (I.c.flags & sfx::F_QUADTREE) ? ((f >> 4) & 3) - 1 : 0;
-      in.idir = (f >> 6) & 1;
-      in.qx_nm = q0[ip]; in.uu_nm = u0[ip];
-      if (I.c.flags & sfx::F_NBRS) {
-        const int a = nb[2 * npuv + ip], b = nb[3 * npuv + ip], cdn = nb[4 * npuv + ip], d = nb[5 * npuv + ip];
-        const int e = nb[6 * npuv + ip], f2 = nb[7 * npuv + ip], g = nb[8 * npuv + ip], h = nb[9 * npuv + ip];
-        in.qx_nmd = q0[a]; in.qx_nmu = q0[b];
-        in.qy_nm = q0[f2]; in.qy_nmu = q0[h]; in.qy_ndm = q0[e]; in.qy_ndmu = q0[g];
-        in.uu_nmd = u0[a]; in.uu_nmu = u0[b]; in.uu_ndm = u0[cdn]; in.uu_num = u0[d];
-        in.v_ndm = u0[e]; in.v_ndmu = u0[g]; in.v_nm = u0[f2]; in.v_nmu = u0[h];
-        in.kcuv_nmd = G(I.kcuv)[a]; in.kcuv_nmu = G(I.kcuv)[b];
-      } else {
-        in.qx_nmd = in.qx_nmu = in.qy_nm = in.qy_nmu = in.qy_ndm = in.qy_ndmu = R(0);
-        in.uu_nmd = in.uu_nmu = in.uu_ndm = in.uu_num = in.v_ndm = in.v_ndmu = in.v_nm = in.v_nmu = R(0);
-        in.kcuv_nmd = in.kcuv_nmu = 0;
-      }
-      in.kfuv = G(I.kfuv)[ip]; in.mask_adv = G(I.mask_adv)[ip];
-      in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
-      if (SUBGRID) {
-        in.vol_nm = G(I.vol)[nm]; in.vol_nmu = G(I.vol)[nmu];
-        in.zsderv_nm = (G((const R*)I.zsderv))[nm]; in.zsderv_nmu = (G((const R*)I.zsderv))[nmu];
-      } else {
-        in.zb_nm = G(I.zb)[nm]; in.zb_nmu = G(I.zb)[nmu];
-      }
-      finish_uv<R, SUBGRID>(I, ip, in, fl, dt, q1, u1, mdt, br);
-    } else {
-      q1[ip] = q0[ip]; u1[ip] = u0[ip];
-    }
+  if (ip >= npuv) return;
+  const auto nb = G(I.nb);
+  const int kc = G(I.kcuv)[ip];
+  const int nm = nb[ip], nmu = nb[npuv + ip];
+  sfx::UvIn<R> in;
+  in.zs_nm = G(I.zs)[nm]; in.zs_nmu = G(I.zs)[nmu];
+  sfx::UvTables tb;
+  if (SUBGRID) tb.zmin = G(I.uv_zmin)[ip];
+  else tb.zbuvmx = G(I.zbuvmx)[ip];
+  if (!(kc == 1 || kc == 6)) {
+    q1[ip] = q0[ip]; u1[ip] = u0[ip];
+    cnt[0]++;
+    return;
   }
-  if (ip < npuv) cnt[br]++;
+  R zsu;
+  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
+    q1[ip] = R(0); u1[ip] = R(0); G(I.kfuv)[ip] = 0;
+    cnt[1]++;
+    return;
+  }
+  const uint8_t f = G(I.uvf)[ip];
+  in.iref = (I.c.flags & sfx::F_QUADTREE) ? (f & 15) : 1;
+  in.itype = (I.c.flags & sfx::F_QUADTREE) ? ((f >> 4) & 3) - 1 : 0;
+  in.idir = (f >> 6) & 1;
+  in.qx_nm = q0[ip]; in.uu_nm = u0[ip];
+  if (I.c.flags & sfx::F_NBRS) {
+    const int a = nb[2 * npuv + ip], b = nb[3 * npuv + ip], cdn = nb[4 * npuv + ip], d = nb[5 * npuv + ip];
+    const int e = nb[6 * npuv + ip], f2 = nb[7 * npuv + ip], g = nb[8 * npuv + ip], h = nb[9 * npuv + ip];
+    in.qx_nmd = q0[a]; in.qx_nmu = q0[b];
+    in.qy_nm = q0[f2]; in.qy_nmu = q0[h]; in.qy_ndm = q0[e]; in.qy_ndmu = q0[g];
+    in.uu_nmd = u0[a]; in.uu_nmu = u0[b]; in.uu_ndm = u0[cdn]; in.uu_num = u0[d];
+    in.v_ndm = u0[e]; in.v_ndmu = u0[g]; in.v_nm = u0[f2]; in.v_nmu = u0[h];
+    in.kcuv_nmd = G(I.kcuv)[a]; in.kcuv_nmu = G(I.kcuv)[b];
+  } else {
+    in.qx_nmd = in.qx_nmu = in.qy_nm = in.qy_nmu = in.qy_ndm = in.qy_ndmu = R(0);
+    in.uu_nmd = in.uu_nmu = in.uu_ndm = in.uu_num = in.v_ndm = in.v_ndmu = in.v_nm = in.v_nmu = R(0);
+    in.kcuv_nmd = in.kcuv_nmu = 0;
+  }
+  in.kfuv = G(I.kfuv)[ip]; in.mask_adv = G(I.mask_adv)[ip];
+  in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+  if (SUBGRID) {
+    in.vol_nm = G(I.vol)[nm]; in.vol_nmu = G(I.vol)[nmu];
+    in.zsderv_nm = G((const R*)I.zsderv)[nm]; in.zsderv_nmu = G((const R*)I.zsderv)[nmu];
+  } else {
+    in.zb_nm = G(I.zb)[nm]; in.zb_nmu = G(I.zb)[nmu];
+  }
+  int br = 0;
+  wet_uv<R, SUBGRID>(I, ip, in, tb, zsu, dt, q1, u1, mdt, br);
+  cnt[br]++;
 }
 
 // Block epilogue of a momentum tile: the tile's minimum time step into blkmin[tile] (NaN

@@ -37,6 +37,16 @@ MOM_UV = {"copied": 17, "dry": 14, "wet": {True: 44, False: 28}, "partial": 72}
 CONT_CELL = {True: 81, False: 33}
 
 
+def traffic_of(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), or None."""
+    tf = os.path.join(ROOT, "profiles", "momentum_traffic.json")
+    if not os.path.exists(tf):
+        return None
+    with open(tf) as f:
+        return json.load(f).get("bytes_per_launch", {}).get(kernel)
+
+
 def byte_model(sizes, counts_per_step):
     sg = bool(sizes["subgrid"])
     np_ = sizes["np"]
@@ -114,9 +124,10 @@ def quadtree(args):
     batch.advance(args.warmup)
     torch.cuda.synchronize()
     inst.set_counting(True)
-    ms = inst.profile(args.profile_steps)
+    inst.profile(args.profile_steps)
     counts = [c / args.profile_steps for c in inst.counters()]
     inst.set_counting(False)
+    ms = inst.profile(args.profile_steps)
     batch.close()
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
@@ -142,7 +153,8 @@ def quadtree(args):
                    "np": sizes["np"], "npuv": sizes["npuv"], "ncuv": sizes["ncuv"], "model_time_s": t_model,
                    "dt_s": dt_model},
         "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": mom_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_mom_idx<float, regular> + k_cuv",
+                     "frac": mom_gbs / PEAK_HBM_GBS, "traffic": traffic_of("k_mom_idx<float, false>"),
+                     "kernel": "k_mom_idx<float, regular> + k_cuv",
                      "bytes_per_launch": mom_b, "launch_ms": ms[1]},
         "continuity": {"achieved_GBs": cont_b / (ms[2] * 1e-3) / 1e9, "bytes_per_launch": cont_b, "launch_ms": ms[2]},
         "forcing_launch_ms": ms[0],
@@ -162,14 +174,19 @@ def ensemble(args):
     from sfincs_amd import workloads as W
     t0 = time.perf_counter()
     scs = W.ensemble_members(args.members, t_end=args.t_end, scale=args.scale)
-    models = [sc.build_model() for sc in scs]
+    models = []
+    for j, sc in enumerate(scs):
+        models.append(sc.build_model())
+        print(f"# member {j}: {sc.name}", file=sys.stderr, flush=True)
     t_ends = [sc.t_end for sc in scs]
     cells = [m.sizes()["np"] for m in models]
     print(f"# generated {len(models)} members, {sum(cells)} cells in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
+    print("# running the members one by one", file=sys.stderr, flush=True)
     seq = E.run_sequential(models, t_ends, keep_state=True)
     print(f"# sequential {seq['wall_s']:.2f}s {seq['cell_updates_per_s']:.3e}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    print("# running the members in one batch", file=sys.stderr, flush=True)
     bat = E.run_batched(models, t_ends, round_steps=args.round_steps, keep_state=True)
     print(f"# batched {bat['wall_s']:.2f}s {bat['cell_updates_per_s']:.3e}", file=sys.stderr, flush=True)
     dzs = max(float(np.max(np.abs(a - b))) for a, b in zip(seq["zs"], bat["zs"]))
@@ -217,7 +234,12 @@ def serve(args):
     srv = E.Server(budget, round_steps=args.round_steps, workers=args.workers)
     for a, sc in reqs:
         srv.submit(a, sc)
-    res = srv.run()
+    print(f"# serving {len(reqs)} requests", file=sys.stderr, flush=True)
+
+    def progress(event, info):
+        print(f"# {event} {info}", file=sys.stderr, flush=True)
+
+    res = srv.run(progress=progress)
     out = {
         "metric": "served stream: p50/p99 time-to-result and aggregate cell-updates/s",
         "value": res["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True,
@@ -288,11 +310,13 @@ def main():
 
     batch.advance(args.warmup)
     torch.cuda.synchronize()
-    # live kernel timing with HIP events on the launch stream + branch counters
+    # branch counters over a short run, then live kernel timing with HIP events on the
+    # launch stream (counting off)
     inst.set_counting(True)
-    ms = inst.profile(args.profile_steps)
+    inst.profile(args.profile_steps)
     counts = [c / args.profile_steps for c in inst.counters()]
     inst.set_counting(False)
+    ms = inst.profile(args.profile_steps)
     batch.close()
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
@@ -324,11 +348,9 @@ def main():
     mom_gbs = mom_b / (ms[1] * 1e-3) / 1e9
     cont_gbs = cont_b / (ms[2] * 1e-3) / 1e9
     step_gbs = (mom_b + cont_b) / (secs / args.steps) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "momentum_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    kernel = {-1: "k_mom_point<float, true>", 0: "k_mom_cell<float, true>"}.get(tuning["momentum_kernel"],
+                                                                             "k_mom_strip<float, true>")
+    traffic = traffic_of(kernel)
     out = {
         "metric": "cell-updates/s (mom+cont) & % HBM peak; ensemble p50/p99 time-to-result",
         "value": value,
@@ -348,8 +370,7 @@ def main():
                    "model_time_s": t_model, "dt_s": dt_model},
         "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": mom_gbs / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": {-1: "k_mom_point<float, subgrid>", 0: "k_mom_cell<float, subgrid>"}.get(
-                         tuning["momentum_kernel"], "k_mom_strip<float, subgrid>"),
+                     "kernel": kernel,
                      "bytes_per_launch": mom_b, "launch_ms": ms[1]},
         "continuity": {"achieved_GBs": cont_gbs, "frac": cont_gbs / PEAK_HBM_GBS, "bytes_per_launch": cont_b,
                        "launch_ms": ms[2]},

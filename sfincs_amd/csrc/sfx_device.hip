@@ -190,10 +190,14 @@ __device__ void fold_sweep(const DevInst& I) {
   float v = I.c.dtmax;
   const auto p = G(I.blkmin);
   const int n = I.nmt;
-  int i = threadIdx.x;
-  for (; i + 3 * FBLOCK < n; i += 4 * FBLOCK)
-    v = fminf(fminf(v, fminf(p[i], p[i + FBLOCK])), fminf(p[i + 2 * FBLOCK], p[i + 3 * FBLOCK]));
-  for (; i < n; i += FBLOCK) v = fminf(v, p[i]);
+  // the first 32 entries of each thread in one round trip (clamped indices repeat the
+  // last entry, which leaves the minimum unchanged), the rest in a loop
+  float w[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) w[j] = p[min((int)threadIdx.x + j * FBLOCK, n - 1)];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) v = fminf(v, w[j]);
+  for (int i = threadIdx.x + 32 * FBLOCK; i < n; i += FBLOCK) v = fminf(v, p[i]);
   v = block_min<FBLOCK>(v);
   if (threadIdx.x == 0) { S->cur ^= 1; S->mom_done = 0; S->min_dt = v; }
   __syncthreads();
@@ -545,8 +549,9 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
   }
 }
 
-// Dense scenarios, one thread per uv-point: tiles 0 .. T-1 cover the u-points, tiles T ..
-// 2T-1 the v-points of the same cells (T tiles of mtn x mtm cells). A point first reads
+// Dense scenarios, one thread per uv-point: tile 2t covers the u-points and tile 2t+1 the
+// v-points of cell tile t (mtn x mtm cells), so both planes of a tile run next to each
+// other on one XCD and share its cached cell data. A point first reads
 // what decides its branch (kcuv, the two water levels, its wet/dry threshold); only wet
 // points read the rest of their stencil, so waves without wet points finish after one
 // round trip to memory.
@@ -560,9 +565,8 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
   const auto q1 = G((R*)I.q[cur ^ 1]);
   const auto u1 = G((R*)I.uv[cur ^ 1]);
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
-  const int ntile = I.mntn * ((mmax + I.mtm - 1) / I.mtm);
-  const int vplane = tile >= ntile;
-  const int t = vplane ? tile - ntile : tile;
+  const int vplane = tile & 1;
+  const int t = tile >> 1;
   const int tn = t % I.mntn, tmi = t / I.mntn;
   const int n = tn * I.mtn + (threadIdx.x % I.mtn);
   const int m = tmi * I.mtm + (threadIdx.x / I.mtn);
@@ -845,9 +849,13 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
   __shared__ unsigned int bc[4];
   if (threadIdx.x < 4) bc[threadIdx.x] = 0;
   const float m = block_min<BLOCK>(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);   // contains a barrier
-  if (I.count)
-    for (int k = 0; k < 4; ++k)
-      if (cnt[k]) atomicAdd(&bc[k], (unsigned int)cnt[k]);
+  if (I.count) {   // wave sums, one LDS add per wave and branch
+    for (int k = 0; k < 4; ++k) {
+      int v = cnt[k];
+      for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+      if (threadIdx.x % WAVE == 0 && v) atomicAdd(&bc[k], (unsigned int)v);
+    }
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     G(I.blkmin)[tile] = m;

@@ -105,7 +105,9 @@ class Server:
         heapq.heappush(self.queue, (arrival, self.seq, scenario))
         self.seq += 1
 
-    def run(self, clock=None):
+    def run(self, clock=None, progress=None):
+        """Serves every submitted request. `progress(event, info)` is called on each
+        admission and completion when given."""
         clock = clock or time.perf_counter
         t0 = clock()
         batch = _lib.Batch(self.device)
@@ -134,6 +136,9 @@ class Server:
                 batch.add(inst, sc.t_end)
                 live[seq] = (inst, model, arrival, need, sc, clock() - t0)
                 used += need
+                if progress:
+                    progress("admit", dict(seq=seq, live=len(live), used=used, pending=len(pending),
+                                           queued=len(self.queue)))
             max_live = max(max_live, len(live))
             max_used = max(max_used, used)
             if not live:
@@ -155,6 +160,8 @@ class Server:
                     if self.keep_state:
                         r["zs"] = inst.get("zs")
                     results.append(r)
+                    if progress:
+                        progress("done", dict(seq=seq, ttr=r["ttr"], steps=nt, done=len(results)))
                     batch.remove(inst)
                     inst.close()
                     model.close()

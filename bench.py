@@ -260,7 +260,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=2048)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=1200,
+                    help="steps of the reference's CPU path timed for cpu_baseline (about 10 s of work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--mode", choices=("step", "quadtree", "ensemble", "serve"), default="step",

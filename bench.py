@@ -213,6 +213,15 @@ def serve(args):
     import torch
     from sfincs_amd import ensemble as E
     from sfincs_amd import workloads as W
+    class Request:
+        """A request as it arrives: its recipe; the server builds the model after arrival."""
+
+        def __init__(self, make, t_end):
+            self.make, self.t_end = make, t_end
+
+        def build_model(self):
+            return self.make().build_model()
+
     r = W.rng(1004)
     t = 0.0
     reqs = []
@@ -222,12 +231,14 @@ def serve(args):
         if t > args.window:
             break
         dur = float(r.uniform(3600.0, 43200.0)) * args.duration_scale
+        seed = 1004 * 100000 + j
         if r.uniform() < 0.1:
             nb = int(r.integers(40, 120) * args.scale) + 4
-            sc = W.quadtree_case(nb, nb, nlev=4, dx0=100.0, seed=1004 * 100000 + j, t_end=dur, wind=False)
+            make = (lambda nb=nb, seed=seed, dur=dur:
+                    W.quadtree_case(nb, nb, nlev=4, dx0=100.0, seed=seed, t_end=dur, wind=False))
         else:
-            sc = W.member(1004 * 100000 + j, t_end=dur, scale=args.scale, name=f"request{j}")
-        reqs.append((t, sc))
+            make = (lambda seed=seed, dur=dur, j=j: W.member(seed, t_end=dur, scale=args.scale, name=f"request{j}"))
+        reqs.append((t, Request(make, dur)))
         j += 1
     total, free = torch.cuda.mem_get_info()
     budget = int(min(args.hbm_fraction * free, free - (8 << 30)))

@@ -128,7 +128,7 @@ struct Item { int32_t inst, tile; };
 // Dense tile shapes (cells along n x along m, BLOCK threads) of the per-cell momentum and
 // continuity kernels, and strip-march lengths (rows per wave) of the strip momentum
 // kernel, that the autotuner tries; the first of each is the default.
-constexpr int CELL_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}};
+constexpr int CELL_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {16, 16}};
 constexpr int N_CELL_SHAPES = sizeof(CELL_SHAPES) / sizeof(CELL_SHAPES[0]);
 constexpr int STRIP_ROWS[] = {4, 8, 16};
 constexpr int N_STRIP_ROWS = sizeof(STRIP_ROWS) / sizeof(STRIP_ROWS[0]);

@@ -240,7 +240,7 @@ def test_branch_counters_cover_every_point():
 
 SHAPES = [(0, (64, 4), (64, 4)), (0, (256, 1), (128, 2)), (0, (32, 8), (256, 1)), (4, (64, 4), (32, 8)),
           (8, (64, 4), (64, 4)), (16, (64, 4), (64, 4)), (-1, (64, 4), (64, 4)), (-1, (32, 8), (128, 2)),
-          (-1, (256, 1), (64, 4))]
+          (-1, (256, 1), (64, 4)), (-1, (16, 16), (16, 16)), (0, (16, 16), (32, 8))]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"kernel{a}_mom{b[0]}x{b[1]}_cont{c[0]}x{c[1]}" for a, b, c in SHAPES])

@@ -1081,6 +1081,8 @@ struct sfx_batch {
   bool own_stream = true;
   std::vector<sfx_handle*> members;
   std::vector<double> tend;
+  std::vector<char> live;        // members short of their end time; finished ones leave the work lists
+  int n_live = 0;
   DevInst* dInsts = nullptr;
   size_t dInsts_cap = 0;
   Item* dItems = nullptr;
@@ -1565,14 +1567,16 @@ int rebuild(sfx_batch* B) {
   const size_t nm = B->members.size();
   std::vector<DevInst> insts(nm);
   std::vector<Item> mom[8], cu, cd, ci;
-  std::vector<int32_t> ids(nm);
+  std::vector<int32_t> ids;
   int fp64 = -1;
+  if (B->live.size() != nm) B->live.assign(nm, 1);
   for (size_t i = 0; i < nm; ++i) {
     sfx_handle* H = B->members[i];
     insts[i] = H->hI;
-    ids[i] = (int32_t)i;
     if (fp64 < 0) fp64 = H->fp64;
     if (fp64 != H->fp64) { set_error("a batch cannot mix FP32 and FP64 members"); return SFX_ERR_ARG; }
+    if (!B->live[i]) continue;
+    ids.push_back((int32_t)i);
     const DevInst& I = H->hI;
     std::vector<Item>& mv = mom[(I.dense ? (I.sk > 0 ? 2 : (I.sk < 0 ? 6 : 0)) : 4) + (I.subgrid ? 0 : 1)];
     const size_t tm = mom_tiles(I), tc = cont_tiles(I);
@@ -1612,7 +1616,8 @@ int rebuild(sfx_batch* B) {
   }
   if (nm) HIPCHK(hipMemcpy(B->dInsts, insts.data(), nm * sizeof(DevInst), hipMemcpyHostToDevice));
   if (all.size()) HIPCHK(hipMemcpy(B->dItems, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice));
-  if (nm) HIPCHK(hipMemcpy(B->dIds, ids.data(), nm * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (ids.size()) HIPCHK(hipMemcpy(B->dIds, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  B->n_live = (int)ids.size();
   if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
   if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
   B->graph_steps = 0;
@@ -1624,7 +1629,7 @@ int rebuild(sfx_batch* B) {
 template <typename R>
 void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   hipStream_t st = B->stream;
-  const int nm = (int)B->members.size();
+  const int nm = B->n_live;
   if ((what & 1) && nm) hipLaunchKernelGGL(k_forcing<R>, dim3(nm), dim3(FBLOCK), 0, st, B->dInsts, B->dIds, forcing_mode);
   if (what & 2) {
     const Item* it = B->dItems;
@@ -1839,6 +1844,7 @@ extern "C" int sfx_batch_add(sfx_batch* B, sfx_handle* H, double t_end) {
   if ((rc = write_scalars(H, s))) return rc;
   B->members.push_back(H);
   B->tend.push_back(t_end);
+  B->live.push_back(1);
   B->dirty = true;
   H->in_batches++;
   return SFX_OK;
@@ -1852,6 +1858,7 @@ extern "C" int sfx_batch_remove(sfx_batch* B, sfx_handle* H) {
     if (B->members[i] == H) {
       B->members.erase(B->members.begin() + i);
       B->tend.erase(B->tend.begin() + i);
+      if (i < B->live.size()) B->live.erase(B->live.begin() + i);
       H->in_batches--;
       B->dirty = true;
       return SFX_OK;
@@ -1865,17 +1872,21 @@ extern "C" int sfx_batch_advance(sfx_batch* B, int32_t nsteps, int32_t* n_active
   HIPCHK(hipSetDevice(B->device));
   int rc;
   if (B->members.empty()) { if (n_active) *n_active = 0; return SFX_OK; }
-  const int K = std::max(1, std::min(nsteps, 64));
-  if ((rc = ensure_graph(B, K))) return rc;
-  for (int r = 0; r < nsteps / K; ++r) HIPCHK(hipGraphLaunch(B->exec, B->stream));
-  for (int r = 0; r < nsteps % K; ++r) launch(B, 7, 0);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(B->stream));
+  if (B->dirty && (rc = rebuild(B))) return rc;
+  if (B->n_live > 0) {
+    const int K = std::max(1, std::min(nsteps, 64));
+    if ((rc = ensure_graph(B, K))) return rc;
+    for (int r = 0; r < nsteps / K; ++r) HIPCHK(hipGraphLaunch(B->exec, B->stream));
+    for (int r = 0; r < nsteps % K; ++r) launch(B, 7, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(B->stream));
+  }
   int act = 0;
-  for (auto* H : B->members) {
+  for (size_t i = 0; i < B->members.size(); ++i) {
     Scalars s;
-    HIPCHK(hipMemcpy(&s, H->dS, sizeof(Scalars), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&s, B->members[i]->dS, sizeof(Scalars), hipMemcpyDeviceToHost));
     if (!s.error && !s.error_pending && s.t < s.tend) ++act;
+    else if (B->live[i]) { B->live[i] = 0; B->dirty = true; }   // leaves the work lists
   }
   if (n_active) *n_active = act;
   return SFX_OK;

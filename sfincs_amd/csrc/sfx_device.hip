@@ -346,8 +346,9 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   }
   __syncthreads();
   // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
-  const auto q = G((R*)I.q[S->cur]);
-  const auto uv = G((R*)I.uv[S->cur]);
+  const int curb = __builtin_amdgcn_readfirstlane(S->cur);
+  const auto q = G((R*)I.q[curb]);
+  const auto uv = G((R*)I.uv[curb]);
   const auto uvmean = G((R*)I.uvmean);
   const auto zsmax = G((R*)I.zsmax);
   const int nl = c.nl;
@@ -433,7 +434,7 @@ __device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::Uv
     tb.gn2uv = G(I.gn2uv)[slot];
   }
   const sfx::UvOut<R> o = sfx::uv_update<R, SUBGRID>(in, tb, I.c, zsu,
-                                                     SUBGRID ? G(I.uv_lev) + (size_t)slot * I.c.nl : G(I.uv_lev),
+                                                     sfx::GRow<const float4>{G(I.uv_lev), SUBGRID ? (uint32_t)slot * I.c.nl : 0u},
                                                      G(I.x73), sfx::pow_tables_g(I.pw), dt);
   q1[slot] = o.q; uv1[slot] = o.uv; G(I.kfuv)[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
@@ -487,7 +488,7 @@ __device__ __forceinline__ void gather_uv(const DevInst& I, QP q0, QP u0, int s,
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
-  const int cur = S->cur;
+  const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
   const auto q0 = G((const R*)I.q[cur]);
   const auto u0 = G((const R*)I.uv[cur]);
@@ -558,7 +559,7 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
-  const int cur = S->cur;
+  const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
   const auto q0 = G((const R*)I.q[cur]);
   const auto u0 = G((const R*)I.uv[cur]);
@@ -685,7 +686,7 @@ __device__ __forceinline__ void strip_nbrs(Row<R>& w) {
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
-  const int cur = S->cur;
+  const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
   const auto q0 = G((const R*)I.q[cur]);
   const auto u0 = G((const R*)I.uv[cur]);
@@ -783,7 +784,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
-  const int cur = S->cur;
+  const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
   const auto q0 = G((const R*)I.q[cur]);
   const auto u0 = G((const R*)I.uv[cur]);
@@ -956,7 +957,8 @@ __device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_m
     R zs0 = zs0p[c], zsd = zsdp[c];
     const int nl = I.c.nl;
     sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, G(I.z_zmin)[c],
-                         G(I.z_zmax)[c], G(I.z_volmax)[c], G(I.z_dep) + (size_t)c * nl, I.c, dt);
+                         G(I.z_zmax)[c], G(I.z_volmax)[c], sfx::GRow<const float>{G(I.z_dep), (uint32_t)c * nl},
+                         I.c, dt);
     G(I.vol)[c] = vol;
     if (kcs == 1 || kcs == 4) {
       zs0p[c] = zs0;
@@ -984,7 +986,7 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   const DevInst& I = insts[it.inst];
   const auto S = G(I.sc);
   if (!S->active) return;
-  const int qb = S->mom_done ? (S->cur ^ 1) : S->cur;
+  const int qb = __builtin_amdgcn_readfirstlane(S->mom_done ? (S->cur ^ 1) : S->cur);
   const auto q = G((const R*)I.q[qb]);
   const R dt = (R)S->dt;
   const R netprcp = (R)S->netprcp;
@@ -1052,7 +1054,7 @@ struct sfx_handle {
   struct sfx_batch* solo = nullptr;
   // tuned shapes: continuity tile, momentum per-cell tile, momentum strip rows (0: per-cell)
   int tile_tn = CELL_SHAPES[0][0], tile_tm = CELL_SHAPES[0][1];
-  int mtile_tn = CELL_SHAPES[0][0], mtile_tm = CELL_SHAPES[0][1], strip_rows = 0;
+  int mtile_tn = 32, mtile_tm = 8, strip_rows = -1;   // per-point kernel on 32 x 8 tiles
   size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
   int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;

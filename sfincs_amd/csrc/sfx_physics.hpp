@@ -39,6 +39,15 @@ struct GA {
 };
 template <typename T> __device__ __forceinline__ GA<T> ga(T* p) { return GA<T>{(gptr<T>)p}; }
 
+// A row of a level table (z_dep, uv_lev): element `row + i` of the array, so that the
+// access keeps the scalar base and a 32-bit offset.
+template <typename T>
+struct GRow {
+  GA<T> base;
+  uint32_t row;
+  __device__ __forceinline__ T operator[](uint32_t i) const { return base[row + i]; }
+};
+
 // 16-byte loads of float4 tables.
 typedef float f4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ldf4(const float4* p, size_t i) { return p[i]; }
@@ -47,6 +56,8 @@ __device__ __forceinline__ float4 ldf4(GA<T> p, uint32_t i) {
   const f4v v = *(gptr<const f4v>)(gptr<const void>)(p + i).p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
+template <typename T>
+__device__ __forceinline__ float4 ldf4(GRow<T> r, uint32_t i) { return ldf4(r.base, r.row + i); }
 
 // Physics constants of one scenario (sfincs_input.f90 defaults + derived values).
 struct Consts {

@@ -190,14 +190,15 @@ __device__ void fold_sweep(const DevInst& I) {
   float v = I.c.dtmax;
   const auto p = G(I.blkmin);
   const int n = I.nmt;
-  // the first 32 entries of each thread in one round trip (clamped indices repeat the
-  // last entry, which leaves the minimum unchanged), the rest in a loop
-  float w[32];
+  // chunks of 32 entries per thread, each chunk in one round trip (clamped indices
+  // repeat the last entry, which leaves the minimum unchanged)
+  for (int base = 0; base < n; base += 32 * FBLOCK) {
+    float w[32];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) w[j] = p[min((int)threadIdx.x + j * FBLOCK, n - 1)];
+    for (int j = 0; j < 32; ++j) w[j] = p[min(base + (int)threadIdx.x + j * FBLOCK, n - 1)];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) v = fminf(v, w[j]);
-  for (int i = threadIdx.x + 32 * FBLOCK; i < n; i += FBLOCK) v = fminf(v, p[i]);
+    for (int j = 0; j < 32; ++j) v = fminf(v, w[j]);
+  }
   v = block_min<FBLOCK>(v);
   if (threadIdx.x == 0) { S->cur ^= 1; S->mom_done = 0; S->min_dt = v; }
   __syncthreads();

@@ -74,6 +74,7 @@ struct DevInst {
   int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
   int32_t sk, nstrip;            // dense momentum kernel: strip march with sk rows per wave (sk > 0), per-cell (0), per-point (-1)
   int32_t mtn, mtm, mntn;        // dense momentum, per-cell kernel: tile shape and tiles along n
+  int32_t xchunk;                // momentum work-list runs per XCD (0: one contiguous band per XCD)
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
@@ -1056,6 +1057,7 @@ struct sfx_handle {
   // tuned shapes: continuity tile, momentum per-cell tile, momentum strip rows (0: per-cell)
   int tile_tn = CELL_SHAPES[0][0], tile_tm = CELL_SHAPES[0][1];
   int mtile_tn = 32, mtile_tm = 8, strip_rows = -1;   // per-point kernel on 32 x 8 tiles
+  int xchunk = 64;
   size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
   int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;
@@ -1352,6 +1354,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   I.sk = H->strip_rows;
   I.nstrip = M->dense ? (M->nmax + STRIP_COLS - 1) / STRIP_COLS : 0;
   I.mtn = H->mtile_tn; I.mtm = H->mtile_tm;
+  I.xchunk = H->xchunk;
   I.mntn = M->dense ? (M->nmax + I.mtn - 1) / I.mtn : 0;
   I.nmt = (int32_t)mom_tiles(I);
   H->nmt_cap = I.nmt;
@@ -1552,17 +1555,29 @@ extern "C" int sfx_get_time(sfx_handle* H, double* t, float* dt, int64_t* nt) {
 namespace {
 
 
-// XCD-aware ordering: the hardware hands workgroup b to XCD b % 8, so item b is the
-// (b%8)-th eighth of the natural tile order; each XCD sweeps a contiguous strip and
-// the neighbours a tile reads stay in that XCD's L2.
-void xcd_order(std::vector<Item>& v) {
+// XCD-aware ordering: the hardware hands workgroup b to XCD b % 8. With chunk == 0, item
+// b is the (b%8)-th eighth of the natural tile order: each XCD sweeps one contiguous band
+// and the neighbours a tile reads stay in its L2. With chunk > 0 the natural order is cut
+// into runs of `chunk` items dealt to the XCDs in turn (XCD x gets runs x, x+8, ...): a
+// run still shares its neighbours in one L2, and spatially clustered work (wet regions
+// in the momentum sweep) is spread over all XCDs instead of loading some of them more.
+void xcd_order(std::vector<Item>& v, int chunk) {
   const size_t n = v.size();
-  size_t padded = (n + 7) / 8 * 8;
+  const size_t group = chunk > 0 ? 8 * (size_t)chunk : 8;
+  const size_t padded = (n + group - 1) / group * group;
   std::vector<Item> nat(v);
   nat.resize(padded, Item{-1, 0});
-  const size_t per = padded / 8;
   v.assign(padded, Item{-1, 0});
-  for (size_t b = 0; b < padded; ++b) v[b] = nat[(b % 8) * per + b / 8];
+  if (chunk <= 0) {
+    const size_t per = padded / 8;
+    for (size_t b = 0; b < padded; ++b) v[b] = nat[(b % 8) * per + b / 8];
+  } else {
+    const size_t run = (size_t)chunk;
+    for (size_t b = 0; b < padded; ++b) {
+      const size_t x = b % 8, k = b / 8;
+      v[b] = nat[((k / run) * 8 + x) * run + k % run];
+    }
+  }
 }
 
 int rebuild(sfx_batch* B) {
@@ -1592,13 +1607,14 @@ int rebuild(sfx_batch* B) {
   }
   B->fp64 = fp64 < 0 ? 0 : fp64;
   std::vector<Item> all;
+  const int xchunk = nm ? B->members[0]->hI.xchunk : 0;
   for (int k = 0; k < 8; ++k) {
-    if (mom[k].size()) xcd_order(mom[k]);
+    if (mom[k].size()) xcd_order(mom[k], xchunk);
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
   }
-  if (cu.size()) xcd_order(cu);
-  if (cd.size()) xcd_order(cd);
-  if (ci.size()) xcd_order(ci);
+  if (cu.size()) xcd_order(cu, 0);
+  if (cd.size()) xcd_order(cd, 0);
+  if (ci.size()) xcd_order(ci, 0);
   B->off_cuv = all.size(); all.insert(all.end(), cu.begin(), cu.end()); B->n_cuv = cu.size();
   B->off_cd = all.size(); all.insert(all.end(), cd.begin(), cd.end()); B->n_cd = cd.size();
   B->off_ci = all.size(); all.insert(all.end(), ci.begin(), ci.end()); B->n_ci = ci.size();
@@ -1906,10 +1922,10 @@ extern "C" int sfx_batch_done(sfx_batch* B, sfx_handle* H, int32_t* done) {
 }
 
 // -------------------------------------------------------------------------- autotuner
-extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[6]) {
+extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[7]) {
   if (!H || !out) { set_error("null"); return SFX_ERR_ARG; }
   const DevInst& I = H->hI;
-  out[0] = I.dense; out[1] = I.sk; out[2] = I.mtn; out[3] = I.mtm; out[4] = I.tn; out[5] = I.tm;
+  out[0] = I.dense; out[1] = I.sk; out[2] = I.mtn; out[3] = I.mtm; out[4] = I.tn; out[5] = I.tm; out[6] = I.xchunk;
   return SFX_OK;
 }
 
@@ -2013,6 +2029,18 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
     rc = time_phases(H, 2, trials, e0, e1, ms);
     if (!rc && ms < best) { best = ms; bsk = sk; bmtn = mtn; bmtm = mtm; }
   }
+  // momentum work-list order
+  set_shapes(H, bsk, bmtn, bmtm, tn0, tm0);
+  const int chunks[] = {64, 0, 16, 256};
+  best = 1e30f;
+  int bx = 64;
+  for (int k = 0; k < 4 && !rc; ++k) {
+    H->hI.xchunk = chunks[k];
+    float ms = 0.f;
+    rc = time_phases(H, 2, trials, e0, e1, ms);
+    if (!rc && ms < best) { best = ms; bx = chunks[k]; }
+  }
+  H->hI.xchunk = bx;
   // continuity tiles
   best = 1e30f;
   int bc = 0;
@@ -2028,7 +2056,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   set_shapes(H, bsk, bmtn, bmtm, CELL_SHAPES[bc][0], CELL_SHAPES[bc][1]);
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
   H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm; H->strip_rows = H->hI.sk;
-  H->mtile_tn = H->hI.mtn; H->mtile_tm = H->hI.mtm;
+  H->mtile_tn = H->hI.mtn; H->mtile_tm = H->hI.mtm; H->xchunk = H->hI.xchunk;
   if ((rc = write_scalars(H, s0))) return rc;
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;

@@ -119,6 +119,8 @@ def quadtree(args):
     print(f"# generated quadtree: np {sizes['np']} npuv {sizes['npuv']} ncuv {sizes['ncuv']} in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     inst = _lib.Instance(model)
+    tuning = inst.autotune(3)
+    print(f"# tuning {tuning}", file=sys.stderr, flush=True)
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     batch.advance(args.warmup)
@@ -151,7 +153,7 @@ def quadtree(args):
         "data": "synthetic (seeded fBm DEM, PCG64(1002))",
         "config": {"workload": "config 2: quadtree 4 levels 100..12.5 m, surge 3 m, wind 20 m/s SW",
                    "np": sizes["np"], "npuv": sizes["npuv"], "ncuv": sizes["ncuv"], "model_time_s": t_model,
-                   "dt_s": dt_model},
+                   "dt_s": dt_model, "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": mom_gbs / PEAK_HBM_GBS, "traffic": traffic_of("k_mom_idx<float, false>"),
                      "kernel": "k_mom_idx<float, regular> + k_cuv",

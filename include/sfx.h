@@ -207,15 +207,16 @@ int  sfx_profile_steps(sfx_handle* h, int32_t nsteps, double ms[3]);
 int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle** out);
 
 /* Autotuner: on this handle's mesh, times the dense momentum kernels (one thread per cell
- * or per uv-point on tiles of several shapes, strip marches of several lengths), the order
- * in which momentum tiles are dealt to the XCDs, and the tile shapes of the dense
- * continuity kernel, and keeps the fastest. Call it before the handle joins a batch
- * (SFX_ERR_STATE otherwise). */
+ * or per uv-point on tiles of several shapes, strip marches of several lengths), the tile
+ * shapes of the dense continuity kernel, and for both sweeps (dense or indexed) the runs
+ * in which tiles are dealt to the XCDs, and keeps the fastest. Call it before the handle
+ * joins a batch (SFX_ERR_STATE otherwise). */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
 /* out = {dense, momentum kernel (-1: per uv-point, 0: per cell, k > 0: strip march of k
  *        rows per wave), momentum tile n, momentum tile m, continuity tile n, continuity
- *        tile m, momentum work-list run per XCD (0: one contiguous band per XCD)} */
-int  sfx_get_tuning(sfx_handle* h, int32_t out[7]);
+ *        tile m, momentum and continuity work-list runs per XCD (0: one contiguous band
+ *        per XCD)} */
+int  sfx_get_tuning(sfx_handle* h, int32_t out[8]);
 /* Pins the dense shapes without timing: in = {momentum kernel as above, momentum tile n,
  * momentum tile m, continuity tile n, continuity tile m}; tiles hold 256 cells. No effect
  * on indexed meshes. Call it before the handle joins a batch. */

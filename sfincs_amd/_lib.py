@@ -349,10 +349,10 @@ class Instance:
 
     def autotune(self, trials=3):
         check(lib().sfx_autotune(self.h, trials), "autotune")
-        a = (C.c_int32 * 7)()
+        a = (C.c_int32 * 8)()
         check(lib().sfx_get_tuning(self.h, a), "get_tuning")
         return dict(dense=a[0], momentum_kernel=a[1], momentum_tile=[a[2], a[3]], continuity_tile=[a[4], a[5]],
-                    xcd_run=a[6])
+                    xcd_run=[a[6], a[7]])
 
     def set_tuning(self, momentum_kernel=0, momentum_tile=(64, 4), continuity_tile=(64, 4)):
         """momentum_kernel: -1 one thread per uv-point, 0 one thread per cell, k > 0 strip

@@ -74,7 +74,7 @@ struct DevInst {
   int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
   int32_t sk, nstrip;            // dense momentum kernel: strip march with sk rows per wave (sk > 0), per-cell (0), per-point (-1)
   int32_t mtn, mtm, mntn;        // dense momentum, per-cell kernel: tile shape and tiles along n
-  int32_t xchunk;                // momentum work-list runs per XCD (0: one contiguous band per XCD)
+  int32_t xchunk, xchunk_c;      // momentum / continuity work-list runs per XCD (0: one contiguous band per XCD)
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
@@ -1057,7 +1057,7 @@ struct sfx_handle {
   // tuned shapes: continuity tile, momentum per-cell tile, momentum strip rows (0: per-cell)
   int tile_tn = CELL_SHAPES[0][0], tile_tm = CELL_SHAPES[0][1];
   int mtile_tn = 32, mtile_tm = 8, strip_rows = -1;   // per-point kernel on 32 x 8 tiles
-  int xchunk = 64;
+  int xchunk = 64, xchunk_c = 0;
   size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
   int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;
@@ -1354,7 +1354,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   I.sk = H->strip_rows;
   I.nstrip = M->dense ? (M->nmax + STRIP_COLS - 1) / STRIP_COLS : 0;
   I.mtn = H->mtile_tn; I.mtm = H->mtile_tm;
-  I.xchunk = H->xchunk;
+  I.xchunk = H->xchunk; I.xchunk_c = H->xchunk_c;
   I.mntn = M->dense ? (M->nmax + I.mtn - 1) / I.mtn : 0;
   I.nmt = (int32_t)mom_tiles(I);
   H->nmt_cap = I.nmt;
@@ -1612,9 +1612,10 @@ int rebuild(sfx_batch* B) {
     if (mom[k].size()) xcd_order(mom[k], xchunk);
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
   }
+  const int xchunk_c = nm ? B->members[0]->hI.xchunk_c : 0;
   if (cu.size()) xcd_order(cu, 0);
-  if (cd.size()) xcd_order(cd, 0);
-  if (ci.size()) xcd_order(ci, 0);
+  if (cd.size()) xcd_order(cd, xchunk_c);
+  if (ci.size()) xcd_order(ci, xchunk_c);
   B->off_cuv = all.size(); all.insert(all.end(), cu.begin(), cu.end()); B->n_cuv = cu.size();
   B->off_cd = all.size(); all.insert(all.end(), cd.begin(), cd.end()); B->n_cd = cd.size();
   B->off_ci = all.size(); all.insert(all.end(), ci.begin(), ci.end()); B->n_ci = ci.size();
@@ -1922,10 +1923,11 @@ extern "C" int sfx_batch_done(sfx_batch* B, sfx_handle* H, int32_t* done) {
 }
 
 // -------------------------------------------------------------------------- autotuner
-extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[7]) {
+extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[8]) {
   if (!H || !out) { set_error("null"); return SFX_ERR_ARG; }
   const DevInst& I = H->hI;
   out[0] = I.dense; out[1] = I.sk; out[2] = I.mtn; out[3] = I.mtm; out[4] = I.tn; out[5] = I.tm; out[6] = I.xchunk;
+  out[7] = I.xchunk_c;
   return SFX_OK;
 }
 
@@ -1990,8 +1992,8 @@ extern "C" int sfx_set_tuning(sfx_handle* H, const int32_t in[5]) {
 extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
-  if (!H->hI.dense) return SFX_OK;   // indexed sweeps have one shape (a uv-point per lane)
   if (H->in_batches) { set_error("sfx_autotune: tune a scenario before adding it to a batch"); return SFX_ERR_STATE; }
+  const bool dense = H->hI.dense;   // indexed meshes have one kernel shape; only the work order is tuned
   trials = std::max(1, (int)trials);
   Scalars s0;
   int rc = normalise(H, s0);
@@ -2016,10 +2018,10 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int tn0 = H->hI.tn, tm0 = H->hI.tm;
-  // momentum: per-cell tiles (sk = 0) and strip marches
+  // momentum: per-cell and per-point tiles, strip marches
   float best = 1e30f;
-  int bsk = 0, bmtn = CELL_SHAPES[0][0], bmtm = CELL_SHAPES[0][1];
-  for (int k = 0; k < 2 * N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
+  int bsk = H->hI.sk, bmtn = H->hI.mtn, bmtm = H->hI.mtm;
+  for (int k = 0; dense && k < 2 * N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
     const bool tiled = k < 2 * N_CELL_SHAPES;
     const int sk = tiled ? (k < N_CELL_SHAPES ? 0 : -1) : STRIP_ROWS[k - 2 * N_CELL_SHAPES];
     const int mtn = tiled ? CELL_SHAPES[k % N_CELL_SHAPES][0] : bmtn;
@@ -2029,34 +2031,43 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
     rc = time_phases(H, 2, trials, e0, e1, ms);
     if (!rc && ms < best) { best = ms; bsk = sk; bmtn = mtn; bmtm = mtm; }
   }
-  // momentum work-list order
+  // momentum work-list order (runs of tiles per XCD)
   set_shapes(H, bsk, bmtn, bmtm, tn0, tm0);
-  const int chunks[] = {64, 0, 16, 256};
+  const int chunks[] = {64, 0, 16, 256, 1024};
   best = 1e30f;
-  int bx = 64;
-  for (int k = 0; k < 4 && !rc; ++k) {
+  int bx = H->hI.xchunk;
+  for (int k = 0; k < 5 && !rc; ++k) {
     H->hI.xchunk = chunks[k];
     float ms = 0.f;
     rc = time_phases(H, 2, trials, e0, e1, ms);
     if (!rc && ms < best) { best = ms; bx = chunks[k]; }
   }
   H->hI.xchunk = bx;
-  // continuity tiles
+  // continuity tiles, then their work-list order
   best = 1e30f;
-  int bc = 0;
-  for (int k = 0; k < N_CELL_SHAPES && !rc; ++k) {
+  int btn = tn0, btm = tm0;
+  for (int k = 0; dense && k < N_CELL_SHAPES && !rc; ++k) {
     set_shapes(H, bsk, bmtn, bmtm, CELL_SHAPES[k][0], CELL_SHAPES[k][1]);
     float ms = 0.f;
     rc = time_phases(H, 4, trials, e0, e1, ms);
-    if (!rc && ms < best) { best = ms; bc = k; }
+    if (!rc && ms < best) { best = ms; btn = CELL_SHAPES[k][0]; btm = CELL_SHAPES[k][1]; }
   }
+  set_shapes(H, bsk, bmtn, bmtm, btn, btm);
+  best = 1e30f;
+  int bxc = H->hI.xchunk_c;
+  for (int k = 0; k < 5 && !rc; ++k) {
+    H->hI.xchunk_c = chunks[k];
+    float ms = 0.f;
+    rc = time_phases(H, 4, trials, e0, e1, ms);
+    if (!rc && ms < best) { best = ms; bxc = chunks[k]; }
+  }
+  H->hI.xchunk_c = bxc;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc) return rc;
-  set_shapes(H, bsk, bmtn, bmtm, CELL_SHAPES[bc][0], CELL_SHAPES[bc][1]);
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
   H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm; H->strip_rows = H->hI.sk;
-  H->mtile_tn = H->hI.mtn; H->mtile_tm = H->hI.mtm; H->xchunk = H->hI.xchunk;
+  H->mtile_tn = H->hI.mtn; H->mtile_tm = H->hI.mtm; H->xchunk = H->hI.xchunk; H->xchunk_c = H->hI.xchunk_c;
   if ((rc = write_scalars(H, s0))) return rc;
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;

@@ -25,7 +25,7 @@ namespace sfx {
 namespace {
 
 constexpr uint32_t MAGIC = 0x5346504Du;   // "SFPM"
-constexpr uint32_t VERSION = 4;
+constexpr uint32_t VERSION = 5;
 
 struct Entry { uint32_t x, g; };
 
@@ -59,7 +59,7 @@ bool enumerate(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std
     const double mid = 0.5 * ((double)fd + (double)nb);
     const double dist = std::fabs(r - mid);
     if (g != f) {
-      if (!(dist < 4.656612873077393e-10 * r)) {            // 2^-31: well inside the device band
+      if (!(dist < 1.6462644155254631e-10 * r)) {           // 2^-32.5: inside the device band of 2^-32
         err = "powf differs outside the midpoint band at x=" + std::to_string(x);
         return false;
       }

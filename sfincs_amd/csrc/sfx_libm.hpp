@@ -5,8 +5,8 @@
 // updates branch on thresholds (wet/dry tests, the table inversion switch at
 // V = 0.999*volmax), so a one-ulp difference in powf can move a cell to the other branch.
 // These routines return powf's exact result: a double-precision estimate (pow_estimate)
-// is rounded to float unless it lies within 2^-30 (relative) of a rounding midpoint;
-// those inputs (about 2% of calls) are looked up in a table of powf's own results that
+// is rounded to float unless it lies within 2^-32 (relative) of a rounding midpoint;
+// those inputs (about 0.5% of calls) are looked up in a table of powf's own results that
 // sfx_libm.hip enumerates over every float in [2^-40, 2^14) and verifies. Outside that
 // range the rounded estimate is returned.
 #pragma once
@@ -83,7 +83,7 @@ SFX_HD inline double pow_estimate(float x, int which) {
   return which == 0 ? y * corr : (double)x * y * y * corr;
 }
 
-// True when the double value r lies within 2^-30 (relative) of a midpoint between two
+// True when the double value r lies within 2^-32 (relative) of a midpoint between two
 // adjacent floats, i.e. when its float rounding is not certain.
 SFX_HD inline bool near_midpoint(double r) {
   const float f = (float)r;
@@ -91,7 +91,7 @@ SFX_HD inline bool near_midpoint(double r) {
   const float nb = r >= fd ? nextafterf(f, INFINITY) : nextafterf(f, 0.0f);
   const double mid = 0.5 * (fd + (double)nb);
   const double d = r > mid ? r - mid : mid - r;
-  return d < 9.313225746154785e-10 * r;              // 2^-30
+  return d < 2.3283064365386963e-10 * r;             // 2^-32
 }
 
 // Table entries read as 64-bit words: key in the low half, powf bits in the high half

@@ -82,7 +82,7 @@ struct DevInst {
   // blkmin[tile] (no atomics); the fold at the start of the next step reduces them.
   float* blkmin;
   uint32_t* blkcnt;              // per tile: uv-points copied (kcuv not 1/6), dry, wet/full, partial
-  uint8_t* wetflag;              // per wave of a momentum tile: it held a wet point in the last sweep
+  uint8_t* wetflag;              // per momentum tile: it held a wet point in the last sweep
   // cells
   double* zs;
   double* vol;
@@ -558,8 +558,8 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 // other on one XCD and share its cached cell data. A point first reads
 // what decides its branch (kcuv, the two water levels, its wet/dry threshold); only wet
 // points read the rest of their stencil, so waves without wet points finish after one
-// round trip to memory. Waves that held a wet point in the last sweep (wetflag) read the
-// stencil together with the branch inputs, saving wet waves a round trip.
+// round trip to memory. In tiles that held a wet point in the last sweep (wetflag) the
+// stencil is read together with the branch inputs, saving wet waves a round trip.
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
@@ -581,7 +581,7 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
   if (vplane ? !hv : !hu) { cnt[0]++; return; }
   const int s = vplane ? np + c : c;
   const int c_nmu = vplane ? c + 1 : c + nmax;
-  const bool spec = __builtin_amdgcn_readfirstlane(G(I.wetflag)[tile * (BLOCK / WAVE) + threadIdx.x / WAVE]) != 0;
+  const bool spec = G(I.wetflag)[tile] != 0;
   // branch inputs
   const int kc = G(I.kcuv)[s];
   sfx::UvIn<R> in;
@@ -866,11 +866,10 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
       if (threadIdx.x % WAVE == 0 && v) atomicAdd(&bc[k], (unsigned int)v);
     }
   }
-  const bool wave_wet = __any(cnt[2] + cnt[3] > 0);
-  if (threadIdx.x % WAVE == 0) G(I.wetflag)[tile * (BLOCK / WAVE) + threadIdx.x / WAVE] = wave_wet ? 1 : 0;
-  __syncthreads();
+  const int wet = __syncthreads_or(cnt[2] + cnt[3] > 0);
   if (threadIdx.x == 0) {
     G(I.blkmin)[tile] = m;
+    G(I.wetflag)[tile] = wet ? 1 : 0;
     if (tile == 0) G(I.sc)->mom_done = 1;
     if (I.count) {
       const auto c = G(I.blkcnt) + 4 * (size_t)tile;
@@ -1373,7 +1372,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
       H->nmt_cap = std::max(H->nmt_cap, 2 * cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
   }
   I.blkmin = H->alloc<float>(H->nmt_cap);
-  I.wetflag = H->alloc<uint8_t>(H->nmt_cap * (BLOCK / WAVE));
+  I.wetflag = H->alloc<uint8_t>(H->nmt_cap);
   I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
   H->dI = H->upload(&I, 1);

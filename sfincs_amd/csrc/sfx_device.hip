@@ -82,7 +82,6 @@ struct DevInst {
   // blkmin[tile] (no atomics); the fold at the start of the next step reduces them.
   float* blkmin;
   uint32_t* blkcnt;              // per tile: uv-points copied (kcuv not 1/6), dry, wet/full, partial
-  uint8_t* wetflag;              // per momentum tile: it held a wet point in the last sweep
   // cells
   double* zs;
   double* vol;
@@ -558,8 +557,7 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 // other on one XCD and share its cached cell data. A point first reads
 // what decides its branch (kcuv, the two water levels, its wet/dry threshold); only wet
 // points read the rest of their stencil, so waves without wet points finish after one
-// round trip to memory. In tiles that held a wet point in the last sweep (wetflag) the
-// stencil is read together with the branch inputs, saving wet waves a round trip.
+// round trip to memory.
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, float& mdt) {
   const auto S = G(I.sc);
@@ -581,7 +579,6 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
   if (vplane ? !hv : !hu) { cnt[0]++; return; }
   const int s = vplane ? np + c : c;
   const int c_nmu = vplane ? c + 1 : c + nmax;
-  const bool spec = G(I.wetflag)[tile] != 0;
   // branch inputs
   const int kc = G(I.kcuv)[s];
   sfx::UvIn<R> in;
@@ -589,7 +586,18 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
   sfx::UvTables tb;
   if (SUBGRID) tb.zmin = G(I.uv_zmin)[s];
   else tb.zbuvmx = G(I.zbuvmx)[s];
-  // the rest of the stencil
+  if (!(kc == 1 || kc == 6)) {
+    q1[s] = q0[s]; u1[s] = u0[s];
+    cnt[0]++;
+    return;
+  }
+  R zsu;
+  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
+    q1[s] = R(0); u1[s] = R(0); G(I.kfuv)[s] = 0;
+    cnt[1]++;
+    return;
+  }
+  // the rest of the stencil of a wet point
   int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
   if (!vplane) {
     s_nmd = m >= 1 ? c - nmax : dm; s_nmu = m + 1 < mmax - 1 ? c + nmax : dm;
@@ -602,34 +610,19 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
     v_nm = hu ? c : dm; v_ndm = m >= 1 ? c - nmax : dm;
     v_nmu = hu ? c + 1 : dm; v_ndmu = m >= 1 ? c + 1 - nmax : dm;
   }
-  auto stencil = [&](sfx::UvIn<R>& x) {
-    x.qx_nm = q0[s]; x.uu_nm = u0[s];
-    x.qx_nmd = q0[s_nmd]; x.qx_nmu = q0[s_nmu];
-    x.qy_nm = q0[v_nm]; x.qy_nmu = q0[v_nmu]; x.qy_ndm = q0[v_ndm]; x.qy_ndmu = q0[v_ndmu];
-    x.uu_nmd = u0[s_nmd]; x.uu_nmu = u0[s_nmu]; x.uu_ndm = u0[s_ndm]; x.uu_num = u0[s_num];
-    x.v_ndm = u0[v_ndm]; x.v_ndmu = u0[v_ndmu]; x.v_nm = u0[v_nm]; x.v_nmu = u0[v_nmu];
-    x.kcuv_nmd = G(I.kcuv)[s_nmd]; x.kcuv_nmu = G(I.kcuv)[s_nmu];
-    x.kfuv = G(I.kfuv)[s]; x.mask_adv = G(I.mask_adv)[s];
-    if (SUBGRID) {
-      x.vol_nm = G(I.vol)[c]; x.vol_nmu = G(I.vol)[c_nmu];
-      x.zsderv_nm = G((const R*)I.zsderv)[c]; x.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
-    } else {
-      x.zb_nm = G(I.zb)[c]; x.zb_nmu = G(I.zb)[c_nmu];
-    }
-  };
-  if (spec) stencil(in);
-  if (!(kc == 1 || kc == 6)) {
-    q1[s] = spec ? in.qx_nm : q0[s]; u1[s] = spec ? in.uu_nm : u0[s];
-    cnt[0]++;
-    return;
+  in.qx_nm = q0[s]; in.uu_nm = u0[s];
+  in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
+  in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
+  in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
+  in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
+  in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
+  in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s];
+  if (SUBGRID) {
+    in.vol_nm = G(I.vol)[c]; in.vol_nmu = G(I.vol)[c_nmu];
+    in.zsderv_nm = G((const R*)I.zsderv)[c]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
+  } else {
+    in.zb_nm = G(I.zb)[c]; in.zb_nmu = G(I.zb)[c_nmu];
   }
-  R zsu;
-  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
-    q1[s] = R(0); u1[s] = R(0); G(I.kfuv)[s] = 0;
-    cnt[1]++;
-    return;
-  }
-  if (!spec) stencil(in);
   in.idir = vplane; in.iref = 1; in.itype = 0; in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
   int br = 0;
   wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br);
@@ -866,10 +859,9 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
       if (threadIdx.x % WAVE == 0 && v) atomicAdd(&bc[k], (unsigned int)v);
     }
   }
-  const int wet = __syncthreads_or(cnt[2] + cnt[3] > 0);
+  __syncthreads();
   if (threadIdx.x == 0) {
     G(I.blkmin)[tile] = m;
-    G(I.wetflag)[tile] = wet ? 1 : 0;
     if (tile == 0) G(I.sc)->mom_done = 1;
     if (I.count) {
       const auto c = G(I.blkcnt) + 4 * (size_t)tile;
@@ -1372,7 +1364,6 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
       H->nmt_cap = std::max(H->nmt_cap, 2 * cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
   }
   I.blkmin = H->alloc<float>(H->nmt_cap);
-  I.wetflag = H->alloc<uint8_t>(H->nmt_cap);
   I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
   H->dI = H->upload(&I, 1);

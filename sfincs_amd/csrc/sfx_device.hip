@@ -78,8 +78,10 @@ struct DevInst {
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
-  // CFL reduction: each momentum tile stores the minimum of its points' time steps in
-  // blkmin[tile] (no atomics); the fold at the start of the next step reduces them.
+  // CFL reduction: each wave of a momentum tile lowers blkmin[tile] to the minimum of its
+  // points' time steps (an atomic min on the tile's own word, so no two tiles share an
+  // address and no wave waits for the others of its block); the fold at the start of the
+  // next step reduces the tiles and sets every word back to dtmax.
   float* blkmin;
   uint32_t* blkcnt;              // per tile: uv-points copied (kcuv not 1/6), dry, wet/full, partial
   // cells
@@ -199,6 +201,11 @@ __device__ void fold_sweep(const DevInst& I) {
     for (int j = 0; j < 32; ++j) w[j] = p[min(base + (int)threadIdx.x + j * FBLOCK, n - 1)];
 #pragma unroll
     for (int j = 0; j < 32; ++j) v = fminf(v, w[j]);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const int i = base + (int)threadIdx.x + j * FBLOCK;
+      if (i < n) p[i] = I.c.dtmax;   // ready for the next sweep
+    }
   }
   v = block_min<FBLOCK>(v);
   if (threadIdx.x == 0) { S->cur ^= 1; S->mom_done = 0; S->min_dt = v; }
@@ -845,28 +852,26 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
   cnt[br]++;
 }
 
-// Block epilogue of a momentum tile: the tile's minimum time step into blkmin[tile] (NaN
-// or non-positive values do not take part; the reduction starts at dtmax), the
-// per-branch counters into the tile's own blkcnt slots, and the sweep-done flag.
+// Wave epilogue of a momentum tile, without a block barrier: the wave's minimum time step
+// lowers blkmin[tile] (NaN or non-positive values do not take part; the reduction starts
+// at dtmax), the wave's per-branch counts are added to the tile's blkcnt slots, and the
+// sweep-done flag is raised. Positive floats order like their bit patterns, so the
+// minimum is an unsigned atomic min.
 __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float mdt, const int* cnt) {
-  __shared__ unsigned int bc[4];
-  if (threadIdx.x < 4) bc[threadIdx.x] = 0;
-  const float m = block_min<BLOCK>(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);   // contains a barrier
-  if (I.count) {   // wave sums, one LDS add per wave and branch
+  const float m = wave_min(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);
+  int c4[4] = {0, 0, 0, 0};
+  if (I.count)
     for (int k = 0; k < 4; ++k) {
       int v = cnt[k];
       for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-      if (threadIdx.x % WAVE == 0 && v) atomicAdd(&bc[k], (unsigned int)v);
+      c4[k] = v;
     }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    G(I.blkmin)[tile] = m;
-    if (tile == 0) G(I.sc)->mom_done = 1;
-    if (I.count) {
-      const auto c = G(I.blkcnt) + 4 * (size_t)tile;
-      for (int k = 0; k < 4; ++k) c[k] += bc[k];
-    }
+  if (threadIdx.x % WAVE == 0) {
+    if (m < I.c.dtmax) atomicMin((unsigned int*)&G(I.blkmin)[tile], __float_as_uint(m));
+    if (tile == 0 && threadIdx.x == 0) G(I.sc)->mom_done = 1;
+    if (I.count)
+      for (int k = 0; k < 4; ++k)
+        if (c4[k]) atomicAdd((unsigned int*)&G(I.blkcnt)[4 * (uint32_t)tile + k], (unsigned int)c4[k]);
   }
 }
 
@@ -1363,7 +1368,10 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     for (int k = 0; k < N_CELL_SHAPES; ++k)
       H->nmt_cap = std::max(H->nmt_cap, 2 * cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
   }
-  I.blkmin = H->alloc<float>(H->nmt_cap);
+  {
+    std::vector<float> init(H->nmt_cap, I.c.dtmax);
+    I.blkmin = H->upload(init.data(), init.size());
+  }
   I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
   H->dI = H->upload(&I, 1);
@@ -1954,6 +1962,13 @@ int time_phases(sfx_handle* H, int what, int trials, hipEvent_t e0, hipEvent_t e
   return SFX_OK;
 }
 
+// Tile minima back to dtmax (after sweeps that were not folded, e.g. timing runs).
+int reset_blkmin(sfx_handle* H) {
+  std::vector<float> init(H->nmt_cap, H->hI.c.dtmax);
+  HIPCHK(hipMemcpy(H->hI.blkmin, init.data(), init.size() * sizeof(float), hipMemcpyHostToDevice));
+  return SFX_OK;
+}
+
 void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
   DevInst& I = H->hI;
   I.sk = sk; I.mtn = mtn; I.mtm = mtm; I.tn = tn; I.tm = tm;
@@ -1985,6 +2000,7 @@ extern "C" int sfx_set_tuning(sfx_handle* H, const int32_t in[5]) {
   }
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
   H->tile_tn = tn; H->tile_tm = tm; H->strip_rows = sk; H->mtile_tn = mtn; H->mtile_tm = mtm;
+  if ((rc = reset_blkmin(H))) return rc;
   if (H->solo) H->solo->dirty = true;
   return SFX_OK;
 }
@@ -2069,6 +2085,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   H->tile_tn = H->hI.tn; H->tile_tm = H->hI.tm; H->strip_rows = H->hI.sk;
   H->mtile_tn = H->hI.mtn; H->mtile_tm = H->hI.mtm; H->xchunk = H->hI.xchunk; H->xchunk_c = H->hI.xchunk_c;
   if ((rc = write_scalars(H, s0))) return rc;
+  if ((rc = reset_blkmin(H))) return rc;
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;
   if (H->solo) H->solo->dirty = true;

@@ -235,7 +235,10 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   __shared__ int sh_go;
   __shared__ int sh_itb0, sh_itb1;
   __shared__ float sh_tbfac;
+  __shared__ double sh_t;
+  __shared__ float sh_dt;
   fold_sweep(I);
+  // step scalars (sfincs_lib.f90:379-392): one thread
   if (threadIdx.x == 0) {
     int go = 1;
     if (mode == 0) {
@@ -252,78 +255,85 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     } else {
       S->active = 1;
     }
-    if (go) {
-      const double t = S->t;
-      const float dt = S->dt;
-      if (c.flags & sfx::F_WIND) {
-        for (int itw = max(S->itwndlast, 2); itw <= I.ntwnd; ++itw) {
-          if ((double)G(I.twnd)[itw - 1] > t) {
-            const float twfac = (float)((t - (double)G(I.twnd)[itw - 2]) / (double)(G(I.twnd)[itw - 1] - G(I.twnd)[itw - 2]));
-            const float vmag = G(I.wndmag)[itw - 2] * (1.0f - twfac) + G(I.wndmag)[itw - 1] * twfac;
-            float dr0 = f_modulo(G(I.wnddir)[itw - 2], 2 * c.pi);
-            float dr1 = f_modulo(G(I.wnddir)[itw - 1], 2 * c.pi);
-            if (dr1 > dr0 + c.pi) dr0 = dr0 + 2 * c.pi;
-            else if (dr0 > dr1 + c.pi) dr1 = dr1 + 2 * c.pi;
-            const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
-            const float cd = G(I.cdval)[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
-            S->tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
-            S->tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
-            S->itwndlast = itw - 1;
-            break;
-          }
-        }
-      }
-      if (c.flags & sfx::F_PRECIP) {
-        int itp;
-        for (itp = S->itprcplast; itp <= I.ntprcp; ++itp)
-          if ((double)G(I.tprcpt)[itp - 1] > t) break;
-        if (itp > I.ntprcp) itp = I.ntprcp;
-        if (itp < 2) itp = 2;   // series start at or before t0 (checked at initialisation)
-        const float twfac = (float)((t - (double)G(I.tprcpt)[itp - 2]) / (double)(G(I.tprcpt)[itp - 1] - G(I.tprcpt)[itp - 2]));
-        S->netprcp = (G(I.tprcpv)[itp - 2] * (1.0f - twfac) + G(I.tprcpv)[itp - 1] * twfac) / (float)(1000 * 3600);
-        S->itprcplast = itp - 1;
-      }
-      if (I.nbnd > 0) {
-        int itb0 = 1, itb1 = 1;
-        float tb = 0.f;
-        const int nt = I.ntbnd;
-        if ((double)G(I.t_bnd)[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = G(I.t_bnd)[0]; }
-        else if ((double)G(I.t_bnd)[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = G(I.t_bnd)[nt - 1]; }
-        else {
-          for (int itb = S->itbndlast; itb <= nt; ++itb)
-            if ((double)G(I.t_bnd)[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; S->itbndlast = itb - 1; break; }
-        }
-        sh_itb0 = itb0; sh_itb1 = itb1;
-        sh_tbfac = (tb - G(I.t_bnd)[itb0 - 1]) / sfx::vmax<float>(G(I.t_bnd)[itb1 - 1] - G(I.t_bnd)[itb0 - 1], 1.0e-6f);
-      }
-      if (I.nsrc > 0) {
-        for (int it = max(S->itsrclast, 2); it <= I.ntsrc; ++it) {
-          if ((double)G(I.tsrc)[it - 1] > t) {
-            for (int s = 0; s < I.nsrc; ++s) {
-              const float q0v = G(I.qsrc)[(it - 2) * I.nsrc + s], q1v = G(I.qsrc)[(it - 1) * I.nsrc + s];
-              G(I.qtsrc)[s] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)G(I.tsrc)[it - 2]) /
-                                                     (double)(G(I.tsrc)[it - 1] - G(I.tsrc)[it - 2]));
-            }
-            S->itsrclast = it - 1;
-            break;
-          }
-        }
-      }
-      S->factime = sfx::vmin<float>(dt / c.btfilter, 1.0f);
-      S->one_minus_factime = 1.0f - S->factime;
-      S->facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
-    }
     sh_go = go;
+    sh_t = S->t;
+    sh_dt = S->dt;
   }
   __syncthreads();
   if (!sh_go) return;
+  // the forcing series at the new time, independent of each other: one wave each
+  const double t = sh_t;
+  const float dt = sh_dt;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (wave == 0 && lane == 0) {
+    S->factime = sfx::vmin<float>(dt / c.btfilter, 1.0f);
+    S->one_minus_factime = 1.0f - S->factime;
+    S->facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
+  }
+  if (wave == 1 && lane == 0 && (c.flags & sfx::F_WIND)) {
+    for (int itw = max(S->itwndlast, 2); itw <= I.ntwnd; ++itw) {
+      if ((double)G(I.twnd)[itw - 1] > t) {
+        const float twfac = (float)((t - (double)G(I.twnd)[itw - 2]) / (double)(G(I.twnd)[itw - 1] - G(I.twnd)[itw - 2]));
+        const float vmag = G(I.wndmag)[itw - 2] * (1.0f - twfac) + G(I.wndmag)[itw - 1] * twfac;
+        float dr0 = f_modulo(G(I.wnddir)[itw - 2], 2 * c.pi);
+        float dr1 = f_modulo(G(I.wnddir)[itw - 1], 2 * c.pi);
+        if (dr1 > dr0 + c.pi) dr0 = dr0 + 2 * c.pi;
+        else if (dr0 > dr1 + c.pi) dr1 = dr1 + 2 * c.pi;
+        const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
+        const float cd = G(I.cdval)[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
+        S->tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
+        S->tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
+        S->itwndlast = itw - 1;
+        break;
+      }
+    }
+  }
+  if (wave == 2 && lane == 0 && (c.flags & sfx::F_PRECIP)) {
+    int itp;
+    for (itp = S->itprcplast; itp <= I.ntprcp; ++itp)
+      if ((double)G(I.tprcpt)[itp - 1] > t) break;
+    if (itp > I.ntprcp) itp = I.ntprcp;
+    if (itp < 2) itp = 2;   // series start at or before t0 (checked at initialisation)
+    const float twfac = (float)((t - (double)G(I.tprcpt)[itp - 2]) / (double)(G(I.tprcpt)[itp - 1] - G(I.tprcpt)[itp - 2]));
+    S->netprcp = (G(I.tprcpv)[itp - 2] * (1.0f - twfac) + G(I.tprcpv)[itp - 1] * twfac) / (float)(1000 * 3600);
+    S->itprcplast = itp - 1;
+  }
+  if (wave == 3 && lane == 0 && I.nbnd > 0) {
+    int itb0 = 1, itb1 = 1;
+    float tb = 0.f;
+    const int nt = I.ntbnd;
+    if ((double)G(I.t_bnd)[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = G(I.t_bnd)[0]; }
+    else if ((double)G(I.t_bnd)[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = G(I.t_bnd)[nt - 1]; }
+    else {
+      for (int itb = S->itbndlast; itb <= nt; ++itb)
+        if ((double)G(I.t_bnd)[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; S->itbndlast = itb - 1; break; }
+    }
+    sh_itb0 = itb0; sh_itb1 = itb1;
+    sh_tbfac = (tb - G(I.t_bnd)[itb0 - 1]) / sfx::vmax<float>(G(I.t_bnd)[itb1 - 1] - G(I.t_bnd)[itb0 - 1], 1.0e-6f);
+  }
+  if (wave == 4 && I.nsrc > 0) {   // discharges: the series position once, the sources one lane each
+    int itsel = 0;
+    if (lane == 0) {
+      for (int it = max(S->itsrclast, 2); it <= I.ntsrc; ++it)
+        if ((double)G(I.tsrc)[it - 1] > t) { itsel = it; S->itsrclast = it - 1; break; }
+    }
+    itsel = __shfl(itsel, 0, WAVE);
+    if (itsel) {
+      const int it = itsel;
+      for (int src = lane; src < I.nsrc; src += WAVE) {
+        const float q0v = G(I.qsrc)[(it - 2) * I.nsrc + src], q1v = G(I.qsrc)[(it - 1) * I.nsrc + src];
+        G(I.qtsrc)[src] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)G(I.tsrc)[it - 2]) /
+                                                  (double)(G(I.tsrc)[it - 1] - G(I.tsrc)[it - 2]));
+      }
+    }
+  }
+  __syncthreads();
   // water level at boundary points (update_boundary_points, :728-756)
   for (int ib = threadIdx.x; ib < I.nbnd; ib += blockDim.x) {
     const float a = G(I.zs_bnd)[(sh_itb0 - 1) * I.nbnd + ib], b = G(I.zs_bnd)[(sh_itb1 - 1) * I.nbnd + ib];
     G(I.zst_bnd)[ib] = a + (b - a) * sh_tbfac;
   }
   __syncthreads();
-  const double t = S->t;
   // update_boundary_conditions (:789-917)
   for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
     const int nmb = G(I.gb_cell)[ib];

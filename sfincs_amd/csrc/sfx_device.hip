@@ -117,7 +117,8 @@ struct DevInst {
   const int32_t* k2_slot; const int32_t* k2_nmb; const int32_t* k2_nmi; const int32_t* k2_ib; const int8_t* k2_dir;
   const float* k2_zbuv;
   void* uvmean;
-  const float* t_bnd; const float* zs_bnd; float* zst_bnd;
+  const float* t_bnd; const float* zs_bnd;
+  int32_t bnd_overlap;           // a Neumann copy and a boundary flux point share a cell
   // sources and meteo
   int32_t nsrc, ntsrc, ntprcp, ntwnd;
   const int32_t* src_cell; const float* tsrc; const float* qsrc; float* qtsrc;
@@ -328,23 +329,48 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     }
   }
   __syncthreads();
-  // water level at boundary points (update_boundary_points, :728-756)
-  for (int ib = threadIdx.x; ib < I.nbnd; ib += blockDim.x) {
-    const float a = G(I.zs_bnd)[(sh_itb0 - 1) * I.nbnd + ib], b = G(I.zs_bnd)[(sh_itb1 - 1) * I.nbnd + ib];
-    G(I.zst_bnd)[ib] = a + (b - a) * sh_tbfac;
-  }
-  __syncthreads();
-  // update_boundary_conditions (:789-917)
+  // Boundary points, conditions and fluxes (:728-1117) in two passes. A flux point evaluates
+  // the level of its water-level boundary cell itself (update_boundary_points and the kcs 2
+  // branch of update_boundary_conditions, the same arithmetic), so only the Neumann copies
+  // (kcs 6) run first, and the barrier after them is needed only when a copy reads a cell
+  // whose level a flux point sets or sets a cell that a flux point reads (bnd_overlap, found
+  // on the host).
   for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
     const int nmb = G(I.gb_cell)[ib];
-    const int k = G(I.kcs)[nmb] & 0x7f;
-    if (k == 2) {
+    if ((G(I.kcs)[nmb] & 0x7f) == 6) G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]];
+  }
+  if (I.bnd_overlap) __syncthreads();
+  // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
+  const int curb = __builtin_amdgcn_readfirstlane(S->cur);
+  const auto q = G((R*)I.q[curb]);
+  const auto uv = G((R*)I.uv[curb]);
+  const auto uvmean = G((R*)I.uvmean);
+  const auto zsmax = G((R*)I.zsmax);
+  const int nl = c.nl;
+  const int itb0 = sh_itb0, itb1 = sh_itb1;
+  const float tbfac = sh_tbfac;
+  for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
+    const int nmb = G(I.k2_nmb)[ib];
+    const int kb = G(I.kcs)[nmb] & 0x7f;
+    if (kb == 6) continue;
+    const int indb = G(I.k2_ib)[ib];
+    const int nmi = G(I.k2_nmi)[ib];
+    const int ip = G(I.k2_slot)[ib];
+    const R zsnmi = (R)G(I.zs)[nmi];
+    float zsb1, zsb0;
+    if (kb == 2) {
+      // update_boundary_points (:728-756) at the points this cell interpolates between
+      auto zst_at = [&](int k) -> float {
+        const float za = G(I.zs_bnd)[(itb0 - 1) * I.nbnd + k], zb = G(I.zs_bnd)[(itb1 - 1) * I.nbnd + k];
+        return za + (zb - za) * tbfac;
+      };
+      // update_boundary_conditions, water-level boundary (:789-917)
       double zst;
       if (I.nbnd > 1) {
-        const float f = G(I.gb_fac)[ib];
-        zst = (double)(G(I.zst_bnd)[G(I.gb_ind1)[ib] - 1] * f + G(I.zst_bnd)[G(I.gb_ind2)[ib] - 1] * (1.0f - f));
+        const float f = G(I.gb_fac)[indb];
+        zst = (double)(zst_at(G(I.gb_ind1)[indb] - 1) * f + zst_at(G(I.gb_ind2)[indb] - 1) * (1.0f - f));
       } else {
-        zst = (double)G(I.zst_bnd)[0];
+        zst = (double)zst_at(0);
       }
       float zb0, zb1;
       if (t < (double)(c.tspinup - 1.0e-3f)) {
@@ -357,29 +383,14 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
         zb1 = (float)zst;
       }
       const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
-      G(I.zsb0)[ib] = sfx::vmax<float>(zb0, floorz);
-      G(I.zsb)[ib] = sfx::vmax<float>(zb1, floorz);
-    } else if (k == 6) {
-      G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]];
+      zsb0 = sfx::vmax<float>(zb0, floorz);
+      zsb1 = sfx::vmax<float>(zb1, floorz);
+    } else {
+      zsb1 = G(I.zsb)[indb];
+      zsb0 = G(I.zsb0)[indb];
     }
-  }
-  __syncthreads();
-  // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
-  const int curb = __builtin_amdgcn_readfirstlane(S->cur);
-  const auto q = G((R*)I.q[curb]);
-  const auto uv = G((R*)I.uv[curb]);
-  const auto uvmean = G((R*)I.uvmean);
-  const auto zsmax = G((R*)I.zsmax);
-  const int nl = c.nl;
-  for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
-    const int nmb = G(I.k2_nmb)[ib];
-    if ((G(I.kcs)[nmb] & 0x7f) == 6) continue;
-    const int indb = G(I.k2_ib)[ib];
-    const int nmi = G(I.k2_nmi)[ib];
-    const int ip = G(I.k2_slot)[ib];
-    const R zsnmi = (R)G(I.zs)[nmi];
-    R zsnmb = G(I.zsb)[indb];
-    R zs0nmb = G(I.zsb0)[indb];
+    R zsnmb = zsb1;
+    R zs0nmb = zsb0;
     R hnmb;
     const R huth = c.huthresh;
     if (I.subgrid) {
@@ -428,8 +439,8 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     }
     if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)S->factime * q[ip] + (R)S->facrel * (R)S->one_minus_factime * uvmean[ib];
     else uvmean[ib] = R(0);
-    G(I.zs)[nmb] = (double)G(I.zsb)[indb];
-    if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], G(I.zs)[nmb]);
+    G(I.zs)[nmb] = (double)zsb1;
+    if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], (double)zsb1);
   }
 }
 
@@ -1340,7 +1351,13 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.k2_nmi = H->upload(nmi2.data(), nmi2.size()); I.k2_ib = H->upload(ib.data(), ib.size());
     I.k2_dir = H->upload(M->ibuvdir.data(), M->ibuvdir.size()); I.k2_zbuv = H->upload(zbuv.data(), zbuv.size());
     I.t_bnd = H->upload(M->t_bnd.data(), M->t_bnd.size()); I.zs_bnd = H->upload(M->zs_bnd.data(), M->zs_bnd.size());
-    I.zst_bnd = H->alloc<float>(std::max(1, M->nbnd));
+    // Neumann copies (kcs 6) run before the flux points only when they share a cell
+    std::vector<uint8_t> fw(M->np, 0), fr(M->np, 0);
+    for (int i = 0; i < M->nkcuv2; ++i)
+      if (M->kcs[nmb[i]] != 6) { fw[nmb[i]] = 1; fr[nmi2[i]] = 1; }
+    I.bnd_overlap = 0;
+    for (int i = 0; i < M->ngbnd; ++i)
+      if (M->kcs[cell[i]] == 6 && (fr[cell[i]] || (nmi[i] >= 0 && fw[nmi[i]]))) I.bnd_overlap = 1;
   }
   I.nsrc = M->nsrc; I.ntsrc = M->ntsrc; I.ntprcp = M->ntprcp; I.ntwnd = M->ntwnd;
   {

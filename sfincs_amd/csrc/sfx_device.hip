@@ -72,6 +72,7 @@ struct DevInst {
   int32_t dense, subgrid, fp64;
   int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
   int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
+  int32_t tn_sh, mtn_sh;         // log2 of tn and mtn (tiles hold 256 cells, so both are powers of two)
   int32_t sk, nstrip;            // dense momentum kernel: strip march with sk rows per wave (sk > 0), per-cell (0), per-point (-1)
   int32_t mtn, mtm, mntn;        // dense momentum, per-cell kernel: tile shape and tiles along n
   int32_t xchunk, xchunk_c;      // momentum / continuity work-list runs per XCD (0: one contiguous band per XCD)
@@ -129,11 +130,30 @@ struct DevInst {
 
 struct Item { int32_t inst, tile; };
 
+// Work list of one launch. A direct list holds the tiles 0..n-1 of one scenario in the
+// XCD order that xcd_order builds on the host: block b runs on XCD b mod 8, and runs of
+// 2^run_sh consecutive tiles are dealt to the XCDs in turn (run_sh < 0: one contiguous
+// band per XCD). Its blocks compute their tile from blockIdx, the kernel's descriptor
+// argument points at that scenario's descriptor, and the step scalars come from the
+// launch arguments, so a block's first memory reads are the descriptor fields and the
+// step scalars, together. Other lists read one Item per block. The descriptor is always
+// reached from the kernel's restrict-qualified argument, which lets the compiler keep
+// its fields in scalar loads after the kernel's first stores.
+struct WorkList {
+  const Item* items;
+  Scalars* sc;
+  int32_t n, run_sh, direct;
+  int32_t inst;                  // host side: the scenario of a direct list
+};
+
 // Dense tile shapes (cells along n x along m, BLOCK threads) of the per-cell momentum and
 // continuity kernels, and strip-march lengths (rows per wave) of the strip momentum
 // kernel, that the autotuner tries; the first of each is the default.
 constexpr int CELL_SHAPES[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {16, 16}};
 constexpr int N_CELL_SHAPES = sizeof(CELL_SHAPES) / sizeof(CELL_SHAPES[0]);
+
+// log2 of a power of two (tile widths: tiles hold BLOCK = 256 cells).
+inline int32_t ilog2(int32_t v) { int32_t k = 0; while ((1 << (k + 1)) <= v) ++k; return k; }
 constexpr int STRIP_ROWS[] = {4, 8, 16};
 constexpr int N_STRIP_ROWS = sizeof(STRIP_ROWS) / sizeof(STRIP_ROWS[0]);
 constexpr int STRIP_COLS = WAVE - 2;             // columns owned by one wave
@@ -163,6 +183,26 @@ size_t cont_tiles(const DevInst& I) {
 
 __device__ __forceinline__ float wave_min(float v) {
   for (int o = WAVE / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// Minimum over the 64 lanes of a wave of positive floats, as their bit patterns (which
+// order like the values), valid in lane 63 only; all lanes must be active. DPP row shifts
+// reduce each row of 16 lanes into its last lane; row_bcast:15 then folds row 0 into
+// row 1 and row 2 into row 3, row_bcast:31 lane 31 into rows 2-3. Lanes without a source
+// keep their own value (bound_ctrl off, old = v), which the minimum ignores.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t v) {
+  const uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xf, false);
+  return o < v ? o : v;
+}
+__device__ __forceinline__ uint32_t wave_min_last(uint32_t v) {
+  v = dpp_min<0x111, 0xf>(v);   // row_shr:1
+  v = dpp_min<0x112, 0xf>(v);   // row_shr:2
+  v = dpp_min<0x114, 0xf>(v);   // row_shr:4
+  v = dpp_min<0x118, 0xf>(v);   // row_shr:8
+  v = dpp_min<0x142, 0xa>(v);   // row_bcast:15
+  v = dpp_min<0x143, 0xc>(v);   // row_bcast:31
   return v;
 }
 
@@ -526,8 +566,8 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
   const auto u1 = G((R*)I.uv[cur ^ 1]);
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
   const int tn = tile % I.mntn, tmi = tile / I.mntn;
-  const int n = tn * I.mtn + (threadIdx.x % I.mtn);
-  const int m = tmi * I.mtm + (threadIdx.x / I.mtn);
+  const int n = tn * I.mtn + (threadIdx.x & (I.mtn - 1));
+  const int m = tmi * I.mtm + (threadIdx.x >> I.mtn_sh);
   if (n >= nmax || m >= mmax) return;
   const int c = m * nmax + n;
   const bool hu = m < mmax - 1, hv = n < nmax - 1;
@@ -587,43 +627,55 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 // points read the rest of their stencil, so waves without wet points finish after one
 // round trip to memory.
 template <typename R, bool SUBGRID>
-__device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, float& mdt) {
-  const auto S = G(I.sc);
+__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, float& mdt) {
+  // The point's geometry and branch inputs (kcuv, the two water levels, the wet/dry
+  // threshold) do not depend on the step scalars, so their loads are issued before the
+  // scalars are read and the active test is made. Returns false if the scenario is not
+  // active (nothing is written).
+  const auto S = G(Sp);
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
+  const int vplane = tile & 1;
+  const int t = tile >> 1;
+  const int tn = t % I.mntn, tmi = t / I.mntn;
+  const int n = tn * I.mtn + (threadIdx.x & (I.mtn - 1));
+  const int m = tmi * I.mtm + (threadIdx.x >> I.mtn_sh);
+  const bool inside = n < nmax && m < mmax;
+  const bool hu = m < mmax - 1, hv = n < nmax - 1;
+  const bool has = inside && (vplane ? hv : hu);
+  const int c = m * nmax + n;
+  const int s = vplane ? np + c : c;
+  const int c_nmu = vplane ? c + 1 : c + nmax;
+  int kc = 0;
+  float fl = 0.f;
+  sfx::UvIn<R> in;
+  in.zs_nm = 0.0; in.zs_nmu = 0.0;
+  if (has) {
+    kc = G(I.kcuv)[s];
+    in.zs_nm = G(I.zs)[c]; in.zs_nmu = G(I.zs)[c_nmu];
+    fl = uv_floor<SUBGRID>(I, s);
+  }
+  if (!S->active) return false;
   const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
   const auto q0 = G((const R*)I.q[cur]);
   const auto u0 = G((const R*)I.uv[cur]);
   const auto q1 = G((R*)I.q[cur ^ 1]);
   const auto u1 = G((R*)I.uv[cur ^ 1]);
-  const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
-  const int vplane = tile & 1;
-  const int t = tile >> 1;
-  const int tn = t % I.mntn, tmi = t / I.mntn;
-  const int n = tn * I.mtn + (threadIdx.x % I.mtn);
-  const int m = tmi * I.mtm + (threadIdx.x / I.mtn);
-  if (n >= nmax || m >= mmax) return;
-  const int c = m * nmax + n;
-  const bool hu = m < mmax - 1, hv = n < nmax - 1;
-  if (vplane ? !hv : !hu) { cnt[0]++; return; }
-  const int s = vplane ? np + c : c;
-  const int c_nmu = vplane ? c + 1 : c + nmax;
-  // branch inputs
-  const int kc = G(I.kcuv)[s];
-  sfx::UvIn<R> in;
-  in.zs_nm = G(I.zs)[c]; in.zs_nmu = G(I.zs)[c_nmu];
+  if (!inside) return true;
+  if (!has) { cnt[0]++; return true; }
   sfx::UvTables tb;
-  if (SUBGRID) tb.zmin = G(I.uv_zmin)[s];
-  else tb.zbuvmx = G(I.zbuvmx)[s];
+  if (SUBGRID) tb.zmin = fl;
+  else tb.zbuvmx = fl;
   if (!(kc == 1 || kc == 6)) {
     q1[s] = q0[s]; u1[s] = u0[s];
     cnt[0]++;
-    return;
+    return true;
   }
   R zsu;
   if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
     q1[s] = R(0); u1[s] = R(0); G(I.kfuv)[s] = 0;
     cnt[1]++;
-    return;
+    return true;
   }
   // the rest of the stencil of a wet point
   int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
@@ -655,6 +707,7 @@ __device__ __forceinline__ void mom_point(const DevInst& I, int tile, int* cnt, 
   int br = 0;
   wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br);
   cnt[br]++;
+  return true;
 }
 
 // Dense scenarios, strip march. A wave covers columns n = n0-1 .. n0+62 of one strip
@@ -877,9 +930,14 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
 // lowers blkmin[tile] (NaN or non-positive values do not take part; the reduction starts
 // at dtmax), the wave's per-branch counts are added to the tile's blkcnt slots, and the
 // sweep-done flag is raised. Positive floats order like their bit patterns, so the
-// minimum is an unsigned atomic min.
+// minimum is an unsigned atomic min. Waves with no value below dtmax (every dry wave)
+// skip the reduction.
 __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float mdt, const int* cnt) {
-  const float m = wave_min(mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax);
+  const float v = mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax;
+  if (__ballot(v < I.c.dtmax)) {
+    const uint32_t m = wave_min_last(__float_as_uint(v));
+    if (threadIdx.x % WAVE == WAVE - 1) atomicMin((unsigned int*)&G(I.blkmin)[tile], m);
+  }
   int c4[4] = {0, 0, 0, 0};
   if (I.count)
     for (int k = 0; k < 4; ++k) {
@@ -888,7 +946,6 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
       c4[k] = v;
     }
   if (threadIdx.x % WAVE == 0) {
-    if (m < I.c.dtmax) atomicMin((unsigned int*)&G(I.blkmin)[tile], __float_as_uint(m));
     if (tile == 0 && threadIdx.x == 0) G(I.sc)->mom_done = 1;
     if (I.count)
       for (int k = 0; k < 4; ++k)
@@ -896,65 +953,87 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
   }
 }
 
+// The (scenario, tile) of this block; false for the padding of a list. The scenario is an
+// index into the launch's descriptor argument (0 for a direct list).
+__device__ __forceinline__ bool work_item(const WorkList& L, int& tile, int& inst) {
+  if (L.direct) {
+    const uint32_t b = blockIdx.x, x = b & 7u, k = b >> 3;
+    uint32_t nat;
+    if (L.run_sh < 0) nat = x * (gridDim.x >> 3) + k;
+    else nat = ((((k >> L.run_sh) << 3) + x) << L.run_sh) + (k & ((1u << L.run_sh) - 1u));
+    if (nat >= (uint32_t)L.n) return false;
+    tile = (int)nat; inst = 0;
+    return true;
+  }
+  const Item it = L.items[blockIdx.x];
+  if (it.inst < 0) return false;
+  tile = it.tile; inst = it.inst;
+  return true;
+}
+
 // One launch per (layout, mesh kind): every block handles one (scenario, tile) item.
 template <typename R, bool SUBGRID>
-__global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  if (it.inst < 0) return;
-  const DevInst& I = insts[it.inst];
-  if (!G(I.sc)->active) return;
+__global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ insts, WorkList L) {
+  int tile, inst;
+  if (!work_item(L, tile, inst)) return;
+  const DevInst& I = insts[inst];
+  Scalars* Sp = L.direct ? L.sc : I.sc;
+  if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
   int cnt[4] = {0, 0, 0, 0};
-  mom_cell<R, SUBGRID>(I, it.tile, cnt, mdt);
-  mom_epilogue(I, it.tile, mdt, cnt);
+  mom_cell<R, SUBGRID>(I, tile, cnt, mdt);
+  mom_epilogue(I, tile, mdt, cnt);
 }
 
 template <typename R, bool SUBGRID>
-__global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  if (it.inst < 0) return;
-  const DevInst& I = insts[it.inst];
-  if (!G(I.sc)->active) return;
+__global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__ insts, WorkList L) {
+  int tile, inst;
+  if (!work_item(L, tile, inst)) return;
+  const DevInst& I = insts[inst];
+  Scalars* Sp = L.direct ? L.sc : I.sc;
   float mdt = I.c.dtmax;
   int cnt[4] = {0, 0, 0, 0};
-  mom_point<R, SUBGRID>(I, it.tile, cnt, mdt);
-  mom_epilogue(I, it.tile, mdt, cnt);
+  if (!mom_point<R, SUBGRID>(I, Sp, tile, cnt, mdt)) return;   // scenario not active
+  mom_epilogue(I, tile, mdt, cnt);
 }
 
 template <typename R, bool SUBGRID>
-__global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  if (it.inst < 0) return;
-  const DevInst& I = insts[it.inst];
-  if (!G(I.sc)->active) return;
+__global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__ insts, WorkList L) {
+  int tile, inst;
+  if (!work_item(L, tile, inst)) return;
+  const DevInst& I = insts[inst];
+  Scalars* Sp = L.direct ? L.sc : I.sc;
+  if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
   int cnt[4] = {0, 0, 0, 0};
-  mom_strip<R, SUBGRID>(I, it.tile, cnt, mdt);
-  mom_epilogue(I, it.tile, mdt, cnt);
+  mom_strip<R, SUBGRID>(I, tile, cnt, mdt);
+  mom_epilogue(I, tile, mdt, cnt);
 }
 
 template <typename R, bool SUBGRID>
-__global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  if (it.inst < 0) return;
-  const DevInst& I = insts[it.inst];
-  if (!G(I.sc)->active) return;
+__global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ insts, WorkList L) {
+  int tile, inst;
+  if (!work_item(L, tile, inst)) return;
+  const DevInst& I = insts[inst];
+  Scalars* Sp = L.direct ? L.sc : I.sc;
+  if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
   int cnt[4] = {0, 0, 0, 0};
-  mom_idx<R, SUBGRID>(I, it.tile, cnt, mdt);
-  mom_epilogue(I, it.tile, mdt, cnt);
+  mom_idx<R, SUBGRID>(I, tile, cnt, mdt);
+  mom_epilogue(I, tile, mdt, cnt);
 }
 
 // Combined uv-points at refinement transitions (sfincs_momentum.f90:757-779).
 template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts, const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  if (it.inst < 0) return;
-  const DevInst& I = insts[it.inst];
-  if (!G(I.sc)->active) return;
-  const int j = it.tile * BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts, WorkList L) {
+  int tile, inst;
+  if (!work_item(L, tile, inst)) return;
+  const DevInst& I = insts[inst];
+  Scalars* Sp = L.direct ? L.sc : I.sc;
+  if (!G(Sp)->active) return;
+  const int j = tile * BLOCK + threadIdx.x;
   if (j >= I.ncuv) return;
-  const int cur1 = G(I.sc)->cur ^ 1;
+  const int cur1 = G(Sp)->cur ^ 1;
   const auto q = G((R*)I.q[cur1]);
   const auto uv = G((R*)I.uv[cur1]);
   const int c = G(I.cuv)[j], a = G(I.cuv)[I.ncuv + j], b = G(I.cuv)[2 * I.ncuv + j];
@@ -1007,12 +1086,12 @@ __device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_m
 }
 
 template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts,
-                                                      const Item* __restrict__ items, int indexed) {
-  const Item it = items[blockIdx.x];
-  if (it.inst < 0) return;
-  const DevInst& I = insts[it.inst];
-  const auto S = G(I.sc);
+__global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts, WorkList L, int indexed) {
+  int tile, inst;
+  if (!work_item(L, tile, inst)) return;
+  const DevInst& I = insts[inst];
+  Scalars* Sp = L.direct ? L.sc : I.sc;
+  const auto S = G(Sp);
   if (!S->active) return;
   const int qb = __builtin_amdgcn_readfirstlane(S->mom_done ? (S->cur ^ 1) : S->cur);
   const auto q = G((const R*)I.q[qb]);
@@ -1020,9 +1099,9 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   const R netprcp = (R)S->netprcp;
   if (!indexed) {
     const int nmax = I.nmax, mmax = I.mmax, np = I.np;
-    const int tn = it.tile % I.ntn, tmi = it.tile / I.ntn;
-    const int n = tn * I.tn + (threadIdx.x % I.tn);
-    const int m = tmi * I.tm + (threadIdx.x / I.tn);
+    const int tn = tile % I.ntn, tmi = tile / I.ntn;
+    const int n = tn * I.tn + (threadIdx.x & (I.tn - 1));
+    const int m = tmi * I.tm + (threadIdx.x >> I.tn_sh);
     if (n >= nmax || m >= mmax) return;
     const int c = m * nmax + n;
     const R q_md = m >= 1 ? q[c - nmax] : R(0);
@@ -1032,7 +1111,7 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     if (I.subgrid) cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
     else cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
   } else {
-    const int c = it.tile * BLOCK + threadIdx.x;
+    const int c = tile * BLOCK + threadIdx.x;
     if (c >= I.np) return;
     const R q_md = q[G(I.zmd)[c]], q_mu = q[G(I.zmu)[c]], q_nd = q[G(I.znd)[c]], q_nu = q[G(I.znu)[c]];
     const int iref = G(I.z_iref)[c];
@@ -1124,6 +1203,7 @@ struct sfx_batch {
   // work lists: momentum dense/indexed x subgrid/regular, combined points, continuity
   // [cell sg, cell reg, strip sg, strip reg, idx sg, idx reg, point sg, point reg]
   size_t off_m[8] = {0, 0, 0, 0, 0, 0, 0, 0}, n_m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  WorkList wl_m[8] = {}, wl_cuv = {}, wl_cd = {}, wl_ci = {};   // launch arguments of the lists
   size_t off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
   bool dirty = true;
   hipGraph_t graph = nullptr;
@@ -1388,6 +1468,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   I.mtn = H->mtile_tn; I.mtm = H->mtile_tm;
   I.xchunk = H->xchunk; I.xchunk_c = H->xchunk_c;
   I.mntn = M->dense ? (M->nmax + I.mtn - 1) / I.mtn : 0;
+  I.tn_sh = ilog2(I.tn); I.mtn_sh = ilog2(I.mtn);
   I.nmt = (int32_t)mom_tiles(I);
   H->nmt_cap = I.nmt;
   if (M->dense) {
@@ -1643,11 +1724,23 @@ int rebuild(sfx_batch* B) {
   B->fp64 = fp64 < 0 ? 0 : fp64;
   std::vector<Item> all;
   const int xchunk = nm ? B->members[0]->hI.xchunk : 0;
+  const int xchunk_c = nm ? B->members[0]->hI.xchunk_c : 0;
+  // A list filled by one member (the lists are appended member by member) with a run
+  // length of 0 or a power of two is launched direct: its blocks compute their tiles.
+  struct Src { int inst = -1; int32_t n = 0, run_sh = 0; };
+  auto source = [](const std::vector<Item>& v, int chunk) {
+    Src r;
+    if (v.empty() || v.front().inst != v.back().inst) return r;
+    if (chunk > 0 && (chunk & (chunk - 1))) return r;
+    r.inst = v.front().inst; r.n = (int32_t)v.size(); r.run_sh = chunk > 0 ? ilog2(chunk) : -1;
+    return r;
+  };
+  Src src_m[8], src_cu = source(cu, 0), src_cd = source(cd, xchunk_c), src_ci = source(ci, xchunk_c);
   for (int k = 0; k < 8; ++k) {
+    src_m[k] = source(mom[k], xchunk);
     if (mom[k].size()) xcd_order(mom[k], xchunk);
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
   }
-  const int xchunk_c = nm ? B->members[0]->hI.xchunk_c : 0;
   if (cu.size()) xcd_order(cu, 0);
   if (cd.size()) xcd_order(cd, xchunk_c);
   if (ci.size()) xcd_order(ci, xchunk_c);
@@ -1673,11 +1766,26 @@ int rebuild(sfx_batch* B) {
   if (all.size()) HIPCHK(hipMemcpy(B->dItems, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice));
   if (ids.size()) HIPCHK(hipMemcpy(B->dIds, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   B->n_live = (int)ids.size();
+  auto wl = [&](size_t off, const Src& r) {
+    WorkList w{};
+    w.items = B->dItems + off;
+    if (r.inst >= 0) {
+      w.direct = 1; w.inst = r.inst; w.sc = B->members[r.inst]->hI.sc; w.n = r.n; w.run_sh = r.run_sh;
+    }
+    return w;
+  };
+  for (int k = 0; k < 8; ++k) B->wl_m[k] = wl(B->off_m[k], src_m[k]);
+  B->wl_cuv = wl(B->off_cuv, src_cu); B->wl_cd = wl(B->off_cd, src_cd); B->wl_ci = wl(B->off_ci, src_ci);
   if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
   if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
   B->graph_steps = 0;
   B->dirty = false;
   return SFX_OK;
+}
+
+// Descriptor argument of a launch: the scenario's own descriptor for a direct list.
+inline const DevInst* base_of(const sfx_batch* B, const WorkList& w) {
+  return w.direct ? B->dInsts + w.inst : B->dInsts;
 }
 
 // Launch phases of one step. what: bit 0 forcing, bit 1 momentum (+cuv), bit 2 continuity.
@@ -1687,20 +1795,19 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   const int nm = B->n_live;
   if ((what & 1) && nm) hipLaunchKernelGGL(k_forcing<R>, dim3(nm), dim3(FBLOCK), 0, st, B->dInsts, B->dIds, forcing_mode);
   if (what & 2) {
-    const Item* it = B->dItems;
-    if (B->n_m[0]) hipLaunchKernelGGL((k_mom_cell<R, true>), dim3(B->n_m[0]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[0]);
-    if (B->n_m[1]) hipLaunchKernelGGL((k_mom_cell<R, false>), dim3(B->n_m[1]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[1]);
-    if (B->n_m[2]) hipLaunchKernelGGL((k_mom_strip<R, true>), dim3(B->n_m[2]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[2]);
-    if (B->n_m[3]) hipLaunchKernelGGL((k_mom_strip<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[3]);
-    if (B->n_m[4]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[4]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[4]);
-    if (B->n_m[5]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[5]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[5]);
-    if (B->n_m[6]) hipLaunchKernelGGL((k_mom_point<R, true>), dim3(B->n_m[6]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[6]);
-    if (B->n_m[7]) hipLaunchKernelGGL((k_mom_point<R, false>), dim3(B->n_m[7]), dim3(BLOCK), 0, st, B->dInsts, it + B->off_m[7]);
-    if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cuv);
+    if (B->n_m[0]) hipLaunchKernelGGL((k_mom_cell<R, true>), dim3(B->n_m[0]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[0]), B->wl_m[0]);
+    if (B->n_m[1]) hipLaunchKernelGGL((k_mom_cell<R, false>), dim3(B->n_m[1]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[1]), B->wl_m[1]);
+    if (B->n_m[2]) hipLaunchKernelGGL((k_mom_strip<R, true>), dim3(B->n_m[2]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[2]), B->wl_m[2]);
+    if (B->n_m[3]) hipLaunchKernelGGL((k_mom_strip<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[3]), B->wl_m[3]);
+    if (B->n_m[4]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[4]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[4]), B->wl_m[4]);
+    if (B->n_m[5]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[5]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[5]), B->wl_m[5]);
+    if (B->n_m[6]) hipLaunchKernelGGL((k_mom_point<R, true>), dim3(B->n_m[6]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[6]), B->wl_m[6]);
+    if (B->n_m[7]) hipLaunchKernelGGL((k_mom_point<R, false>), dim3(B->n_m[7]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[7]), B->wl_m[7]);
+    if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, base_of(B, B->wl_cuv), B->wl_cuv);
   }
   if (what & 4) {
-    if (B->n_cd) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_cd, 0);
-    if (B->n_ci) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci), dim3(BLOCK), 0, st, B->dInsts, B->dItems + B->off_ci, 1);
+    if (B->n_cd) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0);
+    if (B->n_ci) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci), dim3(BLOCK), 0, st, base_of(B, B->wl_ci), B->wl_ci, 1);
   }
 }
 
@@ -2001,6 +2108,7 @@ void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
   I.sk = sk; I.mtn = mtn; I.mtm = mtm; I.tn = tn; I.tm = tm;
   I.mntn = (I.nmax + mtn - 1) / mtn;
   I.ntn = (I.nmax + tn - 1) / tn;
+  I.tn_sh = ilog2(tn); I.mtn_sh = ilog2(mtn);
   I.nmt = (int32_t)mom_tiles(I);
 }
 }  // namespace

@@ -25,7 +25,7 @@ namespace sfx {
 namespace {
 
 constexpr uint32_t MAGIC = 0x5346504Du;   // "SFPM"
-constexpr uint32_t VERSION = 5;
+constexpr uint32_t VERSION = 6;
 
 struct Entry { uint32_t x, g; };
 
@@ -52,6 +52,10 @@ bool enumerate(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std
     const double est = pow_estimate(x, which);
     if (std::fabs(est - r) > 1.4210854715202004e-14 * r) {   // 2^-46
       err = "estimate error too large at x=" + std::to_string(x);
+      return false;
+    }
+    if (near_midpoint(est) != near_midpoint_fp(est)) {       // bit form of the band test
+      err = "midpoint band test disagrees with its double form at x=" + std::to_string(x);
       return false;
     }
     const float fd = f;

@@ -84,8 +84,26 @@ SFX_HD inline double pow_estimate(float x, int which) {
 }
 
 // True when the double value r lies within 2^-32 (relative) of a midpoint between two
-// adjacent floats, i.e. when its float rounding is not certain.
+// adjacent floats, i.e. when its float rounding is not certain. r must be a positive
+// double in the normal float range. On the bits of r = M 2^(E-52) (M the 53-bit
+// significand): float midpoints sit at offset 2^28 of every block of 2^29 units of
+// 2^(E-52) (the 29 significand bits below float precision), so |r - mid| =
+// |low29 - 2^28| units, and 2^-32 r = M / 2^32 units = H + lo / 2^32 with H the top 21
+// bits of M and lo the low word. An integer k is below H + lo / 2^32 when k < H, or
+// k == H and lo != 0.
 SFX_HD inline bool near_midpoint(double r) {
+  union { double d; uint64_t u; } v;
+  v.d = r;
+  const uint32_t lo = (uint32_t)v.u, hi = (uint32_t)(v.u >> 32);
+  const uint32_t low29 = lo & 0x1fffffffu;
+  const uint32_t k = low29 >= 0x10000000u ? low29 - 0x10000000u : 0x10000000u - low29;
+  const uint32_t H = 0x100000u | (hi & 0xfffffu);
+  return k < H || (k == H && lo != 0u);
+}
+
+// The same test evaluated in double arithmetic (kept for the host-side check of the bit
+// form above).
+SFX_HD inline bool near_midpoint_fp(double r) {
   const float f = (float)r;
   const double fd = f;
   const float nb = r >= fd ? nextafterf(f, INFINITY) : nextafterf(f, 0.0f);

@@ -375,8 +375,9 @@ def main():
     mom_gbs = mom_b / (ms[1] * 1e-3) / 1e9
     cont_gbs = cont_b / (ms[2] * 1e-3) / 1e9
     step_gbs = (mom_b + cont_b) / (secs / args.steps) / 1e9
-    kernel = {-1: "k_mom_point<float, true>", 0: "k_mom_cell<float, true>"}.get(tuning["momentum_kernel"],
-                                                                             "k_mom_strip<float, true>")
+    mk = tuning["momentum_kernel"]
+    kernel = (f"k_mom_point<float, true, {-mk}>" if mk < 0 else
+              "k_mom_cell<float, true>" if mk == 0 else "k_mom_strip<float, true>")
     traffic = traffic_of(kernel)
     out = {
         "metric": "cell-updates/s (mom+cont) & % HBM peak; ensemble p50/p99 time-to-result",

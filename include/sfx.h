@@ -206,14 +206,16 @@ int  sfx_profile_steps(sfx_handle* h, int32_t nsteps, double ms[3]);
  * zsderv, zsmax), used to state the FP32 path's rounding error. */
 int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle** out);
 
-/* Autotuner: on this handle's mesh, times the dense momentum kernels (one thread per cell
- * or per uv-point on tiles of several shapes, strip marches of several lengths), the tile
+/* Autotuner: on this handle's mesh, times the dense momentum kernels (one thread per cell,
+ * or 1, 2 or 4 uv-points per thread, on tiles of several shapes; strip marches of several
+ * lengths), the tile
  * shapes of the dense continuity kernel, and for both sweeps (dense or indexed) the runs
  * in which tiles are dealt to the XCDs, and keeps the fastest. Call it before the handle
  * joins a batch (SFX_ERR_STATE otherwise). */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
-/* out = {dense, momentum kernel (-1: per uv-point, 0: per cell, k > 0: strip march of k
- *        rows per wave), momentum tile n, momentum tile m, continuity tile n, continuity
+/* out = {dense, momentum kernel (-1, -2, -4: per uv-point with 1, 2 or 4 points per
+ *        thread, the points of a thread momentum-tile-m rows apart; 0: per cell; k > 0:
+ *        strip march of k rows per wave), momentum tile n, momentum tile m, continuity tile n, continuity
  *        tile m, momentum and continuity work-list runs per XCD (0: one contiguous band
  *        per XCD)} */
 int  sfx_get_tuning(sfx_handle* h, int32_t out[8]);

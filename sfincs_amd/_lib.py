@@ -355,8 +355,8 @@ class Instance:
                     xcd_run=[a[6], a[7]])
 
     def set_tuning(self, momentum_kernel=0, momentum_tile=(64, 4), continuity_tile=(64, 4)):
-        """momentum_kernel: -1 one thread per uv-point, 0 one thread per cell, k > 0 strip
-        march of k rows per wave."""
+        """momentum_kernel: -1, -2, -4 one thread per 1, 2 or 4 uv-points (momentum_tile[1]
+        rows apart), 0 one thread per cell, k > 0 strip march of k rows per wave."""
         a = (C.c_int32 * 5)(momentum_kernel, momentum_tile[0], momentum_tile[1], continuity_tile[0],
                             continuity_tile[1])
         check(lib().sfx_set_tuning(self.h, a), "set_tuning")

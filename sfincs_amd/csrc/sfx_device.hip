@@ -73,7 +73,7 @@ struct DevInst {
   int32_t np, npuv, ncuv, nslot, nmax, mmax, nl;
   int32_t tn, tm, ntn;           // dense continuity tile shape and tiles along n
   int32_t tn_sh, mtn_sh;         // log2 of tn and mtn (tiles hold 256 cells, so both are powers of two)
-  int32_t sk, nstrip;            // dense momentum kernel: strip march with sk rows per wave (sk > 0), per-cell (0), per-point (-1)
+  int32_t sk, nstrip;            // dense momentum kernel: strip march with sk rows per wave (sk > 0), per-cell (0), per-point with -sk points per thread (-1, -2, -4)
   int32_t mtn, mtm, mntn;        // dense momentum, per-cell kernel: tile shape and tiles along n
   int32_t xchunk, xchunk_c;      // momentum / continuity work-list runs per XCD (0: one contiguous band per XCD)
   int32_t count;                 // per-branch counting on
@@ -168,12 +168,12 @@ size_t cell_tiles(const DevInst& I, int tn, int tm) {
   return (size_t)((I.nmax + tn - 1) / tn) * (size_t)((I.mmax + tm - 1) / tm);
 }
 
-// Momentum tiles of one scenario: per-cell tiles or strip-march blocks (dense), or BLOCK
-// uv-points (indexed).
+// Momentum tiles of one scenario: per-cell tiles, per-point tile pairs (u and v points of
+// mtn x (P mtm) cells, P = -sk) or strip-march blocks (dense), or BLOCK uv-points (indexed).
 size_t mom_tiles(const DevInst& I) {
   if (!I.dense) return (size_t)((I.npuv + BLOCK - 1) / BLOCK);
   if (I.sk > 0) return strip_tiles(I, I.sk);
-  return (I.sk < 0 ? 2 : 1) * cell_tiles(I, I.mtn, I.mtm);
+  return I.sk < 0 ? 2 * cell_tiles(I, I.mtn, I.mtm * -I.sk) : cell_tiles(I, I.mtn, I.mtm);
 }
 
 // Continuity tiles: tn x tm cells, or BLOCK cells (indexed).
@@ -626,56 +626,34 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 // what decides its branch (kcuv, the two water levels, its wet/dry threshold); only wet
 // points read the rest of their stencil, so waves without wet points finish after one
 // round trip to memory.
-template <typename R, bool SUBGRID>
-__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, float& mdt) {
-  // The point's geometry and branch inputs (kcuv, the two water levels, the wet/dry
-  // threshold) do not depend on the step scalars, so their loads are issued before the
-  // scalars are read and the active test is made. Returns false if the scenario is not
-  // active (nothing is written).
-  const auto S = G(Sp);
+// One uv-point of the per-point kernel from its preloaded branch inputs: kcuv `kc`, the
+// water levels of its two cells and its wet/dry threshold `fl`.
+template <typename R, bool SUBGRID, typename QC, typename QW>
+__device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, int n, int m, int vplane, int kc,
+                                             double zs_nm, double zs_nmu, float fl, R dt, QC q0, QC u0, QW q1,
+                                             QW u1, int* cnt, float& mdt) {
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
-  const int vplane = tile & 1;
-  const int t = tile >> 1;
-  const int tn = t % I.mntn, tmi = t / I.mntn;
-  const int n = tn * I.mtn + (threadIdx.x & (I.mtn - 1));
-  const int m = tmi * I.mtm + (threadIdx.x >> I.mtn_sh);
-  const bool inside = n < nmax && m < mmax;
+  if (n >= nmax || m >= mmax) return;
   const bool hu = m < mmax - 1, hv = n < nmax - 1;
-  const bool has = inside && (vplane ? hv : hu);
+  if (vplane ? !hv : !hu) { cnt[0]++; return; }
   const int c = m * nmax + n;
   const int s = vplane ? np + c : c;
   const int c_nmu = vplane ? c + 1 : c + nmax;
-  int kc = 0;
-  float fl = 0.f;
   sfx::UvIn<R> in;
-  in.zs_nm = 0.0; in.zs_nmu = 0.0;
-  if (has) {
-    kc = G(I.kcuv)[s];
-    in.zs_nm = G(I.zs)[c]; in.zs_nmu = G(I.zs)[c_nmu];
-    fl = uv_floor<SUBGRID>(I, s);
-  }
-  if (!S->active) return false;
-  const int cur = __builtin_amdgcn_readfirstlane(S->cur);
-  const R dt = (R)S->dt;
-  const auto q0 = G((const R*)I.q[cur]);
-  const auto u0 = G((const R*)I.uv[cur]);
-  const auto q1 = G((R*)I.q[cur ^ 1]);
-  const auto u1 = G((R*)I.uv[cur ^ 1]);
-  if (!inside) return true;
-  if (!has) { cnt[0]++; return true; }
+  in.zs_nm = zs_nm; in.zs_nmu = zs_nmu;
   sfx::UvTables tb;
   if (SUBGRID) tb.zmin = fl;
   else tb.zbuvmx = fl;
   if (!(kc == 1 || kc == 6)) {
     q1[s] = q0[s]; u1[s] = u0[s];
     cnt[0]++;
-    return true;
+    return;
   }
   R zsu;
   if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
     q1[s] = R(0); u1[s] = R(0); G(I.kfuv)[s] = 0;
     cnt[1]++;
-    return true;
+    return;
   }
   // the rest of the stencil of a wet point
   int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
@@ -707,6 +685,49 @@ __device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int til
   int br = 0;
   wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br);
   cnt[br]++;
+}
+
+// Dense scenarios, per-point kernel: tile 2t holds the u-points and tile 2t+1 the v-points
+// of an mtn x (P mtm) block of cells; each thread updates P points, mtm rows apart. The
+// geometry and branch inputs (kcuv, the two water levels, the wet/dry threshold) of all P
+// points do not depend on the step scalars, so their loads are issued together before the
+// scalars are read and the active test is made. Returns false if the scenario is not
+// active (nothing is written).
+template <typename R, bool SUBGRID, int P>
+__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, float& mdt) {
+  const auto S = G(Sp);
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np;
+  const int vplane = tile & 1;
+  const int t = tile >> 1;
+  const int tn = t % I.mntn, tmi = t / I.mntn;
+  const int n = tn * I.mtn + (threadIdx.x & (I.mtn - 1));
+  const int m0 = tmi * I.mtm * P + (threadIdx.x >> I.mtn_sh);
+  int kc[P];
+  float fl[P];
+  double za[P], zb[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int m = m0 + j * I.mtm;
+    const bool has = n < nmax && m < mmax && (vplane ? n < nmax - 1 : m < mmax - 1);
+    const int c = m * nmax + n;
+    const int s = vplane ? np + c : c;
+    kc[j] = 0; fl[j] = 0.f; za[j] = 0.0; zb[j] = 0.0;
+    if (has) {
+      kc[j] = G(I.kcuv)[s];
+      za[j] = G(I.zs)[c]; zb[j] = G(I.zs)[vplane ? c + 1 : c + nmax];
+      fl[j] = uv_floor<SUBGRID>(I, s);
+    }
+  }
+  if (!S->active) return false;
+  const int cur = __builtin_amdgcn_readfirstlane(S->cur);
+  const R dt = (R)S->dt;
+  const auto q0 = G((const R*)I.q[cur]);
+  const auto u0 = G((const R*)I.uv[cur]);
+  const auto q1 = G((R*)I.q[cur ^ 1]);
+  const auto u1 = G((R*)I.uv[cur ^ 1]);
+#pragma unroll
+  for (int j = 0; j < P; ++j)
+    point_finish<R, SUBGRID>(I, S, n, m0 + j * I.mtm, vplane, kc[j], za[j], zb[j], fl[j], dt, q0, u0, q1, u1, cnt, mdt);
   return true;
 }
 
@@ -985,7 +1006,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ 
   mom_epilogue(I, tile, mdt, cnt);
 }
 
-template <typename R, bool SUBGRID>
+template <typename R, bool SUBGRID, int P>
 __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__ insts, WorkList L) {
   int tile, inst;
   if (!work_item(L, tile, inst)) return;
@@ -993,7 +1014,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__
   Scalars* Sp = L.direct ? L.sc : I.sc;
   float mdt = I.c.dtmax;
   int cnt[4] = {0, 0, 0, 0};
-  if (!mom_point<R, SUBGRID>(I, Sp, tile, cnt, mdt)) return;   // scenario not active
+  if (!mom_point<R, SUBGRID, P>(I, Sp, tile, cnt, mdt)) return;   // scenario not active
   mom_epilogue(I, tile, mdt, cnt);
 }
 
@@ -1185,6 +1206,20 @@ template <typename T> T* sfx_handle::upload(const T* src, size_t n) {
   return p;
 }
 
+// Momentum work lists of a batch, one per (kernel, mesh kind): [cell sg, cell reg, strip sg,
+// strip reg, idx sg, idx reg, point P=1 sg, reg, point P=2 sg, reg, point P=4 sg, reg].
+constexpr int NLIST_M = 12;
+
+// Momentum list of a scenario's kernel choice.
+inline int mom_list(const DevInst& I) {
+  int k;
+  if (!I.dense) k = 4;
+  else if (I.sk > 0) k = 2;
+  else if (I.sk == 0) k = 0;
+  else k = 6 + 2 * ilog2(-I.sk);
+  return k + (I.subgrid ? 0 : 1);
+}
+
 struct sfx_batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1202,8 +1237,8 @@ struct sfx_batch {
   // launch descriptors (offsets into dItems)
   // work lists: momentum dense/indexed x subgrid/regular, combined points, continuity
   // [cell sg, cell reg, strip sg, strip reg, idx sg, idx reg, point sg, point reg]
-  size_t off_m[8] = {0, 0, 0, 0, 0, 0, 0, 0}, n_m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  WorkList wl_m[8] = {}, wl_cuv = {}, wl_cd = {}, wl_ci = {};   // launch arguments of the lists
+  size_t off_m[NLIST_M] = {}, n_m[NLIST_M] = {};
+  WorkList wl_m[NLIST_M] = {}, wl_cuv = {}, wl_cd = {}, wl_ci = {};   // launch arguments of the lists
   size_t off_cuv = 0, n_cuv = 0, off_cd = 0, n_cd = 0, off_ci = 0, n_ci = 0;
   bool dirty = true;
   hipGraph_t graph = nullptr;
@@ -1700,7 +1735,7 @@ int rebuild(sfx_batch* B) {
   HIPCHK(hipSetDevice(B->device));
   const size_t nm = B->members.size();
   std::vector<DevInst> insts(nm);
-  std::vector<Item> mom[8], cu, cd, ci;
+  std::vector<Item> mom[NLIST_M], cu, cd, ci;
   std::vector<int32_t> ids;
   int fp64 = -1;
   if (B->live.size() != nm) B->live.assign(nm, 1);
@@ -1712,7 +1747,7 @@ int rebuild(sfx_batch* B) {
     if (!B->live[i]) continue;
     ids.push_back((int32_t)i);
     const DevInst& I = H->hI;
-    std::vector<Item>& mv = mom[(I.dense ? (I.sk > 0 ? 2 : (I.sk < 0 ? 6 : 0)) : 4) + (I.subgrid ? 0 : 1)];
+    std::vector<Item>& mv = mom[mom_list(I)];
     const size_t tm = mom_tiles(I), tc = cont_tiles(I);
     for (size_t t = 0; t < tm; ++t) mv.push_back(Item{(int)i, (int)t});
     for (size_t t = 0; t < tc; ++t) (I.dense ? cd : ci).push_back(Item{(int)i, (int)t});
@@ -1735,8 +1770,8 @@ int rebuild(sfx_batch* B) {
     r.inst = v.front().inst; r.n = (int32_t)v.size(); r.run_sh = chunk > 0 ? ilog2(chunk) : -1;
     return r;
   };
-  Src src_m[8], src_cu = source(cu, 0), src_cd = source(cd, xchunk_c), src_ci = source(ci, xchunk_c);
-  for (int k = 0; k < 8; ++k) {
+  Src src_m[NLIST_M], src_cu = source(cu, 0), src_cd = source(cd, xchunk_c), src_ci = source(ci, xchunk_c);
+  for (int k = 0; k < NLIST_M; ++k) {
     src_m[k] = source(mom[k], xchunk);
     if (mom[k].size()) xcd_order(mom[k], xchunk);
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
@@ -1774,7 +1809,7 @@ int rebuild(sfx_batch* B) {
     }
     return w;
   };
-  for (int k = 0; k < 8; ++k) B->wl_m[k] = wl(B->off_m[k], src_m[k]);
+  for (int k = 0; k < NLIST_M; ++k) B->wl_m[k] = wl(B->off_m[k], src_m[k]);
   B->wl_cuv = wl(B->off_cuv, src_cu); B->wl_cd = wl(B->off_cd, src_cd); B->wl_ci = wl(B->off_ci, src_ci);
   if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
   if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
@@ -1801,8 +1836,12 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
     if (B->n_m[3]) hipLaunchKernelGGL((k_mom_strip<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[3]), B->wl_m[3]);
     if (B->n_m[4]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[4]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[4]), B->wl_m[4]);
     if (B->n_m[5]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[5]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[5]), B->wl_m[5]);
-    if (B->n_m[6]) hipLaunchKernelGGL((k_mom_point<R, true>), dim3(B->n_m[6]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[6]), B->wl_m[6]);
-    if (B->n_m[7]) hipLaunchKernelGGL((k_mom_point<R, false>), dim3(B->n_m[7]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[7]), B->wl_m[7]);
+    if (B->n_m[6]) hipLaunchKernelGGL((k_mom_point<R, true, 1>), dim3(B->n_m[6]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[6]), B->wl_m[6]);
+    if (B->n_m[7]) hipLaunchKernelGGL((k_mom_point<R, false, 1>), dim3(B->n_m[7]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[7]), B->wl_m[7]);
+    if (B->n_m[8]) hipLaunchKernelGGL((k_mom_point<R, true, 2>), dim3(B->n_m[8]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[8]), B->wl_m[8]);
+    if (B->n_m[9]) hipLaunchKernelGGL((k_mom_point<R, false, 2>), dim3(B->n_m[9]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[9]), B->wl_m[9]);
+    if (B->n_m[10]) hipLaunchKernelGGL((k_mom_point<R, true, 4>), dim3(B->n_m[10]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[10]), B->wl_m[10]);
+    if (B->n_m[11]) hipLaunchKernelGGL((k_mom_point<R, false, 4>), dim3(B->n_m[11]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[11]), B->wl_m[11]);
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, base_of(B, B->wl_cuv), B->wl_cuv);
   }
   if (what & 4) {
@@ -2116,8 +2155,10 @@ void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
 extern "C" int sfx_set_tuning(sfx_handle* H, const int32_t in[5]) {
   if (!H || !in) { set_error("null"); return SFX_ERR_ARG; }
   const int sk = in[0], mtn = in[1], mtm = in[2], tn = in[3], tm = in[4];
-  if (sk < -1 || sk > 256 || mtn <= 0 || mtm <= 0 || mtn * mtm != BLOCK || tn <= 0 || tm <= 0 || tn * tm != BLOCK) {
-    set_error("sfx_set_tuning: the momentum kernel must be -1, 0 or 1..256 strip rows and tiles must hold 256 cells");
+  const bool sk_ok = sk == -1 || sk == -2 || sk == -4 || (sk >= 0 && sk <= 256);
+  if (!sk_ok || mtn <= 0 || mtm <= 0 || mtn * mtm != BLOCK || tn <= 0 || tm <= 0 || tn * tm != BLOCK) {
+    set_error("sfx_set_tuning: the momentum kernel must be -1, -2, -4 (per-point), 0 (per-cell) or 1..256 strip rows, "
+              "and tiles must hold 256 cells");
     return SFX_ERR_ARG;
   }
   if (H->in_batches) { set_error("sfx_set_tuning: set shapes before adding the scenario to a batch"); return SFX_ERR_STATE; }
@@ -2169,12 +2210,14 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int tn0 = H->hI.tn, tm0 = H->hI.tm;
-  // momentum: per-cell and per-point tiles, strip marches
+  // momentum: per-cell tiles, per-point tiles with 1, 2 or 4 points per thread, strip marches
   float best = 1e30f;
   int bsk = H->hI.sk, bmtn = H->hI.mtn, bmtm = H->hI.mtm;
-  for (int k = 0; dense && k < 2 * N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
-    const bool tiled = k < 2 * N_CELL_SHAPES;
-    const int sk = tiled ? (k < N_CELL_SHAPES ? 0 : -1) : STRIP_ROWS[k - 2 * N_CELL_SHAPES];
+  const int kinds[] = {0, -1, -2, -4};
+  constexpr int NK = sizeof(kinds) / sizeof(kinds[0]);
+  for (int k = 0; dense && k < NK * N_CELL_SHAPES + N_STRIP_ROWS && !rc; ++k) {
+    const bool tiled = k < NK * N_CELL_SHAPES;
+    const int sk = tiled ? kinds[k / N_CELL_SHAPES] : STRIP_ROWS[k - NK * N_CELL_SHAPES];
     const int mtn = tiled ? CELL_SHAPES[k % N_CELL_SHAPES][0] : bmtn;
     const int mtm = tiled ? CELL_SHAPES[k % N_CELL_SHAPES][1] : bmtm;
     set_shapes(H, sk, mtn, mtm, tn0, tm0);

@@ -240,20 +240,44 @@ def test_branch_counters_cover_every_point():
 
 SHAPES = [(0, (64, 4), (64, 4)), (0, (256, 1), (128, 2)), (0, (32, 8), (256, 1)), (4, (64, 4), (32, 8)),
           (8, (64, 4), (64, 4)), (16, (64, 4), (64, 4)), (-1, (64, 4), (64, 4)), (-1, (32, 8), (128, 2)),
-          (-1, (256, 1), (64, 4)), (-1, (16, 16), (16, 16)), (0, (16, 16), (32, 8))]
+          (-1, (256, 1), (64, 4)), (-1, (16, 16), (16, 16)), (0, (16, 16), (32, 8)),
+          (-2, (64, 4), (64, 4)), (-2, (256, 1), (128, 2)), (-2, (16, 16), (64, 4)), (-4, (64, 4), (32, 8)),
+          (-4, (32, 8), (64, 4)), (-4, (128, 2), (256, 1))]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"kernel{a}_mom{b[0]}x{b[1]}_cont{c[0]}x{c[1]}" for a, b, c in SHAPES])
 def test_every_dense_kernel_shape_matches_oracle(shape):
-    """Each dense momentum variant (per-cell and per-point tiles, strip marches) and continuity tile the
-    autotuner can pick gives the oracle's result; the grid (70 x 83) is not a multiple of
-    any tile, so partial tiles and strips are exercised."""
+    """Each dense momentum variant (per-cell tiles, per-point tiles with 1, 2 or 4 points per
+    thread, strip marches) and continuity tile the autotuner can pick gives the oracle's
+    result; the grid (70 x 83) is not a multiple of any tile, so partial tiles and strips
+    are exercised."""
     strip, mt, ct = shape
     print(f"case shape {shape}", flush=True)
     sc = W.subgrid_case(70, 83, dx=25.0, seed=19, t_end=900.0, nsrc=2, rain_mmh=20.0)
     m, orc = oracle_for(sc)
     inst = gpu_for(m)
     inst.set_tuning(strip, mt, ct)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    print(shape, st["nt"], d, flush=True)
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+
+
+REG_POINT = [(-1, (64, 4)), (-2, (64, 4)), (-2, (16, 16)), (-4, (32, 8)), (-4, (256, 1))]
+
+
+@pytest.mark.parametrize("shape", REG_POINT, ids=[f"kernel{a}_mom{b[0]}x{b[1]}" for a, b in REG_POINT])
+def test_regular_point_kernels_match_oracle(shape):
+    """The per-point momentum kernels with 1, 2 and 4 points per thread on a regular
+    (non-subgrid) mesh of 53 x 77 cells, which no tile divides."""
+    kind, mt = shape
+    print(f"case regular point kernel {shape}", flush=True)
+    sc = W.regular_case(53, 77, dx=50.0, seed=23, t_end=1200.0, rain_mmh=30.0)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    inst.set_tuning(kind, mt, (64, 4))
     rc, st = inst.update(-1000.0)
     orc.update(sc.t_end)
     d = compare_state(inst, orc)

@@ -27,11 +27,13 @@ PEAK_HBM_GBS = 8000.0
 # subgrid, zb 4 for regular). Per uv-point by branch:
 #   copied (kcuv not 1/6): kcuv 1 + old q,uv 8 + new q,uv 8                      = 17
 #   dry:                   kcuv 1 + zmin/zbuvmx 4 + new q,uv 8 + kfuv 1           = 14
+#     (per-point kernel:   + kfuv read 1                                          = 15)
+#   dry, zeros held:       kcuv 1 + kfuv 1 + zmin/zbuvmx 4 (per-point kernel)     =  6
 #   wet (regular):         kcuv 1 + zbuvmx 4 + gn2uv 4 + kfuv 2 + mask 1 + q,uv 16 = 28
 #   full (subgrid):        kcuv 1 + zmin 4 + s4 16 + fnfit 4 + kfuv 2 + mask 1 + q,uv 16 = 44
 #   partial (subgrid):     kcuv 1 + zmin 4 + s4 16 + 2 levels 32 + kfuv 2 + mask 1 + q,uv 16 = 72
 MOM_CELL = {True: 20, False: 12}
-MOM_UV = {"copied": 17, "dry": 14, "wet": {True: 44, False: 28}, "partial": 72}
+MOM_UV = {"copied": 17, "dry": 14, "dry_point": 15, "held": 6, "wet": {True: 44, False: 28}, "partial": 72}
 # Continuity per cell: subgrid kcs 1 + faces 8 + V 16 + zs 16 + zs0 8 + zsderv 4 + zsmax 8
 # + zmin/zmax/volmax 12 + 2 levels 8 = 81; regular kcs 1 + faces 8 + zs 16 + zsmax 8 = 33.
 CONT_CELL = {True: 81, False: 33}
@@ -47,16 +49,17 @@ def traffic_of(kernel):
         return json.load(f).get("bytes_per_launch", {}).get(kernel)
 
 
-def byte_model(sizes, counts_per_step):
+def byte_model(sizes, counts_per_step, point_kernel):
     sg = bool(sizes["subgrid"])
     np_ = sizes["np"]
-    c0, c1, c2, c3 = counts_per_step
+    c0, c1, c2, c3, c4 = counts_per_step
     invalid = 2 * np_ - sizes["npuv"]
     copied = max(0.0, c0 - invalid)
-    mom = np_ * MOM_CELL[sg] + copied * MOM_UV["copied"] + c1 * MOM_UV["dry"] + c2 * MOM_UV["wet"][sg] + \
-        c3 * MOM_UV["partial"]
+    dry = MOM_UV["dry_point"] if point_kernel else MOM_UV["dry"]
+    mom = np_ * MOM_CELL[sg] + copied * MOM_UV["copied"] + c1 * dry + c2 * MOM_UV["wet"][sg] + \
+        c3 * MOM_UV["partial"] + c4 * MOM_UV["held"]
     cont = np_ * CONT_CELL[sg]
-    return mom, cont, dict(copied=copied, dry=c1, wet=c2, partial=c3)
+    return mom, cont, dict(copied=copied, dry=c1, dry_zeros_held=c4, wet=c2, partial=c3)
 
 
 def cpu_baseline(sc, model, inst, nsteps):
@@ -119,7 +122,7 @@ def quadtree(args):
     print(f"# generated quadtree: np {sizes['np']} npuv {sizes['npuv']} ncuv {sizes['ncuv']} in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     inst = _lib.Instance(model)
-    tuning = inst.autotune(3)
+    tuning = inst.autotune(11)
     print(f"# tuning {tuning}", file=sys.stderr, flush=True)
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
@@ -134,7 +137,7 @@ def quadtree(args):
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     batch.advance(8)
-    copied, dry, wet, part = counts
+    copied, dry, wet, part, _ = counts
     mom_b = sizes["np"] * IDX_MOM_CELL + copied * IDX_MOM_UV["copied"] + dry * IDX_MOM_UV["dry"] + \
         (wet + part) * IDX_MOM_UV["wet"] + sizes["ncuv"] * 36
     cont_b = sizes["np"] * IDX_CONT_CELL
@@ -330,7 +333,7 @@ def main():
     t_gen = time.perf_counter() - t0
     sizes = model.sizes()
     inst = _lib.Instance(model)
-    tuning = inst.autotune(3)
+    tuning = inst.autotune(11)
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     print(f"# generated {sizes['np']} cells in {t_gen:.1f}s; tuning {tuning}", file=sys.stderr, flush=True)
@@ -348,7 +351,7 @@ def main():
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     batch.advance(8)
-    mom_b, cont_b, branches = byte_model(sizes, counts)
+    mom_b, cont_b, branches = byte_model(sizes, counts, tuning["momentum_kernel"] < 0)
     print(f"# per launch ms forcing/momentum/continuity = {ms}; branches {branches}", file=sys.stderr, flush=True)
 
     # timed region: exactly K steps (graph-captured), barrier + synchronize on both sides

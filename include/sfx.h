@@ -175,10 +175,12 @@ int  sfx_update_forcing(sfx_handle* h, double t, float dt);
  * one step, dtrange > 0 runs to t + dtrange. Device-resident loop, graph-captured. */
 int  sfx_update(sfx_handle* h, double dtrange, sfx_step_stats* st);
 int  sfx_get_time(sfx_handle* h, double* t, float* dt, int64_t* nt);
-/* Per-branch uv-point counters of the last sweeps (dry, wet-regular, subgrid-full,
- * subgrid-partial), enabled by sfx_set_counting(h,1); used for the byte model. */
+/* Per-branch uv-point counters of the sweeps since counting was enabled by
+ * sfx_set_counting(h,1): {copied (kcuv not 1/6, and the absent slots of a dense grid),
+ * dry with zeros stored, wet regular or subgrid full, subgrid partial, dry with the zeros
+ * already in both q/uv buffers (nothing stored)}; used for the byte model. */
 int  sfx_set_counting(sfx_handle* h, int32_t on);
-int  sfx_get_counters(sfx_handle* h, int64_t counts[4]);
+int  sfx_get_counters(sfx_handle* h, int64_t counts[5]);
 /* FP64 volume accumulator: total water volume (sum of zs*A or V) in double. */
 int  sfx_total_volume(sfx_handle* h, double* vol);
 
@@ -210,8 +212,9 @@ int  sfx_create_ex(const sfx_model* m, int32_t device, int32_t fp64, sfx_handle*
  * or 1, 2 or 4 uv-points per thread, on tiles of several shapes; strip marches of several
  * lengths), the tile
  * shapes of the dense continuity kernel, and for both sweeps (dense or indexed) the runs
- * in which tiles are dealt to the XCDs, and keeps the fastest. Call it before the handle
- * joins a batch (SFX_ERR_STATE otherwise). */
+ * in which tiles are dealt to the XCDs, and keeps the fastest by the median of `trials`
+ * timed launches per candidate. Call it before the handle joins a batch (SFX_ERR_STATE
+ * otherwise). */
 int  sfx_autotune(sfx_handle* h, int32_t trials);
 /* out = {dense, momentum kernel (-1, -2, -4: per uv-point with 1, 2 or 4 points per
  *        thread, the points of a thread momentum-tile-m rows apart; 0: per cell; k > 0:

@@ -338,7 +338,7 @@ class Instance:
         check(lib().sfx_set_counting(self.h, int(on)), "set_counting")
 
     def counters(self):
-        a = (C.c_int64 * 4)()
+        a = (C.c_int64 * 5)()
         check(lib().sfx_get_counters(self.h, a), "get_counters")
         return list(a)
 

@@ -84,7 +84,7 @@ struct DevInst {
   // address and no wave waits for the others of its block); the fold at the start of the
   // next step reduces the tiles and sets every word back to dtmax.
   float* blkmin;
-  uint32_t* blkcnt;              // per tile: uv-points copied (kcuv not 1/6), dry, wet/full, partial
+  uint32_t* blkcnt;              // per tile, NCNT words: uv-points copied (kcuv not 1/6), dry, wet/full, partial, dry with zeros held
   // cells
   double* zs;
   double* vol;
@@ -99,7 +99,7 @@ struct DevInst {
   // uv slots
   void* q[2];
   void* uv[2];
-  int8_t* kfuv;
+  int8_t* kfuv;                  // bit 0: kfuv of the reference; bits 1-2: zero flags of the per-point kernel (KF_*)
   const int8_t* kcuv;
   const int8_t* mask_adv;
   const float* zbuvmx;
@@ -127,6 +127,17 @@ struct DevInst {
   const float* cdval; const float* x73;
   sfx::PowTables pw;
 };
+
+// Flags of a dry uv-point kept in its kfuv byte by the per-point momentum kernel (bit 0
+// is the reference's kfuv and every reader masks it): KF_ZERO1, this kernel stored q = uv = 0
+// at the point in the last step; KF_ZERO2, it did so in two consecutive steps, which write
+// the two ping-pong buffers, so both hold zeros and a dry point needs no store. Every other
+// write of kfuv (wet points, the other kernels, the host) stores 0 or 1 and clears them.
+constexpr int8_t KF_ZERO2 = 2, KF_ZERO1 = 4;
+
+// Per-branch uv-point counters of a momentum tile: copied (kcuv not 1 or 6), dry with zeros
+// stored, wet (regular) or full (subgrid), partial (subgrid), dry with zeros already held.
+constexpr int NCNT = 5;
 
 struct Item { int32_t inst, tile; };
 
@@ -546,7 +557,7 @@ __device__ __forceinline__ void gather_uv(const DevInst& I, QP q0, QP u0, int s,
   in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
   in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
   in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
-  in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s];
+  in.kfuv = G(I.kfuv)[s] & 1; in.mask_adv = G(I.mask_adv)[s];
   if (SUBGRID) {
     in.vol_nm = G(I.vol)[c_nm]; in.vol_nmu = G(I.vol)[c_nmu];
     in.zsderv_nm = G((const R*)I.zsderv)[c_nm]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
@@ -627,9 +638,10 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 // points read the rest of their stencil, so waves without wet points finish after one
 // round trip to memory.
 // One uv-point of the per-point kernel from its preloaded branch inputs: kcuv `kc`, the
-// water levels of its two cells and its wet/dry threshold `fl`.
+// kfuv byte `kf` (with the zero flags), the water levels of its two cells and its wet/dry
+// threshold `fl`.
 template <typename R, bool SUBGRID, typename QC, typename QW>
-__device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, int n, int m, int vplane, int kc,
+__device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, int n, int m, int vplane, int kc, int kf,
                                              double zs_nm, double zs_nmu, float fl, R dt, QC q0, QC u0, QW q1,
                                              QW u1, int* cnt, float& mdt) {
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
@@ -651,7 +663,9 @@ __device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, in
   }
   R zsu;
   if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
-    q1[s] = R(0); u1[s] = R(0); G(I.kfuv)[s] = 0;
+    if (kf & KF_ZERO2) { cnt[4]++; return; }   // both buffers already hold the zeros
+    q1[s] = R(0); u1[s] = R(0);
+    G(I.kfuv)[s] = (kf & KF_ZERO1) ? KF_ZERO2 : KF_ZERO1;
     cnt[1]++;
     return;
   }
@@ -674,7 +688,7 @@ __device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, in
   in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
   in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
   in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
-  in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s];
+  in.kfuv = kf & 1; in.mask_adv = G(I.mask_adv)[s];
   if (SUBGRID) {
     in.vol_nm = G(I.vol)[c]; in.vol_nmu = G(I.vol)[c_nmu];
     in.zsderv_nm = G((const R*)I.zsderv)[c]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
@@ -702,7 +716,7 @@ __device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int til
   const int tn = t % I.mntn, tmi = t / I.mntn;
   const int n = tn * I.mtn + (threadIdx.x & (I.mtn - 1));
   const int m0 = tmi * I.mtm * P + (threadIdx.x >> I.mtn_sh);
-  int kc[P];
+  int kc[P], kf[P];
   float fl[P];
   double za[P], zb[P];
 #pragma unroll
@@ -711,9 +725,9 @@ __device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int til
     const bool has = n < nmax && m < mmax && (vplane ? n < nmax - 1 : m < mmax - 1);
     const int c = m * nmax + n;
     const int s = vplane ? np + c : c;
-    kc[j] = 0; fl[j] = 0.f; za[j] = 0.0; zb[j] = 0.0;
+    kc[j] = 0; kf[j] = 0; fl[j] = 0.f; za[j] = 0.0; zb[j] = 0.0;
     if (has) {
-      kc[j] = G(I.kcuv)[s];
+      kc[j] = G(I.kcuv)[s]; kf[j] = G(I.kfuv)[s];
       za[j] = G(I.zs)[c]; zb[j] = G(I.zs)[vplane ? c + 1 : c + nmax];
       fl[j] = uv_floor<SUBGRID>(I, s);
     }
@@ -727,7 +741,8 @@ __device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int til
   const auto u1 = G((R*)I.uv[cur ^ 1]);
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    point_finish<R, SUBGRID>(I, S, n, m0 + j * I.mtm, vplane, kc[j], za[j], zb[j], fl[j], dt, q0, u0, q1, u1, cnt, mdt);
+    point_finish<R, SUBGRID>(I, S, n, m0 + j * I.mtm, vplane, kc[j], kf[j], za[j], zb[j], fl[j], dt, q0, u0, q1, u1, cnt,
+                             mdt);
   return true;
 }
 
@@ -834,7 +849,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           in.qy_ndm = B.l_vq; in.v_ndm = B.l_vu;
           in.qy_nmu = C.vq; in.v_nmu = C.vu;
           in.qy_ndmu = C.l_vq; in.v_ndmu = C.l_vu;
-          in.kfuv = G(I.kfuv)[c]; in.mask_adv = G(I.mask_adv)[c]; in.idir = 0; in.iref = 1; in.itype = 0;
+          in.kfuv = G(I.kfuv)[c] & 1; in.mask_adv = G(I.mask_adv)[c]; in.idir = 0; in.iref = 1; in.itype = 0;
           in.tauwu = tauwu; in.tauwv = tauwv;
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = C.vol; in.zsderv_nm = B.zd; in.zsderv_nmu = C.zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)C.zd; }
@@ -863,7 +878,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           in.qy_ndm = A.uq; in.v_ndm = A.uu;
           in.qy_nmu = B.r_uq; in.v_nmu = B.r_uu;
           in.qy_ndmu = A.r_uq; in.v_ndmu = A.r_uu;
-          in.kfuv = G(I.kfuv)[s]; in.mask_adv = G(I.mask_adv)[s]; in.idir = 1; in.iref = 1; in.itype = 0;
+          in.kfuv = G(I.kfuv)[s] & 1; in.mask_adv = G(I.mask_adv)[s]; in.idir = 1; in.iref = 1; in.itype = 0;
           in.tauwu = tauwu; in.tauwv = tauwv;
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = B.r_vol; in.zsderv_nm = B.zd; in.zsderv_nmu = B.r_zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)B.r_zd; }
@@ -934,7 +949,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
     in.uu_nmd = in.uu_nmu = in.uu_ndm = in.uu_num = in.v_ndm = in.v_ndmu = in.v_nm = in.v_nmu = R(0);
     in.kcuv_nmd = in.kcuv_nmu = 0;
   }
-  in.kfuv = G(I.kfuv)[ip]; in.mask_adv = G(I.mask_adv)[ip];
+  in.kfuv = G(I.kfuv)[ip] & 1; in.mask_adv = G(I.mask_adv)[ip];
   in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
   if (SUBGRID) {
     in.vol_nm = G(I.vol)[nm]; in.vol_nmu = G(I.vol)[nmu];
@@ -959,18 +974,18 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
     const uint32_t m = wave_min_last(__float_as_uint(v));
     if (threadIdx.x % WAVE == WAVE - 1) atomicMin((unsigned int*)&G(I.blkmin)[tile], m);
   }
-  int c4[4] = {0, 0, 0, 0};
+  int cw[NCNT] = {0, 0, 0, 0, 0};
   if (I.count)
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NCNT; ++k) {
       int v = cnt[k];
       for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-      c4[k] = v;
+      cw[k] = v;
     }
   if (threadIdx.x % WAVE == 0) {
     if (tile == 0 && threadIdx.x == 0) G(I.sc)->mom_done = 1;
     if (I.count)
-      for (int k = 0; k < 4; ++k)
-        if (c4[k]) atomicAdd((unsigned int*)&G(I.blkcnt)[4 * (uint32_t)tile + k], (unsigned int)c4[k]);
+      for (int k = 0; k < NCNT; ++k)
+        if (cw[k]) atomicAdd((unsigned int*)&G(I.blkcnt)[NCNT * (uint32_t)tile + k], (unsigned int)cw[k]);
   }
 }
 
@@ -1001,7 +1016,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ 
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
-  int cnt[4] = {0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0};
   mom_cell<R, SUBGRID>(I, tile, cnt, mdt);
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1013,7 +1028,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   float mdt = I.c.dtmax;
-  int cnt[4] = {0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0};
   if (!mom_point<R, SUBGRID, P>(I, Sp, tile, cnt, mdt)) return;   // scenario not active
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1026,7 +1041,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
-  int cnt[4] = {0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0};
   mom_strip<R, SUBGRID>(I, tile, cnt, mdt);
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1039,7 +1054,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ i
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
-  int cnt[4] = {0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0};
   mom_idx<R, SUBGRID>(I, tile, cnt, mdt);
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1515,7 +1530,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     std::vector<float> init(H->nmt_cap, I.c.dtmax);
     I.blkmin = H->upload(init.data(), init.size());
   }
-  I.blkcnt = H->alloc<uint32_t>(4 * H->nmt_cap);
+  I.blkcnt = H->alloc<uint32_t>(NCNT * H->nmt_cap);
   H->dvol = H->alloc<double>(1);
   H->dI = H->upload(&I, 1);
   for (void* p : H->allocs) if (!p) { set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
@@ -1639,7 +1654,7 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
     if (nbytes != n) { set_error("size mismatch"); return SFX_ERR_ARG; }
     std::vector<int8_t> b(H->nslot + 1);
     HIPCHK(hipMemcpy(b.data(), I.kfuv, b.size(), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n; ++i) ((int8_t*)dst)[i] = b[H->ref2slot[i]];
+    for (size_t i = 0; i < n; ++i) ((int8_t*)dst)[i] = (int8_t)(b[H->ref2slot[i]] & 1);
     return SFX_OK;
   }
   if (k == "zs0" || k == "zsderv" || k == "zsmax") {
@@ -1670,7 +1685,14 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
   }
   if (k == "q" || k == "uv") {
     void* dst = k == "q" ? I.q[s.cur] : I.uv[s.cur];
-    return H->fp64 ? set_uv_field<double>(H, dst, src, nbytes) : set_uv_field<float>(H, dst, src, nbytes);
+    rc = H->fp64 ? set_uv_field<double>(H, dst, src, nbytes) : set_uv_field<float>(H, dst, src, nbytes);
+    if (rc) return rc;
+    // only the current buffer changed: the per-point kernel's zero flags no longer hold
+    std::vector<int8_t> b(H->nslot + 1);
+    HIPCHK(hipMemcpy(b.data(), I.kfuv, b.size(), hipMemcpyDeviceToHost));
+    for (auto& v : b) v = (int8_t)(v & 1);
+    HIPCHK(hipMemcpy(I.kfuv, b.data(), b.size(), hipMemcpyHostToDevice));
+    return SFX_OK;
   }
   if (k == "kfuv") {
     const size_t n = H->sz.npuv;
@@ -1976,19 +1998,19 @@ extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {
   H->hI.count = on ? 1 : 0;
   HIPCHK(hipStreamSynchronize(H->stream));
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(H->hI.blkcnt, 0, 4 * H->nmt_cap * sizeof(uint32_t)));
+  HIPCHK(hipMemset(H->hI.blkcnt, 0, NCNT * H->nmt_cap * sizeof(uint32_t)));
   if (H->solo) H->solo->dirty = true;
   return SFX_OK;
 }
 
-extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[4]) {
+extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[5]) {
   if (!H || !counts) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
   HIPCHK(hipStreamSynchronize(H->stream));
-  std::vector<uint32_t> c(4 * H->nmt_cap);
+  std::vector<uint32_t> c(NCNT * H->nmt_cap);
   HIPCHK(hipMemcpy(c.data(), H->hI.blkcnt, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  for (int k = 0; k < 4; ++k) counts[k] = 0;
-  for (size_t i = 0; i < c.size(); ++i) counts[i % 4] += c[i];
+  for (int k = 0; k < NCNT; ++k) counts[k] = 0;
+  for (size_t i = 0; i < c.size(); ++i) counts[i % NCNT] += c[i];
   return SFX_OK;
 }
 
@@ -2114,24 +2136,24 @@ extern "C" int sfx_get_tuning(sfx_handle* H, int32_t out[8]) {
 
 namespace {
 // Mean duration (ms) of `trials` launches of the phases in `what` on the handle's stream.
-int time_phases(sfx_handle* H, int what, int trials, hipEvent_t e0, hipEvent_t e1, float& mean) {
+// Median launch time of the phases `what` over `trials` launches (after one warm-up).
+int time_phases(sfx_handle* H, int what, int trials, hipEvent_t e0, hipEvent_t e1, float& median) {
   sfx_batch* B = solo(H);
   B->dirty = true;
   int rc = rebuild(B);
   if (rc) return rc;
   launch(B, what, 0);   // warm-up
   HIPCHK(hipStreamSynchronize(H->stream));
-  float tot = 0.f;
-  for (int t = 0; t < trials; ++t) {
+  std::vector<float> t(trials);
+  for (int k = 0; k < trials; ++k) {
     HIPCHK(hipEventRecord(e0, H->stream));
     launch(B, what, 0);
     HIPCHK(hipEventRecord(e1, H->stream));
     HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    tot += ms;
+    HIPCHK(hipEventElapsedTime(&t[k], e0, e1));
   }
-  mean = tot / trials;
+  std::sort(t.begin(), t.end());
+  median = t[trials / 2];
   return SFX_OK;
 }
 

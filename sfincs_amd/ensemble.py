@@ -30,7 +30,7 @@ def _instances(models, device, tune_min_cells):
     for m in models:
         inst = _lib.Instance(m, device)
         if tune_min_cells is not None and _np(m) >= tune_min_cells:
-            inst.autotune(2)
+            inst.autotune(5)
         insts.append(inst)
     return insts
 

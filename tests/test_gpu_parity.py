@@ -285,3 +285,28 @@ def test_regular_point_kernels_match_oracle(shape):
     assert st["nt"] == int(orc.get("nt"))
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
 
+
+
+def test_state_write_over_long_dry_points_matches_oracle():
+    """The per-point kernel stops storing zeros at points that stayed dry for two steps (both
+    ping-pong buffers already hold them). A host write of q over such points must end that:
+    after 900 s of a beach with a tide, q + 1 mm2/s is written into both runs (the oracle's
+    q, so the two start equal) and the run continues to its end on both."""
+    print("case state write over dry points", flush=True)
+    sc = W.regular_case(40, 56, dx=50.0, seed=37, t_end=1800.0, rain_mmh=0.0, tide_amp=1.5)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    inst.set_tuning(-2, (64, 4), (64, 4))
+    inst.update(900.0)
+    orc.update(900.0)
+    assert inst.time()[2] == int(orc.get("nt"))
+    npuv = inst.sizes["npuv"]
+    q = orc.state("q")
+    q[:npuv] += np.float32(1e-3)          # in place: the oracle's own state
+    inst.set("q", q)
+    inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    print(d, flush=True)
+    assert inst.time()[2] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL

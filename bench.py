@@ -34,9 +34,12 @@ PEAK_HBM_GBS = 8000.0
 #   partial (subgrid):     kcuv 1 + zmin 4 + s4 16 + 2 levels 32 + kfuv 2 + mask 1 + q,uv 16 = 72
 MOM_CELL = {True: 20, False: 12}
 MOM_UV = {"copied": 17, "dry": 14, "dry_point": 15, "held": 6, "wet": {True: 44, False: 28}, "partial": 72}
-# Continuity per cell: subgrid kcs 1 + faces 8 + V 16 + zs 16 + zs0 8 + zsderv 4 + zsmax 8
-# + zmin/zmax/volmax 12 + 2 levels 8 = 81; regular kcs 1 + faces 8 + zs 16 + zsmax 8 = 33.
-CONT_CELL = {True: 81, False: 33}
+# Continuity per cell, reads: subgrid kcs 1 + faces 8 + V 8 + zs 8 + zs0 4 + zsderv 4 + zsmax 4
+# + zmin/zmax/volmax 12 + 2 levels 8 = 57; regular kcs 1 + faces 8 + zs 8 + zsmax 4 = 21.
+# Writes: a cell stores a field only when its value changes, counted on the device per
+# field (zs 8, V 8, zs0 4, zsderv 4, zsmax 4).
+CONT_READ = {True: 57, False: 21}
+CONT_WRITE = (8, 8, 4, 4, 4)
 
 
 def traffic_of(kernel):
@@ -49,7 +52,11 @@ def traffic_of(kernel):
         return json.load(f).get("bytes_per_launch", {}).get(kernel)
 
 
-def byte_model(sizes, counts_per_step, point_kernel):
+def cont_bytes(reads_per_cell, n_cells, stores_per_step):
+    return n_cells * reads_per_cell + sum(n * b for n, b in zip(stores_per_step, CONT_WRITE))
+
+
+def byte_model(sizes, counts_per_step, stores_per_step, point_kernel):
     sg = bool(sizes["subgrid"])
     np_ = sizes["np"]
     c0, c1, c2, c3, c4 = counts_per_step
@@ -58,7 +65,7 @@ def byte_model(sizes, counts_per_step, point_kernel):
     dry = MOM_UV["dry_point"] if point_kernel else MOM_UV["dry"]
     mom = np_ * MOM_CELL[sg] + copied * MOM_UV["copied"] + c1 * dry + c2 * MOM_UV["wet"][sg] + \
         c3 * MOM_UV["partial"] + c4 * MOM_UV["held"]
-    cont = np_ * CONT_CELL[sg]
+    cont = cont_bytes(CONT_READ[sg], np_, stores_per_step)
     return mom, cont, dict(copied=copied, dry=c1, dry_zeros_held=c4, wet=c2, partial=c3)
 
 
@@ -100,11 +107,12 @@ def cpu_baseline(sc, model, inst, nsteps):
 
 # Indexed (quadtree / masked) regular meshes, per uv-point by branch: as the dense regular
 # figures plus the two cell indices (8 B) and the flags byte; wet points also read the
-# eight neighbour indices (32 B). Per cell: zs 8 + zb 4. Continuity per cell: kcs 1,
-# four face indices 16, iref 1, zs r/w 16, zsmax r/w 8, four faces 16.
+# eight neighbour indices (32 B). Per cell: zs 8 + zb 4. Continuity per cell, reads: kcs 1,
+# four face indices 16, iref 1, zs 8, zsmax 4, four faces 16; writes zs 8 and zsmax 4 when
+# they change (counted).
 IDX_MOM_CELL = 12
 IDX_MOM_UV = {"copied": 17, "dry": 23, "wet": 69, "partial": 69}
-IDX_CONT_CELL = 58
+IDX_CONT_READ = 46
 
 
 def quadtree(args):
@@ -131,6 +139,7 @@ def quadtree(args):
     inst.set_counting(True)
     inst.profile(args.profile_steps)
     counts = [c / args.profile_steps for c in inst.counters()]
+    stores = [c / args.profile_steps for c in inst.store_counters()]
     inst.set_counting(False)
     ms = inst.profile(args.profile_steps)
     batch.close()
@@ -140,7 +149,7 @@ def quadtree(args):
     copied, dry, wet, part, _ = counts
     mom_b = sizes["np"] * IDX_MOM_CELL + copied * IDX_MOM_UV["copied"] + dry * IDX_MOM_UV["dry"] + \
         (wet + part) * IDX_MOM_UV["wet"] + sizes["ncuv"] * 36
-    cont_b = sizes["np"] * IDX_CONT_CELL
+    cont_b = cont_bytes(IDX_CONT_READ, sizes["np"], stores)
     torch.cuda.synchronize()
     t_a = time.perf_counter()
     batch.advance(args.steps)
@@ -164,6 +173,7 @@ def quadtree(args):
         "continuity": {"achieved_GBs": cont_b / (ms[2] * 1e-3) / 1e9, "bytes_per_launch": cont_b, "launch_ms": ms[2]},
         "forcing_launch_ms": ms[0],
         "branches_per_step": dict(copied=copied, dry=dry, wet=wet, partial=part),
+        "continuity_stores_per_step": dict(zs=stores[0], zsmax=stores[4]),
     }
     print(json.dumps(out))
 
@@ -345,13 +355,14 @@ def main():
     inst.set_counting(True)
     inst.profile(args.profile_steps)
     counts = [c / args.profile_steps for c in inst.counters()]
+    stores = [c / args.profile_steps for c in inst.store_counters()]
     inst.set_counting(False)
     ms = inst.profile(args.profile_steps)
     batch.close()
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     batch.advance(8)
-    mom_b, cont_b, branches = byte_model(sizes, counts, tuning["momentum_kernel"] < 0)
+    mom_b, cont_b, branches = byte_model(sizes, counts, stores, tuning["momentum_kernel"] < 0)
     print(f"# per launch ms forcing/momentum/continuity = {ms}; branches {branches}", file=sys.stderr, flush=True)
 
     # timed region: exactly K steps (graph-captured), barrier + synchronize on both sides
@@ -408,6 +419,7 @@ def main():
         "forcing_launch_ms": ms[0],
         "step_GBs": step_gbs,
         "branches_per_step": branches,
+        "continuity_stores_per_step": dict(zip(("zs", "z_volume", "zs0", "zsderv", "zsmax"), stores)),
     }
     if not args.no_cpu:
         try:

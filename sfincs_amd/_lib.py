@@ -89,6 +89,7 @@ def _sig(lib):
         "sfx_get_time": (I, [P, C.POINTER(d), C.POINTER(f), C.POINTER(C.c_int64)]),
         "sfx_set_counting": (I, [P, i32]),
         "sfx_get_counters": (I, [P, C.POINTER(C.c_int64)]),
+        "sfx_get_store_counters": (I, [P, C.POINTER(C.c_int64)]),
         "sfx_total_volume": (I, [P, C.POINTER(d)]),
         "sfx_batch_create": (I, [i32, C.POINTER(P)]),
         "sfx_batch_destroy": (I, [P]),
@@ -340,6 +341,12 @@ class Instance:
     def counters(self):
         a = (C.c_int64 * 5)()
         check(lib().sfx_get_counters(self.h, a), "get_counters")
+        return list(a)
+
+    def store_counters(self):
+        """Continuity stores since counting was enabled: [zs, z_volume, zs0, zsderv, zsmax]."""
+        a = (C.c_int64 * 5)()
+        check(lib().sfx_get_store_counters(self.h, a), "get_store_counters")
         return list(a)
 
     def total_volume(self):

@@ -85,6 +85,7 @@ struct DevInst {
   // next step reduces the tiles and sets every word back to dtmax.
   float* blkmin;
   uint32_t* blkcnt;              // per tile, NCNT words: uv-points copied (kcuv not 1/6), dry, wet/full, partial, dry with zeros held
+  uint32_t* cntstore;            // per continuity tile, NSTORE words: cells that stored zs, V, zs0, zsderv, zsmax
   // cells
   double* zs;
   double* vol;
@@ -138,6 +139,9 @@ constexpr int8_t KF_ZERO2 = 2, KF_ZERO1 = 4;
 // Per-branch uv-point counters of a momentum tile: copied (kcuv not 1 or 6), dry with zeros
 // stored, wet (regular) or full (subgrid), partial (subgrid), dry with zeros already held.
 constexpr int NCNT = 5;
+// Continuity stores counted per tile (a cell stores a field only when its value changes):
+// zs, V, zs0, zsderv, zsmax.
+constexpr int NSTORE = 5;
 
 struct Item { int32_t inst, tile; };
 
@@ -504,10 +508,11 @@ __device__ __forceinline__ float uv_floor(const DevInst& I, int slot) {
 
 // Completes one active uv-point from its gathered inputs and its preloaded wet/dry
 // threshold `floor` (uv_floor): dry points get q = uv = 0, wet points the full update.
-// The update of a wet uv-point (zsu from uv_wet): its tables, the physics, the stores.
+// The update of a wet uv-point (zsu from uv_wet): its tables, the physics, the stores
+// (kfuv only when `kf_old`, its kfuv byte as loaded, is not already 1; -1: not loaded).
 template <typename R, bool SUBGRID, typename QP>
 __device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, sfx::UvTables& tb, R zsu,
-                                       R dt, QP q1, QP uv1, float& mdt, int& br) {
+                                       R dt, QP q1, QP uv1, float& mdt, int& br, int kf_old = -1) {
   if (SUBGRID) {
     const float4 s4 = ldf4(G(I.uv_s4), slot);
     tb.zmax = s4.x; tb.havg_zmax = s4.y; tb.nrep_zmax = s4.z; tb.navg_w = s4.w; tb.fnfit = G(I.uv_fnfit)[slot];
@@ -517,7 +522,8 @@ __device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::Uv
   const sfx::UvOut<R> o = sfx::uv_update<R, SUBGRID>(in, tb, I.c, zsu,
                                                      sfx::GRow<const float4>{G(I.uv_lev), SUBGRID ? (uint32_t)slot * I.c.nl : 0u},
                                                      G(I.x73), sfx::pow_tables_g(I.pw), dt);
-  q1[slot] = o.q; uv1[slot] = o.uv; G(I.kfuv)[slot] = 1;
+  q1[slot] = o.q; uv1[slot] = o.uv;
+  if (kf_old != 1) G(I.kfuv)[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
   br = o.branch;
 }
@@ -697,7 +703,7 @@ __device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, in
   }
   in.idir = vplane; in.iref = 1; in.itype = 0; in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
   int br = 0;
-  wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br);
+  wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br, kf);
   cnt[br]++;
 }
 
@@ -1078,14 +1084,24 @@ __global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts
 }
 
 // ----------------------------------------------------------------------------- continuity
+// Bitwise equality: a cell stores a value only when its bits change (dry cells without
+// inflow keep every value, so most of the grid writes nothing; storing identical bits
+// would change nothing).
+__device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+
+// Returns the fields it stored, one bit each in NSTORE order.
 template <typename R, bool SUBGRID>
-__device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref, R dt,
-                                          R netprcp) {
+__device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref, R dt,
+                                              R netprcp) {
+  unsigned wrote = 0;
   const uint8_t kraw = G(I.kcs)[c];
   const int kcs = kraw & 0x7f;
   double zs = G(I.zs)[c];
   const auto zsmaxp = G((R*)I.zsmax);
   R zsmax = zsmaxp[c];
+  const double zs_in = zs;
+  const R zsmax_in = zsmax;
   if (SUBGRID) {
     double vol = G(I.vol)[c];
     if (kraw & 0x80) {   // point sources of this cell, in source order (:311-325)
@@ -1098,14 +1114,16 @@ __device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_m
     const auto zs0p = G((R*)I.zs0);
     const auto zsdp = G((R*)I.zsderv);
     R zs0 = zs0p[c], zsd = zsdp[c];
+    const double vol_in = vol;
+    const R zs0_in = zs0, zsd_in = zsd;
     const int nl = I.c.nl;
     sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, G(I.z_zmin)[c],
                          G(I.z_zmax)[c], G(I.z_volmax)[c], sfx::GRow<const float>{G(I.z_dep), (uint32_t)c * nl},
                          I.c, dt);
-    G(I.vol)[c] = vol;
+    if (!same_bits(vol, vol_in)) { G(I.vol)[c] = vol; wrote |= 2u; }
     if (kcs == 1 || kcs == 4) {
-      zs0p[c] = zs0;
-      zsdp[c] = zsd;
+      if (!same_bits(zs0, zs0_in)) { zs0p[c] = zs0; wrote |= 4u; }
+      if (!same_bits(zsd, zsd_in)) { zsdp[c] = zsd; wrote |= 8u; }
     }
   } else {
     if (kraw & 0x80) {   // (:87-106)
@@ -1117,8 +1135,19 @@ __device__ __forceinline__ void cont_cell(const DevInst& I, int c, R q_md, R q_m
     }
     sfx::cell_regular<R>(zs, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, I.c, dt);
   }
-  G(I.zs)[c] = zs;
-  if (I.c.flags & sfx::F_ZSMAX) zsmaxp[c] = zsmax;
+  if (!same_bits(zs, zs_in)) { G(I.zs)[c] = zs; wrote |= 1u; }
+  if ((I.c.flags & sfx::F_ZSMAX) && !same_bits(zsmax, zsmax_in)) { zsmaxp[c] = zsmax; wrote |= 16u; }
+  return wrote;
+}
+
+// Adds a wave's continuity stores per field to its tile's counters (active lanes only).
+__device__ __forceinline__ void count_stores(const DevInst& I, int tile, unsigned wrote) {
+  const int first = __ffsll((long long)__ballot(1)) - 1;
+  for (int k = 0; k < NSTORE; ++k) {
+    const int n = __popcll(__ballot((wrote >> k) & 1u));
+    if ((int)(threadIdx.x % WAVE) == first && n)
+      atomicAdd((unsigned int*)&G(I.cntstore)[NSTORE * (uint32_t)tile + k], (unsigned int)n);
+  }
 }
 
 template <typename R>
@@ -1144,15 +1173,17 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     const R q_mu = m < mmax - 1 ? q[c] : R(0);
     const R q_nd = n >= 1 ? q[np + c - 1] : R(0);
     const R q_nu = n < nmax - 1 ? q[np + c] : R(0);
-    if (I.subgrid) cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
-    else cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
+    const unsigned w = I.subgrid ? cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp)
+                                 : cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
+    if (I.count) count_stores(I, tile, w);
   } else {
     const int c = tile * BLOCK + threadIdx.x;
     if (c >= I.np) return;
     const R q_md = q[G(I.zmd)[c]], q_mu = q[G(I.zmu)[c]], q_nd = q[G(I.znd)[c]], q_nu = q[G(I.znu)[c]];
     const int iref = G(I.z_iref)[c];
-    if (I.subgrid) cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
-    else cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+    const unsigned w = I.subgrid ? cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp)
+                                 : cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+    if (I.count) count_stores(I, tile, w);
   }
 }
 
@@ -1200,6 +1231,7 @@ struct sfx_handle {
   int mtile_tn = 32, mtile_tm = 8, strip_rows = -1;   // per-point kernel on 32 x 8 tiles
   int xchunk = 64, xchunk_c = 0;
   size_t nmt_cap = 0;            // capacity of blkmin (tiles of the largest tile count)
+  size_t nct_cap = 0;            // continuity tiles of the largest tile count (cntstore)
   int in_batches = 0;            // user batches holding this scenario (tile shape is frozen)
   double* dvol = nullptr;
   template <typename T> T* upload(const T* src, size_t n);
@@ -1531,6 +1563,11 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.blkmin = H->upload(init.data(), init.size());
   }
   I.blkcnt = H->alloc<uint32_t>(NCNT * H->nmt_cap);
+  H->nct_cap = M->dense ? 0 : (size_t)((I.np + BLOCK - 1) / BLOCK);
+  if (M->dense)
+    for (int k = 0; k < N_CELL_SHAPES; ++k)
+      H->nct_cap = std::max(H->nct_cap, cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
+  I.cntstore = H->alloc<uint32_t>(NSTORE * std::max<size_t>(1, H->nct_cap));
   H->dvol = H->alloc<double>(1);
   H->dI = H->upload(&I, 1);
   for (void* p : H->allocs) if (!p) { set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
@@ -1999,6 +2036,7 @@ extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {
   HIPCHK(hipStreamSynchronize(H->stream));
   HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(H->hI.blkcnt, 0, NCNT * H->nmt_cap * sizeof(uint32_t)));
+  HIPCHK(hipMemset(H->hI.cntstore, 0, NSTORE * std::max<size_t>(1, H->nct_cap) * sizeof(uint32_t)));
   if (H->solo) H->solo->dirty = true;
   return SFX_OK;
 }
@@ -2011,6 +2049,17 @@ extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[5]) {
   HIPCHK(hipMemcpy(c.data(), H->hI.blkcnt, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   for (int k = 0; k < NCNT; ++k) counts[k] = 0;
   for (size_t i = 0; i < c.size(); ++i) counts[i % NCNT] += c[i];
+  return SFX_OK;
+}
+
+extern "C" int sfx_get_store_counters(sfx_handle* H, int64_t counts[5]) {
+  if (!H || !counts) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  HIPCHK(hipStreamSynchronize(H->stream));
+  std::vector<uint32_t> c(NSTORE * std::max<size_t>(1, H->nct_cap));
+  HIPCHK(hipMemcpy(c.data(), H->hI.cntstore, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (int k = 0; k < NSTORE; ++k) counts[k] = 0;
+  for (size_t i = 0; i < c.size(); ++i) counts[i % NSTORE] += c[i];
   return SFX_OK;
 }
 

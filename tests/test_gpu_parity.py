@@ -234,6 +234,8 @@ def test_branch_counters_cover_every_point():
             b.update(0.0)
         per_sweep = 2 * sz["np"] if sz["dense"] else sz["npuv"]
         assert sum(a.counters()) == 25 * per_sweep
+        st = a.store_counters()        # continuity stores only where a value changed
+        assert all(0 <= x <= 25 * sz["np"] for x in st) and st[0] > 0
         assert a.time() == b.time()
         assert np.array_equal(a.get("zs"), b.get("zs"))
 

@@ -1007,7 +1007,9 @@ __device__ __forceinline__ bool work_item(const WorkList& L, int& tile, int& ins
     tile = (int)nat; inst = 0;
     return true;
   }
-  const Item it = L.items[blockIdx.x];
+  // items are stored XCD-major (see xcd_major): the blocks of one XCD read consecutive items
+  const uint32_t b = blockIdx.x;
+  const Item it = L.items[(b & 7u) * (gridDim.x >> 3) + (b >> 3)];
   if (it.inst < 0) return false;
   tile = it.tile; inst = it.inst;
   return true;
@@ -1790,6 +1792,16 @@ void xcd_order(std::vector<Item>& v, int chunk) {
   }
 }
 
+// Storage order of a launch's items: item b (run by block b, on XCD b mod 8) at
+// (b mod 8) n/8 + b/8, so the items of one XCD are contiguous and each XCD's L2 fetches
+// only its own lines. n is a multiple of 8 (xcd_order pads).
+void xcd_major(std::vector<Item>& v) {
+  const size_t n = v.size(), per = n / 8;
+  std::vector<Item> w(n);
+  for (size_t b = 0; b < n; ++b) w[(b % 8) * per + b / 8] = v[b];
+  v.swap(w);
+}
+
 int rebuild(sfx_batch* B) {
   HIPCHK(hipSetDevice(B->device));
   const size_t nm = B->members.size();
@@ -1832,12 +1844,12 @@ int rebuild(sfx_batch* B) {
   Src src_m[NLIST_M], src_cu = source(cu, 0), src_cd = source(cd, xchunk_c), src_ci = source(ci, xchunk_c);
   for (int k = 0; k < NLIST_M; ++k) {
     src_m[k] = source(mom[k], xchunk);
-    if (mom[k].size()) xcd_order(mom[k], xchunk);
+    if (mom[k].size()) { xcd_order(mom[k], xchunk); xcd_major(mom[k]); }
     B->off_m[k] = all.size(); all.insert(all.end(), mom[k].begin(), mom[k].end()); B->n_m[k] = mom[k].size();
   }
-  if (cu.size()) xcd_order(cu, 0);
-  if (cd.size()) xcd_order(cd, xchunk_c);
-  if (ci.size()) xcd_order(ci, xchunk_c);
+  if (cu.size()) { xcd_order(cu, 0); xcd_major(cu); }
+  if (cd.size()) { xcd_order(cd, xchunk_c); xcd_major(cd); }
+  if (ci.size()) { xcd_order(ci, xchunk_c); xcd_major(ci); }
   B->off_cuv = all.size(); all.insert(all.end(), cu.begin(), cu.end()); B->n_cuv = cu.size();
   B->off_cd = all.size(); all.insert(all.end(), cd.begin(), cd.end()); B->n_cd = cd.size();
   B->off_ci = all.size(); all.insert(all.end(), ci.begin(), ci.end()); B->n_ci = ci.size();

@@ -106,12 +106,13 @@ def cpu_baseline(sc, model, inst, nsteps):
 
 
 # Indexed (quadtree / masked) regular meshes, per uv-point by branch: as the dense regular
-# figures plus the two cell indices (8 B) and the flags byte; wet points also read the
-# eight neighbour indices (32 B). Per cell: zs 8 + zb 4. Continuity per cell, reads: kcs 1,
+# figures plus the two cell indices (8 B), the flags byte and the kfuv byte read up front
+# (dry with zeros held: kcuv 1, indices 8, threshold 4, kfuv 1 = 14); wet points also read
+# the eight neighbour indices (32 B). Per cell: zs 8 + zb 4. Continuity per cell, reads: kcs 1,
 # four face indices 16, iref 1, zs 8, zsmax 4, four faces 16; writes zs 8 and zsmax 4 when
 # they change (counted).
 IDX_MOM_CELL = 12
-IDX_MOM_UV = {"copied": 17, "dry": 23, "wet": 69, "partial": 69}
+IDX_MOM_UV = {"copied": 17, "dry": 24, "held": 14, "wet": 69, "partial": 69}
 IDX_CONT_READ = 46
 
 
@@ -146,9 +147,9 @@ def quadtree(args):
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     batch.advance(8)
-    copied, dry, wet, part, _ = counts
+    copied, dry, wet, part, held = counts
     mom_b = sizes["np"] * IDX_MOM_CELL + copied * IDX_MOM_UV["copied"] + dry * IDX_MOM_UV["dry"] + \
-        (wet + part) * IDX_MOM_UV["wet"] + sizes["ncuv"] * 36
+        held * IDX_MOM_UV["held"] + (wet + part) * IDX_MOM_UV["wet"] + sizes["ncuv"] * 36
     cont_b = cont_bytes(IDX_CONT_READ, sizes["np"], stores)
     torch.cuda.synchronize()
     t_a = time.perf_counter()
@@ -172,7 +173,7 @@ def quadtree(args):
                      "bytes_per_launch": mom_b, "launch_ms": ms[1]},
         "continuity": {"achieved_GBs": cont_b / (ms[2] * 1e-3) / 1e9, "bytes_per_launch": cont_b, "launch_ms": ms[2]},
         "forcing_launch_ms": ms[0],
-        "branches_per_step": dict(copied=copied, dry=dry, wet=wet, partial=part),
+        "branches_per_step": dict(copied=copied, dry=dry, dry_zeros_held=held, wet=wet, partial=part),
         "continuity_stores_per_step": dict(zs=stores[0], zsmax=stores[4]),
     }
     print(json.dumps(out))

@@ -129,11 +129,12 @@ struct DevInst {
   sfx::PowTables pw;
 };
 
-// Flags of a dry uv-point kept in its kfuv byte by the per-point momentum kernel (bit 0
-// is the reference's kfuv and every reader masks it): KF_ZERO1, this kernel stored q = uv = 0
-// at the point in the last step; KF_ZERO2, it did so in two consecutive steps, which write
-// the two ping-pong buffers, so both hold zeros and a dry point needs no store. Every other
-// write of kfuv (wet points, the other kernels, the host) stores 0 or 1 and clears them.
+// Flags of a dry uv-point kept in its kfuv byte by the per-point and indexed momentum
+// kernels (bit 0 is the reference's kfuv and every reader masks it): KF_ZERO1, the kernel
+// stored q = uv = 0 at the point in the last step; KF_ZERO2, it did so in two consecutive
+// steps, which write the two ping-pong buffers, so both hold zeros and a dry point needs
+// no store. Every other write of kfuv (wet points, the other kernels, the host) stores 0
+// or 1 and clears them.
 constexpr int8_t KF_ZERO2 = 2, KF_ZERO1 = 4;
 
 // Per-branch uv-point counters of a momentum tile: copied (kcuv not 1 or 6), dry with zeros
@@ -919,7 +920,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
   const int npuv = I.npuv;
   if (ip >= npuv) return;
   const auto nb = G(I.nb);
-  const int kc = G(I.kcuv)[ip];
+  const int kc = G(I.kcuv)[ip], kf = G(I.kfuv)[ip];
   const int nm = nb[ip], nmu = nb[npuv + ip];
   sfx::UvIn<R> in;
   in.zs_nm = G(I.zs)[nm]; in.zs_nmu = G(I.zs)[nmu];
@@ -932,8 +933,10 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
     return;
   }
   R zsu;
-  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
-    q1[ip] = R(0); u1[ip] = R(0); G(I.kfuv)[ip] = 0;
+  if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {   // zero flags as in the per-point kernel
+    if (kf & KF_ZERO2) { cnt[4]++; return; }
+    q1[ip] = R(0); u1[ip] = R(0);
+    G(I.kfuv)[ip] = (kf & KF_ZERO1) ? KF_ZERO2 : KF_ZERO1;
     cnt[1]++;
     return;
   }
@@ -955,7 +958,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
     in.uu_nmd = in.uu_nmu = in.uu_ndm = in.uu_num = in.v_ndm = in.v_ndmu = in.v_nm = in.v_nmu = R(0);
     in.kcuv_nmd = in.kcuv_nmu = 0;
   }
-  in.kfuv = G(I.kfuv)[ip] & 1; in.mask_adv = G(I.mask_adv)[ip];
+  in.kfuv = kf & 1; in.mask_adv = G(I.mask_adv)[ip];
   in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
   if (SUBGRID) {
     in.vol_nm = G(I.vol)[nm]; in.vol_nmu = G(I.vol)[nmu];
@@ -964,7 +967,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
     in.zb_nm = G(I.zb)[nm]; in.zb_nmu = G(I.zb)[nmu];
   }
   int br = 0;
-  wet_uv<R, SUBGRID>(I, ip, in, tb, zsu, dt, q1, u1, mdt, br);
+  wet_uv<R, SUBGRID>(I, ip, in, tb, zsu, dt, q1, u1, mdt, br, kf);
   cnt[br]++;
 }
 

@@ -289,16 +289,22 @@ def test_regular_point_kernels_match_oracle(shape):
 
 
 
-def test_state_write_over_long_dry_points_matches_oracle():
-    """The per-point kernel stops storing zeros at points that stayed dry for two steps (both
-    ping-pong buffers already hold them). A host write of q over such points must end that:
-    after 900 s of a beach with a tide, q + 1 mm2/s is written into both runs (the oracle's
-    q, so the two start equal) and the run continues to its end on both."""
-    print("case state write over dry points", flush=True)
-    sc = W.regular_case(40, 56, dx=50.0, seed=37, t_end=1800.0, rain_mmh=0.0, tide_amp=1.5)
+@pytest.mark.parametrize("mesh", ["dense", "quadtree"])
+def test_state_write_over_long_dry_points_matches_oracle(mesh):
+    """The per-point and indexed kernels stop storing zeros at points that stayed dry for
+    two steps (both ping-pong buffers already hold them). A host write of q over such
+    points must end that: after 900 s of a beach with a tide, q + 1 mm2/s is written into
+    both runs (the oracle's q, so the two start equal) and the run continues to its end
+    on both."""
+    print(f"case state write over dry points, {mesh}", flush=True)
+    if mesh == "dense":
+        sc = W.regular_case(40, 56, dx=50.0, seed=37, t_end=1800.0, rain_mmh=0.0, tide_amp=1.5)
+    else:
+        sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=38, t_end=1800.0)
     m, orc = oracle_for(sc)
     inst = gpu_for(m)
-    inst.set_tuning(-2, (64, 4), (64, 4))
+    if mesh == "dense":
+        inst.set_tuning(-2, (64, 4), (64, 4))
     inst.update(900.0)
     orc.update(900.0)
     assert inst.time()[2] == int(orc.get("nt"))

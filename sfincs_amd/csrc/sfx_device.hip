@@ -65,7 +65,21 @@ struct Scalars {
   int32_t itbndlast, itsrclast, itwndlast, itprcplast;
   float tauwu, tauwv, netprcp;   // uniform meteo forcing of this step
   float factime, facrel, one_minus_factime;
+  int32_t ahead;                 // the next step's fold, scalars and series are staged (StepImage)
+  int32_t pad_;
 };
+
+// Step scalars and forcing series of one scenario as a forcing block works on them: the
+// Scalars as they will be after the step's fold and forcing (s), whether the step runs
+// (go), whether the fold consumed a momentum sweep (folded), and the step's position in
+// the boundary series (itb0, itb1, tbfac).
+struct StepImage {
+  Scalars s;
+  int32_t go, folded, itb0, itb1;
+  float tbfac;
+  int32_t pad_;
+};
+static_assert(sizeof(Scalars) % 4 == 0 && sizeof(StepImage) % 4 == 0, "copied as words");
 
 struct DevInst {
   Consts c;
@@ -79,6 +93,8 @@ struct DevInst {
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
   Scalars* sc;
+  StepImage* snext;              // the next step computed ahead (valid while sc->ahead)
+  float* qtsrc_nx;               // its discharges
   // CFL reduction: each wave of a momentum tile lowers blkmin[tile] to the minimum of its
   // points' time steps (an atomic min on the tile's own word, so no two tiles share an
   // address and no wave waits for the others of its block); the fold at the start of the
@@ -270,7 +286,9 @@ __device__ void fold_sweep(const DevInst& I) {
 }
 
 __global__ void __launch_bounds__(FBLOCK) k_fold(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids) {
-  fold_sweep(insts[ids ? ids[blockIdx.x] : blockIdx.x]);
+  const DevInst& I = insts[ids ? ids[blockIdx.x] : blockIdx.x];
+  fold_sweep(I);
+  if (threadIdx.x == 0) G(I.sc)->ahead = 0;   // a staged next step folded this sweep too
 }
 
 // ----------------------------------------------------------------------------- forcing
@@ -283,52 +301,80 @@ __device__ float f_modulo(float a, float p) {
   return r;
 }
 
-template <typename R>
-__global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids,
-                                                   int mode) {
-  const DevInst& I = insts[ids[blockIdx.x]];
-  const auto S = G(I.sc);
-  const Consts& c = I.c;
-  __shared__ int sh_go;
-  __shared__ int sh_itb0, sh_itb1;
-  __shared__ float sh_tbfac;
-  __shared__ double sh_t;
-  __shared__ float sh_dt;
-  fold_sweep(I);
-  // step scalars (sfincs_lib.f90:379-392): one thread
-  if (threadIdx.x == 0) {
-    int go = 1;
-    if (mode == 0) {
-      if (S->error_pending) { S->error = 1; S->error_pending = 0; S->t = (double)c.t1 + 1.0; }
-      go = (S->t < S->tend) && !S->error;
-      S->active = go;
-      if (go) {
-        S->nt += 1;
-        S->dt = c.alpha * S->min_dt;
-        S->dtchk = c.alpha * S->min_dt;
-        S->t = S->t + (double)S->dt;
-        if (S->dtchk < c.dtmin && S->nt > 1) S->error_pending = 1;
-      }
-    } else {
-      S->active = 1;
+// Word-parallel copies of a scenario's scalars and step images (one word per thread).
+template <typename T>
+__device__ __forceinline__ void copy_words(T* dst, const T* src) {
+  constexpr int NW = sizeof(T) / 4;
+  if ((int)threadIdx.x < NW) ((uint32_t*)dst)[threadIdx.x] = ((const uint32_t*)src)[threadIdx.x];
+}
+
+// Fold of the tile minima into the image, leaving the minima in place (a step computed
+// ahead of its turn leaves them to the launch that commits it, which resets them, and to
+// a host-side fold, which re-reads them).
+template <int NT>
+__device__ void fold_image(const DevInst& I, StepImage& F) {
+  if (F.s.mom_done) {
+    float v = I.c.dtmax;
+    const auto p = G(I.blkmin);
+    const int n = I.nmt;
+    for (int base = 0; base < n; base += 16 * NT) {
+      float w[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) w[j] = p[min(base + (int)threadIdx.x + j * NT, n - 1)];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v = fminf(v, w[j]);
     }
-    sh_go = go;
-    sh_t = S->t;
-    sh_dt = S->dt;
+    v = block_min<NT>(v);
+    if (threadIdx.x == 0) { F.s.cur ^= 1; F.s.mom_done = 0; F.s.min_dt = v; F.folded = 1; }
+  } else if (threadIdx.x == 0) {
+    F.folded = 0;
   }
   __syncthreads();
-  if (!sh_go) return;
-  // the forcing series at the new time, independent of each other: one wave each
-  const double t = sh_t;
-  const float dt = sh_dt;
+}
+
+__device__ void reset_tiles(const DevInst& I) {
+  const auto p = G(I.blkmin);
+  for (int i = threadIdx.x; i < I.nmt; i += blockDim.x) p[i] = I.c.dtmax;
+}
+
+// The step scalars (sfincs_lib.f90:379-392) and the forcing series at the new time
+// (meteo, boundary series, discharges), on the image F after its fold; discharges go to
+// qout. The series are skipped when the step does not run (F.go = 0).
+__device__ void step_series(const DevInst& I, StepImage& F, float* qout, int mode) {
+  const Consts& c = I.c;
+  if (threadIdx.x == 0) {
+    Scalars& sc = F.s;
+    int go = 1;
+    if (mode == 0) {
+      if (sc.error_pending) { sc.error = 1; sc.error_pending = 0; sc.t = (double)c.t1 + 1.0; }
+      go = (sc.t < sc.tend) && !sc.error;
+      sc.active = go;
+      if (go) {
+        sc.nt += 1;
+        sc.dt = c.alpha * sc.min_dt;
+        sc.dtchk = c.alpha * sc.min_dt;
+        sc.t = sc.t + (double)sc.dt;
+        if (sc.dtchk < c.dtmin && sc.nt > 1) sc.error_pending = 1;
+      }
+    } else {
+      sc.active = 1;
+    }
+    F.go = go;
+  }
+  __syncthreads();
+  if (!F.go) return;
+  // the forcing series at the new time, independent of each other: wave 0 time factors and
+  // discharges, wave 1 wind, wave 2 rain, wave 3 boundary series (any block of 4+ waves)
+  const double t = F.s.t;
+  const float dt = F.s.dt;
   const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   if (wave == 0 && lane == 0) {
-    S->factime = sfx::vmin<float>(dt / c.btfilter, 1.0f);
-    S->one_minus_factime = 1.0f - S->factime;
-    S->facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
+    F.s.factime = sfx::vmin<float>(dt / c.btfilter, 1.0f);
+    F.s.one_minus_factime = 1.0f - F.s.factime;
+    F.s.facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
   }
   if (wave == 1 && lane == 0 && (c.flags & sfx::F_WIND)) {
-    for (int itw = max(S->itwndlast, 2); itw <= I.ntwnd; ++itw) {
+    for (int itw = max(F.s.itwndlast, 2); itw <= I.ntwnd; ++itw) {
       if ((double)G(I.twnd)[itw - 1] > t) {
         const float twfac = (float)((t - (double)G(I.twnd)[itw - 2]) / (double)(G(I.twnd)[itw - 1] - G(I.twnd)[itw - 2]));
         const float vmag = G(I.wndmag)[itw - 2] * (1.0f - twfac) + G(I.wndmag)[itw - 1] * twfac;
@@ -338,22 +384,22 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
         else if (dr0 > dr1 + c.pi) dr1 = dr1 + 2 * c.pi;
         const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
         const float cd = G(I.cdval)[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
-        S->tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
-        S->tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
-        S->itwndlast = itw - 1;
+        F.s.tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
+        F.s.tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
+        F.s.itwndlast = itw - 1;
         break;
       }
     }
   }
   if (wave == 2 && lane == 0 && (c.flags & sfx::F_PRECIP)) {
     int itp;
-    for (itp = S->itprcplast; itp <= I.ntprcp; ++itp)
+    for (itp = F.s.itprcplast; itp <= I.ntprcp; ++itp)
       if ((double)G(I.tprcpt)[itp - 1] > t) break;
     if (itp > I.ntprcp) itp = I.ntprcp;
     if (itp < 2) itp = 2;   // series start at or before t0 (checked at initialisation)
     const float twfac = (float)((t - (double)G(I.tprcpt)[itp - 2]) / (double)(G(I.tprcpt)[itp - 1] - G(I.tprcpt)[itp - 2]));
-    S->netprcp = (G(I.tprcpv)[itp - 2] * (1.0f - twfac) + G(I.tprcpv)[itp - 1] * twfac) / (float)(1000 * 3600);
-    S->itprcplast = itp - 1;
+    F.s.netprcp = (G(I.tprcpv)[itp - 2] * (1.0f - twfac) + G(I.tprcpv)[itp - 1] * twfac) / (float)(1000 * 3600);
+    F.s.itprcplast = itp - 1;
   }
   if (wave == 3 && lane == 0 && I.nbnd > 0) {
     int itb0 = 1, itb1 = 1;
@@ -362,29 +408,82 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     if ((double)G(I.t_bnd)[0] > t - (double)1.0e-3f) { itb0 = 1; itb1 = 1; tb = G(I.t_bnd)[0]; }
     else if ((double)G(I.t_bnd)[nt - 1] < t + (double)1.0e-3f) { itb0 = nt; itb1 = nt; tb = G(I.t_bnd)[nt - 1]; }
     else {
-      for (int itb = S->itbndlast; itb <= nt; ++itb)
-        if ((double)G(I.t_bnd)[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; S->itbndlast = itb - 1; break; }
+      for (int itb = F.s.itbndlast; itb <= nt; ++itb)
+        if ((double)G(I.t_bnd)[itb - 1] > t + (double)1.0e-6f) { itb0 = itb - 1; itb1 = itb; tb = (float)t; F.s.itbndlast = itb - 1; break; }
     }
-    sh_itb0 = itb0; sh_itb1 = itb1;
-    sh_tbfac = (tb - G(I.t_bnd)[itb0 - 1]) / sfx::vmax<float>(G(I.t_bnd)[itb1 - 1] - G(I.t_bnd)[itb0 - 1], 1.0e-6f);
+    F.itb0 = itb0; F.itb1 = itb1;
+    F.tbfac = (tb - G(I.t_bnd)[itb0 - 1]) / sfx::vmax<float>(G(I.t_bnd)[itb1 - 1] - G(I.t_bnd)[itb0 - 1], 1.0e-6f);
   }
-  if (wave == 4 && I.nsrc > 0) {   // discharges: the series position once, the sources one lane each
+  if (wave == 0 && I.nsrc > 0) {   // discharges: the series position once, the sources one lane each
     int itsel = 0;
     if (lane == 0) {
-      for (int it = max(S->itsrclast, 2); it <= I.ntsrc; ++it)
-        if ((double)G(I.tsrc)[it - 1] > t) { itsel = it; S->itsrclast = it - 1; break; }
+      for (int it = max(F.s.itsrclast, 2); it <= I.ntsrc; ++it)
+        if ((double)G(I.tsrc)[it - 1] > t) { itsel = it; F.s.itsrclast = it - 1; break; }
     }
     itsel = __shfl(itsel, 0, WAVE);
     if (itsel) {
       const int it = itsel;
       for (int src = lane; src < I.nsrc; src += WAVE) {
         const float q0v = G(I.qsrc)[(it - 2) * I.nsrc + src], q1v = G(I.qsrc)[(it - 1) * I.nsrc + src];
-        G(I.qtsrc)[src] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)G(I.tsrc)[it - 2]) /
+        G(qout)[src] = (float)((double)q0v + (double)(q1v - q0v) * (t - (double)G(I.tsrc)[it - 2]) /
                                                   (double)(G(I.tsrc)[it - 1] - G(I.tsrc)[it - 2]));
       }
     }
   }
   __syncthreads();
+}
+
+// The fold, step scalars and forcing series of the next step, computed while the current
+// step's continuity runs (they depend on the momentum sweep, not on the water levels):
+// one BLOCK-thread block of the continuity launch per scenario, staged in snext /
+// qtsrc_nx with sc->ahead set. No other step kernel reads these, so the continuity
+// blocks beside it are undisturbed. The next k_forcing commits them; a host write of the
+// scalars, a new momentum sweep or a host-side fold clears the mark, and that k_forcing
+// then computes the step itself.
+__device__ void forcing_ahead_block(const DevInst& I) {
+  __shared__ StepImage F;
+  copy_words(&F.s, I.sc);
+  __syncthreads();
+  if (F.s.ahead) return;   // staged already (nothing has run since)
+  fold_image<BLOCK>(I, F);
+  step_series(I, F, I.qtsrc_nx, 0);
+  copy_words(I.snext, &F);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) G(I.sc)->ahead = 1;
+}
+
+// One block per scenario: the step's fold, scalars and forcing series (committed from
+// k_forcing_ahead, or computed here), then boundary points, conditions and fluxes
+// (sfincs_lib.f90:379-550 with sfincs_meteo.f90:1423-1558, sfincs_boundaries.f90:682-1122,
+// sfincs_discharges.f90:353-369).
+template <typename R>
+__global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ insts, const int32_t* __restrict__ ids,
+                                                   int mode) {
+  const DevInst& I = insts[ids[blockIdx.x]];
+  const auto S = G(I.sc);
+  const Consts& c = I.c;
+  __shared__ StepImage F;
+  __shared__ int sh_ahead;
+  if (threadIdx.x == 0) sh_ahead = mode == 0 && S->ahead;
+  __syncthreads();
+  if (sh_ahead) {
+    copy_words(&F, I.snext);
+    __syncthreads();
+    if (F.go)
+      for (int k = threadIdx.x; k < I.nsrc; k += FBLOCK) G(I.qtsrc)[k] = G(I.qtsrc_nx)[k];
+    if (F.folded) reset_tiles(I);
+  } else {
+    fold_sweep(I);
+    copy_words(&F.s, I.sc);
+    __syncthreads();
+    step_series(I, F, I.qtsrc, mode);
+  }
+  if (threadIdx.x == 0) F.s.ahead = 0;
+  __syncthreads();
+  copy_words(I.sc, &F.s);
+  if (!F.go) return;
+  const double t = F.s.t;
   // Boundary points, conditions and fluxes (:728-1117) in two passes. A flux point evaluates
   // the level of its water-level boundary cell itself (update_boundary_points and the kcs 2
   // branch of update_boundary_conditions, the same arithmetic), so only the Neumann copies
@@ -397,14 +496,14 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   }
   if (I.bnd_overlap) __syncthreads();
   // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
-  const int curb = __builtin_amdgcn_readfirstlane(S->cur);
+  const int curb = __builtin_amdgcn_readfirstlane(F.s.cur);
   const auto q = G((R*)I.q[curb]);
   const auto uv = G((R*)I.uv[curb]);
   const auto uvmean = G((R*)I.uvmean);
   const auto zsmax = G((R*)I.zsmax);
   const int nl = c.nl;
-  const int itb0 = sh_itb0, itb1 = sh_itb1;
-  const float tbfac = sh_tbfac;
+  const int itb0 = F.itb0, itb1 = F.itb1;
+  const float tbfac = F.tbfac;
   for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
     const int nmb = G(I.k2_nmb)[ib];
     const int kb = G(I.kcs)[nmb] & 0x7f;
@@ -493,7 +592,7 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
       q[ip] = qb;
       uv[ip] = sfx::vmax<R>(sfx::vmin<R>(qb / hnmb, R(4)), R(-4));
     }
-    if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)S->factime * q[ip] + (R)S->facrel * (R)S->one_minus_factime * uvmean[ib];
+    if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)F.s.factime * q[ip] + (R)F.s.facrel * (R)F.s.one_minus_factime * uvmean[ib];
     else uvmean[ib] = R(0);
     G(I.zs)[nmb] = (double)zsb1;
     if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], (double)zsb1);
@@ -991,7 +1090,7 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
       cw[k] = v;
     }
   if (threadIdx.x % WAVE == 0) {
-    if (tile == 0 && threadIdx.x == 0) G(I.sc)->mom_done = 1;
+    if (tile == 0 && threadIdx.x == 0) { G(I.sc)->mom_done = 1; G(I.sc)->ahead = 0; }   // a new sweep: no staged step
     if (I.count)
       for (int k = 0; k < NCNT; ++k)
         if (cw[k]) atomicAdd((unsigned int*)&G(I.blkcnt)[NCNT * (uint32_t)tile + k], (unsigned int)cw[k]);
@@ -1000,19 +1099,19 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
 
 // The (scenario, tile) of this block; false for the padding of a list. The scenario is an
 // index into the launch's descriptor argument (0 for a direct list).
-__device__ __forceinline__ bool work_item(const WorkList& L, int& tile, int& inst) {
+// b: the block's index in the list, grid: the list's (padded) length.
+__device__ __forceinline__ bool work_item(const WorkList& L, uint32_t b, uint32_t grid, int& tile, int& inst) {
   if (L.direct) {
-    const uint32_t b = blockIdx.x, x = b & 7u, k = b >> 3;
+    const uint32_t x = b & 7u, k = b >> 3;
     uint32_t nat;
-    if (L.run_sh < 0) nat = x * (gridDim.x >> 3) + k;
+    if (L.run_sh < 0) nat = x * (grid >> 3) + k;
     else nat = ((((k >> L.run_sh) << 3) + x) << L.run_sh) + (k & ((1u << L.run_sh) - 1u));
     if (nat >= (uint32_t)L.n) return false;
     tile = (int)nat; inst = 0;
     return true;
   }
   // items are stored XCD-major (see xcd_major): the blocks of one XCD read consecutive items
-  const uint32_t b = blockIdx.x;
-  const Item it = L.items[(b & 7u) * (gridDim.x >> 3) + (b >> 3)];
+  const Item it = L.items[(b & 7u) * (grid >> 3) + (b >> 3)];
   if (it.inst < 0) return false;
   tile = it.tile; inst = it.inst;
   return true;
@@ -1022,7 +1121,7 @@ __device__ __forceinline__ bool work_item(const WorkList& L, int& tile, int& ins
 template <typename R, bool SUBGRID>
 __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ insts, WorkList L) {
   int tile, inst;
-  if (!work_item(L, tile, inst)) return;
+  if (!work_item(L, blockIdx.x, gridDim.x, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
@@ -1035,7 +1134,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ 
 template <typename R, bool SUBGRID, int P>
 __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__ insts, WorkList L) {
   int tile, inst;
-  if (!work_item(L, tile, inst)) return;
+  if (!work_item(L, blockIdx.x, gridDim.x, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   float mdt = I.c.dtmax;
@@ -1047,7 +1146,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__
 template <typename R, bool SUBGRID>
 __global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__ insts, WorkList L) {
   int tile, inst;
-  if (!work_item(L, tile, inst)) return;
+  if (!work_item(L, blockIdx.x, gridDim.x, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
@@ -1060,7 +1159,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__
 template <typename R, bool SUBGRID>
 __global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ insts, WorkList L) {
   int tile, inst;
-  if (!work_item(L, tile, inst)) return;
+  if (!work_item(L, blockIdx.x, gridDim.x, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
@@ -1074,7 +1173,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ i
 template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts, WorkList L) {
   int tile, inst;
-  if (!work_item(L, tile, inst)) return;
+  if (!work_item(L, blockIdx.x, gridDim.x, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
@@ -1155,10 +1254,19 @@ __device__ __forceinline__ void count_stores(const DevInst& I, int tile, unsigne
   }
 }
 
+// The first `nahead` blocks (padded to a multiple of 8, so the cells keep their XCDs)
+// compute the next step of scenario ahead_ids[b] of ahead_insts (forcing_ahead_block).
 template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts, WorkList L, int indexed) {
+__global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts, WorkList L, int indexed,
+                                                      int nahead, const DevInst* __restrict__ ahead_insts,
+                                                      const int32_t* __restrict__ ahead_ids) {
+  const uint32_t lead = (uint32_t)(nahead + 7) / 8 * 8;
+  if (blockIdx.x < lead) {
+    if ((int)blockIdx.x < nahead) forcing_ahead_block(ahead_insts[ahead_ids[blockIdx.x]]);
+    return;
+  }
   int tile, inst;
-  if (!work_item(L, tile, inst)) return;
+  if (!work_item(L, blockIdx.x - lead, gridDim.x - lead, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   const auto S = G(Sp);
@@ -1532,6 +1640,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     for (int s = 0; s < M->nsrc; ++s) sc[s] = M->nmindsrc[s] - 1;
     I.src_cell = H->upload(sc.data(), sc.size()); I.tsrc = H->upload(M->tsrc.data(), M->tsrc.size());
     I.qsrc = H->upload(M->qsrc.data(), M->qsrc.size()); I.qtsrc = H->alloc<float>(std::max(1, M->nsrc));
+    I.qtsrc_nx = H->alloc<float>(std::max(1, M->nsrc)); I.snext = H->alloc<StepImage>(1);
     I.tprcpt = H->upload(M->tprcpt.data(), M->tprcpt.size()); I.tprcpv = H->upload(M->tprcpv.data(), M->tprcpv.size());
     I.twnd = H->upload(M->twnd.data(), M->twnd.size()); I.wndmag = H->upload(M->wndmag.data(), M->wndmag.size());
     I.wnddir = H->upload(M->wnddir.data(), M->wnddir.size()); I.cdval = H->upload(M->cdval.data(), M->cdval.size());
@@ -1603,9 +1712,12 @@ int read_scalars(sfx_handle* H, Scalars& s) {
   HIPCHK(hipMemcpy(&s, H->dS, sizeof(Scalars), hipMemcpyDeviceToHost));
   return SFX_OK;
 }
+// Host write of the step scalars; it also drops a next step computed ahead on the old ones.
 int write_scalars(sfx_handle* H, const Scalars& s) {
   HIPCHK(hipStreamSynchronize(H->stream));
-  HIPCHK(hipMemcpy(H->dS, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+  Scalars w = s;
+  w.ahead = 0;
+  HIPCHK(hipMemcpy(H->dS, &w, sizeof(Scalars), hipMemcpyHostToDevice));
   return SFX_OK;
 }
 // Completes a pending momentum sweep (fold of the tile minima, buffer flip) so that
@@ -1918,9 +2030,19 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
     if (B->n_m[11]) hipLaunchKernelGGL((k_mom_point<R, false, 4>), dim3(B->n_m[11]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[11]), B->wl_m[11]);
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, base_of(B, B->wl_cuv), B->wl_cuv);
   }
+  // A whole step: the next step's fold, scalars and series (forcing_ahead_block) run as
+  // the leading blocks of this step's first continuity launch, beside its cells.
+  const int nahead = ((what & 6) == 6 && forcing_mode == 0) ? nm : 0;
+  const int lead = (nahead + 7) / 8 * 8;   // keeps block b of the cells on XCD b mod 8
   if (what & 4) {
-    if (B->n_cd) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0);
-    if (B->n_ci) hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci), dim3(BLOCK), 0, st, base_of(B, B->wl_ci), B->wl_ci, 1);
+    if (B->n_cd)
+      hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd + lead), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0,
+                         nahead, B->dInsts, B->dIds);
+    if (B->n_ci) {
+      const int la = B->n_cd ? 0 : nahead, ll = B->n_cd ? 0 : lead;
+      hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci + ll), dim3(BLOCK), 0, st, base_of(B, B->wl_ci), B->wl_ci, 1,
+                         la, B->dInsts, B->dIds);
+    }
   }
 }
 

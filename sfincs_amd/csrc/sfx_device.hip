@@ -108,7 +108,7 @@ struct DevInst {
   void* zs0;
   void* zsderv;
   void* zsmax;
-  const uint8_t* kcs;            // kcs | 0x80 when the cell holds a point source
+  uint8_t* kcs;                  // kcs (KCS_MASK) | KCS_SOURCE (point source) | KCS_STEADY (continuity)
   const float* zb;
   const int8_t* z_iref;
   const int32_t* zmd; const int32_t* zmu; const int32_t* znd; const int32_t* znu;
@@ -152,6 +152,10 @@ struct DevInst {
 // no store. Every other write of kfuv (wet points, the other kernels, the host) stores 0
 // or 1 and clears them.
 constexpr int8_t KF_ZERO2 = 2, KF_ZERO1 = 4;
+
+// The cell byte: kcs in the low bits, KCS_SOURCE when the cell holds a point source,
+// KCS_STEADY kept by continuity (see cont_update).
+constexpr uint8_t KCS_MASK = 0x3f, KCS_STEADY = 0x40, KCS_SOURCE = 0x80;
 
 // Per-branch uv-point counters of a momentum tile: copied (kcuv not 1 or 6), dry with zeros
 // stored, wet (regular) or full (subgrid), partial (subgrid), dry with zeros already held.
@@ -492,7 +496,7 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   // on the host).
   for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
     const int nmb = G(I.gb_cell)[ib];
-    if ((G(I.kcs)[nmb] & 0x7f) == 6) G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]];
+    if ((G(I.kcs)[nmb] & KCS_MASK) == 6) G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]];
   }
   if (I.bnd_overlap) __syncthreads();
   // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
@@ -506,7 +510,7 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   const float tbfac = F.tbfac;
   for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
     const int nmb = G(I.k2_nmb)[ib];
-    const int kb = G(I.kcs)[nmb] & 0x7f;
+    const int kb = G(I.kcs)[nmb] & KCS_MASK;
     if (kb == 6) continue;
     const int indb = G(I.k2_ib)[ib];
     const int nmi = G(I.k2_nmi)[ib];
@@ -1196,11 +1200,10 @@ __device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_
 
 // Returns the fields it stored, one bit each in NSTORE order.
 template <typename R, bool SUBGRID>
-__device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref, R dt,
-                                              R netprcp) {
+__device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t kraw, R q_md, R q_mu, R q_nd, R q_nu,
+                                              int iref, R dt, R netprcp) {
   unsigned wrote = 0;
-  const uint8_t kraw = G(I.kcs)[c];
-  const int kcs = kraw & 0x7f;
+  const int kcs = kraw & KCS_MASK;
   double zs = G(I.zs)[c];
   const auto zsmaxp = G((R*)I.zsmax);
   R zsmax = zsmaxp[c];
@@ -1208,7 +1211,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, R q_md, R
   const R zsmax_in = zsmax;
   if (SUBGRID) {
     double vol = G(I.vol)[c];
-    if (kraw & 0x80) {   // point sources of this cell, in source order (:311-325)
+    if (kraw & KCS_SOURCE) {   // point sources of this cell, in source order (:311-325)
       for (int s = 0; s < I.nsrc; ++s)
         if (G(I.src_cell)[s] == c) {
           const float qs = G(I.qtsrc)[s];
@@ -1230,7 +1233,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, R q_md, R
       if (!same_bits(zsd, zsd_in)) { zsdp[c] = zsd; wrote |= 8u; }
     }
   } else {
-    if (kraw & 0x80) {   // (:87-106)
+    if (kraw & KCS_SOURCE) {   // (:87-106)
       for (int s = 0; s < I.nsrc; ++s)
         if (G(I.src_cell)[s] == c) {
           const R inc = (R)G(I.qtsrc)[s] * dt / (R)I.c.cell_area[iref - 1];
@@ -1242,6 +1245,27 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, R q_md, R
   if (!same_bits(zs, zs_in)) { G(I.zs)[c] = zs; wrote |= 1u; }
   if ((I.c.flags & sfx::F_ZSMAX) && !same_bits(zsmax, zsmax_in)) { zsmaxp[c] = zsmax; wrote |= 16u; }
   return wrote;
+}
+
+// One cell of continuity with its steady bit. A cell is quiet when it is an ordinary cell
+// (kcs 1, no point source), its four face fluxes are zero and no rain falls: its update
+// then depends on its own state only (not on dt). A quiet cell whose last update, also
+// quiet, changed nothing (KCS_STEADY) would change nothing again, so it is skipped after
+// reading its byte and faces; any other cell is updated and its bit set or cleared.
+// Boundary cells, whose levels forcing sets, are never steady; a host write of cell
+// state clears every bit.
+template <typename R, bool SUBGRID>
+__device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref,
+                                                R dt, R netprcp, bool dry_sky) {
+  const uint8_t kraw = G(I.kcs)[c];
+  const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
+                     q_nd == R(0) && q_nu == R(0);
+  if (quiet && (kraw & KCS_STEADY)) return 0u;
+  const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+  const bool steady = quiet && w == 0u;
+  if (steady != ((kraw & KCS_STEADY) != 0))
+    G(I.kcs)[c] = (uint8_t)((kraw & ~KCS_STEADY) | (steady ? KCS_STEADY : 0));
+  return w;
 }
 
 // Adds a wave's continuity stores per field to its tile's counters (active lanes only).
@@ -1275,6 +1299,7 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   const auto q = G((const R*)I.q[qb]);
   const R dt = (R)S->dt;
   const R netprcp = (R)S->netprcp;
+  const bool dry_sky = !(I.c.flags & sfx::F_PRECIP) || netprcp == R(0);
   if (!indexed) {
     const int nmax = I.nmax, mmax = I.mmax, np = I.np;
     const int tn = tile % I.ntn, tmi = tile / I.ntn;
@@ -1286,16 +1311,16 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     const R q_mu = m < mmax - 1 ? q[c] : R(0);
     const R q_nd = n >= 1 ? q[np + c - 1] : R(0);
     const R q_nu = n < nmax - 1 ? q[np + c] : R(0);
-    const unsigned w = I.subgrid ? cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp)
-                                 : cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp);
+    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky)
+                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky);
     if (I.count) count_stores(I, tile, w);
   } else {
     const int c = tile * BLOCK + threadIdx.x;
     if (c >= I.np) return;
     const R q_md = q[G(I.zmd)[c]], q_mu = q[G(I.zmu)[c]], q_nd = q[G(I.znd)[c]], q_nu = q[G(I.znu)[c]];
     const int iref = G(I.z_iref)[c];
-    const unsigned w = I.subgrid ? cont_cell<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp)
-                                 : cont_cell<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky)
+                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky);
     if (I.count) count_stores(I, tile, w);
   }
 }
@@ -1305,7 +1330,7 @@ __global__ void __launch_bounds__(BLOCK) k_volume(const DevInst* __restrict__ I0
   const DevInst& I = *I0;
   double acc = 0.0;
   for (int c = blockIdx.x * BLOCK + threadIdx.x; c < I.np; c += gridDim.x * BLOCK) {
-    const int k = G(I.kcs)[c] & 0x7f;
+    const int k = G(I.kcs)[c] & KCS_MASK;
     if (k == 0) continue;
     const int iref = I.dense ? 1 : G(I.z_iref)[c];
     if (I.subgrid) acc += G(I.vol)[c];
@@ -1547,7 +1572,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   {
     std::vector<uint8_t> k(np);
     for (int c = 0; c < np; ++c) k[c] = (uint8_t)M->kcs[c];
-    for (int s = 0; s < M->nsrc; ++s) k[M->nmindsrc[s] - 1] |= 0x80;
+    for (int s = 0; s < M->nsrc; ++s) k[M->nmindsrc[s] - 1] |= KCS_SOURCE;
     I.kcs = H->upload(k.data(), np);
   }
   if (!M->subgrid) I.zb = H->upload(M->zb.data(), np);
@@ -1823,6 +1848,15 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
   return SFX_ERR_ARG;
 }
 
+// A host write of cell state: no cell keeps its steady bit.
+int clear_steady(sfx_handle* H) {
+  std::vector<uint8_t> b(H->hI.np);
+  HIPCHK(hipMemcpy(b.data(), H->hI.kcs, b.size(), hipMemcpyDeviceToHost));
+  for (auto& v : b) v = (uint8_t)(v & ~KCS_STEADY);
+  HIPCHK(hipMemcpy(H->hI.kcs, b.data(), b.size(), hipMemcpyHostToDevice));
+  return SFX_OK;
+}
+
 extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, size_t nbytes) {
   if (!H || !name || !src) { set_error("sfx_set_state: null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
@@ -1832,10 +1866,15 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
   const std::string k = name;
   const size_t np = H->hI.np;
   DevInst& I = H->hI;
-  if (k == "zs") { if (nbytes != np * 8) { set_error("size mismatch"); return SFX_ERR_ARG; } HIPCHK(hipMemcpy(I.zs, src, nbytes, hipMemcpyHostToDevice)); return SFX_OK; }
+  if (k == "zs") {
+    if (nbytes != np * 8) { set_error("size mismatch"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(I.zs, src, nbytes, hipMemcpyHostToDevice));
+    return clear_steady(H);
+  }
   if (k == "z_volume") {
     if (!I.vol || nbytes != np * 8) { set_error("no z_volume / size mismatch"); return SFX_ERR_ARG; }
-    HIPCHK(hipMemcpy(I.vol, src, nbytes, hipMemcpyHostToDevice)); return SFX_OK;
+    HIPCHK(hipMemcpy(I.vol, src, nbytes, hipMemcpyHostToDevice));
+    return clear_steady(H);
   }
   if (k == "q" || k == "uv") {
     void* dst = k == "q" ? I.q[s.cur] : I.uv[s.cur];
@@ -1859,7 +1898,9 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
   }
   if (k == "zs0" || k == "zsderv" || k == "zsmax") {
     void* dst = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
-    return H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
+    rc = H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
+    if (rc) return rc;
+    return clear_steady(H);
   }
   set_error("unknown state field " + k);
   return SFX_ERR_ARG;

@@ -289,21 +289,24 @@ def test_regular_point_kernels_match_oracle(shape):
 
 
 
-@pytest.mark.parametrize("mesh", ["dense", "quadtree"])
+@pytest.mark.parametrize("mesh", ["dense", "quadtree", "subgrid"])
 def test_state_write_over_long_dry_points_matches_oracle(mesh):
-    """The per-point and indexed kernels stop storing zeros at points that stayed dry for
-    two steps (both ping-pong buffers already hold them). A host write of q over such
-    points must end that: after 900 s of a beach with a tide, q + 1 mm2/s is written into
-    both runs (the oracle's q, so the two start equal) and the run continues to its end
-    on both."""
-    print(f"case state write over dry points, {mesh}", flush=True)
+    """Momentum stops storing zeros at points that stayed dry for two steps (both ping-pong
+    buffers hold them) and continuity skips cells that stayed unchanged without inflow.
+    A host write of state must end both: after 900 s of a beach with a tide, q + 1 mm2/s
+    and zs + 1 mm are written into both runs (the oracle's state, so the two start equal;
+    a skipped cell would miss the zsmax rise of a regular cell and the level a subgrid cell
+    takes back from its volume) and the run continues to its end on both."""
+    print(f"case state write over dry points and steady cells, {mesh}", flush=True)
     if mesh == "dense":
         sc = W.regular_case(40, 56, dx=50.0, seed=37, t_end=1800.0, rain_mmh=0.0, tide_amp=1.5)
-    else:
+    elif mesh == "quadtree":
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=38, t_end=1800.0)
+    else:
+        sc = W.subgrid_case(36, 44, dx=25.0, seed=39, t_end=1800.0, nsrc=1, rain_mmh=0.0)
     m, orc = oracle_for(sc)
     inst = gpu_for(m)
-    if mesh == "dense":
+    if mesh != "quadtree":
         inst.set_tuning(-2, (64, 4), (64, 4))
     inst.update(900.0)
     orc.update(900.0)
@@ -312,6 +315,9 @@ def test_state_write_over_long_dry_points_matches_oracle(mesh):
     q = orc.state("q")
     q[:npuv] += np.float32(1e-3)          # in place: the oracle's own state
     inst.set("q", q)
+    zs = orc.state("zs")
+    zs += 1e-3
+    inst.set("zs", zs)
     inst.update(-1000.0)
     orc.update(sc.t_end)
     d = compare_state(inst, orc)

@@ -37,9 +37,11 @@ MOM_UV = {"copied": 17, "dry": 14, "dry_point": 15, "held": 6, "wet": {True: 44,
 # Continuity per cell, reads: subgrid kcs 1 + faces 8 + V 8 + zs 8 + zs0 4 + zsderv 4 + zsmax 4
 # + zmin/zmax/volmax 12 + 2 levels 8 = 57; regular kcs 1 + faces 8 + zs 8 + zsmax 4 = 21.
 # Writes: a cell stores a field only when its value changes, counted on the device per
-# field (zs 8, V 8, zs0 4, zsderv 4, zsmax 4).
+# field (zs 8, V 8, zs0 4, zsderv 4, zsmax 4). Steady cells (counted) read their kcs byte
+# and faces only: 9.
 CONT_READ = {True: 57, False: 21}
 CONT_WRITE = (8, 8, 4, 4, 4)
+CONT_STEADY = 9
 
 
 def traffic_of(kernel):
@@ -53,7 +55,9 @@ def traffic_of(kernel):
 
 
 def cont_bytes(reads_per_cell, n_cells, stores_per_step):
-    return n_cells * reads_per_cell + sum(n * b for n, b in zip(stores_per_step, CONT_WRITE))
+    steady = stores_per_step[5]
+    return (n_cells - steady) * reads_per_cell + steady * CONT_STEADY + \
+        sum(n * b for n, b in zip(stores_per_step[:5], CONT_WRITE))
 
 
 def byte_model(sizes, counts_per_step, stores_per_step, point_kernel):
@@ -174,7 +178,7 @@ def quadtree(args):
         "continuity": {"achieved_GBs": cont_b / (ms[2] * 1e-3) / 1e9, "bytes_per_launch": cont_b, "launch_ms": ms[2]},
         "forcing_launch_ms": ms[0],
         "branches_per_step": dict(copied=copied, dry=dry, dry_zeros_held=held, wet=wet, partial=part),
-        "continuity_stores_per_step": dict(zs=stores[0], zsmax=stores[4]),
+        "continuity_stores_per_step": dict(zs=stores[0], zsmax=stores[4], steady_skipped=stores[5]),
     }
     print(json.dumps(out))
 
@@ -420,7 +424,7 @@ def main():
         "forcing_launch_ms": ms[0],
         "step_GBs": step_gbs,
         "branches_per_step": branches,
-        "continuity_stores_per_step": dict(zip(("zs", "z_volume", "zs0", "zsderv", "zsmax"), stores)),
+        "continuity_stores_per_step": dict(zip(("zs", "z_volume", "zs0", "zsderv", "zsmax", "steady_skipped"), stores)),
     }
     if not args.no_cpu:
         try:

@@ -181,10 +181,12 @@ int  sfx_get_time(sfx_handle* h, double* t, float* dt, int64_t* nt);
  * already in both q/uv buffers (nothing stored)}; used for the byte model. */
 int  sfx_set_counting(sfx_handle* h, int32_t on);
 int  sfx_get_counters(sfx_handle* h, int64_t counts[5]);
-/* Continuity stores since counting was enabled: cells that stored {zs, z_volume, zs0,
- * zsderv, zsmax}. A cell stores a field only when its value changes (dry cells without
- * inflow store nothing); used for the byte model. No counterpart in the reference. */
-int  sfx_get_store_counters(sfx_handle* h, int64_t counts[5]);
+/* Continuity since counting was enabled: cells that stored {zs, z_volume, zs0, zsderv,
+ * zsmax}, and steady cells skipped. A cell stores a field only when its value changes,
+ * and a cell without inflow, rain or source whose last such update changed nothing is
+ * skipped after reading its faces; used for the byte model. No counterpart in the
+ * reference. */
+int  sfx_get_store_counters(sfx_handle* h, int64_t counts[6]);
 /* FP64 volume accumulator: total water volume (sum of zs*A or V) in double. */
 int  sfx_total_volume(sfx_handle* h, double* vol);
 

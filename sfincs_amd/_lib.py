@@ -344,8 +344,9 @@ class Instance:
         return list(a)
 
     def store_counters(self):
-        """Continuity stores since counting was enabled: [zs, z_volume, zs0, zsderv, zsmax]."""
-        a = (C.c_int64 * 5)()
+        """Continuity since counting was enabled: cells that stored [zs, z_volume, zs0,
+        zsderv, zsmax], and steady cells skipped."""
+        a = (C.c_int64 * 6)()
         check(lib().sfx_get_store_counters(self.h, a), "get_store_counters")
         return list(a)
 

@@ -101,7 +101,7 @@ struct DevInst {
   // next step reduces the tiles and sets every word back to dtmax.
   float* blkmin;
   uint32_t* blkcnt;              // per tile, NCNT words: uv-points copied (kcuv not 1/6), dry, wet/full, partial, dry with zeros held
-  uint32_t* cntstore;            // per continuity tile, NSTORE words: cells that stored zs, V, zs0, zsderv, zsmax
+  uint32_t* cntstore;            // per continuity tile, NSTORE words: cells that stored zs, V, zs0, zsderv, zsmax; steady cells skipped
   // cells
   double* zs;
   double* vol;
@@ -160,9 +160,9 @@ constexpr uint8_t KCS_MASK = 0x3f, KCS_STEADY = 0x40, KCS_SOURCE = 0x80;
 // Per-branch uv-point counters of a momentum tile: copied (kcuv not 1 or 6), dry with zeros
 // stored, wet (regular) or full (subgrid), partial (subgrid), dry with zeros already held.
 constexpr int NCNT = 5;
-// Continuity stores counted per tile (a cell stores a field only when its value changes):
-// zs, V, zs0, zsderv, zsmax.
-constexpr int NSTORE = 5;
+// Continuity counted per tile: cells that stored zs, V, zs0, zsderv, zsmax (a cell stores a
+// field only when its value changes), and steady cells skipped.
+constexpr int NSTORE = 6;
 
 struct Item { int32_t inst, tile; };
 
@@ -1260,7 +1260,7 @@ __device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md,
   const uint8_t kraw = G(I.kcs)[c];
   const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
                      q_nd == R(0) && q_nu == R(0);
-  if (quiet && (kraw & KCS_STEADY)) return 0u;
+  if (quiet && (kraw & KCS_STEADY)) return 32u;   // skipped (counted as such)
   const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
   const bool steady = quiet && w == 0u;
   if (steady != ((kraw & KCS_STEADY) != 0))
@@ -2230,7 +2230,7 @@ extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[5]) {
   return SFX_OK;
 }
 
-extern "C" int sfx_get_store_counters(sfx_handle* H, int64_t counts[5]) {
+extern "C" int sfx_get_store_counters(sfx_handle* H, int64_t counts[6]) {
   if (!H || !counts) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
   HIPCHK(hipStreamSynchronize(H->stream));

@@ -29,11 +29,13 @@ PEAK_HBM_GBS = 8000.0
 #   dry:                   kcuv 1 + zmin/zbuvmx 4 + new q,uv 8 + kfuv 1           = 14
 #     (per-point kernel:   + kfuv read 1                                          = 15)
 #   dry, zeros held:       kcuv 1 + kfuv 1 + zmin/zbuvmx 4 (per-point kernel)     =  6
+#   quiescent tile skipped: its flags, 5 B per tile of 512 points (P = 2)
 #   wet (regular):         kcuv 1 + zbuvmx 4 + gn2uv 4 + kfuv 2 + mask 1 + q,uv 16 = 28
 #   full (subgrid):        kcuv 1 + zmin 4 + s4 16 + fnfit 4 + kfuv 2 + mask 1 + q,uv 16 = 44
 #   partial (subgrid):     kcuv 1 + zmin 4 + s4 16 + 2 levels 32 + kfuv 2 + mask 1 + q,uv 16 = 72
 MOM_CELL = {True: 20, False: 12}
-MOM_UV = {"copied": 17, "dry": 14, "dry_point": 15, "held": 6, "wet": {True: 44, False: 28}, "partial": 72}
+MOM_UV = {"copied": 17, "dry": 14, "dry_point": 15, "held": 6, "tile_skipped": 5 / 512,
+          "wet": {True: 44, False: 28}, "partial": 72}
 # Continuity per cell, reads: subgrid kcs 1 + faces 8 + V 8 + zs 8 + zs0 4 + zsderv 4 + zsmax 4
 # + zmin/zmax/volmax 12 + 2 levels 8 = 57; regular kcs 1 + faces 8 + zs 8 + zsmax 4 = 21.
 # Writes: a cell stores a field only when its value changes, counted on the device per
@@ -63,14 +65,15 @@ def cont_bytes(reads_per_cell, n_cells, stores_per_step):
 def byte_model(sizes, counts_per_step, stores_per_step, point_kernel):
     sg = bool(sizes["subgrid"])
     np_ = sizes["np"]
-    c0, c1, c2, c3, c4 = counts_per_step
+    c0, c1, c2, c3, c4, c5 = counts_per_step
     invalid = 2 * np_ - sizes["npuv"]
     copied = max(0.0, c0 - invalid)
     dry = MOM_UV["dry_point"] if point_kernel else MOM_UV["dry"]
-    mom = np_ * MOM_CELL[sg] + copied * MOM_UV["copied"] + c1 * dry + c2 * MOM_UV["wet"][sg] + \
-        c3 * MOM_UV["partial"] + c4 * MOM_UV["held"]
+    # cells of skipped tiles (two points per cell) are not read either
+    mom = (np_ - c5 / 2) * MOM_CELL[sg] + copied * MOM_UV["copied"] + c1 * dry + c2 * MOM_UV["wet"][sg] + \
+        c3 * MOM_UV["partial"] + c4 * MOM_UV["held"] + c5 * MOM_UV["tile_skipped"]
     cont = cont_bytes(CONT_READ[sg], np_, stores_per_step)
-    return mom, cont, dict(copied=copied, dry=c1, dry_zeros_held=c4, wet=c2, partial=c3)
+    return mom, cont, dict(copied=copied, dry=c1, dry_zeros_held=c4, quiescent_tile=c5, wet=c2, partial=c3)
 
 
 def cpu_baseline(sc, model, inst, nsteps):
@@ -151,7 +154,7 @@ def quadtree(args):
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     batch.advance(8)
-    copied, dry, wet, part, held = counts
+    copied, dry, wet, part, held, _ = counts
     mom_b = sizes["np"] * IDX_MOM_CELL + copied * IDX_MOM_UV["copied"] + dry * IDX_MOM_UV["dry"] + \
         held * IDX_MOM_UV["held"] + (wet + part) * IDX_MOM_UV["wet"] + sizes["ncuv"] * 36
     cont_b = cont_bytes(IDX_CONT_READ, sizes["np"], stores)

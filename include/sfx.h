@@ -178,9 +178,10 @@ int  sfx_get_time(sfx_handle* h, double* t, float* dt, int64_t* nt);
 /* Per-branch uv-point counters of the sweeps since counting was enabled by
  * sfx_set_counting(h,1): {copied (kcuv not 1/6, and the absent slots of a dense grid),
  * dry with zeros stored, wet regular or subgrid full, subgrid partial, dry with the zeros
- * already in both q/uv buffers (nothing stored)}; used for the byte model. */
+ * already in both q/uv buffers (nothing stored), points of quiescent tiles skipped as a
+ * whole}; used for the byte model. */
 int  sfx_set_counting(sfx_handle* h, int32_t on);
-int  sfx_get_counters(sfx_handle* h, int64_t counts[5]);
+int  sfx_get_counters(sfx_handle* h, int64_t counts[6]);
 /* Continuity since counting was enabled: cells that stored {zs, z_volume, zs0, zsderv,
  * zsmax}, and steady cells skipped. A cell stores a field only when its value changes,
  * and a cell without inflow, rain or source whose last such update changed nothing is

@@ -339,7 +339,7 @@ class Instance:
         check(lib().sfx_set_counting(self.h, int(on)), "set_counting")
 
     def counters(self):
-        a = (C.c_int64 * 5)()
+        a = (C.c_int64 * 6)()
         check(lib().sfx_get_counters(self.h, a), "get_counters")
         return list(a)
 

@@ -92,6 +92,13 @@ struct DevInst {
   int32_t xchunk, xchunk_c;      // momentum / continuity work-list runs per XCD (0: one contiguous band per XCD)
   int32_t count;                 // per-branch counting on
   int32_t nmt;                   // momentum tiles of the current tile shape
+  int32_t mrow_sh;               // per-point kernel: log2 of the cell rows of a tile (mtm * P)
+  // Quiescent per-point tiles: mq[tile] holds one byte per wave, 1 when all the wave's points
+  // were dry with their zeros held at the tile's last run; md[tile] is raised when a cell
+  // level the tile reads may have changed since (continuity, forcing; host writes reset
+  // mq). A tile with mq = 0x01010101 and md = 0 would do nothing and is skipped.
+  uint32_t* mq;
+  uint8_t* md;
   Scalars* sc;
   StepImage* snext;              // the next step computed ahead (valid while sc->ahead)
   float* qtsrc_nx;               // its discharges
@@ -100,7 +107,7 @@ struct DevInst {
   // address and no wave waits for the others of its block); the fold at the start of the
   // next step reduces the tiles and sets every word back to dtmax.
   float* blkmin;
-  uint32_t* blkcnt;              // per tile, NCNT words: uv-points copied (kcuv not 1/6), dry, wet/full, partial, dry with zeros held
+  uint32_t* blkcnt;              // per tile, NCNT words: uv-points copied (kcuv not 1/6), dry, wet/full, partial, dry with zeros held, skipped
   uint32_t* cntstore;            // per continuity tile, NSTORE words: cells that stored zs, V, zs0, zsderv, zsmax; steady cells skipped
   // cells
   double* zs;
@@ -158,8 +165,9 @@ constexpr int8_t KF_ZERO2 = 2, KF_ZERO1 = 4;
 constexpr uint8_t KCS_MASK = 0x3f, KCS_STEADY = 0x40, KCS_SOURCE = 0x80;
 
 // Per-branch uv-point counters of a momentum tile: copied (kcuv not 1 or 6), dry with zeros
-// stored, wet (regular) or full (subgrid), partial (subgrid), dry with zeros already held.
-constexpr int NCNT = 5;
+// stored, wet (regular) or full (subgrid), partial (subgrid), dry with zeros already held,
+// points of a quiescent tile that was skipped.
+constexpr int NCNT = 6;
 // Continuity counted per tile: cells that stored zs, V, zs0, zsderv, zsmax (a cell stores a
 // field only when its value changes), and steady cells skipped.
 constexpr int NSTORE = 6;
@@ -293,6 +301,41 @@ __global__ void __launch_bounds__(FBLOCK) k_fold(const DevInst* __restrict__ ins
   const DevInst& I = insts[ids ? ids[blockIdx.x] : blockIdx.x];
   fold_sweep(I);
   if (threadIdx.x == 0) G(I.sc)->ahead = 0;   // a staged next step folded this sweep too
+}
+
+// Raises md of the per-point tiles (both planes) covering cells [nlo, nhi] x [mlo, mhi]
+// and the row below and column to the left, whose points read these cells' levels.
+__device__ __forceinline__ void mark_cells(const DevInst& I, int nlo, int nhi, int mlo, int mhi) {
+  const int c0 = max(nlo - 1, 0) >> I.mtn_sh, c1 = min(nhi, I.nmax - 1) >> I.mtn_sh;
+  const int r0 = max(mlo - 1, 0) >> I.mrow_sh, r1 = min(mhi, I.mmax - 1) >> I.mrow_sh;
+  for (int r = r0; r <= r1; ++r)
+    for (int c = c0; c <= c1; ++c) {
+      const uint32_t t = 2u * (uint32_t)(r * I.mntn + c);
+      G(I.md)[t] = 1;
+      G(I.md)[t + 1] = 1;
+    }
+}
+
+// A level set outside continuity (boundary cells in forcing).
+__device__ __forceinline__ void mark_cell(const DevInst& I, int c) {
+  if (!I.dense || I.sk >= 0) return;
+  const int m = c / I.nmax, n = c - m * I.nmax;
+  mark_cells(I, n, n, m, m);
+}
+
+// Continuity: when a level of this wave's cells (part of dense tile `tile`) changed, the
+// first active lane marks the per-point tiles that read them.
+__device__ __forceinline__ void mark_wave(const DevInst& I, int tile, bool changed) {
+  const uint64_t act = __ballot(1);
+  if (__ballot(changed) == 0) return;
+  if ((int)(threadIdx.x % WAVE) != __ffsll((long long)act) - 1) return;
+  const int tn = tile % I.ntn, tmi = tile / I.ntn;
+  const int w0 = threadIdx.x & ~(WAVE - 1);
+  const int nlo = tn * I.tn + (w0 & (I.tn - 1));
+  const int nhi = nlo + min(WAVE, I.tn) - 1;
+  const int mlo = tmi * I.tm + (w0 >> I.tn_sh);
+  const int mhi = tmi * I.tm + ((w0 + WAVE - 1) >> I.tn_sh);
+  mark_cells(I, nlo, nhi, mlo, mhi);
 }
 
 // ----------------------------------------------------------------------------- forcing
@@ -496,7 +539,7 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   // on the host).
   for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
     const int nmb = G(I.gb_cell)[ib];
-    if ((G(I.kcs)[nmb] & KCS_MASK) == 6) G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]];
+    if ((G(I.kcs)[nmb] & KCS_MASK) == 6) { G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]]; mark_cell(I, nmb); }
   }
   if (I.bnd_overlap) __syncthreads();
   // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
@@ -599,6 +642,7 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)F.s.factime * q[ip] + (R)F.s.facrel * (R)F.s.one_minus_factime * uvmean[ib];
     else uvmean[ib] = R(0);
     G(I.zs)[nmb] = (double)zsb1;
+    mark_cell(I, nmb);
     if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], (double)zsb1);
   }
 }
@@ -749,15 +793,15 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 // round trip to memory.
 // One uv-point of the per-point kernel from its preloaded branch inputs: kcuv `kc`, the
 // kfuv byte `kf` (with the zero flags), the water levels of its two cells and its wet/dry
-// threshold `fl`.
+// threshold `fl`. Returns true when there is no point or it is dry with its zeros held.
 template <typename R, bool SUBGRID, typename QC, typename QW>
-__device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, int n, int m, int vplane, int kc, int kf,
+__device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, int n, int m, int vplane, int kc, int kf,
                                              double zs_nm, double zs_nmu, float fl, R dt, QC q0, QC u0, QW q1,
                                              QW u1, int* cnt, float& mdt) {
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
-  if (n >= nmax || m >= mmax) return;
+  if (n >= nmax || m >= mmax) return true;
   const bool hu = m < mmax - 1, hv = n < nmax - 1;
-  if (vplane ? !hv : !hu) { cnt[0]++; return; }
+  if (vplane ? !hv : !hu) { cnt[0]++; return true; }
   const int c = m * nmax + n;
   const int s = vplane ? np + c : c;
   const int c_nmu = vplane ? c + 1 : c + nmax;
@@ -769,15 +813,15 @@ __device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, in
   if (!(kc == 1 || kc == 6)) {
     q1[s] = q0[s]; u1[s] = u0[s];
     cnt[0]++;
-    return;
+    return false;
   }
   R zsu;
   if (!sfx::uv_wet<R, SUBGRID>(in, tb, I.c, zsu)) {
-    if (kf & KF_ZERO2) { cnt[4]++; return; }   // both buffers already hold the zeros
+    if (kf & KF_ZERO2) { cnt[4]++; return true; }   // both buffers already hold the zeros
     q1[s] = R(0); u1[s] = R(0);
     G(I.kfuv)[s] = (kf & KF_ZERO1) ? KF_ZERO2 : KF_ZERO1;
     cnt[1]++;
-    return;
+    return false;
   }
   // the rest of the stencil of a wet point
   int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
@@ -809,6 +853,7 @@ __device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, in
   int br = 0;
   wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br, kf);
   cnt[br]++;
+  return false;
 }
 
 // Dense scenarios, per-point kernel: tile 2t holds the u-points and tile 2t+1 the v-points
@@ -818,7 +863,7 @@ __device__ __forceinline__ void point_finish(const DevInst& I, GA<Scalars> S, in
 // scalars are read and the active test is made. Returns false if the scenario is not
 // active (nothing is written).
 template <typename R, bool SUBGRID, int P>
-__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, float& mdt) {
+__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, float& mdt, bool& held) {
   const auto S = G(Sp);
   const int nmax = I.nmax, mmax = I.mmax, np = I.np;
   const int vplane = tile & 1;
@@ -849,11 +894,28 @@ __device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int til
   const auto u0 = G((const R*)I.uv[cur]);
   const auto q1 = G((R*)I.q[cur ^ 1]);
   const auto u1 = G((R*)I.uv[cur ^ 1]);
+  held = true;
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    point_finish<R, SUBGRID>(I, S, n, m0 + j * I.mtm, vplane, kc[j], kf[j], za[j], zb[j], fl[j], dt, q0, u0, q1, u1, cnt,
-                             mdt);
+    held = point_finish<R, SUBGRID>(I, S, n, m0 + j * I.mtm, vplane, kc[j], kf[j], za[j], zb[j], fl[j], dt, q0, u0, q1,
+                                    u1, cnt, mdt) && held;
   return true;
+}
+
+// Points of a skipped (quiescent) per-point tile, for the counters: class 5, or copied (0)
+// for the absent edge slots, as a run would count them.
+template <int P>
+__device__ __forceinline__ void count_tile_points(const DevInst& I, int tile, int* cnt) {
+  const int nmax = I.nmax, mmax = I.mmax;
+  const int vplane = tile & 1, t = tile >> 1;
+  const int tn = t % I.mntn, tmi = t / I.mntn;
+  const int n = tn * I.mtn + (threadIdx.x & (I.mtn - 1));
+  const int m0 = tmi * I.mtm * P + (threadIdx.x >> I.mtn_sh);
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int m = m0 + j * I.mtm;
+    if (n < nmax && m < mmax) ++cnt[(vplane ? n < nmax - 1 : m < mmax - 1) ? 5 : 0];
+  }
 }
 
 // Dense scenarios, strip march. A wave covers columns n = n0-1 .. n0+62 of one strip
@@ -1086,7 +1148,7 @@ __device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float m
     const uint32_t m = wave_min_last(__float_as_uint(v));
     if (threadIdx.x % WAVE == WAVE - 1) atomicMin((unsigned int*)&G(I.blkmin)[tile], m);
   }
-  int cw[NCNT] = {0, 0, 0, 0, 0};
+  int cw[NCNT] = {0, 0, 0, 0, 0, 0};
   if (I.count)
     for (int k = 0; k < NCNT; ++k) {
       int v = cnt[k];
@@ -1130,7 +1192,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ 
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
-  int cnt[NCNT] = {0, 0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
   mom_cell<R, SUBGRID>(I, tile, cnt, mdt);
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1142,8 +1204,18 @@ __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   float mdt = I.c.dtmax;
-  int cnt[NCNT] = {0, 0, 0, 0, 0};
-  if (!mom_point<R, SUBGRID, P>(I, Sp, tile, cnt, mdt)) return;   // scenario not active
+  int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
+  if (G(I.mq)[tile] == 0x01010101u && G(I.md)[tile] == 0) {   // quiescent: nothing would change
+    if (!G(Sp)->active) return;
+    if (I.count) count_tile_points<P>(I, tile, cnt);
+    mom_epilogue(I, tile, mdt, cnt);
+    return;
+  }
+  bool held;
+  if (!mom_point<R, SUBGRID, P>(I, Sp, tile, cnt, mdt, held)) return;   // scenario not active
+  if (threadIdx.x == 0) G(I.md)[tile] = 0;
+  const bool wave_held = __ballot(!held) == 0;
+  if (threadIdx.x % WAVE == 0) G((uint8_t*)I.mq)[4 * (uint32_t)tile + threadIdx.x / WAVE] = wave_held ? 1 : 0;
   mom_epilogue(I, tile, mdt, cnt);
 }
 
@@ -1155,7 +1227,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
-  int cnt[NCNT] = {0, 0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
   mom_strip<R, SUBGRID>(I, tile, cnt, mdt);
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1168,7 +1240,7 @@ __global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ i
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
   float mdt = I.c.dtmax;
-  int cnt[NCNT] = {0, 0, 0, 0, 0};
+  int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
   mom_idx<R, SUBGRID>(I, tile, cnt, mdt);
   mom_epilogue(I, tile, mdt, cnt);
 }
@@ -1314,6 +1386,7 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky)
                                  : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky);
     if (I.count) count_stores(I, tile, w);
+    if (I.sk < 0) mark_wave(I, tile, (w & 1u) != 0);
   } else {
     const int c = tile * BLOCK + threadIdx.x;
     if (c >= I.np) return;
@@ -1690,6 +1763,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   I.xchunk = H->xchunk; I.xchunk_c = H->xchunk_c;
   I.mntn = M->dense ? (M->nmax + I.mtn - 1) / I.mtn : 0;
   I.tn_sh = ilog2(I.tn); I.mtn_sh = ilog2(I.mtn);
+  I.mrow_sh = ilog2(I.mtm * (I.sk < 0 ? -I.sk : 1));
   I.nmt = (int32_t)mom_tiles(I);
   H->nmt_cap = I.nmt;
   if (M->dense) {
@@ -1702,6 +1776,8 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.blkmin = H->upload(init.data(), init.size());
   }
   I.blkcnt = H->alloc<uint32_t>(NCNT * H->nmt_cap);
+  I.mq = H->alloc<uint32_t>(H->nmt_cap);
+  I.md = H->alloc<uint8_t>(H->nmt_cap);
   H->nct_cap = M->dense ? 0 : (size_t)((I.np + BLOCK - 1) / BLOCK);
   if (M->dense)
     for (int k = 0; k < N_CELL_SHAPES; ++k)
@@ -1848,13 +1924,15 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
   return SFX_ERR_ARG;
 }
 
-// A host write of cell state: no cell keeps its steady bit.
+namespace { int reset_tile_flags(sfx_handle* H); }
+
+// A host write of cell state: no cell keeps its steady bit and no tile is quiescent.
 int clear_steady(sfx_handle* H) {
   std::vector<uint8_t> b(H->hI.np);
   HIPCHK(hipMemcpy(b.data(), H->hI.kcs, b.size(), hipMemcpyDeviceToHost));
   for (auto& v : b) v = (uint8_t)(v & ~KCS_STEADY);
   HIPCHK(hipMemcpy(H->hI.kcs, b.data(), b.size(), hipMemcpyHostToDevice));
-  return SFX_OK;
+  return reset_tile_flags(H);
 }
 
 extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, size_t nbytes) {
@@ -1880,12 +1958,12 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
     void* dst = k == "q" ? I.q[s.cur] : I.uv[s.cur];
     rc = H->fp64 ? set_uv_field<double>(H, dst, src, nbytes) : set_uv_field<float>(H, dst, src, nbytes);
     if (rc) return rc;
-    // only the current buffer changed: the per-point kernel's zero flags no longer hold
+    // only the current buffer changed: the momentum kernels' zero flags no longer hold
     std::vector<int8_t> b(H->nslot + 1);
     HIPCHK(hipMemcpy(b.data(), I.kfuv, b.size(), hipMemcpyDeviceToHost));
     for (auto& v : b) v = (int8_t)(v & 1);
     HIPCHK(hipMemcpy(I.kfuv, b.data(), b.size(), hipMemcpyHostToDevice));
-    return SFX_OK;
+    return reset_tile_flags(H);
   }
   if (k == "kfuv") {
     const size_t n = H->sz.npuv;
@@ -1894,7 +1972,7 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
     HIPCHK(hipMemcpy(b.data(), I.kfuv, b.size(), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i) b[H->ref2slot[i]] = ((const int8_t*)src)[i];
     HIPCHK(hipMemcpy(I.kfuv, b.data(), b.size(), hipMemcpyHostToDevice));
-    return SFX_OK;
+    return reset_tile_flags(H);
   }
   if (k == "zs0" || k == "zsderv" || k == "zsmax") {
     void* dst = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
@@ -2073,7 +2151,7 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   }
   // A whole step: the next step's fold, scalars and series (forcing_ahead_block) run as
   // the leading blocks of this step's first continuity launch, beside its cells.
-  const int nahead = ((what & 6) == 6 && forcing_mode == 0) ? nm : 0;
+  const int nahead = (what == 7 && forcing_mode == 0) ? nm : 0;
   const int lead = (nahead + 7) / 8 * 8;   // keeps block b of the cells on XCD b mod 8
   if (what & 4) {
     if (B->n_cd)
@@ -2219,7 +2297,7 @@ extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {
   return SFX_OK;
 }
 
-extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[5]) {
+extern "C" int sfx_get_counters(sfx_handle* H, int64_t counts[6]) {
   if (!H || !counts) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
   HIPCHK(hipStreamSynchronize(H->stream));
@@ -2384,11 +2462,18 @@ int time_phases(sfx_handle* H, int what, int trials, hipEvent_t e0, hipEvent_t e
   return SFX_OK;
 }
 
-// Tile minima back to dtmax (after sweeps that were not folded, e.g. timing runs).
+// No per-point tile is quiescent (after host writes of state and tile-shape changes).
+int reset_tile_flags(sfx_handle* H) {
+  HIPCHK(hipMemset(H->hI.mq, 0, H->nmt_cap * sizeof(uint32_t)));
+  return SFX_OK;
+}
+
+// Tile minima back to dtmax (after sweeps that were not folded, e.g. timing runs), and no
+// tile quiescent.
 int reset_blkmin(sfx_handle* H) {
   std::vector<float> init(H->nmt_cap, H->hI.c.dtmax);
   HIPCHK(hipMemcpy(H->hI.blkmin, init.data(), init.size() * sizeof(float), hipMemcpyHostToDevice));
-  return SFX_OK;
+  return reset_tile_flags(H);
 }
 
 void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
@@ -2397,6 +2482,7 @@ void set_shapes(sfx_handle* H, int sk, int mtn, int mtm, int tn, int tm) {
   I.mntn = (I.nmax + mtn - 1) / mtn;
   I.ntn = (I.nmax + tn - 1) / tn;
   I.tn_sh = ilog2(tn); I.mtn_sh = ilog2(mtn);
+  I.mrow_sh = ilog2(mtm * (sk < 0 ? -sk : 1));
   I.nmt = (int32_t)mom_tiles(I);
 }
 }  // namespace
@@ -2459,7 +2545,10 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int tn0 = H->hI.tn, tm0 = H->hI.tm;
-  // momentum: per-cell tiles, per-point tiles with 1, 2 or 4 points per thread, strip marches
+  // momentum: per-cell tiles, per-point tiles with 1, 2 or 4 points per thread, strip
+  // marches. Each trial times momentum with the continuity sweep after it (the continuity
+  // tiles stay fixed meanwhile), so that the per-point kernel's quiescent tiles are the ones
+  // a step leaves, not those of repeated sweeps over unchanged levels.
   float best = 1e30f;
   int bsk = H->hI.sk, bmtn = H->hI.mtn, bmtm = H->hI.mtm;
   const int kinds[] = {0, -1, -2, -4};
@@ -2471,7 +2560,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
     const int mtm = tiled ? CELL_SHAPES[k % N_CELL_SHAPES][1] : bmtm;
     set_shapes(H, sk, mtn, mtm, tn0, tm0);
     float ms = 0.f;
-    rc = time_phases(H, 2, trials, e0, e1, ms);
+    rc = time_phases(H, 6, trials, e0, e1, ms);
     if (!rc && ms < best) { best = ms; bsk = sk; bmtn = mtn; bmtm = mtm; }
   }
   // momentum work-list order (runs of tiles per XCD)
@@ -2482,7 +2571,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   for (int k = 0; k < 5 && !rc; ++k) {
     H->hI.xchunk = chunks[k];
     float ms = 0.f;
-    rc = time_phases(H, 2, trials, e0, e1, ms);
+    rc = time_phases(H, 6, trials, e0, e1, ms);
     if (!rc && ms < best) { best = ms; bx = chunks[k]; }
   }
   H->hI.xchunk = bx;

@@ -324,3 +324,29 @@ def test_state_write_over_long_dry_points_matches_oracle(mesh):
     print(d, flush=True)
     assert inst.time()[2] == int(orc.get("nt"))
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+
+
+QUIET = [(-1, (16, 16)), (-2, (64, 4)), (-4, (32, 8))]
+
+
+@pytest.mark.parametrize("shape", QUIET, ids=[f"kernel{a}_mom{b[0]}x{b[1]}" for a, b in QUIET])
+def test_quiescent_tiles_wake_with_the_tide(shape):
+    """Without rain most per-point tiles go quiescent (all points dry with their zeros held,
+    no level they read changing) and are skipped; a rising tide must wake them as the water
+    reaches their cells. Checked against the oracle after 2 h of a 2 m tide, with the
+    skip counted so that the path is known to run."""
+    kind, mt = shape
+    print(f"case quiescent tiles {shape}", flush=True)
+    sc = W.subgrid_case(70, 83, dx=25.0, seed=41, t_end=7200.0, tide_amp=2.0, nsrc=0, rain_mmh=0.0)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    inst.set_tuning(kind, mt, (64, 4))
+    inst.set_counting(True)
+    rc, st = inst.update(-1000.0)
+    skipped = inst.counters()[5]
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    print(shape, st["nt"], skipped, d, flush=True)
+    assert skipped > 0
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL

@@ -73,6 +73,12 @@ typedef struct sfx_params {
   int32_t store_zsmax;       /* 1 (dtmaxout > 0 by default) */
   int32_t bndtype;           /* 1 = weakly reflective */
   int32_t h73table;          /* 0 */
+  /* compute_store_variables (sfincs_continuity.f90:626-720), all off by default; the
+   * reference's switches derived at sfincs_input.f90:454-463, 501-504 */
+  int32_t store_velmax;      /* vmax: storevelmax == 1 and dtmaxout > 0 */
+  int32_t store_fluxmax;     /* qmax: storefluxmax == 1 and dtmaxout > 0 */
+  int32_t store_twet;        /* twet: storetwet == 1 */
+  float twet_threshold;      /* 0.01 (sfincs_input.f90:285) */
 } sfx_params;
 
 void sfx_params_default(sfx_params* p);
@@ -160,7 +166,10 @@ typedef struct sfx_step_stats {
 
 int  sfx_create(const sfx_model* m, int32_t device, sfx_handle** out);
 int  sfx_destroy(sfx_handle* h);
-/* State exchange in reference order; the explicit form of `acc update host/device`. */
+/* State exchange in reference order; the explicit form of `acc update host/device`.
+ * Names: zs, z_volume (f64 per cell); q, uv (per uv-point); kfuv (i8); zs0, zsderv, zsmax,
+ * uvmean; and, when their switch is on, vmax, qmax, twet (f32 per cell; initial values
+ * -999, -999, 0 as sfincs_domain.f90:2300-2306 and sfincs_output.f90:210 reset them). */
 int  sfx_get_state(sfx_handle* h, const char* name, void* dst, size_t nbytes);
 int  sfx_set_state(sfx_handle* h, const char* name, const void* src, size_t nbytes);
 /* compute_fluxes(dt) (sfincs_momentum.f90:9-784): one momentum sweep with the given dt,

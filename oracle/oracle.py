@@ -104,7 +104,7 @@ class OracleRun:
             a = model.table(n)
             self._bind(n, a)
         # state (copies of the initial state)
-        for n in ("zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv"):
+        for n in ("zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv", "vmax", "qmax", "twet"):
             self._bind(n, model.table(n).copy())
         if s["subgrid"]:
             self._bind("z_volume", model.table("z_volume").copy())
@@ -151,6 +151,8 @@ class OracleRun:
                     advection_scheme=p.advection_scheme, viscosity=p.viscosity, coriolis=int(coriolis),
                     friction2d=p.friction2d, thetasmoothing=int(p.theta < np.float32(0.9999)),
                     wiggle_suppression=p.wiggle_suppression, store_zsmax=p.store_zsmax,
+                    store_velmax=p.store_velmax, store_fluxmax=p.store_fluxmax, store_twet=p.store_twet,
+                    twet_threshold=p.twet_threshold,
                     wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     nsrc=s["nsrc"], ntsrc=len(tb("tsrc")), ntprcp=len(tb("tprcpt")), ntwnd=len(tb("twnd")),
@@ -230,7 +232,8 @@ REF_I8 = ("kcs", "kcuv", "mask_adv", "z_flags_iref", "uv_flags_iref", "uv_flags_
 REF_F32_TABLES = ("dxrinv", "dyrinv", "dxr2inv", "dyr2inv", "dxrinvc", "dyrinvc", "dxrm", "dyrm", "cell_area",
                   "nuvisc")
 REF_SCALARS = ("subgrid", "use_quadtree", "advection", "advection_scheme", "viscosity", "coriolis", "friction2d",
-               "thetasmoothing", "wiggle_suppression", "store_zsmax", "wind", "precip", "h73table", "nsrc", "g",
+               "thetasmoothing", "wiggle_suppression", "store_zsmax", "store_velmax", "store_fluxmax", "store_twet",
+               "twet_threshold", "wind", "precip", "h73table", "nsrc", "g",
                "huthresh", "huvmin", "advlim", "slopelim", "uvlim", "nuviscfac", "fcorio", "theta",
                "wiggle_factor", "wiggle_threshold", "dtmax", "rhow", "dx", "dy")
 
@@ -273,13 +276,14 @@ class RefRun:
         R.ref_set_f64(b"zs", k["zs"].ctypes.data, k["zs"].size)
         if self.o.vals["subgrid"]:
             R.ref_set_f64(b"z_volume", k["z_volume"].ctypes.data, k["z_volume"].size)
-        for n in ("q", "uv", "zs0", "zsderv", "zsmax", "tauwu", "tauwv", "netprcp", "qtsrc"):
+        for n in ("q", "uv", "zs0", "zsderv", "zsmax", "tauwu", "tauwv", "netprcp", "qtsrc", "vmax", "qmax",
+                  "twet"):
             R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
         R.ref_set_i8(b"kfuv", k["kfuv"].ctypes.data, k["kfuv"].size)
 
     def pull_state(self):
         R, k = self.R, self.o.keep
-        names = ["zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv"]
+        names = ["zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv", "vmax", "qmax", "twet"]
         if self.o.vals["subgrid"]:
             names.append("z_volume")
         for n in names:

@@ -56,6 +56,10 @@ contains
       case ('thetasmoothing');     thetasmoothing = (v /= 0.0d0)
       case ('wiggle_suppression'); wiggle_suppression = (v /= 0.0d0)
       case ('store_zsmax');        store_maximum_waterlevel = (v /= 0.0d0)
+      case ('store_velmax');       store_maximum_velocity = (v /= 0.0d0)
+      case ('store_fluxmax');      store_maximum_flux = (v /= 0.0d0)
+      case ('store_twet');         store_twet = (v /= 0.0d0)
+      case ('twet_threshold');     twet_threshold = real(v, 4)
       case ('wind');               wind = (v /= 0.0d0)
       case ('precip');             precip = (v /= 0.0d0)
       case ('h73table');           h73table = (v /= 0.0d0)
@@ -162,6 +166,9 @@ contains
       case ('zs0');          call putr(zs0)
       case ('zsderv');       call putr(zsderv)
       case ('zsmax');        call putr(zsmax)
+      case ('vmax');         call putr(vmax)
+      case ('qmax');         call putr(qmax)
+      case ('twet');         call putr(twet)
       case ('tauwu');        call putr(tauwu)
       case ('tauwv');        call putr(tauwv)
       case ('netprcp');      call putr(netprcp)
@@ -233,6 +240,9 @@ contains
       case ('zs0');      call c_f_pointer(p, f, [nbytes / 4]); f = zs0
       case ('zsderv');   call c_f_pointer(p, f, [nbytes / 4]); f = zsderv
       case ('zsmax');    call c_f_pointer(p, f, [nbytes / 4]); f = zsmax
+      case ('vmax');     call c_f_pointer(p, f, [nbytes / 4]); f = vmax
+      case ('qmax');     call c_f_pointer(p, f, [nbytes / 4]); f = qmax
+      case ('twet');     call c_f_pointer(p, f, [nbytes / 4]); f = twet
       case ('zs');       call c_f_pointer(p, d, [nbytes / 8]); d = zs
       case ('z_volume'); call c_f_pointer(p, d, [nbytes / 8]); d = z_volume
       case ('kfuv');     call c_f_pointer(p, b, [nbytes]);     b = kfuv

@@ -7,7 +7,7 @@
  * the reference's own Fortran bit for bit on x86-64 (pinned against oracle/_ref, the
  * reference's sfincs_momentum/sfincs_continuity compiled from source):
  *   oracle_compute_fluxes        sfincs_momentum.f90:9-784
- *   oracle_compute_water_levels  sfincs_continuity.f90:5-624
+ *   oracle_compute_water_levels  sfincs_continuity.f90:5-720 (with compute_store_variables)
  *   oracle_update_meteo          sfincs_meteo.f90:1246-1558 (time-series branch)
  *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
  *   oracle_update_discharges     sfincs_discharges.f90:325-369
@@ -25,6 +25,8 @@ typedef struct OM {
   int32_t np, npuv, ncuv, nref, nl;
   int32_t subgrid, use_quadtree, advection, advection_scheme, viscosity, coriolis, friction2d;
   int32_t thetasmoothing, wiggle_suppression, store_zsmax, wind, precip, h73table;
+  int32_t store_velmax, store_fluxmax, store_twet;
+  float twet_threshold;
   int32_t ngbnd, nkcuv2, nbnd, ntbnd, nsrc, ntsrc, ntprcp, ntwnd;
   float g, pi, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
@@ -45,6 +47,7 @@ typedef struct OM {
   /* state */
   double *zs, *z_volume;
   float *q, *uv, *q0, *uv0, *zs0, *zsderv, *zsmax;
+  float *vmax, *qmax, *twet;   /* compute_store_variables (np each, when switched on) */
   int8_t *kfuv;
   float *tauwu, *tauwv, *netprcp;   /* np each */
   /* boundaries */
@@ -84,6 +87,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(z_zmin) BIND(z_zmax) BIND(z_volmax) BIND(z_dep) BIND(uv_zmin) BIND(uv_zmax) BIND(uv_navg_w) BIND(uv_fnfit)
   BIND(uv_havg) BIND(uv_nrep) BIND(uv_pwet) BIND(uv_havg_zmax) BIND(uv_nrep_zmax)
   BIND(zs) BIND(z_volume) BIND(q) BIND(uv) BIND(q0) BIND(uv0) BIND(zs0) BIND(zsderv) BIND(zsmax) BIND(kfuv)
+  BIND(vmax) BIND(qmax) BIND(twet)
   BIND(tauwu) BIND(tauwv) BIND(netprcp)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
   BIND(nmi_gbp) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
@@ -99,6 +103,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
   SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error)
+  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
   SETF(btfilter) SETF(btrelax) SETF(t0) SETF(t1) SETF(tspinup) SETF(zini) SETF(dx) SETF(dy) SETF(dt) SETF(min_dt)
@@ -311,9 +316,41 @@ float oracle_compute_fluxes(OM* m, float dt) {
   return min_dt;
 }
 
-/* ---- continuity: sfincs_continuity.f90:46-262 and :265-624 -------------------------- */
+/* ---- store variables: sfincs_continuity.f90:626-720 ------------------------------------ */
+static void compute_store_variables(OM* m, float dt) {
+  for (int32_t nm = 0; nm < m->np; ++nm) {
+    if (!(m->kcs[nm] == 1 || m->kcs[nm] == 4)) continue;
+    const int32_t nmd = m->z_index_uv_md[nm] - 1, nmu = m->z_index_uv_mu[nm] - 1;
+    const int32_t ndm = m->z_index_uv_nd[nm] - 1, num = m->z_index_uv_nu[nm] - 1;
+    float uvz = 0.0f, qz = 0.0f;
+    if (m->store_velmax) {
+      const float quz = (m->uv[nmd] + m->uv[nmu]) / 2, qvz = (m->uv[ndm] + m->uv[num]) / 2;
+      uvz = sqrtf(quz * quz + qvz * qvz);
+    }
+    if (m->store_fluxmax) {
+      const float quz = (m->q[nmd] + m->q[nmu]) / 2, qvz = (m->q[ndm] + m->q[num]) / 2;
+      qz = sqrtf(quz * quz + qvz * qvz);
+    }
+    if (m->store_velmax) m->vmax[nm] = fmx(m->vmax[nm], uvz);
+    if (m->store_fluxmax) m->qmax[nm] = fmx(m->qmax[nm], qz);
+    if (m->store_twet) {
+      const double base = m->subgrid ? (double)m->z_zmin[nm] : (double)m->zb[nm];
+      if ((m->zs[nm] - base) > (double)m->twet_threshold) m->twet[nm] = m->twet[nm] + dt;
+    }
+  }
+}
+
+static void water_levels(OM* m, float dt);
+
+/* ---- continuity: sfincs_continuity.f90:5-43 ------------------------------------------ */
 void oracle_compute_water_levels(OM* m, double t, float dt) {
   (void)t;
+  water_levels(m, dt);
+  if (m->store_velmax || m->store_fluxmax || m->store_twet) compute_store_variables(m, dt);
+}
+
+/* ---- continuity: sfincs_continuity.f90:46-262 and :265-624 -------------------------- */
+static void water_levels(OM* m, float dt) {
   if (!m->subgrid) {
     for (int32_t s = 0; s < m->nsrc; ++s) {   /* :87-106 */
       const int32_t nm = m->nmindsrc[s] - 1;

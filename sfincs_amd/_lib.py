@@ -25,7 +25,8 @@ class sfx_params(C.Structure):
         "advlim", "slopelim", "btfilter", "btrelax", "wiggle_factor", "wiggle_threshold", "uvlim",
         "uvmax", "nuvisc", "nuviscfac", "latitude", "g", "pi")] + [(n, C.c_int32) for n in (
         "advection", "advection_scheme", "viscosity", "coriolis", "friction2d", "advection_mask",
-        "wiggle_suppression", "store_zsmax", "bndtype", "h73table")]
+        "wiggle_suppression", "store_zsmax", "bndtype", "h73table", "store_velmax", "store_fluxmax",
+        "store_twet")] + [("twet_threshold", C.c_float)]
 
 
 class sfx_quadtree_input(C.Structure):
@@ -280,7 +281,7 @@ class Model:
 
 STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv": np.float32,
                 "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
-                "uvmean": np.float32}
+                "uvmean": np.float32, "vmax": np.float32, "qmax": np.float32, "twet": np.float32}
 
 
 class Instance:
@@ -295,7 +296,7 @@ class Instance:
 
     def _n(self, name):
         s = self.sizes
-        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax"):
+        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet"):
             return s["np"]
         if name in ("q", "uv"):
             return s["npuv"] + s["ncuv"] + 1

@@ -115,6 +115,9 @@ struct DevInst {
   void* zs0;
   void* zsderv;
   void* zsmax;
+  void* vmax;                    // compute_store_variables fields, allocated when switched on
+  void* qmax;
+  void* twet;
   uint8_t* kcs;                  // kcs (KCS_MASK) | KCS_SOURCE (point source) | KCS_STEADY (continuity)
   const float* zb;
   const int8_t* z_iref;
@@ -1331,13 +1334,46 @@ __device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md,
                                                 R dt, R netprcp, bool dry_sky) {
   const uint8_t kraw = G(I.kcs)[c];
   const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
-                     q_nd == R(0) && q_nu == R(0);
+                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & sfx::F_STORE);
   if (quiet && (kraw & KCS_STEADY)) return 32u;   // skipped (counted as such)
   const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
   const bool steady = quiet && w == 0u;
   if (steady != ((kraw & KCS_STEADY) != 0))
     G(I.kcs)[c] = (uint8_t)((kraw & ~KCS_STEADY) | (steady ? KCS_STEADY : 0));
   return w;
+}
+
+// compute_store_variables (sfincs_continuity.f90:626-720) for one cell after its update,
+// fused into continuity (the same faces, the level just computed): for kcs 1 or 4, vmax
+// and qmax keep the largest magnitude of the face-averaged uv and q, and twet adds dt while
+// the level exceeds the bed (regular) or the cell's lowest subgrid level by
+// twet_threshold. A maximum is stored only when it grows.
+template <typename R>
+__device__ __forceinline__ void store_vars(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, R u_md, R u_mu,
+                                           R u_nd, R u_nu, R dt) {
+  const int kcs = G(I.kcs)[c] & KCS_MASK;
+  if (kcs != 1 && kcs != 4) return;
+  const uint32_t f = I.c.flags;
+  if (f & sfx::F_VELMAX) {
+    const R quz = (u_md + u_mu) / R(2), qvz = (u_nd + u_nu) / R(2);
+    const R uvz = sfx::rsqrt_<R>(quz * quz + qvz * qvz);
+    const auto vm = G((R*)I.vmax);
+    if (uvz > vm[c]) vm[c] = uvz;
+  }
+  if (f & sfx::F_FLUXMAX) {
+    const R quz = (q_md + q_mu) / R(2), qvz = (q_nd + q_nu) / R(2);
+    const R qz = sfx::rsqrt_<R>(quz * quz + qvz * qvz);
+    const auto qm = G((R*)I.qmax);
+    if (qz > qm[c]) qm[c] = qz;
+  }
+  if (f & sfx::F_TWET) {
+    const double zs = G(I.zs)[c];   // this thread's own update of the cell, just stored
+    const double base = (f & sfx::F_SUBGRID) ? (double)G(I.z_zmin)[c] : (double)G(I.zb)[c];
+    if (zs - base > (double)I.c.twet_threshold) {
+      const auto tw = G((R*)I.twet);
+      tw[c] = tw[c] + dt;
+    }
+  }
 }
 
 // Adds a wave's continuity stores per field to its tile's counters (active lanes only).
@@ -1386,6 +1422,17 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky)
                                  : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky);
     if (I.count) count_stores(I, tile, w);
+    if (I.c.flags & sfx::F_STORE) {
+      R u_md = R(0), u_mu = R(0), u_nd = R(0), u_nu = R(0);
+      if (I.c.flags & sfx::F_VELMAX) {
+        const auto uv = G((const R*)I.uv[qb]);
+        u_md = m >= 1 ? uv[c - nmax] : R(0);
+        u_mu = m < mmax - 1 ? uv[c] : R(0);
+        u_nd = n >= 1 ? uv[np + c - 1] : R(0);
+        u_nu = n < nmax - 1 ? uv[np + c] : R(0);
+      }
+      store_vars<R>(I, c, q_md, q_mu, q_nd, q_nu, u_md, u_mu, u_nd, u_nu, dt);
+    }
     if (I.sk < 0) mark_wave(I, tile, (w & 1u) != 0);
   } else {
     const int c = tile * BLOCK + threadIdx.x;
@@ -1395,6 +1442,14 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky)
                                  : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky);
     if (I.count) count_stores(I, tile, w);
+    if (I.c.flags & sfx::F_STORE) {
+      R u_md = R(0), u_mu = R(0), u_nd = R(0), u_nu = R(0);
+      if (I.c.flags & sfx::F_VELMAX) {
+        const auto uv = G((const R*)I.uv[qb]);
+        u_md = uv[G(I.zmd)[c]]; u_mu = uv[G(I.zmu)[c]]; u_nd = uv[G(I.znd)[c]]; u_nu = uv[G(I.znu)[c]];
+      }
+      store_vars<R>(I, c, q_md, q_mu, q_nd, q_nu, u_md, u_mu, u_nd, u_nu, dt);
+    }
   }
 }
 
@@ -1518,6 +1573,9 @@ uint32_t build_flags(const sfx_model* M) {
   if (M->thetasmoothing) f |= sfx::F_THETA;
   if (p.wiggle_suppression) f |= sfx::F_WIGGLE;
   if (p.store_zsmax) f |= sfx::F_ZSMAX;
+  if (p.store_velmax) f |= sfx::F_VELMAX;
+  if (p.store_fluxmax) f |= sfx::F_FLUXMAX;
+  if (p.store_twet) f |= sfx::F_TWET;
   if (M->wind) f |= sfx::F_WIND;
   if (M->precip) f |= sfx::F_PRECIP;
   if (p.h73table) f |= sfx::F_H73;
@@ -1536,7 +1594,7 @@ int fill_consts(const sfx_model* M, Consts& c) {
   c.wiggle_factor = p.wiggle_factor; c.wiggle_threshold = p.wiggle_threshold; c.dtmax = M->dtmax; c.dtmin = M->dtmin;
   c.alpha = p.alpha; c.rhoa = p.rhoa; c.rhow = p.rhow; c.btfilter = p.btfilter; c.btrelax = p.btrelax;
   c.t0 = p.t0; c.t1 = p.t1; c.tspinup = p.t0 + p.tspinup; c.zini = p.zsini; c.dx = M->dxrm[0]; c.dy = M->dyrm[0];
-  c.pi = p.pi;
+  c.pi = p.pi; c.twet_threshold = p.twet_threshold;
   for (int i = 0; i < M->nref; ++i) {
     c.dxrinv[i] = M->dxrinv[i]; c.dyrinv[i] = M->dyrinv[i]; c.dxr2inv[i] = M->dxr2inv[i]; c.dyr2inv[i] = M->dyr2inv[i];
     c.dxrinvc[i] = M->dxrinvc[i]; c.dyrinvc[i] = M->dyrinvc[i]; c.cell_area[i] = M->cell_area[i];
@@ -1562,6 +1620,10 @@ int upload_state(sfx_handle* H, const sfx_model* M) {
   for (int b = 0; b < 2; ++b) { I.q[b] = H->upload(qs.data(), qs.size()); I.uv[b] = H->upload(uvs.data(), uvs.size()); }
   auto zs0 = to_r<R>(M->zs0); auto zsd = to_r<R>(M->zsderv); auto zsm = to_r<R>(M->zsmax);
   I.zs0 = H->upload(zs0.data(), np); I.zsderv = H->upload(zsd.data(), np); I.zsmax = H->upload(zsm.data(), np);
+  const uint32_t f = build_flags(M);
+  if (f & sfx::F_VELMAX) { auto v = to_r<R>(M->vmax); I.vmax = H->upload(v.data(), np); }
+  if (f & sfx::F_FLUXMAX) { auto v = to_r<R>(M->qmax); I.qmax = H->upload(v.data(), np); }
+  if (f & sfx::F_TWET) { auto v = to_r<R>(M->twet); I.twet = H->upload(v.data(), np); }
   auto um = to_r<R>(M->uvmean);
   I.uvmean = H->upload(um.data(), um.size());
   return SFX_OK;
@@ -1916,6 +1978,11 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
     void* src = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
     return H->fp64 ? get_cell_r<double>(H, src, dst, nbytes, np) : get_cell_r<float>(H, src, dst, nbytes, np);
   }
+  if (k == "vmax" || k == "qmax" || k == "twet") {
+    void* src = k == "vmax" ? I.vmax : (k == "qmax" ? I.qmax : I.twet);
+    if (!src) { set_error("sfx_get_state: " + k + " is not stored (its store switch is off)"); return SFX_ERR_ARG; }
+    return H->fp64 ? get_cell_r<double>(H, src, dst, nbytes, np) : get_cell_r<float>(H, src, dst, nbytes, np);
+  }
   if (k == "uvmean") {
     const size_t n = H->sz.nkcuv2;
     return H->fp64 ? get_cell_r<double>(H, I.uvmean, dst, nbytes, n) : get_cell_r<float>(H, I.uvmean, dst, nbytes, n);
@@ -1979,6 +2046,11 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
     rc = H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
     if (rc) return rc;
     return clear_steady(H);
+  }
+  if (k == "vmax" || k == "qmax" || k == "twet") {   // e.g. the reset after each maximum output
+    void* dst = k == "vmax" ? I.vmax : (k == "qmax" ? I.qmax : I.twet);
+    if (!dst) { set_error("sfx_set_state: " + k + " is not stored (its store switch is off)"); return SFX_ERR_ARG; }
+    return H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
   }
   set_error("unknown state field " + k);
   return SFX_ERR_ARG;
@@ -2528,6 +2600,9 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   // momentum and continuity write state: time on a snapshot and restore it afterwards
   std::vector<std::string> fields = {"zs", "q", "uv", "kfuv", "zs0", "zsderv", "zsmax"};
   if (H->hI.vol) fields.push_back("z_volume");
+  if (H->hI.vmax) fields.push_back("vmax");
+  if (H->hI.qmax) fields.push_back("qmax");
+  if (H->hI.twet) fields.push_back("twet");
   std::map<std::string, std::vector<char>> snap;
   for (auto& f : fields) {
     size_t n = 0;

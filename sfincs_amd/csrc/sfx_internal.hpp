@@ -65,6 +65,7 @@ struct sfx_model {
   // initial state
   std::vector<double> zs, z_volume;
   std::vector<float> q, uv, zs0, zsderv, zsmax;
+  std::vector<float> vmax, qmax, twet;   // compute_store_variables fields (used when switched on)
   std::vector<int8_t> kfuv;
 
   // boundaries

@@ -36,6 +36,7 @@ extern "C" void sfx_params_default(sfx_params* p) {
   p->latitude = 0.0f; p->g = 9.81f; p->pi = 3.14159f;
   p->advection = 1; p->advection_scheme = 1; p->viscosity = 0; p->coriolis = 1;
   p->friction2d = 1; p->advection_mask = 1; p->wiggle_suppression = 1; p->store_zsmax = 1;
+  p->store_velmax = 0; p->store_fluxmax = 0; p->store_twet = 0; p->twet_threshold = 0.01f;
   p->bndtype = 1; p->h73table = 0;
 }
 
@@ -712,6 +713,9 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
     M->zs0.assign(np, 0.f); M->zsderv.assign(np, 0.f);
   }
   M->zsmax.assign(np, -999.0f);
+  // sfincs_domain.f90:2300-2306 (qmax is allocated without a value there, :2251; -999 is
+  // what sfincs_output.f90:210 resets it to)
+  M->vmax.assign(np, -999.0f); M->qmax.assign(np, -999.0f); M->twet.assign(np, 0.0f);
   if (M->subgrid) {
     // compute_initial_subgrid_volumes (sfincs_subgrid.F90:907-962)
     const int nl = M->nl;
@@ -861,7 +865,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(dxr) T(dyr) T(dxrm) T(dyrm) T(dxrinv) T(dyrinv) T(dxr2inv) T(dyr2inv) T(dxrinvc) T(dyrinvc) T(cell_area) T(nuvisc)
   T(z_zmin) T(z_zmax) T(z_volmax) T(z_dep) T(uv_zmin) T(uv_zmax) T(uv_navg_w) T(uv_fnfit)
   T(uv_havg) T(uv_nrep) T(uv_pwet) T(uv_havg_zmax) T(uv_nrep_zmax)
-  T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv)
+  T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv) T(vmax) T(qmax) T(twet)
   T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
   T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval) T(x73)
   T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)

@@ -14,7 +14,10 @@ enum : uint32_t {
   F_ADVECTION = 1u << 0, F_UPW1 = 1u << 1, F_VISCOSITY = 1u << 2, F_CORIOLIS = 1u << 3,
   F_FRICTION2D = 1u << 4, F_THETA = 1u << 5, F_WIGGLE = 1u << 6, F_ZSMAX = 1u << 7,
   F_WIND = 1u << 8, F_PRECIP = 1u << 9, F_H73 = 1u << 10, F_QUADTREE = 1u << 11,
-  F_SUBGRID = 1u << 12, F_NBRS = 1u << 13
+  F_SUBGRID = 1u << 12, F_NBRS = 1u << 13,
+  // compute_store_variables switches (sfincs_continuity.f90:626-720)
+  F_VELMAX = 1u << 14, F_FLUXMAX = 1u << 15, F_TWET = 1u << 16,
+  F_STORE = F_VELMAX | F_FLUXMAX | F_TWET
 };
 
 constexpr int MAXREF = 8;
@@ -63,7 +66,7 @@ __device__ __forceinline__ float4 ldf4(GRow<T> r, uint32_t i) { return ldf4(r.ba
 struct Consts {
   float g, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
-  float t0, t1, tspinup, zini, dx, dy, pi;
+  float t0, t1, tspinup, zini, dx, dy, pi, twet_threshold;
   float dxrinv[MAXREF], dyrinv[MAXREF], dxr2inv[MAXREF], dyr2inv[MAXREF], dxrinvc[MAXREF], dyrinvc[MAXREF];
   float cell_area[MAXREF], dxrm[MAXREF], dyrm[MAXREF], nuvisc[MAXREF];
   uint32_t flags;

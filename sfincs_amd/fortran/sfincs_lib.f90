@@ -18,6 +18,8 @@ module sfx_capi
       real(c_float) :: nuviscfac, latitude, g, pi
       integer(c_int32_t) :: advection, advection_scheme, viscosity, coriolis, friction2d
       integer(c_int32_t) :: advection_mask, wiggle_suppression, store_zsmax, bndtype, h73table
+      integer(c_int32_t) :: store_velmax, store_fluxmax, store_twet
+      real(c_float) :: twet_threshold
    end type
    type, bind(C) :: sfx_quadtree_input
       integer(c_int32_t) :: nr_points, nr_levels
@@ -309,7 +311,7 @@ contains
       integer :: ierr
       type(sfx_params), target :: p
       integer :: nmax, mmax, np, ios, u, i, j, n, m, nt, nb, rc
-      real(c_float) :: x0, y0, dx, dy, rot
+      real(c_float) :: x0, y0, dx, dy, rot, dtmaxout
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
       character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, s
       integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
@@ -352,6 +354,17 @@ contains
       call inp_int('advection', i); p%advection = merge(1, 0, i > 0)
       call inp_str('advection_scheme', s, 'upw1')
       if (trim(s) == 'original') p%advection_scheme = 0
+      ! maximum and store switches (sfincs_input.f90:64, 277-285, 449-463, 501-504)
+      dtmaxout = 9999999.0
+      call inp_real('dtmaxout', dtmaxout)
+      p%store_zsmax = merge(1, 0, dtmaxout > 0.0)
+      i = 0
+      call inp_int('storevelmax', i); p%store_velmax = merge(1, 0, i == 1 .and. dtmaxout > 0.0)
+      i = 0
+      call inp_int('storefluxmax', i); p%store_fluxmax = merge(1, 0, i == 1 .and. dtmaxout > 0.0)
+      i = 0
+      call inp_int('storetwet', i); p%store_twet = merge(1, 0, i == 1)
+      call inp_real('twet_threshold', p%twet_threshold)
       host_t0 = p%t0; host_t1 = p%t1
       x0 = 0.0; y0 = 0.0; dx = 0.0; dy = 0.0; rot = 0.0; nmax = 0; mmax = 0
       call inp_int('nmax', nmax); call inp_int('mmax', mmax)

@@ -350,3 +350,51 @@ def test_quiescent_tiles_wake_with_the_tide(shape):
     assert skipped > 0
     assert st["nt"] == int(orc.get("nt"))
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+
+
+STORE = dict(store_velmax=1, store_fluxmax=1, store_twet=1)
+STORE_CASES = [
+    ("regular", lambda: W.regular_case(40, 52, dx=50.0, seed=71, t_end=3600.0, tide_amp=2.0, **STORE)),
+    ("subgrid", lambda: W.subgrid_case(36, 44, dx=25.0, seed=72, t_end=3600.0, nsrc=2, rain_mmh=30.0, tide_amp=2.0,
+                                       twet_threshold=0.05, **STORE)),
+    ("quadtree", lambda: W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=73, t_end=3600.0, surge_amp=4.0, **STORE)),
+]
+
+
+@pytest.mark.parametrize("name,make", STORE_CASES, ids=[c[0] for c in STORE_CASES])
+def test_store_variables_match_oracle(name, make):
+    """vmax, qmax and twet (compute_store_variables, fused into continuity) against the
+    oracle over a whole run, then a reset through set_state (as after a maximum output)
+    and another stretch. Tolerances: vmax 1e-5 m/s and qmax 1e-5 m2/s, as for q; twet
+    1e-3 s (one step is about 1 s, so a cell counted wet in a different step fails)."""
+    print(f"case store variables {name}", flush=True)
+    sc = make()
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    inst.update(1800.0)
+    orc.update(1800.0)
+    assert inst.time()[2] == int(orc.get("nt"))
+    d = compare_state(inst, orc, ("zs", "q", "vmax", "qmax", "twet"))
+    print(name, inst.time()[2], d, flush=True)
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+    assert d["vmax"] <= 1e-5 and d["qmax"] <= 1e-5 and d["twet"] <= 1e-3
+    assert np.max(inst.get("vmax")) > 0.0 and np.max(inst.get("twet")) > 0.0
+    n = m.sizes()["np"]
+    for f, v in (("vmax", -999.0), ("qmax", -999.0), ("twet", 0.0)):
+        inst.set(f, np.full(n, v, np.float32))
+        orc.state(f)[:] = v
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc, ("zs", "q", "vmax", "qmax", "twet"))
+    print(name, st["nt"], d, flush=True)
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["vmax"] <= 1e-5 and d["qmax"] <= 1e-5 and d["twet"] <= 1e-3
+
+
+def test_store_variables_off_are_not_stored():
+    """With the switches off (the default) the fields do not exist on the device."""
+    m, _ = oracle_for(W.regular_case(16, 20, dx=50.0, seed=74, t_end=600.0))
+    inst = gpu_for(m)
+    for f in ("vmax", "qmax", "twet"):
+        with pytest.raises(Exception):
+            inst.get(f)

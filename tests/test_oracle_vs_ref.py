@@ -85,3 +85,35 @@ def test_quadtree_bit_exact():
 def test_quadtree_viscosity_bit_exact():
     sc = W.quadtree_case(8, 12, nlev=3, dx0=100.0, seed=6, t_end=3600.0, surge_amp=4.0, viscosity=1)
     run_pair(sc, 150)
+
+
+STORE = dict(store_velmax=1, store_fluxmax=1, store_twet=1)
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid", "quadtree"])
+def test_store_variables_bit_exact(kind):
+    """compute_store_variables (sfincs_continuity.f90:626-720): vmax, qmax and twet after
+    every step, with all three switches on, through a wetting tide (regular), rain and
+    sources (subgrid) and a surge over a quadtree."""
+    if kind == "regular":
+        sc = W.regular_case(30, 40, dx=50.0, seed=25, t_end=3600.0, tide_amp=2.0, **STORE)
+    elif kind == "subgrid":
+        sc = W.subgrid_case(28, 36, dx=25.0, seed=26, t_end=3600.0, nsrc=2, rain_mmh=40.0, tide_amp=2.0,
+                            twet_threshold=0.05, **STORE)
+    else:
+        sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=27, t_end=3600.0, surge_amp=4.0, **STORE)
+    m = sc.build_model()
+    a = O.OracleRun(m).configure(sc.params, m)
+    b = O.OracleRun(m).configure(sc.params, m)
+    r = O.RefRun(b)
+    fields = ("zs", "q", "uv", "vmax", "qmax", "twet")
+    for i in range(150):
+        a.update(1e30, 1)
+        r.step()
+        for n in fields:
+            x, y = a.state(n), b.state(n)
+            assert np.array_equal(x, y), f"step {i} field {n}: max diff {np.max(np.abs(x.astype(float) - y)):g}"
+    # the switches did something: maxima were raised and some cells were wet for a while
+    assert np.max(a.state("vmax")) > 0.0 and np.max(a.state("qmax")) > 0.0
+    tw = a.state("twet")
+    assert np.max(tw) > 0.0 and np.min(tw) == 0.0

@@ -144,6 +144,13 @@ int  sfx_model_set_rain(sfx_model* m, int32_t nt, const float* t, const float* m
 /* wndfile: uniform wind series, speed m/s and nautical direction in degrees. */
 int  sfx_model_set_wind(sfx_model* m, int32_t nt, const float* t, const float* mag, const float* dir);
 /* Finish initialisation (kcuv, roughness, initial conditions, volumes, advection mask). */
+/* Binary restart file as initial condition (rstfile; read_binary_restart_file,
+ * sfincs_initial_conditions.F90:183-260, applied as :98-174 and sfincs_domain.f90:2272-2311
+ * do): call after the mesh is built and before sfx_model_initialize. Types 1 and 2 give zs,
+ * q and uvmean, type 3 zs only; levels are raised to the bed, combined uv-points take the
+ * mean of their two points and uv starts at 0. An unknown type is skipped (SFX_OK, as the
+ * reference does); types 4-6 (infiltration) and records sized for another mesh are errors. */
+int  sfx_model_set_restart(sfx_model* m, const char* path);
 int  sfx_model_initialize(sfx_model* m);
 /* Initial state by name ("zs" f64, "q" f32, "uv" f32, "kfuv" i8, "z_volume" f64, ...). */
 int  sfx_model_get_state(const sfx_model* m, const char* name, void* dst, size_t nbytes);
@@ -172,6 +179,9 @@ int  sfx_destroy(sfx_handle* h);
  * -999, -999, 0 as sfincs_domain.f90:2300-2306 and sfincs_output.f90:210 reset them). */
 int  sfx_get_state(sfx_handle* h, const char* name, void* dst, size_t nbytes);
 int  sfx_set_state(sfx_handle* h, const char* name, const void* src, size_t nbytes);
+/* write_rst_file (sfincs_output.f90:657-729): a type-1 restart file (zs as f32, q, uvmean)
+ * of the current state at `path` (the reference names it sfincs.<time>.rst). */
+int  sfx_write_restart(sfx_handle* h, const char* path);
 /* compute_fluxes(dt) (sfincs_momentum.f90:9-784): one momentum sweep with the given dt,
  * including the combined-uv averaging; *min_dt receives the CFL reduction (without alpha). */
 int  sfx_compute_fluxes(sfx_handle* h, float dt, float* min_dt);

@@ -76,6 +76,8 @@ def _sig(lib):
         "sfx_model_set_rain": (I, [P, i32, P, P]),
         "sfx_model_set_wind": (I, [P, i32, P, P, P]),
         "sfx_model_initialize": (I, [P]),
+        "sfx_model_set_restart": (I, [P, s]),
+        "sfx_write_restart": (I, [P, s]),
         "sfx_model_get_state": (I, [P, s, P, sz]),
         "sfx_model_destroy": (None, [P]),
         "sfx_model_footprint": (I, [P, C.POINTER(sz)]),
@@ -258,6 +260,10 @@ class Model:
         b = np.ascontiguousarray(direction, np.float32)
         check(lib().sfx_model_set_wind(self.h, len(t), ptr(t), ptr(a), ptr(b)), "set_wind")
 
+    def set_restart(self, path):
+        """Binary restart file as initial condition (rstfile); before initialize()."""
+        check(lib().sfx_model_set_restart(self.h, str(path).encode()), "set_restart")
+
     def initialize(self):
         check(lib().sfx_model_initialize(self.h), "initialize")
         return self
@@ -314,6 +320,10 @@ class Instance:
     def set(self, name, a):
         a = np.ascontiguousarray(a, dtype=STATE_FIELDS[name])
         check(lib().sfx_set_state(self.h, name.encode(), ptr(a), a.nbytes), f"set {name}")
+
+    def write_restart(self, path):
+        """Type-1 binary restart file of the current state (write_rst_file)."""
+        check(lib().sfx_write_restart(self.h, str(path).encode()), "write_restart")
 
     def compute_fluxes(self, dt):
         out = C.c_float()

@@ -2056,6 +2056,20 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
   return SFX_ERR_ARG;
 }
 
+// write_rst_file (sfincs_output.f90:657-729), type 1: the state is copied to the host
+// and written there.
+extern "C" int sfx_write_restart(sfx_handle* H, const char* path) {
+  if (!H || !path) { set_error("sfx_write_restart: null"); return SFX_ERR_ARG; }
+  const int32_t np = H->hI.np, nq = H->sz.npuv + H->sz.ncuv + 1, nu = H->sz.nkcuv2;
+  std::vector<double> zs((size_t)np);
+  std::vector<float> q((size_t)nq), uvm((size_t)std::max(nu, 1));
+  int rc = sfx_get_state(H, "zs", zs.data(), zs.size() * 8);
+  if (!rc) rc = sfx_get_state(H, "q", q.data(), q.size() * 4);
+  if (!rc && nu > 0) rc = sfx_get_state(H, "uvmean", uvm.data(), (size_t)nu * 4);
+  if (rc) return rc;
+  return sfx::write_restart_arrays(path, zs.data(), np, q.data(), nq, uvm.data(), nu);
+}
+
 extern "C" int sfx_get_time(sfx_handle* H, double* t, float* dt, int64_t* nt) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));

@@ -10,6 +10,10 @@
 namespace sfx {
 
 void set_error(const std::string& msg);
+// Writes a type-1 binary restart file (sfincs_output.f90:657-729) from state arrays in
+// reference order; defined in sfx_model.cpp.
+int write_restart_arrays(const char* path, const double* zs, int32_t np, const float* q, int32_t nq,
+                         const float* uvmean, int32_t nuvmean);
 
 // 1-based reference index -> 0-based storage slot.
 inline int32_t z0(int32_t one_based) { return one_based - 1; }
@@ -75,6 +79,9 @@ struct sfx_model {
   std::vector<int32_t> index_kcuv2, nmbkcuv2, nmikcuv2, ibkcuv2;
   std::vector<int8_t> ibuvdir;
   std::vector<float> uvmean;
+  // binary restart file read before initialisation (applied by sfx_model_initialize)
+  int32_t rst_type = 0;                  // 0 = none, 1/2 = zs, q, uvmean, 3 = zs
+  std::vector<float> rst_zs, rst_q, rst_uvmean;
   std::vector<float> x_bnd, y_bnd, t_bnd, zs_bnd;   // zs_bnd[it*nbnd + ib]
   std::vector<int32_t> ind1_bnd_gbp, ind2_bnd_gbp, nmi_gbp;
   std::vector<float> fac_bnd_gbp;

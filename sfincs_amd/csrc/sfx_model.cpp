@@ -4,6 +4,7 @@
 // reference's so that every index table and mask is bit-exact with it.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -698,19 +699,38 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
       if (M->kcuv[i] > 0) M->gn2uv[i] = (M->zbuv[i] > p.rgh_lev_land) ? gn2land : gn2sea;
   }
 
-  // initialize_hydro + set_initial_conditions (:2195-2398; sfincs_initial_conditions.F90:98-174)
+  // initialize_hydro + set_initial_conditions (:2195-2398; sfincs_initial_conditions.F90:22-174).
+  // Initial levels are zsini, or a restart file's (f32) levels, raised to the bed; a
+  // restart also gives q on the uv-points (not the combined ones) and uvmean.
+  if (M->rst_type && M->rst_zs.size() != (size_t)np) { set_error("restart file: zs has the wrong size"); return SFX_ERR_ARG; }
+  const bool rst_q = M->rst_type == 1 || M->rst_type == 2;
+  if (rst_q && M->rst_q.size() != (size_t)(npuv + ncuv + 1)) { set_error("restart file: q has the wrong size"); return SFX_ERR_ARG; }
+  if (rst_q && M->rst_uvmean.size() != (size_t)M->nkcuv2) { set_error("restart file: uvmean has the wrong size"); return SFX_ERR_ARG; }
   M->zs.resize(np);
   const double zini = (double)p.zsini;
-  for (int32_t c = 0; c < np; ++c)
-    M->zs[c] = M->subgrid ? std::max((double)M->z_zmin[c], zini) : std::max((double)M->zb[c], zini);
+  for (int32_t c = 0; c < np; ++c) {
+    const double z = M->rst_type ? (double)M->rst_zs[c] : zini;
+    M->zs[c] = M->subgrid ? std::max((double)M->z_zmin[c], z) : std::max((double)M->zb[c], z);
+  }
   M->q.assign(npuv + ncuv + 1, 0.f);
   M->uv.assign(npuv + ncuv + 1, 0.f);
+  if (rst_q) {
+    for (int32_t i = 0; i < npuv; ++i) M->q[i] = M->rst_q[i];
+    M->uvmean = M->rst_uvmean;
+  }
   M->kfuv.assign(npuv, 0);
   if (p.wiggle_suppression) {
     M->zs0.resize(np); M->zsderv.assign(np, 0.f);
     for (int32_t c = 0; c < np; ++c) M->zs0[c] = (float)M->zs[c];
   } else {
     M->zs0.assign(np, 0.f); M->zsderv.assign(np, 0.f);
+  }
+  // combined uv-points take the mean of their two points (sfincs_domain.f90:2272-2280).
+  // The uv that set_initial_conditions derives from a restart's q is then discarded:
+  // uv = 0 at :2311 runs after it, so every run starts with uv = 0.
+  for (int32_t j = 0; j < ncuv; ++j) {
+    const int32_t c = M->cuv_index_uv[j] - 1, a = M->cuv_index_uv1[j] - 1, b = M->cuv_index_uv2[j] - 1;
+    M->q[c] = 0.5f * (M->q[a] + M->q[b]);
   }
   M->zsmax.assign(np, -999.0f);
   // sfincs_domain.f90:2300-2306 (qmax is allocated without a value there, :2251; -999 is
@@ -886,6 +906,82 @@ extern "C" int sfx_model_get_table(const sfx_model* M, const char* name, void* d
   if (!M || !name || !dst || !lookup(M, name, v)) { set_error(std::string("unknown table ") + (name ? name : "")); return SFX_ERR_ARG; }
   if (nbytes != v.bytes) { set_error(std::string("size mismatch for ") + name); return SFX_ERR_ARG; }
   if (nbytes) std::memcpy(dst, v.ptr, nbytes);
+  return SFX_OK;
+}
+
+// ---- binary restart files (sfincs_output.f90:657-729, sfincs_initial_conditions.F90:183-260)
+// Fortran sequential unformatted records: a 4-byte length, the payload, the length again.
+// The records are the type (int32), zs (f32, np), q (f32, npuv+ncuv+1) and uvmean (f32,
+// nkcuv2); type 3 holds zs only, types 4-6 add infiltration fields.
+namespace {
+bool read_record(std::FILE* f, std::vector<char>& out) {
+  int32_t n1 = 0, n2 = 0;
+  if (std::fread(&n1, 4, 1, f) != 1 || n1 < 0) return false;
+  out.resize((size_t)n1);
+  if (n1 && std::fread(out.data(), 1, (size_t)n1, f) != (size_t)n1) return false;
+  return std::fread(&n2, 4, 1, f) == 1 && n2 == n1;
+}
+bool write_record(std::FILE* f, const void* p, size_t nbytes) {
+  const int32_t n = (int32_t)nbytes;
+  return std::fwrite(&n, 4, 1, f) == 1 && (nbytes == 0 || std::fwrite(p, 1, nbytes, f) == nbytes) &&
+         std::fwrite(&n, 4, 1, f) == 1;
+}
+std::vector<float> as_floats(const std::vector<char>& r) {
+  std::vector<float> v(r.size() / 4);
+  if (!v.empty()) std::memcpy(v.data(), r.data(), v.size() * 4);
+  return v;
+}
+}  // namespace
+
+extern "C" int sfx_model_set_restart(sfx_model* M, const char* path) {
+  if (!M || !path || !M->mesh_built || M->initialized) {
+    set_error("sfx_model_set_restart: needs a built mesh, before sfx_model_initialize");
+    return SFX_ERR_STATE;
+  }
+  std::FILE* f = std::fopen(path, "rb");
+  if (!f) { set_error(std::string("restart file not found: ") + path); return SFX_ERR_IO; }
+  std::vector<char> r;
+  int32_t type = 0;
+  bool ok = read_record(f, r) && r.size() == 4;
+  if (ok) std::memcpy(&type, r.data(), 4);
+  if (ok && (type < 1 || type > 6)) {   // as the reference: an unknown type is skipped
+    std::fclose(f);
+    M->rst_type = 0;
+    return SFX_OK;
+  }
+  if (ok && type >= 4) {
+    std::fclose(f);
+    set_error("restart file types 4-6 carry infiltration fields, which this build does not model");
+    return SFX_ERR_ARG;
+  }
+  std::vector<float> zs, q, uvm;
+  ok = ok && read_record(f, r) && r.size() == (size_t)M->np * 4;
+  if (ok) zs = as_floats(r);
+  if (ok && type != 3) {
+    ok = read_record(f, r) && r.size() == (size_t)(M->npuv + M->ncuv + 1) * 4;
+    if (ok) q = as_floats(r);
+    ok = ok && read_record(f, r) && r.size() % 4 == 0;
+    if (ok) uvm = as_floats(r);
+  }
+  std::fclose(f);
+  if (!ok) { set_error(std::string("restart file unreadable or sized for another mesh: ") + path); return SFX_ERR_IO; }
+  M->rst_type = type;
+  M->rst_zs = std::move(zs); M->rst_q = std::move(q); M->rst_uvmean = std::move(uvm);
+  return SFX_OK;
+}
+
+// Writes a type-1 restart file from state arrays in reference order (zs f64, np; q f32,
+// npuv+ncuv+1; uvmean f32, nkcuv2); zs is stored as f32, as write_rst_file does.
+int sfx::write_restart_arrays(const char* path, const double* zs, int32_t np, const float* q, int32_t nq,
+                               const float* uvmean, int32_t nuvmean) {
+  std::vector<float> zs4((size_t)np);
+  for (int32_t c = 0; c < np; ++c) zs4[c] = (float)zs[c];
+  std::FILE* f = std::fopen(path, "wb");
+  if (!f) { set_error(std::string("cannot write restart file: ") + path); return SFX_ERR_IO; }
+  const int32_t type = 1;
+  const bool ok = write_record(f, &type, 4) && write_record(f, zs4.data(), zs4.size() * 4) &&
+                  write_record(f, q, (size_t)nq * 4) && write_record(f, uvmean, (size_t)nuvmean * 4);
+  if (std::fclose(f) != 0 || !ok) { set_error(std::string("cannot write restart file: ") + path); return SFX_ERR_IO; }
   return SFX_OK;
 }
 

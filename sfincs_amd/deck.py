@@ -44,6 +44,8 @@ def write(sc, folder):
             lines.append(f"advection_scheme = {'original' if int(v) == 0 else 'upw1'}")
         elif k == "advection":
             lines.append(f"advection = {int(v)}")
+        elif k in ("store_velmax", "store_fluxmax", "store_twet"):   # the input names of the switches
+            lines.append(f"{'store' + k[6:]} = {int(v)}")
         else:
             lines.append(f"{k} = {_fmt(v)}")
     if sc.kind == "regular":

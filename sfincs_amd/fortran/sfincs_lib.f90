@@ -99,6 +99,11 @@ module sfx_capi
          import :: c_int, c_ptr
          type(c_ptr), value :: m
       end function
+      integer(c_int) function sfx_model_set_restart(m, path) bind(C, name='sfx_model_set_restart')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: path(*)
+      end function
       subroutine sfx_model_destroy(m) bind(C, name='sfx_model_destroy')
          import :: c_ptr
          type(c_ptr), value :: m
@@ -313,7 +318,7 @@ contains
       integer :: nmax, mmax, np, ios, u, i, j, n, m, nt, nb, rc
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
-      character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, s
+      character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, rstfile, s
       integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
       integer(c_int32_t), allocatable :: indices(:)
       real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:)
@@ -376,6 +381,7 @@ contains
       call inp_str('bndfile', bndfile, 'none'); call inp_str('bzsfile', bzsfile, 'none')
       call inp_str('srcfile', srcfile, 'none'); call inp_str('disfile', disfile, 'none')
       call inp_str('prcfile', prcfile, 'none'); call inp_str('wndfile', wndfile, 'none')
+      call inp_str('rstfile', rstfile, 'none')
 
       if (qtrfile(1:4) /= 'none') then
          call read_quadtree(qtrfile, mskfile, p, rc)
@@ -524,6 +530,15 @@ contains
          rc = sfx_model_set_wind(model, int(nt, c_int32_t), c_loc(tt), c_loc(v1), c_loc(v2))
          if (rc < 0) call fail('wind')
          deallocate(tt, v1, v2)
+      endif
+      if (rstfile(1:4) /= 'none') then   ! binary restart (sfincs_initial_conditions.F90:44-54)
+         call logmsg('Info    : reading restart file '//trim(rstfile))
+         rc = sfx_model_set_restart(model, trim(rstfile)//c_null_char)
+         if (rc < 0) then
+            call fail('restart file')
+            ierr = -1
+            return
+         endif
       endif
       rc = sfx_model_initialize(model)
       if (rc < 0) then

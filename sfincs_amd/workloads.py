@@ -38,6 +38,7 @@ class Scenario:
     rain: tuple = None                    # (t, mm/h)
     wind: tuple = None                    # (t, mag, dir)
     t_end: float = 3600.0
+    rstfile: str = None                   # binary restart file as initial condition
 
     def build_model(self):
         p = _lib.default_params(**self.params)
@@ -59,6 +60,8 @@ class Scenario:
             m.set_rain(*self.rain)
         if self.wind is not None:
             m.set_wind(*self.wind)
+        if self.rstfile is not None:
+            m.set_restart(self.rstfile)
         m.initialize()
         return m
 

@@ -60,13 +60,16 @@ def _table(lib, m, name):
 
 
 @needs_host
-@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
-    model that the Python builder makes from the same scenario."""
+    model that the Python builder makes from the same scenario (parameters included)."""
     from sfincs_amd import _lib
     if kind == "regular":
         sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0)
+    elif kind == "store_switches":
+        sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, store_velmax=1, store_fluxmax=1, store_twet=1,
+                            twet_threshold=0.02)
     elif kind == "quadtree":
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     else:
@@ -112,3 +115,37 @@ def test_fortran_host_matches_python_path(kind, tmp_path):
     zs_p = inst.get("zs")
     assert zs_f.shape == zs_p.shape
     assert np.array_equal(zs_f, zs_p)
+
+
+@needs_host
+def test_deck_reader_applies_the_restart_file(tmp_path):
+    """`rstfile` in sfincs.inp: the Fortran host reads the binary restart file into the same
+    initial state as the Python builder does (no GPU)."""
+    from sfincs_amd import _lib
+    from test_restart import write_rst
+    sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=44, t_end=900.0)
+    d = deck.write(sc, str(tmp_path))
+    base = sc.build_model()
+    s = base.sizes()
+    r = np.random.default_rng(5)
+    rst = os.path.join(d, "sfincs.rst")
+    write_rst(rst, 1, base.table("zb") + r.uniform(-0.5, 1.0, s["np"]),
+              r.uniform(-0.2, 0.2, s["npuv"] + s["ncuv"] + 1), r.uniform(-0.1, 0.1, s["nkcuv2"]))
+    with open(os.path.join(d, "sfincs.inp"), "a") as f:
+        f.write("rstfile          = sfincs.rst\n")
+    host = ctypes.CDLL(HOSTLIB)
+    host.sfincs_host_model.restype = ctypes.c_void_p
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        assert host.sfincs_host_read_model() == 0
+    finally:
+        os.chdir(cwd)
+    mf = host.sfincs_host_model()
+    sc.rstfile = rst
+    mp = sc.build_model()
+    lib = _lib.lib()
+    for name in ("zs", "q", "uv", "zs0", "uvmean"):
+        a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
+        assert a is not None and a == b, name
+    assert _table(lib, mf, "zs") != _table(lib, base.h.value, "zs")

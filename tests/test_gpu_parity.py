@@ -398,3 +398,53 @@ def test_store_variables_off_are_not_stored():
     for f in ("vmax", "qmax", "twet"):
         with pytest.raises(Exception):
             inst.get(f)
+
+
+def _records(raw):
+    out, i = [], 0
+    while i < len(raw):
+        n = int(np.frombuffer(raw[i:i + 4], np.int32)[0])
+        out.append(raw[i + 4:i + 4 + n])
+        assert int(np.frombuffer(raw[i + 4 + n:i + 8 + n], np.int32)[0]) == n
+        i += n + 8
+    return out
+
+
+RESTART_CASES = [
+    ("subgrid", lambda: W.subgrid_case(32, 40, dx=25.0, seed=76, t_end=1800.0, nsrc=2, tide_amp=2.0)),
+    ("quadtree", lambda: W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=77, t_end=1800.0, surge_amp=4.0)),
+]
+
+
+@pytest.mark.parametrize("name,make", RESTART_CASES, ids=[c[0] for c in RESTART_CASES])
+def test_restart_round_trip_matches_oracle(name, make, tmp_path):
+    """sfx_write_restart after 900 s holds the instance's state in write_rst_file's layout
+    (type 1; zs as f32, q, uvmean; exact); a model initialised from that file then runs
+    900 s on the GPU as on the oracle (parity tolerances)."""
+    print(f"case restart round trip {name}", flush=True)
+    sc = make()
+    m = sc.build_model()
+    inst = gpu_for(m)
+    inst.update(900.0)
+    path = tmp_path / "sfincs.rst"
+    inst.write_restart(path)
+    rec = _records(path.read_bytes())
+    assert len(rec) == 4 and int(np.frombuffer(rec[0], np.int32)[0]) == 1
+    assert np.array_equal(np.frombuffer(rec[1], np.float32), inst.get("zs").astype(np.float32))
+    assert np.array_equal(np.frombuffer(rec[2], np.float32), inst.get("q"))
+    assert np.array_equal(np.frombuffer(rec[3], np.float32), inst.get("uvmean"))
+    assert np.any(inst.get("q"))
+    sc2 = make()
+    sc2.rstfile = str(path)
+    sc2.t_end = 900.0
+    sc2.params["t1"] = 900.0
+    m2, orc = oracle_for(sc2)
+    assert np.array_equal(m2.table("zs"), np.maximum(m2.table("z_zmin" if m2.sizes()["subgrid"] else "zb"),
+                                                     inst.get("zs").astype(np.float32)).astype(np.float64))
+    inst2 = gpu_for(m2)
+    rc, st = inst2.update(-1000.0)
+    orc.update(sc2.t_end)
+    d = compare_state(inst2, orc)
+    print(name, st["nt"], d, flush=True)
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL

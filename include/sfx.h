@@ -140,6 +140,13 @@ int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const fl
 int  sfx_model_set_sources(sfx_model* m, int32_t nsrc, const float* x, const float* y,
                            int32_t nt, const float* t, const float* q);
 /* prcfile: uniform rain series in mm/h. */
+/* Downstream river boundary points for msk=5 cells (bdrfile; sfincs_boundaries.f90:217-285,
+ * 587-613, 878-901): per point x, y, an interior point x_in, y_in, the slope and the
+ * distance to use (< 0: the distance between the two points). Each river cell takes the
+ * level of its nearest point's interior cell plus dzs = -slope*distance, raised to the bed.
+ * The interior point must lie in an active interior cell (kcs 1). */
+int  sfx_model_set_river_boundary(sfx_model* m, int32_t nbdr, const float* x, const float* y, const float* x_in,
+                                  const float* y_in, const float* slope, const float* distance);
 int  sfx_model_set_rain(sfx_model* m, int32_t nt, const float* t, const float* mmh);
 /* wndfile: uniform wind series, speed m/s and nautical direction in degrees. */
 int  sfx_model_set_wind(sfx_model* m, int32_t nt, const float* t, const float* mag, const float* dir);

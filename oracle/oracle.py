@@ -29,7 +29,8 @@ TABLES = (
     "uv_index_v_ndm", "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
     "cuv_index_uv2", "kcuv", "mask_adv", "uv_flags_iref", "uv_flags_type", "uv_flags_dir",
     "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp", "ind2_bnd_gbp", "nmi_gbp",
-    "ibuvdir", "fac_bnd_gbp", "t_bnd", "zs_bnd", "nmindsrc", "tsrc", "qsrc", "tprcpt", "tprcpv", "twnd",
+    "ibuvdir", "fac_bnd_gbp", "index_bdr_gbp", "index_zsi_bdr", "dzs_bdr", "t_bnd", "zs_bnd", "nmindsrc", "tsrc",
+    "qsrc", "tprcpt", "tprcpv", "twnd",
     "wndmag", "wnddir", "cdval",
 )
 REGULAR_TABLES = ("zb", "zbuv", "zbuvmx", "gn2uv")

@@ -54,6 +54,8 @@ typedef struct OM {
   const int32_t *nmindbnd, *index_kcuv2, *nmbkcuv2, *nmikcuv2, *ibkcuv2, *ind1_bnd_gbp, *ind2_bnd_gbp, *nmi_gbp;
   const int8_t *ibuvdir;
   const float *fac_bnd_gbp, *t_bnd, *zs_bnd;
+  const int32_t *index_bdr_gbp, *index_zsi_bdr;   /* river cells: bdr point per cell, interior cell per point */
+  const float *dzs_bdr;
   float *zsb, *zsb0, *uvmean, *zst_bnd;
   /* sources, meteo */
   const int32_t *nmindsrc;
@@ -90,7 +92,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(vmax) BIND(qmax) BIND(twet)
   BIND(tauwu) BIND(tauwv) BIND(netprcp)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
-  BIND(nmi_gbp) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
+  BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
   BIND(nmindsrc) BIND(tsrc) BIND(qsrc) BIND(tprcpt) BIND(tprcpv) BIND(twnd) BIND(wndmag) BIND(wnddir) BIND(cdval) BIND(qtsrc) BIND(x73)
   return -1;
 }
@@ -516,6 +518,12 @@ static void update_boundary_conditions(OM* m, double t, float dt) {
         m->zsb0[ib] = fmx(m->zsb0[ib], m->zb[nmb]);
         m->zsb[ib] = fmx(m->zsb[ib], m->zb[nmb]);
       }
+    } else if (k == 5) {   /* downstream river point (sfincs_boundaries.f90:878-901) */
+      const int32_t j = m->index_bdr_gbp[ib] - 1;
+      double zst = m->zs[m->index_zsi_bdr[j] - 1] + (double)m->dzs_bdr[j];
+      zst = dmax(zst, m->subgrid ? (double)m->z_zmin[nmb] : (double)m->zb[nmb]);
+      m->zsb[ib] = (float)zst;
+      m->zsb0[ib] = (float)zst;
     } else if (k == 6) {
       m->zs[nmb] = m->zs[m->nmi_gbp[ib] - 1];
     }

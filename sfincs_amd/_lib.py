@@ -74,6 +74,7 @@ def _sig(lib):
         "sfx_model_set_boundary": (I, [P, i32, P, P, i32, P, P]),
         "sfx_model_set_sources": (I, [P, i32, P, P, i32, P, P]),
         "sfx_model_set_rain": (I, [P, i32, P, P]),
+        "sfx_model_set_river_boundary": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_set_wind": (I, [P, i32, P, P, P]),
         "sfx_model_initialize": (I, [P]),
         "sfx_model_set_restart": (I, [P, s]),
@@ -162,7 +163,7 @@ TABLE_DTYPES = {
             "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm", "uv_index_u_num", "uv_index_v_ndm",
             "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
             "cuv_index_uv2", "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp",
-            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc"),
+            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc", "index_bdr_gbp", "index_zsi_bdr"),
 }
 
 
@@ -254,6 +255,12 @@ class Model:
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)
         check(lib().sfx_model_set_rain(self.h, len(t), ptr(t), ptr(v)), "set_rain")
+
+    def set_river_boundary(self, x, y, x_in, y_in, slope, distance):
+        """Downstream river boundary points (bdr file rows) for msk=5 cells."""
+        a = [np.ascontiguousarray(v, np.float32) for v in (x, y, x_in, y_in, slope, distance)]
+        self._keep_bdr = a
+        check(lib().sfx_model_set_river_boundary(self.h, len(a[0]), *[ptr(v) for v in a]), "set_river_boundary")
 
     def set_wind(self, t, mag, direction):
         t = np.ascontiguousarray(t, np.float32); a = np.ascontiguousarray(mag, np.float32)

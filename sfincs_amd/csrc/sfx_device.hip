@@ -141,6 +141,7 @@ struct DevInst {
   // boundaries
   int32_t ngbnd, nkcuv2, nbnd, ntbnd;
   const int32_t* gb_cell; const int32_t* gb_ind1; const int32_t* gb_ind2; const float* gb_fac; const int32_t* gb_nmi;
+  const int32_t* gb_rcell; const float* gb_rdzs;   // river cells (kcs 5): interior cell and level offset
   float* zsb; float* zsb0;
   const int32_t* k2_slot; const int32_t* k2_nmb; const int32_t* k2_nmi; const int32_t* k2_ib; const int8_t* k2_dir;
   const float* k2_zbuv;
@@ -590,6 +591,13 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
       const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
       zsb0 = sfx::vmax<float>(zb0, floorz);
       zsb1 = sfx::vmax<float>(zb1, floorz);
+    } else if (kb == 5) {
+      // downstream river cell (:878-901): the interior level plus the slope offset, raised
+      // to the bed, in f64 (the interior cell is an ordinary cell, which no boundary step writes)
+      const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
+      const double zst = sfx::vmax<double>(G(I.zs)[G(I.gb_rcell)[indb]] + (double)G(I.gb_rdzs)[indb], (double)floorz);
+      zsb0 = (float)zst;
+      zsb1 = zsb0;
     } else {
       zsb1 = G(I.zsb)[indb];
       zsb0 = G(I.zsb0)[indb];
@@ -1286,6 +1294,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
   const R zsmax_in = zsmax;
   if (SUBGRID) {
     double vol = G(I.vol)[c];
+    const double vol_in = vol;   // as loaded: a source alone changes the volume too
     if (kraw & KCS_SOURCE) {   // point sources of this cell, in source order (:311-325)
       for (int s = 0; s < I.nsrc; ++s)
         if (G(I.src_cell)[s] == c) {
@@ -1296,7 +1305,6 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
     const auto zs0p = G((R*)I.zs0);
     const auto zsdp = G((R*)I.zsderv);
     R zs0 = zs0p[c], zsd = zsdp[c];
-    const double vol_in = vol;
     const R zs0_in = zs0, zsd_in = zsd;
     const int nl = I.c.nl;
     sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, G(I.z_zmin)[c],
@@ -1774,6 +1782,14 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.gb_cell = H->upload(cell.data(), cell.size()); I.gb_nmi = H->upload(nmi.data(), nmi.size());
     I.gb_ind1 = H->upload(M->ind1_bnd_gbp.data(), M->ngbnd); I.gb_ind2 = H->upload(M->ind2_bnd_gbp.data(), M->ngbnd);
     I.gb_fac = H->upload(M->fac_bnd_gbp.data(), M->ngbnd);
+    std::vector<int32_t> rcell(M->ngbnd, -1);
+    std::vector<float> rdzs(M->ngbnd, 0.f);
+    for (int i = 0; i < M->ngbnd; ++i)
+      if (M->index_bdr_gbp[i] > 0) {
+        rcell[i] = M->index_zsi_bdr[M->index_bdr_gbp[i] - 1] - 1;
+        rdzs[i] = M->dzs_bdr[M->index_bdr_gbp[i] - 1];
+      }
+    I.gb_rcell = H->upload(rcell.data(), rcell.size()); I.gb_rdzs = H->upload(rdzs.data(), rdzs.size());
     I.zsb = H->upload(M->zsb.data(), M->ngbnd); I.zsb0 = H->upload(M->zsb0.data(), M->ngbnd);
     std::vector<int32_t> slot(M->nkcuv2), nmb(M->nkcuv2), nmi2(M->nkcuv2), ib(M->nkcuv2);
     std::vector<float> zbuv(M->nkcuv2, 0.f);
@@ -1789,7 +1805,10 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     // Neumann copies (kcs 6) run before the flux points only when they share a cell
     std::vector<uint8_t> fw(M->np, 0), fr(M->np, 0);
     for (int i = 0; i < M->nkcuv2; ++i)
-      if (M->kcs[nmb[i]] != 6) { fw[nmb[i]] = 1; fr[nmi2[i]] = 1; }
+      if (M->kcs[nmb[i]] != 6) {
+        fw[nmb[i]] = 1; fr[nmi2[i]] = 1;
+        if (ib[i] >= 0 && rcell[ib[i]] >= 0) fr[rcell[ib[i]]] = 1;   // a river cell's interior level
+      }
     I.bnd_overlap = 0;
     for (int i = 0; i < M->ngbnd; ++i)
       if (M->kcs[cell[i]] == 6 && (fr[cell[i]] || (nmi[i] >= 0 && fw[nmi[i]]))) I.bnd_overlap = 1;

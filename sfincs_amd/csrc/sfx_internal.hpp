@@ -85,6 +85,10 @@ struct sfx_model {
   std::vector<float> x_bnd, y_bnd, t_bnd, zs_bnd;   // zs_bnd[it*nbnd + ib]
   std::vector<int32_t> ind1_bnd_gbp, ind2_bnd_gbp, nmi_gbp;
   std::vector<float> fac_bnd_gbp;
+  // downstream river boundaries (bdr file, sfincs_boundaries.f90:217-285, 587-613)
+  int32_t nbdr = 0;
+  std::vector<float> x_bdr, y_bdr, x_bdr_in, y_bdr_in, dzs_bdr;
+  std::vector<int32_t> index_zsi_bdr, index_bdr_gbp;   // 1-based cell / bdr point (0 = none)
 
   // point sources
   int32_t nsrc = 0, ntsrc = 0;

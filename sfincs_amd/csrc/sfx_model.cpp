@@ -607,6 +607,31 @@ extern "C" int sfx_model_set_boundary(sfx_model* M, int32_t nbnd, const float* x
   return SFX_OK;
 }
 
+// Downstream river boundary points (bdrfile, sfincs_boundaries.f90:217-285): per point the
+// boundary location, an interior point, the slope and the distance between them (< 0: the
+// distance of the two points). dzs = -slope * distance.
+extern "C" int sfx_model_set_river_boundary(sfx_model* M, int32_t nbdr, const float* x, const float* y,
+                                            const float* x_in, const float* y_in, const float* slope,
+                                            const float* distance) {
+  if (!M || M->initialized || nbdr < 0 || (nbdr > 0 && (!x || !y || !x_in || !y_in || !slope || !distance))) {
+    set_error("sfx_model_set_river_boundary: bad args");
+    return SFX_ERR_ARG;
+  }
+  M->nbdr = nbdr;
+  M->x_bdr.assign(x, x + nbdr); M->y_bdr.assign(y, y + nbdr);
+  M->x_bdr_in.assign(x_in, x_in + nbdr); M->y_bdr_in.assign(y_in, y_in + nbdr);
+  M->dzs_bdr.resize(nbdr);
+  for (int32_t j = 0; j < nbdr; ++j) {
+    if (distance[j] < 0.0f) {
+      const float ddx = x_in[j] - x[j], ddy = y_in[j] - y[j];
+      M->dzs_bdr[j] = -slope[j] * std::sqrt(ddx * ddx + ddy * ddy);
+    } else {
+      M->dzs_bdr[j] = -slope[j] * distance[j];
+    }
+  }
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_set_sources(sfx_model* M, int32_t nsrc, const float* x, const float* y,
                                      int32_t nt, const float* t, const float* q) {
   if (!M || M->initialized || nsrc < 0 || (nsrc > 0 && (!x || !y || !t || !q || nt < 2))) {
@@ -645,11 +670,14 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
 
   // initialize_boundaries (sfincs_domain.f90:1786-1956)
   M->ngbnd = 0; M->nmindbnd.clear();
-  bool has2 = false;
+  bool has2 = false, has5 = false;
   for (int32_t c = 0; c < np; ++c) {
     const int k = M->kcs[c];
-    if (k == 5) { set_error("downstream river boundaries (msk=5) are not supported"); return SFX_ERR_ARG; }
-    if (k == 2 || k == 3 || k == 6) { M->nmindbnd.push_back(c + 1); if (k == 2) has2 = true; }
+    if (k == 2 || k == 3 || k == 5 || k == 6) {
+      M->nmindbnd.push_back(c + 1);
+      if (k == 2) has2 = true;
+      if (k == 5) has5 = true;
+    }
     if (k == 4) { set_error("wave maker cells (msk=4) are not supported"); return SFX_ERR_ARG; }
   }
   M->ngbnd = (int32_t)M->nmindbnd.size();
@@ -763,6 +791,9 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
   M->ind1_bnd_gbp.assign(M->ngbnd, 0); M->ind2_bnd_gbp.assign(M->ngbnd, 0);
   M->fac_bnd_gbp.assign(M->ngbnd, 0.f); M->nmi_gbp.assign(M->ngbnd, 0);
   if (has2 && M->nbnd == 0) { set_error("water level boundary cells (msk=2) need boundary points"); return SFX_ERR_ARG; }
+  // (sfincs_boundaries.f90:272-275: river cells without a bdr file stop the model)
+  if (has5 && M->nbdr == 0) { set_error("downstream river boundary cells (msk=5) need a bdr file"); return SFX_ERR_ARG; }
+  M->index_bdr_gbp.assign(M->ngbnd, 0);
   for (int32_t ib = 0; ib < M->ngbnd; ++ib) {
     const int32_t nm = M->nmindbnd[ib];
     const float x = M->z_xz[nm - 1], y = M->z_yz[nm - 1];
@@ -781,6 +812,15 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
       } else {
         M->ind1_bnd_gbp[ib] = 1; M->ind2_bnd_gbp[ib] = 1; M->fac_bnd_gbp[ib] = 1.0f;
       }
+    } else if (k == 5) {   // nearest bdr point (:587-613)
+      float dst1 = 1.0e10f; int32_t ib1 = 0;
+      for (int32_t j = 0; j < M->nbdr; ++j) {
+        const float ddx = M->x_bdr[j] - x, ddy = M->y_bdr[j] - y;
+        const float dst = std::sqrt(ddx * ddx + ddy * ddy);
+        if (dst < dst1) { dst1 = dst; ib1 = j + 1; }
+      }
+      if (ib1 == 0) { set_error("a downstream river boundary cell has no bdr point within 1e10 m"); return SFX_ERR_ARG; }
+      M->index_bdr_gbp[ib] = ib1;
     } else if (k == 6) {
       const int32_t n = M->z_index_z_n[nm - 1], m = M->z_index_z_m[nm - 1];
       const int iref = M->z_flags_iref[nm - 1];
@@ -791,6 +831,19 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
       }
       if (M->nmi_gbp[ib] == 0) M->kcs[nm - 1] = 0;
     }
+  }
+
+  // the cell holding each bdr interior point (find_quadtree_cell, :248). Its level is read
+  // while the boundary levels are set, so it must be an ordinary cell, which no boundary
+  // step writes (the reference does not check this; outside the mesh it reads zs(0)).
+  M->index_zsi_bdr.assign(M->nbdr, 0);
+  for (int32_t j = 0; j < M->nbdr; ++j) {
+    const int32_t c = M->find_cell_xy(M->x_bdr_in[j], M->y_bdr_in[j]);
+    if (c == 0 || M->kcs[c - 1] != 1) {
+      set_error("a river boundary's interior point must lie in an active interior cell (kcs 1)");
+      return SFX_ERR_ARG;
+    }
+    M->index_zsi_bdr[j] = c;
   }
 
   // read_discharges cell lookup (sfincs_discharges.f90:159-171)
@@ -888,6 +941,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv) T(vmax) T(qmax) T(twet)
   T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
   T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval) T(x73)
+  T(index_bdr_gbp) T(index_zsi_bdr) T(dzs_bdr)
   T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)
 #undef T
   return false;

@@ -83,6 +83,13 @@ module sfx_capi
          integer(c_int32_t), value :: nsrc, nt
          type(c_ptr), value :: x, y, t, q
       end function
+      integer(c_int) function sfx_model_set_river_boundary(m, nbdr, x, y, xi, yi, slope, dist) &
+            bind(C, name='sfx_model_set_river_boundary')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: nbdr
+         type(c_ptr), value :: x, y, xi, yi, slope, dist
+      end function
       integer(c_int) function sfx_model_set_rain(m, nt, t, v) bind(C, name='sfx_model_set_rain')
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: m
@@ -318,7 +325,7 @@ contains
       integer :: nmax, mmax, np, ios, u, i, j, n, m, nt, nb, rc
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
-      character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, rstfile, s
+      character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
       integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
       integer(c_int32_t), allocatable :: indices(:)
       real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:)
@@ -381,7 +388,7 @@ contains
       call inp_str('bndfile', bndfile, 'none'); call inp_str('bzsfile', bzsfile, 'none')
       call inp_str('srcfile', srcfile, 'none'); call inp_str('disfile', disfile, 'none')
       call inp_str('prcfile', prcfile, 'none'); call inp_str('wndfile', wndfile, 'none')
-      call inp_str('rstfile', rstfile, 'none')
+      call inp_str('rstfile', rstfile, 'none'); call inp_str('bdrfile', bdrfile, 'none')
 
       if (qtrfile(1:4) /= 'none') then
          call read_quadtree(qtrfile, mskfile, p, rc)
@@ -505,6 +512,26 @@ contains
                                     c_loc(tt), c_loc(zsb))
          if (rc < 0) call fail('discharges')
          deallocate(tt)
+      endif
+      ! downstream river points: x y x_in y_in slope distance (sfincs_boundaries.f90:221-268)
+      if (bdrfile(1:4) /= 'none') then
+         nb = count_lines(bdrfile)
+         if (allocated(tt)) deallocate(tt)
+         if (allocated(v1)) deallocate(v1)
+         if (allocated(v2)) deallocate(v2)
+         if (allocated(tmp)) deallocate(tmp)
+         allocate(tt(nb), v1(nb), v2(nb), tmp(nb))
+         if (allocated(xb)) deallocate(xb, yb)
+         allocate(xb(nb), yb(nb))
+         open(newunit=u, file=trim(bdrfile), status='old', action='read')
+         do i = 1, nb
+            read(u, *) xb(i), yb(i), tt(i), v1(i), v2(i), tmp(i)
+         enddo
+         close(u)
+         rc = sfx_model_set_river_boundary(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), c_loc(tt), c_loc(v1), &
+                                           c_loc(v2), c_loc(tmp))
+         if (rc < 0) call fail('river boundaries')
+         deallocate(tt, v1, v2, tmp)
       endif
       ! uniform rain and wind series (sfincs_meteo.f90:247-312)
       if (prcfile(1:4) /= 'none') then

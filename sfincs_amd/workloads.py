@@ -39,6 +39,7 @@ class Scenario:
     wind: tuple = None                    # (t, mag, dir)
     t_end: float = 3600.0
     rstfile: str = None                   # binary restart file as initial condition
+    bdr: tuple = None                     # river boundary points (x, y, x_in, y_in, slope, distance)
 
     def build_model(self):
         p = _lib.default_params(**self.params)
@@ -60,6 +61,8 @@ class Scenario:
             m.set_rain(*self.rain)
         if self.wind is not None:
             m.set_wind(*self.wind)
+        if self.bdr is not None:
+            m.set_river_boundary(*self.bdr)
         if self.rstfile is not None:
             m.set_restart(self.rstfile)
         m.initialize()
@@ -270,6 +273,21 @@ def regular_case(nmax, mmax, dx=50.0, seed=1000, t_end=3600.0, rain_mmh=20.0, ra
                     dx=dx, dy=dx, msk=msk.reshape(-1), zb_grid=zb,
                     bnd=(np.array([-dx / 2], np.float32), np.array([nmax * dx / 2], np.float32), t, zs),
                     rain=rain, wind=wind, t_end=float(t_end))
+
+
+def with_river(sc, first=1, ncells=8, slope=1.0e-3):
+    """Turns cells `first`..`first+ncells-1` of the south row (n = 0) of a regular-kind
+    scenario into downstream river boundary cells (msk=5), forced by two bdr points whose
+    interior points lie two cells inside (distance < 0: taken from the two points)."""
+    msk = sc.msk.reshape(sc.mmax, sc.nmax).copy()
+    msk[first:first + ncells, 0] = 5
+    sc.msk = msk.reshape(-1)
+    xs = np.array([(first + 1.5) * sc.dx, (first + ncells - 1.5) * sc.dx], np.float32)
+    ys = np.full(2, -0.5 * sc.dy, np.float32)
+    sc.bdr = (xs, ys, xs, np.full(2, 2.5 * sc.dy, np.float32), np.full(2, slope, np.float32),
+              np.full(2, -1.0, np.float32))
+    sc.name += "_river"
+    return sc
 
 
 def subgrid_case(nmax, mmax, dx=25.0, refi=5, nl=10, seed=1001, t_end=21600.0, tide_amp=1.5,

@@ -30,3 +30,15 @@ def compare_state(inst, orc, fields=("zs", "q", "uv", "zsmax")):
     for f in fields:
         out[f] = maxdiff(inst.get(f), orc.state(f))
     return out
+
+
+SUBGRID_FIELDS = ("zs0", "zsderv")
+
+
+def compare_subgrid_state(inst, orc):
+    """The subgrid cell state beside zs: the volume (largest difference relative to
+    1 + |V|) and the wiggle-suppression levels zs0 / zsderv."""
+    va, vb = inst.get("z_volume"), orc.state("z_volume")
+    out = {"z_volume_rel": float(np.max(np.abs(va - vb) / (1.0 + np.abs(vb)))) if va.size else 0.0}
+    out.update(compare_state(inst, orc, SUBGRID_FIELDS))
+    return out

@@ -5,7 +5,7 @@ and masks are compared bit-exactly in test_tables.py."""
 import numpy as np
 import pytest
 
-from helpers import Q_TOL, ZS_TOL, compare_state, gpu_for, oracle_for
+from helpers import Q_TOL, ZS_TOL, compare_state, compare_subgrid_state, gpu_for, oracle_for
 from sfincs_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -28,11 +28,15 @@ def test_single_steps_match_oracle(name, sc):
         rc, st = inst.update(0.0)           # one step (sfincs_update with dtrange <= 0)
         orc.update(1e30, 1)
         d = compare_state(inst, orc)
+        if m.sizes()["subgrid"]:
+            d.update(compare_subgrid_state(inst, orc))
         for k, v in d.items():
             worst[k] = max(worst.get(k, 0.0), v)
         assert abs(st["t"] - orc.get("t")) < 1e-6, (step, st["t"], orc.get("t"))
     print(name, worst, flush=True)
     assert worst["zs"] <= ZS_TOL and worst["q"] <= Q_TOL
+    if m.sizes()["subgrid"]:   # volumes to 1e-7 relative; zs0 / zsderv as zs
+        assert worst["z_volume_rel"] <= 1e-7 and worst["zs0"] <= ZS_TOL and worst["zsderv"] <= ZS_TOL
 
 
 @pytest.mark.parametrize("name,sc", _cases(), ids=[c[0] for c in _cases()])
@@ -43,9 +47,13 @@ def test_full_period_matches_oracle(name, sc):
     rc, st = inst.update(-1000.0)           # run to t1 (graph-captured loop)
     orc.update(sc.t_end)
     d = compare_state(inst, orc)
+    if m.sizes()["subgrid"]:
+        d.update(compare_subgrid_state(inst, orc))
     print(name, st["nt"], int(orc.get("nt")), d, flush=True)
     assert st["nt"] == int(orc.get("nt"))
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+    if m.sizes()["subgrid"]:
+        assert d["z_volume_rel"] <= 1e-7 and d["zs0"] <= ZS_TOL and d["zsderv"] <= ZS_TOL
 
 
 def _masked_case():
@@ -321,9 +329,13 @@ def test_state_write_over_long_dry_points_matches_oracle(mesh):
     inst.update(-1000.0)
     orc.update(sc.t_end)
     d = compare_state(inst, orc)
+    if m.sizes()["subgrid"]:
+        d.update(compare_subgrid_state(inst, orc))
     print(d, flush=True)
     assert inst.time()[2] == int(orc.get("nt"))
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+    if m.sizes()["subgrid"]:
+        assert d["z_volume_rel"] <= 1e-7 and d["zs0"] <= ZS_TOL and d["zsderv"] <= ZS_TOL
 
 
 QUIET = [(-1, (16, 16)), (-2, (64, 4)), (-4, (32, 8))]

@@ -140,6 +140,11 @@ int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const fl
 int  sfx_model_set_sources(sfx_model* m, int32_t nsrc, const float* x, const float* y,
                            int32_t nt, const float* t, const float* q);
 /* prcfile: uniform rain series in mm/h. */
+/* Incoming infragravity water levels at the boundary points (bzifile;
+ * sfincs_boundaries.f90:126-140, 748-753, 829-866): zsi[it*nbnd + ib] at the times of the
+ * series given to sfx_model_set_boundary (call after it). They are added to the total
+ * boundary level but not to the still level, so they drive the boundary flux. */
+int  sfx_model_set_boundary_ig(sfx_model* m, const float* zsi);
 /* Downstream river boundary points for msk=5 cells (bdrfile; sfincs_boundaries.f90:217-285,
  * 587-613, 878-901): per point x, y, an interior point x_in, y_in, the slope and the
  * distance to use (< 0: the distance between the two points). Each river cell takes the

@@ -29,7 +29,8 @@ TABLES = (
     "uv_index_v_ndm", "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
     "cuv_index_uv2", "kcuv", "mask_adv", "uv_flags_iref", "uv_flags_type", "uv_flags_dir",
     "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp", "ind2_bnd_gbp", "nmi_gbp",
-    "ibuvdir", "fac_bnd_gbp", "index_bdr_gbp", "index_zsi_bdr", "dzs_bdr", "t_bnd", "zs_bnd", "nmindsrc", "tsrc",
+    "ibuvdir", "fac_bnd_gbp", "index_bdr_gbp", "index_zsi_bdr", "dzs_bdr", "t_bnd", "zs_bnd", "zsi_bnd", "nmindsrc",
+    "tsrc",
     "qsrc", "tprcpt", "tprcpv", "twnd",
     "wndmag", "wnddir", "cdval",
 )
@@ -119,6 +120,7 @@ class OracleRun:
         self._bind("zsb0", model.table("zsb0").copy())
         self._bind("uvmean", model.table("uvmean").copy())
         self._bind("zst_bnd", np.zeros(max(1, s["nbnd"]), np.float32))
+        self._bind("zsit_bnd", np.zeros(max(1, s["nbnd"]), np.float32))
         self._bind("qtsrc", np.zeros(max(1, s["nsrc"]), np.float32))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
@@ -156,6 +158,7 @@ class OracleRun:
                     twet_threshold=p.twet_threshold,
                     wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
+                    has_bzi=int(len(tb("zsi_bnd")) > 0),
                     nsrc=s["nsrc"], ntsrc=len(tb("tsrc")), ntprcp=len(tb("tprcpt")), ntwnd=len(tb("twnd")),
                     g=p.g, pi=p.pi, huthresh=(0.0 if s["subgrid"] else p.huthresh), huvmin=p.huvmin,
                     advlim=p.advlim, slopelim=p.slopelim, uvlim=p.uvlim, nuviscfac=p.nuviscfac, fcorio=fcorio,

@@ -56,6 +56,9 @@ typedef struct OM {
   const float *fac_bnd_gbp, *t_bnd, *zs_bnd;
   const int32_t *index_bdr_gbp, *index_zsi_bdr;   /* river cells: bdr point per cell, interior cell per point */
   const float *dzs_bdr;
+  const float *zsi_bnd;   /* bzi infragravity levels (NULL: none), as zs_bnd */
+  float *zsit_bnd;        /* their value at the current time, nbnd */
+  int32_t has_bzi;
   float *zsb, *zsb0, *uvmean, *zst_bnd;
   /* sources, meteo */
   const int32_t *nmindsrc;
@@ -92,7 +95,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(vmax) BIND(qmax) BIND(twet)
   BIND(tauwu) BIND(tauwv) BIND(netprcp)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
-  BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
+  BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsi_bnd) BIND(zsit_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
   BIND(nmindsrc) BIND(tsrc) BIND(qsrc) BIND(tprcpt) BIND(tprcpv) BIND(twnd) BIND(wndmag) BIND(wnddir) BIND(cdval) BIND(qtsrc) BIND(x73)
   return -1;
 }
@@ -105,7 +108,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
   SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error)
-  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold)
+  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
   SETF(btfilter) SETF(btrelax) SETF(t0) SETF(t1) SETF(tspinup) SETF(zini) SETF(dx) SETF(dy) SETF(dt) SETF(min_dt)
@@ -484,6 +487,10 @@ static void update_boundary_points(OM* m, double t) {
   for (int32_t ib = 0; ib < m->nbnd; ++ib) {
     const float a = m->zs_bnd[(size_t)(itb0 - 1) * m->nbnd + ib], b = m->zs_bnd[(size_t)(itb1 - 1) * m->nbnd + ib];
     m->zst_bnd[ib] = a + (b - a) * tbfac;
+    if (m->has_bzi) {   /* :748-753 */
+      const float ia = m->zsi_bnd[(size_t)(itb0 - 1) * m->nbnd + ib], ibv = m->zsi_bnd[(size_t)(itb1 - 1) * m->nbnd + ib];
+      m->zsit_bnd[ib] = ia + (ibv - ia) * tbfac;
+    }
   }
 }
 
@@ -500,7 +507,16 @@ static void update_boundary_conditions(OM* m, double t, float dt) {
       } else {
         zst = (double)m->zst_bnd[0];
       }
-      const float zsetup = 0.0f, zig = 0.0f;
+      const float zsetup = 0.0f;
+      float zig = 0.0f;
+      if (m->has_bzi) {   /* :829-848 */
+        if (m->nbnd > 1) {
+          const float f = m->fac_bnd_gbp[ib];
+          zig = m->zsit_bnd[m->ind1_bnd_gbp[ib] - 1] * f + m->zsit_bnd[m->ind2_bnd_gbp[ib] - 1] * (1.0f - f);
+        } else {
+          zig = m->zsit_bnd[0];
+        }
+      }
       if (t < (double)(m->tspinup - 1.0e-3f)) {
         const float smfac = (float)(1.0 - (t - (double)m->t0) / (double)(m->tspinup - m->t0));
         float zs0act = (float)(zst + (double)zsetup);

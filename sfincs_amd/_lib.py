@@ -75,6 +75,7 @@ def _sig(lib):
         "sfx_model_set_sources": (I, [P, i32, P, P, i32, P, P]),
         "sfx_model_set_rain": (I, [P, i32, P, P]),
         "sfx_model_set_river_boundary": (I, [P, i32, P, P, P, P, P, P]),
+        "sfx_model_set_boundary_ig": (I, [P, P]),
         "sfx_model_set_wind": (I, [P, i32, P, P, P]),
         "sfx_model_initialize": (I, [P]),
         "sfx_model_set_restart": (I, [P, s]),
@@ -255,6 +256,12 @@ class Model:
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)
         check(lib().sfx_model_set_rain(self.h, len(t), ptr(t), ptr(v)), "set_rain")
+
+    def set_boundary_ig(self, zsi):
+        """Incoming infragravity levels (bzi file): [nt, nbnd] at the water-level series times."""
+        a = np.ascontiguousarray(zsi, np.float32)
+        self._keep_bzi = a
+        check(lib().sfx_model_set_boundary_ig(self.h, ptr(a)), "set_boundary_ig")
 
     def set_river_boundary(self, x, y, x_in, y_in, slope, distance):
         """Downstream river boundary points (bdr file rows) for msk=5 cells."""

@@ -142,6 +142,7 @@ struct DevInst {
   int32_t ngbnd, nkcuv2, nbnd, ntbnd;
   const int32_t* gb_cell; const int32_t* gb_ind1; const int32_t* gb_ind2; const float* gb_fac; const int32_t* gb_nmi;
   const int32_t* gb_rcell; const float* gb_rdzs;   // river cells (kcs 5): interior cell and level offset
+  const float* zsi_bnd;                            // infragravity levels as zs_bnd (null: none)
   float* zsb; float* zsb0;
   const int32_t* k2_slot; const int32_t* k2_nmb; const int32_t* k2_nmi; const int32_t* k2_ib; const int8_t* k2_dir;
   const float* k2_zbuv;
@@ -566,27 +567,33 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     float zsb1, zsb0;
     if (kb == 2) {
       // update_boundary_points (:728-756) at the points this cell interpolates between
-      auto zst_at = [&](int k) -> float {
-        const float za = G(I.zs_bnd)[(itb0 - 1) * I.nbnd + k], zb = G(I.zs_bnd)[(itb1 - 1) * I.nbnd + k];
+      auto series_at = [&](const float* ser, int k) -> float {
+        const float za = G(ser)[(itb0 - 1) * I.nbnd + k], zb = G(ser)[(itb1 - 1) * I.nbnd + k];
         return za + (zb - za) * tbfac;
       };
-      // update_boundary_conditions, water-level boundary (:789-917)
+      // update_boundary_conditions, water-level boundary (:789-917); the infragravity level
+      // zig (bzi, :829-848) is interpolated as the water level
       double zst;
+      float zig = 0.0f;
       if (I.nbnd > 1) {
         const float f = G(I.gb_fac)[indb];
-        zst = (double)(zst_at(G(I.gb_ind1)[indb] - 1) * f + zst_at(G(I.gb_ind2)[indb] - 1) * (1.0f - f));
+        const int i1 = G(I.gb_ind1)[indb] - 1, i2 = G(I.gb_ind2)[indb] - 1;
+        zst = (double)(series_at(I.zs_bnd, i1) * f + series_at(I.zs_bnd, i2) * (1.0f - f));
+        if (I.zsi_bnd) zig = series_at(I.zsi_bnd, i1) * f + series_at(I.zsi_bnd, i2) * (1.0f - f);
       } else {
-        zst = (double)zst_at(0);
+        zst = (double)series_at(I.zs_bnd, 0);
+        if (I.zsi_bnd) zig = series_at(I.zsi_bnd, 0);
       }
       float zb0, zb1;
       if (t < (double)(c.tspinup - 1.0e-3f)) {
         const float smfac = (float)(1.0 - (t - (double)c.t0) / (double)(c.tspinup - c.t0));
         const float zs0act = (float)(zst + 0.0);
         zb0 = c.zini * smfac + zs0act * (1.0f - smfac);
-        zb1 = zb0;
+        const float zs1act = (float)(zst + 0.0 + (double)zig);
+        zb1 = c.zini * smfac + zs1act * (1.0f - smfac);
       } else {
-        zb0 = (float)zst;
-        zb1 = (float)zst;
+        zb0 = (float)(zst + 0.0);
+        zb1 = (float)(zst + 0.0 + (double)zig);
       }
       const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
       zsb0 = sfx::vmax<float>(zb0, floorz);
@@ -1802,6 +1809,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.k2_nmi = H->upload(nmi2.data(), nmi2.size()); I.k2_ib = H->upload(ib.data(), ib.size());
     I.k2_dir = H->upload(M->ibuvdir.data(), M->ibuvdir.size()); I.k2_zbuv = H->upload(zbuv.data(), zbuv.size());
     I.t_bnd = H->upload(M->t_bnd.data(), M->t_bnd.size()); I.zs_bnd = H->upload(M->zs_bnd.data(), M->zs_bnd.size());
+    I.zsi_bnd = M->zsi_bnd.empty() ? nullptr : H->upload(M->zsi_bnd.data(), M->zsi_bnd.size());
     // Neumann copies (kcs 6) run before the flux points only when they share a cell
     std::vector<uint8_t> fw(M->np, 0), fr(M->np, 0);
     for (int i = 0; i < M->nkcuv2; ++i)

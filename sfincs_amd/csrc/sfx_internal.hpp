@@ -83,6 +83,7 @@ struct sfx_model {
   int32_t rst_type = 0;                  // 0 = none, 1/2 = zs, q, uvmean, 3 = zs
   std::vector<float> rst_zs, rst_q, rst_uvmean;
   std::vector<float> x_bnd, y_bnd, t_bnd, zs_bnd;   // zs_bnd[it*nbnd + ib]
+  std::vector<float> zsi_bnd;                       // bzi: infragravity levels, as zs_bnd (empty: none)
   std::vector<int32_t> ind1_bnd_gbp, ind2_bnd_gbp, nmi_gbp;
   std::vector<float> fac_bnd_gbp;
   // downstream river boundaries (bdr file, sfincs_boundaries.f90:217-285, 587-613)

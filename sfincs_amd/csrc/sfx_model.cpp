@@ -607,6 +607,17 @@ extern "C" int sfx_model_set_boundary(sfx_model* M, int32_t nbnd, const float* x
   return SFX_OK;
 }
 
+// Incoming infragravity levels at the boundary points (bzifile, sfincs_boundaries.f90:126-140):
+// one value per boundary point at each time of the water-level series, same layout.
+extern "C" int sfx_model_set_boundary_ig(sfx_model* M, const float* zsi) {
+  if (!M || M->initialized || !zsi || M->nbnd <= 0) {
+    set_error("sfx_model_set_boundary_ig: set the boundary points and series first");
+    return SFX_ERR_ARG;
+  }
+  M->zsi_bnd.assign(zsi, zsi + (size_t)M->ntbnd * M->nbnd);
+  return SFX_OK;
+}
+
 // Downstream river boundary points (bdrfile, sfincs_boundaries.f90:217-285): per point the
 // boundary location, an interior point, the slope and the distance between them (< 0: the
 // distance of the two points). dzs = -slope * distance.
@@ -941,7 +952,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv) T(vmax) T(qmax) T(twet)
   T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
   T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval) T(x73)
-  T(index_bdr_gbp) T(index_zsi_bdr) T(dzs_bdr)
+  T(index_bdr_gbp) T(index_zsi_bdr) T(dzs_bdr) T(zsi_bnd)
   T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)
 #undef T
   return false;

@@ -94,6 +94,11 @@ def write(sc, folder):
         with open(os.path.join(folder, "sfincs.bzs"), "w") as f:
             for it in range(len(t)):
                 f.write(_fmt(t[it]) + " " + " ".join(_fmt(v) for v in zs[it]) + "\n")
+        if getattr(sc, "bzi", None) is not None:
+            lines.append("bzifile = sfincs.bzi")
+            with open(os.path.join(folder, "sfincs.bzi"), "w") as f:
+                for it in range(len(t)):
+                    f.write(_fmt(t[it]) + " " + " ".join(_fmt(v) for v in sc.bzi[it]) + "\n")
     if getattr(sc, "bdr", None) is not None:
         lines.append("bdrfile = sfincs.bdr")
         with open(os.path.join(folder, "sfincs.bdr"), "w") as f:

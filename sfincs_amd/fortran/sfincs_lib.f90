@@ -83,6 +83,10 @@ module sfx_capi
          integer(c_int32_t), value :: nsrc, nt
          type(c_ptr), value :: x, y, t, q
       end function
+      integer(c_int) function sfx_model_set_boundary_ig(m, zsi) bind(C, name='sfx_model_set_boundary_ig')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m, zsi
+      end function
       integer(c_int) function sfx_model_set_river_boundary(m, nbdr, x, y, xi, yi, slope, dist) &
             bind(C, name='sfx_model_set_river_boundary')
          import :: c_int, c_ptr, c_int32_t
@@ -323,12 +327,12 @@ contains
       integer :: ierr
       type(sfx_params), target :: p
       integer :: nmax, mmax, np, ios, u, i, j, n, m, nt, nb, rc
-      real(c_float) :: x0, y0, dx, dy, rot, dtmaxout
+      real(c_float) :: x0, y0, dx, dy, rot, dtmaxout, tdum
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
-      character(256) :: bndfile, bzsfile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
+      character(256) :: bndfile, bzsfile, bzifile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
       integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
       integer(c_int32_t), allocatable :: indices(:)
-      real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:)
+      real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:), zsi(:,:)
       real(c_float), allocatable, target :: tt(:), v1(:), v2(:)
       integer(c_size_t) :: nbytes
       logical :: f
@@ -389,6 +393,7 @@ contains
       call inp_str('srcfile', srcfile, 'none'); call inp_str('disfile', disfile, 'none')
       call inp_str('prcfile', prcfile, 'none'); call inp_str('wndfile', wndfile, 'none')
       call inp_str('rstfile', rstfile, 'none'); call inp_str('bdrfile', bdrfile, 'none')
+      call inp_str('bzifile', bzifile, 'none')
 
       if (qtrfile(1:4) /= 'none') then
          call read_quadtree(qtrfile, mskfile, p, rc)
@@ -484,6 +489,17 @@ contains
             close(u)
             rc = sfx_model_set_boundary(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), int(nt, c_int32_t), &
                                         c_loc(tb), c_loc(zsb))
+            ! incoming infragravity levels, one row per bzs time (sfincs_boundaries.f90:126-140)
+            if (rc >= 0 .and. bzifile(1:4) /= 'none') then
+               allocate(zsi(nb, nt))
+               open(newunit=u, file=trim(bzifile), status='old', action='read')
+               do i = 1, nt
+                  read(u, *) tdum, (zsi(j, i), j = 1, nb)
+               enddo
+               close(u)
+               rc = sfx_model_set_boundary_ig(model, c_loc(zsi))
+               deallocate(zsi)
+            endif
          else
             rc = sfx_model_set_boundary(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), 0_c_int32_t, &
                                         c_null_ptr, c_null_ptr)

@@ -40,6 +40,7 @@ class Scenario:
     t_end: float = 3600.0
     rstfile: str = None                   # binary restart file as initial condition
     bdr: tuple = None                     # river boundary points (x, y, x_in, y_in, slope, distance)
+    bzi: np.ndarray = None                # infragravity levels [nt, nbnd] at the bnd series times
 
     def build_model(self):
         p = _lib.default_params(**self.params)
@@ -54,6 +55,8 @@ class Scenario:
         if self.bnd is not None:
             x, y, t, zs = self.bnd
             m.set_boundary(x, y, t, zs)
+            if self.bzi is not None:
+                m.set_boundary_ig(self.bzi)
         if self.src is not None:
             x, y, t, q = self.src
             m.set_sources(x, y, t, q)
@@ -273,6 +276,20 @@ def regular_case(nmax, mmax, dx=50.0, seed=1000, t_end=3600.0, rain_mmh=20.0, ra
                     dx=dx, dy=dx, msk=msk.reshape(-1), zb_grid=zb,
                     bnd=(np.array([-dx / 2], np.float32), np.array([nmax * dx / 2], np.float32), t, zs),
                     rain=rain, wind=wind, t_end=float(t_end))
+
+
+def with_ig_waves(sc, amp=0.2, period=120.0):
+    """Adds incoming infragravity levels (a bzi series) to a scenario's water-level
+    boundary: a sine of `amp` metres and `period` seconds, sampled at the series times
+    refined to 10 s so that the wave is resolved."""
+    x, y, t, zs = sc.bnd
+    tt = np.arange(float(t[0]), float(t[-1]) + 1.0, 10.0, dtype=np.float64)
+    zs2 = np.stack([np.interp(tt, t.astype(np.float64), zs[:, k].astype(np.float64)) for k in range(zs.shape[1])], 1)
+    sc.bnd = (x, y, tt.astype(np.float32), zs2.astype(np.float32))
+    ig = amp * np.sin(2.0 * np.pi * tt / period)
+    sc.bzi = np.repeat(ig[:, None], zs.shape[1], 1).astype(np.float32)
+    sc.name += "_ig"
+    return sc
 
 
 def with_river(sc, first=1, ncells=8, slope=1.0e-3):

@@ -47,7 +47,7 @@ TABLES = ("params", "kcs", "zb", "z_flags_iref", "uv_index_z_nm", "uv_index_z_nm
           "mask_adv", "cuv_index_uv", "zbuv", "zbuvmx", "gn2uv", "dxrinv", "cell_area", "zs", "zsb", "zsb0",
           "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibuvdir", "ind1_bnd_gbp", "ind2_bnd_gbp",
           "fac_bnd_gbp", "nmi_gbp", "nmindsrc", "t_bnd", "zs_bnd", "tsrc", "qsrc", "tprcpt", "tprcpv",
-          "twnd", "wndmag", "wnddir", "cdval", "index_bdr_gbp", "index_zsi_bdr", "dzs_bdr")
+          "twnd", "wndmag", "wnddir", "cdval", "index_bdr_gbp", "index_zsi_bdr", "dzs_bdr", "zsi_bnd")
 
 
 def _table(lib, m, name):
@@ -60,13 +60,15 @@ def _table(lib, m, name):
 
 
 @needs_host
-@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
     from sfincs_amd import _lib
     if kind == "regular":
         sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0)
+    elif kind == "ig_waves":
+        sc = W.with_ig_waves(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0))
     elif kind == "river":
         sc = W.with_river(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0), slope=2.0e-3)
     elif kind == "store_switches":

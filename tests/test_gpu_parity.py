@@ -502,3 +502,28 @@ def test_source_into_dry_cell_keeps_its_volume():
         va, vb = inst.get("z_volume"), orc.state("z_volume")
         assert np.all(np.abs(va - vb) <= 1e-7 * (1.0 + np.abs(vb))), (step, va[c], vb[c])
     assert orc.state("z_volume")[c] > 0.0
+
+
+IG_CASES = [
+    ("regular", lambda: W.with_ig_waves(W.regular_case(40, 52, dx=50.0, seed=96, t_end=1800.0, tide_amp=1.0))),
+    ("subgrid", lambda: W.with_ig_waves(W.subgrid_case(32, 40, dx=25.0, seed=97, t_end=1800.0, nsrc=1, tide_amp=1.5),
+                                        amp=0.3, period=90.0)),
+]
+
+
+@pytest.mark.parametrize("name,make", IG_CASES, ids=[c[0] for c in IG_CASES])
+def test_ig_waves_match_oracle(name, make):
+    """Incoming infragravity levels (bzi) added to the water-level boundary in the boundary
+    pass of k_forcing; whole run against the oracle."""
+    print(f"case ig waves {name}", flush=True)
+    sc = make()
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    if m.sizes()["subgrid"]:
+        d.update(compare_subgrid_state(inst, orc))
+    print(name, st["nt"], int(orc.get("nt")), d, flush=True)
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL

@@ -79,6 +79,8 @@ typedef struct sfx_params {
   int32_t store_fluxmax;     /* qmax: storefluxmax == 1 and dtmaxout > 0 */
   int32_t store_twet;        /* twet: storetwet == 1 */
   float twet_threshold;      /* 0.01 (sfincs_input.f90:285) */
+  int32_t timestep_analysis; /* 0 (sfincs_input.f90:288): per uv-point time step statistics,
+                              * read as tsa_sum / tsa_wet / tsa_lim (sfx_get_state) */
 } sfx_params;
 
 void sfx_params_default(sfx_params* p);
@@ -188,7 +190,11 @@ int  sfx_destroy(sfx_handle* h);
 /* State exchange in reference order; the explicit form of `acc update host/device`.
  * Names: zs, z_volume (f64 per cell); q, uv (per uv-point); kfuv (i8); zs0, zsderv, zsmax,
  * uvmean; and, when their switch is on, vmax, qmax, twet (f32 per cell; initial values
- * -999, -999, 0 as sfincs_domain.f90:2300-2306 and sfincs_output.f90:210 reset them). */
+ * -999, -999, 0 as sfincs_domain.f90:2300-2306 and sfincs_output.f90:210 reset them).
+ * With timestep_analysis on (read only, per uv-point, npuv entries): tsa_sum (f32, the sum
+ * of the required time steps of the sweeps in which the point was wet), tsa_wet and
+ * tsa_lim (i32: those sweeps, and those in which it limited the step), the accumulators of
+ * sfincs_timestep_analysis.f90:26-74; their average is tsa_sum / max(tsa_wet, 1). */
 int  sfx_get_state(sfx_handle* h, const char* name, void* dst, size_t nbytes);
 int  sfx_set_state(sfx_handle* h, const char* name, const void* src, size_t nbytes);
 /* write_rst_file (sfincs_output.f90:657-729): a type-1 restart file (zs as f32, q, uvmean)

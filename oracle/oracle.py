@@ -121,6 +121,10 @@ class OracleRun:
         self._bind("uvmean", model.table("uvmean").copy())
         self._bind("zst_bnd", np.zeros(max(1, s["nbnd"]), np.float32))
         self._bind("zsit_bnd", np.zeros(max(1, s["nbnd"]), np.float32))
+        self._bind("tsa_req", np.full(max(1, s["npuv"]), s["dtmax"], np.float32))
+        self._bind("tsa_sum", np.zeros(max(1, s["npuv"]), np.float32))
+        self._bind("tsa_wet", np.zeros(max(1, s["npuv"]), np.int32))
+        self._bind("tsa_lim", np.zeros(max(1, s["npuv"]), np.int32))
         self._bind("qtsrc", np.zeros(max(1, s["nsrc"]), np.float32))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
@@ -155,7 +159,7 @@ class OracleRun:
                     friction2d=p.friction2d, thetasmoothing=int(p.theta < np.float32(0.9999)),
                     wiggle_suppression=p.wiggle_suppression, store_zsmax=p.store_zsmax,
                     store_velmax=p.store_velmax, store_fluxmax=p.store_fluxmax, store_twet=p.store_twet,
-                    twet_threshold=p.twet_threshold,
+                    twet_threshold=p.twet_threshold, timestep_analysis=p.timestep_analysis,
                     wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
@@ -239,7 +243,7 @@ REF_SCALARS = ("subgrid", "use_quadtree", "advection", "advection_scheme", "visc
                "thetasmoothing", "wiggle_suppression", "store_zsmax", "store_velmax", "store_fluxmax", "store_twet",
                "twet_threshold", "wind", "precip", "h73table", "nsrc", "g",
                "huthresh", "huvmin", "advlim", "slopelim", "uvlim", "nuviscfac", "fcorio", "theta",
-               "wiggle_factor", "wiggle_threshold", "dtmax", "rhow", "dx", "dy")
+               "wiggle_factor", "wiggle_threshold", "dtmax", "rhow", "dx", "dy", "timestep_analysis")
 
 
 class RefRun:
@@ -290,6 +294,8 @@ class RefRun:
         names = ["zs", "q", "uv", "zs0", "zsderv", "zsmax", "kfuv", "vmax", "qmax", "twet"]
         if self.o.vals["subgrid"]:
             names.append("z_volume")
+        if self.o.vals["timestep_analysis"]:
+            names += ["tsa_sum", "tsa_wet", "tsa_lim"]
         for n in names:
             R.ref_get(n.encode(), k[n].ctypes.data, k[n].nbytes)
 

@@ -13,6 +13,7 @@ module ref_driver
    use sfincs_data
    use sfincs_momentum
    use sfincs_continuity
+   use sfincs_timestep_analysis
    use sfincs_input
    use sfincs_log
    implicit none
@@ -60,6 +61,15 @@ contains
       case ('store_fluxmax');      store_maximum_flux = (v /= 0.0d0)
       case ('store_twet');         store_twet = (v /= 0.0d0)
       case ('twet_threshold');     twet_threshold = real(v, 4)
+      case ('timestep_analysis')   ! after dtmax and the sizes (initialize_timestep_analysis)
+         timestep_analysis = (v /= 0.0d0)
+         if (timestep_analysis) then
+            if (allocated(timestep_analysis_average_required_timestep)) then
+               deallocate(timestep_analysis_average_required_timestep, timestep_analysis_required_timestep, &
+                          timestep_analysis_times_limiting, timestep_analysis_times_wet)
+            endif
+            call initialize_timestep_analysis()
+         endif
       case ('wind');               wind = (v /= 0.0d0)
       case ('precip');             precip = (v /= 0.0d0)
       case ('h73table');           h73table = (v /= 0.0d0)
@@ -234,6 +244,7 @@ contains
       real(c_float), pointer :: f(:)
       real(c_double), pointer :: d(:)
       integer(c_int8_t), pointer :: b(:)
+      integer(c_int32_t), pointer :: iv(:)
       select case (trim(cstr(name)))
       case ('q');        call c_f_pointer(p, f, [nbytes / 4]); f = q
       case ('uv');       call c_f_pointer(p, f, [nbytes / 4]); f = uv
@@ -243,6 +254,9 @@ contains
       case ('vmax');     call c_f_pointer(p, f, [nbytes / 4]); f = vmax
       case ('qmax');     call c_f_pointer(p, f, [nbytes / 4]); f = qmax
       case ('twet');     call c_f_pointer(p, f, [nbytes / 4]); f = twet
+      case ('tsa_sum');  call c_f_pointer(p, f, [nbytes / 4]); f = timestep_analysis_average_required_timestep
+      case ('tsa_wet');  call c_f_pointer(p, iv, [nbytes / 4]); iv = timestep_analysis_times_wet
+      case ('tsa_lim');  call c_f_pointer(p, iv, [nbytes / 4]); iv = timestep_analysis_times_limiting
       case ('zs');       call c_f_pointer(p, d, [nbytes / 8]); d = zs
       case ('z_volume'); call c_f_pointer(p, d, [nbytes / 8]); d = z_volume
       case ('kfuv');     call c_f_pointer(p, b, [nbytes]);     b = kfuv
@@ -258,6 +272,7 @@ contains
       real*4 :: dtl
       dtl = dt_
       call compute_fluxes(dtl, tloopflux)
+      if (timestep_analysis) call timestep_analysis_update(min_dt)   ! sfincs_lib.f90:588
       mdt = min_dt
    end function
 

@@ -6,7 +6,9 @@
  * type for type (real*4 arithmetic, real*8 zs / z_volume / t), so that it reproduces
  * the reference's own Fortran bit for bit on x86-64 (pinned against oracle/_ref, the
  * reference's sfincs_momentum/sfincs_continuity compiled from source):
- *   oracle_compute_fluxes        sfincs_momentum.f90:9-784
+ *   oracle_compute_fluxes        sfincs_momentum.f90:9-784, with timestep_analysis_update
+ *                                (sfincs_timestep_analysis.f90:26-74), which sfincs_lib.f90:588
+ *                                runs right after it
  *   oracle_compute_water_levels  sfincs_continuity.f90:5-720 (with compute_store_variables)
  *   oracle_update_meteo          sfincs_meteo.f90:1246-1558 (time-series branch)
  *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
@@ -27,6 +29,7 @@ typedef struct OM {
   int32_t thetasmoothing, wiggle_suppression, store_zsmax, wind, precip, h73table;
   int32_t store_velmax, store_fluxmax, store_twet;
   float twet_threshold;
+  int32_t timestep_analysis;
   int32_t ngbnd, nkcuv2, nbnd, ntbnd, nsrc, ntsrc, ntprcp, ntwnd;
   float g, pi, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
@@ -48,6 +51,8 @@ typedef struct OM {
   double *zs, *z_volume;
   float *q, *uv, *q0, *uv0, *zs0, *zsderv, *zsmax;
   float *vmax, *qmax, *twet;   /* compute_store_variables (np each, when switched on) */
+  float *tsa_req, *tsa_sum;    /* timestep analysis (npuv each, when switched on) */
+  int32_t *tsa_wet, *tsa_lim;
   int8_t *kfuv;
   float *tauwu, *tauwv, *netprcp;   /* np each */
   /* boundaries */
@@ -92,7 +97,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(z_zmin) BIND(z_zmax) BIND(z_volmax) BIND(z_dep) BIND(uv_zmin) BIND(uv_zmax) BIND(uv_navg_w) BIND(uv_fnfit)
   BIND(uv_havg) BIND(uv_nrep) BIND(uv_pwet) BIND(uv_havg_zmax) BIND(uv_nrep_zmax)
   BIND(zs) BIND(z_volume) BIND(q) BIND(uv) BIND(q0) BIND(uv0) BIND(zs0) BIND(zsderv) BIND(zsmax) BIND(kfuv)
-  BIND(vmax) BIND(qmax) BIND(twet)
+  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim)
   BIND(tauwu) BIND(tauwv) BIND(netprcp)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
   BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsi_bnd) BIND(zsit_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
@@ -108,7 +113,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
   SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error)
-  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi)
+  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
   SETF(btfilter) SETF(btrelax) SETF(t0) SETF(t1) SETF(tspinup) SETF(zini) SETF(dx) SETF(dy) SETF(dt) SETF(min_dt)
@@ -153,6 +158,8 @@ float oracle_compute_fluxes(OM* m, float dt) {
   const int32_t nuv = m->npuv + m->ncuv;
   memcpy(m->q0, m->q, sizeof(float) * nuv);   /* :125-138 */
   memcpy(m->uv0, m->uv, sizeof(float) * nuv);
+  if (m->timestep_analysis)   /* :101-119 */
+    for (int32_t i = 0; i < m->npuv; ++i) m->tsa_req[i] = m->dtmax;
   const int nbrs = m->advection || m->coriolis || m->viscosity || m->friction2d || m->thetasmoothing;
   for (int32_t i = 0; i < m->npuv; ++i) {
     const int kc = m->kcuv[i];
@@ -311,6 +318,7 @@ float oracle_compute_fluxes(OM* m, float dt) {
     m->kfuv[i] = 1;
     const float min_dt_ip = 1.0f / (fmx(sqrtf(m->g * hu), fabsf(uvn)) * dxuvinv);
     min_dt = fmn(min_dt, min_dt_ip);
+    if (m->timestep_analysis) m->tsa_req[i] = min_dt_ip;   /* :737-741 */
   }
   for (int32_t j = 0; j < m->ncuv; ++j) {   /* :757-779 */
     const int32_t c = m->cuv_index_uv[j] - 1, a = m->cuv_index_uv1[j] - 1, b = m->cuv_index_uv2[j] - 1;
@@ -318,6 +326,14 @@ float oracle_compute_fluxes(OM* m, float dt) {
     m->uv[c] = (m->uv[a] + m->uv[b]) / 2;
   }
   m->min_dt = min_dt;
+  if (m->timestep_analysis) {   /* timestep_analysis_update (sfincs_timestep_analysis.f90:26-74) */
+    for (int32_t i = 0; i < m->npuv; ++i) {
+      if (!(m->kcuv[i] == 1 || m->kcuv[i] == 6) || m->kfuv[i] != 1) continue;
+      m->tsa_sum[i] = m->tsa_sum[i] + m->tsa_req[i];
+      m->tsa_wet[i] = m->tsa_wet[i] + 1;
+      if (m->tsa_req[i] <= min_dt + 1.0e-6f) m->tsa_lim[i] = m->tsa_lim[i] + 1;
+    }
+  }
   return min_dt;
 }
 

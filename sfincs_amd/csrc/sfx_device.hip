@@ -118,6 +118,10 @@ struct DevInst {
   void* vmax;                    // compute_store_variables fields, allocated when switched on
   void* qmax;
   void* twet;
+  // timestep_analysis, per uv slot (allocated when switched on): the required time step of
+  // the last sweep (wet points), its running sum, the sweeps wet and those limiting
+  float* tsa_req; float* tsa_sum; int32_t* tsa_wet; int32_t* tsa_lim;
+  float* tsa_min;                // the last sweep's minimum time step
   uint8_t* kcs;                  // kcs (KCS_MASK) | KCS_SOURCE (point source) | KCS_STEADY (continuity)
   const float* zb;
   const int8_t* z_iref;
@@ -691,6 +695,7 @@ __device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::Uv
   q1[slot] = o.q; uv1[slot] = o.uv;
   if (kf_old != 1) G(I.kfuv)[slot] = 1;
   mdt = fminf(mdt, (float)o.min_dt);
+  if (I.c.flags & sfx::F_TSA) G(I.tsa_req)[slot] = (float)o.min_dt;   // sfincs_momentum.f90:737-741
   br = o.branch;
 }
 
@@ -1497,6 +1502,7 @@ __global__ void k_zero_f64(double* p) { *p = 0.0; }
 // =============================================================================== host side
 struct sfx_handle {
   int device = 0;
+  bool tsa_paused = false;      // timestep analysis off while sweeps are only timed (autotune)
   int fp64 = 0;
   sfx_sizes sz{};
   DevInst hI{};
@@ -1591,6 +1597,7 @@ uint32_t build_flags(const sfx_model* M) {
   if (p.store_velmax) f |= sfx::F_VELMAX;
   if (p.store_fluxmax) f |= sfx::F_FLUXMAX;
   if (p.store_twet) f |= sfx::F_TWET;
+  if (p.timestep_analysis) f |= sfx::F_TSA;
   if (M->wind) f |= sfx::F_WIND;
   if (M->precip) f |= sfx::F_PRECIP;
   if (p.h73table) f |= sfx::F_H73;
@@ -1639,6 +1646,14 @@ int upload_state(sfx_handle* H, const sfx_model* M) {
   if (f & sfx::F_VELMAX) { auto v = to_r<R>(M->vmax); I.vmax = H->upload(v.data(), np); }
   if (f & sfx::F_FLUXMAX) { auto v = to_r<R>(M->qmax); I.qmax = H->upload(v.data(), np); }
   if (f & sfx::F_TWET) { auto v = to_r<R>(M->twet); I.twet = H->upload(v.data(), np); }
+  if (f & sfx::F_TSA) {   // initialize_timestep_analysis (sfincs_timestep_analysis.f90:10-22)
+    const size_t ns = (size_t)H->nslot + 1;
+    std::vector<float> req(ns, M->dtmax), zero(ns, 0.f);
+    std::vector<int32_t> izero(ns, 0);
+    I.tsa_req = H->upload(req.data(), ns); I.tsa_sum = H->upload(zero.data(), ns);
+    I.tsa_wet = H->upload(izero.data(), ns); I.tsa_lim = H->upload(izero.data(), ns);
+    I.tsa_min = H->upload(zero.data(), 1);
+  }
   auto um = to_r<R>(M->uvmean);
   I.uvmean = H->upload(um.data(), um.size());
   return SFX_OK;
@@ -1920,6 +1935,17 @@ int normalise(sfx_handle* H, Scalars& s) {
   return read_scalars(H, s);
 }
 
+// 4-byte per-slot words (f32 or i32) of the npuv uv-points, in reference order.
+int get_uv_words(sfx_handle* H, const void* src, void* dst, size_t nbytes) {
+  const size_t n = H->sz.npuv;
+  if (nbytes != n * 4) { set_error("size mismatch"); return SFX_ERR_ARG; }
+  std::vector<uint32_t> buf((size_t)H->nslot + 1);
+  HIPCHK(hipMemcpy(buf.data(), src, buf.size() * 4, hipMemcpyDeviceToHost));
+  uint32_t* d = (uint32_t*)dst;
+  for (size_t k = 0; k < n; ++k) d[k] = H->ref2slot[k] >= 0 ? buf[H->ref2slot[k]] : 0u;
+  return SFX_OK;
+}
+
 template <typename R>
 int get_uv_field(sfx_handle* H, void* src, void* dst, size_t nbytes, bool is_q) {
   (void)is_q;
@@ -2004,6 +2030,11 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
   if (k == "zs0" || k == "zsderv" || k == "zsmax") {
     void* src = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
     return H->fp64 ? get_cell_r<double>(H, src, dst, nbytes, np) : get_cell_r<float>(H, src, dst, nbytes, np);
+  }
+  if (k == "tsa_sum" || k == "tsa_wet" || k == "tsa_lim") {   // timestep analysis, per uv-point
+    if (!I.tsa_req) { set_error("sfx_get_state: " + k + ": timestep analysis is off"); return SFX_ERR_ARG; }
+    const void* src = k == "tsa_sum" ? (const void*)I.tsa_sum : (k == "tsa_wet" ? (const void*)I.tsa_wet : (const void*)I.tsa_lim);
+    return get_uv_words(H, src, dst, nbytes);
   }
   if (k == "vmax" || k == "qmax" || k == "twet") {
     void* src = k == "vmax" ? I.vmax : (k == "qmax" ? I.qmax : I.twet);
@@ -2242,6 +2273,37 @@ inline const DevInst* base_of(const sfx_batch* B, const WorkList& w) {
 }
 
 // Launch phases of one step. what: bit 0 forcing, bit 1 momentum (+cuv), bit 2 continuity.
+// timestep_analysis_update (sfincs_timestep_analysis.f90:26-74, called after compute_fluxes
+// at sfincs_lib.f90:588) in two launches after a momentum sweep. First, per scenario, the
+// sweep's minimum time step from its tile minima (read only: the next fold still uses
+// them); then, per uv slot, the running sum of the wet points' required steps, the sweeps
+// they were wet and those in which they limited the step (required <= min_dt + 1e-6).
+__global__ void __launch_bounds__(FBLOCK) k_tsa_min(const DevInst* __restrict__ insts,
+                                                   const int32_t* __restrict__ ids) {
+  const DevInst& I = insts[ids[blockIdx.x]];
+  if (!(I.c.flags & sfx::F_TSA) || !G(I.sc)->mom_done) return;
+  float v = I.c.dtmax;
+  for (int i = threadIdx.x; i < I.nmt; i += FBLOCK) v = fminf(v, G(I.blkmin)[i]);
+  v = block_min<FBLOCK>(v);
+  if (threadIdx.x == 0) G(I.tsa_min)[0] = v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_tsa_update(const DevInst* __restrict__ insts,
+                                                     const int32_t* __restrict__ ids) {
+  const DevInst& I = insts[ids[blockIdx.y]];
+  if (!(I.c.flags & sfx::F_TSA) || !G(I.sc)->mom_done) return;
+  const float lim = G(I.tsa_min)[0] + 1.0e-6f;
+  for (int s = blockIdx.x * BLOCK + threadIdx.x; s < I.nslot; s += gridDim.x * BLOCK) {
+    const int kc = G(I.kcuv)[s];
+    if ((kc == 1 || kc == 6) && (G(I.kfuv)[s] & 1)) {
+      const float r = G(I.tsa_req)[s];
+      G(I.tsa_sum)[s] = G(I.tsa_sum)[s] + r;
+      G(I.tsa_wet)[s] = G(I.tsa_wet)[s] + 1;
+      if (r <= lim) G(I.tsa_lim)[s] = G(I.tsa_lim)[s] + 1;
+    }
+  }
+}
+
 template <typename R>
 void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   hipStream_t st = B->stream;
@@ -2261,6 +2323,14 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
     if (B->n_m[10]) hipLaunchKernelGGL((k_mom_point<R, true, 4>), dim3(B->n_m[10]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[10]), B->wl_m[10]);
     if (B->n_m[11]) hipLaunchKernelGGL((k_mom_point<R, false, 4>), dim3(B->n_m[11]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[11]), B->wl_m[11]);
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, base_of(B, B->wl_cuv), B->wl_cuv);
+    int tsa_slots = 0;   // timestep analysis of the members that have it on (not while tuning)
+    for (sfx_handle* H : B->members)
+      if (H->hI.tsa_req && !H->tsa_paused) tsa_slots = std::max(tsa_slots, (int)H->hI.nslot);
+    if (tsa_slots && nm) {
+      hipLaunchKernelGGL(k_tsa_min, dim3(nm), dim3(FBLOCK), 0, st, B->dInsts, B->dIds);
+      const int nb = std::min(1024, (tsa_slots + BLOCK - 1) / BLOCK);
+      hipLaunchKernelGGL(k_tsa_update, dim3(nb, nm), dim3(BLOCK), 0, st, B->dInsts, B->dIds);
+    }
   }
   // A whole step: the next step's fold, scalars and series (forcing_ahead_block) run as
   // the leading blocks of this step's first continuity launch, beside its cells.
@@ -2635,6 +2705,8 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (H->in_batches) { set_error("sfx_autotune: tune a scenario before adding it to a batch"); return SFX_ERR_STATE; }
   const bool dense = H->hI.dense;   // indexed meshes have one kernel shape; only the work order is tuned
   trials = std::max(1, (int)trials);
+  struct Pause { sfx_handle* h; ~Pause() { h->tsa_paused = false; } } pause{H};
+  H->tsa_paused = true;   // timed sweeps are not steps of the run
   Scalars s0;
   int rc = normalise(H, s0);
   if (rc) return rc;

@@ -17,7 +17,8 @@ enum : uint32_t {
   F_SUBGRID = 1u << 12, F_NBRS = 1u << 13,
   // compute_store_variables switches (sfincs_continuity.f90:626-720)
   F_VELMAX = 1u << 14, F_FLUXMAX = 1u << 15, F_TWET = 1u << 16,
-  F_STORE = F_VELMAX | F_FLUXMAX | F_TWET
+  F_STORE = F_VELMAX | F_FLUXMAX | F_TWET,
+  F_TSA = 1u << 17   // timestep_analysis (sfincs_timestep_analysis.f90)
 };
 
 constexpr int MAXREF = 8;

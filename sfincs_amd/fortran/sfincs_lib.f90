@@ -20,6 +20,7 @@ module sfx_capi
       integer(c_int32_t) :: advection_mask, wiggle_suppression, store_zsmax, bndtype, h73table
       integer(c_int32_t) :: store_velmax, store_fluxmax, store_twet
       real(c_float) :: twet_threshold
+      integer(c_int32_t) :: timestep_analysis
    end type
    type, bind(C) :: sfx_quadtree_input
       integer(c_int32_t) :: nr_points, nr_levels
@@ -381,6 +382,7 @@ contains
       i = 0
       call inp_int('storetwet', i); p%store_twet = merge(1, 0, i == 1)
       call inp_real('twet_threshold', p%twet_threshold)
+      call inp_flag('timestep_analysis', p%timestep_analysis)
       host_t0 = p%t0; host_t1 = p%t1
       x0 = 0.0; y0 = 0.0; dx = 0.0; dy = 0.0; rot = 0.0; nmax = 0; mmax = 0
       call inp_int('nmax', nmax); call inp_int('mmax', mmax)

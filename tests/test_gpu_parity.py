@@ -527,3 +527,35 @@ def test_ig_waves_match_oracle(name, make):
     print(name, st["nt"], int(orc.get("nt")), d, flush=True)
     assert st["nt"] == int(orc.get("nt"))
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+
+
+TSA_CASES = [
+    ("regular", lambda: W.regular_case(40, 52, dx=50.0, seed=98, t_end=1800.0, tide_amp=2.0, timestep_analysis=1)),
+    ("subgrid", lambda: W.subgrid_case(32, 40, dx=25.0, seed=99, t_end=1800.0, nsrc=2, timestep_analysis=1)),
+    ("quadtree", lambda: W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=100, t_end=1800.0, surge_amp=4.0,
+                                         timestep_analysis=1)),
+]
+
+
+@pytest.mark.parametrize("name,make", TSA_CASES, ids=[c[0] for c in TSA_CASES])
+def test_timestep_analysis_matches_oracle(name, make):
+    """timestep_analysis accumulators (k_tsa_min + k_tsa_update after each momentum sweep)
+    against the oracle over a whole run: wet and limiting counts exact, the sums of
+    required time steps to 1e-6 relative. Autotuning before the run must not count its
+    timed sweeps."""
+    print(f"case timestep analysis {name}", flush=True)
+    sc = make()
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    inst.autotune(2)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    assert st["nt"] == int(orc.get("nt"))
+    n = m.sizes()["npuv"]
+    wa, wb = inst.get("tsa_wet"), orc.state("tsa_wet")[:n]
+    la, lb = inst.get("tsa_lim"), orc.state("tsa_lim")[:n]
+    sa, sb = inst.get("tsa_sum"), orc.state("tsa_sum")[:n]
+    print(name, st["nt"], int(wb.max()), int(lb.sum()), float(np.max(np.abs(sa - sb) / (1.0 + np.abs(sb)))), flush=True)
+    assert np.array_equal(wa, wb) and np.array_equal(la, lb)
+    assert np.all(np.abs(sa - sb) <= 1e-6 * (1.0 + np.abs(sb)))
+    assert lb.sum() >= st["nt"]

@@ -117,3 +117,27 @@ def test_store_variables_bit_exact(kind):
     assert np.max(a.state("vmax")) > 0.0 and np.max(a.state("qmax")) > 0.0
     tw = a.state("twet")
     assert np.max(tw) > 0.0 and np.min(tw) == 0.0
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid", "quadtree"])
+def test_timestep_analysis_bit_exact(kind):
+    """timestep_analysis (sfincs_timestep_analysis.f90:26-74 after compute_fluxes): the
+    per-uv sums of required time steps, wet counts and limiting counts after every step."""
+    if kind == "regular":
+        sc = W.regular_case(30, 40, dx=50.0, seed=28, t_end=3600.0, tide_amp=2.0, timestep_analysis=1)
+    elif kind == "subgrid":
+        sc = W.subgrid_case(28, 36, dx=25.0, seed=29, t_end=3600.0, nsrc=2, rain_mmh=40.0, timestep_analysis=1)
+    else:
+        sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=30, t_end=3600.0, surge_amp=4.0, timestep_analysis=1)
+    m = sc.build_model()
+    a = O.OracleRun(m).configure(sc.params, m)
+    b = O.OracleRun(m).configure(sc.params, m)
+    r = O.RefRun(b)
+    for i in range(150):
+        a.update(1e30, 1)
+        r.step()
+        for n in ("zs", "q", "tsa_sum", "tsa_wet", "tsa_lim"):
+            x, y = a.state(n), b.state(n)
+            assert np.array_equal(x, y), f"step {i} field {n}"
+    assert a.state("tsa_wet").max() > 0 and a.state("tsa_lim").max() > 0
+    assert a.state("tsa_lim").sum() >= 150   # at least one limiting point per step

@@ -168,6 +168,12 @@ int  sfx_model_set_restart(sfx_model* m, const char* path);
 int  sfx_model_initialize(sfx_model* m);
 /* Initial state by name ("zs" f64, "q" f32, "uv" f32, "kfuv" i8, "z_volume" f64, ...). */
 int  sfx_model_get_state(const sfx_model* m, const char* name, void* dst, size_t nbytes);
+/* timestep_analysis_finalize (sfincs_timestep_analysis.f90:77-200), on the host: from the
+ * per-uv accumulators (tsa_sum, tsa_wet, tsa_lim of sfx_get_state) after nt steps, per
+ * cell the average required time step (alpha-scaled, capped at dtmax; -1 where no
+ * uv-point of the cell was counted) and the percentage of steps it limited. np each. */
+int  sfx_model_timestep_analysis_per_cell(const sfx_model* m, const float* tsa_sum, const int32_t* tsa_wet,
+                                          const int32_t* tsa_lim, int64_t nt, float* dt_cell, float* pct_cell);
 void sfx_model_destroy(sfx_model* m);
 
 /* ---- device instance ---------------------------------------------------------------

@@ -97,6 +97,7 @@ class OracleRun:
     def __init__(self, model):
         L = olib()
         self.L = L
+        self.model = model
         self.m = C.c_void_p(L.oracle_new())
         s = model.sizes()
         self.sizes = s
@@ -225,6 +226,7 @@ def rlib():
         _rlib.ref_compute_water_levels.argtypes = [C.c_double, C.c_float]
         _rlib.ref_timers.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _rlib.ref_read_input.argtypes = [C.c_void_p, C.c_int32]
+        _rlib.ref_tsa_finalize.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     return _rlib
 
 
@@ -235,7 +237,8 @@ def ref_available():
 REF_I32 = ("uv_index_z_nm", "uv_index_z_nmu", "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm",
            "uv_index_u_num", "uv_index_v_ndm", "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu",
            "cuv_index_uv", "cuv_index_uv1", "cuv_index_uv2", "z_index_uv_md", "z_index_uv_mu", "z_index_uv_nd",
-           "z_index_uv_nu", "nmindsrc")
+           "z_index_uv_nu", "nmindsrc", "z_index_uv_md1", "z_index_uv_md2", "z_index_uv_mu1", "z_index_uv_mu2",
+           "z_index_uv_nd1", "z_index_uv_nd2", "z_index_uv_nu1", "z_index_uv_nu2")
 REF_I8 = ("kcs", "kcuv", "mask_adv", "z_flags_iref", "uv_flags_iref", "uv_flags_type", "uv_flags_dir")
 REF_F32_TABLES = ("dxrinv", "dyrinv", "dxr2inv", "dyr2inv", "dxrinvc", "dyrinvc", "dxrm", "dyrm", "cell_area",
                   "nuvisc")
@@ -243,7 +246,7 @@ REF_SCALARS = ("subgrid", "use_quadtree", "advection", "advection_scheme", "visc
                "thetasmoothing", "wiggle_suppression", "store_zsmax", "store_velmax", "store_fluxmax", "store_twet",
                "twet_threshold", "wind", "precip", "h73table", "nsrc", "g",
                "huthresh", "huvmin", "advlim", "slopelim", "uvlim", "nuviscfac", "fcorio", "theta",
-               "wiggle_factor", "wiggle_threshold", "dtmax", "rhow", "dx", "dy", "timestep_analysis")
+               "wiggle_factor", "wiggle_threshold", "dtmax", "alpha", "rhow", "dx", "dy", "timestep_analysis")
 
 
 class RefRun:
@@ -263,8 +266,9 @@ class RefRun:
             R.ref_set_scalar(k.encode(), float(v[k]))
         k = orc.keep
         for n in REF_I32:
-            if n in k and k[n].size:
-                R.ref_set_i32(n.encode(), k[n].ctypes.data, k[n].size)
+            a = k[n] if n in k else np.ascontiguousarray(orc.model.table(n), np.int32)
+            if a.size:
+                R.ref_set_i32(n.encode(), a.ctypes.data, a.size)
         for n in REF_I8:
             R.ref_set_i8(n.encode(), k[n].ctypes.data, k[n].size)
         names = list(REF_F32_TABLES)
@@ -298,6 +302,13 @@ class RefRun:
             names += ["tsa_sum", "tsa_wet", "tsa_lim"]
         for n in names:
             R.ref_get(n.encode(), k[n].ctypes.data, k[n].nbytes)
+
+    def tsa_finalize(self, nt):
+        """The reference's timestep_analysis_finalize: per-cell (average step, % limiting)."""
+        n = self.o.vals["np"]
+        dtc, pct = np.empty(n, np.float32), np.empty(n, np.float32)
+        self.R.ref_tsa_finalize(int(nt), dtc.ctypes.data, pct.ctypes.data)
+        return dtc, pct
 
     def step(self):
         """One sfincs_update iteration with the reference hot path (sfincs_lib.f90:379-648)."""

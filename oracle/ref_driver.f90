@@ -86,6 +86,7 @@ contains
       case ('wiggle_factor');      wiggle_factor = real(v, 4)
       case ('wiggle_threshold');   wiggle_threshold = real(v, 4)
       case ('dtmax');              dtmax = real(v, 4)
+      case ('alpha');              alfa = real(v, 4)
       case ('rhow');               rhow = real(v, 4)
       case ('dx');                 dx = real(v, 4)
       case ('dy');                 dy = real(v, 4)
@@ -115,6 +116,14 @@ contains
       case ('z_index_uv_mu');   call put4(z_index_uv_mu)
       case ('z_index_uv_nd');   call put4(z_index_uv_nd)
       case ('z_index_uv_nu');   call put4(z_index_uv_nu)
+      case ('z_index_uv_md1');  call put4(z_index_uv_md1)
+      case ('z_index_uv_md2');  call put4(z_index_uv_md2)
+      case ('z_index_uv_mu1');  call put4(z_index_uv_mu1)
+      case ('z_index_uv_mu2');  call put4(z_index_uv_mu2)
+      case ('z_index_uv_nd1');  call put4(z_index_uv_nd1)
+      case ('z_index_uv_nd2');  call put4(z_index_uv_nd2)
+      case ('z_index_uv_nu1');  call put4(z_index_uv_nu1)
+      case ('z_index_uv_nu2');  call put4(z_index_uv_nu2)
       case ('nmindsrc');        call put4(nmindsrc)
       case default
       end select
@@ -263,6 +272,16 @@ contains
       case ('x73');      call c_f_pointer(p, f, [nbytes / 4]); f = reshape(x73, [10000])
       case default
       end select
+   end subroutine
+
+   ! timestep_analysis_finalize(nt) (sfincs_timestep_analysis.f90:77): the per-cell fields
+   subroutine ref_tsa_finalize(nt_, dtc, pct) bind(C, name='ref_tsa_finalize')
+      integer(c_int32_t), value :: nt_
+      real(c_float), intent(out) :: dtc(np), pct(np)
+      call timestep_analysis_finalize(int(nt_))
+      dtc = timestep_analysis_average_required_timestep_per_cell
+      pct = timestep_analysis_percentage_limiting_per_cell
+      deallocate(timestep_analysis_average_required_timestep_per_cell, timestep_analysis_percentage_limiting_per_cell)
    end subroutine
 
    ! compute_fluxes(dt, tloop) (sfincs_momentum.f90:9); returns min_dt

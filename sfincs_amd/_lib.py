@@ -76,6 +76,7 @@ def _sig(lib):
         "sfx_model_set_rain": (I, [P, i32, P, P]),
         "sfx_model_set_river_boundary": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_set_boundary_ig": (I, [P, P]),
+        "sfx_model_timestep_analysis_per_cell": (I, [P, P, P, P, C.c_int64, P, P]),
         "sfx_model_set_wind": (I, [P, i32, P, P, P]),
         "sfx_model_initialize": (I, [P]),
         "sfx_model_set_restart": (I, [P, s]),
@@ -256,6 +257,17 @@ class Model:
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)
         check(lib().sfx_model_set_rain(self.h, len(t), ptr(t), ptr(v)), "set_rain")
+
+    def timestep_analysis_per_cell(self, tsa_sum, tsa_wet, tsa_lim, nt):
+        """timestep_analysis_finalize: per cell (average required step, % of steps limiting)."""
+        a = np.ascontiguousarray(tsa_sum, np.float32)
+        b = np.ascontiguousarray(tsa_wet, np.int32)
+        c = np.ascontiguousarray(tsa_lim, np.int32)
+        n = self.sizes()["np"]
+        dtc, pct = np.empty(n, np.float32), np.empty(n, np.float32)
+        check(lib().sfx_model_timestep_analysis_per_cell(self.h, ptr(a), ptr(b), ptr(c), int(nt), ptr(dtc), ptr(pct)),
+              "timestep_analysis_per_cell")
+        return dtc, pct
 
     def set_boundary_ig(self, zsi):
         """Incoming infragravity levels (bzi file): [nt, nbnd] at the water-level series times."""

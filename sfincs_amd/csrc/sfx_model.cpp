@@ -1051,6 +1051,36 @@ int sfx::write_restart_arrays(const char* path, const double* zs, int32_t np, co
   return SFX_OK;
 }
 
+// timestep_analysis_finalize (sfincs_timestep_analysis.f90:77-200): per active cell, over
+// its (up to 8) uv-points with kcuv 1 or 6, the smallest average required step
+// (sum / max(wet, 1)) times alpha, capped at dtmax (-1 when the cell has no such point),
+// and the largest count of limiting sweeps in percent of nt; inactive cells get 0.
+extern "C" int sfx_model_timestep_analysis_per_cell(const sfx_model* M, const float* tsa_sum, const int32_t* tsa_wet,
+                                                    const int32_t* tsa_lim, int64_t nt, float* dt_cell,
+                                                    float* pct_cell) {
+  if (!M || !M->initialized || !tsa_sum || !tsa_wet || !tsa_lim || !dt_cell || !pct_cell || nt <= 0) {
+    set_error("sfx_model_timestep_analysis_per_cell: bad args");
+    return SFX_ERR_ARG;
+  }
+  const std::vector<int32_t>* nbr[8] = {&M->z_index_uv_md1, &M->z_index_uv_md2, &M->z_index_uv_mu1, &M->z_index_uv_mu2,
+                                        &M->z_index_uv_nd1, &M->z_index_uv_nd2, &M->z_index_uv_nu1, &M->z_index_uv_nu2};
+  for (int32_t nm = 0; nm < M->np; ++nm) {
+    dt_cell[nm] = 0.0f; pct_cell[nm] = 0.0f;
+    if (M->kcs[nm] <= 0) continue;
+    float dtm = 1.0e6f;
+    int32_t tmsl = 0;
+    for (int j = 0; j < 8; ++j) {
+      const int32_t ip = (*nbr[j])[nm];
+      if (ip <= 0 || !(M->kcuv[ip - 1] == 1 || M->kcuv[ip - 1] == 6)) continue;
+      dtm = std::min(dtm, tsa_sum[ip - 1] / (float)std::max(tsa_wet[ip - 1], 1));
+      tmsl = std::max(tmsl, tsa_lim[ip - 1]);
+    }
+    dt_cell[nm] = dtm < 1.0e5f ? std::min(dtm * M->p.alpha, M->dtmax) : -1.0f;
+    pct_cell[nm] = 100.0f * (float)tmsl / (float)nt;
+  }
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_get_state(const sfx_model* M, const char* name, void* dst, size_t nbytes) {
   return sfx_model_get_table(M, name, dst, nbytes);
 }

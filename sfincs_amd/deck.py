@@ -46,6 +46,8 @@ def write(sc, folder):
             lines.append(f"advection = {int(v)}")
         elif k in ("store_velmax", "store_fluxmax", "store_twet"):   # the input names of the switches
             lines.append(f"{'store' + k[6:]} = {int(v)}")
+        elif k == "timestep_analysis":
+            lines.append(f"timestep_analysis = {int(v)}")
         else:
             lines.append(f"{k} = {_fmt(v)}")
     if sc.kind == "regular":

@@ -143,6 +143,12 @@ module sfx_capi
          type(c_ptr), value :: dst
          integer(c_size_t), value :: nbytes
       end function
+      integer(c_int) function sfx_model_timestep_analysis_per_cell(m, tsum, twet, tlim, nt, dtc, pct) &
+            bind(C, name='sfx_model_timestep_analysis_per_cell')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: m, tsum, twet, tlim, dtc, pct
+         integer(c_int64_t), value :: nt
+      end function
       integer(c_int) function sfx_get_time(h, t, dt, nt) bind(C, name='sfx_get_time')
          import :: c_int, c_ptr, c_double, c_float, c_int64_t
          type(c_ptr), value :: h
@@ -171,6 +177,7 @@ module sfincs_lib
    real*4, target :: dt = 0.0
    real*8 :: host_t0 = 0.0d0, host_t1 = 0.0d0
    integer :: host_np = 0, host_npuv = 0, host_ncuv = 0
+   integer :: host_flags(4) = 0   ! store_velmax, store_fluxmax, store_twet, timestep_analysis
    real*8, allocatable, target :: host_zs(:)
    real*4, allocatable, target :: host_zsmax(:), host_zb(:)
    type(c_ptr) :: model = c_null_ptr
@@ -383,6 +390,7 @@ contains
       call inp_int('storetwet', i); p%store_twet = merge(1, 0, i == 1)
       call inp_real('twet_threshold', p%twet_threshold)
       call inp_flag('timestep_analysis', p%timestep_analysis)
+      host_flags = [p%store_velmax, p%store_fluxmax, p%store_twet, p%timestep_analysis]
       host_t0 = p%t0; host_t1 = p%t1
       x0 = 0.0; y0 = 0.0; dx = 0.0; dy = 0.0; rot = 0.0; nmax = 0; mmax = 0
       call inp_int('nmax', nmax); call inp_int('mmax', mmax)
@@ -681,6 +689,61 @@ contains
       rc = sfx_get_state(handle, 'zsmax'//c_null_char, c_loc(host_zsmax), int(4 * host_np, c_size_t))
    end subroutine
 
+   ! A per-cell f32 state field of the device to a raw file (active cells, reference order).
+   subroutine write_cell_f32(name, n, fname)
+      character(*), intent(in) :: name, fname
+      integer, intent(in) :: n
+      real(c_float), allocatable, target :: buf(:)
+      integer :: u
+      integer(c_int) :: rc
+      allocate(buf(n))
+      rc = sfx_get_state(handle, name//c_null_char, c_loc(buf), int(4 * n, c_size_t))
+      if (rc == 0) then
+         open(newunit=u, file=fname, form='unformatted', access='stream', status='replace')
+         write(u) buf
+         close(u)
+      endif
+      deallocate(buf)
+   end subroutine
+
+   ! timestep_analysis_finalize and _write_log (sfincs_timestep_analysis.f90:77-247): the
+   ! per-cell average required step and percentage limiting, and the most limiting uv-point.
+   subroutine write_timestep_analysis()
+      real(c_float), allocatable, target :: tsum(:), dtc(:), pct(:)
+      integer(c_int32_t), allocatable, target :: twet(:), tlim(:)
+      real(c_double) :: tt
+      real(c_float) :: dtt
+      integer(c_int64_t) :: nt
+      integer(c_int) :: rc
+      integer :: u, ip
+      character(256) :: line
+      allocate(tsum(host_npuv), twet(host_npuv), tlim(host_npuv), dtc(host_np), pct(host_np))
+      rc = sfx_get_state(handle, 'tsa_sum'//c_null_char, c_loc(tsum), int(4 * host_npuv, c_size_t))
+      if (rc == 0) rc = sfx_get_state(handle, 'tsa_wet'//c_null_char, c_loc(twet), int(4 * host_npuv, c_size_t))
+      if (rc == 0) rc = sfx_get_state(handle, 'tsa_lim'//c_null_char, c_loc(tlim), int(4 * host_npuv, c_size_t))
+      if (rc == 0) rc = sfx_get_time(handle, tt, dtt, nt)
+      if (rc == 0 .and. nt > 0) rc = sfx_model_timestep_analysis_per_cell(model, c_loc(tsum), c_loc(twet), &
+                                                                         c_loc(tlim), nt, c_loc(dtc), c_loc(pct))
+      if (rc == 0 .and. nt > 0) then
+         open(newunit=u, file='timestep_analysis_dt.dat', form='unformatted', access='stream', status='replace')
+         write(u) dtc
+         close(u)
+         open(newunit=u, file='timestep_analysis_pct.dat', form='unformatted', access='stream', status='replace')
+         write(u) pct
+         close(u)
+         call logmsg('----------- Time step analysis -----------')
+         if (maxval(tlim) > 0) then
+            ip = maxloc(tlim, dim=1)
+            write(line, '(a,i12)') ' Index of most time step limiting uv point : ', ip
+            call logmsg(trim(line))
+            write(line, '(a,f10.1,a)') ' Percentage of time point was limiting     : ', &
+                 real(maxval(tlim)) / real(max(maxval(twet), 1)) * 100.0, ' %'
+            call logmsg(trim(line))
+         endif
+      endif
+      deallocate(tsum, twet, tlim, dtc, pct)
+   end subroutine
+
    ! ---- sfincs_finalize (sfincs_lib.f90:687-819) ---------------------------------------
    function sfincs_finalize() result(ierr)
       integer :: ierr, u
@@ -694,6 +757,11 @@ contains
          open(newunit=u, file='zsmax.dat', form='unformatted', access='stream', status='replace')
          write(u) host_zsmax
          close(u)
+         ! compute_store_variables fields, as the maximum output writes them (sfincs_output.f90:490-512)
+         if (host_flags(1) == 1) call write_cell_f32('vmax', host_np, 'vmax.dat')
+         if (host_flags(2) == 1) call write_cell_f32('qmax', host_np, 'qmax.dat')
+         if (host_flags(3) == 1) call write_cell_f32('twet', host_np, 'twet.dat')
+         if (host_flags(4) == 1) call write_timestep_analysis()
          rc = sfx_destroy(handle)
          handle = c_null_ptr
       endif

@@ -73,7 +73,7 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
         sc = W.with_river(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0), slope=2.0e-3)
     elif kind == "store_switches":
         sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, store_velmax=1, store_fluxmax=1, store_twet=1,
-                            twet_threshold=0.02)
+                            twet_threshold=0.02, timestep_analysis=1)
     elif kind == "quadtree":
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     else:
@@ -153,3 +153,28 @@ def test_deck_reader_applies_the_restart_file(tmp_path):
         a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
         assert a is not None and a == b, name
     assert _table(lib, mf, "zs") != _table(lib, base.h.value, "zs")
+
+
+@needs_host
+@pytest.mark.gpu
+def test_fortran_host_writes_store_fields_and_timestep_analysis(tmp_path):
+    """storevelmax / storefluxmax / storetwet and timestep_analysis in sfincs.inp: the host
+    executable writes vmax, qmax, twet and the per-cell time step analysis at finalize,
+    equal to the Python-driven run of the same scenario."""
+    print("case fortran host store fields", flush=True)
+    from sfincs_amd import _lib
+    sc = W.regular_case(28, 36, dx=50.0, seed=45, t_end=900.0, tide_amp=2.0, store_velmax=1, store_fluxmax=1,
+                        store_twet=1, timestep_analysis=1)
+    d = deck.write(sc, str(tmp_path))
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = sc.build_model()
+    inst = _lib.Instance(m)
+    inst.update(-1000.0)
+    for f in ("vmax", "qmax", "twet"):
+        a = np.fromfile(os.path.join(d, f + ".dat"), np.float32)
+        assert np.array_equal(a, inst.get(f)), f
+    dtc, pct = m.timestep_analysis_per_cell(inst.get("tsa_sum"), inst.get("tsa_wet"), inst.get("tsa_lim"), inst.time()[2])
+    assert np.array_equal(np.fromfile(os.path.join(d, "timestep_analysis_dt.dat"), np.float32), dtc)
+    assert np.array_equal(np.fromfile(os.path.join(d, "timestep_analysis_pct.dat"), np.float32), pct)
+    assert np.any(pct > 0)

@@ -141,3 +141,24 @@ def test_timestep_analysis_bit_exact(kind):
             assert np.array_equal(x, y), f"step {i} field {n}"
     assert a.state("tsa_wet").max() > 0 and a.state("tsa_lim").max() > 0
     assert a.state("tsa_lim").sum() >= 150   # at least one limiting point per step
+
+
+@pytest.mark.parametrize("kind", ["regular", "quadtree"])
+def test_timestep_analysis_per_cell_bit_exact(kind):
+    """sfx_model_timestep_analysis_per_cell against the reference's own
+    timestep_analysis_finalize on the same accumulators (after 200 reference steps)."""
+    if kind == "regular":
+        sc = W.regular_case(30, 40, dx=50.0, seed=31, t_end=3600.0, tide_amp=2.0, timestep_analysis=1)
+    else:
+        sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=32, t_end=3600.0, surge_amp=4.0, timestep_analysis=1)
+    m = sc.build_model()
+    b = O.OracleRun(m).configure(sc.params, m)
+    r = O.RefRun(b)
+    for _ in range(200):
+        r.step()
+    n = m.sizes()["npuv"]
+    nt = int(b.get("nt"))
+    dtc, pct = m.timestep_analysis_per_cell(b.state("tsa_sum")[:n], b.state("tsa_wet")[:n], b.state("tsa_lim")[:n], nt)
+    rdt, rpct = r.tsa_finalize(nt)
+    assert np.array_equal(dtc, rdt) and np.array_equal(pct, rpct)
+    assert np.any(dtc > 0) and np.any(dtc == -1.0) and np.any(pct > 0)

@@ -174,6 +174,9 @@ int  sfx_model_get_state(const sfx_model* m, const char* name, void* dst, size_t
  * uv-point of the cell was counted) and the percentage of steps it limited. np each. */
 int  sfx_model_timestep_analysis_per_cell(const sfx_model* m, const float* tsa_sum, const int32_t* tsa_wet,
                                           const int32_t* tsa_lim, int64_t nt, float* dt_cell, float* pct_cell);
+/* The active cell (1-based; 0 outside the mesh) holding point (x, y): the lookup behind
+ * the BMI get_sfincs_cell_index / get_sfincs_cell_indices (sfincs_bmi.f90:299-349). */
+int  sfx_model_find_cell(const sfx_model* m, float x, float y, int32_t* idx);
 void sfx_model_destroy(sfx_model* m);
 
 /* ---- device instance ---------------------------------------------------------------

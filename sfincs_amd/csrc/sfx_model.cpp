@@ -1081,6 +1081,14 @@ extern "C" int sfx_model_timestep_analysis_per_cell(const sfx_model* M, const fl
   return SFX_OK;
 }
 
+// The active cell holding point (x, y), 1-based, 0 outside the mesh: the lookup of the
+// BMI get_sfincs_cell_index (sfincs_bmi.f90:299-316, find_quadtree_cell).
+extern "C" int sfx_model_find_cell(const sfx_model* M, float x, float y, int32_t* idx) {
+  if (!M || !M->mesh_built || !idx) { set_error("sfx_model_find_cell: bad args"); return SFX_ERR_ARG; }
+  *idx = M->find_cell_xy(x, y);
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_get_state(const sfx_model* M, const char* name, void* dst, size_t nbytes) {
   return sfx_model_get_table(M, name, dst, nbytes);
 }

@@ -149,6 +149,12 @@ module sfx_capi
          type(c_ptr), value :: m, tsum, twet, tlim, dtc, pct
          integer(c_int64_t), value :: nt
       end function
+      integer(c_int) function sfx_model_find_cell(m, x, y, idx) bind(C, name='sfx_model_find_cell')
+         import :: c_int, c_ptr, c_float, c_int32_t
+         type(c_ptr), value :: m
+         real(c_float), value :: x, y
+         integer(c_int32_t), intent(out) :: idx
+      end function
       integer(c_int) function sfx_get_time(h, t, dt, nt) bind(C, name='sfx_get_time')
          import :: c_int, c_ptr, c_double, c_float, c_int64_t
          type(c_ptr), value :: h
@@ -172,6 +178,7 @@ module sfincs_lib
    private
    public :: sfincs_initialize, sfincs_update, sfincs_finalize, t, dt, sfincs_read_model, host_model
    public :: host_np, host_npuv, host_ncuv, host_zs, host_zsmax, host_zb, host_sync, host_t0, host_t1
+   public :: host_xz, host_yz, host_zmin, host_area, host_iref, host_mesh_fields, host_last_error, host_find_cell
 
    real*8, target :: t = 0.0d0
    real*4, target :: dt = 0.0
@@ -180,6 +187,9 @@ module sfincs_lib
    integer :: host_flags(4) = 0   ! store_velmax, store_fluxmax, store_twet, timestep_analysis
    real*8, allocatable, target :: host_zs(:)
    real*4, allocatable, target :: host_zsmax(:), host_zb(:)
+   ! mesh fields for the BMI (cell centres, bed or lowest subgrid level, cell areas)
+   real*4, allocatable, target :: host_xz(:), host_yz(:), host_zmin(:), host_area(:)
+   integer(c_int8_t), allocatable, target :: host_iref(:)
    type(c_ptr) :: model = c_null_ptr
    type(c_ptr) :: handle = c_null_ptr
    integer :: logu = 0
@@ -306,7 +316,6 @@ contains
       close(u)
    end function
 
-   ! ---- sfincs_initialize (sfincs_lib.f90:85-333) -------------------------------------
    ! ---- sfincs_initialize (sfincs_lib.f90:85-335) --------------------------------------
    ! Reads the input deck into an initialized host model, then moves it to HBM.
    function sfincs_initialize() result(ierr)
@@ -314,6 +323,7 @@ contains
       integer(c_int) :: rc
       ierr = sfincs_read_model()
       if (ierr /= 0) return
+      call host_mesh_fields()
       call table_size('zs', 8, host_np)
       call table_size('uv_index_z_nm', 4, host_npuv)
       allocate(host_zs(host_np), host_zsmax(host_np))
@@ -606,6 +616,46 @@ contains
       host_model = model
    end function
 
+   ! The library's last error message, NUL-terminated, into buf(1:n).
+   subroutine host_last_error(buf, n)
+      integer, intent(in) :: n
+      character(kind=c_char), intent(out) :: buf(n)
+      integer(c_int) :: rc
+      buf = c_null_char
+      rc = sfx_last_error(buf, int(n, c_size_t))
+   end subroutine
+
+   ! The active cell (1-based, 0 outside) holding (x, y).
+   integer function host_find_cell(x, y)
+      real(c_double), intent(in) :: x, y
+      integer(c_int32_t) :: idx
+      integer(c_int) :: rc
+      idx = 0
+      if (c_associated(model)) rc = sfx_model_find_cell(model, real(x, c_float), real(y, c_float), idx)
+      host_find_cell = idx
+   end function
+
+   ! Host copies of the mesh fields the BMI hands out (get_value_ptr, get_sfincs_cell_area).
+   subroutine host_mesh_fields()
+      integer :: n
+      call table_size('z_xz', 4, n)
+      if (allocated(host_xz)) deallocate(host_xz, host_yz, host_zb, host_zmin, host_iref, host_area)
+      host_np = n
+      allocate(host_xz(n), host_yz(n), host_iref(n))
+      call get_f32('z_xz', host_xz); call get_f32('z_yz', host_yz)
+      call table_size('zb', 4, n); allocate(host_zb(n)); if (n > 0) call get_f32('zb', host_zb)
+      call table_size('z_zmin', 4, n); allocate(host_zmin(n)); if (n > 0) call get_f32('z_zmin', host_zmin)
+      call table_size('cell_area', 4, n); allocate(host_area(n)); call get_f32('cell_area', host_area)
+      n = sfx_model_get_table(model, 'z_flags_iref'//c_null_char, c_loc(host_iref), int(size(host_iref), c_size_t))
+   contains
+      subroutine get_f32(name, a)
+         character(*), intent(in) :: name
+         real*4, target, intent(inout) :: a(:)
+         integer :: rc
+         rc = sfx_model_get_table(model, trim(name)//c_null_char, c_loc(a), int(4 * size(a), c_size_t))
+      end subroutine
+   end subroutine
+
    subroutine table_size(name, esize, n)
       character(*), intent(in) :: name
       integer, intent(in) :: esize
@@ -800,6 +850,7 @@ contains
    ! Deck reader alone (no GPU): builds the host model that initialize would move to HBM.
    integer(c_int) function sfincs_host_read_model() result(ierr) bind(C, name='sfincs_host_read_model')
       ierr = sfincs_read_model()
+      if (ierr == 0) call host_mesh_fields()
    end function
    type(c_ptr) function sfincs_host_model() bind(C, name='sfincs_host_model')
       sfincs_host_model = host_model()
@@ -826,26 +877,148 @@ contains
    end function
    ! Host pointers to zs / zsmax / zb after an explicit device-to-host copy
    ! (the reference hands out c_loc of its arrays, sfincs_bmi.f90:95-129).
+   ! get_value_ptr (sfincs_bmi.f90:95-129): zs and zsmax after an explicit device-to-host
+   ! copy; the mesh fields z_xz, z_yz, zb, subgrid_z_zmin from their host copies. qext and
+   ! uorb (coupling inputs of processes outside this build) are not available.
    integer(c_int) function get_value_ptr(c_var_name, x) result(ierr) bind(C, name='get_value_ptr')
       character(kind=c_char), intent(in) :: c_var_name(*)
       type(c_ptr), intent(out) :: x
       character(64) :: nm
-      integer :: i
-      nm = ' '
-      do i = 1, 64
-         if (c_var_name(i) == c_null_char) exit
-         nm(i:i) = c_var_name(i)
-      enddo
-      call host_sync()
+      nm = bmi_name(c_var_name)
       ierr = 0
       select case (trim(nm))
       case ('zs')
+         call host_sync()
          x = c_loc(host_zs)
       case ('zsmax')
+         call host_sync()
          x = c_loc(host_zsmax)
+      case ('z_xz')
+         x = c_loc(host_xz)
+      case ('z_yz')
+         x = c_loc(host_yz)
+      case ('zb')
+         x = c_loc(host_zb)
+      case ('subgrid_z_zmin')
+         x = c_loc(host_zmin)
       case default
          x = c_null_ptr
          ierr = -1
       end select
+   end function
+
+   character(64) function bmi_name(c)
+      character(kind=c_char), intent(in) :: c(*)
+      integer :: i
+      bmi_name = ' '
+      do i = 1, 64
+         if (c(i) == c_null_char) exit
+         bmi_name(i:i) = c(i)
+      enddo
+   end function
+
+   ! get_var_shape / get_var_type / get_var_rank (sfincs_bmi.f90:131-224)
+   integer(c_int) function get_var_shape(c_var_name, var_shape) result(ierr) bind(C, name='get_var_shape')
+      character(kind=c_char), intent(in) :: c_var_name(*)
+      integer(c_int), intent(inout) :: var_shape(6)
+      ierr = 0
+      var_shape = 0
+      select case (trim(bmi_name(c_var_name)))
+      case ('z_xz', 'z_yz', 'zs', 'zsmax')
+         var_shape(1) = host_np
+      case ('zb')
+         var_shape(1) = size(host_zb)
+      case ('subgrid_z_zmin')
+         var_shape(1) = size(host_zmin)
+      case default
+         ierr = -1
+      end select
+   end function
+
+   integer(c_int) function get_var_type(c_var_name, c_type) result(ierr) bind(C, name='get_var_type')
+      character(kind=c_char), intent(in) :: c_var_name(*)
+      character(kind=c_char), intent(out) :: c_type(64)
+      character(8) :: tn
+      integer :: i
+      ierr = 0
+      select case (trim(bmi_name(c_var_name)))
+      case ('zs')
+         tn = 'double'
+      case ('z_xz', 'z_yz', 'zb', 'subgrid_z_zmin', 'zsmax')
+         tn = 'float'
+      case default
+         tn = ' '
+         ierr = -1
+      end select
+      c_type = c_null_char
+      do i = 1, len_trim(tn)
+         c_type(i) = tn(i:i)
+      enddo
+   end function
+
+   integer(c_int) function get_var_rank(c_var_name, rank) result(ierr) bind(C, name='get_var_rank')
+      character(kind=c_char), intent(in) :: c_var_name(*)
+      integer(c_int), intent(out) :: rank
+      ierr = 0
+      rank = 1
+      select case (trim(bmi_name(c_var_name)))
+      case ('z_xz', 'z_yz', 'zs', 'zb', 'subgrid_z_zmin', 'zsmax')
+      case default
+         ierr = -1
+      end select
+   end function
+
+   ! get_sfincs_cell_index(es) and get_sfincs_cell_area (sfincs_bmi.f90:299-370)
+   integer(c_int) function get_sfincs_cell_index(x, y, indx) result(ierr) bind(C, name='get_sfincs_cell_index')
+      real(c_double), intent(in) :: x, y
+      integer(c_int), intent(out) :: indx
+      indx = host_find_cell(x, y)
+      ierr = 0
+   end function
+
+   integer(c_int) function get_sfincs_cell_indices(x, y, indx, n) result(ierr) bind(C, name='get_sfincs_cell_indices')
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: x(n), y(n)
+      integer(c_int), intent(out) :: indx(n)
+      integer :: i
+      do i = 1, n
+         indx(i) = host_find_cell(x(i), y(i))
+      enddo
+      ierr = 0
+   end function
+
+   integer(c_int) function get_sfincs_cell_area(indx, area) result(ierr) bind(C, name='get_sfincs_cell_area')
+      integer(c_int), intent(in) :: indx
+      real(c_double), intent(out) :: area
+      ierr = 0
+      area = 0.0d0
+      if (indx < 1 .or. indx > size(host_iref)) then
+         ierr = -1
+      else
+         area = real(host_area(host_iref(indx)), c_double)
+      endif
+   end function
+
+   integer(c_int) function get_last_bmi_error(c_error) result(ierr) bind(C, name='get_last_bmi_error')
+      character(kind=c_char), intent(out) :: c_error(1024)
+      call host_last_error(c_error, 1024)
+      ierr = 0
+   end function
+
+   ! Hooks of processes outside this build: the qext switch (external sources into
+   ! continuity), zbuv updates after a bed change and wave-enhanced roughness. They return
+   ! -1 so that a coupler notices (sfincs_bmi.f90:226-297).
+   integer(c_int) function set_logical(c_flag_name, ival) result(ierr) bind(C, name='set_logical')
+      character(kind=c_char), intent(in) :: c_flag_name(*)
+      integer(c_int) :: ival(*)
+      ierr = -1
+   end function
+
+   integer(c_int) function update_zbuv() result(ierr) bind(C, name='update_zbuv')
+      ierr = -1
+   end function
+
+   integer(c_int) function update_apparent_roughness() result(ierr) bind(C, name='update_apparent_roughness')
+      ierr = -1
    end function
 end module sfincs_bmi

@@ -25,7 +25,9 @@ needs_host = pytest.mark.skipif(not os.path.exists(HOSTLIB), reason="Fortran hos
 def test_host_exports_driver_and_bmi_names():
     lib = ctypes.CDLL(HOSTLIB)
     for name in ("initialize", "update", "update_until", "finalize", "get_value_ptr", "get_current_time",
-                 "get_start_time", "get_end_time", "get_time_step"):
+                 "get_start_time", "get_end_time", "get_time_step", "get_var_shape", "get_var_type",
+                 "get_var_rank", "get_sfincs_cell_index", "get_sfincs_cell_indices", "get_sfincs_cell_area",
+                 "get_last_bmi_error", "set_logical", "update_zbuv", "update_apparent_roughness"):
         assert hasattr(lib, name), name
     syms = subprocess.run(["nm", "-D", HOSTLIB], capture_output=True, text=True).stdout
     for name in ("sfincs_initialize", "sfincs_update", "sfincs_finalize"):
@@ -178,3 +180,52 @@ def test_fortran_host_writes_store_fields_and_timestep_analysis(tmp_path):
     assert np.array_equal(np.fromfile(os.path.join(d, "timestep_analysis_dt.dat"), np.float32), dtc)
     assert np.array_equal(np.fromfile(os.path.join(d, "timestep_analysis_pct.dat"), np.float32), pct)
     assert np.any(pct > 0)
+
+
+@needs_host
+@pytest.mark.parametrize("kind", ["regular", "quadtree"])
+def test_bmi_mesh_queries(kind, tmp_path):
+    """The BMI's mesh queries after a host-only deck read (no GPU): cell lookup by point
+    (sfincs_bmi.f90:299-349), cell areas, and the mesh fields with their shape, type and
+    rank (:95-224); switches of processes outside this build answer -1."""
+    if kind == "regular":
+        sc = W.regular_case(12, 16, dx=50.0, seed=46, t_end=600.0)
+    else:
+        sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=47, t_end=600.0)
+    d = deck.write(sc, str(tmp_path))
+    host = ctypes.CDLL(HOSTLIB)
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        assert host.sfincs_host_read_model() == 0
+    finally:
+        os.chdir(cwd)
+    m = sc.build_model()
+    np_ = m.sizes()["np"]
+    xz, yz, iref = m.table("z_xz"), m.table("z_yz"), m.table("z_flags_iref")
+    area = m.table("cell_area")
+    pick = np.arange(0, np_, max(1, np_ // 17))
+    idx = (ctypes.c_int * len(pick))()
+    xs = (ctypes.c_double * len(pick))(*xz[pick].astype(np.float64))
+    ys = (ctypes.c_double * len(pick))(*yz[pick].astype(np.float64))
+    assert host.get_sfincs_cell_indices(xs, ys, idx, ctypes.c_int(len(pick))) == 0
+    assert list(idx) == list(pick + 1)
+    one = ctypes.c_int()
+    assert host.get_sfincs_cell_index(ctypes.byref(ctypes.c_double(-1.0e6)), ctypes.byref(ctypes.c_double(0.0)),
+                                      ctypes.byref(one)) == 0 and one.value == 0
+    a = ctypes.c_double()
+    c = ctypes.c_int(int(pick[-1]) + 1)
+    assert host.get_sfincs_cell_area(ctypes.byref(c), ctypes.byref(a)) == 0
+    assert a.value == float(area[iref[pick[-1]] - 1])
+    shape = (ctypes.c_int * 6)()
+    assert host.get_var_shape(b"z_xz", shape) == 0 and shape[0] == np_
+    tbuf = ctypes.create_string_buffer(64)
+    assert host.get_var_type(b"zs", tbuf) == 0 and tbuf.value == b"double"
+    rank = ctypes.c_int()
+    assert host.get_var_rank(b"zb", ctypes.byref(rank)) == 0 and rank.value == 1
+    ptr = ctypes.c_void_p()
+    assert host.get_value_ptr(b"z_yz", ctypes.byref(ptr)) == 0
+    got = np.ctypeslib.as_array((ctypes.c_float * np_).from_address(ptr.value))
+    assert np.array_equal(got, yz)
+    assert host.get_value_ptr(b"qext", ctypes.byref(ptr)) == -1
+    assert host.update_zbuv() == -1

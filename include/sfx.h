@@ -81,6 +81,9 @@ typedef struct sfx_params {
   float twet_threshold;      /* 0.01 (sfincs_input.f90:285) */
   int32_t timestep_analysis; /* 0 (sfincs_input.f90:288): per uv-point time step statistics,
                               * read as tsa_sum / tsa_wet / tsa_lim (sfx_get_state) */
+  int32_t use_qext;          /* 0: external sources per cell (m/s), written with sfx_set_state
+                              * "qext" (sfincs_continuity.f90:122-128, 473-497); the BMI
+                              * host turns it on as sfincs_bmi.f90:60 does */
 } sfx_params;
 
 void sfx_params_default(sfx_params* p);

@@ -61,6 +61,7 @@ contains
       case ('store_fluxmax');      store_maximum_flux = (v /= 0.0d0)
       case ('store_twet');         store_twet = (v /= 0.0d0)
       case ('twet_threshold');     twet_threshold = real(v, 4)
+      case ('use_qext');           use_qext = (v /= 0.0d0)
       case ('timestep_analysis')   ! after dtmax and the sizes (initialize_timestep_analysis)
          timestep_analysis = (v /= 0.0d0)
          if (timestep_analysis) then
@@ -188,6 +189,7 @@ contains
       case ('vmax');         call putr(vmax)
       case ('qmax');         call putr(qmax)
       case ('twet');         call putr(twet)
+      case ('qext');         call putr(qext)
       case ('tauwu');        call putr(tauwu)
       case ('tauwv');        call putr(tauwv)
       case ('netprcp');      call putr(netprcp)

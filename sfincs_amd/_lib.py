@@ -26,7 +26,7 @@ class sfx_params(C.Structure):
         "uvmax", "nuvisc", "nuviscfac", "latitude", "g", "pi")] + [(n, C.c_int32) for n in (
         "advection", "advection_scheme", "viscosity", "coriolis", "friction2d", "advection_mask",
         "wiggle_suppression", "store_zsmax", "bndtype", "h73table", "store_velmax", "store_fluxmax",
-        "store_twet")] + [("twet_threshold", C.c_float), ("timestep_analysis", C.c_int32)]
+        "store_twet")] + [("twet_threshold", C.c_float), ("timestep_analysis", C.c_int32), ("use_qext", C.c_int32)]
 
 
 class sfx_quadtree_input(C.Structure):
@@ -314,7 +314,7 @@ class Model:
 STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv": np.float32,
                 "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
                 "uvmean": np.float32, "vmax": np.float32, "qmax": np.float32, "twet": np.float32,
-                "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32}
+                "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32, "qext": np.float32}
 
 
 class Instance:
@@ -329,7 +329,7 @@ class Instance:
 
     def _n(self, name):
         s = self.sizes
-        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet"):
+        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet", "qext"):
             return s["np"]
         if name in ("q", "uv"):
             return s["npuv"] + s["ncuv"] + 1

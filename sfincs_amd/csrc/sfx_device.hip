@@ -122,6 +122,7 @@ struct DevInst {
   // the last sweep (wet points), its running sum, the sweeps wet and those limiting
   float* tsa_req; float* tsa_sum; int32_t* tsa_wet; int32_t* tsa_lim;
   float* tsa_min;                // the last sweep's minimum time step
+  float* qext;                   // external source per cell, m/s (use_qext)
   uint8_t* kcs;                  // kcs (KCS_MASK) | KCS_SOURCE (point source) | KCS_STEADY (continuity)
   const float* zb;
   const int8_t* z_iref;
@@ -1304,6 +1305,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
   R zsmax = zsmaxp[c];
   const double zs_in = zs;
   const R zsmax_in = zsmax;
+  const R qx = (I.c.flags & sfx::F_QEXT) ? (R)G(I.qext)[c] : R(0);
   if (SUBGRID) {
     double vol = G(I.vol)[c];
     const double vol_in = vol;   // as loaded: a source alone changes the volume too
@@ -1319,7 +1321,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
     R zs0 = zs0p[c], zsd = zsdp[c];
     const R zs0_in = zs0, zsd_in = zsd;
     const int nl = I.c.nl;
-    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, G(I.z_zmin)[c],
+    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, qx, iref, G(I.z_zmin)[c],
                          G(I.z_zmax)[c], G(I.z_volmax)[c], sfx::GRow<const float>{G(I.z_dep), (uint32_t)c * nl},
                          I.c, dt);
     if (!same_bits(vol, vol_in)) { G(I.vol)[c] = vol; wrote |= 2u; }
@@ -1335,7 +1337,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
           zs = sfx::vmax<double>(zs + (double)inc, (double)G(I.zb)[c]);
         }
     }
-    sfx::cell_regular<R>(zs, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, iref, I.c, dt);
+    sfx::cell_regular<R>(zs, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, qx, iref, I.c, dt);
   }
   if (!same_bits(zs, zs_in)) { G(I.zs)[c] = zs; wrote |= 1u; }
   if ((I.c.flags & sfx::F_ZSMAX) && !same_bits(zsmax, zsmax_in)) { zsmaxp[c] = zsmax; wrote |= 16u; }
@@ -1354,7 +1356,7 @@ __device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md,
                                                 R dt, R netprcp, bool dry_sky) {
   const uint8_t kraw = G(I.kcs)[c];
   const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
-                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & sfx::F_STORE);
+                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & (sfx::F_STORE | sfx::F_QEXT));
   if (quiet && (kraw & KCS_STEADY)) return 32u;   // skipped (counted as such)
   const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
   const bool steady = quiet && w == 0u;
@@ -1598,6 +1600,7 @@ uint32_t build_flags(const sfx_model* M) {
   if (p.store_fluxmax) f |= sfx::F_FLUXMAX;
   if (p.store_twet) f |= sfx::F_TWET;
   if (p.timestep_analysis) f |= sfx::F_TSA;
+  if (p.use_qext) f |= sfx::F_QEXT;
   if (M->wind) f |= sfx::F_WIND;
   if (M->precip) f |= sfx::F_PRECIP;
   if (p.h73table) f |= sfx::F_H73;
@@ -1653,6 +1656,10 @@ int upload_state(sfx_handle* H, const sfx_model* M) {
     I.tsa_req = H->upload(req.data(), ns); I.tsa_sum = H->upload(zero.data(), ns);
     I.tsa_wet = H->upload(izero.data(), ns); I.tsa_lim = H->upload(izero.data(), ns);
     I.tsa_min = H->upload(zero.data(), 1);
+  }
+  if (f & sfx::F_QEXT) {   // sfincs_domain.f90:2383-2389
+    std::vector<float> z(np, 0.f);
+    I.qext = H->upload(z.data(), np);
   }
   auto um = to_r<R>(M->uvmean);
   I.uvmean = H->upload(um.data(), um.size());
@@ -2104,6 +2111,11 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
     rc = H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
     if (rc) return rc;
     return clear_steady(H);
+  }
+  if (k == "qext") {   // external sources (m/s per cell), e.g. from a coupled model through the BMI
+    if (!I.qext || nbytes != np * 4) { set_error("sfx_set_state: qext needs use_qext and np f32 values"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(I.qext, src, nbytes, hipMemcpyHostToDevice));
+    return SFX_OK;
   }
   if (k == "vmax" || k == "qmax" || k == "twet") {   // e.g. the reset after each maximum output
     void* dst = k == "vmax" ? I.vmax : (k == "qmax" ? I.qmax : I.twet);

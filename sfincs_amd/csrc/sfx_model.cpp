@@ -38,7 +38,7 @@ extern "C" void sfx_params_default(sfx_params* p) {
   p->advection = 1; p->advection_scheme = 1; p->viscosity = 0; p->coriolis = 1;
   p->friction2d = 1; p->advection_mask = 1; p->wiggle_suppression = 1; p->store_zsmax = 1;
   p->store_velmax = 0; p->store_fluxmax = 0; p->store_twet = 0; p->twet_threshold = 0.01f;
-  p->timestep_analysis = 0;
+  p->timestep_analysis = 0; p->use_qext = 0;
   p->bndtype = 1; p->h73table = 0;
 }
 

@@ -18,7 +18,8 @@ enum : uint32_t {
   // compute_store_variables switches (sfincs_continuity.f90:626-720)
   F_VELMAX = 1u << 14, F_FLUXMAX = 1u << 15, F_TWET = 1u << 16,
   F_STORE = F_VELMAX | F_FLUXMAX | F_TWET,
-  F_TSA = 1u << 17   // timestep_analysis (sfincs_timestep_analysis.f90)
+  F_TSA = 1u << 17,  // timestep_analysis (sfincs_timestep_analysis.f90)
+  F_QEXT = 1u << 18  // external sources per cell (use_qext, BMI coupling)
 };
 
 constexpr int MAXREF = 8;
@@ -296,9 +297,10 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
 // Cell update of the regular continuity (sfincs_continuity.f90:112-155, 239-255).
 template <typename R>
 __device__ __forceinline__ void cell_regular(double& zs, R& zsmax, int kcs, R q_md, R q_mu, R q_nd, R q_nu,
-                                             R netprcp, int iref, const Consts& c, R dt) {
+                                             R netprcp, R qext, int iref, const Consts& c, R dt) {
   if (kcs == 1) {
     if (c.flags & F_PRECIP) zs = zs + (double)(netprcp * dt);
+    if (c.flags & F_QEXT) zs = zs + (double)(qext * dt);   // :122-128
     const R d = ((q_md - q_mu) * (R)c.dxrinv[iref - 1] + (q_nd - q_nu) * (R)c.dyrinv[iref - 1]) * dt;
     zs = zs + (double)d;
   }
@@ -308,7 +310,7 @@ __device__ __forceinline__ void cell_regular(double& zs, R& zsmax, int kcs, R q_
 // Cell update of the subgrid continuity (sfincs_continuity.f90:337-618).
 template <typename R, typename DepP>
 __device__ __forceinline__ void cell_subgrid(double& zs, double& vol, R& zs0, R& zsderv, R& zsmax, int kcs,
-                                             R q_md, R q_mu, R q_nd, R q_nu, R netprcp, int iref,
+                                             R q_md, R q_mu, R q_nd, R q_nu, R netprcp, R qext, int iref,
                                              float zmin, float zmax, float volmax, DepP dep,
                                              const Consts& c, R dt) {
   R dvol = R(0);
@@ -320,9 +322,10 @@ __device__ __forceinline__ void cell_subgrid(double& zs, double& vol, R& zs0, R&
   }
   if (kcs == 1 || kcs == 4) {
     const R a = c.cell_area[iref - 1];
-    if (c.flags & F_PRECIP) {
+    if (c.flags & (F_PRECIP | F_QEXT)) {   // :473-497
       R dzsdt = R(0);
-      dzsdt = dzsdt + netprcp;
+      if (c.flags & F_PRECIP) dzsdt = dzsdt + netprcp;
+      if (c.flags & F_QEXT) dzsdt = dzsdt + qext;
       dvol = dvol + dzsdt * a * dt;
     }
     vol = vol + (double)dvol;

@@ -20,7 +20,7 @@ module sfx_capi
       integer(c_int32_t) :: advection_mask, wiggle_suppression, store_zsmax, bndtype, h73table
       integer(c_int32_t) :: store_velmax, store_fluxmax, store_twet
       real(c_float) :: twet_threshold
-      integer(c_int32_t) :: timestep_analysis
+      integer(c_int32_t) :: timestep_analysis, use_qext
    end type
    type, bind(C) :: sfx_quadtree_input
       integer(c_int32_t) :: nr_points, nr_levels
@@ -143,6 +143,13 @@ module sfx_capi
          type(c_ptr), value :: dst
          integer(c_size_t), value :: nbytes
       end function
+      integer(c_int) function sfx_set_state(h, name, src, nbytes) bind(C, name='sfx_set_state')
+         import :: c_int, c_ptr, c_char, c_size_t
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: name(*)
+         type(c_ptr), value :: src
+         integer(c_size_t), value :: nbytes
+      end function
       integer(c_int) function sfx_model_timestep_analysis_per_cell(m, tsum, twet, tlim, nt, dtc, pct) &
             bind(C, name='sfx_model_timestep_analysis_per_cell')
          import :: c_int, c_ptr, c_int64_t
@@ -179,12 +186,18 @@ module sfincs_lib
    public :: sfincs_initialize, sfincs_update, sfincs_finalize, t, dt, sfincs_read_model, host_model
    public :: host_np, host_npuv, host_ncuv, host_zs, host_zsmax, host_zb, host_sync, host_t0, host_t1
    public :: host_xz, host_yz, host_zmin, host_area, host_iref, host_mesh_fields, host_last_error, host_find_cell
+   public :: host_qext_requested, host_qext_on, host_qext, host_qext_switch
 
    real*8, target :: t = 0.0d0
    real*4, target :: dt = 0.0
    real*8 :: host_t0 = 0.0d0, host_t1 = 0.0d0
    integer :: host_np = 0, host_npuv = 0, host_ncuv = 0
    integer :: host_flags(4) = 0   ! store_velmax, store_fluxmax, store_twet, timestep_analysis
+   ! external sources (qext, m/s per cell): requested by the BMI initialize as
+   ! sfincs_bmi.f90:60 does, written by a coupler through get_value_ptr, uploaded before
+   ! each update while switched on (set_logical)
+   logical :: host_qext_requested = .false., host_qext_on = .false.
+   real*4, allocatable, target :: host_qext(:)
    real*8, allocatable, target :: host_zs(:)
    real*4, allocatable, target :: host_zsmax(:), host_zb(:)
    ! mesh fields for the BMI (cell centres, bed or lowest subgrid level, cell areas)
@@ -325,6 +338,12 @@ contains
       if (ierr /= 0) return
       call host_mesh_fields()
       call table_size('zs', 8, host_np)
+      if (host_qext_requested) then
+         if (allocated(host_qext)) deallocate(host_qext)
+         allocate(host_qext(host_np))
+         host_qext = 0.0
+         host_qext_on = .true.
+      endif
       call table_size('uv_index_z_nm', 4, host_npuv)
       allocate(host_zs(host_np), host_zsmax(host_np))
       ! the replacement of initialize_openacc: tables and state to HBM once
@@ -401,6 +420,7 @@ contains
       call inp_real('twet_threshold', p%twet_threshold)
       call inp_flag('timestep_analysis', p%timestep_analysis)
       host_flags = [p%store_velmax, p%store_fluxmax, p%store_twet, p%timestep_analysis]
+      if (host_qext_requested) p%use_qext = 1
       host_t0 = p%t0; host_t1 = p%t1
       x0 = 0.0; y0 = 0.0; dx = 0.0; dy = 0.0; rot = 0.0; nmax = 0; mmax = 0
       call inp_int('nmax', nmax); call inp_int('mmax', mmax)
@@ -635,6 +655,24 @@ contains
       host_find_cell = idx
    end function
 
+   ! set_logical("qext") (sfincs_bmi.f90:226-245): external sources on or off; off writes
+   ! zeros to the device, which adds nothing. -1 when the model has no qext field.
+   integer function host_qext_switch(on)
+      logical, intent(in) :: on
+      real*4, allocatable, target :: z(:)
+      integer(c_int) :: rc
+      host_qext_switch = -1
+      if (.not. allocated(host_qext)) return
+      host_qext_on = on
+      if (.not. on .and. c_associated(handle)) then
+         allocate(z(host_np))
+         z = 0.0
+         rc = sfx_set_state(handle, 'qext'//c_null_char, c_loc(z), int(4 * host_np, c_size_t))
+         deallocate(z)
+      endif
+      host_qext_switch = 0
+   end function
+
    ! Host copies of the mesh fields the BMI hands out (get_value_ptr, get_sfincs_cell_area).
    subroutine host_mesh_fields()
       integer :: n
@@ -719,6 +757,7 @@ contains
       integer :: ierr
       type(sfx_step_stats) :: st
       integer(c_int) :: rc
+      if (host_qext_on) rc = sfx_set_state(handle, 'qext'//c_null_char, c_loc(host_qext), int(4 * host_np, c_size_t))
       rc = sfx_update(handle, real(dtrange, c_double), st)
       t = st%t
       dt = st%dt
@@ -835,6 +874,7 @@ module sfincs_bmi
 contains
    integer(c_int) function initialize(c_config_file) result(ierr) bind(C, name='initialize')
       type(c_ptr), value :: c_config_file
+      host_qext_requested = .true.   ! the BMI always carries the qext field (sfincs_bmi.f90:60)
       ierr = sfincs_initialize()
    end function
    integer(c_int) function update() result(ierr) bind(C, name='update')
@@ -901,6 +941,13 @@ contains
          x = c_loc(host_zb)
       case ('subgrid_z_zmin')
          x = c_loc(host_zmin)
+      case ('qext')
+         if (allocated(host_qext)) then
+            x = c_loc(host_qext)
+         else
+            x = c_null_ptr
+            ierr = -1
+         endif
       case default
          x = c_null_ptr
          ierr = -1
@@ -930,6 +977,8 @@ contains
          var_shape(1) = size(host_zb)
       case ('subgrid_z_zmin')
          var_shape(1) = size(host_zmin)
+      case ('qext')
+         if (allocated(host_qext)) var_shape(1) = size(host_qext)
       case default
          ierr = -1
       end select
@@ -944,7 +993,7 @@ contains
       select case (trim(bmi_name(c_var_name)))
       case ('zs')
          tn = 'double'
-      case ('z_xz', 'z_yz', 'zb', 'subgrid_z_zmin', 'zsmax')
+      case ('z_xz', 'z_yz', 'zb', 'subgrid_z_zmin', 'zsmax', 'qext')
          tn = 'float'
       case default
          tn = ' '
@@ -962,7 +1011,7 @@ contains
       ierr = 0
       rank = 1
       select case (trim(bmi_name(c_var_name)))
-      case ('z_xz', 'z_yz', 'zs', 'zb', 'subgrid_z_zmin', 'zsmax')
+      case ('z_xz', 'z_yz', 'zs', 'zb', 'subgrid_z_zmin', 'zsmax', 'qext')
       case default
          ierr = -1
       end select
@@ -1005,13 +1054,14 @@ contains
       ierr = 0
    end function
 
-   ! Hooks of processes outside this build: the qext switch (external sources into
-   ! continuity), zbuv updates after a bed change and wave-enhanced roughness. They return
-   ! -1 so that a coupler notices (sfincs_bmi.f90:226-297).
+   ! set_logical switches qext (sfincs_bmi.f90:226-245). The hooks of processes outside this
+   ! build, zbuv updates after a bed change and wave-enhanced roughness, return -1 so that
+   ! a coupler notices (:277-297).
    integer(c_int) function set_logical(c_flag_name, ival) result(ierr) bind(C, name='set_logical')
       character(kind=c_char), intent(in) :: c_flag_name(*)
       integer(c_int) :: ival(*)
       ierr = -1
+      if (trim(bmi_name(c_flag_name)) == 'qext') ierr = host_qext_switch(ival(1) /= 0)
    end function
 
    integer(c_int) function update_zbuv() result(ierr) bind(C, name='update_zbuv')

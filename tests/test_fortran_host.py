@@ -229,3 +229,38 @@ def test_bmi_mesh_queries(kind, tmp_path):
     assert np.array_equal(got, yz)
     assert host.get_value_ptr(b"qext", ctypes.byref(ptr)) == -1
     assert host.update_zbuv() == -1
+
+
+@needs_host
+@pytest.mark.gpu
+def test_bmi_qext_coupling(tmp_path):
+    """A coupler's external sources through the BMI (sfincs_bmi.f90:60, 95-129): initialize
+    carries a qext field, get_value_ptr("qext") hands it out, update_until steps with what
+    was written there; equal to the Python-driven run given the same field."""
+    print("case bmi qext coupling", flush=True)
+    from sfincs_amd import _lib
+    sc = W.regular_case(28, 36, dx=50.0, seed=48, t_end=1800.0, tide_amp=1.5)
+    d = deck.write(sc, str(tmp_path))
+    host = ctypes.CDLL(HOSTLIB)
+    host.update_until.argtypes = [ctypes.c_double]
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        assert host.initialize(None) == 0
+        np_ = sc.build_model().sizes()["np"]
+        ptr = ctypes.c_void_p()
+        assert host.get_value_ptr(b"qext", ctypes.byref(ptr)) == 0
+        qext = np.ctypeslib.as_array((ctypes.c_float * np_).from_address(ptr.value))
+        field = np.random.default_rng(49).uniform(-2e-5, 5e-5, np_).astype(np.float32)
+        qext[:] = field
+        assert host.update_until(ctypes.c_double(900.0)) == 0
+        assert host.get_value_ptr(b"zs", ctypes.byref(ptr)) == 0
+        zs_f = np.ctypeslib.as_array((ctypes.c_double * np_).from_address(ptr.value)).copy()
+        assert host.finalize() == 0
+    finally:
+        os.chdir(cwd)
+    sc.params["use_qext"] = 1
+    inst = _lib.Instance(sc.build_model())
+    inst.set("qext", field)
+    inst.update(900.0)
+    assert np.array_equal(zs_f, inst.get("zs"))

@@ -559,3 +559,32 @@ def test_timestep_analysis_matches_oracle(name, make):
     assert np.array_equal(wa, wb) and np.array_equal(la, lb)
     assert np.all(np.abs(sa - sb) <= 1e-6 * (1.0 + np.abs(sb)))
     assert lb.sum() >= st["nt"]
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid", "quadtree"])
+def test_qext_matches_oracle(kind):
+    """External sources per cell (use_qext, written with set_state "qext"), changed halfway
+    as a coupled model would; whole run against the oracle."""
+    print(f"case qext {kind}", flush=True)
+    if kind == "regular":
+        sc = W.regular_case(40, 52, dx=50.0, seed=101, t_end=1800.0, tide_amp=1.5, use_qext=1)
+    elif kind == "subgrid":
+        sc = W.subgrid_case(32, 40, dx=25.0, seed=102, t_end=1800.0, nsrc=1, use_qext=1)
+    else:
+        sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=103, t_end=1800.0, surge_amp=4.0, use_qext=1)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    rng = np.random.default_rng(104)
+    n = m.sizes()["np"]
+    for t_stop in (900.0, sc.t_end):
+        field = rng.uniform(-2e-5, 5e-5, n).astype(np.float32)
+        inst.set("qext", field)
+        orc.state("qext")[:] = field
+        inst.update(t_stop - inst.time()[0])
+        orc.update(t_stop)
+        assert inst.time()[2] == int(orc.get("nt"))
+    d = compare_state(inst, orc)
+    if m.sizes()["subgrid"]:
+        d.update(compare_subgrid_state(inst, orc))
+    print(kind, inst.time()[2], d, flush=True)
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL

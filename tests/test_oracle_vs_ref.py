@@ -162,3 +162,28 @@ def test_timestep_analysis_per_cell_bit_exact(kind):
     rdt, rpct = r.tsa_finalize(nt)
     assert np.array_equal(dtc, rdt) and np.array_equal(pct, rpct)
     assert np.any(dtc > 0) and np.any(dtc == -1.0) and np.any(pct > 0)
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+def test_qext_bit_exact(kind):
+    """use_qext (sfincs_continuity.f90:122-128, 473-497): an external source field in m/s,
+    here a seeded mix of inflow and outflow, after every step."""
+    if kind == "regular":
+        sc = W.regular_case(30, 40, dx=50.0, seed=33, t_end=3600.0, tide_amp=1.5, rain_mmh=20.0, use_qext=1)
+    else:
+        sc = W.subgrid_case(28, 36, dx=25.0, seed=34, t_end=3600.0, nsrc=1, rain_mmh=30.0, use_qext=1)
+    m = sc.build_model()
+    field = np.random.default_rng(35).uniform(-2e-5, 5e-5, m.sizes()["np"]).astype(np.float32)
+    a = O.OracleRun(m).configure(sc.params, m)
+    b = O.OracleRun(m).configure(sc.params, m)
+    a.state("qext")[:] = field
+    b.state("qext")[:] = field
+    r = O.RefRun(b)
+    base = O.OracleRun(m).configure(dict(sc.params, use_qext=0), m)
+    for i in range(120):
+        a.update(1e30, 1)
+        r.step()
+        base.update(1e30, 1)
+        for n in ("zs", "q") + (("z_volume",) if kind == "subgrid" else ()):
+            assert np.array_equal(a.state(n), b.state(n)), f"step {i} field {n}"
+    assert not np.array_equal(a.state("zs"), base.state("zs"))   # the sources did act

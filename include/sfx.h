@@ -145,6 +145,10 @@ int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const fl
 int  sfx_model_set_sources(sfx_model* m, int32_t nsrc, const float* x, const float* y,
                            int32_t nt, const float* t, const float* q);
 /* prcfile: uniform rain series in mm/h. */
+/* Storage volume per cell, m3 (volfile; initialize_storage_volume, sfincs_domain.f90:2063-2112):
+ * on subgrid meshes water entering a cell fills its storage before the cell's volume grows
+ * (sfincs_continuity.f90:499-523). State name storage_volume (f32 per cell). */
+int  sfx_model_set_storage_volume(sfx_model* m, const float* vol);
 /* Incoming infragravity water levels at the boundary points (bzifile;
  * sfincs_boundaries.f90:126-140, 748-753, 829-866): zsi[it*nbnd + ib] at the times of the
  * series given to sfx_model_set_boundary (call after it). They are added to the total

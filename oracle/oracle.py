@@ -127,6 +127,8 @@ class OracleRun:
         self._bind("tsa_wet", np.zeros(max(1, s["npuv"]), np.int32))
         self._bind("tsa_lim", np.zeros(max(1, s["npuv"]), np.int32))
         self._bind("qext", np.zeros(max(1, s["np"]), np.float32))
+        sv = model.table("storage_volume")
+        self._bind("storage_volume", sv.copy() if sv.size else np.zeros(max(1, s["np"]), np.float32))
         self._bind("qtsrc", np.zeros(max(1, s["nsrc"]), np.float32))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
@@ -162,6 +164,7 @@ class OracleRun:
                     wiggle_suppression=p.wiggle_suppression, store_zsmax=p.store_zsmax,
                     store_velmax=p.store_velmax, store_fluxmax=p.store_fluxmax, store_twet=p.store_twet,
                     twet_threshold=p.twet_threshold, timestep_analysis=p.timestep_analysis, use_qext=p.use_qext,
+                    use_storage_volume=int(bool(s["subgrid"]) and len(tb("storage_volume")) > 0),
                     wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
@@ -247,7 +250,7 @@ REF_SCALARS = ("subgrid", "use_quadtree", "advection", "advection_scheme", "visc
                "thetasmoothing", "wiggle_suppression", "store_zsmax", "store_velmax", "store_fluxmax", "store_twet",
                "twet_threshold", "wind", "precip", "h73table", "nsrc", "g",
                "huthresh", "huvmin", "advlim", "slopelim", "uvlim", "nuviscfac", "fcorio", "theta",
-               "wiggle_factor", "wiggle_threshold", "dtmax", "alpha", "rhow", "dx", "dy", "timestep_analysis", "use_qext")
+               "wiggle_factor", "wiggle_threshold", "dtmax", "alpha", "rhow", "dx", "dy", "timestep_analysis", "use_qext", "use_storage_volume")
 
 
 class RefRun:
@@ -290,7 +293,7 @@ class RefRun:
         if self.o.vals["subgrid"]:
             R.ref_set_f64(b"z_volume", k["z_volume"].ctypes.data, k["z_volume"].size)
         for n in ("q", "uv", "zs0", "zsderv", "zsmax", "tauwu", "tauwv", "netprcp", "qtsrc", "vmax", "qmax",
-                  "twet", "qext"):
+                  "twet", "qext", "storage_volume"):
             R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
         R.ref_set_i8(b"kfuv", k["kfuv"].ctypes.data, k["kfuv"].size)
 
@@ -301,6 +304,8 @@ class RefRun:
             names.append("z_volume")
         if self.o.vals["timestep_analysis"]:
             names += ["tsa_sum", "tsa_wet", "tsa_lim"]
+        if self.o.vals["use_storage_volume"]:
+            names.append("storage_volume")
         for n in names:
             R.ref_get(n.encode(), k[n].ctypes.data, k[n].nbytes)
 

@@ -62,6 +62,7 @@ contains
       case ('store_twet');         store_twet = (v /= 0.0d0)
       case ('twet_threshold');     twet_threshold = real(v, 4)
       case ('use_qext');           use_qext = (v /= 0.0d0)
+      case ('use_storage_volume'); use_storage_volume = (v /= 0.0d0)
       case ('timestep_analysis')   ! after dtmax and the sizes (initialize_timestep_analysis)
          timestep_analysis = (v /= 0.0d0)
          if (timestep_analysis) then
@@ -190,6 +191,7 @@ contains
       case ('qmax');         call putr(qmax)
       case ('twet');         call putr(twet)
       case ('qext');         call putr(qext)
+      case ('storage_volume'); call putr(storage_volume)
       case ('tauwu');        call putr(tauwu)
       case ('tauwv');        call putr(tauwv)
       case ('netprcp');      call putr(netprcp)
@@ -265,6 +267,7 @@ contains
       case ('vmax');     call c_f_pointer(p, f, [nbytes / 4]); f = vmax
       case ('qmax');     call c_f_pointer(p, f, [nbytes / 4]); f = qmax
       case ('twet');     call c_f_pointer(p, f, [nbytes / 4]); f = twet
+      case ('storage_volume'); call c_f_pointer(p, f, [nbytes / 4]); f = storage_volume
       case ('tsa_sum');  call c_f_pointer(p, f, [nbytes / 4]); f = timestep_analysis_average_required_timestep
       case ('tsa_wet');  call c_f_pointer(p, iv, [nbytes / 4]); iv = timestep_analysis_times_wet
       case ('tsa_lim');  call c_f_pointer(p, iv, [nbytes / 4]); iv = timestep_analysis_times_limiting

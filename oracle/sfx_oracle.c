@@ -29,7 +29,7 @@ typedef struct OM {
   int32_t thetasmoothing, wiggle_suppression, store_zsmax, wind, precip, h73table;
   int32_t store_velmax, store_fluxmax, store_twet;
   float twet_threshold;
-  int32_t timestep_analysis, use_qext;
+  int32_t timestep_analysis, use_qext, use_storage_volume;
   int32_t ngbnd, nkcuv2, nbnd, ntbnd, nsrc, ntsrc, ntprcp, ntwnd;
   float g, pi, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
@@ -53,6 +53,7 @@ typedef struct OM {
   float *vmax, *qmax, *twet;   /* compute_store_variables (np each, when switched on) */
   float *tsa_req, *tsa_sum;    /* timestep analysis (npuv each, when switched on) */
   float *qext;                 /* external sources, m/s per cell (use_qext) */
+  float *storage_volume;       /* storage left per subgrid cell, m3 (use_storage_volume) */
   int32_t *tsa_wet, *tsa_lim;
   int8_t *kfuv;
   float *tauwu, *tauwv, *netprcp;   /* np each */
@@ -98,7 +99,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(z_zmin) BIND(z_zmax) BIND(z_volmax) BIND(z_dep) BIND(uv_zmin) BIND(uv_zmax) BIND(uv_navg_w) BIND(uv_fnfit)
   BIND(uv_havg) BIND(uv_nrep) BIND(uv_pwet) BIND(uv_havg_zmax) BIND(uv_nrep_zmax)
   BIND(zs) BIND(z_volume) BIND(q) BIND(uv) BIND(q0) BIND(uv0) BIND(zs0) BIND(zsderv) BIND(zsmax) BIND(kfuv)
-  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim) BIND(qext)
+  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim) BIND(qext) BIND(storage_volume)
   BIND(tauwu) BIND(tauwv) BIND(netprcp)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
   BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsi_bnd) BIND(zsit_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
@@ -114,7 +115,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
   SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error)
-  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext)
+  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
   SETF(btfilter) SETF(btrelax) SETF(t0) SETF(t1) SETF(tspinup) SETF(zini) SETF(dx) SETF(dy) SETF(dt) SETF(min_dt)
@@ -418,6 +419,11 @@ static void water_levels(OM* m, float dt) {
         if (m->precip) dzsdt = dzsdt + m->netprcp[nm];
         if (m->use_qext) dzsdt = dzsdt + m->qext[nm];
         dvol = dvol + dzsdt * a * dt;
+      }
+      if (m->use_storage_volume && m->storage_volume[nm] > 1.0e-6f && dvol > 0.0f) {   /* :499-523 */
+        const float dv = m->storage_volume[nm] - dvol;
+        m->storage_volume[nm] = fmx(dv, 0.0f);
+        dvol = dv < 0.0f ? -dv : 0.0f;
       }
       m->z_volume[nm] = m->z_volume[nm] + (double)dvol;
       float zs00 = 0.f, zs11 = 0.f;

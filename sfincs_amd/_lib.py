@@ -76,6 +76,7 @@ def _sig(lib):
         "sfx_model_set_rain": (I, [P, i32, P, P]),
         "sfx_model_set_river_boundary": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_set_boundary_ig": (I, [P, P]),
+        "sfx_model_set_storage_volume": (I, [P, P]),
         "sfx_model_timestep_analysis_per_cell": (I, [P, P, P, P, C.c_int64, P, P]),
         "sfx_model_set_wind": (I, [P, i32, P, P, P]),
         "sfx_model_initialize": (I, [P]),
@@ -269,6 +270,12 @@ class Model:
               "timestep_analysis_per_cell")
         return dtc, pct
 
+    def set_storage_volume(self, vol):
+        """Storage volume per cell in m3 (volfile); subgrid meshes."""
+        a = np.ascontiguousarray(vol, np.float32)
+        self._keep_vol = a
+        check(lib().sfx_model_set_storage_volume(self.h, ptr(a)), "set_storage_volume")
+
     def set_boundary_ig(self, zsi):
         """Incoming infragravity levels (bzi file): [nt, nbnd] at the water-level series times."""
         a = np.ascontiguousarray(zsi, np.float32)
@@ -314,7 +321,8 @@ class Model:
 STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv": np.float32,
                 "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
                 "uvmean": np.float32, "vmax": np.float32, "qmax": np.float32, "twet": np.float32,
-                "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32, "qext": np.float32}
+                "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32, "qext": np.float32,
+                "storage_volume": np.float32}
 
 
 class Instance:
@@ -329,7 +337,7 @@ class Instance:
 
     def _n(self, name):
         s = self.sizes
-        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet", "qext"):
+        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet", "qext", "storage_volume"):
             return s["np"]
         if name in ("q", "uv"):
             return s["npuv"] + s["ncuv"] + 1

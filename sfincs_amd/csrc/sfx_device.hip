@@ -123,6 +123,7 @@ struct DevInst {
   float* tsa_req; float* tsa_sum; int32_t* tsa_wet; int32_t* tsa_lim;
   float* tsa_min;                // the last sweep's minimum time step
   float* qext;                   // external source per cell, m/s (use_qext)
+  void* storage;                 // storage volume left per subgrid cell, m3 (use_storage_volume)
   uint8_t* kcs;                  // kcs (KCS_MASK) | KCS_SOURCE (point source) | KCS_STEADY (continuity)
   const float* zb;
   const int8_t* z_iref;
@@ -1321,10 +1322,13 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
     R zs0 = zs0p[c], zsd = zsdp[c];
     const R zs0_in = zs0, zsd_in = zsd;
     const int nl = I.c.nl;
-    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, qx, iref, G(I.z_zmin)[c],
+    R stor = (I.c.flags & sfx::F_STORAGE) ? G((R*)I.storage)[c] : R(0);
+    const R stor_in = stor;
+    sfx::cell_subgrid<R>(zs, vol, zs0, zsd, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, qx, stor, iref, G(I.z_zmin)[c],
                          G(I.z_zmax)[c], G(I.z_volmax)[c], sfx::GRow<const float>{G(I.z_dep), (uint32_t)c * nl},
                          I.c, dt);
     if (!same_bits(vol, vol_in)) { G(I.vol)[c] = vol; wrote |= 2u; }
+    if (!same_bits(stor, stor_in)) G((R*)I.storage)[c] = stor;
     if (kcs == 1 || kcs == 4) {
       if (!same_bits(zs0, zs0_in)) { zs0p[c] = zs0; wrote |= 4u; }
       if (!same_bits(zsd, zsd_in)) { zsdp[c] = zsd; wrote |= 8u; }
@@ -1601,6 +1605,7 @@ uint32_t build_flags(const sfx_model* M) {
   if (p.store_twet) f |= sfx::F_TWET;
   if (p.timestep_analysis) f |= sfx::F_TSA;
   if (p.use_qext) f |= sfx::F_QEXT;
+  if (M->subgrid && !M->storage_volume.empty()) f |= sfx::F_STORAGE;
   if (M->wind) f |= sfx::F_WIND;
   if (M->precip) f |= sfx::F_PRECIP;
   if (p.h73table) f |= sfx::F_H73;
@@ -1657,6 +1662,7 @@ int upload_state(sfx_handle* H, const sfx_model* M) {
     I.tsa_wet = H->upload(izero.data(), ns); I.tsa_lim = H->upload(izero.data(), ns);
     I.tsa_min = H->upload(zero.data(), 1);
   }
+  if (f & sfx::F_STORAGE) { auto v = to_r<R>(M->storage_volume); I.storage = H->upload(v.data(), np); }
   if (f & sfx::F_QEXT) {   // sfincs_domain.f90:2383-2389
     std::vector<float> z(np, 0.f);
     I.qext = H->upload(z.data(), np);
@@ -2043,6 +2049,10 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
     const void* src = k == "tsa_sum" ? (const void*)I.tsa_sum : (k == "tsa_wet" ? (const void*)I.tsa_wet : (const void*)I.tsa_lim);
     return get_uv_words(H, src, dst, nbytes);
   }
+  if (k == "storage_volume") {
+    if (!I.storage) { set_error("sfx_get_state: no storage volume"); return SFX_ERR_ARG; }
+    return H->fp64 ? get_cell_r<double>(H, I.storage, dst, nbytes, np) : get_cell_r<float>(H, I.storage, dst, nbytes, np);
+  }
   if (k == "vmax" || k == "qmax" || k == "twet") {
     void* src = k == "vmax" ? I.vmax : (k == "qmax" ? I.qmax : I.twet);
     if (!src) { set_error("sfx_get_state: " + k + " is not stored (its store switch is off)"); return SFX_ERR_ARG; }
@@ -2109,6 +2119,12 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
   if (k == "zs0" || k == "zsderv" || k == "zsmax") {
     void* dst = k == "zs0" ? I.zs0 : (k == "zsderv" ? I.zsderv : I.zsmax);
     rc = H->fp64 ? set_cell_r<double>(H, dst, src, nbytes, np) : set_cell_r<float>(H, dst, src, nbytes, np);
+    if (rc) return rc;
+    return clear_steady(H);
+  }
+  if (k == "storage_volume") {
+    if (!I.storage) { set_error("sfx_set_state: no storage volume"); return SFX_ERR_ARG; }
+    rc = H->fp64 ? set_cell_r<double>(H, I.storage, src, nbytes, np) : set_cell_r<float>(H, I.storage, src, nbytes, np);
     if (rc) return rc;
     return clear_steady(H);
   }
@@ -2728,6 +2744,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (H->hI.vmax) fields.push_back("vmax");
   if (H->hI.qmax) fields.push_back("qmax");
   if (H->hI.twet) fields.push_back("twet");
+  if (H->hI.storage) fields.push_back("storage_volume");
   std::map<std::string, std::vector<char>> snap;
   for (auto& f : fields) {
     size_t n = 0;

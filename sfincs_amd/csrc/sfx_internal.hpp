@@ -70,6 +70,7 @@ struct sfx_model {
   std::vector<double> zs, z_volume;
   std::vector<float> q, uv, zs0, zsderv, zsmax;
   std::vector<float> vmax, qmax, twet;   // compute_store_variables fields (used when switched on)
+  std::vector<float> storage_volume;     // storage per subgrid cell (empty: none; volfile)
   std::vector<int8_t> kfuv;
 
   // boundaries

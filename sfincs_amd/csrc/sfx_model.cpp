@@ -608,6 +608,14 @@ extern "C" int sfx_model_set_boundary(sfx_model* M, int32_t nbnd, const float* x
   return SFX_OK;
 }
 
+// Storage volume per cell (volfile; initialize_storage_volume, sfincs_domain.f90:2063-2112):
+// water entering a subgrid cell fills it before the cell's volume grows.
+extern "C" int sfx_model_set_storage_volume(sfx_model* M, const float* vol) {
+  if (!M || M->initialized || !M->mesh_built || !vol) { set_error("sfx_model_set_storage_volume: bad args"); return SFX_ERR_ARG; }
+  M->storage_volume.assign(vol, vol + M->np);
+  return SFX_OK;
+}
+
 // Incoming infragravity levels at the boundary points (bzifile, sfincs_boundaries.f90:126-140):
 // one value per boundary point at each time of the water-level series, same layout.
 extern "C" int sfx_model_set_boundary_ig(sfx_model* M, const float* zsi) {
@@ -953,7 +961,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv) T(vmax) T(qmax) T(twet)
   T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
   T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval) T(x73)
-  T(index_bdr_gbp) T(index_zsi_bdr) T(dzs_bdr) T(zsi_bnd)
+  T(index_bdr_gbp) T(index_zsi_bdr) T(dzs_bdr) T(zsi_bnd) T(storage_volume)
   T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)
 #undef T
   return false;

@@ -19,7 +19,8 @@ enum : uint32_t {
   F_VELMAX = 1u << 14, F_FLUXMAX = 1u << 15, F_TWET = 1u << 16,
   F_STORE = F_VELMAX | F_FLUXMAX | F_TWET,
   F_TSA = 1u << 17,  // timestep_analysis (sfincs_timestep_analysis.f90)
-  F_QEXT = 1u << 18  // external sources per cell (use_qext, BMI coupling)
+  F_QEXT = 1u << 18, // external sources per cell (use_qext, BMI coupling)
+  F_STORAGE = 1u << 19  // storage volume per subgrid cell (use_storage_volume)
 };
 
 constexpr int MAXREF = 8;
@@ -310,7 +311,7 @@ __device__ __forceinline__ void cell_regular(double& zs, R& zsmax, int kcs, R q_
 // Cell update of the subgrid continuity (sfincs_continuity.f90:337-618).
 template <typename R, typename DepP>
 __device__ __forceinline__ void cell_subgrid(double& zs, double& vol, R& zs0, R& zsderv, R& zsmax, int kcs,
-                                             R q_md, R q_mu, R q_nd, R q_nu, R netprcp, R qext, int iref,
+                                             R q_md, R q_mu, R q_nd, R q_nu, R netprcp, R qext, R& stor, int iref,
                                              float zmin, float zmax, float volmax, DepP dep,
                                              const Consts& c, R dt) {
   R dvol = R(0);
@@ -327,6 +328,11 @@ __device__ __forceinline__ void cell_subgrid(double& zs, double& vol, R& zs0, R&
       if (c.flags & F_PRECIP) dzsdt = dzsdt + netprcp;
       if (c.flags & F_QEXT) dzsdt = dzsdt + qext;
       dvol = dvol + dzsdt * a * dt;
+    }
+    if ((c.flags & F_STORAGE) && stor > R(1.0e-6f) && dvol > R(0)) {   // :499-523: storage fills first
+      const R dv = stor - dvol;
+      stor = vmax<R>(dv, R(0));
+      dvol = dv < R(0) ? -dv : R(0);
     }
     vol = vol + (double)dvol;
     R zs00 = R(0), zs11 = R(0);

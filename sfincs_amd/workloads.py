@@ -41,6 +41,7 @@ class Scenario:
     rstfile: str = None                   # binary restart file as initial condition
     bdr: tuple = None                     # river boundary points (x, y, x_in, y_in, slope, distance)
     bzi: np.ndarray = None                # infragravity levels [nt, nbnd] at the bnd series times
+    storage: object = None                # callable(model) -> storage volume per cell (m3), or None
 
     def build_model(self):
         p = _lib.default_params(**self.params)
@@ -66,6 +67,8 @@ class Scenario:
             m.set_wind(*self.wind)
         if self.bdr is not None:
             m.set_river_boundary(*self.bdr)
+        if self.storage is not None:
+            m.set_storage_volume(self.storage(m))
         if self.rstfile is not None:
             m.set_restart(self.rstfile)
         m.initialize()

@@ -588,3 +588,26 @@ def test_qext_matches_oracle(kind):
         d.update(compare_subgrid_state(inst, orc))
     print(kind, inst.time()[2], d, flush=True)
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL
+
+
+def test_storage_volume_matches_oracle():
+    """Storage volume per subgrid cell (use_storage_volume) with rain filling it; whole run
+    against the oracle, the storage left included."""
+    print("case storage volume", flush=True)
+    def storage(m):
+        r = np.random.default_rng(105)
+        v = r.uniform(0.0, 40.0, m.sizes()["np"]).astype(np.float32)
+        v[r.random(v.size) < 0.3] = 0.0
+        return v
+    sc = W.subgrid_case(32, 40, dx=25.0, seed=106, t_end=1800.0, nsrc=1, rain_mmh=60.0)
+    sc.storage = storage
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc, ("zs", "q", "uv", "zsmax", "storage_volume"))
+    d.update(compare_subgrid_state(inst, orc))
+    print(st["nt"], d, flush=True)
+    assert st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL and d["z_volume_rel"] <= 1e-7
+    assert d["storage_volume"] <= 1e-3   # m3

@@ -187,3 +187,28 @@ def test_qext_bit_exact(kind):
         for n in ("zs", "q") + (("z_volume",) if kind == "subgrid" else ()):
             assert np.array_equal(a.state(n), b.state(n)), f"step {i} field {n}"
     assert not np.array_equal(a.state("zs"), base.state("zs"))   # the sources did act
+
+
+def test_storage_volume_bit_exact():
+    """use_storage_volume (sfincs_continuity.f90:499-523): storage per subgrid cell fills
+    before the cell's volume grows; zs, z_volume and the storage left after every step."""
+    def storage(m):
+        r = np.random.default_rng(36)
+        v = r.uniform(0.0, 40.0, m.sizes()["np"]).astype(np.float32)
+        v[r.random(v.size) < 0.3] = 0.0
+        return v
+    sc = W.subgrid_case(28, 36, dx=25.0, seed=37, t_end=3600.0, nsrc=1, rain_mmh=60.0)
+    sc.storage = storage
+    m = sc.build_model()
+    a = O.OracleRun(m).configure(sc.params, m)
+    b = O.OracleRun(m).configure(sc.params, m)
+    assert a.vals["use_storage_volume"] == 1
+    r = O.RefRun(b)
+    s0 = a.state("storage_volume").copy()
+    for i in range(150):
+        a.update(1e30, 1)
+        r.step()
+        for n in ("zs", "z_volume", "storage_volume"):
+            assert np.array_equal(a.state(n), b.state(n)), f"step {i} field {n}"
+    s1 = a.state("storage_volume")
+    assert np.any(s1 < s0) and np.all(s1 <= s0) and np.all(s1 >= 0)

@@ -802,6 +802,10 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
         const float facint = (float)((zsc - (double)d0) / (double)std::max(d1 - d0, 0.001f));
         M->z_volume[c] = (double)((float)(ivol - 1) * dzvol + facint * dzvol);
       }
+      // a cell that starts wet has no storage left for the water it already holds
+      // (sfincs_subgrid.F90:952-956; f64 difference, stored as f32)
+      if (!M->storage_volume.empty())
+        M->storage_volume[c] = (float)std::max((double)M->storage_volume[c] - M->z_volume[c], 0.0);
     }
   } else {
     M->z_volume.clear();

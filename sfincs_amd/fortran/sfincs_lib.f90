@@ -757,7 +757,14 @@ contains
       integer :: ierr
       type(sfx_step_stats) :: st
       integer(c_int) :: rc
-      if (host_qext_on) rc = sfx_set_state(handle, 'qext'//c_null_char, c_loc(host_qext), int(4 * host_np, c_size_t))
+      if (host_qext_on) then
+         rc = sfx_set_state(handle, 'qext'//c_null_char, c_loc(host_qext), int(4 * host_np, c_size_t))
+         if (rc < 0) then
+            call fail('qext upload')
+            ierr = -1
+            return
+         endif
+      endif
       rc = sfx_update(handle, real(dtrange, c_double), st)
       t = st%t
       dt = st%dt

@@ -295,6 +295,33 @@ def with_ig_waves(sc, amp=0.2, period=120.0):
     return sc
 
 
+def with_sources(sc, x, y, q):
+    """Adds point discharges (src/dis files) at points (x, y) with constant rates q in m3/s
+    (negative rates drain the cell) to a scenario without sources."""
+    q = np.asarray(q, np.float32)
+    ts = np.array([0.0, sc.t_end + 3600.0], np.float32)
+    sc.src = (np.asarray(x, np.float32), np.asarray(y, np.float32), ts, np.stack([q, q]))
+    sc.name += "_src"
+    return sc
+
+
+def regular_sources(sc):
+    """Four sources on a configuration-0 style beach (the west half lies below 0 m): a
+    filling and a draining one on dry land (the draining one is held at the bed by the
+    clamp of the regular branch), one draining a cell the tide wets, and one draining deep
+    water."""
+    zb = sc.zb_grid                                   # (mmax, nmax)
+    inner = np.zeros_like(zb, bool)
+    inner[2:-2, 2:-2] = True
+
+    def pick(lo, hi, k):
+        cand = np.argwhere(inner & (zb > lo) & (zb < hi))
+        m, n = cand[min(k, len(cand) - 1)]
+        return (m + 0.5) * sc.dx, (n + 0.5) * sc.dy
+    pts = [pick(2.0, 99.0, 3), pick(2.0, 99.0, 40), pick(0.2, 1.0, 7), pick(-99.0, -5.0, 11)]
+    return with_sources(sc, [p[0] for p in pts], [p[1] for p in pts], [2.0, -1.0, -3.0, -5.0])
+
+
 def with_river(sc, first=1, ncells=8, slope=1.0e-3):
     """Turns cells `first`..`first+ncells-1` of the south row (n = 0) of a regular-kind
     scenario into downstream river boundary cells (msk=5), forced by two bdr points whose

@@ -212,3 +212,23 @@ def test_storage_volume_bit_exact():
             assert np.array_equal(a.state(n), b.state(n)), f"step {i} field {n}"
     s1 = a.state("storage_volume")
     assert np.any(s1 < s0) and np.all(s1 <= s0) and np.all(s1 >= 0)
+
+
+def test_regular_sources_bit_exact():
+    """Point sources on a regular mesh (sfincs_continuity.f90:87-106): zs += Q dt / A, held
+    at the bed by max(..., zb); two of the four sources drain dry or drying cells, so the
+    clamp acts. Bit-exact after every step, and the draining land cell sits on its bed."""
+    sc = W.regular_sources(W.regular_case(40, 52, dx=50.0, seed=28, t_end=3600.0, tide_amp=1.5, rain_mmh=0.0))
+    o = run_pair(sc, 200)
+    m = o.model
+    cells = m.table("nmindsrc") - 1
+    zs, zb = o.state("zs"), m.table("zb")
+    assert zs[cells[1]] == zb[cells[1]]          # drained dry land: at the bed
+    assert zs[cells[0]] > zb[cells[0]] + 0.01    # filled dry land: water above the bed
+
+
+def test_quadtree_sources_bit_exact():
+    """Point sources on a quadtree mesh (cell areas per refinement level)."""
+    sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=29, t_end=3600.0, surge_amp=3.0)
+    W.with_sources(sc, [350.0, 1020.0, 610.0], [420.0, 530.0, 880.0], [3.0, -2.0, 1.5])
+    run_pair(sc, 200)

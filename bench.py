@@ -76,10 +76,36 @@ def byte_model(sizes, counts_per_step, stores_per_step, point_kernel):
     return mom, cont, dict(copied=copied, dry=c1, dry_zeros_held=c4, quiescent_tile=c5, wet=c2, partial=c3)
 
 
+def host_info():
+    """lscpu's model, sockets, cores, SMT and NUMA figures, the CPUs this process may run
+    on, and the OpenMP placement used by the CPU baseline."""
+    import subprocess
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "NUMA node(s)": "numa_nodes", "CPU(s)": "cpus"}
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except Exception as e:     # reported, never hidden
+        info["lscpu_error"] = repr(e)
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES"):
+        info[k] = os.environ.get(k)
+    return info
+
+
 def cpu_baseline(sc, model, inst, nsteps):
     """The reference's own compute_fluxes + compute_water_levels (oracle/_ref, OpenMP)
     timed with the reference's per-phase timers on a bounded sample of the same run,
     started from the GPU state after warm-up. Falls back to the C restatement."""
+    # one OpenMP thread per core of this process's CPU share, kept on its core (set before
+    # the reference library starts its OpenMP runtime)
+    os.environ.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     orc = O.OracleRun(model).configure(sc.params, model)
@@ -90,7 +116,7 @@ def cpu_baseline(sc, model, inst, nsteps):
     orc.set("t", t)
     orc.set("nt", nt)
     orc.set("min_dt", dt / float(np.float32(sc.params.get("alpha", 0.5))))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     if O.ref_available():
         r = O.RefRun(orc)
         r.R.ref_reset_timers()
@@ -106,10 +132,47 @@ def cpu_baseline(sc, model, inst, nsteps):
         kind = "port"
         cores = 1
     np_ = model.sizes()["np"]
-    return dict(value=np_ * nsteps / secs, unit="cell-updates/s", cores=cores, kind=kind,
+    return dict(value=np_ * nsteps / secs, unit="cell-updates/s", cores=cores, kind=kind, host=host_info(),
                 sample=f"{nsteps} consecutive steps of the same 2048x2048 subgrid run, started from the GPU "
                        f"state after warm-up; timed by the reference's own tloopflux+tloopcont accumulators",
                 seconds=secs)
+
+
+def whole_period(model, tuning, t_end, twin=True):
+    """The whole simulated period from t = 0 on a fresh device instance (the tuning of the
+    timed window pinned), timed end to end: sfincs_update to t_end, graph-captured. With
+    `twin`, the FP64 twin runs the same period on its own adaptive steps and the FP32
+    result's water-level error against it is stated over the active cells."""
+    import torch
+    from sfincs_amd import _lib
+    a = _lib.Instance(model)
+    a.pin_tuning(tuning)
+    v0 = a.total_volume()
+    torch.cuda.synchronize()
+    t_a = time.perf_counter()
+    rc, st = a.update(float(t_end))
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t_a
+    zs = a.get("zs")
+    assert np.all(np.isfinite(zs)), "non-finite water levels"
+    np_ = model.sizes()["np"]
+    out = {"model_time_s": st["t"], "steps": st["nt"], "seconds": secs,
+           "cell_updates_per_s": np_ * st["nt"] / secs, "ms_per_step": secs / max(1, st["nt"]) * 1e3,
+           "rc": rc, "volume_start_m3": v0, "volume_end_m3": a.total_volume()}
+    a.close()
+    if twin:
+        b = _lib.Instance(model, fp64=True)
+        t_b = time.perf_counter()
+        rc64, st64 = b.update(float(t_end))
+        torch.cuda.synchronize()
+        d = np.abs(zs - b.get("zs"))
+        out["fp64_twin"] = {
+            "steps": st64["nt"], "seconds": time.perf_counter() - t_b, "max_abs_dzs_m": float(d.max()),
+            "mean_abs_dzs_m": float(d.mean()), "p99_abs_dzs_m": float(np.quantile(d, 0.99)),
+            "cells_over_1e-4_m": int(np.count_nonzero(d > 1e-4)),
+            "note": "FP32 arithmetic (zs, V f64) against the same kernels in FP64, each on its own adaptive dt"}
+        b.close()
+    return out
 
 
 # Indexed (quadtree / masked) regular meshes, per uv-point by branch: as the dense regular
@@ -310,6 +373,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1200,
                     help="steps of the reference's CPU path timed for cpu_baseline (about 10 s of work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--start-time", type=float, default=10800.0,
+                    help="model time (s) the run is advanced to, untimed, before tuning, warm-up and the timed "
+                         "window (default 3 h: the tide's crest, halfway through the 6 h period)")
+    ap.add_argument("--no-whole-period", action="store_true",
+                    help="skip the whole 6 h period (and its FP64 twin) run after the timed window")
+    ap.add_argument("--no-twin", action="store_true", help="skip the FP64 twin of the whole period")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--mode", choices=("step", "quadtree", "ensemble", "serve"), default="step",
                     help="step: configuration 1 (the default line); quadtree: configuration 2; ensemble: "
@@ -346,15 +415,22 @@ def main():
 
     t0 = time.perf_counter()
     sc = W.config1(args.size)
+    t_period = sc.t_end
     sc.params["t1"] = 1.0e7          # the timed window must not reach the end time
     model = sc.build_model()
     t_gen = time.perf_counter() - t0
     sizes = model.sizes()
     inst = _lib.Instance(model)
+    # the window starts mid-run (untimed advance): tuning, warm-up and timing see the
+    # state of the tide's crest, not the quiet first minutes
+    if args.start_time > 0:
+        inst.update(float(args.start_time))
+    t_start = inst.time()[0]
     tuning = inst.autotune(11)
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
-    print(f"# generated {sizes['np']} cells in {t_gen:.1f}s; tuning {tuning}", file=sys.stderr, flush=True)
+    print(f"# generated {sizes['np']} cells in {t_gen:.1f}s; window starts at t={t_start:.0f}s; tuning {tuning}",
+          file=sys.stderr, flush=True)
 
     batch.advance(args.warmup)
     torch.cuda.synchronize()
@@ -416,7 +492,7 @@ def main():
         "data": "synthetic (seeded fBm DEM, PCG64(1001))",
         "config": {"workload": f"config 1: regular {args.size}x{args.size} subgrid (10 levels), dx=25 m, "
                                f"8 discharges, 1.5 m tide", "np": sizes["np"], "npuv": sizes["npuv"],
-                   "tuning": tuning,
+                   "tuning": tuning, "window_start_model_time_s": t_start,
                    "model_time_s": t_model, "dt_s": dt_model},
         "roofline": {"bound": "hbm", "achieved": mom_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": mom_gbs / PEAK_HBM_GBS, "traffic": traffic,
@@ -434,6 +510,9 @@ def main():
             out["cpu_baseline"] = cpu_baseline(sc, model, inst, args.cpu_steps)
         except Exception as e:    # reported, never hidden
             out["cpu_baseline"] = {"error": repr(e)}
+    batch.close()
+    if not args.no_whole_period:
+        out["whole_period"] = whole_period(model, tuning, t_period, twin=not args.no_twin)
     print(json.dumps(out))
 
 

@@ -40,7 +40,8 @@ extern "C" {
  * derived flags :347-745, constants g=9.81, pi=3.14159 (:372-373). */
 typedef struct sfx_params {
   float t0, t1;              /* simulation start/stop in seconds (tstart/tstop - tref) */
-  float tspinup;             /* absolute; default t0 (sfincs_input.f90:369) */
+  float tspinup;             /* seconds after t0 (the sfincs.inp key, default 0); boundary levels
+                              * ramp from zsini until t0 + tspinup (sfincs_input.f90:60, 369) */
   float alpha;               /* CFL factor, default 0.5 */
   float theta;               /* 1.0 = no theta smoothing */
   float hmin_cfl;            /* 0.1 */
@@ -138,6 +139,11 @@ int  sfx_model_get_table(const sfx_model* m, const char* name, void* dst, size_t
 int  sfx_model_table_bytes(const sfx_model* m, const char* name, size_t* nbytes);
 int  sfx_model_set_bathymetry(sfx_model* m, const float* zb_active);           /* np values */
 int  sfx_model_set_subgrid(sfx_model* m, const sfx_subgrid_input* sg);
+/* The legacy binary subgrid file (sbgfile that is not NetCDF; read_subgrid_file_original,
+ * sfincs_subgrid.F90:458-903, dispatched at :28-53): hrep/navg per level, huthresh kept,
+ * pwet = 1, no fit. Regular and quadtree meshes (the quadtree file's layout over all its
+ * points). Call after the mesh is built and before sfx_model_initialize. */
+int  sfx_model_read_subgrid_binary(sfx_model* m, const char* path);
 /* bzs: water level boundaries (bnd/bzs files): zs is nt*nbnd, point fastest. */
 int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const float* y,
                             int32_t nt, const float* t, const float* zs);
@@ -212,7 +218,13 @@ int  sfx_destroy(sfx_handle* h);
  * tsa_lim (i32: those sweeps, and those in which it limited the step), the accumulators of
  * sfincs_timestep_analysis.f90:26-74; their average is tsa_sum / max(tsa_wet, 1). */
 int  sfx_get_state(sfx_handle* h, const char* name, void* dst, size_t nbytes);
+/* Also "zb" (f32 per cell, regular meshes without subgrid tables): the bed level, as a BMI
+ * coupler writes it through get_value_ptr; zbuvmx follows only with sfx_update_zbuvmx. */
 int  sfx_set_state(sfx_handle* h, const char* name, const void* src, size_t nbytes);
+/* update_zbuv (sfincs_bmi.f90:277-284 -> compute_zbuvmx, sfincs_domain.f90:1746-1762):
+ * zbuvmx = max(zb(nm), zb(nmu)) + huthresh for every uv-point from the device's zb.
+ * Regular meshes without subgrid tables. */
+int  sfx_update_zbuvmx(sfx_handle* h);
 /* write_rst_file (sfincs_output.f90:657-729): a type-1 restart file (zs as f32, q, uvmean)
  * of the current state at `path` (the reference names it sfincs.<time>.rst). */
 int  sfx_write_restart(sfx_handle* h, const char* path);
@@ -243,6 +255,17 @@ int  sfx_get_counters(sfx_handle* h, int64_t counts[6]);
 int  sfx_get_store_counters(sfx_handle* h, int64_t counts[6]);
 /* FP64 volume accumulator: total water volume (sum of zs*A or V) in double. */
 int  sfx_total_volume(sfx_handle* h, double* vol);
+/* FP64 volume balance of the interior cells (kcs 1). Every continuity sweep adds, in
+ * double, the volume it moved into the interior cells through faces whose other side is
+ * not interior (water-level, river and Neumann boundaries, inactive cells), from point
+ * sources as applied (sfincs_continuity.f90:87-106 with its bed clamp, :311-325), rain
+ * (:116-121, 473-497) and qext. out = {interior volume now (sum of V, or zs*A),
+ * storage volume left, boundary inflow, source inflow, rain, qext} in m3; the last four
+ * are sums since creation or sfx_reset_balance. Between two calls the balance is
+ * dV + d(storage used) = d(inflows). Host writes of cell state are not counted. No
+ * counterpart in the reference, which keeps no volume balance. */
+int  sfx_get_balance(sfx_handle* h, double out[6]);
+int  sfx_reset_balance(sfx_handle* h);
 
 /* ---- batched ensembles ---------------------------------------------------------------
  * No counterpart in the reference (it runs scenarios one at a time, docs/output.rst:19-20).
@@ -286,6 +309,10 @@ int  sfx_get_tuning(sfx_handle* h, int32_t out[8]);
  * momentum tile m, continuity tile n, continuity tile m}; tiles hold 256 cells. No effect
  * on indexed meshes. Call it before the handle joins a batch. */
 int  sfx_set_tuning(sfx_handle* h, const int32_t in[5]);
+/* Pins the work-list order without timing: the runs of tiles dealt to the XCDs in turn,
+ * for the momentum and the continuity sweep (out[6], out[7] of sfx_get_tuning; 0: one
+ * contiguous band per XCD). Call it before the handle joins a batch. */
+int  sfx_set_work_order(sfx_handle* h, int32_t mom_run, int32_t cont_run);
 
 int  sfx_last_error(char* buf, size_t n);
 

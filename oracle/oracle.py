@@ -169,7 +169,7 @@ class OracleRun:
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
                     nsrc=s["nsrc"], ntsrc=len(tb("tsrc")), ntprcp=len(tb("tprcpt")), ntwnd=len(tb("twnd")),
-                    g=p.g, pi=p.pi, huthresh=(0.0 if s["subgrid"] else p.huthresh), huvmin=p.huvmin,
+                    g=p.g, pi=p.pi, huthresh=model.params().huthresh, huvmin=p.huvmin,
                     advlim=p.advlim, slopelim=p.slopelim, uvlim=p.uvlim, nuviscfac=p.nuviscfac, fcorio=fcorio,
                     theta=p.theta, wiggle_factor=p.wiggle_factor, wiggle_threshold=p.wiggle_threshold,
                     dtmax=s["dtmax"], dtmin=s["dtmin"], alpha=p.alpha, rhoa=p.rhoa, rhow=p.rhow,

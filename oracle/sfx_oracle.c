@@ -163,6 +163,9 @@ float oracle_compute_fluxes(OM* m, float dt) {
   if (m->timestep_analysis)   /* :101-119 */
     for (int32_t i = 0; i < m->npuv; ++i) m->tsa_req[i] = m->dtmax;
   const int nbrs = m->advection || m->coriolis || m->viscosity || m->friction2d || m->thetasmoothing;
+  /* every uv-point reads the snapshot q0/uv0 and writes only its own entries, so the
+   * points are independent (the reference's own OpenMP loop, :142-752) */
+#pragma omp parallel for schedule(static) reduction(min : min_dt)
   for (int32_t i = 0; i < m->npuv; ++i) {
     const int kc = m->kcuv[i];
     if (!(kc == 1 || kc == 6)) continue;
@@ -329,6 +332,7 @@ float oracle_compute_fluxes(OM* m, float dt) {
   }
   m->min_dt = min_dt;
   if (m->timestep_analysis) {   /* timestep_analysis_update (sfincs_timestep_analysis.f90:26-74) */
+#pragma omp parallel for schedule(static)
     for (int32_t i = 0; i < m->npuv; ++i) {
       if (!(m->kcuv[i] == 1 || m->kcuv[i] == 6) || m->kfuv[i] != 1) continue;
       m->tsa_sum[i] = m->tsa_sum[i] + m->tsa_req[i];
@@ -341,6 +345,7 @@ float oracle_compute_fluxes(OM* m, float dt) {
 
 /* ---- store variables: sfincs_continuity.f90:626-720 ------------------------------------ */
 static void compute_store_variables(OM* m, float dt) {
+#pragma omp parallel for schedule(static)
   for (int32_t nm = 0; nm < m->np; ++nm) {
     if (!(m->kcs[nm] == 1 || m->kcs[nm] == 4)) continue;
     const int32_t nmd = m->z_index_uv_md[nm] - 1, nmu = m->z_index_uv_mu[nm] - 1;
@@ -380,6 +385,7 @@ static void water_levels(OM* m, float dt) {
       const float inc = m->qtsrc[s] * dt / m->cell_area[m->z_flags_iref[nm] - 1];
       m->zs[nm] = dmax(m->zs[nm] + (double)inc, (double)m->zb[nm]);
     }
+#pragma omp parallel for schedule(static)
     for (int32_t nm = 0; nm < m->np; ++nm) {
       if (m->kcs[nm] == 1) {
         if (m->precip) m->zs[nm] = m->zs[nm] + (double)(m->netprcp[nm] * dt);
@@ -400,6 +406,8 @@ static void water_levels(OM* m, float dt) {
       m->z_volume[nm] = m->z_volume[nm] + (double)(m->qtsrc[s] * dt);
   }
   const int nl = m->nl;
+  /* cells read the new q and write only their own state: independent */
+#pragma omp parallel for schedule(static)
   for (int32_t nm = 0; nm < m->np; ++nm) {
     float dvol = 0.0f;
     const int k = m->kcs[nm];

@@ -66,6 +66,7 @@ def _sig(lib):
         "sfx_params_default": (None, [C.POINTER(sfx_params)]),
         "sfx_model_create_regular": (I, [C.POINTER(sfx_params), i32, i32, f, f, f, f, f, P, C.POINTER(P)]),
         "sfx_model_create_quadtree": (I, [C.POINTER(sfx_params), C.POINTER(sfx_quadtree_input), C.POINTER(P)]),
+        "sfx_model_read_subgrid_binary": (I, [P, C.c_char_p]),
         "sfx_model_sizes": (I, [P, C.POINTER(sfx_sizes)]),
         "sfx_model_get_table": (I, [P, s, P, sz]),
         "sfx_model_table_bytes": (I, [P, s, C.POINTER(sz)]),
@@ -98,6 +99,9 @@ def _sig(lib):
         "sfx_get_counters": (I, [P, C.POINTER(C.c_int64)]),
         "sfx_get_store_counters": (I, [P, C.POINTER(C.c_int64)]),
         "sfx_total_volume": (I, [P, C.POINTER(d)]),
+        "sfx_get_balance": (I, [P, C.POINTER(d)]),
+        "sfx_reset_balance": (I, [P]),
+        "sfx_update_zbuvmx": (I, [P]),
         "sfx_batch_create": (I, [i32, C.POINTER(P)]),
         "sfx_batch_destroy": (I, [P]),
         "sfx_batch_add": (I, [P, P, d]),
@@ -107,6 +111,7 @@ def _sig(lib):
         "sfx_autotune": (I, [P, i32]),
         "sfx_get_tuning": (I, [P, C.POINTER(i32)]),
         "sfx_set_tuning": (I, [P, C.POINTER(i32)]),
+        "sfx_set_work_order": (I, [P, i32, i32]),
         "sfx_last_error": (I, [C.c_char_p, sz]),
         "sfx_profile_steps": (I, [P, i32, C.POINTER(d)]),
         "sfx_create_ex": (I, [P, i32, i32, C.POINTER(P)]),
@@ -227,6 +232,17 @@ class Model:
         check(lib().sfx_model_get_table(self.h, name.encode(), ptr(a), n.value), name)
         return a
 
+    def params(self):
+        """The model's sfx_params after set-up (huthresh is 0 once NetCDF-style subgrid
+        tables are set, as the reference does; the legacy binary file keeps it)."""
+        p = sfx_params()
+        check(lib().sfx_model_get_table(self.h, b"params", C.byref(p), C.sizeof(p)), "params")
+        return p
+
+    def read_subgrid_binary(self, path):
+        """Legacy binary subgrid file (read_subgrid_file_original)."""
+        check(lib().sfx_model_read_subgrid_binary(self.h, str(path).encode()), "read_subgrid_binary")
+
     def set_bathymetry(self, zb):
         zb = np.ascontiguousarray(zb, dtype=np.float32)
         check(lib().sfx_model_set_bathymetry(self.h, ptr(zb)), "set_bathymetry")
@@ -322,7 +338,7 @@ STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv":
                 "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
                 "uvmean": np.float32, "vmax": np.float32, "qmax": np.float32, "twet": np.float32,
                 "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32, "qext": np.float32,
-                "storage_volume": np.float32}
+                "storage_volume": np.float32, "zb": np.float32}
 
 
 class Instance:
@@ -337,7 +353,8 @@ class Instance:
 
     def _n(self, name):
         s = self.sizes
-        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet", "qext", "storage_volume"):
+        if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet", "qext", "storage_volume",
+                    "zb"):
             return s["np"]
         if name in ("q", "uv"):
             return s["npuv"] + s["ncuv"] + 1
@@ -401,6 +418,20 @@ class Instance:
         check(lib().sfx_total_volume(self.h, C.byref(v)), "total_volume")
         return v.value
 
+    def balance(self):
+        """FP64 volume balance of the interior cells (sfx_get_balance): volume and storage
+        left now, and the boundary, source, rain and qext inflows since creation or reset."""
+        a = (C.c_double * 6)()
+        check(lib().sfx_get_balance(self.h, a), "get_balance")
+        return dict(zip(("volume", "storage", "boundary", "sources", "rain", "qext"), list(a)))
+
+    def update_zbuvmx(self):
+        """update_zbuv: zbuvmx from the device's bed levels (after a write of "zb")."""
+        check(lib().sfx_update_zbuvmx(self.h), "update_zbuvmx")
+
+    def reset_balance(self):
+        check(lib().sfx_reset_balance(self.h), "reset_balance")
+
     def autotune(self, trials=3):
         check(lib().sfx_autotune(self.h, trials), "autotune")
         a = (C.c_int32 * 8)()
@@ -414,6 +445,16 @@ class Instance:
         a = (C.c_int32 * 5)(momentum_kernel, momentum_tile[0], momentum_tile[1], continuity_tile[0],
                             continuity_tile[1])
         check(lib().sfx_set_tuning(self.h, a), "set_tuning")
+
+    def set_work_order(self, mom_run, cont_run):
+        """Runs of tiles dealt to the XCDs in turn (0: one band per XCD), as autotune picks."""
+        check(lib().sfx_set_work_order(self.h, int(mom_run), int(cont_run)), "set_work_order")
+
+    def pin_tuning(self, tuning):
+        """Pin a tuning dict returned by autotune() on this handle (same mesh shape)."""
+        if tuning["dense"]:
+            self.set_tuning(tuning["momentum_kernel"], tuning["momentum_tile"], tuning["continuity_tile"])
+        self.set_work_order(*tuning["xcd_run"])
 
     def profile(self, nsteps):
         ms = (C.c_double * 3)()

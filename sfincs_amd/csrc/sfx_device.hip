@@ -123,6 +123,16 @@ struct DevInst {
   float* tsa_req; float* tsa_sum; int32_t* tsa_wet; int32_t* tsa_lim;
   float* tsa_min;                // the last sweep's minimum time step
   float* qext;                   // external source per cell, m/s (use_qext)
+  // FP64 volume balance of the interior cells (kcs 1). Each step adds to bal[0..3] the
+  // volume that entered them through faces whose other side is not interior (boundary,
+  // Neumann and inactive cells; summed by one block of the continuity launch), from point
+  // sources as applied (the regular branch's bed clamp included), rain and qext; bal[4..5]
+  // receive the interior volume and the storage left when they are summed.
+  double* bal;
+  int32_t nbf;                   // faces read by an interior cell whose other side is not interior
+  const int32_t* bf_slot;        // their q slots
+  const double* bf_coef;         // volume per unit q and dt as the interior cell counts it (+- width)
+  double rain_area;              // total area of the cells rain falls on (kcs 1)
   void* storage;                 // storage volume left per subgrid cell, m3 (use_storage_volume)
   uint8_t* kcs;                  // kcs (KCS_MASK) | KCS_SOURCE (point source) | KCS_STEADY (continuity)
   const float* zb;
@@ -1307,15 +1317,22 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
   const double zs_in = zs;
   const R zsmax_in = zsmax;
   const R qx = (I.c.flags & sfx::F_QEXT) ? (R)G(I.qext)[c] : R(0);
+  if (qx != R(0) && kcs == 1)   // qext volume of the balance (sparse: BMI coupling cells)
+    atomicAdd(I.bal + 3, (double)(qx * dt) * (double)I.c.cell_area[iref - 1]);
   if (SUBGRID) {
     double vol = G(I.vol)[c];
     const double vol_in = vol;   // as loaded: a source alone changes the volume too
     if (kraw & KCS_SOURCE) {   // point sources of this cell, in source order (:311-325)
+      double applied = 0.0;
       for (int s = 0; s < I.nsrc; ++s)
         if (G(I.src_cell)[s] == c) {
           const float qs = G(I.qtsrc)[s];
-          if ((vol >= 0) || ((qs < 0.0f) && (vol >= 0))) vol = vol + (double)((R)qs * dt);
+          if ((vol >= 0) || ((qs < 0.0f) && (vol >= 0))) {
+            vol = vol + (double)((R)qs * dt);
+            applied += (double)((R)qs * dt);
+          }
         }
+      if (applied != 0.0) atomicAdd(I.bal + 1, applied);
     }
     const auto zs0p = G((R*)I.zs0);
     const auto zsdp = G((R*)I.zsderv);
@@ -1335,11 +1352,13 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
     }
   } else {
     if (kraw & KCS_SOURCE) {   // (:87-106)
+      const double before = zs;
       for (int s = 0; s < I.nsrc; ++s)
         if (G(I.src_cell)[s] == c) {
           const R inc = (R)G(I.qtsrc)[s] * dt / (R)I.c.cell_area[iref - 1];
           zs = sfx::vmax<double>(zs + (double)inc, (double)G(I.zb)[c]);
         }
+      if (zs != before) atomicAdd(I.bal + 1, (zs - before) * (double)I.c.cell_area[iref - 1]);
     }
     sfx::cell_regular<R>(zs, zsmax, kcs, q_md, q_mu, q_nd, q_nu, netprcp, qx, iref, I.c, dt);
   }
@@ -1412,15 +1431,43 @@ __device__ __forceinline__ void count_stores(const DevInst& I, int tile, unsigne
   }
 }
 
-// The first `nahead` blocks (padded to a multiple of 8, so the cells keep their XCDs)
-// compute the next step of scenario ahead_ids[b] of ahead_insts (forcing_ahead_block).
+// One block of a continuity launch per scenario: the volume this step's update moves
+// through the faces between interior cells and cells that are not interior, from the same
+// q buffer and dt as the cells, and the step's rain volume, added in FP64 to bal[0], bal[2].
+template <typename R>
+__device__ void balance_block(const DevInst& I) {
+  const auto S = G(I.sc);
+  if (!S->active) return;
+  const int qb = __builtin_amdgcn_readfirstlane(S->mom_done ? (S->cur ^ 1) : S->cur);
+  const auto q = G((const R*)I.q[qb]);
+  const R dt = (R)S->dt;
+  double acc = 0.0;
+  for (int f = threadIdx.x; f < I.nbf; f += BLOCK) acc += G(I.bf_coef)[f] * (double)q[G(I.bf_slot)[f]];
+  for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
+  __shared__ double red[BLOCK / WAVE];
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int i = 0; i < BLOCK / WAVE; ++i) sum += red[i];
+    const auto bal = G(I.bal);
+    bal[0] = bal[0] + sum * (double)dt;
+    if (I.c.flags & sfx::F_PRECIP) bal[2] = bal[2] + (double)((R)S->netprcp * dt) * I.rain_area;
+  }
+}
+
+// The first `nahead` blocks compute the next step of scenario ahead_ids[b] of ahead_insts
+// (forcing_ahead_block), the next `nbal` blocks the volume balance of scenario
+// ahead_ids[b - nahead] (balance_block); the leading blocks are padded to a multiple of 8,
+// so the cells keep their XCDs.
 template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts, WorkList L, int indexed,
-                                                      int nahead, const DevInst* __restrict__ ahead_insts,
+                                                      int nahead, int nbal, const DevInst* __restrict__ ahead_insts,
                                                       const int32_t* __restrict__ ahead_ids) {
-  const uint32_t lead = (uint32_t)(nahead + 7) / 8 * 8;
+  const uint32_t lead = (uint32_t)(nahead + nbal + 7) / 8 * 8;
   if (blockIdx.x < lead) {
     if ((int)blockIdx.x < nahead) forcing_ahead_block(ahead_insts[ahead_ids[blockIdx.x]]);
+    else if ((int)blockIdx.x < nahead + nbal) balance_block<R>(ahead_insts[ahead_ids[blockIdx.x - nahead]]);
     return;
   }
   int tile, inst;
@@ -1502,6 +1549,56 @@ __global__ void __launch_bounds__(BLOCK) k_volume(const DevInst* __restrict__ I0
 }
 
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+// Volume balance sums: bal[4] = interior volume (V, or zs*A, over kcs 1 cells), bal[5] =
+// storage volume left; both set to zero before the launch.
+__global__ void __launch_bounds__(BLOCK) k_balance_volume(const DevInst* __restrict__ I0) {
+  const DevInst& I = *I0;
+  double v = 0.0, st = 0.0;
+  for (int c = blockIdx.x * BLOCK + threadIdx.x; c < I.np; c += gridDim.x * BLOCK) {
+    if ((G(I.kcs)[c] & KCS_MASK) != 1) continue;
+    const int iref = I.dense ? 1 : G(I.z_iref)[c];
+    v += I.subgrid ? G(I.vol)[c] : G(I.zs)[c] * (double)I.c.cell_area[iref - 1];
+    if (I.c.flags & sfx::F_STORAGE)
+      st += I.fp64 ? G((const double*)I.storage)[c] : (double)G((const float*)I.storage)[c];
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) { v += __shfl_xor(v, o, WAVE); st += __shfl_xor(st, o, WAVE); }
+  __shared__ double red[2][BLOCK / WAVE];
+  if (threadIdx.x % WAVE == 0) { red[0][threadIdx.x / WAVE] = v; red[1][threadIdx.x / WAVE] = st; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int i = 0; i < BLOCK / WAVE; ++i) { a += red[0][i]; b += red[1][i]; }
+    atomicAdd(I.bal + 4, a);
+    if (I.c.flags & sfx::F_STORAGE) atomicAdd(I.bal + 5, b);
+  }
+}
+
+// compute_zbuvmx (sfincs_domain.f90:1746-1762): zbuvmx = max(zb(nm), zb(nmu)) + huthresh for
+// every uv-point from the bed levels on the device (the uv-points of a dense mesh are the
+// slots whose right / upper cell exists).
+__global__ void __launch_bounds__(BLOCK) k_zbuvmx(const DevInst* __restrict__ I0) {
+  const DevInst& I = *I0;
+  float* zbuvmx = (float*)I.zbuvmx;
+  const float huth = I.c.huthresh;
+  for (int s = blockIdx.x * BLOCK + threadIdx.x; s < (I.dense ? 2 * I.np : I.npuv); s += gridDim.x * BLOCK) {
+    int nm, nmu;
+    if (I.dense) {
+      const int c = s < I.np ? s : s - I.np;
+      const int n = c % I.nmax, m = c / I.nmax;
+      if (s < I.np) { if (m >= I.mmax - 1) continue; nm = c; nmu = c + I.nmax; }
+      else { if (n >= I.nmax - 1) continue; nm = c; nmu = c + 1; }
+    } else {
+      nm = G(I.nb)[s];
+      nmu = G(I.nb)[(uint32_t)I.npuv + s];
+    }
+    zbuvmx[s] = sfx::vmax<float>(G(I.zb)[nm], G(I.zb)[nmu]) + huth;
+  }
+}
+
+__global__ void k_zero_bal(double* p, int from, int to) {
+  for (int k = from + (int)threadIdx.x; k < to; k += blockDim.x) p[k] = 0.0;
+}
 
 }  // namespace
 
@@ -1901,6 +1998,34 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
       H->nct_cap = std::max(H->nct_cap, cell_tiles(I, CELL_SHAPES[k][0], CELL_SHAPES[k][1]));
   I.cntstore = H->alloc<uint32_t>(NSTORE * std::max<size_t>(1, H->nct_cap));
   H->dvol = H->alloc<double>(1);
+  {   // volume balance: faces of interior cells whose other side is not interior
+    const std::vector<int32_t>* faces[4] = {&M->z_index_uv_md, &M->z_index_uv_mu, &M->z_index_uv_nd, &M->z_index_uv_nu};
+    const int dummy = npuv + ncuv + 1;
+    std::vector<int32_t> slot;
+    std::vector<double> coef;
+    double rain_area = 0.0;
+    for (int c = 0; c < np; ++c) {
+      if (M->kcs[c] != 1) continue;
+      const int ir = M->z_flags_iref[c] - 1;
+      const double area = (double)M->cell_area[ir];
+      rain_area += area;
+      for (int k = 0; k < 4; ++k) {
+        const int f = (*faces[k])[c];
+        if (f <= 0 || f == dummy || M->kcuv[f - 1] == 1) continue;
+        // continuity counts md and nd faces in, mu and nu faces out (sfincs_continuity.f90:130-148, 337-352)
+        double w;
+        if (M->subgrid) w = k < 2 ? (double)(M->use_quadtree ? M->dyrm[ir] : I.c.dy) : (double)(M->use_quadtree ? M->dxrm[ir] : I.c.dx);
+        else w = area * (double)(k < 2 ? M->dxrinv[ir] : M->dyrinv[ir]);
+        slot.push_back(H->ref2slot[f - 1]);
+        coef.push_back((k % 2 == 0) ? w : -w);
+      }
+    }
+    I.nbf = (int32_t)slot.size();
+    I.bf_slot = H->upload(slot.data(), slot.size());
+    I.bf_coef = H->upload(coef.data(), coef.size());
+    I.rain_area = rain_area;
+    I.bal = H->alloc<double>(6);
+  }
   H->dI = H->upload(&I, 1);
   for (void* p : H->allocs) if (!p) { set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
   HIPCHK(hipDeviceSynchronize());
@@ -2024,6 +2149,11 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
   const size_t np = H->hI.np;
   DevInst& I = H->hI;
   if (k == "zs") { if (nbytes != np * 8) { set_error("size mismatch"); return SFX_ERR_ARG; } HIPCHK(hipMemcpy(dst, I.zs, nbytes, hipMemcpyDeviceToHost)); return SFX_OK; }
+  if (k == "zb") {
+    if (!I.zb || nbytes != np * 4) { set_error("sfx_get_state: zb needs a regular (not subgrid) mesh and np f32 values"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(dst, I.zb, nbytes, hipMemcpyDeviceToHost));
+    return SFX_OK;
+  }
   if (k == "z_volume") {
     if (!I.vol || nbytes != np * 8) { set_error("no z_volume / size mismatch"); return SFX_ERR_ARG; }
     HIPCHK(hipMemcpy(dst, I.vol, nbytes, hipMemcpyDeviceToHost)); return SFX_OK;
@@ -2094,6 +2224,11 @@ extern "C" int sfx_set_state(sfx_handle* H, const char* name, const void* src, s
   if (k == "z_volume") {
     if (!I.vol || nbytes != np * 8) { set_error("no z_volume / size mismatch"); return SFX_ERR_ARG; }
     HIPCHK(hipMemcpy(I.vol, src, nbytes, hipMemcpyHostToDevice));
+    return clear_steady(H);
+  }
+  if (k == "zb") {   // bed level of a regular mesh, e.g. written by a coupler through the BMI
+    if (!I.zb || nbytes != np * 4) { set_error("sfx_set_state: zb needs a regular (not subgrid) mesh and np f32 values"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy((void*)I.zb, src, nbytes, hipMemcpyHostToDevice));
     return clear_steady(H);
   }
   if (k == "q" || k == "uv") {
@@ -2362,16 +2497,18 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   }
   // A whole step: the next step's fold, scalars and series (forcing_ahead_block) run as
   // the leading blocks of this step's first continuity launch, beside its cells.
+  // The volume balance of every member rides in the same leading blocks.
   const int nahead = (what == 7 && forcing_mode == 0) ? nm : 0;
-  const int lead = (nahead + 7) / 8 * 8;   // keeps block b of the cells on XCD b mod 8
+  const int nbal = nm;
+  const int lead = (nahead + nbal + 7) / 8 * 8;   // keeps block b of the cells on XCD b mod 8
   if (what & 4) {
     if (B->n_cd)
       hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd + lead), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0,
-                         nahead, B->dInsts, B->dIds);
+                         nahead, nbal, B->dInsts, B->dIds);
     if (B->n_ci) {
-      const int la = B->n_cd ? 0 : nahead, ll = B->n_cd ? 0 : lead;
+      const int la = B->n_cd ? 0 : nahead, lb = B->n_cd ? 0 : nbal, ll = B->n_cd ? 0 : lead;
       hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci + ll), dim3(BLOCK), 0, st, base_of(B, B->wl_ci), B->wl_ci, 1,
-                         la, B->dInsts, B->dIds);
+                         la, lb, B->dInsts, B->dIds);
     }
   }
 }
@@ -2539,6 +2676,41 @@ extern "C" int sfx_total_volume(sfx_handle* H, double* vol) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(H->stream));
   HIPCHK(hipMemcpy(vol, H->dvol, sizeof(double), hipMemcpyDeviceToHost));
+  return SFX_OK;
+}
+
+extern "C" int sfx_get_balance(sfx_handle* H, double out[6]) {
+  if (!H || !out) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  hipLaunchKernelGGL(k_zero_bal, dim3(1), dim3(64), 0, H->stream, H->hI.bal, 4, 6);
+  const int nb = std::min(1024, (H->hI.np + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_balance_volume, dim3(nb), dim3(BLOCK), 0, H->stream, H->dI);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(H->stream));
+  double b[6];
+  HIPCHK(hipMemcpy(b, H->hI.bal, sizeof(b), hipMemcpyDeviceToHost));
+  out[0] = b[4]; out[1] = b[5]; out[2] = b[0]; out[3] = b[1]; out[4] = b[2]; out[5] = b[3];
+  return SFX_OK;
+}
+
+extern "C" int sfx_update_zbuvmx(sfx_handle* H) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  if (!H->hI.zb || !H->hI.zbuvmx) { set_error("sfx_update_zbuvmx: needs a regular (not subgrid) mesh"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  const int n = H->hI.dense ? 2 * H->hI.np : H->hI.npuv;
+  const int nb = std::max(1, std::min(4096, (n + BLOCK - 1) / BLOCK));
+  hipLaunchKernelGGL(k_zbuvmx, dim3(nb), dim3(BLOCK), 0, H->stream, H->dI);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(H->stream));
+  return clear_steady(H);
+}
+
+extern "C" int sfx_reset_balance(sfx_handle* H) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  HIPCHK(hipSetDevice(H->device));
+  hipLaunchKernelGGL(k_zero_bal, dim3(1), dim3(64), 0, H->stream, H->hI.bal, 0, 6);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(H->stream));
   return SFX_OK;
 }
 
@@ -2727,6 +2899,21 @@ extern "C" int sfx_set_tuning(sfx_handle* H, const int32_t in[5]) {
   return SFX_OK;
 }
 
+extern "C" int sfx_set_work_order(sfx_handle* H, int32_t mom_run, int32_t cont_run) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  if (mom_run < 0 || cont_run < 0 || mom_run > (1 << 20) || cont_run > (1 << 20)) {
+    set_error("sfx_set_work_order: runs per XCD must lie in 0..2^20 (0: one contiguous band per XCD)");
+    return SFX_ERR_ARG;
+  }
+  if (H->in_batches) { set_error("sfx_set_work_order: set the order before adding the scenario to a batch"); return SFX_ERR_STATE; }
+  HIPCHK(hipSetDevice(H->device));
+  H->hI.xchunk = mom_run; H->hI.xchunk_c = cont_run;
+  H->xchunk = mom_run; H->xchunk_c = cont_run;
+  HIPCHK(hipMemcpy(H->dI, &H->hI, sizeof(DevInst), hipMemcpyHostToDevice));
+  if (H->solo) H->solo->dirty = true;
+  return SFX_OK;
+}
+
 extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
@@ -2755,6 +2942,8 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
     snap[f].resize(n);
     if ((rc = sfx_get_state(H, f.c_str(), snap[f].data(), n))) return rc;
   }
+  double bal_keep[6];   // timed sweeps add to the volume balance: restored afterwards
+  HIPCHK(hipMemcpy(bal_keep, H->hI.bal, sizeof(bal_keep), hipMemcpyDeviceToHost));
   Scalars s = s0;
   s.active = 1; s.dt = std::max(s0.dt, 1.0e-3f);
   if ((rc = write_scalars(H, s))) return rc;
@@ -2821,6 +3010,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
   if ((rc = reset_blkmin(H))) return rc;
   for (auto& f : fields)
     if ((rc = sfx_set_state(H, f.c_str(), snap[f].data(), snap[f].size()))) return rc;
+  HIPCHK(hipMemcpy(H->hI.bal, bal_keep, sizeof(bal_keep), hipMemcpyHostToDevice));
   if (H->solo) H->solo->dirty = true;
   return SFX_OK;
 }

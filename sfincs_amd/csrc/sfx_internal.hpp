@@ -37,6 +37,9 @@ struct sfx_model {
   // quadtree point list (for point lookup: level-major, nm sorted)
   std::vector<int8_t> qt_level;
   std::vector<int32_t> qt_nm, qt_first, qt_last, index_sfincs_in_quadtree;
+  // quadtree-file right/above neighbour flags and points, for the legacy binary subgrid file
+  std::vector<int8_t> qt_mu, qt_nu;
+  std::vector<int32_t> qt_mu1, qt_mu2, qt_nu1, qt_nu2;
 
   // cells
   std::vector<int8_t> kcs, z_flags_iref;

@@ -505,6 +505,7 @@ extern "C" int sfx_model_create_quadtree(const sfx_params* p, const sfx_quadtree
   M->use_quadtree = true;
   int rc = build_mesh(M, Q);
   if (rc != SFX_OK) { delete M; return rc; }
+  M->qt_mu = Q.mu; M->qt_nu = Q.nu; M->qt_mu1 = Q.mu1; M->qt_mu2 = Q.mu2; M->qt_nu1 = Q.nu1; M->qt_nu2 = Q.nu2;
   M->nmax = 0; M->mmax = 0;
   for (int32_t k = 0; k < n; ++k) {
     if (Q.level[k] != 1) continue;
@@ -586,6 +587,157 @@ extern "C" int sfx_model_set_subgrid(sfx_model* M, const sfx_subgrid_input* s) {
   for (int32_t i = 0; i < npuv; ++i) {
     M->uv_havg_zmax[i] = M->uv_havg[(size_t)i * nl + (nl - 1)] - M->uv_zmax[i];
     M->uv_nrep_zmax[i] = M->uv_nrep[(size_t)i * nl + (nl - 1)];
+  }
+  return SFX_OK;
+}
+
+// read_subgrid_file_original (sfincs_subgrid.F90:458-903): the legacy binary subgrid file
+// (stream of int32 headers and float32 records, no NetCDF). It holds hrep and navg per
+// level: levels 2..nl come from the file, level 1 of havg is huthresh (which keeps its
+// input value on this path, unlike the NetCDF path), nrep = g max(navg, nmin)^2 with level 1
+// a copy of level 2, pwet = 1 and no fit (fnfit = 0). Regular meshes: records of np values
+// per cell, the uv records per cell through its right (u) and upper (v) point. Quadtree
+// meshes: records over every quadtree point / every quadtree uv-point, re-mapped to the
+// active cells and uv-points.
+extern "C" int sfx_model_read_subgrid_binary(sfx_model* M, const char* path) {
+  if (!M || !path || !M->mesh_built || M->initialized) { set_error("sfx_model_read_subgrid_binary: bad state"); return SFX_ERR_STATE; }
+  FILE* f = std::fopen(path, "rb");
+  if (!f) { set_error(std::string("cannot open subgrid file ") + path); return SFX_ERR_IO; }
+  struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
+  bool ok = true;
+  auto rd_i = [&](int32_t& v) { ok = ok && std::fread(&v, 4, 1, f) == 1; };
+  auto rd_f = [&](std::vector<float>& v, size_t n) { v.resize(n); ok = ok && std::fread(v.data(), 4, n, f) == n; };
+  const int32_t np = M->np, npuv = M->npuv;
+  const float g = M->p.g, huth = M->p.huthresh;
+  // regular: cells (not checked, as the reference does not), option, bins;
+  // quadtree: version, quadtree points, quadtree uv-points, bins
+  int32_t h0 = 0, h1 = 0, h2 = 0, nbins = 0, nzq = np, nuvq = 0;
+  rd_i(h0); rd_i(h1); rd_i(h2);
+  if (M->use_quadtree) { nzq = h1; nuvq = h2; rd_i(nbins); } else { nbins = h2; }
+  if (!ok || nbins < 1 || nbins > 1000) { set_error(std::string("subgrid file ") + path + ": bad header"); return SFX_ERR_IO; }
+  {   // the records must fill the file exactly
+    const long hdr = M->use_quadtree ? 16 : 12;
+    const long expect = M->use_quadtree
+                            ? hdr + 4L * ((long)nzq * (4 + nbins) + (long)nuvq * (2 + 2 * nbins))
+                            : hdr + 4L * (long)np * ((3 + nbins) + 2 * (3 + 2 * nbins));
+    const long here = std::ftell(f);
+    std::fseek(f, 0, SEEK_END);
+    const long size = std::ftell(f);
+    std::fseek(f, here, SEEK_SET);
+    if (size != expect) {
+      set_error(std::string("subgrid file ") + path + ": " + std::to_string(size) + " bytes, the mesh needs " +
+                std::to_string(expect));
+      return SFX_ERR_IO;
+    }
+  }
+  (void)h0;
+  // source index of each active cell and uv-point in the file's records
+  std::vector<int32_t> zsrc(np), uvsrc(npuv, -1);
+  if (M->use_quadtree) {
+    if (nzq != (int32_t)M->index_sfincs_in_quadtree.size()) {
+      set_error(std::string("subgrid file ") + path + ": header does not match the quadtree");
+      return SFX_ERR_IO;
+    }
+    for (int32_t k = 0; k < nzq; ++k)
+      if (M->index_sfincs_in_quadtree[k] > 0) zsrc[M->index_sfincs_in_quadtree[k] - 1] = k;
+    // the file's uv-points: per quadtree point, right then above, one per neighbour
+    // (sfincs_subgrid.F90:530-600); the active ones map to the sfincs uv-points in order
+    int32_t nq = 0, ns = 0;
+    auto take = [&](int32_t nm, const std::vector<int32_t>& idx) {
+      ++nq;
+      if (nm > 0 && idx[nm - 1] > 0 && ns < npuv) uvsrc[ns++] = nq - 1;
+    };
+    for (int32_t ip = 0; ip < nzq; ++ip) {
+      const int32_t nm = M->index_sfincs_in_quadtree[ip];
+      if (M->qt_mu1[ip] > 0) take(nm, M->z_index_uv_mu1);
+      if (M->qt_mu[ip] >= 1 && M->qt_mu2[ip] > 0) take(nm, M->z_index_uv_mu2);
+      if (M->qt_nu1[ip] > 0) take(nm, M->z_index_uv_nu1);
+      if (M->qt_nu[ip] >= 1 && M->qt_nu2[ip] > 0) take(nm, M->z_index_uv_nu2);
+    }
+    if (nq != nuvq || ns != npuv) {
+      set_error(std::string("subgrid file ") + path + ": uv-points do not match the quadtree");
+      return SFX_ERR_IO;
+    }
+  } else {
+    for (int32_t c = 0; c < np; ++c) zsrc[c] = c;
+  }
+  const int32_t nlev = nbins + 1;
+  M->subgrid = true;
+  M->nl = nlev;
+  M->z_zmin.assign(np, 0.f); M->z_zmax.assign(np, 0.f); M->z_volmax.assign(np, 0.f);
+  M->z_dep.assign((size_t)np * nlev, 0.f);
+  M->uv_zmin.assign(npuv, 0.f); M->uv_zmax.assign(npuv, 0.f);
+  M->uv_havg.assign((size_t)npuv * nlev, huth);
+  M->uv_nrep.assign((size_t)npuv * nlev, 0.f);
+  M->uv_pwet.assign((size_t)npuv * nlev, 1.0f);
+  M->uv_fnfit.assign(npuv, 0.0f);
+  std::vector<float> r;
+  auto cells = [&](std::vector<float>& dst, size_t stride, size_t off) {
+    rd_f(r, nzq);
+    if (!ok) return;
+    for (int32_t c = 0; c < np; ++c) dst[(size_t)c * stride + off] = r[zsrc[c]];
+  };
+  cells(M->z_zmin, 1, 0);
+  cells(M->z_zmax, 1, 0);
+  if (M->use_quadtree) rd_f(r, nzq);   // zmean (not used)
+  cells(M->z_volmax, 1, 0);
+  for (int32_t c = 0; c < np; ++c) M->z_dep[(size_t)c * nlev] = std::max(M->z_zmin[c], -20.0f);
+  for (int32_t l = 1; l < nlev; ++l) cells(M->z_dep, nlev, l);
+  if (M->use_quadtree) {
+    auto uvs = [&](std::vector<float>& dst, size_t stride, size_t off, int kind) {
+      rd_f(r, nuvq);
+      if (!ok) return;
+      for (int32_t i = 0; i < npuv; ++i) {
+        const float v = r[uvsrc[i]];
+        float& d = dst[(size_t)i * stride + off];
+        if (kind == 0) d = v;
+        else if (kind == 1) d = std::max(v, huth);
+        else { const float w = std::max(v, 0.005f); d = g * (w * w); }
+      }
+    };
+    uvs(M->uv_zmin, 1, 0, 0);
+    uvs(M->uv_zmax, 1, 0, 0);
+    for (int32_t l = 1; l < nlev; ++l) uvs(M->uv_havg, nlev, l, 1);
+    for (int32_t l = 1; l < nlev; ++l) uvs(M->uv_nrep, nlev, l, 2);
+  } else {
+    // u records then v records, each per cell through its right / upper point
+    for (int dir = 0; dir < 2 && ok; ++dir) {
+      const std::vector<int32_t>& idx = dir == 0 ? M->z_index_uv_mu1 : M->z_index_uv_nu1;
+      auto uvs = [&](std::vector<float>& dst, size_t stride, size_t off, int kind) {
+        rd_f(r, np);
+        if (!ok) return;
+        for (int32_t c = 0; c < np; ++c) {
+          const int32_t ip = idx[c];
+          if (ip <= 0) continue;
+          float& d = dst[(size_t)(ip - 1) * stride + off];
+          if (kind == 0) d = r[c];
+          else { const float w = std::max(r[c], 0.0001f); d = g * (w * w); }
+        }
+      };
+      uvs(M->uv_zmin, 1, 0, 0);
+      uvs(M->uv_zmax, 1, 0, 0);
+      rd_f(r, np);   // dhdz (not used)
+      for (int32_t l = 1; l < nlev; ++l) uvs(M->uv_havg, nlev, l, 0);
+      for (int32_t l = 1; l < nlev; ++l) uvs(M->uv_nrep, nlev, l, 2);
+    }
+  }
+  if (!ok) { set_error(std::string("subgrid file ") + path + ": file ends early"); return SFX_ERR_IO; }
+  for (int32_t i = 0; i < npuv; ++i) M->uv_nrep[(size_t)i * nlev] = M->uv_nrep[(size_t)i * nlev + 1];
+  // sanitising of this path (:873-900)
+  for (int32_t i = 0; i < npuv; ++i) {
+    const int32_t nm = M->uv_index_z_nm[i] - 1, nmu = M->uv_index_z_nmu[i] - 1;
+    M->uv_zmin[i] = std::max(M->uv_zmin[i], M->z_zmin[nm]);
+    M->uv_zmin[i] = std::max(M->uv_zmin[i], M->z_zmin[nmu]);
+  }
+  for (int32_t c = 0; c < np; ++c)
+    if (M->z_zmax[c] - M->z_zmin[c] < 0.001f) M->z_zmax[c] = M->z_zmax[c] + 0.001f;
+  for (int32_t i = 0; i < npuv; ++i)
+    if (M->uv_zmax[i] - M->uv_zmin[i] < 0.001f) M->uv_zmax[i] = M->uv_zmax[i] + 0.001f;
+  M->uv_havg_zmax.resize(npuv); M->uv_nrep_zmax.resize(npuv); M->uv_navg_w.resize(npuv);
+  for (int32_t i = 0; i < npuv; ++i) {
+    M->uv_havg_zmax[i] = M->uv_havg[(size_t)i * nlev + (nlev - 1)] - M->uv_zmax[i];
+    M->uv_nrep_zmax[i] = M->uv_nrep[(size_t)i * nlev + (nlev - 1)];
+    M->uv_navg_w[i] = M->uv_nrep_zmax[i];
   }
   return SFX_OK;
 }

@@ -50,7 +50,11 @@ class Scenario:
         else:
             m = _lib.Model.quadtree(p, self.qt)
         if self.subgrid is not None:
-            m.set_subgrid(self.subgrid(m))
+            sg = self.subgrid(m)
+            if isinstance(sg, str):
+                m.read_subgrid_binary(sg)      # legacy binary subgrid file
+            else:
+                m.set_subgrid(sg)
         else:
             m.set_bathymetry(self.active_zb(m))
         if self.bnd is not None:
@@ -255,6 +259,53 @@ def regular_subgrid_tables(dem, nman, refi, dx, nl, model):
     return sg
 
 
+def write_subgrid_binary(path, sg, model):
+    """Writes NetCDF-style tables `sg` of a regular mesh (as regular_subgrid_tables returns
+    them) as a legacy binary subgrid file, the layout read_subgrid_file_original reads
+    (sfincs_subgrid.F90:671-866): int32 cells, option, bins (= levels - 1); per cell zmin,
+    zmax, volmax and the levels 2..nl; then for the u-points and the v-points, per cell
+    through its right / upper point, zmin, zmax, dhdz (unused, 0), hrep (havg) and navg
+    (Manning's n) of levels 2..nl. Cells without such a point write 0."""
+    nl = int(sg["nlevels"])
+    np_ = len(sg["z_zmin"])
+    zl = np.asarray(sg["z_level"], np.float32).reshape(np_, nl)
+    havg = np.asarray(sg["uv_havg"], np.float32).reshape(-1, nl)
+    nrep = np.asarray(sg["uv_nrep"], np.float32).reshape(-1, nl)
+    with open(path, "wb") as f:
+        np.array([np_, 0, nl - 1], np.int32).tofile(f)
+        for k in ("z_zmin", "z_zmax", "z_volmax"):
+            np.asarray(sg[k], np.float32).tofile(f)
+        for lv in range(1, nl):
+            np.ascontiguousarray(zl[:, lv]).tofile(f)
+        for name in ("z_index_uv_mu1", "z_index_uv_nu1"):
+            ip = model.table(name).astype(np.int64)
+            has = ip > 0
+            sel = np.where(has, ip - 1, 0)
+
+            def per_cell(a):
+                return np.where(has, np.asarray(a, np.float32)[sel], np.float32(0.0)).astype(np.float32)
+            per_cell(sg["uv_zmin"]).tofile(f)
+            per_cell(sg["uv_zmax"]).tofile(f)
+            np.zeros(np_, np.float32).tofile(f)
+            for lv in range(1, nl):
+                per_cell(havg[:, lv]).tofile(f)
+            for lv in range(1, nl):
+                per_cell(nrep[:, lv]).tofile(f)
+
+
+def with_subgrid_binary(sc, path):
+    """The scenario with its subgrid tables read from a legacy binary file written at `path`
+    (huthresh keeps its input value on that path)."""
+    make = sc.subgrid
+
+    def tables_via_file(model, _make=make, _path=str(path)):
+        write_subgrid_binary(_path, _make(model), model)
+        return _path
+    sc.subgrid = tables_via_file
+    sc.name += "_sbgbin"
+    return sc
+
+
 # ---------------------------------------------------------------- scenario recipes
 def tide(amplitude, t_end, period=43200.0, step=10.0, phase=0.0):
     t = np.arange(0.0, t_end + 2 * step, step, dtype=np.float64)
@@ -337,11 +388,30 @@ def with_river(sc, first=1, ncells=8, slope=1.0e-3):
     return sc
 
 
+def face_low_edge_west(dem, refi):
+    """The DEM turned so that its edge with the most cells whose lowest pixel lies below
+    0 m is the west edge (m = 0), where the tidal boundary is. Candidates: as generated,
+    mirrored east-west, and on square grids the two transposes; the first of the best
+    is kept. A seeded fBm can rise above 0 m along its whole original west edge
+    (configuration 1's surface does: 2.1 .. 18.1 m there), which would keep the tide out
+    of the domain altogether. dem: (nmax*refi, mmax*refi), rows = n (y), cols = m (x)."""
+    cands = [dem, dem[:, ::-1]]
+    if dem.shape[0] == dem.shape[1]:
+        cands += [dem.T, dem.T[:, ::-1]]
+    best, best_n = cands[0], -1
+    for c in cands:
+        west = c[:, :refi].reshape(-1, refi, refi).min(axis=(1, 2))
+        n_low = int(np.count_nonzero(west < 0.0))
+        if n_low > best_n:
+            best, best_n = c, n_low
+    return np.ascontiguousarray(best)
+
+
 def subgrid_case(nmax, mmax, dx=25.0, refi=5, nl=10, seed=1001, t_end=21600.0, tide_amp=1.5,
                  nsrc=8, rain_mmh=0.0, **params):
     """Configuration-1 recipe at any size (regular mesh with subgrid tables from an fBm DEM)."""
     r = rng(seed)
-    dem = fbm(r, nmax * refi, mmax * refi)
+    dem = face_low_edge_west(fbm(r, nmax * refi, mmax * refi), refi)
     nman = landuse_manning(r, nmax * refi, mmax * refi, refi=refi)
     zcell_min = dem.reshape(nmax, refi, mmax, refi).min(axis=(1, 3))     # (nmax, mmax)
     msk = np.ones((mmax, nmax), np.int8)

@@ -1,0 +1,229 @@
+"""The five benchmark workloads at their full sizes on the GPU, against the oracle and the
+reference's own hot path (oracle/_ref) on identical inputs.
+
+* Configuration 0 (regular 512 x 512, 1 h): the whole period against the oracle.
+* Configuration 1 (2048 x 2048 subgrid, 6 h): two windows of 500 steps, one from t = 0 and
+  one from the GPU state at the tide's crest (t = 3 h), each against the oracle started
+  from the same state; the crest window also against the reference's own
+  compute_fluxes / compute_water_levels.
+* Configuration 2 (quadtree, 4 levels, about 7.7M cells): the first 2000 steps.
+* Configuration 3 (ensemble): full-size members drawn from the configuration-3 recipe, run
+  batched and alone over a shortened duration; the largest also against the oracle.
+* Configuration 4 (served stream): full-size requests drawn from the configuration-4
+  recipe, including a quadtree of 1-8M cells, served together and alone.
+
+Gate (after each window): max|dzs| <= 1e-4 m, max|dq| <= 1e-5 m2/s, identical step count
+and model time; subgrid volumes to 1e-7 relative."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import Q_TOL, ZS_TOL, compare_state, compare_subgrid_state, gpu_for, oracle_for
+from sfincs_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WINDOW = 500
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _check(d, subgrid):
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL, d
+    if subgrid:
+        assert d["z_volume_rel"] <= 1e-7 and d["zs0"] <= ZS_TOL and d["zsderv"] <= ZS_TOL, d
+
+
+def _diff(inst, orc, subgrid):
+    d = compare_state(inst, orc)
+    if subgrid:
+        d.update(compare_subgrid_state(inst, orc))
+    return d
+
+
+def _oracle_from_gpu(sc, model, inst):
+    """A fresh oracle holding the GPU's state: fields, model time, step count and the
+    pending CFL minimum (dt = alpha * min_dt)."""
+    orc = O.OracleRun(model).configure(sc.params, model)
+    fields = ["zs", "q", "uv", "kfuv", "zs0", "zsderv", "zsmax", "uvmean"]
+    if model.sizes()["subgrid"]:
+        fields.append("z_volume")
+    for f in fields:
+        orc.state(f)[...] = inst.get(f)
+    t, dt, nt = inst.time()
+    orc.set("t", t)
+    orc.set("nt", nt)
+    orc.set("min_dt", float(np.float32(dt) / np.float32(sc.params.get("alpha", 0.5))))
+    return orc
+
+
+def _gpu_steps(inst, n):
+    from sfincs_amd import _lib
+    b = _lib.Batch()
+    b.add(inst, 1.0e9)
+    b.advance(n)
+    b.close()
+
+
+def test_config0_whole_hour_matches_oracle():
+    sc = W.config0()
+    m, orc = oracle_for(sc)
+    _say("case config0 512x512 regular, 1 h:", m.sizes()["np"], "cells")
+    inst = gpu_for(m)
+    rc, st = inst.update(-1000.0)
+    t0 = time.perf_counter()
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    _say("config0", st["nt"], int(orc.get("nt")), st["t"], d, f"oracle {time.perf_counter() - t0:.1f}s")
+    assert rc == 0 and st["nt"] == int(orc.get("nt")) and abs(st["t"] - orc.get("t")) < 1e-6
+    assert st["t"] >= sc.t_end
+    _check(d, False)
+    # the tide entered the domain: boundary inflow wetted points and moved water
+    assert np.count_nonzero(inst.get("q")) > 1000
+
+
+@pytest.fixture(scope="module")
+def config1():
+    t0 = time.perf_counter()
+    sc = W.config1()
+    sc.params["t1"] = 1.0e7
+    m = sc.build_model()
+    _say(f"config1 generated in {time.perf_counter() - t0:.1f}s", m.sizes())
+    assert m.sizes()["ngbnd"] > 0, "the tidal boundary holds no cells"
+    return sc, m
+
+
+def test_config1_window_from_start_matches_oracle(config1):
+    sc, m = config1
+    _say("case config1 2048x2048 subgrid, first", WINDOW, "steps")
+    inst = gpu_for(m)
+    orc = O.OracleRun(m).configure(sc.params, m)
+    _gpu_steps(inst, WINDOW)
+    t0 = time.perf_counter()
+    orc.update(1.0e30, WINDOW)
+    d = _diff(inst, orc, True)
+    t, _, nt = inst.time()
+    _say("config1 start window", nt, t, d, f"oracle {time.perf_counter() - t0:.1f}s")
+    assert nt == int(orc.get("nt")) == WINDOW and abs(t - orc.get("t")) < 1e-6
+    _check(d, True)
+
+
+def test_config1_window_at_tide_crest_matches_oracle_and_reference(config1):
+    sc, m = config1
+    _say("case config1 2048x2048 subgrid, window at t = 3 h")
+    inst = gpu_for(m)
+    inst.update(10800.0)
+    t_start, _, nt0 = inst.time()
+    orc = _oracle_from_gpu(sc, m, inst)
+    ref = _oracle_from_gpu(sc, m, inst) if O.ref_available() else None
+    _gpu_steps(inst, WINDOW)
+    t0 = time.perf_counter()
+    orc.update(1.0e30, WINDOW)
+    d = _diff(inst, orc, True)
+    t, _, nt = inst.time()
+    _say("config1 crest window from", t_start, "to", t, d, f"oracle {time.perf_counter() - t0:.1f}s")
+    assert nt == int(orc.get("nt")) == nt0 + WINDOW and abs(t - orc.get("t")) < 1e-6
+    _check(d, True)
+    wet = np.count_nonzero(inst.get("kfuv") & 1)
+    _say("wet uv-points at the crest:", wet)
+    assert wet > 100000
+    if ref is None:
+        pytest.skip("oracle/_ref is not built")
+    r = O.RefRun(ref)
+    t0 = time.perf_counter()
+    for _ in range(WINDOW):
+        r.step()
+    d = _diff(inst, ref, True)
+    _say("config1 crest window, reference", int(ref.get("nt")), ref.get("t"), d,
+         f"reference {time.perf_counter() - t0:.1f}s")
+    assert int(ref.get("nt")) == nt and abs(ref.get("t") - t) < 1e-6
+    _check(d, True)
+
+
+def test_config2_first_2000_steps_match_oracle():
+    t0 = time.perf_counter()
+    sc = W.config2()
+    m, orc = oracle_for(sc)
+    s = m.sizes()
+    _say(f"case config2 quadtree 4 levels, first 2000 steps: np {s['np']} npuv {s['npuv']} ncuv {s['ncuv']} "
+         f"(generated in {time.perf_counter() - t0:.1f}s)")
+    assert s["nref"] == 4 and s["ncuv"] > 0 and 7.0e6 < s["np"] < 8.5e6
+    inst = gpu_for(m)
+    _gpu_steps(inst, 2000)
+    t0 = time.perf_counter()
+    for k in range(4):
+        orc.update(1.0e30, 500)
+        _say(f"  oracle at step {int(orc.get('nt'))}, {time.perf_counter() - t0:.0f}s")
+    d = compare_state(inst, orc)
+    t, _, nt = inst.time()
+    _say("config2", nt, t, d)
+    assert nt == int(orc.get("nt")) == 2000 and abs(t - orc.get("t")) < 1e-6
+    _check(d, False)
+
+
+def test_config3_full_size_members_batched_equal_alone():
+    """The first eight configuration-3 members at full size (PCG64((1000+3)*100000+j)),
+    shortened to 600 s each: batched == alone member by member, and the largest member
+    against the oracle."""
+    from sfincs_amd import ensemble as E
+    t0 = time.perf_counter()
+    scs = W.ensemble_members(8, t_end=600.0)
+    models = [sc.build_model() for sc in scs]
+    cells = [mm.sizes()["np"] for mm in models]
+    _say(f"case config3 members {[sc.name for sc in scs]} cells {cells} "
+         f"(generated in {time.perf_counter() - t0:.1f}s)")
+    t_ends = [sc.t_end for sc in scs]
+    a = E.run_sequential(models, t_ends, keep_state=True)
+    b = E.run_batched(models, t_ends, round_steps=32, keep_state=True)
+    assert a["steps"] == b["steps"]
+    for za, zb in zip(a["zs"], b["zs"]):
+        assert np.max(np.abs(za - zb)) <= ZS_TOL
+    j = int(np.argmax(cells))
+    orc = O.OracleRun(models[j]).configure(scs[j].params, models[j])
+    orc.update(scs[j].t_end)
+    d = float(np.max(np.abs(b["zs"][j] - orc.state("zs"))))
+    _say(f"config3 largest member {scs[j].name}: steps {b['steps'][j]} oracle {int(orc.get('nt'))} max|dzs| {d}")
+    assert b["steps"][j] == int(orc.get("nt")) and d <= ZS_TOL
+
+
+def test_config4_full_size_requests_served_equal_alone():
+    """Six requests drawn from the configuration-4 recipe at full size (one a quadtree of
+    1-8M cells), durations shortened to 300-600 s, arriving 0.05 s apart: each served
+    result equals the same request run alone."""
+    from sfincs_amd import ensemble as E
+    r = W.rng(1004)
+    reqs = []
+    for j in range(5):
+        reqs.append(W.member(1004 * 100000 + j, t_end=float(r.uniform(300.0, 600.0)), name=f"request{j}"))
+    reqs.append(W.quadtree_case(200, 200, nlev=4, dx0=100.0, seed=1004 * 100000 + 5, t_end=300.0, wind=False))
+    models = [sc.build_model() for sc in reqs]
+    cells = [mm.sizes()["np"] for mm in models]
+    _say(f"case config4 requests {[sc.name for sc in reqs]} cells {cells}")
+    assert 1.0e6 <= cells[-1] <= 8.0e6
+    alone = []
+    for mm, sc in zip(models, reqs):
+        inst = gpu_for(mm)
+        inst.update(-1000.0)
+        alone.append((inst.get("zs"), inst.time()[2]))
+        inst.close()
+
+    class Prepared:
+        def __init__(self, model, t_end):
+            self.model, self.t_end = model, t_end
+
+        def build_model(self):
+            return self.model
+
+    srv = E.Server(hbm_budget=1 << 36, round_steps=16, workers=2, keep_state=True)
+    for k, (mm, sc) in enumerate(zip(models, reqs)):
+        srv.submit(0.05 * k, Prepared(mm, sc.t_end))
+    out = srv.run()
+    assert out["requests"] == len(reqs) and out["max_concurrent"] >= 2
+    by_seq = {res["seq"]: res for res in out["results"]}
+    for k, (zs, nt) in enumerate(alone):
+        assert by_seq[k]["steps"] == nt
+        assert np.max(np.abs(by_seq[k]["zs"] - zs)) <= ZS_TOL

@@ -1,0 +1,124 @@
+"""GPU parity of branches the benchmark configurations do not reach, against the oracle:
+the h^(7/3) table (sfincs_momentum.f90:787-820), point sources on regular and quadtree
+meshes with the bed clamp of the regular branch (sfincs_continuity.f90:87-106), Neumann
+side boundaries (msk = 6, sfincs_boundaries.f90:903-913), boundary spin-up
+(:789-826 with weighted_average :1179-1200), a four-level quadtree with wind, and the
+instability stop (sfincs_lib.f90:634-648).
+
+Gate: identical step count and model time, max|dzs| <= 1e-4 m, max|dq| <= 1e-5 m2/s."""
+import numpy as np
+import pytest
+
+from helpers import Q_TOL, ZS_TOL, compare_state, compare_subgrid_state, gpu_for, oracle_for
+from sfincs_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _full_run(sc):
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    if m.sizes()["subgrid"]:
+        d.update(compare_subgrid_state(inst, orc))
+    print(sc.name, st["nt"], int(orc.get("nt")), d, flush=True)
+    assert rc == 0 and st["nt"] == int(orc.get("nt")) and abs(st["t"] - orc.get("t")) < 1e-6
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL, d
+    if m.sizes()["subgrid"]:
+        assert d["z_volume_rel"] <= 1e-7, d
+    return m, inst, orc
+
+
+def _h73_regular():
+    return W.regular_case(40, 48, dx=50.0, seed=61, t_end=1800.0, tide_amp=2.0, h73table=1)
+
+
+def _h73_subgrid():
+    return W.subgrid_case(30, 36, dx=25.0, seed=62, t_end=1800.0, nsrc=2, rain_mmh=30.0, h73table=1)
+
+
+@pytest.mark.parametrize("make", [_h73_regular, _h73_subgrid], ids=["regular", "subgrid"])
+def test_h73_table_matches_oracle(make):
+    print("case h73table", make.__name__, flush=True)
+    _full_run(make())
+
+
+def test_regular_sources_match_oracle():
+    """Four sources on a regular beach, two of them draining dry or drying cells (the
+    clamp to the bed acts), and the tide."""
+    print("case regular sources", flush=True)
+    sc = W.regular_sources(W.regular_case(40, 52, dx=50.0, seed=28, t_end=3600.0, tide_amp=1.5, rain_mmh=0.0))
+    m, inst, orc = _full_run(sc)
+    cells = m.table("nmindsrc") - 1
+    zs, zb = inst.get("zs"), m.table("zb")
+    assert zs[cells[1]] == zb[cells[1]]          # the drained dry cell sits on its bed
+    assert zs[cells[0]] > zb[cells[0]] + 0.01
+
+
+def test_quadtree_sources_match_oracle():
+    print("case quadtree sources", flush=True)
+    sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=29, t_end=3600.0, surge_amp=3.0)
+    W.with_sources(sc, [350.0, 1020.0, 610.0], [420.0, 530.0, 880.0], [3.0, -2.0, 1.5])
+    _full_run(sc)
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+def test_neumann_side_boundaries_match_oracle(kind):
+    """The north row of a tidal domain is a Neumann boundary (msk = 6): its cells copy the
+    level of the cell inside and their uv-points are solved by momentum."""
+    print("case neumann", kind, flush=True)
+    if kind == "regular":
+        sc = W.regular_case(36, 44, dx=50.0, seed=63, t_end=1800.0, tide_amp=2.0)
+        nmax, mmax = 36, 44
+    else:
+        sc = W.subgrid_case(30, 34, dx=25.0, seed=64, t_end=1800.0, nsrc=1, tide_amp=2.0)
+        nmax, mmax = 30, 34
+    msk = sc.msk.reshape(mmax, nmax).copy()
+    msk[1:, nmax - 1] = 6
+    sc.msk = msk.reshape(-1)
+    m, inst, orc = _full_run(sc)
+    assert np.count_nonzero(m.table("kcuv") == 6) > 0
+    assert np.count_nonzero(m.table("kcs") == 6) == mmax - 1
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+def test_boundary_spinup_matches_oracle(kind):
+    """tspinup = 900 s with zini = 0.3 m: the boundary level blends from zini to the tide."""
+    print("case spin-up", kind, flush=True)
+    if kind == "regular":
+        sc = W.regular_case(36, 40, dx=50.0, seed=65, t_end=1800.0, tide_amp=2.0, tspinup=900.0, zsini=0.3)
+    else:
+        sc = W.subgrid_case(30, 32, dx=25.0, seed=66, t_end=1800.0, nsrc=1, tide_amp=2.0, tspinup=900.0,
+                            zsini=0.3)
+    _full_run(sc)
+
+
+def test_four_level_quadtree_with_wind_matches_oracle():
+    """The configuration-2 recipe (4 levels, surge, 20 m/s SW wind) on a small base grid;
+    wind enters through cosf/sinf, so the gate is the tolerance, not bit equality."""
+    print("case quadtree 4 levels", flush=True)
+    sc = W.quadtree_case(12, 16, nlev=4, dx0=100.0, seed=67, t_end=1800.0, surge_amp=3.0)
+    m, inst, orc = _full_run(sc)
+    assert m.sizes()["nref"] == 4 and m.sizes()["ncuv"] > 0
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+def test_instability_stop_matches_oracle(kind):
+    """uvmax = 0.5 m/s gives dtmin = alpha dx / (1.25 uvmax) far above the CFL step: the run
+    stops with status 1 at the same step as the oracle."""
+    print("case instability stop", kind, flush=True)
+    if kind == "regular":
+        sc = W.regular_case(24, 30, dx=50.0, seed=68, t_end=1800.0, uvmax=0.5)
+    else:
+        sc = W.subgrid_case(24, 30, dx=25.0, seed=69, t_end=1800.0, nsrc=1, uvmax=0.5)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    print(kind, rc, st["nt"], st["error"], int(orc.get("nt")), int(orc.get("error")), flush=True)
+    assert int(orc.get("error")) == 1
+    assert rc == 1 and st["error"] == 1 and st["nt"] == int(orc.get("nt"))
+    d = compare_state(inst, orc)
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL, d

@@ -147,7 +147,7 @@ def whole_period(model, tuning, t_end, twin=True):
     from sfincs_amd import _lib
     a = _lib.Instance(model)
     a.pin_tuning(tuning)
-    v0 = a.total_volume()
+    b0 = a.balance()
     torch.cuda.synchronize()
     t_a = time.perf_counter()
     rc, st = a.update(float(t_end))
@@ -155,10 +155,19 @@ def whole_period(model, tuning, t_end, twin=True):
     secs = time.perf_counter() - t_a
     zs = a.get("zs")
     assert np.all(np.isfinite(zs)), "non-finite water levels"
+    b1 = a.balance()
     np_ = model.sizes()["np"]
+    kcs = model.table("kcs")
+    water = float(np.sum(a.get("z_volume")[kcs == 1])) if model.sizes()["subgrid"] else None
+    dv = b1["volume"] - b0["volume"] + (b0["storage"] - b1["storage"])
+    inflow = {k: b1[k] - b0[k] for k in ("boundary", "sources", "rain", "qext")}
     out = {"model_time_s": st["t"], "steps": st["nt"], "seconds": secs,
-           "cell_updates_per_s": np_ * st["nt"] / secs, "ms_per_step": secs / max(1, st["nt"]) * 1e3,
-           "rc": rc, "volume_start_m3": v0, "volume_end_m3": a.total_volume()}
+           "cell_updates_per_s": np_ * st["nt"] / secs, "ms_per_step": secs / max(1, st["nt"]) * 1e3, "rc": rc,
+           "volume_balance": {"interior_volume_start_m3": b0["volume"], "interior_volume_end_m3": b1["volume"],
+                              "inflow_m3": inflow, "abs_error_m3": abs(dv - sum(inflow.values())),
+                              "rel_error": abs(dv - sum(inflow.values())) / water if water else None,
+                              "note": "FP64 accumulators on the device: |dV - (boundary + sources + rain + qext)| "
+                                      "over the interior cells, relative to their water volume at the end"}}
     a.close()
     if twin:
         b = _lib.Instance(model, fp64=True)

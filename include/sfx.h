@@ -239,6 +239,10 @@ int  sfx_update_forcing(sfx_handle* h, double t, float dt);
 /* sfincs_update(dtrange) (sfincs_lib.f90:337-683): dtrange < -999 runs to t1, dtrange <= 0
  * one step, dtrange > 0 runs to t + dtrange. Device-resident loop, graph-captured. */
 int  sfx_update(sfx_handle* h, double dtrange, sfx_step_stats* st);
+/* Steps while t < t_end (an absolute model time): the form a host uses to stop at the
+ * first step whose time reaches an output time, as the reference's output triggers fire
+ * (sfincs_lib.f90:419-489). Nothing is done when t >= t_end already. */
+int  sfx_update_to(sfx_handle* h, double t_end, sfx_step_stats* st);
 int  sfx_get_time(sfx_handle* h, double* t, float* dt, int64_t* nt);
 /* Per-branch uv-point counters of the sweeps since counting was enabled by
  * sfx_set_counting(h,1): {copied (kcuv not 1/6, and the absent slots of a dense grid),

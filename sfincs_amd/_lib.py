@@ -94,6 +94,7 @@ def _sig(lib):
         "sfx_compute_water_levels": (I, [P, d, f]),
         "sfx_update_forcing": (I, [P, d, f]),
         "sfx_update": (I, [P, d, C.POINTER(sfx_step_stats)]),
+        "sfx_update_to": (I, [P, d, C.POINTER(sfx_step_stats)]),
         "sfx_get_time": (I, [P, C.POINTER(d), C.POINTER(f), C.POINTER(C.c_int64)]),
         "sfx_set_counting": (I, [P, i32]),
         "sfx_get_counters": (I, [P, C.POINTER(C.c_int64)]),
@@ -391,6 +392,12 @@ class Instance:
     def update(self, dtrange=-1000.0):
         st = sfx_step_stats()
         rc = check(lib().sfx_update(self.h, dtrange, C.byref(st)), "update")
+        return rc, {k: getattr(st, k) for k, _ in sfx_step_stats._fields_}
+
+    def update_to(self, t_end):
+        """Steps while t < t_end (absolute model time)."""
+        st = sfx_step_stats()
+        rc = check(lib().sfx_update_to(self.h, t_end, C.byref(st)), "update_to")
         return rc, {k: getattr(st, k) for k, _ in sfx_step_stats._fields_}
 
     def time(self):

@@ -2595,8 +2595,10 @@ extern "C" int sfx_update_forcing(sfx_handle* H, double t, float dt) {
   return SFX_OK;
 }
 
-extern "C" int sfx_update(sfx_handle* H, double dtrange, sfx_step_stats* st) {
-  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+namespace {
+// The step loop of sfincs_update (sfincs_lib.f90:379-648): one step (single), or steps
+// while t < tend, in graph launches of 32 steps whose steps past tend do nothing.
+int update_loop(sfx_handle* H, int mode, double arg, sfx_step_stats* st) {
   HIPCHK(hipSetDevice(H->device));
   auto t_start = std::chrono::steady_clock::now();
   Scalars s;
@@ -2605,9 +2607,10 @@ extern "C" int sfx_update(sfx_handle* H, double dtrange, sfx_step_stats* st) {
   if (s.error_pending) { s.error = 1; s.error_pending = 0; s.t = (double)H->hI.c.t1 + 1.0; }
   const int64_t nt0 = s.nt;
   bool single = false;
-  if (dtrange < -999.0) s.tend = (double)H->hI.c.t1;
-  else if (dtrange <= 0.0) { s.tend = (double)H->hI.c.t1; single = true; }
-  else s.tend = s.t + dtrange;
+  if (mode == 1) s.tend = arg;                          // absolute end time
+  else if (arg < -999.0) s.tend = (double)H->hI.c.t1;
+  else if (arg <= 0.0) { s.tend = (double)H->hI.c.t1; single = true; }
+  else s.tend = s.t + arg;
   if ((rc = write_scalars(H, s))) return rc;
   sfx_batch* B = solo(H);
   if (B->dirty && (rc = rebuild(B))) return rc;
@@ -2631,6 +2634,17 @@ extern "C" int sfx_update(sfx_handle* H, double dtrange, sfx_step_stats* st) {
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   }
   return s.error ? SFX_UNSTABLE : SFX_OK;
+}
+}  // namespace
+
+extern "C" int sfx_update(sfx_handle* H, double dtrange, sfx_step_stats* st) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  return update_loop(H, 0, dtrange, st);
+}
+
+extern "C" int sfx_update_to(sfx_handle* H, double t_end, sfx_step_stats* st) {
+  if (!H) { set_error("null"); return SFX_ERR_ARG; }
+  return update_loop(H, 1, t_end, st);
 }
 
 extern "C" int sfx_set_counting(sfx_handle* H, int32_t on) {

@@ -3,7 +3,8 @@
 sfincs.inp (key = value), inputformat = bin: indexfile (int32 np, then the column-major
 indices (m-1)*nmax+n of the active cells), mskfile (int8 per index), depfile (float32 per
 index) -- sfincs_domain.f90:244-260, 1670-1676; quadtree meshes as the legacy binary
-qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile; ASCII bnd/bzs, src/dis,
+qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile; the legacy binary sbgfile
+(sfincs_subgrid.F90:458-903) for subgrid scenarios; ASCII bnd/bzs, src/dis,
 prc and wnd files (sfincs_boundaries.f90:8-140, sfincs_discharges.f90:36-126,
 sfincs_meteo.f90:247-312). Values are written with 9 significant digits so that
 float32 round-trips exactly.
@@ -23,6 +24,24 @@ def _fmt(v):
 def _date(seconds):
     d = datetime.datetime(2000, 1, 1) + datetime.timedelta(seconds=float(seconds))
     return d.strftime("%Y%m%d %H%M%S")
+
+
+def _write_subgrid(sc, folder):
+    """The scenario's legacy binary subgrid file as sfincs.sbg. Only scenarios whose tables
+    come from such a file (workloads.with_subgrid_binary) can be written: NetCDF subgrid
+    files need a library this image does not have, and the two formats do not describe the
+    same tables (huthresh, pwet and the fit differ)."""
+    import shutil
+    from . import _lib
+    p = _lib.default_params(**sc.params)
+    if sc.kind == "regular":
+        mesh = _lib.Model.regular(p, sc.nmax, sc.mmax, sc.dx, sc.dy, sc.msk)
+    else:
+        mesh = _lib.Model.quadtree(p, sc.qt)
+    sg = sc.subgrid(mesh)
+    if not isinstance(sg, str):
+        raise ValueError("deck.write: subgrid tables must come from a legacy binary file (with_subgrid_binary)")
+    shutil.copyfile(sg, os.path.join(folder, "sfincs.sbg"))
 
 
 def write(sc, folder):
@@ -50,10 +69,15 @@ def write(sc, folder):
             lines.append(f"timestep_analysis = {int(v)}")
         else:
             lines.append(f"{k} = {_fmt(v)}")
+    if sc.subgrid is not None:
+        _write_subgrid(sc, folder)
+        lines.append("sbgfile = sfincs.sbg")
     if sc.kind == "regular":
         lines += [f"nmax = {sc.nmax}", f"mmax = {sc.mmax}", f"dx = {_fmt(sc.dx)}", f"dy = {_fmt(sc.dy)}",
                   "x0 = 0", "y0 = 0", "rotation = 0", "inputformat = bin", "indexfile = sfincs.ind",
-                  "mskfile = sfincs.msk", "depfile = sfincs.dep"]
+                  "mskfile = sfincs.msk"]
+        if sc.subgrid is None:
+            lines.append("depfile = sfincs.dep")
         msk = sc.msk.reshape(sc.mmax, sc.nmax)          # [m][n]
         mm, nn = np.nonzero(msk > 0)
         order = np.lexsort((nn, mm))
@@ -64,8 +88,9 @@ def write(sc, folder):
             f.write(ind.tobytes())
         with open(os.path.join(folder, "sfincs.msk"), "wb") as f:
             f.write(msk[mm, nn].astype(np.int8).tobytes())
-        with open(os.path.join(folder, "sfincs.dep"), "wb") as f:
-            f.write(sc.zb_grid[mm, nn].astype(np.float32).tobytes())
+        if sc.subgrid is None:
+            with open(os.path.join(folder, "sfincs.dep"), "wb") as f:
+                f.write(sc.zb_grid[mm, nn].astype(np.float32).tobytes())
     else:
         qt = sc.qt
         lines += ["qtrfile = sfincs.qtr", "mskfile = sfincs.msk"]

@@ -107,6 +107,15 @@ module sfx_capi
          integer(c_int32_t), value :: nt
          type(c_ptr), value :: t, mag, dir
       end function
+      integer(c_int) function sfx_model_read_subgrid_binary(m, path) bind(C, name='sfx_model_read_subgrid_binary')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function sfx_update_zbuvmx(h) bind(C, name='sfx_update_zbuvmx')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
       integer(c_int) function sfx_model_initialize(m) bind(C, name='sfx_model_initialize')
          import :: c_int, c_ptr
          type(c_ptr), value :: m
@@ -135,6 +144,17 @@ module sfx_capi
          type(c_ptr), value :: h
          real(c_double), value :: dtrange
          type(sfx_step_stats), intent(out) :: st
+      end function
+      integer(c_int) function sfx_update_to(h, tend, st) bind(C, name='sfx_update_to')
+         import :: c_int, c_ptr, c_double, sfx_step_stats
+         type(c_ptr), value :: h
+         real(c_double), value :: tend
+         type(sfx_step_stats), intent(out) :: st
+      end function
+      integer(c_int) function sfx_write_restart(h, path) bind(C, name='sfx_write_restart')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: path(*)
       end function
       integer(c_int) function sfx_get_state(h, name, dst, nbytes) bind(C, name='sfx_get_state')
          import :: c_int, c_ptr, c_char, c_size_t
@@ -187,6 +207,7 @@ module sfincs_lib
    public :: host_np, host_npuv, host_ncuv, host_zs, host_zsmax, host_zb, host_sync, host_t0, host_t1
    public :: host_xz, host_yz, host_zmin, host_area, host_iref, host_mesh_fields, host_last_error, host_find_cell
    public :: host_qext_requested, host_qext_on, host_qext, host_qext_switch
+   public :: host_expose, host_update_zbuv, host_zn, host_zm
 
    real*8, target :: t = 0.0d0
    real*4, target :: dt = 0.0
@@ -200,12 +221,31 @@ module sfincs_lib
    real*4, allocatable, target :: host_qext(:)
    real*8, allocatable, target :: host_zs(:)
    real*4, allocatable, target :: host_zsmax(:), host_zb(:)
+   ! get_value_ptr hands out host copies of zs and zb, which a coupler may write, as it
+   ! writes the reference's own arrays (sfincs_bmi.f90:95-129): once handed out, a copy is
+   ! compared with its shadow before each update and uploaded when it changed, and zs is
+   ! copied back after each update so that the pointer keeps showing the model's levels
+   logical :: host_zs_out = .false., host_zb_out = .false.
+   real*8, allocatable :: shadow_zs(:)
+   real*4, allocatable :: shadow_zb(:)
+   integer(c_int32_t), allocatable :: host_zn(:), host_zm(:)
    ! mesh fields for the BMI (cell centres, bed or lowest subgrid level, cell areas)
    real*4, allocatable, target :: host_xz(:), host_yz(:), host_zmin(:), host_area(:)
    integer(c_int8_t), allocatable, target :: host_iref(:)
    type(c_ptr) :: model = c_null_ptr
    type(c_ptr) :: handle = c_null_ptr
    integer :: logu = 0
+   ! scheduled output (sfincs_output.f90:8-60, 80-258; triggers sfincs_lib.f90:419-489):
+   ! binary map (zs.dat, z_volume.dat), maximum (zsmax.dat, velmax.dat, tmax.dat, qmax.dat)
+   ! and restart files; NetCDF output would need a library this host does not link
+   real*8 :: tmapout = 1.0d9, tmaxout = 1.0d9, trstout = 1.0d9
+   real*8 :: out_dtmap = 0.0d0, out_dtmax = 0.0d0, out_dtrst = 0.0d0, out_trst = -999.0d0, out_t0 = 0.0d0
+   real*8 :: host_tref_sec = 0.0d0
+   integer :: ntmapout = 0, ntmaxout = 0
+   integer :: u_zs = 0, u_zv = 0, u_zsmax = 0, u_vmax = 0, u_twet = 0, u_qmax = 0
+   logical :: out_zvolume = .false., host_subgrid = .false.
+   integer :: host_store_zsmax = 0
+   real*4 :: host_huthresh = 0.0
 
 contains
 
@@ -311,6 +351,21 @@ contains
       date_seconds = dble(jdn) * 86400.0d0 + dble(h) * 3600.0d0 + dble(mi) * 60.0d0 + dble(se)
    end function
 
+   ! NetCDF files start with 'CDF' (classic) or the HDF5 signature (NetCDF-4).
+   logical function is_netcdf(fname)
+      character(*), intent(in) :: fname
+      character(4) :: magic
+      integer :: u, ios
+      is_netcdf = .false.
+      magic = '    '
+      open(newunit=u, file=trim(fname), form='unformatted', access='stream', status='old', action='read', iostat=ios)
+      if (ios /= 0) return
+      read(u, iostat=ios) magic
+      close(u)
+      if (ios /= 0) return
+      is_netcdf = magic(1:3) == 'CDF' .or. magic(2:4) == 'HDF'
+   end function
+
    integer function count_lines(fname)
       character(*), intent(in) :: fname
       character(1024) :: line
@@ -345,7 +400,10 @@ contains
          host_qext_on = .true.
       endif
       call table_size('uv_index_z_nm', 4, host_npuv)
+      if (allocated(host_zs)) deallocate(host_zs, host_zsmax)
       allocate(host_zs(host_np), host_zsmax(host_np))
+      host_zs_out = .false.
+      host_zb_out = .false.
       ! the replacement of initialize_openacc: tables and state to HBM once
       rc = sfx_create(model, 0_c_int32_t, handle)
       if (rc < 0) then
@@ -355,6 +413,7 @@ contains
       endif
       t = host_t0
       dt = 1.0e-6
+      call output_init()
       call logmsg('---------- Starting simulation -----------')
       ierr = 0
    end function
@@ -367,6 +426,7 @@ contains
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout, tdum
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
       character(256) :: bndfile, bzsfile, bzifile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
+      character(256) :: sbgfile
       integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
       integer(c_int32_t), allocatable :: indices(:)
       real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:), zsi(:,:)
@@ -419,6 +479,32 @@ contains
       call inp_int('storetwet', i); p%store_twet = merge(1, 0, i == 1)
       call inp_real('twet_threshold', p%twet_threshold)
       call inp_flag('timestep_analysis', p%timestep_analysis)
+      ! output times (sfincs_input.f90:61-66, 341-343, 441-444, 577-582)
+      out_dtmap = 0.0d0
+      tdum = 0.0
+      call inp_real('dtout', tdum)
+      if (tdum == 0.0) call inp_real('dtmapout', tdum)
+      out_dtmap = tdum
+      out_dtmax = dtmaxout
+      tdum = 0.0
+      call inp_real('dtrstout', tdum)
+      out_dtrst = tdum
+      tdum = -999.0
+      call inp_real('trstout', tdum)
+      out_trst = tdum
+      tdum = -999.0
+      call inp_real('t0out', tdum)
+      out_t0 = tdum
+      if (out_t0 < -900.0d0) out_t0 = p%t0
+      out_t0 = max(out_t0, dble(p%t0))
+      i = 0
+      call inp_int('storezvolume', i)
+      out_zvolume = i == 1
+      host_tref_sec = date_seconds(tref)
+      host_store_zsmax = p%store_zsmax
+      call inp_str('outputformat', s, 'net')
+      if (s(1:3) == 'net' .and. (out_dtmap > 1.0d-6 .or. out_dtmax > 1.0d-6)) &
+         call logmsg('Info    : NetCDF output is not available in this host; map and maximum output are binary')
       host_flags = [p%store_velmax, p%store_fluxmax, p%store_twet, p%timestep_analysis]
       if (host_qext_requested) p%use_qext = 1
       host_t0 = p%t0; host_t1 = p%t1
@@ -434,9 +520,10 @@ contains
       call inp_str('prcfile', prcfile, 'none'); call inp_str('wndfile', wndfile, 'none')
       call inp_str('rstfile', rstfile, 'none'); call inp_str('bdrfile', bdrfile, 'none')
       call inp_str('bzifile', bzifile, 'none')
+      call inp_str('sbgfile', sbgfile, 'none')
 
       if (qtrfile(1:4) /= 'none') then
-         call read_quadtree(qtrfile, mskfile, p, rc)
+         call read_quadtree(qtrfile, mskfile, p, sbgfile(1:4) == 'none', rc)
          if (rc /= 0) then
             ierr = -1
             return
@@ -481,7 +568,9 @@ contains
          endif
          call table_size('kcs', 1, host_np)
          allocate(zb(host_np))
-         if (inputformat(1:3) == 'asc') then
+         if (sbgfile(1:4) /= 'none') then
+            ! subgrid tables replace the bed level (initialize_bathymetry, sfincs_domain.f90:1588-1640)
+         else if (inputformat(1:3) == 'asc') then
             allocate(zbg(nmax, mmax))
             open(newunit=u, file=trim(depfile), status='old', action='read')
             do n = 1, nmax
@@ -502,9 +591,28 @@ contains
             read(u) zb
             close(u)
          endif
-         rc = sfx_model_set_bathymetry(model, c_loc(zb))
+         if (sbgfile(1:4) == 'none') then
+            rc = sfx_model_set_bathymetry(model, c_loc(zb))
+            if (rc < 0) then
+               call fail('bathymetry')
+               ierr = -1
+               return
+            endif
+         endif
+      endif
+      ! subgrid tables (read_subgrid_file, sfincs_subgrid.F90:20-53): the legacy binary
+      ! format; a NetCDF file needs the NetCDF library, which this host does not link
+      if (sbgfile(1:4) /= 'none') then
+         if (is_netcdf(sbgfile)) then
+            call logmsg('Error   : '//trim(sbgfile)//' is a NetCDF subgrid file; this host reads the binary '// &
+                        'subgrid format only')
+            ierr = -1
+            return
+         endif
+         call logmsg('Info    : reading subgrid file '//trim(sbgfile))
+         rc = sfx_model_read_subgrid_binary(model, trim(sbgfile)//c_null_char)
          if (rc < 0) then
-            call fail('bathymetry')
+            call fail('subgrid file')
             ierr = -1
             return
          endif
@@ -629,6 +737,11 @@ contains
          ierr = -1
          return
       endif
+      nbytes = int(c_sizeof(p), c_size_t)
+      rc = sfx_model_get_table(model, 'params'//c_null_char, c_loc(p), nbytes)
+      host_huthresh = p%huthresh           ! 0 once NetCDF-style subgrid tables are set
+      host_subgrid = sbgfile(1:4) /= 'none'
+      out_zvolume = out_zvolume .and. host_subgrid
       ierr = 0
    end function
 
@@ -685,7 +798,16 @@ contains
       call table_size('z_zmin', 4, n); allocate(host_zmin(n)); if (n > 0) call get_f32('z_zmin', host_zmin)
       call table_size('cell_area', 4, n); allocate(host_area(n)); call get_f32('cell_area', host_area)
       n = sfx_model_get_table(model, 'z_flags_iref'//c_null_char, c_loc(host_iref), int(size(host_iref), c_size_t))
+      if (allocated(host_zn)) deallocate(host_zn, host_zm)
+      allocate(host_zn(host_np), host_zm(host_np))
+      call get_i32('z_index_z_n', host_zn); call get_i32('z_index_z_m', host_zm)
    contains
+      subroutine get_i32(name, a)
+         character(*), intent(in) :: name
+         integer(c_int32_t), target, intent(inout) :: a(:)
+         integer :: rc
+         rc = sfx_model_get_table(model, trim(name)//c_null_char, c_loc(a), int(4 * size(a), c_size_t))
+      end subroutine
       subroutine get_f32(name, a)
          character(*), intent(in) :: name
          real*4, target, intent(inout) :: a(:)
@@ -704,10 +826,12 @@ contains
       n = int(nb / esize)
    end subroutine
 
-   subroutine read_quadtree(qtrfile, mskfile, p, rc)
-      ! legacy binary quadtree file (sfincs_quadtree.F90:210-292) + binary mask file
+   subroutine read_quadtree(qtrfile, mskfile, p, set_zb, rc)
+      ! legacy binary quadtree file (sfincs_quadtree.F90:210-292) + binary mask file; the
+      ! bed level from the file unless subgrid tables replace it
       character(256), intent(in) :: qtrfile, mskfile
       type(sfx_params), intent(in) :: p
+      logical, intent(in) :: set_zb
       integer, intent(out) :: rc
       type(sfx_quadtree_input) :: q
       integer(c_int8_t) :: iversion, nlev
@@ -737,6 +861,7 @@ contains
          call fail('quadtree mesh')
          return
       endif
+      if (.not. set_zb) return
       ! bed level from the quadtree file for active points (sfincs_domain.f90:1635-1637)
       allocate(zb(count(msk > 0)))
       j = 0
@@ -757,6 +882,7 @@ contains
       integer :: ierr
       type(sfx_step_stats) :: st
       integer(c_int) :: rc
+      real*8 :: tend, tnext
       if (host_qext_on) then
          rc = sfx_set_state(handle, 'qext'//c_null_char, c_loc(host_qext), int(4 * host_np, c_size_t))
          if (rc < 0) then
@@ -765,10 +891,32 @@ contains
             return
          endif
       endif
-      rc = sfx_update(handle, real(dtrange, c_double), st)
-      t = st%t
-      dt = st%dt
+      ierr = host_upload_exposed()
+      if (ierr /= 0) return
+      if (dtrange < -999.0d0 .or. dtrange > 0.0d0) then
+         ! run to the end of the interval, stopping at each output time on the way: the
+         ! device stops at the first step whose time reaches it, the step at which the
+         ! reference's trigger fires
+         tend = host_t1
+         if (dtrange > 0.0d0) tend = t + dtrange
+         rc = 0
+         do while (t < tend)
+            tnext = min(tend, tmapout, tmaxout, trstout)
+            if (tnext <= t) then
+               rc = sfx_update(handle, 0.0d0, st)      ! an output is due: one step
+            else
+               rc = sfx_update_to(handle, real(tnext, c_double), st)
+            endif
+            if (rc < 0) exit
+            call after_steps(st, rc)
+            if (rc /= 0) exit
+         enddo
+      else
+         rc = sfx_update(handle, 0.0d0, st)             ! one step (BMI update)
+         if (rc >= 0) call after_steps(st, rc)
+      endif
       ierr = 0
+      if (host_zs_out) call host_sync()
       if (rc < 0) then
          call fail('update')
          ierr = -1
@@ -783,7 +931,66 @@ contains
       integer(c_int) :: rc
       rc = sfx_get_state(handle, 'zs'//c_null_char, c_loc(host_zs), int(8 * host_np, c_size_t))
       rc = sfx_get_state(handle, 'zsmax'//c_null_char, c_loc(host_zsmax), int(4 * host_np, c_size_t))
+      if (host_zs_out) shadow_zs = host_zs
    end subroutine
+
+   ! get_value_ptr hands out zs or zb: from then on their host copies are live for the coupler
+   subroutine host_expose(name)
+      character(*), intent(in) :: name
+      select case (name)
+      case ('zs')
+         host_zs_out = .true.
+         call host_sync()
+      case ('zb')
+         if (.not. host_zb_out .and. size(host_zb) > 0) then
+            host_zb_out = .true.
+            shadow_zb = host_zb
+         endif
+      end select
+   end subroutine
+
+   ! Uploads the copies a coupler changed since they were handed out or last uploaded.
+   integer function host_upload_exposed() result(ierr)
+      integer(c_int) :: rc
+      ierr = 0
+      if (host_zs_out) then
+         if (any(host_zs /= shadow_zs)) then
+            rc = sfx_set_state(handle, 'zs'//c_null_char, c_loc(host_zs), int(8 * host_np, c_size_t))
+            if (rc < 0) then
+               call fail('zs upload')
+               ierr = -1
+               return
+            endif
+            shadow_zs = host_zs
+         endif
+      endif
+      if (host_zb_out) then
+         if (any(host_zb /= shadow_zb)) then
+            rc = sfx_set_state(handle, 'zb'//c_null_char, c_loc(host_zb), int(4 * size(host_zb), c_size_t))
+            if (rc < 0) then
+               call fail('zb upload')
+               ierr = -1
+               return
+            endif
+            shadow_zb = host_zb
+         endif
+      endif
+   end function
+
+   ! update_zbuv (sfincs_bmi.f90:277-284): the bed written by the coupler reaches the device,
+   ! then zbuvmx is derived again from it (compute_zbuvmx, sfincs_domain.f90:1746-1762)
+   integer function host_update_zbuv() result(ierr)
+      integer(c_int) :: rc
+      ierr = -1
+      if (.not. c_associated(handle)) return
+      ierr = host_upload_exposed()
+      if (ierr /= 0) return
+      rc = sfx_update_zbuvmx(handle)
+      if (rc < 0) then
+         call fail('update_zbuv')
+         ierr = -1
+      endif
+   end function
 
    ! A per-cell f32 state field of the device to a raw file (active cells, reference order).
    subroutine write_cell_f32(name, n, fname)
@@ -840,24 +1047,197 @@ contains
       deallocate(tsum, twet, tlim, dtc, pct)
    end subroutine
 
+   ! ---- scheduled output ------------------------------------------------------------
+   ! initialize_output (sfincs_output.f90:8-60) with the binary writers' files (:314-373)
+   subroutine output_init()
+      ntmapout = 0
+      ntmaxout = 0
+      call output_close()
+      if (out_dtmap > 1.0d-6) then
+         tmapout = out_t0
+         open(newunit=u_zs, file='zs.dat', form='unformatted', status='replace')
+         if (out_zvolume) open(newunit=u_zv, file='z_volume.dat', form='unformatted', status='replace')
+      else
+         tmapout = 1.0d9
+      endif
+      if (out_dtmax > 1.0d-6) then
+         tmaxout = out_t0 + out_dtmax
+         if (host_store_zsmax == 1) open(newunit=u_zsmax, file='zsmax.dat', form='unformatted', status='replace')
+         if (host_flags(1) == 1) open(newunit=u_vmax, file='velmax.dat', form='unformatted', status='replace')
+         if (host_flags(3) == 1) open(newunit=u_twet, file='tmax.dat', form='unformatted', status='replace')
+         if (host_flags(2) == 1) open(newunit=u_qmax, file='qmax.dat', form='unformatted', status='replace')
+      else
+         tmaxout = 1.0d9
+      endif
+      if (out_dtrst > 1.0d-6) then
+         trstout = out_t0 + out_dtrst
+      else if (out_trst > 1.0d-6) then
+         trstout = out_trst
+      else
+         trstout = 1.0d9
+      endif
+   end subroutine
+
+   subroutine output_close()
+      if (u_zs /= 0) close(u_zs)
+      if (u_zv /= 0) close(u_zv)
+      if (u_zsmax /= 0) close(u_zsmax)
+      if (u_vmax /= 0) close(u_vmax)
+      if (u_twet /= 0) close(u_twet)
+      if (u_qmax /= 0) close(u_qmax)
+      u_zs = 0; u_zv = 0; u_zsmax = 0; u_vmax = 0; u_twet = 0; u_qmax = 0
+   end subroutine
+
+   ! After the device stopped: the output triggers of the step it stopped at
+   ! (sfincs_lib.f90:419-489), the output (:624-630) and, at an instability stop, the last
+   ! map and maximum output (:634-648).
+   subroutine after_steps(st, rc)
+      type(sfx_step_stats), intent(in) :: st
+      integer(c_int), intent(in) :: rc
+      logical :: wmap, wmax, wrst
+      real*8 :: tout
+      t = st%t
+      dt = st%dt
+      if (rc == 1 .or. st%error /= 0) then
+         ntmaxout = ntmaxout + 1
+         call write_output(t, u_zs /= 0, .true., .false.)
+         return
+      endif
+      wmap = .false.; wmax = .false.; wrst = .false.
+      tout = t
+      if (t >= tmapout) then
+         wmap = .true.
+         ntmapout = ntmapout + 1
+         tout = max(tmapout, t - dt)
+         tmapout = tmapout + out_dtmap
+      endif
+      if (t >= tmaxout) then
+         wmax = .true.
+         ntmaxout = ntmaxout + 1
+         tout = max(tmaxout, t - dt)
+         if (t < host_t1) tmaxout = tmaxout + out_dtmax
+      endif
+      if (t >= trstout) then
+         wrst = .true.
+         tout = trstout
+         if (out_dtrst > 1.0d-6) then
+            trstout = trstout + out_dtrst
+         else
+            trstout = 1.0d9
+         endif
+      endif
+      if (wmap .or. wmax .or. wrst) call write_output(tout, wmap, wmax, wrst)
+   end subroutine
+
+   ! write_output (sfincs_output.f90:80-258), binary flavour: the fields come from the
+   ! device first (the form of `acc update host`); after a maximum output the maxima start
+   ! again from -999 (twet from 0) on the device, as the reference resets them
+   subroutine write_output(tout, wmap, wmax, wrst)
+      real*8, intent(in) :: tout
+      logical, intent(in) :: wmap, wmax, wrst
+      real*8, allocatable, target :: v8(:)
+      real*4, allocatable, target :: f4(:)
+      integer(c_int) :: rc
+      integer :: nm
+      character(15) :: ts
+      if (wmap .and. u_zs /= 0) then
+         call host_sync()
+         write(u_zs) host_zs
+         if (u_zv /= 0) then
+            allocate(v8(host_np))
+            rc = sfx_get_state(handle, 'z_volume'//c_null_char, c_loc(v8), int(8 * host_np, c_size_t))
+            write(u_zv) v8
+            deallocate(v8)
+         endif
+      endif
+      if (wmax .and. out_dtmax > 0.0d0) then
+         allocate(f4(host_np))
+         if (u_zsmax /= 0) then
+            rc = sfx_get_state(handle, 'zsmax'//c_null_char, c_loc(f4), int(4 * host_np, c_size_t))
+            ! dry cells: -999 (write_max_output, sfincs_output.f90:466-487)
+            if (host_subgrid) then
+               do nm = 1, host_np
+                  if (f4(nm) < host_zmin(nm) + host_huthresh) f4(nm) = -999.0
+               enddo
+            else
+               do nm = 1, host_np
+                  if (f4(nm) < host_zb(nm) + host_huthresh) f4(nm) = -999.0
+               enddo
+            endif
+            write(u_zsmax) f4
+            f4 = -999.0
+            rc = sfx_set_state(handle, 'zsmax'//c_null_char, c_loc(f4), int(4 * host_np, c_size_t))
+         endif
+         if (u_vmax /= 0) call max_field('vmax', u_vmax, -999.0)
+         if (u_twet /= 0) call max_field('twet', u_twet, 0.0)
+         if (u_qmax /= 0) call max_field('qmax', u_qmax, -999.0)
+         deallocate(f4)
+      endif
+      if (wrst) then
+         ts = time_string(tout)
+         call logmsg('Info    : writing restart file sfincs.'//ts//'.rst')
+         rc = sfx_write_restart(handle, 'sfincs.'//ts//'.rst'//c_null_char)
+         if (rc < 0) call fail('restart file')
+      endif
+   contains
+      subroutine max_field(name, u, reset)
+         character(*), intent(in) :: name
+         integer, intent(in) :: u
+         real*4, intent(in) :: reset
+         integer(c_int) :: r
+         r = sfx_get_state(handle, name//c_null_char, c_loc(f4), int(4 * host_np, c_size_t))
+         write(u) f4
+         f4 = reset
+         r = sfx_set_state(handle, name//c_null_char, c_loc(f4), int(4 * host_np, c_size_t))
+      end subroutine
+   end subroutine
+
+   ! finalize_output (sfincs_output.f90:262-310): the maxima of the last (or only) interval
+   subroutine output_finalize()
+      if (out_dtmax > 1.0d-6 .and. ntmaxout == 0) then
+         ntmaxout = 1
+         call write_output(t, .false., .true., .false.)
+      else if (out_dtmax > 1.0d-6 .and. ntmaxout > 0 .and. t < tmaxout) then
+         ntmaxout = ntmaxout + 1
+         call write_output(host_t1, .false., .true., .false.)
+      endif
+      call output_close()
+   end subroutine
+
+   ! time_to_string (sfincs_date.f90:298-327): 'yyyymmdd.hhmmss' of int(t) seconds after tref
+   character(15) function time_string(tsec)
+      real*8, intent(in) :: tsec
+      integer(8) :: total, days, secs, l, n, i, j, y, mo, d
+      total = int(host_tref_sec, 8) + int(tsec, 8)
+      days = total / 86400
+      secs = total - days * 86400
+      ! Julian day number to the proleptic Gregorian date (inverse of date_seconds)
+      l = days + 68569
+      n = 4 * l / 146097
+      l = l - (146097 * n + 3) / 4
+      i = 4000 * (l + 1) / 1461001
+      l = l - 1461 * i / 4 + 31
+      j = 80 * l / 2447
+      d = l - 2447 * j / 80
+      l = j / 11
+      mo = j + 2 - 12 * l
+      y = 100 * (n - 49) + i + l
+      write(time_string, '(i4.4,i2.2,i2.2,a1,i2.2,i2.2,i2.2)') y, mo, d, '.', secs / 3600, mod(secs, 3600_8) / 60, &
+         mod(secs, 60_8)
+   end function
+
    ! ---- sfincs_finalize (sfincs_lib.f90:687-819) ---------------------------------------
    function sfincs_finalize() result(ierr)
       integer :: ierr, u
       integer(c_int) :: rc
       ierr = 0
       if (c_associated(handle)) then
+         if (host_flags(4) == 1) call write_timestep_analysis()
+         call output_finalize()
          call host_sync()
          open(newunit=u, file='zs_final.dat', form='unformatted', access='stream', status='replace')
          write(u) host_zs
          close(u)
-         open(newunit=u, file='zsmax.dat', form='unformatted', access='stream', status='replace')
-         write(u) host_zsmax
-         close(u)
-         ! compute_store_variables fields, as the maximum output writes them (sfincs_output.f90:490-512)
-         if (host_flags(1) == 1) call write_cell_f32('vmax', host_np, 'vmax.dat')
-         if (host_flags(2) == 1) call write_cell_f32('qmax', host_np, 'qmax.dat')
-         if (host_flags(3) == 1) call write_cell_f32('twet', host_np, 'twet.dat')
-         if (host_flags(4) == 1) call write_timestep_analysis()
          rc = sfx_destroy(handle)
          handle = c_null_ptr
       endif
@@ -878,9 +1258,16 @@ module sfincs_bmi
    use iso_c_binding
    use sfincs_lib
    implicit none
+   ! string lengths of the BMI (sfincs_bmi.f90:41-52)
+   integer(c_int), bind(C, name='BMI_LENVARADDRESS') :: BMI_LENVARADDRESS = 64
+   integer(c_int), bind(C, name='BMI_LENVARTYPE') :: BMI_LENVARTYPE = 64
+   integer(c_int), bind(C, name='BMI_LENGRIDTYPE') :: BMI_LENGRIDTYPE = 64
+   integer(c_int), bind(C, name='BMI_LENCOMPONENTNAME') :: BMI_LENCOMPONENTNAME = 64
+   integer(c_int), bind(C, name='BMI_LENVERSION') :: BMI_LENVERSION = 256
+   integer(c_int), bind(C, name='BMI_LENERRMESSAGE') :: BMI_LENERRMESSAGE = 1024
 contains
-   integer(c_int) function initialize(c_config_file) result(ierr) bind(C, name='initialize')
-      type(c_ptr), value :: c_config_file
+   ! initialize() (sfincs_bmi.f90:56-63): reads sfincs.inp of the working directory
+   integer(c_int) function initialize() result(ierr) bind(C, name='initialize')
       host_qext_requested = .true.   ! the BMI always carries the qext field (sfincs_bmi.f90:60)
       ierr = sfincs_initialize()
    end function
@@ -935,7 +1322,7 @@ contains
       ierr = 0
       select case (trim(nm))
       case ('zs')
-         call host_sync()
+         call host_expose('zs')
          x = c_loc(host_zs)
       case ('zsmax')
          call host_sync()
@@ -945,6 +1332,7 @@ contains
       case ('z_yz')
          x = c_loc(host_yz)
       case ('zb')
+         call host_expose('zb')
          x = c_loc(host_zb)
       case ('subgrid_z_zmin')
          x = c_loc(host_zmin)
@@ -986,6 +1374,8 @@ contains
          var_shape(1) = size(host_zmin)
       case ('qext')
          if (allocated(host_qext)) var_shape(1) = size(host_qext)
+      case ('z_index_z_n', 'z_index_z_m')
+         if (allocated(host_zn)) var_shape(1) = size(host_zn)
       case default
          ierr = -1
       end select
@@ -1002,6 +1392,8 @@ contains
          tn = 'double'
       case ('z_xz', 'z_yz', 'zb', 'subgrid_z_zmin', 'zsmax', 'qext')
          tn = 'float'
+      case ('z_index_z_n', 'z_index_z_m')
+         tn = 'integer'
       case default
          tn = ' '
          ierr = -1
@@ -1061,9 +1453,9 @@ contains
       ierr = 0
    end function
 
-   ! set_logical switches qext (sfincs_bmi.f90:226-245). The hooks of processes outside this
-   ! build, zbuv updates after a bed change and wave-enhanced roughness, return -1 so that
-   ! a coupler notices (:277-297).
+   ! set_logical switches qext (sfincs_bmi.f90:226-245). update_zbuv re-derives zbuvmx after
+   ! a bed change (:277-284); wave-enhanced roughness, a process outside this build, returns
+   ! -1 so that a coupler notices (:286-297).
    integer(c_int) function set_logical(c_flag_name, ival) result(ierr) bind(C, name='set_logical')
       character(kind=c_char), intent(in) :: c_flag_name(*)
       integer(c_int) :: ival(*)
@@ -1072,7 +1464,7 @@ contains
    end function
 
    integer(c_int) function update_zbuv() result(ierr) bind(C, name='update_zbuv')
-      ierr = -1
+      ierr = host_update_zbuv()
    end function
 
    integer(c_int) function update_apparent_roughness() result(ierr) bind(C, name='update_apparent_roughness')

@@ -21,6 +21,18 @@ EXE = os.path.join(FDIR, "sfincs_mi355x")
 needs_host = pytest.mark.skipif(not os.path.exists(HOSTLIB), reason="Fortran host not built (amdflang)")
 
 
+def records(path, dtype):
+    """The records of a Fortran unformatted sequential file (4-byte length markers)."""
+    raw = open(path, "rb").read()
+    out, k = [], 0
+    while k < len(raw):
+        n = int(np.frombuffer(raw, np.int32, 1, k)[0])
+        out.append(np.frombuffer(raw, dtype, n // np.dtype(dtype).itemsize, k + 4).copy())
+        assert int(np.frombuffer(raw, np.int32, 1, k + 4 + n)[0]) == n
+        k += n + 8
+    return out
+
+
 @needs_host
 def test_host_exports_driver_and_bmi_names():
     lib = ctypes.CDLL(HOSTLIB)
@@ -32,6 +44,10 @@ def test_host_exports_driver_and_bmi_names():
     syms = subprocess.run(["nm", "-D", HOSTLIB], capture_output=True, text=True).stdout
     for name in ("sfincs_initialize", "sfincs_update", "sfincs_finalize"):
         assert name in syms, name
+    # the BMI string lengths (sfincs_bmi.f90:41-52)
+    for name, v in (("BMI_LENVARADDRESS", 64), ("BMI_LENVARTYPE", 64), ("BMI_LENGRIDTYPE", 64),
+                    ("BMI_LENCOMPONENTNAME", 64), ("BMI_LENVERSION", 256), ("BMI_LENERRMESSAGE", 1024)):
+        assert ctypes.c_int.in_dll(lib, name).value == v, name
 
 
 def test_deck_round_trip_of_regular_mesh(tmp_path):
@@ -61,8 +77,13 @@ def _table(lib, m, name):
     return bytes(buf)[:n.value]
 
 
+SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax", "uv_havg", "uv_nrep", "uv_pwet",
+                  "uv_fnfit", "uv_havg_zmax", "uv_nrep_zmax", "uv_navg_w", "z_volume")
+
+
 @needs_host
-@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves",
+                                  "subgrid_binary"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
@@ -78,6 +99,9 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
                             twet_threshold=0.02, timestep_analysis=1)
     elif kind == "quadtree":
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
+    elif kind == "subgrid_binary":
+        sc = W.with_subgrid_binary(W.subgrid_case(22, 26, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0),
+                                   tmp_path / "gen.sbg")
     else:
         sc = W.subgrid_case(20, 24, dx=25.0, seed=43, t_end=900.0, nsrc=2, rain_mmh=10.0)
         sc.subgrid = None
@@ -96,20 +120,25 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     mf = host.sfincs_host_model()
     mp = sc.build_model()
     lib = _lib.lib()
-    for name in TABLES:
+    for name in TABLES + (SUBGRID_TABLES if kind == "subgrid_binary" else ()):
         a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
         assert (a is None) == (b is None), name
         assert a == b, name
+    if kind == "subgrid_binary":
+        assert mp.sizes()["subgrid"] == 1 and len(_table(lib, mf, "uv_havg")) > 0
 
 
 @needs_host
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["regular", "quadtree"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary"])
 def test_fortran_host_matches_python_path(kind, tmp_path):
     print(f"case fortran host {kind}", flush=True)
     from sfincs_amd import _lib
     if kind == "regular":
         sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0)
+    elif kind == "subgrid_binary":   # a subgrid deck through sfincs_initialize (sbgfile)
+        sc = W.with_subgrid_binary(W.subgrid_case(30, 34, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0),
+                                   tmp_path / "gen.sbg")
     else:
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     d = deck.write(sc, str(tmp_path))
@@ -173,9 +202,10 @@ def test_fortran_host_writes_store_fields_and_timestep_analysis(tmp_path):
     m = sc.build_model()
     inst = _lib.Instance(m)
     inst.update(-1000.0)
-    for f in ("vmax", "qmax", "twet"):
-        a = np.fromfile(os.path.join(d, f + ".dat"), np.float32)
-        assert np.array_equal(a, inst.get(f)), f
+    # the maximum output at the end of the run (finalize_output, sfincs_output.f90:262-310)
+    for f, name in (("vmax", "velmax.dat"), ("qmax", "qmax.dat"), ("twet", "tmax.dat")):
+        rec = records(os.path.join(d, name), np.float32)
+        assert len(rec) == 1 and np.array_equal(rec[0], inst.get(f)), f
     dtc, pct = m.timestep_analysis_per_cell(inst.get("tsa_sum"), inst.get("tsa_wet"), inst.get("tsa_lim"), inst.time()[2])
     assert np.array_equal(np.fromfile(os.path.join(d, "timestep_analysis_dt.dat"), np.float32), dtc)
     assert np.array_equal(np.fromfile(os.path.join(d, "timestep_analysis_pct.dat"), np.float32), pct)
@@ -223,6 +253,9 @@ def test_bmi_mesh_queries(kind, tmp_path):
     assert host.get_var_type(b"zs", tbuf) == 0 and tbuf.value == b"double"
     rank = ctypes.c_int()
     assert host.get_var_rank(b"zb", ctypes.byref(rank)) == 0 and rank.value == 1
+    for name in (b"z_index_z_n", b"z_index_z_m"):   # shape and type only, as the reference (:150-176)
+        assert host.get_var_shape(name, shape) == 0 and shape[0] == np_
+        assert host.get_var_type(name, tbuf) == 0 and tbuf.value == b"integer"
     ptr = ctypes.c_void_p()
     assert host.get_value_ptr(b"z_yz", ctypes.byref(ptr)) == 0
     got = np.ctypeslib.as_array((ctypes.c_float * np_).from_address(ptr.value))
@@ -246,7 +279,7 @@ def test_bmi_qext_coupling(tmp_path):
     cwd = os.getcwd()
     os.chdir(d)
     try:
-        assert host.initialize(None) == 0
+        assert host.initialize() == 0
         np_ = sc.build_model().sizes()["np"]
         ptr = ctypes.c_void_p()
         assert host.get_value_ptr(b"qext", ctypes.byref(ptr)) == 0
@@ -264,3 +297,96 @@ def test_bmi_qext_coupling(tmp_path):
     inst.set("qext", field)
     inst.update(900.0)
     assert np.array_equal(zs_f, inst.get("zs"))
+
+
+@needs_host
+@pytest.mark.gpu
+def test_bmi_bed_and_level_writes(tmp_path):
+    """A coupler writes the bed through get_value_ptr("zb") and calls update_zbuv
+    (sfincs_bmi.f90:277-284), later raises the water level through get_value_ptr("zs"):
+    the host uploads both before stepping, and the run equals the Python-driven run given
+    the same writes (sfx_set_state "zb" + sfx_update_zbuvmx, "zs")."""
+    print("case bmi zb / zs writes", flush=True)
+    from sfincs_amd import _lib
+    sc = W.regular_case(28, 36, dx=50.0, seed=50, t_end=1800.0, tide_amp=1.5)
+    d = deck.write(sc, str(tmp_path))
+    host = ctypes.CDLL(HOSTLIB)
+    host.update_until.argtypes = [ctypes.c_double]
+    np_ = sc.build_model().sizes()["np"]
+    r = np.random.default_rng(51)
+    bump = np.zeros(np_, np.float32)
+    bump[r.choice(np_, 60, replace=False)] = 2.0
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        assert host.initialize() == 0
+        ptr = ctypes.c_void_p()
+        assert host.update_until(ctypes.c_double(300.0)) == 0
+        assert host.get_value_ptr(b"zb", ctypes.byref(ptr)) == 0
+        zb = np.ctypeslib.as_array((ctypes.c_float * np_).from_address(ptr.value))
+        zb_new = (zb + bump).astype(np.float32)
+        zb[:] = zb_new
+        assert host.update_zbuv() == 0
+        assert host.update_until(ctypes.c_double(900.0)) == 0
+        assert host.get_value_ptr(b"zs", ctypes.byref(ptr)) == 0
+        zs = np.ctypeslib.as_array((ctypes.c_double * np_).from_address(ptr.value))
+        zs_mid = zs.copy()
+        zs_new = zs_mid + 0.05
+        zs[:] = zs_new
+        assert host.update_until(ctypes.c_double(1500.0)) == 0
+        zs_f = zs.copy()                     # the live copy shows the model's levels again
+        assert host.finalize() == 0
+    finally:
+        os.chdir(cwd)
+    inst = _lib.Instance(sc.build_model())
+    inst.update(300.0)
+    inst.set("zb", zb_new)
+    inst.update_zbuvmx()
+    inst.update(600.0)
+    assert np.array_equal(inst.get("zs"), zs_mid)
+    inst.set("zs", zs_new)
+    inst.update(600.0)
+    assert np.array_equal(zs_f, inst.get("zs"))
+    assert np.array_equal(inst.get("zb"), zb_new)
+
+
+@needs_host
+@pytest.mark.gpu
+def test_fortran_host_scheduled_output(tmp_path):
+    """dtout, dtmaxout and dtrstout in sfincs.inp (sfincs_lib.f90:419-489,
+    sfincs_output.f90:80-258, 457-519, 657-729): zs.dat holds zs at the first step reaching
+    t0out, t0out + dtout, ...; zsmax.dat the maxima of each dtmaxout interval with dry
+    cells at -999, after which the maxima start again from -999; sfincs.<date>.rst the
+    restart at dtrstout. Equal to the Python-driven run stopped at the same times."""
+    print("case fortran host scheduled output", flush=True)
+    from sfincs_amd import _lib
+    sc = W.regular_case(28, 36, dx=50.0, seed=52, t_end=1800.0, tide_amp=2.0)
+    d = deck.write(sc, str(tmp_path))
+    with open(os.path.join(d, "sfincs.inp"), "a") as f:
+        f.write("dtout = 300\ndtmaxout = 600\ndtrstout = 900\noutputformat = bin\n")
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = sc.build_model()
+    zb = m.table("zb")
+    inst = _lib.Instance(m)
+    zs_rec = records(os.path.join(d, "zs.dat"), np.float64)
+    mx_rec = records(os.path.join(d, "zsmax.dat"), np.float32)
+    assert len(zs_rec) == 7 and len(mx_rec) == 3
+    inst.update(0.0)                                   # t0out = t0: the first step
+    assert np.array_equal(zs_rec[0], inst.get("zs"))
+    k_max = 0
+    for k in range(1, 7):
+        inst.update_to(300.0 * k)
+        assert np.array_equal(zs_rec[k], inst.get("zs")), k
+        if k == 3:
+            rst = tmp_path / "py.rst"
+            inst.write_restart(rst)
+            assert open(rst, "rb").read() == open(os.path.join(d, "sfincs.20000101.001500.rst"), "rb").read()
+        if k % 2 == 0:
+            zmx = inst.get("zsmax")
+            zmx = np.where(zmx < zb + np.float32(0.05), np.float32(-999.0), zmx).astype(np.float32)
+            assert np.array_equal(mx_rec[k_max], zmx), k
+            assert np.count_nonzero(zmx > -999.0) > 0
+            k_max += 1
+            inst.set("zsmax", np.full(m.sizes()["np"], -999.0, np.float32))
+    assert np.array_equal(np.fromfile(os.path.join(d, "zs_final.dat"), np.float64), inst.get("zs"))

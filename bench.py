@@ -97,7 +97,7 @@ def host_info():
     return info
 
 
-def cpu_baseline(sc, model, inst, nsteps):
+def cpu_baseline(sc, model, inst, nsteps, host):
     """The reference's own compute_fluxes + compute_water_levels (oracle/_ref, OpenMP)
     timed with the reference's per-phase timers on a bounded sample of the same run,
     started from the GPU state after warm-up. Falls back to the C restatement."""
@@ -112,10 +112,11 @@ def cpu_baseline(sc, model, inst, nsteps):
     for f in ("zs", "q", "uv", "kfuv", "zs0", "zsderv", "zsmax") + (("z_volume",) if model.sizes()["subgrid"] else ()):
         orc.state(f)[...] = inst.get(f)
     t, dt, nt = inst.time()
+    _, st = inst.update_to(t)            # no step: the CFL minimum of the last momentum sweep
     orc.state("uvmean")[...] = inst.get("uvmean")
     orc.set("t", t)
     orc.set("nt", nt)
-    orc.set("min_dt", dt / float(np.float32(sc.params.get("alpha", 0.5))))
+    orc.set("min_dt", st["min_dt"])
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     if O.ref_available():
         r = O.RefRun(orc)
@@ -132,9 +133,10 @@ def cpu_baseline(sc, model, inst, nsteps):
         kind = "port"
         cores = 1
     np_ = model.sizes()["np"]
-    return dict(value=np_ * nsteps / secs, unit="cell-updates/s", cores=cores, kind=kind, host=host_info(),
+    return dict(value=np_ * nsteps / secs, unit="cell-updates/s", cores=cores, kind=kind, host=host,
                 sample=f"{nsteps} consecutive steps of the same 2048x2048 subgrid run, started from the GPU "
-                       f"state after warm-up; timed by the reference's own tloopflux+tloopcont accumulators",
+                       f"state after the timed window (t = {t:.0f} s); timed by the reference's own "
+                       f"tloopflux+tloopcont accumulators",
                 seconds=secs)
 
 
@@ -422,6 +424,7 @@ def main():
     from sfincs_amd import _lib
     from sfincs_amd import workloads as W
 
+    host = host_info()     # before any OpenMP runtime binds this thread to a core
     t0 = time.perf_counter()
     sc = W.config1(args.size)
     t_period = sc.t_end
@@ -514,12 +517,12 @@ def main():
         "branches_per_step": branches,
         "continuity_stores_per_step": dict(zip(("zs", "z_volume", "zs0", "zsderv", "zsmax", "steady_skipped"), stores)),
     }
+    batch.close()
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(sc, model, inst, args.cpu_steps)
+            out["cpu_baseline"] = cpu_baseline(sc, model, inst, args.cpu_steps, host)
         except Exception as e:    # reported, never hidden
             out["cpu_baseline"] = {"error": repr(e)}
-    batch.close()
     if not args.no_whole_period:
         out["whole_period"] = whole_period(model, tuning, t_period, twin=not args.no_twin)
     print(json.dumps(out))

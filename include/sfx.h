@@ -85,6 +85,8 @@ typedef struct sfx_params {
   int32_t use_qext;          /* 0: external sources per cell (m/s), written with sfx_set_state
                               * "qext" (sfincs_continuity.f90:122-128, 473-497); the BMI
                               * host turns it on as sfincs_bmi.f90:60 does */
+  float structure_relax;     /* 10 s: relaxation of drainage structure discharges
+                              * (sfincs_input.f90:181, sfincs_discharges.f90:621-625) */
 } sfx_params;
 
 void sfx_params_default(sfx_params* p);
@@ -113,6 +115,7 @@ typedef struct sfx_sizes {
   int32_t ngbnd, nkcuv2, nsrc, nbnd;
   int32_t subgrid, quadtree, dense;   /* dense = full rectangle, implicit neighbours on device */
   float dtmax, dtmin;
+  int32_t ndrn;                       /* drainage structures (each adds an intake and an outfall point) */
 } sfx_sizes;
 
 /* ---- host-side model (CPU only; no GPU needed) -------------------------------------
@@ -150,6 +153,16 @@ int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const fl
 /* src/dis: point discharges; q is nt*nsrc, source fastest. */
 int  sfx_model_set_sources(sfx_model* m, int32_t nsrc, const float* x, const float* y,
                            int32_t nt, const float* t, const float* q);
+/* drnfile: drainage structures (read_discharges, sfincs_discharges.f90:46-57, 176-290;
+ * update, :371-655): per structure the intake point (x_in, y_in), the outfall point
+ * (x_out, y_out), the type (1 pump: params[0] = Q m3/s; 2 culvert and 3 check valve:
+ * params[0] = coefficient of sqrt(dz); 4 gate opened by the intake level: width, sill
+ * level, Manning's n, zmin, zmax, closing time; 5 gate closed and opened at set times:
+ * width, sill level, Manning's n, t_close, t_open, closing time) and 6 params per
+ * structure (params[6*k + j]). Both points must lie in active cells. The intake and
+ * outfall discharges join the point sources in continuity. */
+int  sfx_model_set_drainage(sfx_model* m, int32_t ndrn, const float* x_in, const float* y_in, const float* x_out,
+                            const float* y_out, const int32_t* type, const float* params);
 /* prcfile: uniform rain series in mm/h. */
 /* Storage volume per cell, m3 (volfile; initialize_storage_volume, sfincs_domain.f90:2063-2112):
  * on subgrid meshes water entering a cell fills its storage before the cell's volume grows

@@ -32,7 +32,7 @@ TABLES = (
     "ibuvdir", "fac_bnd_gbp", "index_bdr_gbp", "index_zsi_bdr", "dzs_bdr", "t_bnd", "zs_bnd", "zsi_bnd", "nmindsrc",
     "tsrc",
     "qsrc", "tprcpt", "tprcpv", "twnd",
-    "wndmag", "wnddir", "cdval",
+    "wndmag", "wnddir", "cdval", "drainage_type", "drainage_params", "drainage_distance",
 )
 REGULAR_TABLES = ("zb", "zbuv", "zbuvmx", "gn2uv")
 SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax", "uv_navg_w", "uv_fnfit",
@@ -129,7 +129,9 @@ class OracleRun:
         self._bind("qext", np.zeros(max(1, s["np"]), np.float32))
         sv = model.table("storage_volume")
         self._bind("storage_volume", sv.copy() if sv.size else np.zeros(max(1, s["np"]), np.float32))
-        self._bind("qtsrc", np.zeros(max(1, s["nsrc"]), np.float32))
+        self._bind("qtsrc", np.zeros(max(1, s["nsrc"] + 2 * s["ndrn"]), np.float32))
+        self._bind("drainage_status", np.ones(max(1, s["ndrn"]), np.int32))           # open
+        self._bind("drainage_fraction_open", np.ones(max(1, s["ndrn"]), np.float32))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
         self._bind("x73", x73_table())
@@ -169,6 +171,7 @@ class OracleRun:
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
                     nsrc=s["nsrc"], ntsrc=len(tb("tsrc")), ntprcp=len(tb("tprcpt")), ntwnd=len(tb("twnd")),
+                    ndrn=s["ndrn"], structure_relax=p.structure_relax,
                     g=p.g, pi=p.pi, huthresh=model.params().huthresh, huvmin=p.huvmin,
                     advlim=p.advlim, slopelim=p.slopelim, uvlim=p.uvlim, nuviscfac=p.nuviscfac, fcorio=fcorio,
                     theta=p.theta, wiggle_factor=p.wiggle_factor, wiggle_threshold=p.wiggle_threshold,
@@ -268,6 +271,8 @@ class RefRun:
         R.ref_set_sizes(v["np"], v["npuv"], v["ncuv"], v["nref"], max(v["nl"], 1))
         for k in REF_SCALARS:
             R.ref_set_scalar(k.encode(), float(v[k]))
+        # continuity loops over the point sources and the drainage points (nsrcdrn)
+        R.ref_set_scalar(b"nsrc", float(v["nsrc"] + 2 * v["ndrn"]))
         k = orc.keep
         for n in REF_I32:
             a = k[n] if n in k else np.ascontiguousarray(orc.model.table(n), np.int32)

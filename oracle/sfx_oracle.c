@@ -12,7 +12,8 @@
  *   oracle_compute_water_levels  sfincs_continuity.f90:5-720 (with compute_store_variables)
  *   oracle_update_meteo          sfincs_meteo.f90:1246-1558 (time-series branch)
  *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
- *   oracle_update_discharges     sfincs_discharges.f90:325-369
+ *   oracle_update_discharges     sfincs_discharges.f90:325-655 (point sources and drainage
+ *                                structures)
  *   oracle_update                sfincs_lib.f90:337-683 (step order, lagged dt)
  * Arrays use 0-based storage; index tables hold the reference's 1-based values.
  * Each function's parity check is named in the test files under tests/.
@@ -71,7 +72,14 @@ typedef struct OM {
   const int32_t *nmindsrc;
   const float *tsrc, *qsrc, *tprcpt, *tprcpv, *twnd, *wndmag, *wnddir, *cdval;
   const float *x73;   /* h**(7/3) table, 1000 x 10, column-major (sfincs_data.f90:858-881) */
-  float *qtsrc;
+  float *qtsrc;   /* nsrc point sources, then each drainage structure's intake and outfall */
+  /* drainage structures (sfincs_discharges.f90:176-290, 371-655) */
+  int32_t ndrn;
+  float structure_relax;
+  const int32_t *drainage_type;
+  const float *drainage_params, *drainage_distance;   /* params: 6 per structure */
+  int32_t *drainage_status;
+  float *drainage_fraction_open;
   /* time state */
   double t;
   float dt, min_dt;
@@ -104,6 +112,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
   BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsi_bnd) BIND(zsit_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
   BIND(nmindsrc) BIND(tsrc) BIND(qsrc) BIND(tprcpt) BIND(tprcpv) BIND(twnd) BIND(wndmag) BIND(wnddir) BIND(cdval) BIND(qtsrc) BIND(x73)
+  BIND(drainage_type) BIND(drainage_params) BIND(drainage_distance) BIND(drainage_status) BIND(drainage_fraction_open)
   return -1;
 }
 #undef BIND
@@ -114,7 +123,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(np) SETI(npuv) SETI(ncuv) SETI(nref) SETI(nl) SETI(subgrid) SETI(use_quadtree) SETI(advection)
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
-  SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error)
+  SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error) SETI(ndrn) SETF(structure_relax)
   SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
@@ -380,7 +389,7 @@ void oracle_compute_water_levels(OM* m, double t, float dt) {
 /* ---- continuity: sfincs_continuity.f90:46-262 and :265-624 -------------------------- */
 static void water_levels(OM* m, float dt) {
   if (!m->subgrid) {
-    for (int32_t s = 0; s < m->nsrc; ++s) {   /* :87-106 */
+    for (int32_t s = 0; s < m->nsrc + 2 * m->ndrn; ++s) {   /* :87-106, nsrcdrn */
       const int32_t nm = m->nmindsrc[s] - 1;
       const float inc = m->qtsrc[s] * dt / m->cell_area[m->z_flags_iref[nm] - 1];
       m->zs[nm] = dmax(m->zs[nm] + (double)inc, (double)m->zb[nm]);
@@ -400,7 +409,7 @@ static void water_levels(OM* m, float dt) {
     }
     return;
   }
-  for (int32_t s = 0; s < m->nsrc; ++s) {   /* :311-325 */
+  for (int32_t s = 0; s < m->nsrc + 2 * m->ndrn; ++s) {   /* :311-325, nsrcdrn */
     const int32_t nm = m->nmindsrc[s] - 1;
     if ((m->z_volume[nm] >= 0) || ((m->qtsrc[s] < 0.0f) && (m->z_volume[nm] >= 0)))
       m->z_volume[nm] = m->z_volume[nm] + (double)(m->qtsrc[s] * dt);
@@ -649,8 +658,10 @@ void oracle_update_boundaries(OM* m, double t, float dt) {
 }
 
 /* ---- discharges: sfincs_discharges.f90:353-369 ---------------------------------------- */
+static void update_drainage(OM* m, double t, float dt);
+
 void oracle_update_discharges(OM* m, double t, float dt) {
-  (void)dt;
+  if (m->ndrn > 0) update_drainage(m, t, dt);
   if (m->nsrc <= 0) return;
   for (int32_t it = m->itsrclast; it <= m->ntsrc; ++it) {
     if ((double)m->tsrc[it - 1] > t) {
@@ -663,6 +674,80 @@ void oracle_update_discharges(OM* m, double t, float dt) {
       m->itsrclast = it - 1;
       break;
     }
+  }
+}
+
+/* ---- drainage structures: sfincs_discharges.f90:371-655 --------------------------------- */
+/* Runs after the point sources' series (:353-369) in the reference; the oracle's
+ * oracle_update_discharges calls it first because the series write only qtsrc[0..nsrc). */
+static void update_drainage(OM* m, double t, float dt) {
+  for (int32_t idrn = 0; idrn < m->ndrn; ++idrn) {
+    const int32_t jin = m->nsrc + 2 * idrn, jout = jin + 1;   /* 0-based of nsrc + idrn*2 - 1, nsrc + idrn*2 */
+    const int32_t nmin = m->nmindsrc[jin] - 1, nmout = m->nmindsrc[jout] - 1;
+    if (nmin < 0 || nmout < 0) continue;
+    const float* par = m->drainage_params + 6 * idrn;
+    float qq = 0.0f;
+    switch (m->drainage_type[idrn]) {
+      case 1:   /* pump */
+        qq = par[0];
+        break;
+      case 2:   /* culvert */
+      case 3:   /* check valve */
+        if (m->zs[nmin] > m->zs[nmout]) qq = (float)((double)par[0] * sqrt(m->zs[nmin] - m->zs[nmout]));
+        else qq = (float)((double)(-par[0]) * sqrt(m->zs[nmout] - m->zs[nmin]));
+        if (m->drainage_type[idrn] == 3) qq = fmx(qq, 0.0f);
+        break;
+      case 4:
+      case 5: {
+        const float wdt = par[0], zsill = par[1], mng = par[2], tcls = par[5];
+        const float dzds = (float)((m->zs[nmout] - m->zs[nmin]) / (double)m->drainage_distance[idrn]);
+        float frac = m->drainage_fraction_open[idrn];
+        const float hgate = (float)dmax(dmax(m->zs[nmin], m->zs[nmout]) - (double)zsill, 0.0);
+        const float dfrac = dt / tcls;
+        const float qq0 = -m->qtsrc[jin] / (wdt * fmx(frac, 0.001f));
+        int32_t* st = &m->drainage_status[idrn];
+        if (m->drainage_type[idrn] == 4) {
+          const float zmin = par[3], zmax = par[4];
+          if (*st == 0) {
+            if (m->zs[nmin] > (double)zmin && m->zs[nmin] < (double)zmax) *st = 3;
+          } else if (*st == 1) {
+            if (m->zs[nmin] <= (double)zmin || m->zs[nmin] >= (double)zmax) *st = 2;
+          }
+        } else {
+          const float tclose = par[3], topen = par[4];
+          if (*st == 0) {
+            if (t >= (double)topen) *st = 3;
+          } else if (*st == 1) {
+            if (t >= (double)tclose && t < (double)topen) *st = 2;
+          }
+        }
+        if (*st == 2) {
+          frac = frac - dfrac;
+          if (frac < 0.0f) { frac = 0.0f; *st = 0; }
+        } else if (*st == 3) {
+          frac = frac + dfrac;
+          if (frac > 1.0f) { frac = 1.0f; *st = 1; }
+        }
+        m->drainage_fraction_open[idrn] = frac;
+        qq = (qq0 - m->g * hgate * dzds * dt) / (1.0f + m->g * (mng * mng) * dt * fabsf(qq0) / powf(hgate, 7.0f / 3.0f));
+        qq = qq * wdt * frac;
+        break;
+      }
+    }
+    qq = 1.0f / (m->structure_relax / dt) * qq + (1.0f - (1.0f / (m->structure_relax / dt))) * -m->qtsrc[jin];
+    if (m->subgrid) {
+      if (qq > 0.0f) qq = (float)fmin((double)qq, dmax(m->z_volume[nmin], 0.0) / (double)dt);
+      else qq = (float)fmax((double)qq, -(dmax(m->z_volume[nmout], 0.0) / (double)dt));
+    } else {
+      if (qq > 0.0f)
+        qq = (float)fmin((double)qq, dmax((m->zs[nmin] - (double)m->zb[nmin]) *
+                                            (double)m->cell_area[m->z_flags_iref[nmin] - 1], 0.0) / (double)dt);
+      else
+        qq = (float)fmax((double)qq, -(dmax((m->zs[nmout] - (double)m->zb[nmout]) *
+                                              (double)m->cell_area[m->z_flags_iref[nmout] - 1], 0.0) / (double)dt));
+    }
+    m->qtsrc[jin] = -qq;
+    m->qtsrc[jout] = qq;
   }
 }
 

@@ -26,7 +26,8 @@ class sfx_params(C.Structure):
         "uvmax", "nuvisc", "nuviscfac", "latitude", "g", "pi")] + [(n, C.c_int32) for n in (
         "advection", "advection_scheme", "viscosity", "coriolis", "friction2d", "advection_mask",
         "wiggle_suppression", "store_zsmax", "bndtype", "h73table", "store_velmax", "store_fluxmax",
-        "store_twet")] + [("twet_threshold", C.c_float), ("timestep_analysis", C.c_int32), ("use_qext", C.c_int32)]
+        "store_twet")] + [("twet_threshold", C.c_float), ("timestep_analysis", C.c_int32), ("use_qext", C.c_int32),
+                                       ("structure_relax", C.c_float)]
 
 
 class sfx_quadtree_input(C.Structure):
@@ -46,7 +47,7 @@ class sfx_subgrid_input(C.Structure):
 class sfx_sizes(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "np", "npuv", "ncuv", "nref", "nlevels", "nmax", "mmax", "ngbnd", "nkcuv2", "nsrc", "nbnd",
-        "subgrid", "quadtree", "dense")] + [("dtmax", C.c_float), ("dtmin", C.c_float)]
+        "subgrid", "quadtree", "dense")] + [("dtmax", C.c_float), ("dtmin", C.c_float), ("ndrn", C.c_int32)]
 
 
 class sfx_step_stats(C.Structure):
@@ -67,6 +68,7 @@ def _sig(lib):
         "sfx_model_create_regular": (I, [C.POINTER(sfx_params), i32, i32, f, f, f, f, f, P, C.POINTER(P)]),
         "sfx_model_create_quadtree": (I, [C.POINTER(sfx_params), C.POINTER(sfx_quadtree_input), C.POINTER(P)]),
         "sfx_model_read_subgrid_binary": (I, [P, C.c_char_p]),
+        "sfx_model_set_drainage": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_sizes": (I, [P, C.POINTER(sfx_sizes)]),
         "sfx_model_get_table": (I, [P, s, P, sz]),
         "sfx_model_table_bytes": (I, [P, s, C.POINTER(sz)]),
@@ -172,7 +174,7 @@ TABLE_DTYPES = {
             "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm", "uv_index_u_num", "uv_index_v_ndm",
             "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
             "cuv_index_uv2", "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp",
-            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc", "index_bdr_gbp", "index_zsi_bdr"),
+            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc", "index_bdr_gbp", "index_zsi_bdr", "drainage_type"),
 }
 
 
@@ -271,6 +273,15 @@ class Model:
         x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
         t = np.ascontiguousarray(t, np.float32); q = np.ascontiguousarray(q, np.float32)
         check(lib().sfx_model_set_sources(self.h, len(x), ptr(x), ptr(y), len(t), ptr(t), ptr(q)), "set_sources")
+
+    def set_drainage(self, x_in, y_in, x_out, y_out, types, params):
+        """Drainage structures (drnfile): intake and outfall points, type 1-5, 6 params each."""
+        a = [np.ascontiguousarray(v, np.float32) for v in (x_in, y_in, x_out, y_out)]
+        ty = np.ascontiguousarray(types, np.int32)
+        par = np.zeros((len(ty), 6), np.float32)
+        pp = np.asarray(params, np.float32).reshape(len(ty), -1)
+        par[:, :pp.shape[1]] = pp
+        check(lib().sfx_model_set_drainage(self.h, len(ty), *[ptr(v) for v in a], ptr(ty), ptr(par)), "set_drainage")
 
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)

@@ -165,8 +165,13 @@ struct DevInst {
   void* uvmean;
   const float* t_bnd; const float* zs_bnd;
   int32_t bnd_overlap;           // a Neumann copy and a boundary flux point share a cell
-  // sources and meteo
+  // sources and meteo; src_cell and qtsrc hold nsrc point sources, then each drainage
+  // structure's intake and outfall
   int32_t nsrc, ntsrc, ntprcp, ntwnd;
+  int32_t ndrn;                  // drainage structures
+  float structure_relax;
+  const int32_t* drn_type; const float* drn_par; const float* drn_dist;   // per structure: type, 6 params, distance
+  int32_t* drn_status; float* drn_frac;    // gate state: 0 closed, 1 open, 2 closing, 3 opening; fraction open
   const int32_t* src_cell; const float* tsrc; const float* qsrc; float* qtsrc;
   const float* tprcpt; const float* tprcpv; const float* twnd; const float* wndmag; const float* wnddir;
   const float* cdval; const float* x73;
@@ -521,6 +526,76 @@ __device__ void forcing_ahead_block(const DevInst& I) {
   if (threadIdx.x == 0) G(I.sc)->ahead = 1;
 }
 
+// One drainage structure (sfincs_discharges.f90:371-655): its discharge from the levels of
+// its intake and outfall cells (pump, culvert, check valve, or a gate whose opening follows
+// the intake level or set times, with the Bates et al. (2010) inertial formula), relaxed
+// towards the last step's over structure_relax seconds and limited by the water the
+// giving cell holds; the intake loses it and the outfall gains it as point sources. The
+// arithmetic and its mix of f32 and f64 (zs, z_volume) are the reference's.
+__device__ void drainage_update(const DevInst& I, int k, double t, float dt) {
+  const int jin = I.nsrc + 2 * k, jout = jin + 1;
+  const int nmin = G(I.src_cell)[jin], nmout = G(I.src_cell)[jout];
+  const double zin = G(I.zs)[nmin], zout = G(I.zs)[nmout];
+  const auto qt = G(I.qtsrc);
+  const auto par = G(I.drn_par);
+  const uint32_t b = 6u * (uint32_t)k;
+  const int type = G(I.drn_type)[k];
+  float qq = 0.0f;
+  if (type == 1) {
+    qq = par[b];
+  } else if (type == 2 || type == 3) {
+    if (zin > zout) qq = (float)((double)par[b] * sqrt(zin - zout));
+    else qq = (float)((double)(-par[b]) * sqrt(zout - zin));
+    if (type == 3) qq = sfx::vmax<float>(qq, 0.0f);   // a check valve passes intake -> outfall only
+  } else {
+    const float wdt = par[b], zsill = par[b + 1], mng = par[b + 2], p4 = par[b + 3], p5 = par[b + 4];
+    const float tcls = par[b + 5];
+    const float dzds = (float)((zout - zin) / (double)G(I.drn_dist)[k]);
+    float frac = G(I.drn_frac)[k];
+    const float hgate = (float)sfx::vmax<double>(sfx::vmax<double>(zin, zout) - (double)zsill, 0.0);
+    const float dfrac = dt / tcls;
+    const float qq0 = -qt[jin] / (wdt * sfx::vmax<float>(frac, 0.001f));
+    int st = G(I.drn_status)[k];
+    if (type == 4) {   // open while the intake level lies between zmin (p4) and zmax (p5)
+      if (st == 0) { if (zin > (double)p4 && zin < (double)p5) st = 3; }
+      else if (st == 1) { if (zin <= (double)p4 || zin >= (double)p5) st = 2; }
+    } else {           // closes at t_close (p4), opens at t_open (p5)
+      if (st == 0) { if (t >= (double)p5) st = 3; }
+      else if (st == 1) { if (t >= (double)p4 && t < (double)p5) st = 2; }
+    }
+    if (st == 2) {
+      frac = frac - dfrac;
+      if (frac < 0.0f) { frac = 0.0f; st = 0; }
+    } else if (st == 3) {
+      frac = frac + dfrac;
+      if (frac > 1.0f) { frac = 1.0f; st = 1; }
+    }
+    G(I.drn_frac)[k] = frac;
+    G(I.drn_status)[k] = st;
+    const float g = I.c.g;
+    const float h73 = sfx::libm_pow(hgate, 2, (const uint64_t*)I.pw.t13, (const uint64_t*)I.pw.t53,
+                                    (const uint64_t*)I.pw.t73);
+    qq = (qq0 - g * hgate * dzds * dt) / (1.0f + g * (mng * mng) * dt * fabsf(qq0) / h73);
+    qq = qq * wdt * frac;
+  }
+  const float rr = 1.0f / (I.structure_relax / dt);
+  qq = rr * qq + (1.0f - rr) * -qt[jin];
+  if (I.subgrid) {
+    if (qq > 0.0f) qq = (float)sfx::vmin<double>((double)qq, sfx::vmax<double>(G(I.vol)[nmin], 0.0) / (double)dt);
+    else qq = (float)sfx::vmax<double>((double)qq, -(sfx::vmax<double>(G(I.vol)[nmout], 0.0) / (double)dt));
+  } else {
+    const int ri = I.dense ? 0 : G(I.z_iref)[nmin] - 1, ro = I.dense ? 0 : G(I.z_iref)[nmout] - 1;
+    if (qq > 0.0f)
+      qq = (float)sfx::vmin<double>((double)qq, sfx::vmax<double>((zin - (double)G(I.zb)[nmin]) *
+                                                                      (double)I.c.cell_area[ri], 0.0) / (double)dt);
+    else
+      qq = (float)sfx::vmax<double>((double)qq, -(sfx::vmax<double>((zout - (double)G(I.zb)[nmout]) *
+                                                                         (double)I.c.cell_area[ro], 0.0) / (double)dt));
+  }
+  qt[jin] = -qq;
+  qt[jout] = qq;
+}
+
 // One block per scenario: the step's fold, scalars and forcing series (committed from
 // k_forcing_ahead, or computed here), then boundary points, conditions and fluxes
 // (sfincs_lib.f90:379-550 with sfincs_meteo.f90:1423-1558, sfincs_boundaries.f90:682-1122,
@@ -678,6 +753,10 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
     G(I.zs)[nmb] = (double)zsb1;
     mark_cell(I, nmb);
     if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], (double)zsb1);
+  }
+  if (I.ndrn > 0) {   // update_discharges' drainage structures, after the boundaries (sfincs_lib.f90:530-550)
+    __syncthreads();
+    for (int k = threadIdx.x; k < I.ndrn; k += blockDim.x) drainage_update(I, k, t, F.s.dt);
   }
 }
 
@@ -1324,7 +1403,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
     const double vol_in = vol;   // as loaded: a source alone changes the volume too
     if (kraw & KCS_SOURCE) {   // point sources of this cell, in source order (:311-325)
       double applied = 0.0;
-      for (int s = 0; s < I.nsrc; ++s)
+      for (int s = 0; s < I.nsrc + 2 * I.ndrn; ++s)
         if (G(I.src_cell)[s] == c) {
           const float qs = G(I.qtsrc)[s];
           if ((vol >= 0) || ((qs < 0.0f) && (vol >= 0))) {
@@ -1353,7 +1432,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
   } else {
     if (kraw & KCS_SOURCE) {   // (:87-106)
       const double before = zs;
-      for (int s = 0; s < I.nsrc; ++s)
+      for (int s = 0; s < I.nsrc + 2 * I.ndrn; ++s)
         if (G(I.src_cell)[s] == c) {
           const R inc = (R)G(I.qtsrc)[s] * dt / (R)I.c.cell_area[iref - 1];
           zs = sfx::vmax<double>(zs + (double)inc, (double)G(I.zb)[c]);
@@ -1847,7 +1926,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   {
     std::vector<uint8_t> k(np);
     for (int c = 0; c < np; ++c) k[c] = (uint8_t)M->kcs[c];
-    for (int s = 0; s < M->nsrc; ++s) k[M->nmindsrc[s] - 1] |= KCS_SOURCE;
+    for (int s = 0; s < M->nsrc + 2 * M->ndrn; ++s) k[M->nmindsrc[s] - 1] |= KCS_SOURCE;
     I.kcs = H->upload(k.data(), np);
   }
   if (!M->subgrid) I.zb = H->upload(M->zb.data(), np);
@@ -1947,11 +2026,20 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
       if (M->kcs[cell[i]] == 6 && (fr[cell[i]] || (nmi[i] >= 0 && fw[nmi[i]]))) I.bnd_overlap = 1;
   }
   I.nsrc = M->nsrc; I.ntsrc = M->ntsrc; I.ntprcp = M->ntprcp; I.ntwnd = M->ntwnd;
+  I.ndrn = M->ndrn; I.structure_relax = M->p.structure_relax;
   {
-    std::vector<int32_t> sc(M->nsrc);
-    for (int s = 0; s < M->nsrc; ++s) sc[s] = M->nmindsrc[s] - 1;
+    const int nsd = M->nsrc + 2 * M->ndrn;
+    std::vector<int32_t> sc(nsd);
+    for (int s = 0; s < nsd; ++s) sc[s] = M->nmindsrc[s] - 1;
     I.src_cell = H->upload(sc.data(), sc.size()); I.tsrc = H->upload(M->tsrc.data(), M->tsrc.size());
-    I.qsrc = H->upload(M->qsrc.data(), M->qsrc.size()); I.qtsrc = H->alloc<float>(std::max(1, M->nsrc));
+    I.qsrc = H->upload(M->qsrc.data(), M->qsrc.size()); I.qtsrc = H->alloc<float>(std::max(1, nsd));
+    I.drn_type = H->upload(M->drainage_type.data(), M->drainage_type.size());
+    I.drn_par = H->upload(M->drainage_params.data(), M->drainage_params.size());
+    I.drn_dist = H->upload(M->drainage_distance.data(), M->drainage_distance.size());
+    std::vector<int32_t> st(M->ndrn, 1);           // open, fully (sfincs_discharges.f90:189-190)
+    std::vector<float> fr(M->ndrn, 1.0f);
+    I.drn_status = H->upload(st.data(), st.size());
+    I.drn_frac = H->upload(fr.data(), fr.size());
     I.qtsrc_nx = H->alloc<float>(std::max(1, M->nsrc)); I.snext = H->alloc<StepImage>(1);
     I.tprcpt = H->upload(M->tprcpt.data(), M->tprcpt.size()); I.tprcpv = H->upload(M->tprcpv.data(), M->tprcpv.size());
     I.twnd = H->upload(M->twnd.data(), M->twnd.size()); I.wndmag = H->upload(M->wndmag.data(), M->wndmag.size());

@@ -95,10 +95,15 @@ struct sfx_model {
   std::vector<float> x_bdr, y_bdr, x_bdr_in, y_bdr_in, dzs_bdr;
   std::vector<int32_t> index_zsi_bdr, index_bdr_gbp;   // 1-based cell / bdr point (0 = none)
 
-  // point sources
+  // point sources; nmindsrc holds nsrc + 2 ndrn cells (each drainage structure's intake
+  // then outfall after the point sources, sfincs_discharges.f90:258-290)
   int32_t nsrc = 0, ntsrc = 0;
   std::vector<float> x_src, y_src;
   std::vector<int32_t> nmindsrc;
+  // drainage structures (drnfile)
+  int32_t ndrn = 0;
+  std::vector<float> x_drn_in, y_drn_in, x_drn_out, y_drn_out, drainage_params, drainage_distance;
+  std::vector<int32_t> drainage_type;
   std::vector<float> tsrc, qsrc;   // qsrc[it*nsrc + isrc]
 
   // meteo series

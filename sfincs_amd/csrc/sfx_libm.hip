@@ -25,13 +25,13 @@ namespace sfx {
 namespace {
 
 constexpr uint32_t MAGIC = 0x5346504Du;   // "SFPM"
-constexpr uint32_t VERSION = 6;
+constexpr uint32_t VERSION = 7;
 
 struct Entry { uint32_t x, g; };
 
 struct PowSet {
-  std::vector<Entry> e[2];
-  std::vector<Entry> probe[2];
+  std::vector<Entry> e[NPOW];
+  std::vector<Entry> probe[NPOW];
   bool ok = false;
 };
 
@@ -39,7 +39,7 @@ std::mutex g_mu;
 PowSet g_set;
 std::map<int, PowTables> g_dev;
 
-float exponent(int which) { return which == 0 ? 1.0f / 3.0f : 5.0f / 3.0f; }
+float exponent(int which) { return which == 0 ? 1.0f / 3.0f : (which == 1 ? 5.0f / 3.0f : 7.0f / 3.0f); }
 
 // Enumerate one exponent over [lo, hi) bit patterns; returns false if a verification fails.
 bool enumerate(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std::string& err) {
@@ -50,7 +50,10 @@ bool enumerate(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std
     const double r = pow((double)x, (double)y);
     const float f = (float)r;
     const double est = pow_estimate(x, which);
-    if (std::fabs(est - r) > 1.4210854715202004e-14 * r) {   // 2^-46
+    // 2^-46 (1/3, 5/3) or 2^-44 (7/3, whose exponent offset scales the error of ln x four
+    // times more); rounding decisions only need the error below the 2^-40 straddle guard
+    const double tol = which == 2 ? 5.6843418860808015e-14 : 1.4210854715202004e-14;
+    if (std::fabs(est - r) > tol * r) {
       err = "estimate error too large at x=" + std::to_string(x);
       return false;
     }
@@ -80,7 +83,7 @@ bool generate(PowSet& S, std::string& err) {
   unsigned nt = std::thread::hardware_concurrency();
   if (nt == 0) nt = 4;
   nt = std::min(nt, 64u);
-  for (int which = 0; which < 2; ++which) {
+  for (int which = 0; which < NPOW; ++which) {
     std::vector<std::vector<Entry>> parts(nt);
     std::vector<std::string> errs(nt);
     std::vector<char> oks(nt, 1);
@@ -106,7 +109,7 @@ bool generate(PowSet& S, std::string& err) {
 }
 
 bool probes_match(const PowSet& S) {
-  for (int which = 0; which < 2; ++which)
+  for (int which = 0; which < NPOW; ++which)
     for (const Entry& p : S.probe[which])
       if (f2u(powf(u2f(p.x), exponent(which))) != p.g) return false;
   return true;
@@ -118,7 +121,7 @@ bool load_file(const std::string& path, PowSet& S) {
   uint32_t hdr[4];
   bool ok = std::fread(hdr, 4, 4, f) == 4 && hdr[0] == MAGIC && hdr[1] == VERSION && hdr[2] == f2u(POW_XLO) &&
             hdr[3] == f2u(POW_XHI);
-  for (int which = 0; ok && which < 2; ++which) {
+  for (int which = 0; ok && which < NPOW; ++which) {
     uint64_t n = 0, np = 0;
     ok = std::fread(&n, 8, 1, f) == 1 && std::fread(&np, 8, 1, f) == 1 && n <= POW_HASH_SIZE / 2 && np <= n;
     if (!ok) break;
@@ -138,7 +141,7 @@ void save_file(const std::string& path, const PowSet& S) {
   if (!f) return;
   const uint32_t hdr[4] = {MAGIC, VERSION, f2u(POW_XLO), f2u(POW_XHI)};
   bool ok = std::fwrite(hdr, 4, 4, f) == 4;
-  for (int which = 0; ok && which < 2; ++which) {
+  for (int which = 0; ok && which < NPOW; ++which) {
     const uint64_t n = S.e[which].size(), np = S.probe[which].size();
     ok = std::fwrite(&n, 8, 1, f) == 1 && std::fwrite(&np, 8, 1, f) == 1 &&
          std::fwrite(S.e[which].data(), sizeof(Entry), n, f) == n &&
@@ -183,11 +186,12 @@ bool ensure_host(std::string& err) {
 }  // namespace
 
 // Host copy of the tables (for the host-side checks of the emulation).
-bool pow_tables_host(std::vector<uint2>& t13, std::vector<uint2>& t53, std::string& err) {
+bool pow_tables_host(std::vector<uint2>& t13, std::vector<uint2>& t53, std::vector<uint2>& t73, std::string& err) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!ensure_host(err)) return false;
   t13 = hash_table(g_set.e[0]);
   t53 = hash_table(g_set.e[1]);
+  t73 = hash_table(g_set.e[2]);
   return true;
 }
 
@@ -198,7 +202,7 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
   if (it != g_dev.end()) { out = it->second; return true; }
   if (!ensure_host(err)) return false;
   PowTables T{};
-  for (int which = 0; which < 2; ++which) {
+  for (int which = 0; which < NPOW; ++which) {
     std::vector<uint2> h = hash_table(g_set.e[which]);
     void* p = nullptr;
     if (hipMalloc(&p, h.size() * sizeof(uint2)) != hipSuccess ||
@@ -206,7 +210,7 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
       err = "upload of the powf tables failed";
       return false;
     }
-    (which == 0 ? T.t13 : T.t53) = (const uint2*)p;
+    (which == 0 ? T.t13 : (which == 1 ? T.t53 : T.t73)) = (const uint2*)p;
   }
   g_dev[device] = T;
   out = T;
@@ -216,7 +220,7 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
 }  // namespace sfx
 
 // Build (or load and verify) the powf tables; path NULL = next to the library.
-extern "C" int sfx_libm_tables(const char* path, int64_t counts[2]) {
+extern "C" int sfx_libm_tables(const char* path, int64_t counts[3]) {
   std::lock_guard<std::mutex> lk(sfx::g_mu);
   std::string err;
   if (path && *path) {
@@ -230,16 +234,17 @@ extern "C" int sfx_libm_tables(const char* path, int64_t counts[2]) {
     sfx::set_error(err);
     return SFX_ERR_STATE;
   }
-  if (counts) { counts[0] = (int64_t)sfx::g_set.e[0].size(); counts[1] = (int64_t)sfx::g_set.e[1].size(); }
+  if (counts)
+    for (int k = 0; k < sfx::NPOW; ++k) counts[k] = (int64_t)sfx::g_set.e[k].size();
   return SFX_OK;
 }
 
 // Host evaluation of the emulation, for tests: out[i] = powf(x[i], exponent `which`).
 extern "C" int sfx_libm_pow_host(const float* x, int64_t n, int32_t which, float* out) {
-  std::vector<uint2> a, b;
+  std::vector<uint2> a, b, c;
   std::string err;
-  if (!sfx::pow_tables_host(a, b, err)) { sfx::set_error(err); return SFX_ERR_STATE; }
-  sfx::PowTables T{a.data(), b.data()};
+  if (!sfx::pow_tables_host(a, b, c, err)) { sfx::set_error(err); return SFX_ERR_STATE; }
+  sfx::PowTables T{a.data(), b.data(), c.data()};
   for (int64_t i = 0; i < n; ++i) out[i] = sfx::libm_pow(x[i], which, T);
   return SFX_OK;
 }

@@ -39,6 +39,7 @@ extern "C" void sfx_params_default(sfx_params* p) {
   p->friction2d = 1; p->advection_mask = 1; p->wiggle_suppression = 1; p->store_zsmax = 1;
   p->store_velmax = 0; p->store_fluxmax = 0; p->store_twet = 0; p->twet_threshold = 0.01f;
   p->timestep_analysis = 0; p->use_qext = 0;
+  p->structure_relax = 10.0f;
   p->bndtype = 1; p->h73table = 0;
 }
 
@@ -526,6 +527,7 @@ extern "C" int sfx_model_sizes(const sfx_model* M, sfx_sizes* s) {
   s->nsrc = M->nsrc; s->nbnd = M->nbnd;
   s->subgrid = M->subgrid; s->quadtree = M->use_quadtree; s->dense = M->dense;
   s->dtmax = M->dtmax; s->dtmin = M->dtmin;
+  s->ndrn = M->ndrn;
   return SFX_OK;
 }
 
@@ -804,6 +806,35 @@ extern "C" int sfx_model_set_river_boundary(sfx_model* M, int32_t nbdr, const fl
   return SFX_OK;
 }
 
+extern "C" int sfx_model_set_drainage(sfx_model* M, int32_t ndrn, const float* x_in, const float* y_in,
+                                      const float* x_out, const float* y_out, const int32_t* type,
+                                      const float* params) {
+  if (!M || M->initialized || ndrn < 0 || (ndrn > 0 && (!x_in || !y_in || !x_out || !y_out || !type || !params))) {
+    set_error("sfx_model_set_drainage: bad arguments");
+    return SFX_ERR_ARG;
+  }
+  for (int32_t k = 0; k < ndrn; ++k) {
+    if (type[k] < 1 || type[k] > 5) {
+      set_error("drainage type " + std::to_string(type[k]) + " not recognized");
+      return SFX_ERR_ARG;
+    }
+    if ((type[k] == 4 || type[k] == 5) && !(params[6 * k + 5] > 0.0f && params[6 * k] > 0.0f)) {
+      set_error("a drainage gate needs a positive width and closing time");
+      return SFX_ERR_ARG;
+    }
+  }
+  M->ndrn = ndrn;
+  M->x_drn_in.assign(x_in, x_in + ndrn); M->y_drn_in.assign(y_in, y_in + ndrn);
+  M->x_drn_out.assign(x_out, x_out + ndrn); M->y_drn_out.assign(y_out, y_out + ndrn);
+  M->drainage_type.assign(type, type + ndrn);
+  M->drainage_params.assign(params, params + (size_t)6 * ndrn);
+  for (int32_t k = 0; k < ndrn; ++k) {   // unused parameters of pumps, culverts and valves: 0
+    if (type[k] <= 3)
+      for (int j = 1; j < 6; ++j) M->drainage_params[(size_t)6 * k + j] = 0.0f;
+  }
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_set_sources(sfx_model* M, int32_t nsrc, const float* x, const float* y,
                                      int32_t nt, const float* t, const float* q) {
   if (!M || M->initialized || nsrc < 0 || (nsrc > 0 && (!x || !y || !t || !q || nt < 2))) {
@@ -1022,11 +1053,22 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
     M->index_zsi_bdr[j] = c;
   }
 
-  // read_discharges cell lookup (sfincs_discharges.f90:159-171)
-  M->nmindsrc.assign(M->nsrc, 0);
+  // read_discharges cell lookup (sfincs_discharges.f90:159-171), then each drainage
+  // structure's intake and outfall cells and the distance between their centres (:258-280)
+  M->nmindsrc.assign(M->nsrc + 2 * M->ndrn, 0);
   for (int32_t i = 0; i < M->nsrc; ++i) {
     M->nmindsrc[i] = M->find_cell_xy(M->x_src[i], M->y_src[i]);
     if (M->nmindsrc[i] == 0) { set_error("point source outside the active mesh"); return SFX_ERR_ARG; }
+  }
+  M->drainage_distance.assign(M->ndrn, 0.f);
+  for (int32_t k = 0; k < M->ndrn; ++k) {
+    const int32_t a = M->find_cell_xy(M->x_drn_in[k], M->y_drn_in[k]);
+    const int32_t b = M->find_cell_xy(M->x_drn_out[k], M->y_drn_out[k]);
+    if (a == 0 || b == 0) { set_error("a drainage structure's intake or outfall lies outside the active mesh"); return SFX_ERR_ARG; }
+    M->nmindsrc[M->nsrc + 2 * k] = a;
+    M->nmindsrc[M->nsrc + 2 * k + 1] = b;
+    const float dxs = M->z_xz[b - 1] - M->z_xz[a - 1], dys = M->z_yz[b - 1] - M->z_yz[a - 1];
+    M->drainage_distance[k] = std::sqrt(dxs * dxs + dys * dys);
   }
   // discharge times must cover [t0, t1] (sfincs_discharges.f90:134-155)
   if (M->nsrc > 0) {
@@ -1117,6 +1159,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(zs) T(z_volume) T(q) T(uv) T(zs0) T(zsderv) T(zsmax) T(kfuv) T(vmax) T(qmax) T(twet)
   T(nmindbnd) T(zsb) T(zsb0) T(index_kcuv2) T(nmbkcuv2) T(nmikcuv2) T(ibkcuv2) T(ibuvdir) T(uvmean)
   T(ind1_bnd_gbp) T(ind2_bnd_gbp) T(fac_bnd_gbp) T(nmi_gbp) T(nmindsrc) T(cdval) T(x73)
+  T(drainage_type) T(drainage_params) T(drainage_distance)
   T(index_bdr_gbp) T(index_zsi_bdr) T(dzs_bdr) T(zsi_bnd) T(storage_volume)
   T(t_bnd) T(zs_bnd) T(tsrc) T(qsrc) T(tprcpt) T(tprcpv) T(twnd) T(wndmag) T(wnddir)
 #undef T

@@ -67,6 +67,8 @@ def write(sc, folder):
             lines.append(f"{'store' + k[6:]} = {int(v)}")
         elif k == "timestep_analysis":
             lines.append(f"timestep_analysis = {int(v)}")
+        elif k == "structure_relax":
+            lines.append(f"structure_relax = {_fmt(v)}")
         else:
             lines.append(f"{k} = {_fmt(v)}")
     if sc.subgrid is not None:
@@ -131,6 +133,14 @@ def write(sc, folder):
         with open(os.path.join(folder, "sfincs.bdr"), "w") as f:
             for row in zip(*sc.bdr):
                 f.write(" ".join(_fmt(v) for v in row) + "\n")
+    if getattr(sc, "drn", None) is not None:   # x_in y_in x_out y_out type params (sfincs_discharges.f90:193-224)
+        lines.append("drnfile = sfincs.drn")
+        xi, yi, xo, yo, ty, par = sc.drn
+        with open(os.path.join(folder, "sfincs.drn"), "w") as f:
+            for k in range(len(ty)):
+                npar = 1 if int(ty[k]) <= 3 else 6
+                f.write(" ".join(_fmt(v) for v in (xi[k], yi[k], xo[k], yo[k])) + f" {int(ty[k])} " +
+                        " ".join(_fmt(v) for v in par[k][:npar]) + "\n")
     if sc.src is not None:
         x, y, t, q = sc.src
         lines += ["srcfile = sfincs.src", "disfile = sfincs.dis"]

@@ -2,11 +2,14 @@
 
 run_batched:   members of uneven size advance together in one batch (one launch per phase
                per step over an (instance, tile) work list, graph-captured); each member
-               keeps its own adaptive dt and stops at its own end time.
+               keeps its own adaptive dt and stops at its own end time. Members join the
+               batch shortest expected work first, while the cells on the GPU stay below a
+               fill target, so small members are not held to the rounds of the largest.
 run_sequential: the same members one after the other, each on its own (the reference's
                way of running ensembles, docs/output.rst:19-20), for comparison.
-Server:        continuous batching -- requests are admitted as they arrive while their HBM
-               footprint fits the budget, advance in the shared batch while on the GPU,
+Server:        continuous batching -- arrived requests are admitted shortest expected work
+               first while their HBM footprint fits the budget and the cells on the GPU
+               stay below the fill target, advance in the shared batch while on the GPU,
                and retire (freeing their memory) when they reach their end time.
 
 Model building (table generation, mesh construction) happens on the host before a member
@@ -21,8 +24,22 @@ import numpy as np
 from . import _lib
 
 
+# Cells on the GPU at which one batch round keeps every CU busy: a 2048 x 2048 grid alone
+# runs the step at the device's full rate; smaller batches leave CUs idle, larger ones
+# only lengthen every member's rounds.
+FILL_CELLS = 4 << 20
+
+
 def _np(model):
     return model.sizes()["np"]
+
+
+def expected_work(model, t_end):
+    """Relative work to the end time: cells x simulated time / cell size (the CFL step
+    scales with the grid spacing)."""
+    table = getattr(model, "table", None)
+    dx = float(table("dxrm").min()) if table is not None else 1.0
+    return _np(model) * float(t_end) / max(dx, 1e-3)
 
 
 def _instances(models, device, tune_min_cells):
@@ -46,21 +63,32 @@ def _summary(models, insts, wall, done_at, keep_state):
     return out
 
 
-def run_batched(models, t_ends, round_steps=32, device=0, tune_min_cells=1 << 20, keep_state=False):
-    """All members in one batch until each reaches its end time."""
+def run_batched(models, t_ends, round_steps=32, device=0, tune_min_cells=1 << 20, keep_state=False,
+                fill_cells=FILL_CELLS):
+    """Every member through one batch until each reaches its end time. Members join
+    shortest expected work first while the cells in the batch stay below `fill_cells`
+    (at least one member is always in); a member leaves the batch when it is done."""
     insts = _instances(models, device, tune_min_cells)
+    order = sorted(range(len(insts)), key=lambda j: expected_work(models[j], t_ends[j]))
     batch = _lib.Batch(device)
-    for te, inst in zip(t_ends, insts):
-        batch.add(inst, te)
     done_at = [None] * len(insts)
-    active = len(insts)
+    live = []
+    cells = 0
     t0 = time.perf_counter()
-    while active:
-        active = batch.advance(round_steps)
+    while order or live:
+        while order and (not live or cells + _np(models[order[0]]) <= fill_cells):
+            j = order.pop(0)
+            batch.add(insts[j], t_ends[j])
+            live.append(j)
+            cells += _np(models[j])
+        batch.advance(round_steps)
         now = time.perf_counter() - t0
-        for j, inst in enumerate(insts):
-            if done_at[j] is None and batch.done(inst):
+        for j in list(live):
+            if batch.done(insts[j]):
                 done_at[j] = now
+                batch.remove(insts[j])
+                live.remove(j)
+                cells -= _np(models[j])
     wall = time.perf_counter() - t0
     out = _summary(models, insts, wall, done_at, keep_state)
     batch.close()
@@ -92,8 +120,9 @@ class Server:
     at a time, and retires finished members. Host-side model building overlaps the GPU
     work in a thread pool."""
 
-    def __init__(self, hbm_budget, round_steps=32, device=0, workers=4, keep_state=False):
+    def __init__(self, hbm_budget, round_steps=32, device=0, workers=4, keep_state=False, fill_cells=FILL_CELLS):
         self.budget = hbm_budget
+        self.fill_cells = fill_cells
         self.round_steps = round_steps
         self.device = device
         self.pool = cf.ThreadPoolExecutor(max_workers=workers)
@@ -111,9 +140,10 @@ class Server:
         clock = clock or time.perf_counter
         t0 = clock()
         batch = _lib.Batch(self.device)
-        pending = []             # (arrival, seq, future-model, scenario) in arrival order
+        pending = []             # (arrival, seq, future-model, scenario), arrived, not yet on the GPU
         live = {}                # seq -> (inst, model, arrival, footprint, scenario, admitted)
         used = 0
+        on_gpu = 0               # cells of the requests on the GPU
         results = []
         max_live = 0
         max_used = 0
@@ -123,19 +153,24 @@ class Server:
             while self.queue and self.queue[0][0] <= now:
                 arrival, seq, sc = heapq.heappop(self.queue)
                 pending.append((arrival, seq, self.pool.submit(sc.build_model), sc))
+            # admission: among the arrived requests whose model is built, shortest expected
+            # work first, while the memory fits and the GPU is not already full
             while pending:
-                arrival, seq, fut, sc = pending[0]
-                if not fut.done():
+                ready = [k for k, p in enumerate(pending) if p[2].done()]
+                if not ready:
                     break
+                k = min(ready, key=lambda r: expected_work(pending[r][2].result(), pending[r][3].t_end))
+                arrival, seq, fut, sc = pending[k]
                 model = fut.result()
                 need = model.footprint()
-                if live and used + need > self.budget:
+                if live and (used + need > self.budget or on_gpu + _np(model) > self.fill_cells):
                     break
-                pending.pop(0)
+                pending.pop(k)
                 inst = _lib.Instance(model, self.device)
                 batch.add(inst, sc.t_end)
                 live[seq] = (inst, model, arrival, need, sc, clock() - t0)
                 used += need
+                on_gpu += _np(model)
                 if progress:
                     progress("admit", dict(seq=seq, live=len(live), used=used, pending=len(pending),
                                            queued=len(self.queue)))
@@ -144,7 +179,7 @@ class Server:
             if not live:
                 # nothing on the GPU: wait for the next model or arrival
                 if pending:
-                    cf.wait([pending[0][2]], timeout=0.05)
+                    cf.wait([p[2] for p in pending], timeout=0.05, return_when=cf.FIRST_COMPLETED)
                 elif self.queue:
                     time.sleep(max(0.0, min(0.05, self.queue[0][0] - (clock() - t0))))
                 continue
@@ -164,6 +199,7 @@ class Server:
                         progress("done", dict(seq=seq, ttr=r["ttr"], steps=nt, done=len(results)))
                     batch.remove(inst)
                     inst.close()
+                    on_gpu -= model.sizes()["np"]
                     model.close()
                     used -= need
                     del live[seq]

@@ -21,6 +21,7 @@ module sfx_capi
       integer(c_int32_t) :: store_velmax, store_fluxmax, store_twet
       real(c_float) :: twet_threshold
       integer(c_int32_t) :: timestep_analysis, use_qext
+      real(c_float) :: structure_relax
    end type
    type, bind(C) :: sfx_quadtree_input
       integer(c_int32_t) :: nr_points, nr_levels
@@ -94,6 +95,13 @@ module sfx_capi
          type(c_ptr), value :: m
          integer(c_int32_t), value :: nbdr
          type(c_ptr), value :: x, y, xi, yi, slope, dist
+      end function
+      integer(c_int) function sfx_model_set_drainage(m, ndrn, xi, yi, xo, yo, ty, par) &
+            bind(C, name='sfx_model_set_drainage')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: ndrn
+         type(c_ptr), value :: xi, yi, xo, yo, ty, par
       end function
       integer(c_int) function sfx_model_set_rain(m, nt, t, v) bind(C, name='sfx_model_set_rain')
          import :: c_int, c_ptr, c_int32_t
@@ -426,7 +434,10 @@ contains
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout, tdum
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
       character(256) :: bndfile, bzsfile, bzifile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
-      character(256) :: sbgfile
+      character(256) :: sbgfile, drnfile
+      character(512) :: line
+      integer(c_int32_t), allocatable, target :: drn_type(:)
+      real(c_float), allocatable, target :: drn_par(:,:)
       integer(c_int8_t), allocatable, target :: msk(:), mskg(:,:)
       integer(c_int32_t), allocatable :: indices(:)
       real(c_float), allocatable, target :: zbg(:,:), zb(:), xb(:), yb(:), tb(:), zsb(:,:), tmp(:), zsi(:,:)
@@ -521,6 +532,8 @@ contains
       call inp_str('rstfile', rstfile, 'none'); call inp_str('bdrfile', bdrfile, 'none')
       call inp_str('bzifile', bzifile, 'none')
       call inp_str('sbgfile', sbgfile, 'none')
+      call inp_str('drnfile', drnfile, 'none')
+      call inp_real('structure_relax', p%structure_relax)
 
       if (qtrfile(1:4) /= 'none') then
          call read_quadtree(qtrfile, mskfile, p, sbgfile(1:4) == 'none', rc)
@@ -676,6 +689,40 @@ contains
                                     c_loc(tt), c_loc(zsb))
          if (rc < 0) call fail('discharges')
          deallocate(tt)
+      endif
+      ! drainage structures: x_in y_in x_out y_out type and 1 (types 1-3) or 6 (gates)
+      ! parameters per line (sfincs_discharges.f90:176-224)
+      if (drnfile(1:4) /= 'none') then
+         nb = count_lines(drnfile)
+         if (allocated(xb)) deallocate(xb, yb)
+         if (allocated(tt)) deallocate(tt)
+         if (allocated(v1)) deallocate(v1)
+         allocate(xb(nb), yb(nb), tt(nb), v1(nb), drn_type(nb), drn_par(6, nb))
+         drn_par = 0.0
+         open(newunit=u, file=trim(drnfile), status='old', action='read')
+         do i = 1, nb
+            read(u, '(a)') line
+            read(line, *) xb(i), yb(i), tt(i), v1(i), drn_type(i)
+            if (drn_type(i) >= 4) then
+               read(line, *, iostat=ios) xb(i), yb(i), tt(i), v1(i), drn_type(i), (drn_par(j, i), j = 1, 6)
+            else
+               read(line, *, iostat=ios) xb(i), yb(i), tt(i), v1(i), drn_type(i), drn_par(1, i)
+            endif
+            if (ios /= 0) then
+               call logmsg('Error   : drainage structure line with too few parameters: '//trim(line))
+               ierr = -1
+               return
+            endif
+         enddo
+         close(u)
+         rc = sfx_model_set_drainage(model, int(nb, c_int32_t), c_loc(xb), c_loc(yb), c_loc(tt), c_loc(v1), &
+                                     c_loc(drn_type), c_loc(drn_par))
+         if (rc < 0) then
+            call fail('drainage structures')
+            ierr = -1
+            return
+         endif
+         deallocate(tt, v1, drn_type, drn_par)
       endif
       ! downstream river points: x y x_in y_in slope distance (sfincs_boundaries.f90:221-268)
       if (bdrfile(1:4) /= 'none') then

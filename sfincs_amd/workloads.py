@@ -42,6 +42,7 @@ class Scenario:
     bdr: tuple = None                     # river boundary points (x, y, x_in, y_in, slope, distance)
     bzi: np.ndarray = None                # infragravity levels [nt, nbnd] at the bnd series times
     storage: object = None                # callable(model) -> storage volume per cell (m3), or None
+    drn: tuple = None                     # drainage structures (x_in, y_in, x_out, y_out, type, params[n, 6])
 
     def build_model(self):
         p = _lib.default_params(**self.params)
@@ -71,6 +72,8 @@ class Scenario:
             m.set_wind(*self.wind)
         if self.bdr is not None:
             m.set_river_boundary(*self.bdr)
+        if self.drn is not None:
+            m.set_drainage(*self.drn)
         if self.storage is not None:
             m.set_storage_volume(self.storage(m))
         if self.rstfile is not None:
@@ -371,6 +374,43 @@ def regular_sources(sc):
         return (m + 0.5) * sc.dx, (n + 0.5) * sc.dy
     pts = [pick(2.0, 99.0, 3), pick(2.0, 99.0, 40), pick(0.2, 1.0, 7), pick(-99.0, -5.0, 11)]
     return with_sources(sc, [p[0] for p in pts], [p[1] for p in pts], [2.0, -1.0, -3.0, -5.0])
+
+
+def with_drainage(sc, structures):
+    """Adds drainage structures (drnfile): a list of (x_in, y_in, x_out, y_out, type,
+    params) with up to 6 params (type 1 pump [Q]; 2 culvert, 3 check valve [coefficient];
+    4 gate [width, sill, n, zmin, zmax, closing time]; 5 gate [width, sill, n, t_close,
+    t_open, closing time])."""
+    n = len(structures)
+    par = np.zeros((n, 6), np.float32)
+    for k, st in enumerate(structures):
+        par[k, :len(st[5])] = st[5]
+    col = lambda j: np.array([st[j] for st in structures], np.float32)
+    sc.drn = (col(0), col(1), col(2), col(3), np.array([st[4] for st in structures], np.int32), par)
+    sc.name += "_drn"
+    return sc
+
+
+def beach_drainage(sc):
+    """Five structures on a configuration-0 style beach (sea in the west, land rising to the
+    east): a pump lifting sea water onto land, a culvert and a check valve between the sea
+    and a coastal cell, a gate that closes when the sea rises above 0.5 m, and a gate closed
+    from 600 s to 1500 s."""
+    dx = sc.dx
+    ny = sc.nmax * dx
+
+    def pt(col, row):
+        return (col + 0.5) * dx, (row + 0.5) * dx
+    rows = [int(sc.nmax * f) for f in (0.15, 0.3, 0.45, 0.6, 0.8)]
+    sea, coast, land = 3, 18, 40
+    out = []
+    out.append((*pt(sea, rows[0]), *pt(land, rows[0]), 1, [2.0]))
+    out.append((*pt(sea, rows[1]), *pt(coast, rows[1]), 2, [5.0]))
+    out.append((*pt(sea, rows[2]), *pt(coast, rows[2]), 3, [5.0]))
+    out.append((*pt(sea, rows[3]), *pt(coast, rows[3]), 4, [4.0, -3.0, 0.03, -1.0, 0.5, 300.0]))
+    out.append((*pt(sea, rows[4]), *pt(coast, rows[4]), 5, [4.0, -3.0, 0.03, 600.0, 1500.0, 200.0]))
+    del ny
+    return with_drainage(sc, out)
 
 
 def with_river(sc, first=1, ncells=8, slope=1.0e-3):

@@ -45,9 +45,9 @@ def _diff(inst, orc, subgrid):
     return d
 
 
-def _oracle_from_gpu(sc, model, inst):
+def _oracle_from_gpu(sc, model, inst, min_dt):
     """A fresh oracle holding the GPU's state: fields, model time, step count and the
-    pending CFL minimum (dt = alpha * min_dt)."""
+    pending CFL minimum of the last momentum sweep (the next dt is alpha * min_dt)."""
     orc = O.OracleRun(model).configure(sc.params, model)
     fields = ["zs", "q", "uv", "kfuv", "zs0", "zsderv", "zsmax", "uvmean"]
     if model.sizes()["subgrid"]:
@@ -57,7 +57,7 @@ def _oracle_from_gpu(sc, model, inst):
     t, dt, nt = inst.time()
     orc.set("t", t)
     orc.set("nt", nt)
-    orc.set("min_dt", float(np.float32(dt) / np.float32(sc.params.get("alpha", 0.5))))
+    orc.set("min_dt", float(min_dt))
     return orc
 
 
@@ -116,10 +116,10 @@ def test_config1_window_at_tide_crest_matches_oracle_and_reference(config1):
     sc, m = config1
     _say("case config1 2048x2048 subgrid, window at t = 3 h")
     inst = gpu_for(m)
-    inst.update(10800.0)
+    rc, st = inst.update(10800.0)
     t_start, _, nt0 = inst.time()
-    orc = _oracle_from_gpu(sc, m, inst)
-    ref = _oracle_from_gpu(sc, m, inst) if O.ref_available() else None
+    orc = _oracle_from_gpu(sc, m, inst, st["min_dt"])
+    ref = _oracle_from_gpu(sc, m, inst, st["min_dt"]) if O.ref_available() else None
     _gpu_steps(inst, WINDOW)
     t0 = time.perf_counter()
     orc.update(1.0e30, WINDOW)

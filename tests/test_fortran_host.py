@@ -83,7 +83,7 @@ SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax",
 
 @needs_host
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves",
-                                  "subgrid_binary"])
+                                  "subgrid_binary", "drainage"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
@@ -102,6 +102,9 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     elif kind == "subgrid_binary":
         sc = W.with_subgrid_binary(W.subgrid_case(22, 26, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0),
                                    tmp_path / "gen.sbg")
+    elif kind == "drainage":
+        sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5,
+                                             structure_relax=7.5))
     else:
         sc = W.subgrid_case(20, 24, dx=25.0, seed=43, t_end=900.0, nsrc=2, rain_mmh=10.0)
         sc.subgrid = None
@@ -120,7 +123,11 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     mf = host.sfincs_host_model()
     mp = sc.build_model()
     lib = _lib.lib()
-    for name in TABLES + (SUBGRID_TABLES if kind == "subgrid_binary" else ()):
+    extra = SUBGRID_TABLES if kind == "subgrid_binary" else ()
+    if kind == "drainage":
+        extra = ("drainage_type", "drainage_params", "drainage_distance")
+        assert mp.sizes()["ndrn"] == 5 and mp.params().structure_relax == 7.5
+    for name in TABLES + extra:
         a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
         assert (a is None) == (b is None), name
         assert a == b, name
@@ -130,7 +137,7 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
 
 @needs_host
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage"])
 def test_fortran_host_matches_python_path(kind, tmp_path):
     print(f"case fortran host {kind}", flush=True)
     from sfincs_amd import _lib
@@ -139,6 +146,8 @@ def test_fortran_host_matches_python_path(kind, tmp_path):
     elif kind == "subgrid_binary":   # a subgrid deck through sfincs_initialize (sbgfile)
         sc = W.with_subgrid_binary(W.subgrid_case(30, 34, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0),
                                    tmp_path / "gen.sbg")
+    elif kind == "drainage":         # drnfile
+        sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5))
     else:
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     d = deck.write(sc, str(tmp_path))
@@ -339,13 +348,13 @@ def test_bmi_bed_and_level_writes(tmp_path):
     finally:
         os.chdir(cwd)
     inst = _lib.Instance(sc.build_model())
-    inst.update(300.0)
+    inst.update_to(300.0)
     inst.set("zb", zb_new)
     inst.update_zbuvmx()
-    inst.update(600.0)
+    inst.update_to(900.0)
     assert np.array_equal(inst.get("zs"), zs_mid)
     inst.set("zs", zs_new)
-    inst.update(600.0)
+    inst.update_to(1500.0)
     assert np.array_equal(zs_f, inst.get("zs"))
     assert np.array_equal(inst.get("zb"), zb_new)
 

@@ -3,7 +3,8 @@ the h^(7/3) table (sfincs_momentum.f90:787-820), point sources on regular and qu
 meshes with the bed clamp of the regular branch (sfincs_continuity.f90:87-106), Neumann
 side boundaries (msk = 6, sfincs_boundaries.f90:903-913), boundary spin-up
 (:789-826 with weighted_average :1179-1200), a four-level quadtree with wind, and the
-instability stop (sfincs_lib.f90:634-648).
+instability stop (sfincs_lib.f90:634-648), and drainage structures
+(sfincs_discharges.f90:371-655) on regular and subgrid meshes.
 
 Gate: identical step count and model time, max|dzs| <= 1e-4 m, max|dq| <= 1e-5 m2/s."""
 import numpy as np
@@ -122,3 +123,20 @@ def test_instability_stop_matches_oracle(kind):
     assert rc == 1 and st["error"] == 1 and st["nt"] == int(orc.get("nt"))
     d = compare_state(inst, orc)
     assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL, d
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+def test_drainage_structures_match_oracle(kind):
+    """Pump, culvert, check valve and both gate kinds (the gates' h^(7/3) through the
+    C library's powf); GPU against the oracle over the whole period."""
+    print("case drainage", kind, flush=True)
+    if kind == "regular":
+        sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=1800.0, tide_amp=2.5, rain_mmh=0.0))
+    else:
+        from test_drainage import subgrid_drainage_case
+        sc = subgrid_drainage_case()
+    m, inst, orc = _full_run(sc)
+    s = m.sizes()
+    assert s["ndrn"] >= 3
+    flows = orc.state("qtsrc")[s["nsrc"]:]
+    assert np.count_nonzero(flows) >= 2          # at least one structure still passes water

@@ -1,6 +1,7 @@
-"""x**(1/3) and x**(5/3) with the C library's exact powf results (sfx_libm.hpp).
+"""x**(1/3), x**(5/3) and x**(7/3) with the C library's exact powf results (sfx_libm.hpp).
 
-The reference evaluates hu**expo and hu**(5.0/3.0) with the C library's powf; the
+The reference evaluates hu**expo, hu**(5.0/3.0) and hgate**(7.0/3.0) with the C
+library's powf; the
 emulation must return the same float for every input the tables cover. Checked here on
 the host implementation over every table key and a dense random sample, and on the
 device (GPU test) over the same inputs."""
@@ -14,7 +15,7 @@ from sfincs_amd import _lib
 libm = C.CDLL("libm.so.6")
 libm.powf.restype = C.c_float
 libm.powf.argtypes = [C.c_float, C.c_float]
-EXP = [np.float32(1.0) / np.float32(3.0), np.float32(5.0) / np.float32(3.0)]
+EXP = [np.float32(1.0) / np.float32(3.0), np.float32(5.0) / np.float32(3.0), np.float32(7.0) / np.float32(3.0)]
 
 
 def glibc_powf(x, which):
@@ -39,14 +40,14 @@ def emulate(x, which, device=False):
     return out
 
 
-@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("which", [0, 1, 2])
 def test_host_emulation_equals_glibc(which):
     x = sample(200000, 7 + which)
     assert np.array_equal(emulate(x, which).view(np.uint32), glibc_powf(x, which).view(np.uint32))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("which", [0, 1, 2])
 def test_device_emulation_equals_glibc(which):
     x = sample(2000000, 17 + which)
     want = emulate(x, which)            # host emulation, equal to glibc (test above)

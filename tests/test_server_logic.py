@@ -108,3 +108,42 @@ def test_oversized_request_runs_alone(monkeypatch):
     srv.submit(0.0, Req(FakeModel(10, steps=20, footprint=500)))
     out = srv.run()
     assert out["requests"] == 2 and out["max_concurrent"] == 1
+
+
+def test_shortest_work_first_within_the_fill_target(monkeypatch):
+    """Requests that have arrived join shortest expected work first, and no more join
+    while the cells on the GPU would pass the fill target (unless the GPU is empty)."""
+    joined = []
+
+    class LoggingBatch(FakeBatch):
+        def add(self, inst, t_end):
+            joined.append(inst.model.sizes()["np"])
+            super().add(inst, t_end)
+
+    monkeypatch.setattr(E._lib, "Instance", FakeInstance)
+    monkeypatch.setattr(E._lib, "Batch", LoggingBatch)
+    FakeInstance.live = FakeInstance.peak = 0
+    srv = E.Server(hbm_budget=10 ** 9, round_steps=10, workers=1, keep_state=True, fill_cells=250)
+    srv.submit(0.0, Req(FakeModel(10, steps=200, footprint=10)))     # on the GPU first, long
+    sizes = [200, 50, 120, 60, 300]
+    for n in sizes:
+        srv.submit(0.0, Req(FakeModel(n, steps=40, footprint=10)))
+    out = srv.run()
+    assert out["requests"] == 6
+    assert joined[0] == 10
+    assert joined[1:4] == [50, 60, 120]          # 10 + 50 + 60 + 120 <= 250; adding 200 would pass it
+    assert joined[4:] == [200, 300]
+    assert FakeInstance.peak <= 4                # never more cells than the target (besides a lone request)
+
+
+def test_run_batched_joins_small_members_first(monkeypatch):
+    monkeypatch.setattr(E._lib, "Instance", FakeInstance)
+    monkeypatch.setattr(E._lib, "Batch", FakeBatch)
+    FakeInstance.live = FakeInstance.peak = 0
+    sizes = [400, 100, 300, 50]
+    models = [FakeModel(n, steps=30, footprint=1) for n in sizes]
+    out = E.run_batched(models, [1.0] * 4, round_steps=10, tune_min_cells=None, keep_state=True, fill_cells=450)
+    assert out["steps"] == [30] * 4
+    done = out["time_to_result_s"]
+    assert done[3] <= done[1] <= done[2] <= done[0]           # the small members finish first
+    assert FakeInstance.peak == 4                              # instances exist for every member

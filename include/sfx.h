@@ -272,6 +272,14 @@ int  sfx_get_counters(sfx_handle* h, int64_t counts[6]);
 int  sfx_get_store_counters(sfx_handle* h, int64_t counts[6]);
 /* FP64 volume accumulator: total water volume (sum of zs*A or V) in double. */
 int  sfx_total_volume(sfx_handle* h, double* vol);
+/* sfx_total_volume without allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc). Writes the total volume to `vol_device`, a double in device
+ * memory that the caller owns, asynchronously on the instance's stream (synchronize
+ * before reading it). Its partial sums live in a static device array of 4096 doubles per
+ * GPU, so meshes of at most 67,108,864 cells (larger: SFX_ERR_ARG); calls that overlap on
+ * one GPU share that array and race. */
+int  sfx_total_volume_noalloc(sfx_handle* h, double* vol_device);
 /* FP64 volume balance of the interior cells (kcs 1). Every continuity sweep adds, in
  * double, the volume it moved into the interior cells through faces whose other side is
  * not interior (water-level, river and Neumann boundaries, inactive cells), from point

@@ -105,6 +105,7 @@ def _sig(lib):
         "sfx_get_balance": (I, [P, C.POINTER(d)]),
         "sfx_reset_balance": (I, [P]),
         "sfx_update_zbuvmx": (I, [P]),
+        "sfx_total_volume_noalloc": (I, [P, P]),
         "sfx_batch_create": (I, [i32, C.POINTER(P)]),
         "sfx_batch_destroy": (I, [P]),
         "sfx_batch_add": (I, [P, P, d]),

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2813,6 +2814,83 @@ extern "C" int sfx_reset_balance(sfx_handle* H) {
   hipLaunchKernelGGL(k_zero_bal, dim3(1), dim3(64), 0, H->stream, H->hI.bal, 0, 6);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(H->stream));
+  return SFX_OK;
+}
+
+// ------------------------------------------------------- volume sum without allocation
+// sfx_total_volume_noalloc: the volume sum of sfx_total_volume for callers that cannot
+// allocate (graph capture, serving loops). Its partial sums live in `scratch`, one slot
+// per block of VOL_CHUNK cells, and a one-wave kernel adds them in block order into the
+// caller's device double, so the result is the same bits on every call. The array is
+// allocated with the library image, one copy per GPU, and lives until the process ends;
+// calls that overlap on one GPU share it and race.
+constexpr int VOL_MAX_BLOCKS = 4096;
+constexpr int VOL_CHUNK = BLOCK * 64;   // cells per block: at most 4096 * 16384 = 67,108,864 cells
+static __device__ __align__(256) double scratch[VOL_MAX_BLOCKS];
+
+namespace {
+__global__ void __launch_bounds__(BLOCK) k_volume_partial(const DevInst* __restrict__ I0, double* part) {
+  const DevInst& I = *I0;
+  const int c0 = blockIdx.x * VOL_CHUNK;
+  const int c1 = std::min(I.np, c0 + VOL_CHUNK);
+  double acc = 0.0;
+  for (int c = c0 + (int)threadIdx.x; c < c1; c += BLOCK) {
+    const int k = G(I.kcs)[c] & KCS_MASK;
+    if (k == 0) continue;
+    const int iref = I.dense ? 1 : G(I.z_iref)[c];
+    acc += I.subgrid ? G(I.vol)[c] : G(I.zs)[c] * (double)I.c.cell_area[iref - 1];
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
+  __shared__ double red[BLOCK / WAVE];
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < BLOCK / WAVE; ++i) s += red[i];
+    part[blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(WAVE) k_volume_final(const double* part, int nb, double* out) {
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += WAVE) acc += part[b];
+  for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
+  if (threadIdx.x == 0) *out = acc;
+}
+
+// Device address of `scratch` per device, taken once with that device current.
+constexpr int MAX_DEVICES = 64;
+double* scratch_ptr[MAX_DEVICES] = {};
+std::mutex scratch_mu;
+
+int scratch_of(int device, double** out) {
+  if (device < 0 || device >= MAX_DEVICES) { set_error("sfx_total_volume_noalloc: device index out of range"); return SFX_ERR_ARG; }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_ptr[device] = (double*)p;
+  }
+  *out = scratch_ptr[device];
+  return SFX_OK;
+}
+}  // namespace
+
+extern "C" int sfx_total_volume_noalloc(sfx_handle* H, double* vol_device) {
+  if (!H || !vol_device) { set_error("sfx_total_volume_noalloc: null argument"); return SFX_ERR_ARG; }
+  const int nb = (H->hI.np + VOL_CHUNK - 1) / VOL_CHUNK;
+  if (nb > VOL_MAX_BLOCKS) {
+    set_error("sfx_total_volume_noalloc: the mesh has " + std::to_string(H->hI.np) + " cells, more than the " +
+              std::to_string((long long)VOL_MAX_BLOCKS * VOL_CHUNK) + " its static scratch covers");
+    return SFX_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(H->device));
+  double* part = nullptr;
+  int rc = scratch_of(H->device, &part);
+  if (rc) return rc;
+  if (nb > 0) hipLaunchKernelGGL(k_volume_partial, dim3(nb), dim3(BLOCK), 0, H->stream, H->dI, part);
+  hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(WAVE), 0, H->stream, part, nb, vol_device);
+  HIPCHK(hipGetLastError());
   return SFX_OK;
 }
 

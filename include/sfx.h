@@ -116,6 +116,8 @@ typedef struct sfx_sizes {
   int32_t subgrid, quadtree, dense;   /* dense = full rectangle, implicit neighbours on device */
   float dtmax, dtmin;
   int32_t ndrn;                       /* drainage structures (each adds an intake and an outfall point) */
+  int32_t nstruc;                     /* weir uv-points (kcuv 3) */
+  int32_t nthd;                       /* uv-points closed by thin dams (kcuv 0) */
 } sfx_sizes;
 
 /* ---- host-side model (CPU only; no GPU needed) -------------------------------------
@@ -163,6 +165,21 @@ int  sfx_model_set_sources(sfx_model* m, int32_t nsrc, const float* x, const flo
  * outfall discharges join the point sources in continuity. */
 int  sfx_model_set_drainage(sfx_model* m, int32_t ndrn, const float* x_in, const float* y_in, const float* x_out,
                             const float* y_out, const int32_t* type, const float* params);
+/* Thin dams (thdfile; read_thin_dams, sfincs_structures.f90:385-495): npoly polylines
+ * with nv[i] vertices each, coordinates concatenated in x/y. At initialize every uv-point
+ * whose cell-centre connection a polyline segment crosses
+ * (find_uv_points_intersected_by_polyline, sfincs_quadtree.F90:853-1010) gets kcuv 0:
+ * no flow through it. Table thindam_uv_index (1-based). */
+int  sfx_model_set_thin_dams(sfx_model* m, int32_t npoly, const int32_t* nv, const float* x, const float* y);
+/* Weirs (weirfile; read_structure_file with type 1, sfincs_structures.f90:38-381): as thin
+ * dams, with par[2*k] = crest level and par[2*k + 1] = discharge coefficient Cd at each
+ * vertex k. A crossed uv-point whose interpolated crest lies above its bed (uv_zmin on
+ * subgrid meshes, zbuvmx on regular ones) gets kcuv 3, and each step its discharge is the
+ * broad-crested weir formula (compute_fluxes_over_structures, :592-697) times the crest
+ * length across it. Tables structure_uv_index, structure_height, structure_cd,
+ * structure_length (relative to the grid spacing). */
+int  sfx_model_set_weirs(sfx_model* m, int32_t npoly, const int32_t* nv, const float* x, const float* y,
+                         const float* par);
 /* prcfile: uniform rain series in mm/h. */
 /* Storage volume per cell, m3 (volfile; initialize_storage_volume, sfincs_domain.f90:2063-2112):
  * on subgrid meshes water entering a cell fills its storage before the cell's volume grows

@@ -68,6 +68,7 @@ def olib():
         _olib.oracle_compute_fluxes.argtypes = [C.c_void_p, C.c_float]
         _olib.oracle_compute_fluxes.restype = C.c_float
         _olib.oracle_compute_water_levels.argtypes = [C.c_void_p, C.c_double, C.c_float]
+        _olib.oracle_compute_fluxes_over_structures.argtypes = [C.c_void_p]
         for f in ("oracle_update_meteo", "oracle_update_boundaries", "oracle_update_discharges"):
             getattr(_olib, f).argtypes = [C.c_void_p, C.c_double, C.c_float]
         _olib.oracle_update.argtypes = [C.c_void_p, C.c_double, C.c_int32]
@@ -132,6 +133,10 @@ class OracleRun:
         self._bind("qtsrc", np.zeros(max(1, s["nsrc"] + 2 * s["ndrn"]), np.float32))
         self._bind("drainage_status", np.ones(max(1, s["ndrn"]), np.int32))           # open
         self._bind("drainage_fraction_open", np.ones(max(1, s["ndrn"]), np.float32))
+        for n, dt in (("structure_uv_index", np.int32), ("structure_height", np.float32),
+                      ("structure_cd", np.float32), ("structure_length", np.float32)):
+            a = model.table(n)
+            self._bind(n, a if a.size else np.zeros(1, dt))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
         self._bind("x73", x73_table())
@@ -171,7 +176,7 @@ class OracleRun:
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
                     nsrc=s["nsrc"], ntsrc=len(tb("tsrc")), ntprcp=len(tb("tprcpt")), ntwnd=len(tb("twnd")),
-                    ndrn=s["ndrn"], structure_relax=p.structure_relax,
+                    ndrn=s["ndrn"], structure_relax=p.structure_relax, nstruc=s["nstruc"],
                     g=p.g, pi=p.pi, huthresh=model.params().huthresh, huvmin=p.huvmin,
                     advlim=p.advlim, slopelim=p.slopelim, uvlim=p.uvlim, nuviscfac=p.nuviscfac, fcorio=fcorio,
                     theta=p.theta, wiggle_factor=p.wiggle_factor, wiggle_threshold=p.wiggle_threshold,
@@ -335,6 +340,15 @@ class RefRun:
         o.update_forcing(t, dt)
         self.push_state()
         mdt_new = self.R.ref_compute_fluxes(dt)
+        if o.vals["nstruc"]:
+            # the weirs (compute_fluxes_over_structures, sfincs_structures.f90:592-697; its
+            # module needs NetCDF, so the restatement runs here) between the two routines
+            k = o.keep
+            for n in ("q", "uv"):
+                self.R.ref_get(n.encode(), k[n].ctypes.data, k[n].nbytes)
+            o.L.oracle_compute_fluxes_over_structures(o.m)
+            for n in ("q", "uv"):
+                self.R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
         self.R.ref_compute_water_levels(t, dt)
         self.pull_state()
         o.set("min_dt", mdt_new)

@@ -14,6 +14,8 @@
  *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
  *   oracle_update_discharges     sfincs_discharges.f90:325-655 (point sources and drainage
  *                                structures)
+ *   oracle_compute_fluxes_over_structures
+ *                                sfincs_structures.f90:592-697 (weirs)
  *   oracle_update                sfincs_lib.f90:337-683 (step order, lagged dt)
  * Arrays use 0-based storage; index tables hold the reference's 1-based values.
  * Each function's parity check is named in the test files under tests/.
@@ -80,6 +82,10 @@ typedef struct OM {
   const float *drainage_params, *drainage_distance;   /* params: 6 per structure */
   int32_t *drainage_status;
   float *drainage_fraction_open;
+  /* weirs (sfincs_structures.f90:38-381, 592-697): uv-point (1-based), crest, Cd, length */
+  int32_t nstruc;
+  const int32_t *structure_uv_index;
+  const float *structure_height, *structure_cd, *structure_length;
   /* time state */
   double t;
   float dt, min_dt;
@@ -113,6 +119,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsi_bnd) BIND(zsit_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
   BIND(nmindsrc) BIND(tsrc) BIND(qsrc) BIND(tprcpt) BIND(tprcpv) BIND(twnd) BIND(wndmag) BIND(wnddir) BIND(cdval) BIND(qtsrc) BIND(x73)
   BIND(drainage_type) BIND(drainage_params) BIND(drainage_distance) BIND(drainage_status) BIND(drainage_fraction_open)
+  BIND(structure_uv_index) BIND(structure_height) BIND(structure_cd) BIND(structure_length)
   return -1;
 }
 #undef BIND
@@ -123,7 +130,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(np) SETI(npuv) SETI(ncuv) SETI(nref) SETI(nl) SETI(subgrid) SETI(use_quadtree) SETI(advection)
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
-  SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error) SETI(ndrn) SETF(structure_relax)
+  SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error) SETI(ndrn) SETF(structure_relax) SETI(nstruc)
   SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
@@ -751,6 +758,31 @@ static void update_drainage(OM* m, double t, float dt) {
   }
 }
 
+/* ---- weirs: compute_fluxes_over_structures, sfincs_structures.f90:592-697 ------------- */
+/* Broad-crested weir at each structure uv-point from the levels of its two cells, as
+ * real*4: no flow with both below the crest; submerged (h2 > 2/3 h1) Cd h2 sqrt(2 g (h1 - h2)),
+ * else free Cd 1.7049 h1**1.5 (the C library's powf); times the crest length; uv = 0. */
+void oracle_compute_fluxes_over_structures(OM* m) {
+  for (int32_t k = 0; k < m->nstruc; ++k) {
+    const int32_t ip = m->structure_uv_index[k] - 1;
+    m->q[ip] = 0.0f;
+    m->uv[ip] = 0.0f;
+    const int32_t nm = m->uv_index_z_nm[ip] - 1, nmu = m->uv_index_z_nmu[ip] - 1;
+    const float zsnm = (float)m->zs[nm], zsnmu = (float)m->zs[nmu];
+    const float zc = m->structure_height[k];
+    if (zsnm < zc && zsnmu < zc) continue;
+    const float cweir = 1.7049f, mlim = 0.0f, cd = m->structure_cd[k];
+    int idir;
+    float h1, h2, qstruc;
+    if (zsnm > zsnmu) { idir = 1; h1 = zsnm - zc; h2 = zsnmu - zc; }
+    else { idir = -1; h1 = zsnmu - zc; h2 = zsnm - zc; }
+    if (h2 > 2.0f / 3.0f * h1) qstruc = cd * h2 * sqrtf(2.0f * 9.81f * (h1 - h2) / (1.0f - mlim));
+    else qstruc = cd * cweir * powf(h1, 1.5f);
+    qstruc = qstruc * m->structure_length[k];
+    m->q[ip] = qstruc * (float)idir;
+  }
+}
+
 /* ---- step driver: sfincs_lib.f90:337-683 ---------------------------------------------- */
 /* Runs while t < tend; single step when max_steps == 1. Returns the number of steps. */
 int32_t oracle_update(OM* m, double tend, int32_t max_steps) {
@@ -765,6 +797,7 @@ int32_t oracle_update(OM* m, double tend, int32_t max_steps) {
     oracle_update_boundaries(m, m->t, m->dt);
     oracle_update_discharges(m, m->t, m->dt);
     oracle_compute_fluxes(m, m->dt);
+    if (m->nstruc > 0) oracle_compute_fluxes_over_structures(m);
     oracle_compute_water_levels(m, m->t, m->dt);
     ++steps;
     if (dtchk < m->dtmin && m->nt > 1) {

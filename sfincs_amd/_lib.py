@@ -47,7 +47,8 @@ class sfx_subgrid_input(C.Structure):
 class sfx_sizes(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "np", "npuv", "ncuv", "nref", "nlevels", "nmax", "mmax", "ngbnd", "nkcuv2", "nsrc", "nbnd",
-        "subgrid", "quadtree", "dense")] + [("dtmax", C.c_float), ("dtmin", C.c_float), ("ndrn", C.c_int32)]
+        "subgrid", "quadtree", "dense")] + [("dtmax", C.c_float), ("dtmin", C.c_float), ("ndrn", C.c_int32),
+                                          ("nstruc", C.c_int32), ("nthd", C.c_int32)]
 
 
 class sfx_step_stats(C.Structure):
@@ -69,6 +70,8 @@ def _sig(lib):
         "sfx_model_create_quadtree": (I, [C.POINTER(sfx_params), C.POINTER(sfx_quadtree_input), C.POINTER(P)]),
         "sfx_model_read_subgrid_binary": (I, [P, C.c_char_p]),
         "sfx_model_set_drainage": (I, [P, i32, P, P, P, P, P, P]),
+        "sfx_model_set_thin_dams": (I, [P, i32, P, P, P]),
+        "sfx_model_set_weirs": (I, [P, i32, P, P, P, P]),
         "sfx_model_sizes": (I, [P, C.POINTER(sfx_sizes)]),
         "sfx_model_get_table": (I, [P, s, P, sz]),
         "sfx_model_table_bytes": (I, [P, s, C.POINTER(sz)]),
@@ -175,7 +178,8 @@ TABLE_DTYPES = {
             "uv_index_u_nmd", "uv_index_u_nmu", "uv_index_u_ndm", "uv_index_u_num", "uv_index_v_ndm",
             "uv_index_v_nm", "uv_index_v_ndmu", "uv_index_v_nmu", "cuv_index_uv", "cuv_index_uv1",
             "cuv_index_uv2", "nmindbnd", "index_kcuv2", "nmbkcuv2", "nmikcuv2", "ibkcuv2", "ind1_bnd_gbp",
-            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc", "index_bdr_gbp", "index_zsi_bdr", "drainage_type"),
+            "ind2_bnd_gbp", "nmi_gbp", "nmindsrc", "index_bdr_gbp", "index_zsi_bdr", "drainage_type",
+            "thindam_uv_index", "structure_uv_index"),
 }
 
 
@@ -283,6 +287,27 @@ class Model:
         pp = np.asarray(params, np.float32).reshape(len(ty), -1)
         par[:, :pp.shape[1]] = pp
         check(lib().sfx_model_set_drainage(self.h, len(ty), *[ptr(v) for v in a], ptr(ty), ptr(par)), "set_drainage")
+
+    @staticmethod
+    def _polylines(polylines, npar):
+        nv = np.array([len(p) for p in polylines], np.int32)
+        rows = np.concatenate([np.asarray(p, np.float32).reshape(len(p), -1) for p in polylines]) if len(nv) else \
+            np.zeros((0, 2 + npar), np.float32)
+        if rows.shape[1] != 2 + npar:
+            raise ValueError(f"polyline vertices need {2 + npar} values (x, y{', parameters' if npar else ''})")
+        x = np.ascontiguousarray(rows[:, 0]); y = np.ascontiguousarray(rows[:, 1])
+        par = np.ascontiguousarray(rows[:, 2:]).reshape(-1)
+        return nv, x, y, par
+
+    def set_thin_dams(self, polylines):
+        """Thin dams (thdfile): a list of polylines, each an (n, 2) array of x, y."""
+        nv, x, y, _ = self._polylines(polylines, 0)
+        check(lib().sfx_model_set_thin_dams(self.h, len(nv), ptr(nv), ptr(x), ptr(y)), "set_thin_dams")
+
+    def set_weirs(self, polylines):
+        """Weirs (weirfile): a list of polylines, each an (n, 4) array of x, y, crest level, Cd."""
+        nv, x, y, par = self._polylines(polylines, 2)
+        check(lib().sfx_model_set_weirs(self.h, len(nv), ptr(nv), ptr(x), ptr(y), ptr(par)), "set_weirs")
 
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)

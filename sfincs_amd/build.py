@@ -47,12 +47,12 @@ def tables():
     """Enumerate (or load and re-verify) the C library's powf tables next to the library."""
     import ctypes
     lib = ctypes.CDLL(LIB)
-    counts = (ctypes.c_int64 * 3)()
+    counts = (ctypes.c_int64 * 4)()
     lib.sfx_libm_tables.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
     rc = lib.sfx_libm_tables(None, counts)
     if rc != 0:
         raise RuntimeError("powf table generation failed")
-    print(f"powf tables: {counts[0]} + {counts[1]} + {counts[2]} entries", flush=True)
+    print("powf tables: " + " + ".join(str(c) for c in counts) + " entries", flush=True)
 
 
 def build_fortran_host():

@@ -177,6 +177,9 @@ struct DevInst {
   const float* tprcpt; const float* tprcpv; const float* twnd; const float* wndmag; const float* wnddir;
   const float* cdval; const float* x73;
   sfx::PowTables pw;
+  // weirs: per uv slot the structure index (-1: none), per structure {crest, Cd, length, 0};
+  // null when the scenario has none
+  const int32_t* struc_of; const float4* struc_par;
 };
 
 // Flags of a dry uv-point kept in its kfuv byte by the per-point and indexed momentum
@@ -575,7 +578,7 @@ __device__ void drainage_update(const DevInst& I, int k, double t, float dt) {
     G(I.drn_status)[k] = st;
     const float g = I.c.g;
     const float h73 = sfx::libm_pow(hgate, 2, (const uint64_t*)I.pw.t13, (const uint64_t*)I.pw.t53,
-                                    (const uint64_t*)I.pw.t73);
+                                    (const uint64_t*)I.pw.t73, (const uint64_t*)I.pw.t15);
     qq = (qq0 - g * hgate * dzds * dt) / (1.0f + g * (mng * mng) * dt * fabsf(qq0) / h73);
     qq = qq * wdt * frac;
   }
@@ -812,6 +815,27 @@ __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx:
 // of wet points. Neighbours follow the reference's tables for a full rectangle
 // (sfincs_domain.f90:1005-1261): missing left/right/below/above faces read the dummy
 // point (q = uv = 0), missing cross neighbours of u_ndm/u_num fall back to the point.
+// compute_fluxes_over_structures (sfincs_structures.f90:592-697), which the reference runs
+// after the momentum sweep of the step on the same levels, so it is fused here: at a weir
+// point (kcuv 3) the broad-crested weir formula on the levels of its two cells (as real*4),
+// submerged when the lower head exceeds 2/3 of the upper, times the crest length; uv = 0.
+template <typename R>
+__device__ __forceinline__ R weir_flux(const DevInst& I, int s, double zs_nm, double zs_nmu) {
+  const float4 p = ldf4(G(I.struc_par), G(I.struc_of)[s]);
+  const R zc = (R)p.x, cd = (R)p.y, len = (R)p.z;
+  const R a = (R)zs_nm, b = (R)zs_nmu;
+  if (a < zc && b < zc) return R(0);
+  const bool fwd = a > b;
+  const R h1 = (fwd ? a : b) - zc, h2 = (fwd ? b : a) - zc;
+  R qs;
+  if (h2 > (R)(2.0f / 3.0f) * h1)
+    qs = cd * h2 * sfx::rsqrt_<R>((R)(2.0f * 9.81f) * (h1 - h2));
+  else
+    qs = cd * (R)1.7049f * sfx::pow_third<R>(h1, 3, sfx::pow_tables_g(I.pw));
+  qs = qs * len;
+  return fwd ? qs : -qs;
+}
+
 // Gathers the inputs of one uv-point. The neighbour values are read whether or not the
 // scenario's physics uses them (uv_update reads them only under the flags that need
 // them), so that the loads carry no branch.
@@ -877,6 +901,9 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
       int br = 0;
       finish_uv<R, SUBGRID>(I, c, iu, fu, dt, q1, u1, mdt, br);
       cnt[br]++;
+    } else if (ku == 3) {
+      q1[c] = weir_flux<R>(I, c, iu.zs_nm, iu.zs_nmu); u1[c] = R(0);
+      cnt[0]++;
     } else {
       q1[c] = iu.qx_nm; u1[c] = iu.uu_nm;
       cnt[0]++;
@@ -891,6 +918,9 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
       int br = 0;
       finish_uv<R, SUBGRID>(I, s, iv, fv, dt, q1, u1, mdt, br);
       cnt[br]++;
+    } else if (kv == 3) {
+      q1[s] = weir_flux<R>(I, s, iv.zs_nm, iv.zs_nmu); u1[s] = R(0);
+      cnt[0]++;
     } else {
       q1[s] = iv.qx_nm; u1[s] = iv.uu_nm;
       cnt[0]++;
@@ -925,6 +955,11 @@ __device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, in
   sfx::UvTables tb;
   if (SUBGRID) tb.zmin = fl;
   else tb.zbuvmx = fl;
+  if (kc == 3) {
+    q1[s] = weir_flux<R>(I, s, zs_nm, zs_nmu); u1[s] = R(0);
+    cnt[0]++;
+    return false;
+  }
   if (!(kc == 1 || kc == 6)) {
     q1[s] = q0[s]; u1[s] = u0[s];
     cnt[0]++;
@@ -1143,6 +1178,9 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           int br = 0;
           finish_uv<R, SUBGRID>(I, c, in, uv_floor<SUBGRID>(I, c), dt, q1, u1, mdt, br);
           cnt[br]++;
+        } else if (B.ku == 3) {
+          q1[c] = weir_flux<R>(I, c, B.z, C.z); u1[c] = R(0);
+          cnt[0]++;
         } else {
           q1[c] = B.uq; u1[c] = B.uu;
           cnt[0]++;
@@ -1172,6 +1210,9 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           int br = 0;
           finish_uv<R, SUBGRID>(I, s, in, uv_floor<SUBGRID>(I, s), dt, q1, u1, mdt, br);
           cnt[br]++;
+        } else if (B.kv == 3) {
+          q1[s] = weir_flux<R>(I, s, B.z, B.r_z); u1[s] = R(0);
+          cnt[0]++;
         } else {
           q1[s] = B.vq; u1[s] = B.vu;
           cnt[0]++;
@@ -1207,6 +1248,11 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
   sfx::UvTables tb;
   if (SUBGRID) tb.zmin = G(I.uv_zmin)[ip];
   else tb.zbuvmx = G(I.zbuvmx)[ip];
+  if (kc == 3) {
+    q1[ip] = weir_flux<R>(I, ip, in.zs_nm, in.zs_nmu); u1[ip] = R(0);
+    cnt[0]++;
+    return;
+  }
   if (!(kc == 1 || kc == 6)) {
     q1[ip] = q0[ip]; u1[ip] = u0[ip];
     cnt[0]++;
@@ -1370,11 +1416,17 @@ __global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts
   if (!G(Sp)->active) return;
   const int j = tile * BLOCK + threadIdx.x;
   if (j >= I.ncuv) return;
-  const int cur1 = G(Sp)->cur ^ 1;
+  const int cur = G(Sp)->cur, cur1 = cur ^ 1;
   const auto q = G((R*)I.q[cur1]);
   const auto uv = G((R*)I.uv[cur1]);
   const int c = G(I.cuv)[j], a = G(I.cuv)[I.ncuv + j], b = G(I.cuv)[2 * I.ncuv + j];
-  q[c] = (q[a] + q[b]) / R(2);
+  // The reference averages inside compute_fluxes, before compute_fluxes_over_structures
+  // sets this step's weir discharges, so a weir point (kcuv 3) enters with the last
+  // step's value: the other ping-pong buffer. Its uv is 0 in both.
+  const auto q_last = G((const R*)I.q[cur]);
+  const R qa = G(I.kcuv)[a] == 3 ? q_last[a] : q[a];
+  const R qb = G(I.kcuv)[b] == 3 ? q_last[b] : q[b];
+  q[c] = (qa + qb) / R(2);
   uv[c] = (uv[a] + uv[b]) / R(2);
 }
 
@@ -1961,6 +2013,16 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   {
     auto kc = perm8(M->kcuv, 0);
     I.kcuv = H->upload(kc.data(), ns);
+    if (!M->structure_uv_index.empty()) {   // weir uv-points: slot -> structure
+      std::vector<int32_t> so(ns, -1);
+      std::vector<float4> sp(M->structure_uv_index.size());
+      for (size_t k = 0; k < sp.size(); ++k) {
+        so[H->ref2slot[M->structure_uv_index[k] - 1]] = (int32_t)k;
+        sp[k] = make_float4(M->structure_height[k], M->structure_cd[k], M->structure_length[k], 0.f);
+      }
+      I.struc_of = H->upload(so.data(), ns);
+      I.struc_par = H->upload(sp.data(), sp.size());
+    }
     auto ma = perm8(M->mask_adv, 0);
     I.mask_adv = H->upload(ma.data(), ns);
     auto kf = perm8(M->kfuv, 0);

@@ -106,6 +106,15 @@ struct sfx_model {
   std::vector<int32_t> drainage_type;
   std::vector<float> tsrc, qsrc;   // qsrc[it*nsrc + isrc]
 
+  // thin dams and weirs (thdfile, weirfile; sfincs_structures.f90): polylines, vertex
+  // count per polyline; weirs carry (crest height, discharge coefficient) per vertex
+  std::vector<int32_t> thd_nv, weir_nv;
+  std::vector<float> thd_x, thd_y, weir_x, weir_y, weir_par;
+  // derived at initialize: the thin-dam uv-points (set to kcuv 0) and the weir uv-points
+  // (kcuv 3) with crest height, coefficient and length relative to the grid spacing
+  std::vector<int32_t> thindam_uv_index, structure_uv_index;   // 1-based
+  std::vector<float> structure_height, structure_cd, structure_length;
+
   // meteo series
   int32_t ntprcp = 0, ntwnd = 0;
   std::vector<float> tprcpt, tprcpv, twnd, wndmag, wnddir;

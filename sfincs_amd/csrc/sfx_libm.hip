@@ -1,6 +1,6 @@
 // Tables of the C library's powf results for the inputs whose float rounding the device
 // estimate in sfx_libm.hpp cannot decide (see there). The tables are enumerated once over
-// every float in [2^-40, 2^14) for both exponents, verified while they are built, cached
+// every float in [2^-40, 2^14) for each exponent, verified while they are built, cached
 // in a file next to the library, and uploaded once per device.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
@@ -25,7 +25,7 @@ namespace sfx {
 namespace {
 
 constexpr uint32_t MAGIC = 0x5346504Du;   // "SFPM"
-constexpr uint32_t VERSION = 7;
+constexpr uint32_t VERSION = 8;
 
 struct Entry { uint32_t x, g; };
 
@@ -39,7 +39,9 @@ std::mutex g_mu;
 PowSet g_set;
 std::map<int, PowTables> g_dev;
 
-float exponent(int which) { return which == 0 ? 1.0f / 3.0f : (which == 1 ? 5.0f / 3.0f : 7.0f / 3.0f); }
+float exponent(int which) {
+  return which == 0 ? 1.0f / 3.0f : (which == 1 ? 5.0f / 3.0f : (which == 2 ? 7.0f / 3.0f : 1.5f));
+}
 
 // Enumerate one exponent over [lo, hi) bit patterns; returns false if a verification fails.
 bool enumerate(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std::string& err) {
@@ -186,12 +188,10 @@ bool ensure_host(std::string& err) {
 }  // namespace
 
 // Host copy of the tables (for the host-side checks of the emulation).
-bool pow_tables_host(std::vector<uint2>& t13, std::vector<uint2>& t53, std::vector<uint2>& t73, std::string& err) {
+bool pow_tables_host(std::vector<uint2> (&t)[NPOW], std::string& err) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!ensure_host(err)) return false;
-  t13 = hash_table(g_set.e[0]);
-  t53 = hash_table(g_set.e[1]);
-  t73 = hash_table(g_set.e[2]);
+  for (int which = 0; which < NPOW; ++which) t[which] = hash_table(g_set.e[which]);
   return true;
 }
 
@@ -210,7 +210,8 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
       err = "upload of the powf tables failed";
       return false;
     }
-    (which == 0 ? T.t13 : (which == 1 ? T.t53 : T.t73)) = (const uint2*)p;
+    const uint2** slot[NPOW] = {&T.t13, &T.t53, &T.t73, &T.t15};
+    *slot[which] = (const uint2*)p;
   }
   g_dev[device] = T;
   out = T;
@@ -220,7 +221,7 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
 }  // namespace sfx
 
 // Build (or load and verify) the powf tables; path NULL = next to the library.
-extern "C" int sfx_libm_tables(const char* path, int64_t counts[3]) {
+extern "C" int sfx_libm_tables(const char* path, int64_t counts[4]) {
   std::lock_guard<std::mutex> lk(sfx::g_mu);
   std::string err;
   if (path && *path) {
@@ -241,10 +242,10 @@ extern "C" int sfx_libm_tables(const char* path, int64_t counts[3]) {
 
 // Host evaluation of the emulation, for tests: out[i] = powf(x[i], exponent `which`).
 extern "C" int sfx_libm_pow_host(const float* x, int64_t n, int32_t which, float* out) {
-  std::vector<uint2> a, b, c;
+  std::vector<uint2> t[sfx::NPOW];
   std::string err;
-  if (!sfx::pow_tables_host(a, b, c, err)) { sfx::set_error(err); return SFX_ERR_STATE; }
-  sfx::PowTables T{a.data(), b.data(), c.data()};
+  if (!sfx::pow_tables_host(t, err)) { sfx::set_error(err); return SFX_ERR_STATE; }
+  sfx::PowTables T{t[0].data(), t[1].data(), t[2].data(), t[3].data()};
   for (int64_t i = 0; i < n; ++i) out[i] = sfx::libm_pow(x[i], which, T);
   return SFX_OK;
 }

@@ -1,8 +1,10 @@
-// x**(1/3), x**(5/3) and x**(7/3) with the exact float results of the C library's powf.
+// x**(1/3), x**(5/3), x**(7/3) and x**1.5 with the exact float results of the C library's
+// powf.
 //
 // The reference computes hu**expo (expo = 1.0/3.0 as real*4) and hu**(5.0/3.0)
-// (sfincs_momentum.f90:92,616,673), and hgate**(7.0/3.0) for drainage gates
-// (sfincs_discharges.f90:518,610), with the C library's powf. The momentum and continuity
+// (sfincs_momentum.f90:92,616,673), hgate**(7.0/3.0) for drainage gates
+// (sfincs_discharges.f90:518,610) and h1**1.5 for weirs (sfincs_structures.f90:672), with
+// the C library's powf. The momentum and continuity
 // updates branch on thresholds (wet/dry tests, the table inversion switch at
 // V = 0.999*volmax), so a one-ulp difference in powf can move a cell to the other branch.
 // These routines return powf's exact result: a double-precision estimate (pow_estimate)
@@ -25,12 +27,13 @@ constexpr uint32_t POW_HASH_SIZE = 1u << POW_HASH_BITS;
 constexpr float POW_XLO = 9.094947017729282e-13f;   // 2^-40
 constexpr float POW_XHI = 16384.0f;                 // 2^14
 
-constexpr int NPOW = 3;   // exponents (float)(1/3), (float)(5/3), (float)(7/3)
+constexpr int NPOW = 4;   // exponents (float)(1/3), (float)(5/3), (float)(7/3), 1.5
 
 struct PowTables {
   const uint2* t13;   // open-addressing hash: {x bits, powf(x, 1/3f) bits}; key 0 = empty
   const uint2* t53;
   const uint2* t73;
+  const uint2* t15;
 };
 
 SFX_HD inline uint32_t pow_hash(uint32_t k) {
@@ -41,8 +44,8 @@ SFX_HD inline uint32_t pow_hash(uint32_t k) {
 SFX_HD inline uint32_t f2u(float f) { union { float f; uint32_t u; } v; v.f = f; return v.u; }
 SFX_HD inline float u2f(uint32_t u) { union { float f; uint32_t u; } v; v.u = u; return v.f; }
 
-// Double estimate of x**e for e = (float)(1/3) (which = 0), (float)(5/3) (which = 1) or
-// (float)(7/3) (which = 2),
+// Double estimate of x**e for e = (float)(1/3) (which = 0), (float)(5/3) (which = 1),
+// (float)(7/3) (which = 2) or 1.5 (which = 3, x sqrt(x) in double),
 // relative error below 2^-46 (2^-44 for 7/3) for normal positive x (checked for every float of the table
 // range when the tables are built). Only IEEE-754 basic operations in double are used,
 // evaluated in the order written, so host and device compute bit-identical estimates.
@@ -51,6 +54,7 @@ SFX_HD inline float u2f(uint32_t u) { union { float f; uint32_t u; } v; v.u = u;
 //   (no division); the float exponents differ from 1/3 and 5/3 by d, applied as
 //   x^d = 1 + t + t^2/2 with t = d ln x, ln x = e ln 2 + ln m from a polynomial.
 SFX_HD inline double pow_estimate(float x, int which) {
+  if (which == 3) return (double)x * sqrt((double)x);
   const uint32_t b = f2u(x);
   if (b < 0x00800000u) {   // subnormal: outside the table range, no exactness claimed
     const double c = (double)cbrtf(x);
@@ -137,19 +141,20 @@ SFX_HD inline float pow_lookup(P t, uint32_t key, float fallback) {
   return fallback;
 }
 
-// powf(x, (float)(1/3)), powf(x, (float)(5/3)) or powf(x, (float)(7/3)) as the C library
-// computes it.
+// powf(x, (float)(1/3)), powf(x, (float)(5/3)), powf(x, (float)(7/3)) or powf(x, 1.5f)
+// as the C library computes it.
 template <typename P>
-SFX_HD inline float libm_pow(float x, int which, P t13, P t53, P t73) {
+SFX_HD inline float libm_pow(float x, int which, P t13, P t53, P t73, P t15) {
   if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
   const double r = pow_estimate(x, which);
   const float f = (float)r;
   if (x >= POW_XLO && x < POW_XHI && near_midpoint(r))
-    return pow_lookup(which == 0 ? t13 : (which == 1 ? t53 : t73), f2u(x), f);
+    return pow_lookup(which == 0 ? t13 : (which == 1 ? t53 : (which == 2 ? t73 : t15)), f2u(x), f);
   return f;
 }
 SFX_HD inline float libm_pow(float x, int which, const PowTables& T) {
-  return libm_pow(x, which, (const uint64_t*)T.t13, (const uint64_t*)T.t53, (const uint64_t*)T.t73);
+  return libm_pow(x, which, (const uint64_t*)T.t13, (const uint64_t*)T.t53, (const uint64_t*)T.t73,
+                  (const uint64_t*)T.t15);
 }
 
 }  // namespace sfx

@@ -528,6 +528,8 @@ extern "C" int sfx_model_sizes(const sfx_model* M, sfx_sizes* s) {
   s->subgrid = M->subgrid; s->quadtree = M->use_quadtree; s->dense = M->dense;
   s->dtmax = M->dtmax; s->dtmin = M->dtmin;
   s->ndrn = M->ndrn;
+  s->nstruc = (int32_t)M->structure_uv_index.size();
+  s->nthd = (int32_t)M->thindam_uv_index.size();
   return SFX_OK;
 }
 
@@ -835,6 +837,42 @@ extern "C" int sfx_model_set_drainage(sfx_model* M, int32_t ndrn, const float* x
   return SFX_OK;
 }
 
+namespace {
+bool polylines_ok(int32_t npoly, const int32_t* nv, const float* x, const float* y, size_t& nvert) {
+  nvert = 0;
+  if (npoly < 0 || (npoly > 0 && (!nv || !x || !y))) return false;
+  for (int32_t i = 0; i < npoly; ++i) {
+    if (nv[i] < 2) return false;
+    nvert += (size_t)nv[i];
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" int sfx_model_set_thin_dams(sfx_model* M, int32_t npoly, const int32_t* nv, const float* x, const float* y) {
+  size_t nvert = 0;
+  if (!M || M->initialized || !polylines_ok(npoly, nv, x, y, nvert)) {
+    set_error("sfx_model_set_thin_dams: bad arguments (each polyline needs >= 2 vertices)");
+    return SFX_ERR_ARG;
+  }
+  M->thd_nv.assign(nv, nv + npoly);
+  M->thd_x.assign(x, x + nvert); M->thd_y.assign(y, y + nvert);
+  return SFX_OK;
+}
+
+extern "C" int sfx_model_set_weirs(sfx_model* M, int32_t npoly, const int32_t* nv, const float* x, const float* y,
+                                   const float* par) {
+  size_t nvert = 0;
+  if (!M || M->initialized || !polylines_ok(npoly, nv, x, y, nvert) || (npoly > 0 && !par)) {
+    set_error("sfx_model_set_weirs: bad arguments (each polyline needs >= 2 vertices)");
+    return SFX_ERR_ARG;
+  }
+  M->weir_nv.assign(nv, nv + npoly);
+  M->weir_x.assign(x, x + nvert); M->weir_y.assign(y, y + nvert);
+  M->weir_par.assign(par, par + 2 * nvert);
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_set_sources(sfx_model* M, int32_t nsrc, const float* x, const float* y,
                                      int32_t nt, const float* t, const float* q) {
   if (!M || M->initialized || nsrc < 0 || (nsrc > 0 && (!x || !y || !t || !q || nt < 2))) {
@@ -864,6 +902,175 @@ extern "C" int sfx_model_set_wind(sfx_model* M, int32_t nt, const float* t, cons
   M->wind = true;
   return SFX_OK;
 }
+
+// ---- thin dams and weirs -----------------------------------------------------------------
+namespace {
+
+// direction (geometry.f90:124-139): the side of c from the line a -> b, 0 within 1e-6.
+int side_of(float ax, float ay, float bx, float by, float cx, float cy) {
+  const float val = (by - ay) * (cx - bx) - (bx - ax) * (cy - by);
+  if (val < -1.0e-6f) return -1;
+  if (val > 1.0e-6f) return 1;
+  return 0;
+}
+
+// cross (geometry.f90:157-194): segments 1 and 2 cross when each separates the other's ends.
+bool segments_cross(float x11, float y11, float x12, float y12, float x21, float y21, float x22, float y22) {
+  return side_of(x11, y11, x12, y12, x21, y21) != side_of(x11, y11, x12, y12, x22, y22) &&
+         side_of(x21, y21, x22, y22, x11, y11) != side_of(x21, y21, x22, y22, x12, y12);
+}
+
+// distance_between_points_projected_on_line_segment (geometry.f90:215-296): the length of
+// the part of segment 1-2 between the projections of points a and b onto it.
+float projected_length(float x0a, float y0a, float x0b, float y0b, float x1, float y1, float x2, float y2,
+                       float mxdst) {
+  auto position = [&](float px, float py, float& b) {
+    b = std::fabs((x2 - x1) * (y1 - py) - (x1 - px) * (y2 - y1)) / std::sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    const float c1 = std::sqrt((px - x1) * (px - x1) + (py - y1) * (py - y1));
+    const float c2 = std::sqrt((px - x2) * (px - x2) + (py - y2) * (py - y2));
+    const float aa = std::sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    const float a1 = std::sqrt(std::max(c1 * c1 - b * b, 0.0f));
+    const float a2 = std::sqrt(std::max(c2 * c2 - b * b, 0.0f));
+    if (a1 > aa || a2 > aa) return a1 < a2 ? 0.0f : 1.0f;
+    return a1 / aa;
+  };
+  float b1, b2;
+  const float p1 = position(x0a, y0a, b1);
+  const float p2 = position(x0b, y0b, b2);
+  const float aa = std::sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+  return (b1 < mxdst && b2 < mxdst) ? aa * std::fabs(p2 - p1) : 0.0f;
+}
+
+// find_uv_points_intersected_by_polyline (sfincs_quadtree.F90:853-1010): for each segment,
+// the cells of each level in the segment's index box widened by one cell, and of each the
+// uv-points to its right and top (same level or coarser, and finer); a uv-point is crossed
+// when the segment crosses the line between its two cell centres. Returns, per uv-point,
+// the last crossing segment (1-based, 0 = not crossed).
+std::vector<int32_t> crossed_uv_points(const sfx_model* M, const float* xp, const float* yp, int32_t npol) {
+  std::vector<int32_t> ind(M->npuv, 0);
+  std::vector<int64_t> mmax_level(M->nref + 1, 0);
+  for (size_t k = 0; k < M->qt_nm.size(); ++k) {
+    const int iref = M->qt_level[k];
+    const int64_t nmx = (int64_t)M->qt_nmax << (iref - 1);
+    mmax_level[iref] = std::max<int64_t>(mmax_level[iref], (M->qt_nm[k] - 1) / nmx + 1);
+  }
+  const std::vector<int32_t>* right_top[4] = {&M->z_index_uv_mu1, &M->z_index_uv_mu2, &M->z_index_uv_nu1,
+                                              &M->z_index_uv_nu2};
+  for (int32_t ipol = 1; ipol < npol; ++ipol) {
+    const float xpa = xp[ipol - 1], ypa = yp[ipol - 1], xpb = xp[ipol], ypb = yp[ipol];
+    const float xta = M->cosrot * (xpa - M->x0) + M->sinrot * (ypa - M->y0);
+    const float yta = -M->sinrot * (xpa - M->x0) + M->cosrot * (ypa - M->y0);
+    const float xtb = M->cosrot * (xpb - M->x0) + M->sinrot * (ypb - M->y0);
+    const float ytb = -M->sinrot * (xpb - M->x0) + M->cosrot * (ypb - M->y0);
+    for (int iref = M->nref; iref >= 1; --iref) {
+      // n runs to nmx at this level, m to the largest m of the level's points
+      const int64_t nmx = (int64_t)M->qt_nmax << (iref - 1);
+      const int64_t mmx = mmax_level[iref];
+      const double dyr = M->dyr[iref - 1], dxr = M->dxr[iref - 1];
+      // int() of the reference truncates toward zero; cells outside [1, nmx] x [1, mmx]
+      // hold no uv-point, so the box is clipped to the mesh
+      const int64_t na = (int64_t)std::trunc(std::max(std::min((double)(yta / (float)dyr), 1e15), -1e15)) + 1;
+      const int64_t ma = (int64_t)std::trunc(std::max(std::min((double)(xta / (float)dxr), 1e15), -1e15)) + 1;
+      const int64_t nb = (int64_t)std::trunc(std::max(std::min((double)(ytb / (float)dyr), 1e15), -1e15)) + 1;
+      const int64_t mb = (int64_t)std::trunc(std::max(std::min((double)(xtb / (float)dxr), 1e15), -1e15)) + 1;
+      const int64_t n0 = std::max<int64_t>(std::min(na, nb) - 1, 1), n1 = std::min<int64_t>(std::max(na, nb) + 1, nmx);
+      const int64_t m0 = std::max<int64_t>(std::min(ma, mb) - 1, 1), m1 = std::min<int64_t>(std::max(ma, mb) + 1, mmx);
+      for (int64_t m = m0; m <= m1; ++m)
+        for (int64_t n = n0; n <= n1; ++n) {
+          const int32_t nm = M->find_cell_by_index((int32_t)n, (int32_t)m, iref);
+          if (nm == 0) continue;
+          for (const auto* tab : right_top) {
+            const int32_t ip = (*tab)[nm - 1];
+            if (ip <= 0) continue;
+            const int32_t nmu = M->uv_index_z_nmu[ip - 1];
+            if (segments_cross(xpa, ypa, xpb, ypb, M->z_xz[nm - 1], M->z_yz[nm - 1], M->z_xz[nmu - 1], M->z_yz[nmu - 1]))
+              ind[ip - 1] = ipol;
+          }
+        }
+    }
+  }
+  return ind;
+}
+
+// read_thin_dams (sfincs_structures.f90:385-495) and read_structure_file for weirs
+// (:38-381), run after the domain tables, the boundaries and the advection mask are set
+// (sfincs_lib.f90:154-156), so neither the boundary list nor mask_adv sees them.
+void apply_structures(sfx_model* M) {
+  M->thindam_uv_index.clear();
+  M->structure_uv_index.clear(); M->structure_height.clear(); M->structure_cd.clear(); M->structure_length.clear();
+  const int32_t npuv = M->npuv;
+  if (!M->thd_nv.empty()) {
+    std::vector<char> closed(npuv, 0);
+    size_t v0 = 0;
+    for (int32_t nv : M->thd_nv) {
+      const std::vector<int32_t> ind = crossed_uv_points(M, &M->thd_x[v0], &M->thd_y[v0], nv);
+      for (int32_t ip = 0; ip < npuv; ++ip)
+        if (ind[ip] > 0) { M->kcuv[ip] = 0; closed[ip] = 1; }
+      v0 += (size_t)nv;
+    }
+    for (int32_t ip = 0; ip < npuv; ++ip)
+      if (closed[ip]) M->thindam_uv_index.push_back(ip + 1);
+  }
+  if (M->weir_nv.empty()) return;
+  std::vector<char> istruc(npuv, 0);
+  std::vector<float> height(npuv, -9999.0f), cd(npuv, 0.0f), lngth(npuv, 0.0f);
+  const float pi = M->p.pi, rot = M->rotation;
+  size_t v0 = 0;
+  for (int32_t nv : M->weir_nv) {
+    const float* xt = &M->weir_x[v0];
+    const float* yt = &M->weir_y[v0];
+    const float* pr = &M->weir_par[2 * v0];
+    const std::vector<int32_t> ind = crossed_uv_points(M, xt, yt, nv);
+    for (int32_t ip = 0; ip < npuv; ++ip) {
+      const int32_t irow = ind[ip];
+      if (irow == 0) continue;
+      istruc[ip] = 1;
+      // the uv-point's face: its centre and its two corners (:169-242)
+      const int32_t nm = M->uv_index_z_nm[ip], nmu = M->uv_index_z_nmu[ip];
+      const int iref = M->uv_flags_iref[ip], idir = M->uv_flags_dir[ip];
+      const int irefnm = M->z_flags_iref[nm - 1], irefnmu = M->z_flags_iref[nmu - 1];
+      const float dxs = idir == 0 ? M->dyrm[iref - 1] : M->dxrm[iref - 1];
+      float xuv, yuv;
+      if (irefnm >= irefnmu) {
+        const float a = idir == 0 ? rot : rot + 0.5f * pi;
+        xuv = M->z_xz[nm - 1] + 0.5f * dxs * std::cos(a);
+        yuv = M->z_yz[nm - 1] + 0.5f * dxs * std::sin(a);
+      } else {
+        const float a = idir == 0 ? rot - pi : rot - 0.5f * pi;
+        xuv = M->z_xz[nmu - 1] + 0.5f * dxs * std::cos(a);
+        yuv = M->z_yz[nmu - 1] + 0.5f * dxs * std::sin(a);
+      }
+      const float a1 = idir == 0 ? rot - 0.5f * pi : rot - pi;
+      const float a2 = idir == 0 ? rot + 0.5f * pi : rot;
+      const float xuv1 = xuv + 0.5f * dxs * std::cos(a1), yuv1 = yuv + 0.5f * dxs * std::sin(a1);
+      const float xuv2 = xuv + 0.5f * dxs * std::cos(a2), yuv2 = yuv + 0.5f * dxs * std::sin(a2);
+      const int32_t k = irow - 1;   // the crossing segment runs from vertex k to k + 1
+      lngth[ip] = lngth[ip] + projected_length(xuv1, yuv1, xuv2, yuv2, xt[k], yt[k], xt[k + 1], yt[k + 1], 999999.0f);
+      const float dst1 = std::sqrt((xuv - xt[k]) * (xuv - xt[k]) + (yuv - yt[k]) * (yuv - yt[k]));
+      const float dst2 = std::sqrt((xuv - xt[k + 1]) * (xuv - xt[k + 1]) + (yuv - yt[k + 1]) * (yuv - yt[k + 1]));
+      const float wfac = dst1 / (dst1 + dst2);
+      const float h = pr[2 * k] * (1.0f - wfac) + pr[2 * (k + 1)] * wfac;
+      height[ip] = std::max(height[ip], h);
+      cd[ip] = pr[2 * k + 1] * (1.0f - wfac) + pr[2 * (k + 1) + 1] * wfac;
+    }
+    v0 += (size_t)nv;
+  }
+  // a crest at or below the uv-point's bed is no structure (:293-325)
+  for (int32_t ip = 0; ip < npuv; ++ip) {
+    if (!istruc[ip]) continue;
+    const float bed = M->subgrid ? M->uv_zmin[ip] : M->zbuvmx[ip];
+    if (!(height[ip] > bed)) continue;
+    const int iref = M->uv_flags_iref[ip], idir = M->uv_flags_dir[ip];
+    const float dxs = idir == 0 ? M->dyrm[iref - 1] : M->dxrm[iref - 1];
+    M->structure_uv_index.push_back(ip + 1);
+    M->structure_height.push_back(height[ip]);
+    M->structure_cd.push_back(cd[ip]);
+    M->structure_length.push_back(lngth[ip] / dxs);
+    M->kcuv[ip] = 3;
+  }
+}
+
+}  // namespace
 
 extern "C" int sfx_model_initialize(sfx_model* M) {
   if (!M || !M->mesh_built || M->initialized) { set_error("sfx_model_initialize: bad state"); return SFX_ERR_STATE; }
@@ -1131,6 +1338,7 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
         if (M->kcuv[nb[j] - 1] != 1) { M->mask_adv[i] = 0; break; }
     }
   }
+  apply_structures(M);
   M->initialized = true;
   return SFX_OK;
 }
@@ -1152,6 +1360,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(uv_index_z_nm) T(uv_index_z_nmu) T(uv_index_u_nmd) T(uv_index_u_nmu) T(uv_index_u_ndm) T(uv_index_u_num)
   T(uv_index_v_ndm) T(uv_index_v_nm) T(uv_index_v_ndmu) T(uv_index_v_nmu)
   T(uv_flags_iref) T(uv_flags_type) T(uv_flags_dir) T(kcuv) T(mask_adv)
+  T(thindam_uv_index) T(structure_uv_index) T(structure_height) T(structure_cd) T(structure_length)
   T(cuv_index_uv) T(cuv_index_uv1) T(cuv_index_uv2) T(zbuv) T(zbuvmx) T(gn2uv)
   T(dxr) T(dyr) T(dxrm) T(dyrm) T(dxrinv) T(dyrinv) T(dxr2inv) T(dyr2inv) T(dxrinvc) T(dyrinvc) T(cell_area) T(nuvisc)
   T(z_zmin) T(z_zmax) T(z_volmax) T(z_dep) T(uv_zmin) T(uv_zmax) T(uv_navg_w) T(uv_fnfit)

@@ -82,20 +82,22 @@ template <typename T> __device__ __forceinline__ T vmin(T a, T b) { return a < b
 template <typename R> __device__ __forceinline__ R rpow(R x, R y);
 template <> __device__ __forceinline__ float rpow<float>(float x, float y) { return powf(x, y); }
 template <> __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
-// x**(float)(1/3) and x**(float)(5/3): the C library's exact powf results in FP32,
-// ordinary double pow in the FP64 twin.
+// x**(float)(1/3), x**(float)(5/3), x**(float)(7/3) and x**1.5: the C library's exact
+// powf results in FP32, ordinary double pow in the FP64 twin.
 // Device view of the powf tables: 64-bit entries {key, value} in the global address space.
-struct PowTablesG { GA<const uint64_t> t13, t53, t73; };
+struct PowTablesG { GA<const uint64_t> t13, t53, t73, t15; };
 __device__ __forceinline__ PowTablesG pow_tables_g(const PowTables& T) {
-  return PowTablesG{ga((const uint64_t*)T.t13), ga((const uint64_t*)T.t53), ga((const uint64_t*)T.t73)};
+  return PowTablesG{ga((const uint64_t*)T.t13), ga((const uint64_t*)T.t53), ga((const uint64_t*)T.t73),
+                    ga((const uint64_t*)T.t15)};
 }
 template <typename R> __device__ __forceinline__ R pow_third(R x, int which, const PowTablesG& T);
 template <> __device__ __forceinline__ float pow_third<float>(float x, int which, const PowTablesG& T) {
-  return libm_pow(x, which, T.t13, T.t53, T.t73);
+  return libm_pow(x, which, T.t13, T.t53, T.t73, T.t15);
 }
 template <> __device__ __forceinline__ double pow_third<double>(double x, int which, const PowTablesG& T) {
   (void)T;
-  return pow(x, which == 0 ? (double)(1.0f / 3.0f) : (which == 1 ? (double)(5.0f / 3.0f) : (double)(7.0f / 3.0f)));
+  return pow(x, which == 0 ? (double)(1.0f / 3.0f)
+                           : (which == 1 ? (double)(5.0f / 3.0f) : (which == 2 ? (double)(7.0f / 3.0f) : 1.5)));
 }
 template <typename R> __device__ __forceinline__ R rsqrt_(R x);
 template <> __device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }

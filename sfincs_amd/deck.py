@@ -141,6 +141,18 @@ def write(sc, folder):
                 npar = 1 if int(ty[k]) <= 3 else 6
                 f.write(" ".join(_fmt(v) for v in (xi[k], yi[k], xo[k], yo[k])) + f" {int(ty[k])} " +
                         " ".join(_fmt(v) for v in par[k][:npar]) + "\n")
+    # thin dams and weirs: tekal polyline blocks (sfincs_structures.f90:113-160, 421-470)
+    for key, fname, polys in (("thdfile", "sfincs.thd", getattr(sc, "thd", None)),
+                              ("weirfile", "sfincs.weir", getattr(sc, "weirs", None))):
+        if polys is None:
+            continue
+        lines.append(f"{key} = {fname}")
+        with open(os.path.join(folder, fname), "w") as f:
+            for k, pl in enumerate(polys):
+                pl = np.asarray(pl, np.float32)
+                f.write(f"line{k + 1:04d}\n{pl.shape[0]} {pl.shape[1]}\n")
+                for row in pl:
+                    f.write(" ".join(_fmt(v) for v in row) + "\n")
     if sc.src is not None:
         x, y, t, q = sc.src
         lines += ["srcfile = sfincs.src", "disfile = sfincs.dis"]

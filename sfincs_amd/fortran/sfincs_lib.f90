@@ -103,6 +103,18 @@ module sfx_capi
          integer(c_int32_t), value :: ndrn
          type(c_ptr), value :: xi, yi, xo, yo, ty, par
       end function
+      integer(c_int) function sfx_model_set_thin_dams(m, npoly, nv, x, y) bind(C, name='sfx_model_set_thin_dams')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: npoly
+         type(c_ptr), value :: nv, x, y
+      end function
+      integer(c_int) function sfx_model_set_weirs(m, npoly, nv, x, y, par) bind(C, name='sfx_model_set_weirs')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: npoly
+         type(c_ptr), value :: nv, x, y, par
+      end function
       integer(c_int) function sfx_model_set_rain(m, nt, t, v) bind(C, name='sfx_model_set_rain')
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: m
@@ -392,6 +404,63 @@ contains
       close(u)
    end function
 
+   ! Polylines in the tekal block format of thd and weir files (sfincs_structures.f90:
+   ! 113-160, 421-470): per polyline a name line, a line "nrows ncols", then nrows lines of
+   ! x y and npar parameters. Returns the vertex count per polyline, the coordinates and the
+   ! parameters (npar per vertex, vertex-major), or ok = .false. on a malformed file.
+   subroutine read_polylines(fname, npar, nv, xv, yv, pv, ok)
+      character(*), intent(in) :: fname
+      integer, intent(in) :: npar
+      integer(c_int32_t), allocatable, intent(out) :: nv(:)
+      real(c_float), allocatable, intent(out) :: xv(:), yv(:), pv(:)
+      logical, intent(out) :: ok
+      character(256) :: name
+      integer :: u, ios, npoly, nvert, nrows, ncols, i, k, j
+      real(c_float) :: row(2 + 8)
+      ok = .false.
+      open(newunit=u, file=trim(fname), status='old', action='read', iostat=ios)
+      if (ios /= 0) return
+      npoly = 0
+      nvert = 0
+      do
+         read(u, *, iostat=ios) name
+         if (ios /= 0) exit
+         read(u, *, iostat=ios) nrows, ncols
+         if (ios /= 0) exit
+         if (nrows < 2 .or. ncols < 2 + npar) then
+            close(u)
+            return
+         endif
+         do i = 1, nrows
+            read(u, *, iostat=ios) row(1)
+            if (ios /= 0) then
+               close(u)
+               return
+            endif
+         enddo
+         npoly = npoly + 1
+         nvert = nvert + nrows
+      enddo
+      rewind(u)
+      allocate(nv(npoly), xv(nvert), yv(nvert), pv(max(1, npar * nvert)))
+      pv = 0.0
+      k = 0
+      do i = 1, npoly
+         read(u, *) name
+         read(u, *) nrows, ncols
+         nv(i) = nrows
+         do j = 1, nrows
+            k = k + 1
+            read(u, *) row(1:2 + npar)
+            xv(k) = row(1)
+            yv(k) = row(2)
+            if (npar > 0) pv(npar * (k - 1) + 1:npar * k) = row(3:2 + npar)
+         enddo
+      enddo
+      close(u)
+      ok = .true.
+   end subroutine
+
    ! ---- sfincs_initialize (sfincs_lib.f90:85-335) --------------------------------------
    ! Reads the input deck into an initialized host model, then moves it to HBM.
    function sfincs_initialize() result(ierr)
@@ -434,7 +503,10 @@ contains
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout, tdum
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
       character(256) :: bndfile, bzsfile, bzifile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
-      character(256) :: sbgfile, drnfile
+      character(256) :: sbgfile, drnfile, thdfile, weirfile
+      integer(c_int32_t), allocatable, target :: pl_nv(:)
+      real(c_float), allocatable, target :: pl_x(:), pl_y(:), pl_p(:)
+      logical :: pl_ok
       character(512) :: line
       integer(c_int32_t), allocatable, target :: drn_type(:)
       real(c_float), allocatable, target :: drn_par(:,:)
@@ -533,6 +605,7 @@ contains
       call inp_str('bzifile', bzifile, 'none')
       call inp_str('sbgfile', sbgfile, 'none')
       call inp_str('drnfile', drnfile, 'none')
+      call inp_str('thdfile', thdfile, 'none'); call inp_str('weirfile', weirfile, 'none')
       call inp_real('structure_relax', p%structure_relax)
 
       if (qtrfile(1:4) /= 'none') then
@@ -723,6 +796,38 @@ contains
             return
          endif
          deallocate(tt, v1, drn_type, drn_par)
+      endif
+      ! thin dams and weirs (read_structures, sfincs_structures.f90:8-35): polylines
+      if (thdfile(1:4) /= 'none') then
+         call read_polylines(thdfile, 0, pl_nv, pl_x, pl_y, pl_p, pl_ok)
+         if (.not. pl_ok) then
+            call logmsg('Error   : cannot read thin dam file '//trim(thdfile))
+            ierr = -1
+            return
+         endif
+         rc = sfx_model_set_thin_dams(model, int(size(pl_nv), c_int32_t), c_loc(pl_nv), c_loc(pl_x), c_loc(pl_y))
+         if (rc < 0) then
+            call fail('thin dams')
+            ierr = -1
+            return
+         endif
+         deallocate(pl_nv, pl_x, pl_y, pl_p)
+      endif
+      if (weirfile(1:4) /= 'none') then
+         call read_polylines(weirfile, 2, pl_nv, pl_x, pl_y, pl_p, pl_ok)
+         if (.not. pl_ok) then
+            call logmsg('Error   : cannot read weir file '//trim(weirfile))
+            ierr = -1
+            return
+         endif
+         rc = sfx_model_set_weirs(model, int(size(pl_nv), c_int32_t), c_loc(pl_nv), c_loc(pl_x), c_loc(pl_y), &
+                                  c_loc(pl_p))
+         if (rc < 0) then
+            call fail('weirs')
+            ierr = -1
+            return
+         endif
+         deallocate(pl_nv, pl_x, pl_y, pl_p)
       endif
       ! downstream river points: x y x_in y_in slope distance (sfincs_boundaries.f90:221-268)
       if (bdrfile(1:4) /= 'none') then

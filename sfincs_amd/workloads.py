@@ -43,6 +43,8 @@ class Scenario:
     bzi: np.ndarray = None                # infragravity levels [nt, nbnd] at the bnd series times
     storage: object = None                # callable(model) -> storage volume per cell (m3), or None
     drn: tuple = None                     # drainage structures (x_in, y_in, x_out, y_out, type, params[n, 6])
+    thd: list = None                      # thin dams: polylines, each (n, 2) x, y
+    weirs: list = None                    # weirs: polylines, each (n, 4) x, y, crest level, Cd
 
     def build_model(self):
         p = _lib.default_params(**self.params)
@@ -74,6 +76,10 @@ class Scenario:
             m.set_river_boundary(*self.bdr)
         if self.drn is not None:
             m.set_drainage(*self.drn)
+        if self.thd is not None:
+            m.set_thin_dams(self.thd)
+        if self.weirs is not None:
+            m.set_weirs(self.weirs)
         if self.storage is not None:
             m.set_storage_volume(self.storage(m))
         if self.rstfile is not None:
@@ -411,6 +417,38 @@ def beach_drainage(sc):
     out.append((*pt(sea, rows[4]), *pt(coast, rows[4]), 5, [4.0, -3.0, 0.03, 600.0, 1500.0, 200.0]))
     del ny
     return with_drainage(sc, out)
+
+
+def with_thin_dams(sc, polylines):
+    """Adds thin dams (thdfile): polylines of (x, y) vertices."""
+    sc.thd = [np.asarray(p, np.float32).reshape(-1, 2) for p in polylines]
+    sc.name += "_thd"
+    return sc
+
+
+def with_weirs(sc, polylines):
+    """Adds weirs (weirfile): polylines of (x, y, crest level, Cd) vertices."""
+    sc.weirs = [np.asarray(p, np.float32).reshape(-1, 4) for p in polylines]
+    sc.name += "_weir"
+    return sc
+
+
+def beach_structures(sc, crest=(0.6, 1.2), cd=0.6, extent=None, cell=None, weir_x=0.4, dam_x=(0.55, 0.8, 0.9)):
+    """A weir along a north-south line at `weir_x` of the way inland on a beach with the sea
+    in the west, its crest rising from crest[0] in the south to crest[1] in the north, so
+    the tide overtops it first in the south; and a thin dam running diagonally across the
+    land behind it (vertices at `dam_x` of the length). The lines are offset from cell
+    faces and centres by fractions of a cell (`cell`, default the grid spacing; `extent` =
+    (x length, y length), default the grid's)."""
+    dx = sc.dx if cell is None else cell
+    lx, ly = (sc.mmax * sc.dx, sc.nmax * sc.dy) if extent is None else extent
+    xw = weir_x * lx + 0.31 * dx
+    weir = [(xw, -0.37 * dx, crest[0], cd), (xw + 0.23 * dx, 0.5 * ly, 0.5 * (crest[0] + crest[1]), cd),
+            (xw - 0.11 * dx, ly + 0.41 * dx, crest[1], cd)]
+    dam = [(dam_x[0] * lx + 0.13 * dx, 0.2 * ly + 0.29 * dx), (dam_x[1] * lx + 0.37 * dx, 0.55 * ly + 0.17 * dx),
+           (dam_x[2] * lx + 0.21 * dx, 0.6 * ly + 0.43 * dx)]
+    with_weirs(sc, [weir])
+    return with_thin_dams(sc, [dam])
 
 
 def with_river(sc, first=1, ncells=8, slope=1.0e-3):

@@ -83,7 +83,7 @@ SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax",
 
 @needs_host
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves",
-                                  "subgrid_binary", "drainage"])
+                                  "subgrid_binary", "drainage", "structures"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
@@ -105,6 +105,9 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     elif kind == "drainage":
         sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5,
                                              structure_relax=7.5))
+    elif kind == "structures":
+        sc = W.beach_structures(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5),
+                                crest=(-0.3, 0.4))
     else:
         sc = W.subgrid_case(20, 24, dx=25.0, seed=43, t_end=900.0, nsrc=2, rain_mmh=10.0)
         sc.subgrid = None
@@ -127,6 +130,9 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     if kind == "drainage":
         extra = ("drainage_type", "drainage_params", "drainage_distance")
         assert mp.sizes()["ndrn"] == 5 and mp.params().structure_relax == 7.5
+    if kind == "structures":
+        extra = ("thindam_uv_index", "structure_uv_index", "structure_height", "structure_cd", "structure_length")
+        assert mp.sizes()["nstruc"] > 10 and mp.sizes()["nthd"] > 10
     for name in TABLES + extra:
         a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
         assert (a is None) == (b is None), name
@@ -137,7 +143,7 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
 
 @needs_host
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage", "structures"])
 def test_fortran_host_matches_python_path(kind, tmp_path):
     print(f"case fortran host {kind}", flush=True)
     from sfincs_amd import _lib
@@ -148,6 +154,9 @@ def test_fortran_host_matches_python_path(kind, tmp_path):
                                    tmp_path / "gen.sbg")
     elif kind == "drainage":         # drnfile
         sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5))
+    elif kind == "structures":       # thdfile and weirfile
+        sc = W.beach_structures(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5),
+                                crest=(-0.3, 0.4))
     else:
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     d = deck.write(sc, str(tmp_path))

@@ -3,8 +3,9 @@ the h^(7/3) table (sfincs_momentum.f90:787-820), point sources on regular and qu
 meshes with the bed clamp of the regular branch (sfincs_continuity.f90:87-106), Neumann
 side boundaries (msk = 6, sfincs_boundaries.f90:903-913), boundary spin-up
 (:789-826 with weighted_average :1179-1200), a four-level quadtree with wind, and the
-instability stop (sfincs_lib.f90:634-648), and drainage structures
-(sfincs_discharges.f90:371-655) on regular and subgrid meshes.
+instability stop (sfincs_lib.f90:634-648), drainage structures
+(sfincs_discharges.f90:371-655) on regular and subgrid meshes, and weirs with thin dams
+(sfincs_structures.f90:385-495, 592-697) on regular, subgrid and quadtree meshes.
 
 Gate: identical step count and model time, max|dzs| <= 1e-4 m, max|dq| <= 1e-5 m2/s."""
 import numpy as np
@@ -140,3 +141,29 @@ def test_drainage_structures_match_oracle(kind):
     assert s["ndrn"] >= 3
     flows = orc.state("qtsrc")[s["nsrc"]:]
     assert np.count_nonzero(flows) >= 2          # at least one structure still passes water
+
+
+def structures_case(kind):
+    """A weir the tide overtops and a thin dam behind it (workloads.beach_structures)."""
+    if kind == "regular":
+        return W.beach_structures(W.regular_case(40, 48, dx=50.0, seed=71, t_end=1800.0, tide_amp=2.5,
+                                                 rain_mmh=0.0), crest=(-0.3, 0.4))
+    if kind == "subgrid":
+        sc = W.subgrid_case(30, 44, dx=25.0, seed=73, t_end=1800.0, nsrc=1, tide_amp=2.5)
+        return W.beach_structures(sc, crest=(-0.5, 0.4), weir_x=0.22, dam_x=(0.25, 0.3, 0.35))
+    sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=29, t_end=1800.0, surge_amp=3.0)
+    return W.beach_structures(sc, crest=(-0.5, 0.5), extent=(1400.0, 1000.0), cell=25.0, weir_x=0.09,
+                              dam_x=(0.75, 0.85, 0.95))
+
+
+@pytest.mark.parametrize("kind", ["regular", "subgrid", "quadtree"])
+def test_weirs_and_thin_dams_match_oracle(kind):
+    """The weir discharge is fused into the momentum kernels (kcuv 3); thin dams close
+    their uv-points (kcuv 0). GPU against the oracle over the whole period."""
+    print("case structures", kind, flush=True)
+    m, inst, orc = _full_run(structures_case(kind))
+    s = m.sizes()
+    assert s["nstruc"] > 10 and s["nthd"] > 10
+    ip = m.table("structure_uv_index") - 1
+    assert np.count_nonzero(orc.state("q")[ip]) > 0          # water passes the weir
+    assert np.all(inst.get("uv")[ip] == 0.0)

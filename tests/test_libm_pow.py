@@ -1,7 +1,8 @@
-"""x**(1/3), x**(5/3) and x**(7/3) with the C library's exact powf results (sfx_libm.hpp).
+"""x**(1/3), x**(5/3), x**(7/3) and x**1.5 with the C library's exact powf results
+(sfx_libm.hpp).
 
-The reference evaluates hu**expo, hu**(5.0/3.0) and hgate**(7.0/3.0) with the C
-library's powf; the
+The reference evaluates hu**expo, hu**(5.0/3.0), hgate**(7.0/3.0) and the weir's h1**1.5
+with the C library's powf; the
 emulation must return the same float for every input the tables cover. Checked here on
 the host implementation over every table key and a dense random sample, and on the
 device (GPU test) over the same inputs."""
@@ -15,7 +16,8 @@ from sfincs_amd import _lib
 libm = C.CDLL("libm.so.6")
 libm.powf.restype = C.c_float
 libm.powf.argtypes = [C.c_float, C.c_float]
-EXP = [np.float32(1.0) / np.float32(3.0), np.float32(5.0) / np.float32(3.0), np.float32(7.0) / np.float32(3.0)]
+EXP = [np.float32(1.0) / np.float32(3.0), np.float32(5.0) / np.float32(3.0), np.float32(7.0) / np.float32(3.0),
+       np.float32(1.5)]
 
 
 def glibc_powf(x, which):
@@ -23,11 +25,24 @@ def glibc_powf(x, which):
     return np.array([libm.powf(float(v), y) for v in x], np.float32)
 
 
+def exact_midpoints():
+    """x = a^2 4^k with a odd in [257, 321]: x**1.5 = a^3 8^k has 25 significant bits and
+    its last one set, an exact midpoint between two floats, which powf must round (k <= -2
+    keeps x inside the tables' range, below 2^14)."""
+    a = np.arange(257, 322, 2, dtype=np.float64)
+    k = np.arange(-12, -1, dtype=np.float64)
+    x = (a[:, None] ** 2 * 4.0 ** k[None, :]).reshape(-1)
+    assert np.all((a ** 3 >= 2 ** 24) & (a ** 3 < 2 ** 25))
+    return x.astype(np.float32)
+
+
 def sample(n, seed):
     r = np.random.default_rng(seed)
     # log-uniform over the covered range plus hand-picked mismatching values
     x = np.exp(r.uniform(np.log(2.0 ** -40), np.log(2.0 ** 14), n)).astype(np.float32)
-    return np.concatenate([x, np.array([1.6265767e-05, 8.68599415, 0.05, 1.0, 2.0, 10.0], np.float32)])
+    squares = (np.arange(1, 1000, dtype=np.float32) ** 2 / np.float32(64.0)).astype(np.float32)
+    return np.concatenate([x, np.array([1.6265767e-05, 8.68599415, 0.05, 1.0, 2.0, 10.0], np.float32),
+                           exact_midpoints(), squares])
 
 
 def emulate(x, which, device=False):
@@ -40,14 +55,24 @@ def emulate(x, which, device=False):
     return out
 
 
-@pytest.mark.parametrize("which", [0, 1, 2])
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
 def test_host_emulation_equals_glibc(which):
     x = sample(200000, 7 + which)
     assert np.array_equal(emulate(x, which).view(np.uint32), glibc_powf(x, which).view(np.uint32))
 
 
+def test_midpoints_are_decided_by_the_table():
+    """Every exact midpoint of x**1.5 is a table entry, and powf rounds them to even."""
+    x = exact_midpoints()
+    g = glibc_powf(x, 3)
+    exact = (x.astype(np.float64) ** 1.5)
+    lo = exact.astype(np.float32)
+    assert np.array_equal(emulate(x, 3).view(np.uint32), g.view(np.uint32))
+    assert np.all(np.abs(g.astype(np.float64) - exact) == np.abs(lo.astype(np.float64) - exact))
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", [0, 1, 2])
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
 def test_device_emulation_equals_glibc(which):
     x = sample(2000000, 17 + which)
     want = emulate(x, which)            # host emulation, equal to glibc (test above)

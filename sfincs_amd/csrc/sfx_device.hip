@@ -166,6 +166,10 @@ struct DevInst {
   void* uvmean;
   const float* t_bnd; const float* zs_bnd;
   int32_t bnd_overlap;           // a Neumann copy and a boundary flux point share a cell
+  int32_t n6;                    // Neumann boundary cells (kcs 6) ...
+  const int32_t* gb6;            // ... as pairs {cell, interior cell}
+  int32_t nflux;                 // kcuv 2/6 points whose boundary cell is not a Neumann cell ...
+  const int32_t* k2_flux;        // ... as indices into the k2_* tables
   // sources and meteo; src_cell and qtsrc hold nsrc point sources, then each drainage
   // structure's intake and outfall
   int32_t nsrc, ntsrc, ntprcp, ntwnd;
@@ -600,6 +604,164 @@ __device__ void drainage_update(const DevInst& I, int k, double t, float dt) {
   qt[jout] = qq;
 }
 
+// The step's boundary-series position and uvmean filter factors, for the boundary pass.
+template <typename R>
+struct BndStep {
+  int itb0, itb1;
+  float tbfac;
+  double t;
+  R factime, facrel, one_minus_factime;
+};
+
+// One boundary flux point's inputs: everything update_boundary_fluxes reads from tables and
+// levels before its arithmetic (the level of its boundary cell, the interior level, the uv
+// tables), loaded by bnd_load without a store in between.
+template <typename R>
+struct BndPt {
+  int ib, nmb, nmi, ip, dir, kb;
+  float zsb1, zsb0, floorz, zbuv_or_zmin, zmaxu, havg_zmax, zb_nmi;
+  R zsnmi, uvm;
+  double vol_nmi;
+  int kcuv;
+};
+
+// update_boundary_points / update_boundary_conditions (:728-917) for the point's boundary
+// cell, and the point's other inputs (:941-1000).
+template <typename R>
+__device__ __forceinline__ void bnd_load(const DevInst& I, const BndStep<R>& B, int ib, BndPt<R>& p) {
+  const Consts& c = I.c;
+  p.ib = ib;
+  p.nmb = G(I.k2_nmb)[ib];
+  p.kb = G(I.kcs)[p.nmb] & KCS_MASK;
+  const int indb = G(I.k2_ib)[ib];
+  p.nmi = G(I.k2_nmi)[ib];
+  p.ip = G(I.k2_slot)[ib];
+  p.dir = G(I.k2_dir)[ib];
+  p.zsnmi = (R)G(I.zs)[p.nmi];
+  p.uvm = G((const R*)I.uvmean)[ib];
+  p.kcuv = G(I.kcuv)[p.ip];
+  p.floorz = I.subgrid ? G(I.z_zmin)[p.nmb] : G(I.zb)[p.nmb];
+  if (I.subgrid) {
+    const float4 s4 = ldf4(G(I.uv_s4), p.ip);
+    p.zmaxu = s4.x; p.havg_zmax = s4.y;
+    p.zbuv_or_zmin = G(I.uv_zmin)[p.ip];
+    p.vol_nmi = G(I.vol)[p.nmi];
+    p.zb_nmi = 0.0f;
+  } else {
+    p.zbuv_or_zmin = G(I.k2_zbuv)[ib];
+    p.zb_nmi = G(I.zb)[p.nmi];
+    p.zmaxu = p.havg_zmax = 0.0f;
+    p.vol_nmi = 0.0;
+  }
+  if (p.kb == 2) {
+    // update_boundary_points (:728-756) at the points this cell interpolates between
+    auto series_at = [&](const float* ser, int k) -> float {
+      const float za = G(ser)[(B.itb0 - 1) * I.nbnd + k], zb = G(ser)[(B.itb1 - 1) * I.nbnd + k];
+      return za + (zb - za) * B.tbfac;
+    };
+    // update_boundary_conditions, water-level boundary (:789-917); the infragravity level
+    // zig (bzi, :829-848) is interpolated as the water level
+    double zst;
+    float zig = 0.0f;
+    if (I.nbnd > 1) {
+      const float f = G(I.gb_fac)[indb];
+      const int i1 = G(I.gb_ind1)[indb] - 1, i2 = G(I.gb_ind2)[indb] - 1;
+      zst = (double)(series_at(I.zs_bnd, i1) * f + series_at(I.zs_bnd, i2) * (1.0f - f));
+      if (I.zsi_bnd) zig = series_at(I.zsi_bnd, i1) * f + series_at(I.zsi_bnd, i2) * (1.0f - f);
+    } else {
+      zst = (double)series_at(I.zs_bnd, 0);
+      if (I.zsi_bnd) zig = series_at(I.zsi_bnd, 0);
+    }
+    float zb0, zb1;
+    if (B.t < (double)(c.tspinup - 1.0e-3f)) {
+      const float smfac = (float)(1.0 - (B.t - (double)c.t0) / (double)(c.tspinup - c.t0));
+      const float zs0act = (float)(zst + 0.0);
+      zb0 = c.zini * smfac + zs0act * (1.0f - smfac);
+      const float zs1act = (float)(zst + 0.0 + (double)zig);
+      zb1 = c.zini * smfac + zs1act * (1.0f - smfac);
+    } else {
+      zb0 = (float)(zst + 0.0);
+      zb1 = (float)(zst + 0.0 + (double)zig);
+    }
+    p.zsb0 = sfx::vmax<float>(zb0, p.floorz);
+    p.zsb1 = sfx::vmax<float>(zb1, p.floorz);
+  } else if (p.kb == 5) {
+    // downstream river cell (:878-901): the interior level plus the slope offset, raised
+    // to the bed, in f64 (the interior cell is an ordinary cell, which no boundary step writes)
+    const double zst = sfx::vmax<double>(G(I.zs)[G(I.gb_rcell)[indb]] + (double)G(I.gb_rdzs)[indb], (double)p.floorz);
+    p.zsb0 = (float)zst;
+    p.zsb1 = p.zsb0;
+  } else {
+    p.zsb1 = G(I.zsb)[indb];
+    p.zsb0 = G(I.zsb0)[indb];
+  }
+}
+
+// update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective), from the loaded inputs.
+template <typename R, typename QP, typename MP>
+__device__ __forceinline__ void bnd_finish(const DevInst& I, const BndStep<R>& B, const BndPt<R>& p, QP q, QP uv,
+                                           MP uvmean, MP zsmax) {
+  const Consts& c = I.c;
+  const int ip = p.ip, ib = p.ib, nmb = p.nmb;
+  const R zsnmi = p.zsnmi;
+  R zsnmb = p.zsb1;
+  R zs0nmb = p.zsb0;
+  R hnmb;
+  const R huth = c.huthresh;
+  if (I.subgrid) {
+    const int nl = c.nl;
+    const R zsuv = sfx::vmax<R>(zsnmb, zsnmi);
+    const R zmaxu = p.zmaxu, zminu = p.zbuv_or_zmin;
+    R depthuv;
+    if (zsuv >= zmaxu - R(1.0e-4f)) {
+      depthuv = (R)p.havg_zmax + zsuv;
+    } else if (zsuv > zminu) {
+      const R dzuv = (zmaxu - zminu) / (R)(nl - 1);
+      const int iuv = (int)((zsuv - zminu) / dzuv) + 1;
+      const R facint = (zsuv - (zminu + (R)(iuv - 1) * dzuv)) / dzuv;
+      const auto lv = G(I.uv_lev) + (size_t)ip * nl;
+      depthuv = (R)ldf4(lv, iuv - 1).x + ((R)ldf4(lv, iuv).x - (R)ldf4(lv, iuv - 1).x) * facint;
+    } else {
+      depthuv = R(0);
+    }
+    hnmb = sfx::vmax<R>(depthuv, huth);
+    zsnmb = sfx::vmax<R>(zsnmb, (R)p.floorz);
+    zs0nmb = sfx::vmax<R>(zs0nmb, (R)p.floorz);
+  } else {
+    hnmb = sfx::vmax<R>(R(0.5f) * (zsnmb + zsnmi) - (R)p.zbuv_or_zmin, huth);
+    zsnmb = sfx::vmax<R>(zsnmb, (R)p.floorz);
+    zs0nmb = sfx::vmax<R>(zs0nmb, (R)p.floorz);
+  }
+  R qb, um;   // the point's discharge and uvmean as the filter below reads them
+  const R uvm = p.uvm;
+  if (hnmb < huth + R(1.0e-6f) || p.kcuv == 3) {
+    qb = R(0);
+    um = R(0);
+    q[ip] = R(0); uv[ip] = R(0);
+  } else {
+    const R g = c.g;
+    const R ui = sfx::rsqrt_<R>(g / hnmb) * (zsnmb - zs0nmb);
+    const int dir = p.dir;
+    const R ub = (R)dir * (R(2) * ui - sfx::rsqrt_<R>(g / hnmb) * (zsnmi - zs0nmb));
+    qb = ub * hnmb + uvm;
+    if (I.subgrid) {
+      if (p.vol_nmi <= 0.0) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
+      if (zsnmb - (R)p.floorz < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
+    } else {
+      if (zsnmi - (R)p.zb_nmi <= huth) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
+      if (zsnmb - (R)p.floorz < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
+    }
+    q[ip] = qb;
+    uv[ip] = sfx::vmax<R>(sfx::vmin<R>(qb / hnmb, R(4)), R(-4));
+    um = uvm;
+  }
+  if (c.btfilter >= -1.0e-6f) uvmean[ib] = B.factime * qb + B.facrel * B.one_minus_factime * um;
+  else uvmean[ib] = R(0);
+  G(I.zs)[nmb] = (double)p.zsb1;
+  mark_cell(I, nmb);
+  if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], (double)p.zsb1);
+}
+
 // One block per scenario: the step's fold, scalars and forcing series (committed from
 // k_forcing_ahead, or computed here), then boundary points, conditions and fluxes
 // (sfincs_lib.f90:379-550 with sfincs_meteo.f90:1423-1558, sfincs_boundaries.f90:682-1122,
@@ -637,126 +799,29 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   // (kcs 6) run first, and the barrier after them is needed only when a copy reads a cell
   // whose level a flux point sets or sets a cell that a flux point reads (bnd_overlap, found
   // on the host).
-  for (int ib = threadIdx.x; ib < I.ngbnd; ib += blockDim.x) {
-    const int nmb = G(I.gb_cell)[ib];
-    if ((G(I.kcs)[nmb] & KCS_MASK) == 6) { G(I.zs)[nmb] = G(I.zs)[G(I.gb_nmi)[ib]]; mark_cell(I, nmb); }
+  for (int k = threadIdx.x; k < I.n6; k += blockDim.x) {
+    const int cell = G(I.gb6)[2 * k], inner = G(I.gb6)[2 * k + 1];
+    G(I.zs)[cell] = G(I.zs)[inner];
+    mark_cell(I, cell);
   }
   if (I.bnd_overlap) __syncthreads();
-  // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective)
+  // update_boundary_fluxes (:941-1117), bndtype 1 (weakly reflective), over the flux
+  // points (kcuv 2 points whose boundary cell is not a Neumann cell). Each thread takes two
+  // points and issues the loads of both before it finishes either, so the dependent chain
+  // of table and level reads is paid once per two points.
   const int curb = __builtin_amdgcn_readfirstlane(F.s.cur);
   const auto q = G((R*)I.q[curb]);
   const auto uv = G((R*)I.uv[curb]);
   const auto uvmean = G((R*)I.uvmean);
   const auto zsmax = G((R*)I.zsmax);
-  const int nl = c.nl;
-  const int itb0 = F.itb0, itb1 = F.itb1;
-  const float tbfac = F.tbfac;
-  for (int ib = threadIdx.x; ib < I.nkcuv2; ib += blockDim.x) {
-    const int nmb = G(I.k2_nmb)[ib];
-    const int kb = G(I.kcs)[nmb] & KCS_MASK;
-    if (kb == 6) continue;
-    const int indb = G(I.k2_ib)[ib];
-    const int nmi = G(I.k2_nmi)[ib];
-    const int ip = G(I.k2_slot)[ib];
-    const R zsnmi = (R)G(I.zs)[nmi];
-    float zsb1, zsb0;
-    if (kb == 2) {
-      // update_boundary_points (:728-756) at the points this cell interpolates between
-      auto series_at = [&](const float* ser, int k) -> float {
-        const float za = G(ser)[(itb0 - 1) * I.nbnd + k], zb = G(ser)[(itb1 - 1) * I.nbnd + k];
-        return za + (zb - za) * tbfac;
-      };
-      // update_boundary_conditions, water-level boundary (:789-917); the infragravity level
-      // zig (bzi, :829-848) is interpolated as the water level
-      double zst;
-      float zig = 0.0f;
-      if (I.nbnd > 1) {
-        const float f = G(I.gb_fac)[indb];
-        const int i1 = G(I.gb_ind1)[indb] - 1, i2 = G(I.gb_ind2)[indb] - 1;
-        zst = (double)(series_at(I.zs_bnd, i1) * f + series_at(I.zs_bnd, i2) * (1.0f - f));
-        if (I.zsi_bnd) zig = series_at(I.zsi_bnd, i1) * f + series_at(I.zsi_bnd, i2) * (1.0f - f);
-      } else {
-        zst = (double)series_at(I.zs_bnd, 0);
-        if (I.zsi_bnd) zig = series_at(I.zsi_bnd, 0);
-      }
-      float zb0, zb1;
-      if (t < (double)(c.tspinup - 1.0e-3f)) {
-        const float smfac = (float)(1.0 - (t - (double)c.t0) / (double)(c.tspinup - c.t0));
-        const float zs0act = (float)(zst + 0.0);
-        zb0 = c.zini * smfac + zs0act * (1.0f - smfac);
-        const float zs1act = (float)(zst + 0.0 + (double)zig);
-        zb1 = c.zini * smfac + zs1act * (1.0f - smfac);
-      } else {
-        zb0 = (float)(zst + 0.0);
-        zb1 = (float)(zst + 0.0 + (double)zig);
-      }
-      const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
-      zsb0 = sfx::vmax<float>(zb0, floorz);
-      zsb1 = sfx::vmax<float>(zb1, floorz);
-    } else if (kb == 5) {
-      // downstream river cell (:878-901): the interior level plus the slope offset, raised
-      // to the bed, in f64 (the interior cell is an ordinary cell, which no boundary step writes)
-      const float floorz = I.subgrid ? G(I.z_zmin)[nmb] : G(I.zb)[nmb];
-      const double zst = sfx::vmax<double>(G(I.zs)[G(I.gb_rcell)[indb]] + (double)G(I.gb_rdzs)[indb], (double)floorz);
-      zsb0 = (float)zst;
-      zsb1 = zsb0;
-    } else {
-      zsb1 = G(I.zsb)[indb];
-      zsb0 = G(I.zsb0)[indb];
-    }
-    R zsnmb = zsb1;
-    R zs0nmb = zsb0;
-    R hnmb;
-    const R huth = c.huthresh;
-    if (I.subgrid) {
-      const R zsuv = sfx::vmax<R>(zsnmb, zsnmi);
-      const float4 s4 = ldf4(G(I.uv_s4), ip);
-      const R zmaxu = s4.x, zminu = G(I.uv_zmin)[ip];
-      R depthuv;
-      if (zsuv >= zmaxu - R(1.0e-4f)) {
-        depthuv = (R)s4.y + zsuv;
-      } else if (zsuv > zminu) {
-        const R dzuv = (zmaxu - zminu) / (R)(nl - 1);
-        const int iuv = (int)((zsuv - zminu) / dzuv) + 1;
-        const R facint = (zsuv - (zminu + (R)(iuv - 1) * dzuv)) / dzuv;
-        const auto lv = G(I.uv_lev) + (size_t)ip * nl;
-        depthuv = (R)ldf4(lv, iuv - 1).x + ((R)ldf4(lv, iuv).x - (R)ldf4(lv, iuv - 1).x) * facint;
-      } else {
-        depthuv = R(0);
-      }
-      hnmb = sfx::vmax<R>(depthuv, huth);
-      zsnmb = sfx::vmax<R>(zsnmb, (R)G(I.z_zmin)[nmb]);
-      zs0nmb = sfx::vmax<R>(zs0nmb, (R)G(I.z_zmin)[nmb]);
-    } else {
-      hnmb = sfx::vmax<R>(R(0.5f) * (zsnmb + zsnmi) - (R)G(I.k2_zbuv)[ib], huth);
-      zsnmb = sfx::vmax<R>(zsnmb, (R)G(I.zb)[nmb]);
-      zs0nmb = sfx::vmax<R>(zs0nmb, (R)G(I.zb)[nmb]);
-    }
-    R qb;
-    if (hnmb < huth + R(1.0e-6f) || G(I.kcuv)[ip] == 3) {
-      qb = R(0);
-      q[ip] = R(0); uv[ip] = R(0); uvmean[ib] = R(0);
-    } else {
-      const R g = c.g;
-      const R ui = sfx::rsqrt_<R>(g / hnmb) * (zsnmb - zs0nmb);
-      const int dir = G(I.k2_dir)[ib];
-      const R ub = (R)dir * (R(2) * ui - sfx::rsqrt_<R>(g / hnmb) * (zsnmi - zs0nmb));
-      qb = ub * hnmb + uvmean[ib];
-      if (I.subgrid) {
-        if (G(I.vol)[nmi] <= 0.0) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
-        if (zsnmb - (R)G(I.z_zmin)[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
-      } else {
-        if (zsnmi - (R)G(I.zb)[nmi] <= huth) qb = (dir == 1) ? sfx::vmax<R>(qb, R(0)) : sfx::vmin<R>(qb, R(0));
-        if (zsnmb - (R)G(I.zb)[nmb] < huth) qb = (dir == 1) ? sfx::vmin<R>(qb, R(0)) : sfx::vmax<R>(qb, R(0));
-      }
-      q[ip] = qb;
-      uv[ip] = sfx::vmax<R>(sfx::vmin<R>(qb / hnmb, R(4)), R(-4));
-    }
-    if (c.btfilter >= -1.0e-6f) uvmean[ib] = (R)F.s.factime * q[ip] + (R)F.s.facrel * (R)F.s.one_minus_factime * uvmean[ib];
-    else uvmean[ib] = R(0);
-    G(I.zs)[nmb] = (double)zsb1;
-    mark_cell(I, nmb);
-    if (c.flags & sfx::F_ZSMAX) zsmax[nmb] = (R)sfx::vmax<double>((double)zsmax[nmb], (double)zsb1);
+  const BndStep<R> B{F.itb0, F.itb1, F.tbfac, t, (R)F.s.factime, (R)F.s.facrel, (R)F.s.one_minus_factime};
+  for (int k0 = threadIdx.x; k0 < I.nflux; k0 += 2 * FBLOCK) {
+    const int k1 = k0 + FBLOCK;
+    BndPt<R> a, b;
+    bnd_load<R>(I, B, G(I.k2_flux)[k0], a);
+    if (k1 < I.nflux) bnd_load<R>(I, B, G(I.k2_flux)[k1], b);
+    bnd_finish<R>(I, B, a, q, uv, uvmean, zsmax);
+    if (k1 < I.nflux) bnd_finish<R>(I, B, b, q, uv, uvmean, zsmax);
   }
   if (I.ndrn > 0) {   // update_discharges' drainage structures, after the boundaries (sfincs_lib.f90:530-550)
     __syncthreads();
@@ -777,7 +842,7 @@ __device__ __forceinline__ float uv_floor(const DevInst& I, int slot) {
 // (kfuv only when `kf_old`, its kfuv byte as loaded, is not already 1; -1: not loaded).
 template <typename R, bool SUBGRID, typename QP>
 __device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, sfx::UvTables& tb, R zsu,
-                                       R dt, QP q1, QP uv1, float& mdt, int& br, int kf_old = -1) {
+                                       R dt, QP q1, QP uv1, R& mden, int& br, int kf_old = -1) {
   if (SUBGRID) {
     const float4 s4 = ldf4(G(I.uv_s4), slot);
     tb.zmax = s4.x; tb.havg_zmax = s4.y; tb.nrep_zmax = s4.z; tb.navg_w = s4.w; tb.fnfit = G(I.uv_fnfit)[slot];
@@ -789,14 +854,16 @@ __device__ __forceinline__ void wet_uv(const DevInst& I, int slot, const sfx::Uv
                                                      G(I.x73), sfx::pow_tables_g(I.pw), dt);
   q1[slot] = o.q; uv1[slot] = o.uv;
   if (kf_old != 1) G(I.kfuv)[slot] = 1;
-  mdt = fminf(mdt, (float)o.min_dt);
-  if (I.c.flags & sfx::F_TSA) G(I.tsa_req)[slot] = (float)o.min_dt;   // sfincs_momentum.f90:737-741
+  // the step is the minimum of 1 / den over the points, which is 1 / (the largest den),
+  // since a correctly rounded quotient falls as its divisor grows (NaN is skipped)
+  if (o.den > mden) mden = o.den;
+  if (I.c.flags & sfx::F_TSA) G(I.tsa_req)[slot] = (float)(R(1) / o.den);   // sfincs_momentum.f90:737-741
   br = o.branch;
 }
 
 template <typename R, bool SUBGRID, typename QP>
 __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx::UvIn<R>& in, float floor, R dt,
-                                          QP q1, QP uv1, float& mdt, int& br) {
+                                          QP q1, QP uv1, R& mden, int& br) {
   sfx::UvTables tb;
   if (SUBGRID) tb.zmin = floor;
   else tb.zbuvmx = floor;
@@ -806,7 +873,7 @@ __device__ __forceinline__ void finish_uv(const DevInst& I, int slot, const sfx:
     br = 1;
     return;
   }
-  wet_uv<R, SUBGRID>(I, slot, in, tb, zsu, dt, q1, uv1, mdt, br);
+  wet_uv<R, SUBGRID>(I, slot, in, tb, zsu, dt, q1, uv1, mden, br);
 }
 
 // Dense scenarios, one thread per cell: the u-point on its right and the v-point above
@@ -860,7 +927,7 @@ __device__ __forceinline__ void gather_uv(const DevInst& I, QP q0, QP u0, int s,
 }
 
 template <typename R, bool SUBGRID>
-__device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, float& mdt) {
+__device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, R& mden) {
   const auto S = G(I.sc);
   const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
@@ -899,7 +966,7 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
     if (ku == 1 || ku == 6) {
       iu.idir = 0; iu.iref = 1; iu.itype = 0; iu.tauwu = tauwu; iu.tauwv = tauwv;
       int br = 0;
-      finish_uv<R, SUBGRID>(I, c, iu, fu, dt, q1, u1, mdt, br);
+      finish_uv<R, SUBGRID>(I, c, iu, fu, dt, q1, u1, mden, br);
       cnt[br]++;
     } else if (ku == 3) {
       q1[c] = weir_flux<R>(I, c, iu.zs_nm, iu.zs_nmu); u1[c] = R(0);
@@ -916,7 +983,7 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
     if (kv == 1 || kv == 6) {
       iv.idir = 1; iv.iref = 1; iv.itype = 0; iv.tauwu = tauwu; iv.tauwv = tauwv;
       int br = 0;
-      finish_uv<R, SUBGRID>(I, s, iv, fv, dt, q1, u1, mdt, br);
+      finish_uv<R, SUBGRID>(I, s, iv, fv, dt, q1, u1, mden, br);
       cnt[br]++;
     } else if (kv == 3) {
       q1[s] = weir_flux<R>(I, s, iv.zs_nm, iv.zs_nmu); u1[s] = R(0);
@@ -928,6 +995,46 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
   } else {
     cnt[0]++;
   }
+}
+
+// The stencil of dense uv-point (n, m) of plane `vplane` beyond its branch inputs: the q
+// and uv of its neighbours in both planes (missing ones read the dummy slot or the point
+// itself, as the reference's tables for a full rectangle give them), the kcuv of its two
+// neighbours along its direction, its advection mask and the cell values of its two cells.
+template <typename R, bool SUBGRID, typename QC>
+__device__ __forceinline__ void point_stencil(const DevInst& I, int n, int m, int vplane, int kf, QC q0, QC u0,
+                                              sfx::UvIn<R>& in) {
+  const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
+  const bool hu = m < mmax - 1, hv = n < nmax - 1;
+  const int c = m * nmax + n;
+  const int s = vplane ? np + c : c;
+  const int c_nmu = vplane ? c + 1 : c + nmax;
+  int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
+  if (!vplane) {
+    s_nmd = m >= 1 ? c - nmax : dm; s_nmu = m + 1 < mmax - 1 ? c + nmax : dm;
+    s_ndm = n >= 1 ? c - 1 : c; s_num = n < nmax - 1 ? c + 1 : c;
+    v_nm = hv ? np + c : dm; v_ndm = n >= 1 ? np + c - 1 : dm;
+    v_nmu = hv ? np + c + nmax : dm; v_ndmu = n >= 1 ? np + c + nmax - 1 : dm;
+  } else {
+    s_nmd = n >= 1 ? s - 1 : dm; s_nmu = n + 1 < nmax - 1 ? s + 1 : dm;
+    s_ndm = m >= 1 ? s - nmax : s; s_num = m < mmax - 1 ? s + nmax : s;
+    v_nm = hu ? c : dm; v_ndm = m >= 1 ? c - nmax : dm;
+    v_nmu = hu ? c + 1 : dm; v_ndmu = m >= 1 ? c + 1 - nmax : dm;
+  }
+  in.qx_nm = q0[s]; in.uu_nm = u0[s];
+  in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
+  in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
+  in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
+  in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
+  in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
+  in.kfuv = kf & 1; in.mask_adv = G(I.mask_adv)[s];
+  if (SUBGRID) {
+    in.vol_nm = G(I.vol)[c]; in.vol_nmu = G(I.vol)[c_nmu];
+    in.zsderv_nm = G((const R*)I.zsderv)[c]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
+  } else {
+    in.zb_nm = G(I.zb)[c]; in.zb_nmu = G(I.zb)[c_nmu];
+  }
+  in.idir = vplane; in.iref = 1; in.itype = 0;
 }
 
 // Dense scenarios, one thread per uv-point: tile 2t covers the u-points and tile 2t+1 the
@@ -942,7 +1049,7 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, f
 template <typename R, bool SUBGRID, typename QC, typename QW>
 __device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, int n, int m, int vplane, int kc, int kf,
                                              double zs_nm, double zs_nmu, float fl, R dt, QC q0, QC u0, QW q1,
-                                             QW u1, int* cnt, float& mdt) {
+                                             QW u1, int* cnt, R& mden) {
   const int nmax = I.nmax, mmax = I.mmax, np = I.np, dm = 2 * np;
   if (n >= nmax || m >= mmax) return true;
   const bool hu = m < mmax - 1, hv = n < nmax - 1;
@@ -974,34 +1081,10 @@ __device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, in
     return false;
   }
   // the rest of the stencil of a wet point
-  int s_nmd, s_nmu, s_ndm, s_num, v_nm, v_ndm, v_nmu, v_ndmu;
-  if (!vplane) {
-    s_nmd = m >= 1 ? c - nmax : dm; s_nmu = m + 1 < mmax - 1 ? c + nmax : dm;
-    s_ndm = n >= 1 ? c - 1 : c; s_num = n < nmax - 1 ? c + 1 : c;
-    v_nm = hv ? np + c : dm; v_ndm = n >= 1 ? np + c - 1 : dm;
-    v_nmu = hv ? np + c + nmax : dm; v_ndmu = n >= 1 ? np + c + nmax - 1 : dm;
-  } else {
-    s_nmd = n >= 1 ? s - 1 : dm; s_nmu = n + 1 < nmax - 1 ? s + 1 : dm;
-    s_ndm = m >= 1 ? s - nmax : s; s_num = m < mmax - 1 ? s + nmax : s;
-    v_nm = hu ? c : dm; v_ndm = m >= 1 ? c - nmax : dm;
-    v_nmu = hu ? c + 1 : dm; v_ndmu = m >= 1 ? c + 1 - nmax : dm;
-  }
-  in.qx_nm = q0[s]; in.uu_nm = u0[s];
-  in.qx_nmd = q0[s_nmd]; in.qx_nmu = q0[s_nmu];
-  in.qy_nm = q0[v_nm]; in.qy_nmu = q0[v_nmu]; in.qy_ndm = q0[v_ndm]; in.qy_ndmu = q0[v_ndmu];
-  in.uu_nmd = u0[s_nmd]; in.uu_nmu = u0[s_nmu]; in.uu_ndm = u0[s_ndm]; in.uu_num = u0[s_num];
-  in.v_ndm = u0[v_ndm]; in.v_ndmu = u0[v_ndmu]; in.v_nm = u0[v_nm]; in.v_nmu = u0[v_nmu];
-  in.kcuv_nmd = G(I.kcuv)[s_nmd]; in.kcuv_nmu = G(I.kcuv)[s_nmu];
-  in.kfuv = kf & 1; in.mask_adv = G(I.mask_adv)[s];
-  if (SUBGRID) {
-    in.vol_nm = G(I.vol)[c]; in.vol_nmu = G(I.vol)[c_nmu];
-    in.zsderv_nm = G((const R*)I.zsderv)[c]; in.zsderv_nmu = G((const R*)I.zsderv)[c_nmu];
-  } else {
-    in.zb_nm = G(I.zb)[c]; in.zb_nmu = G(I.zb)[c_nmu];
-  }
-  in.idir = vplane; in.iref = 1; in.itype = 0; in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+  point_stencil<R, SUBGRID>(I, n, m, vplane, kf, q0, u0, in);
+  in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
   int br = 0;
-  wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mdt, br, kf);
+  wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mden, br, kf);
   cnt[br]++;
   return false;
 }
@@ -1013,7 +1096,7 @@ __device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, in
 // scalars are read and the active test is made. Returns false if the scenario is not
 // active (nothing is written).
 template <typename R, bool SUBGRID, int P>
-__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, float& mdt, bool& held) {
+__device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int tile, int* cnt, R& mden, bool& held) {
   const auto S = G(Sp);
   const int nmax = I.nmax, mmax = I.mmax, np = I.np;
   const int vplane = tile & 1;
@@ -1048,7 +1131,7 @@ __device__ __forceinline__ bool mom_point(const DevInst& I, Scalars* Sp, int til
 #pragma unroll
   for (int j = 0; j < P; ++j)
     held = point_finish<R, SUBGRID>(I, S, n, m0 + j * I.mtm, vplane, kc[j], kf[j], za[j], zb[j], fl[j], dt, q0, u0, q1,
-                                    u1, cnt, mdt) && held;
+                                    u1, cnt, mden) && held;
   return true;
 }
 
@@ -1125,7 +1208,7 @@ __device__ __forceinline__ void strip_nbrs(Row<R>& w) {
 }
 
 template <typename R, bool SUBGRID>
-__device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, float& mdt) {
+__device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, R& mden) {
   const auto S = G(I.sc);
   const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
@@ -1176,7 +1259,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = C.vol; in.zsderv_nm = B.zd; in.zsderv_nmu = C.zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)C.zd; }
           int br = 0;
-          finish_uv<R, SUBGRID>(I, c, in, uv_floor<SUBGRID>(I, c), dt, q1, u1, mdt, br);
+          finish_uv<R, SUBGRID>(I, c, in, uv_floor<SUBGRID>(I, c), dt, q1, u1, mden, br);
           cnt[br]++;
         } else if (B.ku == 3) {
           q1[c] = weir_flux<R>(I, c, B.z, C.z); u1[c] = R(0);
@@ -1208,7 +1291,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = B.r_vol; in.zsderv_nm = B.zd; in.zsderv_nmu = B.r_zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)B.r_zd; }
           int br = 0;
-          finish_uv<R, SUBGRID>(I, s, in, uv_floor<SUBGRID>(I, s), dt, q1, u1, mdt, br);
+          finish_uv<R, SUBGRID>(I, s, in, uv_floor<SUBGRID>(I, s), dt, q1, u1, mden, br);
           cnt[br]++;
         } else if (B.kv == 3) {
           q1[s] = weir_flux<R>(I, s, B.z, B.r_z); u1[s] = R(0);
@@ -1229,7 +1312,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
 // As on dense meshes, the branch inputs come first and only wet points read the
 // neighbour tables and values.
 template <typename R, bool SUBGRID>
-__device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, float& mdt) {
+__device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, R& mden) {
   const auto S = G(I.sc);
   const int cur = __builtin_amdgcn_readfirstlane(S->cur);
   const R dt = (R)S->dt;
@@ -1293,7 +1376,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
     in.zb_nm = G(I.zb)[nm]; in.zb_nmu = G(I.zb)[nmu];
   }
   int br = 0;
-  wet_uv<R, SUBGRID>(I, ip, in, tb, zsu, dt, q1, u1, mdt, br, kf);
+  wet_uv<R, SUBGRID>(I, ip, in, tb, zsu, dt, q1, u1, mden, br, kf);
   cnt[br]++;
 }
 
@@ -1303,7 +1386,11 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, fl
 // sweep-done flag is raised. Positive floats order like their bit patterns, so the
 // minimum is an unsigned atomic min. Waves with no value below dtmax (every dry wave)
 // skip the reduction.
-__device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, float mdt, const int* cnt) {
+template <typename R>
+__device__ __forceinline__ void mom_epilogue(const DevInst& I, int tile, R mden, const int* cnt) {
+  // the thread's step: min(dtmax, 1 / den) over its wet points (dtmax when it has none)
+  float mdt = I.c.dtmax;
+  if (mden > R(0)) mdt = fminf(mdt, (float)(R(1) / mden));
   const float v = mdt == mdt && mdt > 0.f ? mdt : I.c.dtmax;
   if (__ballot(v < I.c.dtmax)) {
     const uint32_t m = wave_min_last(__float_as_uint(v));
@@ -1352,10 +1439,10 @@ __global__ void __launch_bounds__(BLOCK) k_mom_cell(const DevInst* __restrict__ 
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
-  float mdt = I.c.dtmax;
+  R mden = R(0);
   int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
-  mom_cell<R, SUBGRID>(I, tile, cnt, mdt);
-  mom_epilogue(I, tile, mdt, cnt);
+  mom_cell<R, SUBGRID>(I, tile, cnt, mden);
+  mom_epilogue<R>(I, tile, mden, cnt);
 }
 
 template <typename R, bool SUBGRID, int P>
@@ -1364,20 +1451,20 @@ __global__ void __launch_bounds__(BLOCK) k_mom_point(const DevInst* __restrict__
   if (!work_item(L, blockIdx.x, gridDim.x, tile, inst)) return;
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
-  float mdt = I.c.dtmax;
+  R mden = R(0);
   int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
   if (G(I.mq)[tile] == 0x01010101u && G(I.md)[tile] == 0) {   // quiescent: nothing would change
     if (!G(Sp)->active) return;
     if (I.count) count_tile_points<P>(I, tile, cnt);
-    mom_epilogue(I, tile, mdt, cnt);
+    mom_epilogue<R>(I, tile, mden, cnt);
     return;
   }
   bool held;
-  if (!mom_point<R, SUBGRID, P>(I, Sp, tile, cnt, mdt, held)) return;   // scenario not active
+  if (!mom_point<R, SUBGRID, P>(I, Sp, tile, cnt, mden, held)) return;   // scenario not active
   if (threadIdx.x == 0) G(I.md)[tile] = 0;
   const bool wave_held = __ballot(!held) == 0;
   if (threadIdx.x % WAVE == 0) G((uint8_t*)I.mq)[4 * (uint32_t)tile + threadIdx.x / WAVE] = wave_held ? 1 : 0;
-  mom_epilogue(I, tile, mdt, cnt);
+  mom_epilogue<R>(I, tile, mden, cnt);
 }
 
 template <typename R, bool SUBGRID>
@@ -1387,10 +1474,10 @@ __global__ void __launch_bounds__(BLOCK) k_mom_strip(const DevInst* __restrict__
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
-  float mdt = I.c.dtmax;
+  R mden = R(0);
   int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
-  mom_strip<R, SUBGRID>(I, tile, cnt, mdt);
-  mom_epilogue(I, tile, mdt, cnt);
+  mom_strip<R, SUBGRID>(I, tile, cnt, mden);
+  mom_epilogue<R>(I, tile, mden, cnt);
 }
 
 template <typename R, bool SUBGRID>
@@ -1400,10 +1487,10 @@ __global__ void __launch_bounds__(BLOCK) k_mom_idx(const DevInst* __restrict__ i
   const DevInst& I = insts[inst];
   Scalars* Sp = L.direct ? L.sc : I.sc;
   if (!G(Sp)->active) return;
-  float mdt = I.c.dtmax;
+  R mden = R(0);
   int cnt[NCNT] = {0, 0, 0, 0, 0, 0};
-  mom_idx<R, SUBGRID>(I, tile, cnt, mdt);
-  mom_epilogue(I, tile, mdt, cnt);
+  mom_idx<R, SUBGRID>(I, tile, cnt, mden);
+  mom_epilogue<R>(I, tile, mden, cnt);
 }
 
 // Combined uv-points at refinement transitions (sfincs_momentum.f90:757-779).
@@ -2087,6 +2174,16 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.bnd_overlap = 0;
     for (int i = 0; i < M->ngbnd; ++i)
       if (M->kcs[cell[i]] == 6 && (fr[cell[i]] || (nmi[i] >= 0 && fw[nmi[i]]))) I.bnd_overlap = 1;
+    std::vector<int32_t> g6;
+    for (int i = 0; i < M->ngbnd; ++i)
+      if (M->kcs[cell[i]] == 6) { g6.push_back(cell[i]); g6.push_back(nmi[i]); }
+    I.n6 = (int32_t)(g6.size() / 2);
+    I.gb6 = g6.empty() ? nullptr : H->upload(g6.data(), g6.size());
+    std::vector<int32_t> fl;
+    for (int i = 0; i < M->nkcuv2; ++i)
+      if (M->kcs[nmb[i]] != 6) fl.push_back(i);
+    I.nflux = (int32_t)fl.size();
+    I.k2_flux = fl.empty() ? nullptr : H->upload(fl.data(), fl.size());
   }
   I.nsrc = M->nsrc; I.ntsrc = M->ntsrc; I.ntprcp = M->ntprcp; I.ntwnd = M->ntwnd;
   I.ndrn = M->ndrn; I.structure_relax = M->p.structure_relax;
@@ -2630,12 +2727,12 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
     if (B->n_m[3]) hipLaunchKernelGGL((k_mom_strip<R, false>), dim3(B->n_m[3]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[3]), B->wl_m[3]);
     if (B->n_m[4]) hipLaunchKernelGGL((k_mom_idx<R, true>), dim3(B->n_m[4]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[4]), B->wl_m[4]);
     if (B->n_m[5]) hipLaunchKernelGGL((k_mom_idx<R, false>), dim3(B->n_m[5]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[5]), B->wl_m[5]);
-    if (B->n_m[6]) hipLaunchKernelGGL((k_mom_point<R, true, 1>), dim3(B->n_m[6]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[6]), B->wl_m[6]);
-    if (B->n_m[7]) hipLaunchKernelGGL((k_mom_point<R, false, 1>), dim3(B->n_m[7]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[7]), B->wl_m[7]);
-    if (B->n_m[8]) hipLaunchKernelGGL((k_mom_point<R, true, 2>), dim3(B->n_m[8]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[8]), B->wl_m[8]);
-    if (B->n_m[9]) hipLaunchKernelGGL((k_mom_point<R, false, 2>), dim3(B->n_m[9]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[9]), B->wl_m[9]);
-    if (B->n_m[10]) hipLaunchKernelGGL((k_mom_point<R, true, 4>), dim3(B->n_m[10]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[10]), B->wl_m[10]);
-    if (B->n_m[11]) hipLaunchKernelGGL((k_mom_point<R, false, 4>), dim3(B->n_m[11]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[11]), B->wl_m[11]);
+#define SFX_MOM_POINT(k, SG, P) \
+    if (B->n_m[k]) hipLaunchKernelGGL((k_mom_point<R, SG, P>), dim3(B->n_m[k]), dim3(BLOCK), 0, st, base_of(B, B->wl_m[k]), B->wl_m[k]);
+    SFX_MOM_POINT(6, true, 1) SFX_MOM_POINT(7, false, 1)
+    SFX_MOM_POINT(8, true, 2) SFX_MOM_POINT(9, false, 2)
+    SFX_MOM_POINT(10, true, 4) SFX_MOM_POINT(11, false, 4)
+#undef SFX_MOM_POINT
     if (B->n_cuv) hipLaunchKernelGGL(k_cuv<R>, dim3(B->n_cuv), dim3(BLOCK), 0, st, base_of(B, B->wl_cuv), B->wl_cuv);
     int tsa_slots = 0;   // timestep analysis of the members that have it on (not while tuning)
     for (sfx_handle* H : B->members)

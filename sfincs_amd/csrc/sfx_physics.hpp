@@ -125,7 +125,8 @@ struct UvTables {              // per-point tables (regular or subgrid)
 
 template <typename R>
 struct UvOut {
-  R q, uv, min_dt;
+  R q, uv;
+  R den;        // the point's CFL denominator: its required step is 1 / den (:731-733)
   int kfuv;
   int branch;   // 1 dry, 2 wet regular / subgrid full, 3 subgrid partial
 };
@@ -200,7 +201,6 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
     hu = vmax<R>(zsu - (R)tb.zbuvmx, (R)c.huthresh);
     gnavg2 = tb.gn2uv;
   }
-  const R hwet = hu / phi;
   R dzdx;
   if (c.slopelim < 9999.0f) {
     double d = (in.zs_nmu - in.zs_nm) * (double)dxuvinv;
@@ -259,6 +259,7 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
   }
   if (F & F_WIND) {
     const R tw = in.idir == 0 ? in.tauwu : in.tauwv;
+    const R hwet = hu / phi;
     if (hwet > R(0.25f)) frc = frc + phi * tw;
     else frc = frc + tw * hu * R(4);
   }
@@ -293,7 +294,7 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
   o.q = qn;
   o.uv = qn / vmax<R>(hu, (R)c.huvmin);
   o.kfuv = 1;
-  o.min_dt = R(1) / (vmax<R>(rsqrt_<R>(g * hu), rabs<R>(o.uv)) * dxuvinv);
+  o.den = vmax<R>(rsqrt_<R>(g * hu), rabs<R>(o.uv)) * dxuvinv;
   return o;
 }
 

@@ -288,6 +288,14 @@ def ensemble(args):
     print(f"# batched {bat['wall_s']:.2f}s {bat['cell_updates_per_s']:.3e}", file=sys.stderr, flush=True)
     dzs = max(float(np.max(np.abs(a - b))) for a, b in zip(seq["zs"], bat["zs"]))
     same_steps = seq["steps"] == bat["steps"]
+    # each member's rate alone (its cells x steps over its own wall time in the sequential
+    # run): the batch can at best run every member's work at the rate of the fastest one
+    done = [0.0] + list(seq["time_to_result_s"])
+    work = [c * n for c, n in zip(cells, seq["steps"])]
+    rates = [w / max(done[j + 1] - done[j], 1e-9) for j, w in enumerate(work)]
+    best = max(rates)
+    alone = {"min": min(rates), "median": float(np.median(rates)), "max": best,
+             "largest_member_rate": rates[int(np.argmax(cells))]}
     out = {
         "metric": "ensemble aggregate cell-updates/s (batched) and p50/p99 time-to-result",
         "value": bat["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True,
@@ -298,6 +306,8 @@ def ensemble(args):
         "batched": {k: bat[k] for k in ("wall_s", "cell_updates", "cell_updates_per_s", "p50_s", "p99_s")},
         "sequential": {k: seq[k] for k in ("wall_s", "cell_updates", "cell_updates_per_s", "p50_s", "p99_s")},
         "speedup": bat["cell_updates_per_s"] / seq["cell_updates_per_s"],
+        "sequential_member_rate": alone,
+        "speedup_if_every_member_ran_at_the_best_alone_rate": seq["wall_s"] * best / sum(work),
         "max_abs_dzs_batched_vs_alone": dzs, "same_step_counts": same_steps,
     }
     print(json.dumps(out))
@@ -403,6 +413,9 @@ def main():
     ap.add_argument("--duration-scale", type=float, default=1.0)
     ap.add_argument("--hbm-fraction", type=float, default=0.9)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--tuning", default="",
+                    help="pin this tuning (the JSON of a bench line's config.tuning) instead of autotuning, "
+                         "so that a profiled run times the shapes of a reported one")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, run a 1 GiB elementwise copy (torch.mul by 1) three "
                          "times: a kernel of known HBM traffic for scaling PMC byte counters")
@@ -438,7 +451,11 @@ def main():
     if args.start_time > 0:
         inst.update(float(args.start_time))
     t_start = inst.time()[0]
-    tuning = inst.autotune(11)
+    if args.tuning:
+        inst.pin_tuning(json.loads(args.tuning))
+        tuning = inst.tuning()
+    else:
+        tuning = inst.autotune(11)
     batch = _lib.Batch()
     batch.add(inst, 1.0e9)
     print(f"# generated {sizes['np']} cells in {t_gen:.1f}s; window starts at t={t_start:.0f}s; tuning {tuning}",

@@ -18,7 +18,34 @@
  *                                sfincs_structures.f90:592-697 (weirs)
  *   oracle_update                sfincs_lib.f90:337-683 (step order, lagged dt)
  * Arrays use 0-based storage; index tables hold the reference's 1-based values.
- * Each function's parity check is named in the test files under tests/.
+ *
+ * What pins each function (tests/):
+ *   oracle_compute_fluxes        bit for bit against the reference's compute_fluxes after
+ *                                every step (test_oracle_vs_ref.py: regular variants, subgrid,
+ *                                wind, quadtree, viscosity, sources; test_subgrid_binary.py,
+ *                                test_drainage.py, test_structures.py); the CFL step on a
+ *                                uniform depth and a dry domain by closed form
+ *                                (test_forcing_pins.py)
+ *   oracle_compute_water_levels  bit for bit against the reference's compute_water_levels as
+ *                                above, with store variables, qext, storage volume and the
+ *                                timestep analysis (test_oracle_vs_ref.py); rain volume R A T
+ *                                on a closed basin (test_forcing_pins.py)
+ *   oracle_update_meteo          wind stress against the Cd table, rain and wind series
+ *                                interpolation at known times (test_forcing_pins.py)
+ *   oracle_update_boundaries     a hand-computed weakly reflective flux and its dry guards,
+ *                                the uvmean filter as a geometric series, series
+ *                                interpolation, the bed floor, spin-up, Neumann copies
+ *                                (test_forcing_pins.py); river cells (test_river.py);
+ *                                infragravity levels (test_ig_waves.py)
+ *   oracle_update_discharges     discharge series interpolation (test_forcing_pins.py);
+ *                                pump, culvert, check valve and gate closed forms
+ *                                (test_drainage.py)
+ *   oracle_compute_fluxes_over_structures
+ *                                the broad-crested weir formula in double, free, submerged
+ *                                and below the crest, both directions (test_structures.py)
+ *   oracle_update                step counts and model time against the GPU and the
+ *                                reference loop (test_oracle_vs_ref.py); the instability stop
+ *                                by closed form (test_forcing_pins.py)
  */
 #include <math.h>
 #include <stdint.h>

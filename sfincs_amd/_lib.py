@@ -478,6 +478,10 @@ class Instance:
 
     def autotune(self, trials=3):
         check(lib().sfx_autotune(self.h, trials), "autotune")
+        return self.tuning()
+
+    def tuning(self):
+        """The shapes and work orders in use, as autotune() reports them."""
         a = (C.c_int32 * 8)()
         check(lib().sfx_get_tuning(self.h, a), "get_tuning")
         return dict(dense=a[0], momentum_kernel=a[1], momentum_tile=[a[2], a[3]], continuity_tile=[a[4], a[5]],

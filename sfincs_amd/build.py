@@ -44,15 +44,17 @@ def build(verbose=True):
 
 
 def tables():
-    """Enumerate (or load and re-verify) the C library's powf tables next to the library."""
+    """Enumerate (or load and re-verify) the C library's powf, cosf and sinf tables next to
+    the library."""
     import ctypes
     lib = ctypes.CDLL(LIB)
-    counts = (ctypes.c_int64 * 4)()
+    counts = (ctypes.c_int64 * 6)()
     lib.sfx_libm_tables.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
     rc = lib.sfx_libm_tables(None, counts)
     if rc != 0:
         raise RuntimeError("powf table generation failed")
-    print("powf tables: " + " + ".join(str(c) for c in counts) + " entries", flush=True)
+    print("libm tables (powf 1/3, 5/3, 7/3, 1.5; cosf; sinf): " + " + ".join(str(c) for c in counts) + " entries",
+          flush=True)
 
 
 def build_fortran_host():

@@ -465,8 +465,11 @@ __device__ void step_series(const DevInst& I, StepImage& F, float* qout, int mod
         else if (dr0 > dr1 + c.pi) dr1 = dr1 + 2 * c.pi;
         const float vdir = dr0 * (1.0f - twfac) + dr1 * twfac;
         const float cd = G(I.cdval)[min(max((int)(vmag * 10) + 1 - 1, 0), 999)];
-        F.s.tauwu = vmag * vmag * cosf(vdir) * c.rhoa * cd / c.rhow;
-        F.s.tauwv = vmag * vmag * sinf(vdir) * c.rhoa * cd / c.rhow;
+        // cos and sin as the C library's cosf and sinf give them (exact tables)
+        const float cv = sfx::libm_trig(vdir, 0, (const uint64_t*)I.pw.tcos, (const uint64_t*)I.pw.tsin);
+        const float sv = sfx::libm_trig(vdir, 1, (const uint64_t*)I.pw.tcos, (const uint64_t*)I.pw.tsin);
+        F.s.tauwu = vmag * vmag * cv * c.rhoa * cd / c.rhow;
+        F.s.tauwv = vmag * vmag * sv * c.rhoa * cd / c.rhow;
         F.s.itwndlast = itw - 1;
         break;
       }

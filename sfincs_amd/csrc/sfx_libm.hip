@@ -25,15 +25,20 @@ namespace sfx {
 namespace {
 
 constexpr uint32_t MAGIC = 0x5346504Du;   // "SFPM"
-constexpr uint32_t VERSION = 8;
+constexpr uint32_t VERSION = 9;
+constexpr int NFUN = NPOW + 2;   // the four powf exponents, then cosf and sinf
 
 struct Entry { uint32_t x, g; };
 
 struct PowSet {
-  std::vector<Entry> e[NPOW];
-  std::vector<Entry> probe[NPOW];
+  std::vector<Entry> e[NFUN];
+  std::vector<Entry> probe[NFUN];
   bool ok = false;
 };
+
+bool is_trig(int which) { return which >= NPOW; }
+uint32_t slots_of(int which) { return is_trig(which) ? TRIG_HASH_SIZE : POW_HASH_SIZE; }
+float libc_value(int which, float x);
 
 std::mutex g_mu;
 PowSet g_set;
@@ -41,6 +46,46 @@ std::map<int, PowTables> g_dev;
 
 float exponent(int which) {
   return which == 0 ? 1.0f / 3.0f : (which == 1 ? 5.0f / 3.0f : (which == 2 ? 7.0f / 3.0f : 1.5f));
+}
+
+float libc_value(int which, float x) {
+  if (which == NPOW) return cosf(x);
+  if (which == NPOW + 1) return sinf(x);
+  return powf(x, exponent(which));
+}
+
+// cosf / sinf over [lo, hi) bit patterns (non-negative floats), as enumerate below: the
+// estimate within 2^-46 of the double result, every misrounding of the C library inside
+// the 2^-27 band, and the table holding those and the x whose result lies within 2^-40
+// of a midpoint.
+bool enumerate_trig(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std::string& err) {
+  const int fn = which - NPOW;
+  for (uint32_t u = lo; u < hi; ++u) {
+    const float x = u2f(u);
+    const float g = fn ? sinf(x) : cosf(x);
+    const double r = fn ? sin((double)x) : cos((double)x);
+    const double est = trig_estimate(x, fn);
+    const double ar = std::fabs(r);
+    if (std::fabs(est - r) > 1.4210854715202004e-14 * ar) {
+      err = "trig estimate error too large at x=" + std::to_string(x);
+      return false;
+    }
+    const float f = (float)r;
+    const float af = std::fabs(f);
+    const float nb = ar >= (double)af ? nextafterf(af, INFINITY) : nextafterf(af, 0.0f);
+    const double mid = 0.5 * ((double)af + (double)nb);
+    const double dist = std::fabs(ar - mid);
+    if (g != f) {
+      if (!(dist < 5.268356063861754e-09 * ar)) {            // 2^-27.5: inside the device band of 2^-27
+        err = "cosf/sinf differ outside the midpoint band at x=" + std::to_string(x);
+        return false;
+      }
+      out.push_back(Entry{u, f2u(g)});
+    } else if (dist < 9.094947017729282e-13 * ar) {         // 2^-40: estimate may straddle
+      out.push_back(Entry{u, f2u(g)});
+    }
+  }
+  return true;
 }
 
 // Enumerate one exponent over [lo, hi) bit patterns; returns false if a verification fails.
@@ -81,11 +126,11 @@ bool enumerate(int which, uint32_t lo, uint32_t hi, std::vector<Entry>& out, std
 }
 
 bool generate(PowSet& S, std::string& err) {
-  const uint32_t lo = f2u(POW_XLO), hi = f2u(POW_XHI);
   unsigned nt = std::thread::hardware_concurrency();
   if (nt == 0) nt = 4;
   nt = std::min(nt, 64u);
-  for (int which = 0; which < NPOW; ++which) {
+  for (int which = 0; which < NFUN; ++which) {
+    const uint32_t lo = is_trig(which) ? 0u : f2u(POW_XLO), hi = is_trig(which) ? f2u(TRIG_XHI) : f2u(POW_XHI);
     std::vector<std::vector<Entry>> parts(nt);
     std::vector<std::string> errs(nt);
     std::vector<char> oks(nt, 1);
@@ -93,7 +138,9 @@ bool generate(PowSet& S, std::string& err) {
     const uint32_t span = (hi - lo + nt - 1) / nt;
     for (unsigned k = 0; k < nt; ++k) {
       const uint32_t a = lo + k * span, b = std::min(hi, a + span);
-      th.emplace_back([&, k, a, b]() { oks[k] = enumerate(which, a, b, parts[k], errs[k]); });
+      th.emplace_back([&, k, a, b]() {
+        oks[k] = is_trig(which) ? enumerate_trig(which, a, b, parts[k], errs[k]) : enumerate(which, a, b, parts[k], errs[k]);
+      });
     }
     for (auto& t : th) t.join();
     S.e[which].clear();
@@ -101,7 +148,7 @@ bool generate(PowSet& S, std::string& err) {
       if (!oks[k]) { err = errs[k]; return false; }
       S.e[which].insert(S.e[which].end(), parts[k].begin(), parts[k].end());
     }
-    if (S.e[which].size() > POW_HASH_SIZE / 2) { err = "powf table larger than the hash capacity"; return false; }
+    if (S.e[which].size() > slots_of(which) / 2) { err = "libm table larger than the hash capacity"; return false; }
     // probes: every 997th entry, re-checked against the live powf when the file is loaded
     S.probe[which].clear();
     for (size_t i = 0; i < S.e[which].size(); i += 997) S.probe[which].push_back(S.e[which][i]);
@@ -111,9 +158,9 @@ bool generate(PowSet& S, std::string& err) {
 }
 
 bool probes_match(const PowSet& S) {
-  for (int which = 0; which < NPOW; ++which)
+  for (int which = 0; which < NFUN; ++which)
     for (const Entry& p : S.probe[which])
-      if (f2u(powf(u2f(p.x), exponent(which))) != p.g) return false;
+      if (f2u(libc_value(which, u2f(p.x))) != p.g) return false;
   return true;
 }
 
@@ -123,9 +170,9 @@ bool load_file(const std::string& path, PowSet& S) {
   uint32_t hdr[4];
   bool ok = std::fread(hdr, 4, 4, f) == 4 && hdr[0] == MAGIC && hdr[1] == VERSION && hdr[2] == f2u(POW_XLO) &&
             hdr[3] == f2u(POW_XHI);
-  for (int which = 0; ok && which < NPOW; ++which) {
+  for (int which = 0; ok && which < NFUN; ++which) {
     uint64_t n = 0, np = 0;
-    ok = std::fread(&n, 8, 1, f) == 1 && std::fread(&np, 8, 1, f) == 1 && n <= POW_HASH_SIZE / 2 && np <= n;
+    ok = std::fread(&n, 8, 1, f) == 1 && std::fread(&np, 8, 1, f) == 1 && n <= slots_of(which) / 2 && np <= n;
     if (!ok) break;
     S.e[which].resize(n);
     S.probe[which].resize(np);
@@ -143,7 +190,7 @@ void save_file(const std::string& path, const PowSet& S) {
   if (!f) return;
   const uint32_t hdr[4] = {MAGIC, VERSION, f2u(POW_XLO), f2u(POW_XHI)};
   bool ok = std::fwrite(hdr, 4, 4, f) == 4;
-  for (int which = 0; ok && which < NPOW; ++which) {
+  for (int which = 0; ok && which < NFUN; ++which) {
     const uint64_t n = S.e[which].size(), np = S.probe[which].size();
     ok = std::fwrite(&n, 8, 1, f) == 1 && std::fwrite(&np, 8, 1, f) == 1 &&
          std::fwrite(S.e[which].data(), sizeof(Entry), n, f) == n &&
@@ -154,11 +201,11 @@ void save_file(const std::string& path, const PowSet& S) {
   else std::remove(tmp.c_str());
 }
 
-std::vector<uint2> hash_table(const std::vector<Entry>& e) {
-  std::vector<uint2> t(POW_HASH_SIZE, make_uint2(0u, 0u));
+std::vector<uint2> hash_table(const std::vector<Entry>& e, uint32_t slots = POW_HASH_SIZE) {
+  std::vector<uint2> t(slots, make_uint2(0u, 0u));
   for (const Entry& x : e) {
-    uint32_t h = pow_hash(x.x);
-    while (t[h].x != 0u) h = (h + 1) & (POW_HASH_SIZE - 1);
+    uint32_t h = hash_bits(x.x) & (slots - 1);
+    while (t[h].x != 0u) h = (h + 1) & (slots - 1);
     t[h] = make_uint2(x.x, x.g);
   }
   return t;
@@ -188,10 +235,10 @@ bool ensure_host(std::string& err) {
 }  // namespace
 
 // Host copy of the tables (for the host-side checks of the emulation).
-bool pow_tables_host(std::vector<uint2> (&t)[NPOW], std::string& err) {
+bool pow_tables_host(std::vector<uint2> (&t)[NFUN], std::string& err) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!ensure_host(err)) return false;
-  for (int which = 0; which < NPOW; ++which) t[which] = hash_table(g_set.e[which]);
+  for (int which = 0; which < NFUN; ++which) t[which] = hash_table(g_set.e[which], slots_of(which));
   return true;
 }
 
@@ -202,15 +249,15 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
   if (it != g_dev.end()) { out = it->second; return true; }
   if (!ensure_host(err)) return false;
   PowTables T{};
-  for (int which = 0; which < NPOW; ++which) {
-    std::vector<uint2> h = hash_table(g_set.e[which]);
+  for (int which = 0; which < NFUN; ++which) {
+    std::vector<uint2> h = hash_table(g_set.e[which], slots_of(which));
     void* p = nullptr;
     if (hipMalloc(&p, h.size() * sizeof(uint2)) != hipSuccess ||
         hipMemcpy(p, h.data(), h.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
       err = "upload of the powf tables failed";
       return false;
     }
-    const uint2** slot[NPOW] = {&T.t13, &T.t53, &T.t73, &T.t15};
+    const uint2** slot[NFUN] = {&T.t13, &T.t53, &T.t73, &T.t15, &T.tcos, &T.tsin};
     *slot[which] = (const uint2*)p;
   }
   g_dev[device] = T;
@@ -221,7 +268,7 @@ bool pow_tables_device(int device, PowTables& out, std::string& err) {
 }  // namespace sfx
 
 // Build (or load and verify) the powf tables; path NULL = next to the library.
-extern "C" int sfx_libm_tables(const char* path, int64_t counts[4]) {
+extern "C" int sfx_libm_tables(const char* path, int64_t counts[6]) {
   std::lock_guard<std::mutex> lk(sfx::g_mu);
   std::string err;
   if (path && *path) {
@@ -236,24 +283,26 @@ extern "C" int sfx_libm_tables(const char* path, int64_t counts[4]) {
     return SFX_ERR_STATE;
   }
   if (counts)
-    for (int k = 0; k < sfx::NPOW; ++k) counts[k] = (int64_t)sfx::g_set.e[k].size();
+    for (int k = 0; k < sfx::NFUN; ++k) counts[k] = (int64_t)sfx::g_set.e[k].size();
   return SFX_OK;
 }
 
-// Host evaluation of the emulation, for tests: out[i] = powf(x[i], exponent `which`).
+// Host evaluation of the emulation, for tests: out[i] = powf(x[i], exponent `which`) for
+// which 0..3, cosf(x[i]) for 4, sinf(x[i]) for 5.
 extern "C" int sfx_libm_pow_host(const float* x, int64_t n, int32_t which, float* out) {
-  std::vector<uint2> t[sfx::NPOW];
+  std::vector<uint2> t[sfx::NFUN];
   std::string err;
   if (!sfx::pow_tables_host(t, err)) { sfx::set_error(err); return SFX_ERR_STATE; }
-  sfx::PowTables T{t[0].data(), t[1].data(), t[2].data(), t[3].data()};
-  for (int64_t i = 0; i < n; ++i) out[i] = sfx::libm_pow(x[i], which, T);
+  sfx::PowTables T{t[0].data(), t[1].data(), t[2].data(), t[3].data(), t[4].data(), t[5].data()};
+  for (int64_t i = 0; i < n; ++i)
+    out[i] = which >= sfx::NPOW ? sfx::libm_trig(x[i], which - sfx::NPOW, T) : sfx::libm_pow(x[i], which, T);
   return SFX_OK;
 }
 
 namespace {
 __global__ void k_libm_pow(const float* __restrict__ x, int64_t n, int which, sfx::PowTables T, float* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = sfx::libm_pow(x[i], which, T);
+  if (i < n) out[i] = which >= sfx::NPOW ? sfx::libm_trig(x[i], which - sfx::NPOW, T) : sfx::libm_pow(x[i], which, T);
 }
 }  // namespace
 

@@ -29,17 +29,29 @@ constexpr float POW_XHI = 16384.0f;                 // 2^14
 
 constexpr int NPOW = 4;   // exponents (float)(1/3), (float)(5/3), (float)(7/3), 1.5
 
+// cosf and sinf of the wind direction (sfincs_meteo.f90:1475-1478), exact like powf for
+// every float in [0, TRIG_XHI): the direction is the interpolation of two angles reduced
+// modulo 2 pi, one of them raised by 2 pi, so it lies in [0, 4 pi]. glibc's cosf/sinf
+// round up to 2^-27.9 (relative) away from a midpoint the other way, so their band is
+// 2^-27 and their tables hold about 0.6M and 1.1M entries: hashes of 2^22 slots.
+constexpr float TRIG_XHI = 12.6f;
+constexpr int TRIG_HASH_BITS = 22;
+constexpr uint32_t TRIG_HASH_SIZE = 1u << TRIG_HASH_BITS;
+
 struct PowTables {
   const uint2* t13;   // open-addressing hash: {x bits, powf(x, 1/3f) bits}; key 0 = empty
   const uint2* t53;
   const uint2* t73;
   const uint2* t15;
+  const uint2* tcos;  // TRIG_HASH_SIZE slots each
+  const uint2* tsin;
 };
 
-SFX_HD inline uint32_t pow_hash(uint32_t k) {
+SFX_HD inline uint32_t hash_bits(uint32_t k) {
   k ^= k >> 16; k *= 0x7feb352dU; k ^= k >> 15; k *= 0x846ca68bU; k ^= k >> 16;
-  return k & (POW_HASH_SIZE - 1);
+  return k;
 }
+SFX_HD inline uint32_t pow_hash(uint32_t k) { return hash_bits(k) & (POW_HASH_SIZE - 1); }
 
 SFX_HD inline uint32_t f2u(float f) { union { float f; uint32_t u; } v; v.f = f; return v.u; }
 SFX_HD inline float u2f(uint32_t u) { union { float f; uint32_t u; } v; v.u = u; return v.f; }
@@ -127,18 +139,67 @@ SFX_HD inline bool near_midpoint_fp(double r) {
 }
 
 // Table entries read as 64-bit words: key in the low half, powf bits in the high half
-// (the little-endian layout of uint2{key, value}).
+// (the little-endian layout of uint2{key, value}). `mask` = slots - 1.
 template <typename P>
-SFX_HD inline float pow_lookup(P t, uint32_t key, float fallback) {
-  uint32_t h = pow_hash(key);
+SFX_HD inline float pow_lookup(P t, uint32_t key, float fallback, uint32_t mask = POW_HASH_SIZE - 1) {
+  uint32_t h = hash_bits(key) & mask;
   for (int probe = 0; probe < 64; ++probe) {
     const uint64_t e = t[h];
     const uint32_t k = (uint32_t)e;
     if (k == key) return u2f((uint32_t)(e >> 32));
     if (k == 0u) break;
-    h = (h + 1) & (POW_HASH_SIZE - 1);
+    h = (h + 1) & mask;
   }
   return fallback;
+}
+
+// Double estimate of cos(x) (which = 0) or sin(x) (which = 1) for a float x >= 0 (below
+// 2^23 pi/2; the tables cover [0, TRIG_XHI)), relative error below 2^-46 (checked for
+// every float of the table range when the tables are built). x = k pi/2 + r with
+// pi/2 = P1 + P2 + P3 (P1, P2 of 30 bits: k P1 and k P2 are exact and so is x - k P1,
+// both being multiples of P1's last bit), |r| <= pi/4, then Taylor series of sin and
+// cos in r^2 to the r^19 and r^20 terms (truncation below 2^-60 on |r| <= pi/4).
+// IEEE basic operations in the order written, so host and device agree to the bit.
+SFX_HD inline double trig_estimate(float x, int which) {
+  const double xd = (double)x;
+  const double k = floor(xd * 0.6366197723675814 + 0.5);
+  const double P1 = 1.5707963276654482, P2 = -8.705515692000731e-10, P3 = -3.50343439808993e-19;
+  const double r = ((xd - k * P1) - k * P2) - k * P3;
+  const double r2 = r * r;
+  const double sp = ((((((((-8.22063524662433e-18 * r2 + 2.8114572543455206e-15) * r2 - 7.647163731819816e-13) * r2 +
+                          1.6059043836821613e-10) * r2 - 2.505210838544172e-08) * r2 + 2.7557319223985893e-06) * r2 -
+                       0.0001984126984126984) * r2 + 0.008333333333333333) * r2 - 0.16666666666666666);
+  const double sn = r + r * r2 * sp;
+  const double cp = (((((((((4.110317623312165e-19 * r2 - 1.5619206968586225e-16) * r2 + 4.779477332387385e-14) * r2 -
+                           1.1470745597729725e-11) * r2 + 2.08767569878681e-09) * r2 - 2.755731922398589e-07) * r2 +
+                        2.48015873015873e-05) * r2 - 0.001388888888888889) * r2 + 0.041666666666666664) * r2 - 0.5);
+  const double cs = 1.0 + r2 * cp;
+  const int q = ((int)k + (which ? 3 : 0)) & 3;   // sin x = cos(x - pi/2)
+  return q == 0 ? cs : (q == 1 ? -sn : (q == 2 ? -cs : sn));
+}
+
+// True when |r| lies within 2^-27 (relative) of a midpoint between two adjacent floats,
+// on the double's bits as near_midpoint: the threshold is M / 2^27 units of 2^(E-52).
+SFX_HD inline bool near_midpoint27(double r) {
+  union { double d; uint64_t u; } v;
+  v.d = r;
+  const uint64_t M = (1ull << 52) | (v.u & ((1ull << 52) - 1));
+  const uint32_t low29 = (uint32_t)v.u & 0x1fffffffu;
+  const uint32_t k = low29 >= 0x10000000u ? low29 - 0x10000000u : 0x10000000u - low29;
+  const uint64_t thr = M >> 27;
+  return (uint64_t)k < thr || ((uint64_t)k == thr && (M & ((1ull << 27) - 1)) != 0);
+}
+
+// cosf(x) (which = 0) or sinf(x) (which = 1) as the C library computes them, for
+// 0 <= x < TRIG_XHI (and -x by symmetry); other arguments get the rounded estimate.
+template <typename P>
+SFX_HD inline float libm_trig(float x, int which, P tcos, P tsin) {
+  const bool neg = (f2u(x) >> 31) != 0u;   // -0.0 included: sinf(-0) = -0
+  const float a = u2f(f2u(x) & 0x7fffffffu);
+  const double r = trig_estimate(a, which);
+  float f = (float)r;
+  if (a < TRIG_XHI && near_midpoint27(r)) f = pow_lookup(which ? tsin : tcos, f2u(a), f, TRIG_HASH_SIZE - 1);
+  return (neg && which) ? -f : f;
 }
 
 // powf(x, (float)(1/3)), powf(x, (float)(5/3)), powf(x, (float)(7/3)) or powf(x, 1.5f)
@@ -151,6 +212,9 @@ SFX_HD inline float libm_pow(float x, int which, P t13, P t53, P t73, P t15) {
   if (x >= POW_XLO && x < POW_XHI && near_midpoint(r))
     return pow_lookup(which == 0 ? t13 : (which == 1 ? t53 : (which == 2 ? t73 : t15)), f2u(x), f);
   return f;
+}
+SFX_HD inline float libm_trig(float x, int which, const PowTables& T) {
+  return libm_trig(x, which, (const uint64_t*)T.tcos, (const uint64_t*)T.tsin);
 }
 SFX_HD inline float libm_pow(float x, int which, const PowTables& T) {
   return libm_pow(x, which, (const uint64_t*)T.t13, (const uint64_t*)T.t53, (const uint64_t*)T.t73,

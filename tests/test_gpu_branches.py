@@ -2,7 +2,8 @@
 the h^(7/3) table (sfincs_momentum.f90:787-820), point sources on regular and quadtree
 meshes with the bed clamp of the regular branch (sfincs_continuity.f90:87-106), Neumann
 side boundaries (msk = 6, sfincs_boundaries.f90:903-913), boundary spin-up
-(:789-826 with weighted_average :1179-1200), a four-level quadtree with wind, and the
+(:789-826 with weighted_average :1179-1200), a four-level quadtree with wind and a turning
+wind (bit-exact: the wind's cosf/sinf through exact tables), and the
 instability stop (sfincs_lib.f90:634-648), drainage structures
 (sfincs_discharges.f90:371-655) on regular and subgrid meshes, and weirs with thin dams
 (sfincs_structures.f90:385-495, 592-697) on regular, subgrid and quadtree meshes.
@@ -98,12 +99,32 @@ def test_boundary_spinup_matches_oracle(kind):
 
 
 def test_four_level_quadtree_with_wind_matches_oracle():
-    """The configuration-2 recipe (4 levels, surge, 20 m/s SW wind) on a small base grid;
-    wind enters through cosf/sinf, so the gate is the tolerance, not bit equality."""
+    """The configuration-2 recipe (4 levels, surge, 20 m/s SW wind) on a small base grid.
+    The wind stress's cos and sin are the C library's cosf/sinf results on the device too
+    (exact tables), so the run is bit-identical to the oracle, not only within tolerance."""
     print("case quadtree 4 levels", flush=True)
     sc = W.quadtree_case(12, 16, nlev=4, dx0=100.0, seed=67, t_end=1800.0, surge_amp=3.0)
     m, inst, orc = _full_run(sc)
     assert m.sizes()["nref"] == 4 and m.sizes()["ncuv"] > 0
+    assert orc.vals["wind"] == 1
+    for f in ("zs", "q", "uv", "zsmax"):
+        assert np.array_equal(inst.get(f), orc.state(f)), f
+
+
+def test_turning_wind_is_bit_exact():
+    """A wind turning through every direction, and across north (where the two series
+    angles are reduced modulo 2 pi and one is raised by 2 pi), over a regular tidal beach:
+    GPU and oracle stay bit-identical."""
+    print("case turning wind", flush=True)
+    sc = W.regular_case(36, 44, dx=50.0, seed=74, t_end=1800.0, tide_amp=1.5, rain_mmh=0.0)
+    tw = np.linspace(0.0, 2000.0, 9).astype(np.float32)
+    mag = np.array([5, 12, 20, 25, 18, 30, 8, 15, 22], np.float32)
+    direc = np.array([350, 10, 95, 181, 270, 359, 5, 200, 300], np.float32)
+    sc.wind = (tw, mag, direc)
+    m, inst, orc = _full_run(sc)
+    assert orc.vals["wind"] == 1
+    for f in ("zs", "q", "uv", "zsmax"):
+        assert np.array_equal(inst.get(f), orc.state(f)), f
 
 
 @pytest.mark.parametrize("kind", ["regular", "subgrid"])

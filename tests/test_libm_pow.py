@@ -1,8 +1,9 @@
-"""x**(1/3), x**(5/3), x**(7/3) and x**1.5 with the C library's exact powf results
-(sfx_libm.hpp).
+"""x**(1/3), x**(5/3), x**(7/3) and x**1.5 with the C library's exact powf results, and
+cos and sin with its exact cosf and sinf results (sfx_libm.hpp).
 
 The reference evaluates hu**expo, hu**(5.0/3.0), hgate**(7.0/3.0) and the weir's h1**1.5
-with the C library's powf; the
+with the C library's powf, and the uniform wind's cos(vdir), sin(vdir) with its cosf and
+sinf (vdir in [0, 4 pi]); the
 emulation must return the same float for every input the tables cover. Checked here on
 the host implementation over every table key and a dense random sample, and on the
 device (GPU test) over the same inputs."""
@@ -16,13 +17,32 @@ from sfincs_amd import _lib
 libm = C.CDLL("libm.so.6")
 libm.powf.restype = C.c_float
 libm.powf.argtypes = [C.c_float, C.c_float]
+for _f in (libm.cosf, libm.sinf):
+    _f.restype = C.c_float
+    _f.argtypes = [C.c_float]
 EXP = [np.float32(1.0) / np.float32(3.0), np.float32(5.0) / np.float32(3.0), np.float32(7.0) / np.float32(3.0),
        np.float32(1.5)]
 
 
 def glibc_powf(x, which):
+    if which == 4:
+        return np.array([libm.cosf(float(v)) for v in x], np.float32)
+    if which == 5:
+        return np.array([libm.sinf(float(v)) for v in x], np.float32)
     y = float(EXP[which])
     return np.array([libm.powf(float(v), y) for v in x], np.float32)
+
+
+def angles(n, seed):
+    """Directions as the wind update makes them: uniform on [0, 4 pi], plus values next to
+    multiples of pi/2 (where cos or sin is small and the argument reduction matters),
+    zeros of both signs and the largest float below 4 pi + 0.03."""
+    r = np.random.default_rng(seed)
+    x = r.uniform(0.0, 4 * np.pi, n).astype(np.float32)
+    k = (np.arange(1, 9) * np.float64(np.pi) / 2).astype(np.float32)
+    near = np.concatenate([np.nextafter(k, np.float32(np.inf)), np.nextafter(k, np.float32(0)), k])
+    return np.concatenate([x, near, np.array([0.0, -0.0, 1e-30, 1e-8, np.nextafter(np.float32(12.6), np.float32(0))],
+                                             np.float32)])
 
 
 def exact_midpoints():
@@ -55,9 +75,9 @@ def emulate(x, which, device=False):
     return out
 
 
-@pytest.mark.parametrize("which", [0, 1, 2, 3])
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5])
 def test_host_emulation_equals_glibc(which):
-    x = sample(200000, 7 + which)
+    x = sample(200000, 7 + which) if which < 4 else angles(200000, 7 + which)
     assert np.array_equal(emulate(x, which).view(np.uint32), glibc_powf(x, which).view(np.uint32))
 
 
@@ -72,9 +92,9 @@ def test_midpoints_are_decided_by_the_table():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", [0, 1, 2, 3])
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5])
 def test_device_emulation_equals_glibc(which):
-    x = sample(2000000, 17 + which)
+    x = sample(2000000, 17 + which) if which < 4 else angles(2000000, 17 + which)
     want = emulate(x, which)            # host emulation, equal to glibc (test above)
     got = emulate(x, which, device=True)
     bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]

@@ -87,6 +87,20 @@ typedef struct sfx_params {
                               * host turns it on as sfincs_bmi.f90:60 does */
   float structure_relax;     /* 10 s: relaxation of drainage structure discharges
                               * (sfincs_input.f90:181, sfincs_discharges.f90:621-625) */
+  float qinf;                /* 0 mm/h: uniform constant infiltration (sfincs_input.f90:78,376),
+                              * on cells whose bed (zb, or the subgrid zmin) lies above qinf_zmin */
+  float qinf_zmin;           /* 0 m (sfincs_input.f90:100) */
+  /* gridded and spiderweb meteo (sfincs_input.f90:69,92-113,200-203; sfincs_meteo.f90) */
+  float dtwnd;               /* 1800 s: interval of the meteo fields' updates */
+  int32_t baro;              /* 1: atmospheric pressure from amp / spiderweb acts */
+  float pavbnd;              /* 0 Pa: > 1 corrects boundary levels by (pavbnd - patm) / (rhow g) */
+  float gapres;              /* 101200 Pa: background pressure */
+  int32_t spinup_meteo;      /* 0: 1 ramps gridded meteo from zero over the spin-up time */
+  int32_t amprblock;         /* 1: gridded rain held per interval (0: linear in time) */
+  int32_t usespwprecip;      /* 1: rain from a spiderweb's fourth quantity */
+  float spw_merge_frac;      /* 0.5 (spwmergefrac): the spiderweb fades into the background from
+                              * this fraction of its radius outwards */
+  float factor_wind, factor_pres, factor_prcp, factor_spw_size;   /* 1.0 each */
 } sfx_params;
 
 void sfx_params_default(sfx_params* p);
@@ -171,6 +185,43 @@ int  sfx_model_set_drainage(sfx_model* m, int32_t ndrn, const float* x_in, const
  * (find_uv_points_intersected_by_polyline, sfincs_quadtree.F90:853-1010) gets kcuv 0:
  * no flow through it. Table thindam_uv_index (1-based). */
 int  sfx_model_set_thin_dams(sfx_model* m, int32_t npoly, const int32_t* nv, const float* x, const float* y);
+/* The active cell (1-based; 0 outside the mesh) that holds each point (x[i], y[i]) -- how
+ * the reference places observation points (sfincs_obspoints.f90:95-130). */
+int  sfx_model_find_cells(sfx_model* m, int32_t n, const float* x, const float* y, int32_t* nm);
+/* Gridded meteo on an equidistant grid: the contents of an amu, amv, amp or ampr file
+ * (read_meteo_data, sfincs_meteo.f90:146-245). quantity: SFX_METEO_WIND_U / _WIND_V (m/s),
+ * _PRESSURE (Pa), _RAIN (mm/h). values[nt][nrows][ncols], row 0 the northernmost (first in
+ * the file); times in s from tref; x/y_llcorner the lower-left corner of the grid. Wind
+ * needs both components on the same grid. The fields at each update time (every dtwnd from
+ * t0) are interpolated to the cells at initialisation (update_amuv_data, update_amp_data,
+ * update_ampr_data, :802-1244), and each step interpolates them in time per cell
+ * (update_meteo_forcing, :1246-1419). Pressure acts only with baro = 1. */
+enum { SFX_METEO_WIND_U = 0, SFX_METEO_WIND_V = 1, SFX_METEO_PRESSURE = 2, SFX_METEO_RAIN = 3 };
+int  sfx_model_set_meteo_grid(sfx_model* m, int32_t quantity, int32_t nt, int32_t nrows, int32_t ncols,
+                              float x_llcorner, float y_llcorner, float dx, float dy, const float* times,
+                              const float* values);
+/* A spiderweb (spwfile; read_spw_file, sfincs_spiderweb.f90:8-133): nt times (s from
+ * tref), eye positions xe/ye[nt], and [nt][nrows][ncols] polar fields -- rows outwards from
+ * the eye over `radius`, columns clockwise from north -- of wind speed (m/s), wind
+ * direction (nautical degrees, where it comes from), pressure drop (Pa) and rain (mm/h, or
+ * NULL). Merged over any gridded background (update_spiderweb_data, sfincs_meteo.f90:450-800). */
+int  sfx_model_set_spiderweb(sfx_model* m, int32_t nt, int32_t nrows, int32_t ncols, float radius,
+                             const float* times, const float* xe, const float* ye, const float* vmag,
+                             const float* vdir, const float* pdrop, const float* rain);
+/* Reads an ASCII meteo file as the reference does -- kind "amu", "amv", "amp", "ampr"
+ * (read_amuv_dimensions / read_amuv_file, sfincs_spiderweb.f90:135-229, 320-408) or "spw"
+ * (read_spw_dimensions / read_spw_file, :8-133, 231-318) -- with its TIME lines taken
+ * relative to tref ("yyyymmdd HHMMSS"; compute_time_in_seconds, :411-484), and passes it
+ * to sfx_model_set_meteo_grid / sfx_model_set_spiderweb. */
+int  sfx_model_read_meteo_file(sfx_model* m, const char* kind, const char* path, const char* tref);
+/* Spatially varying constant infiltration (qinffile; initialize_infiltration 'c2d',
+ * sfincs_infiltration.f90:224-258): qinf_mmh[c] in mm/h per active cell, in sfincs order.
+ * With sfx_params.qinf > 0 the uniform rate ('con', :195-222) takes precedence, as in the
+ * reference (:92-104). Each step a wet cell (z_volume > 0 on subgrid meshes, zs > zb on
+ * regular ones) loses its rate from the net precipitation
+ * (update_infiltration_map, :632-676); it acts only where rain falls (continuity adds
+ * netprcp only with precipitation). Table qinffield (m/s). */
+int  sfx_model_set_infiltration(sfx_model* m, const float* qinf_mmh);
 /* Weirs (weirfile; read_structure_file with type 1, sfincs_structures.f90:38-381): as thin
  * dams, with par[2*k] = crest level and par[2*k + 1] = discharge coefficient Cd at each
  * vertex k. A crossed uv-point whose interpolated crest lies above its bed (uv_zmin on
@@ -246,7 +297,9 @@ int  sfx_destroy(sfx_handle* h);
  * With timestep_analysis on (read only, per uv-point, npuv entries): tsa_sum (f32, the sum
  * of the required time steps of the sweeps in which the point was wet), tsa_wet and
  * tsa_lim (i32: those sweeps, and those in which it limited the step), the accumulators of
- * sfincs_timestep_analysis.f90:26-74; their average is tsa_sum / max(tsa_wet, 1). */
+ * sfincs_timestep_analysis.f90:26-74; their average is tsa_sum / max(tsa_wet, 1). Read only
+ * as well: qtsrc (f32, nsrc + 2 ndrn), the step's point-source discharges followed by each
+ * drainage structure's intake (-Q) and outfall (+Q) discharge. */
 int  sfx_get_state(sfx_handle* h, const char* name, void* dst, size_t nbytes);
 /* Also "zb" (f32 per cell, regular meshes without subgrid tables): the bed level, as a BMI
  * coupler writes it through get_value_ptr; zbuvmx follows only with sfx_update_zbuvmx. */
@@ -301,8 +354,9 @@ int  sfx_total_volume_noalloc(sfx_handle* h, double* vol_device);
  * double, the volume it moved into the interior cells through faces whose other side is
  * not interior (water-level, river and Neumann boundaries, inactive cells), from point
  * sources as applied (sfincs_continuity.f90:87-106 with its bed clamp, :311-325), rain
- * (:116-121, 473-497) and qext. out = {interior volume now (sum of V, or zs*A),
- * storage volume left, boundary inflow, source inflow, rain, qext} in m3; the last four
+ * (:116-121, 473-497) less infiltration (sfincs_infiltration.f90:632-663), and qext.
+ * out = {interior volume now (sum of V, or zs*A), storage volume left, boundary inflow,
+ * source inflow, rain less infiltration, qext} in m3; the last four
  * are sums since creation or sfx_reset_balance. Between two calls the balance is
  * dV + d(storage used) = d(inflows). Host writes of cell state are not counted. No
  * counterpart in the reference, which keeps no volume balance. */

@@ -69,6 +69,12 @@ def olib():
         _olib.oracle_compute_fluxes.restype = C.c_float
         _olib.oracle_compute_water_levels.argtypes = [C.c_void_p, C.c_double, C.c_float]
         _olib.oracle_compute_fluxes_over_structures.argtypes = [C.c_void_p]
+        _olib.oracle_update_infiltration.argtypes = [C.c_void_p]
+        F, I32, P = C.c_float, C.c_int32, C.c_void_p
+        _olib.oracle_set_grid.argtypes = [P, I32, I32, I32, I32, F, F, F, F, P, P]
+        _olib.oracle_set_spw.argtypes = [P, I32, I32, I32, F, P, P, P, P, P, P, P]
+        for f in ("oracle_meteo_factors", "oracle_meteo_fields", "oracle_meteo_window"):
+            getattr(_olib, f).argtypes = [P]
         for f in ("oracle_update_meteo", "oracle_update_boundaries", "oracle_update_discharges"):
             getattr(_olib, f).argtypes = [C.c_void_p, C.c_double, C.c_float]
         _olib.oracle_update.argtypes = [C.c_void_p, C.c_double, C.c_int32]
@@ -118,6 +124,11 @@ class OracleRun:
         self._bind("tauwu", np.zeros(s["np"], np.float32))
         self._bind("tauwv", np.zeros(s["np"], np.float32))
         self._bind("netprcp", np.zeros(s["np"], np.float32))
+        # gridded meteo: cell centres, the pressure and the window's two fields per cell
+        self._bind("z_xz", model.table("z_xz"))
+        self._bind("z_yz", model.table("z_yz"))
+        for n in ("patm", "tauwu0", "tauwu1", "tauwv0", "tauwv1", "patm0", "patm1", "prcp0", "prcp1"):
+            self._bind(n, np.zeros(max(1, s["np"]), np.float32))
         self._bind("zsb", model.table("zsb").copy())
         self._bind("zsb0", model.table("zsb0").copy())
         self._bind("uvmean", model.table("uvmean").copy())
@@ -137,6 +148,8 @@ class OracleRun:
                       ("structure_cd", np.float32), ("structure_length", np.float32)):
             a = model.table(n)
             self._bind(n, a if a.size else np.zeros(1, dt))
+        qi = model.table("qinffield")
+        self._bind("qinffield", qi.copy() if qi.size else np.zeros(1, np.float32))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
         self._bind("x73", x73_table())
@@ -172,6 +185,7 @@ class OracleRun:
                     store_velmax=p.store_velmax, store_fluxmax=p.store_fluxmax, store_twet=p.store_twet,
                     twet_threshold=p.twet_threshold, timestep_analysis=p.timestep_analysis, use_qext=p.use_qext,
                     use_storage_volume=int(bool(s["subgrid"]) and len(tb("storage_volume")) > 0),
+                    infiltration=int(len(tb("qinffield")) > 0),
                     wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
@@ -184,9 +198,40 @@ class OracleRun:
                     btfilter=p.btfilter, btrelax=p.btrelax, t0=p.t0, t1=p.t1,
                     tspinup=np.float32(p.t0) + np.float32(p.tspinup), zini=p.zsini,
                     dx=tb("dxrm")[0], dy=tb("dyrm")[0], t=float(p.t0))
+        # gridded and spiderweb meteo (read_meteo_data, sfincs_domain.f90:99-123): the inputs the
+        # model was given, the switches, the factors
+        mi = getattr(model, "meteo_inputs", {})
+        spw = mi.get("spw")
+        has_wind = "amu" in mi
+        spw_rain = spw is not None and spw["rain"] is not None and bool(p.usespwprecip)
+        met = dict(meteo3d=int(bool(mi)), meteo_wind=int(has_wind or spw is not None),
+                   patmos=int(("amp" in mi or spw is not None) and p.baro == 1),
+                   meteo_rain=int("ampr" in mi or spw_rain), spinup_meteo=p.spinup_meteo, amprblock=p.amprblock,
+                   usespwprecip=p.usespwprecip, dtwnd=p.dtwnd, gapres=p.gapres, pavbnd=p.pavbnd,
+                   spw_merge_frac=p.spw_merge_frac, factor_wind=p.factor_wind, factor_pres=p.factor_pres,
+                   factor_prcp=p.factor_prcp, factor_spw_size=p.factor_spw_size, twindupd=np.float32(p.t0),
+                   cosrot=float(tb("cosrot")[0]), sinrot=float(tb("sinrot")[0]))
+        if met["meteo_wind"]:
+            vals["wind"] = 1
+        if met["meteo_rain"]:
+            vals["precip"] = 1
+        vals.update(met)
         for k, v in vals.items():
             self.set(k, v)
         self.vals = vals
+        for kind, q in (("amu", 0), ("amv", 1), ("amp", 2), ("ampr", 3)):
+            if kind in mi:
+                g = mi[kind]
+                nt, nr, nc = g["values"].shape
+                self.L.oracle_set_grid(self.m, q, nt, nr, nc, g["x_llcorner"], g["y_llcorner"], g["dx"], g["dy"],
+                                       g["times"].ctypes.data, g["values"].ctypes.data)
+        if mi:
+            self.L.oracle_meteo_factors(self.m)
+        if spw is not None:
+            nt, nr, nc = spw["vmag"].shape
+            self.L.oracle_set_spw(self.m, nt, nr, nc, spw["radius"], spw["times"].ctypes.data, spw["xe"].ctypes.data,
+                                  spw["ye"].ctypes.data, spw["vmag"].ctypes.data, spw["vdir"].ctypes.data,
+                                  spw["pdrop"].ctypes.data, spw["rain"].ctypes.data if spw_rain else None)
         return self
 
     # forcing + hot path pieces
@@ -197,8 +242,11 @@ class OracleRun:
         self.L.oracle_compute_water_levels(self.m, t, dt)
 
     def update_forcing(self, t, dt):
-        if self.vals["wind"] or self.vals["precip"]:
+        self.L.oracle_meteo_window(self.m)
+        if self.vals["wind"] or self.vals["precip"] or self.vals["patmos"]:
             self.L.oracle_update_meteo(self.m, t, dt)
+            if self.vals["infiltration"]:
+                self.L.oracle_update_infiltration(self.m)
         self.L.oracle_update_boundaries(self.m, t, dt)
         self.L.oracle_update_discharges(self.m, t, dt)
 
@@ -239,6 +287,8 @@ def rlib():
         _rlib.ref_timers.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _rlib.ref_read_input.argtypes = [C.c_void_p, C.c_int32]
         _rlib.ref_tsa_finalize.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+        _rlib.ref_read_meteo.argtypes = [C.c_int32, C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int32]
     return _rlib
 
 
@@ -256,7 +306,7 @@ REF_F32_TABLES = ("dxrinv", "dyrinv", "dxr2inv", "dyr2inv", "dxrinvc", "dyrinvc"
                   "nuvisc")
 REF_SCALARS = ("subgrid", "use_quadtree", "advection", "advection_scheme", "viscosity", "coriolis", "friction2d",
                "thetasmoothing", "wiggle_suppression", "store_zsmax", "store_velmax", "store_fluxmax", "store_twet",
-               "twet_threshold", "wind", "precip", "h73table", "nsrc", "g",
+               "twet_threshold", "wind", "precip", "patmos", "h73table", "nsrc", "g",
                "huthresh", "huvmin", "advlim", "slopelim", "uvlim", "nuviscfac", "fcorio", "theta",
                "wiggle_factor", "wiggle_threshold", "dtmax", "alpha", "rhow", "dx", "dy", "timestep_analysis", "use_qext", "use_storage_volume")
 
@@ -302,7 +352,7 @@ class RefRun:
         R.ref_set_f64(b"zs", k["zs"].ctypes.data, k["zs"].size)
         if self.o.vals["subgrid"]:
             R.ref_set_f64(b"z_volume", k["z_volume"].ctypes.data, k["z_volume"].size)
-        for n in ("q", "uv", "zs0", "zsderv", "zsmax", "tauwu", "tauwv", "netprcp", "qtsrc", "vmax", "qmax",
+        for n in ("q", "uv", "zs0", "zsderv", "zsmax", "tauwu", "tauwv", "netprcp", "patm", "qtsrc", "vmax", "qmax",
                   "twet", "qext", "storage_volume"):
             R.ref_set_f32(n.encode(), k[n].ctypes.data, k[n].size)
         R.ref_set_i8(b"kfuv", k["kfuv"].ctypes.data, k["kfuv"].size)
@@ -362,17 +412,45 @@ class RefRun:
         return a.value, b.value
 
 
+def ref_read_meteo(path, kind, tref="20000101 000000"):
+    """The reference's own reader of an ASCII meteo file (read_amuv_dimensions/_file or
+    read_spw_dimensions/_file, sfincs_spiderweb.f90). Returns a dict: nt, nrows, ncols,
+    nquant, geo (x_llcorner, y_llcorner, dx, dy, spw_radius), times, and values [nt, nrows,
+    ncols] (spiderweb: vmag, vdir, pdrop, prcp and the eye xe, ye)."""
+    R = rlib()
+    spw = kind == "spw"
+    dims = np.zeros(4, np.int32)
+    geo = np.zeros(5, np.float32)
+    p = str(path).encode()
+    R.ref_read_meteo(1 if spw else 0, p, tref.encode(), dims.ctypes.data, geo.ctypes.data, None, None, 0)
+    nt, nr, nc, nq = (int(x) for x in dims)
+    n = nt * nr * nc
+    times = np.zeros(2 * nt, np.float32)
+    vals = np.zeros(4 * n + nt, np.float32)
+    R.ref_read_meteo(1 if spw else 0, p, tref.encode(), dims.ctypes.data, geo.ctypes.data, times.ctypes.data,
+                     vals.ctypes.data, vals.size)
+    out = dict(nt=nt, nrows=nr, ncols=nc, nquant=nq, geo=geo, times=times[:nt].copy())
+    shape = (nt, nr, nc)
+    if spw:
+        out.update(vmag=vals[:n].reshape(shape), vdir=vals[n:2 * n].reshape(shape),
+                   pdrop=vals[2 * n:3 * n].reshape(shape), prcp=vals[3 * n:4 * n].reshape(shape),
+                   xe=times[nt:2 * nt].copy(), ye=vals[4 * n:4 * n + nt].copy())
+    else:
+        out["values"] = vals[:n].reshape(shape)
+    return out
+
+
 def ref_read_input(workdir, text):
-    """Run the reference's read_sfincs_input on `text` in `workdir`; returns 41 values."""
+    """Run the reference's read_sfincs_input on `text` in `workdir`; returns 43 values."""
     R = rlib()
     cwd = os.getcwd()
     os.makedirs(workdir, exist_ok=True)
     with open(os.path.join(workdir, "sfincs.inp"), "w") as f:
         f.write(text)
-    out = np.zeros(41, np.float64)
+    out = np.zeros(43, np.float64)
     try:
         os.chdir(workdir)
-        R.ref_read_input(out.ctypes.data, 41)
+        R.ref_read_input(out.ctypes.data, 43)
     finally:
         os.chdir(cwd)
     return out

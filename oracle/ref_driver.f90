@@ -16,6 +16,7 @@ module ref_driver
    use sfincs_timestep_analysis
    use sfincs_input
    use sfincs_log
+   use sfincs_spiderweb
    implicit none
    real :: tloopflux = 0.0
    real :: tloopcont = 0.0
@@ -74,6 +75,7 @@ contains
          endif
       case ('wind');               wind = (v /= 0.0d0)
       case ('precip');             precip = (v /= 0.0d0)
+      case ('patmos');             patmos = (v /= 0.0d0)
       case ('h73table');           h73table = (v /= 0.0d0)
       case ('nsrc');               nsrcdrn = int(v)
       case ('g');                  g = real(v, 4)
@@ -195,6 +197,7 @@ contains
       case ('tauwu');        call putr(tauwu)
       case ('tauwv');        call putr(tauwv)
       case ('netprcp');      call putr(netprcp)
+      case ('patm');         call putr(patm)
       case ('qtsrc');        call putr(qtsrc)
       case ('z_zmin');       call putr(subgrid_z_zmin)
       case ('z_zmax');       call putr(subgrid_z_zmax)
@@ -327,6 +330,72 @@ contains
       call fill_h73_tables()
    end subroutine
 
+   ! The reference's own ASCII meteo readers (sfincs_spiderweb.f90): dims(1:4) = nt, nrows,
+   ! ncols, nquant and geo(1:5) = x_llcorner, y_llcorner, dx, dy, spw_radius; when cap holds
+   ! them, times(nt) and the fields, [it][row][col] with the row index slowest after time
+   ! (spiderweb: vmag, vdir, pdrp, prcp one after the other).
+   subroutine ref_read_meteo(kind, path, tref, dims, geo, times, vals, cap) bind(C, name='ref_read_meteo')
+      integer(c_int32_t), value :: kind, cap
+      character(kind=c_char), intent(in) :: path(256), tref(15)
+      integer(c_int32_t), intent(out) :: dims(4)
+      real(c_float), intent(out) :: geo(5)
+      real(c_float), intent(out) :: times(*), vals(*)
+      character(len=256) :: fname
+      character(len=15) :: trefstr
+      integer :: nt, nrows, ncols, nquant, i, it, n, m, k
+      real*4 :: xll, yll, dx, dy, rad
+      real*4, allocatable :: a(:,:,:), b(:,:,:), c(:,:,:), d(:,:,:), tt(:), xe(:), ye(:)
+      fname = ' '
+      do i = 1, 256
+         if (path(i) == c_null_char) exit
+         fname(i:i) = path(i)
+      enddo
+      do i = 1, 15
+         trefstr(i:i) = tref(i)
+      enddo
+      xll = 0.0; yll = 0.0; dx = 0.0; dy = 0.0; rad = 0.0
+      if (kind == 0) then
+         call read_amuv_dimensions(fname, nt, nrows, ncols, xll, yll, dx, dy, nquant)
+      else
+         call read_spw_dimensions(fname, nt, nrows, ncols, rad, nquant)
+      endif
+      dims = [nt, nrows, ncols, nquant]
+      geo = [xll, yll, dx, dy, rad]
+      if (cap < nt * nrows * ncols * merge(1, 4, kind == 0) + merge(0, nt, kind == 0)) return
+      allocate(a(nt, nrows, ncols), tt(nt))
+      if (kind == 0) then
+         call read_amuv_file(fname, nt, nrows, ncols, tt, a, trefstr)
+      else
+         allocate(b(nt, nrows, ncols), c(nt, nrows, ncols), d(nt, nrows, ncols), xe(nt), ye(nt))
+         call read_spw_file(fname, nt, nrows, ncols, rad, tt, xe, ye, a, b, c, d, nquant, trefstr)
+      endif
+      times(1:nt) = tt
+      k = 0
+      do it = 1, nt
+         do n = 1, nrows
+            do m = 1, ncols
+               k = k + 1
+               vals(k) = a(it, n, m)
+            enddo
+         enddo
+      enddo
+      if (kind /= 0) then
+         do it = 1, nt
+            do n = 1, nrows
+               do m = 1, ncols
+                  vals(k + 1) = b(it, n, m)
+                  vals(k + 1 + nt * nrows * ncols) = c(it, n, m)
+                  vals(k + 1 + 2 * nt * nrows * ncols) = d(it, n, m)
+                  k = k + 1
+               enddo
+            enddo
+         enddo
+         times(nt + 1:2 * nt) = xe
+         ! the eye's y after the four fields (times holds the eye's x after the times)
+         vals(4 * nt * nrows * ncols + 1:4 * nt * nrows * ncols + nt) = ye
+      endif
+   end subroutine
+
    ! read_sfincs_input (sfincs_input.f90:5) from ./sfincs.inp; returns the values the hot
    ! path depends on, in the order of ref_input_names.
    subroutine ref_read_input(out, nout) bind(C, name='ref_read_input')
@@ -349,6 +418,7 @@ contains
       out(35) = bndtype; out(36) = merge(1.0d0, 0.0d0, h73table)
       out(37) = t0; out(38) = t1; out(39) = tspinup; out(40) = fcorio
       out(41) = merge(1.0d0, 0.0d0, thetasmoothing)
+      out(42) = qinf; out(43) = qinf_zmin
    end subroutine
 
 end module ref_driver

@@ -11,6 +11,7 @@
  *                                runs right after it
  *   oracle_compute_water_levels  sfincs_continuity.f90:5-720 (with compute_store_variables)
  *   oracle_update_meteo          sfincs_meteo.f90:1246-1558 (time-series branch)
+ *   oracle_update_infiltration   sfincs_infiltration.f90:632-676 ('con' and 'c2d')
  *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
  *   oracle_update_discharges     sfincs_discharges.f90:325-655 (point sources and drainage
  *                                structures)
@@ -43,6 +44,10 @@
  *   oracle_compute_fluxes_over_structures
  *                                the broad-crested weir formula in double, free, submerged
  *                                and below the crest, both directions (test_structures.py)
+ *   oracle_update_infiltration   a flat basin's level falls by the rate less the rain to
+ *                                the float step (regular and subgrid); dry cells keep their
+ *                                rain; the removed volume against the rate times the wet
+ *                                area (test_infiltration.py)
  *   oracle_update                step counts and model time against the GPU and the
  *                                reference loop (test_oracle_vs_ref.py); the instability stop
  *                                by closed form (test_forcing_pins.py)
@@ -52,6 +57,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* A gridded meteo input: [nt][nrows][ncols], row 0 the northernmost (file order). */
+typedef struct MG {
+  int32_t nt, nrows, ncols;
+  float xll, yll, dx, dy;
+  float *times, *v;
+} MG;
+
 typedef struct OM {
   /* sizes and switches */
   int32_t np, npuv, ncuv, nref, nl;
@@ -59,7 +71,7 @@ typedef struct OM {
   int32_t thetasmoothing, wiggle_suppression, store_zsmax, wind, precip, h73table;
   int32_t store_velmax, store_fluxmax, store_twet;
   float twet_threshold;
-  int32_t timestep_analysis, use_qext, use_storage_volume;
+  int32_t timestep_analysis, use_qext, use_storage_volume, infiltration;
   int32_t ngbnd, nkcuv2, nbnd, ntbnd, nsrc, ntsrc, ntprcp, ntwnd;
   float g, pi, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
@@ -84,6 +96,7 @@ typedef struct OM {
   float *tsa_req, *tsa_sum;    /* timestep analysis (npuv each, when switched on) */
   float *qext;                 /* external sources, m/s per cell (use_qext) */
   float *storage_volume;       /* storage left per subgrid cell, m3 (use_storage_volume) */
+  const float *qinffield;      /* infiltration rate per cell, m/s ('con' or 'c2d') */
   int32_t *tsa_wet, *tsa_lim;
   int8_t *kfuv;
   float *tauwu, *tauwv, *netprcp;   /* np each */
@@ -113,6 +126,19 @@ typedef struct OM {
   int32_t nstruc;
   const int32_t *structure_uv_index;
   const float *structure_height, *structure_cd, *structure_length;
+  /* gridded and spiderweb meteo (sfincs_meteo.f90): switches, parameters, the window,
+   * the two fields of the window per cell, the inputs after read_meteo_data's conversions */
+  int32_t meteo3d, patmos, meteo_wind, meteo_rain, spinup_meteo, amprblock, usespwprecip;
+  float dtwnd, gapres, pavbnd, spw_merge_frac, cosrot, sinrot;
+  float factor_wind, factor_pres, factor_prcp, factor_spw_size;
+  float twindupd, meteo_t0, meteo_t1;
+  const float *z_xz, *z_yz;
+  float *patm;
+  float *tauwu0, *tauwu1, *tauwv0, *tauwv1, *patm0, *patm1, *prcp0, *prcp1;
+  MG amu, amv, amp, ampr;
+  int32_t spw_nt, spw_nrows, spw_ncols, spw_rain;
+  float spw_radius, dradspw, dphispw;
+  float *spw_times, *spw_xe, *spw_ye, *spw_wu, *spw_wv, *spw_pabs, *spw_prcp;
   /* time state */
   double t;
   float dt, min_dt;
@@ -126,7 +152,13 @@ OM* oracle_new(void) {
   m->itbndlast = 1; m->itsrclast = 1; m->itwndlast = 1; m->itprcplast = 1;
   return m;
 }
-void oracle_free(OM* m) { free(m); }
+static void mg_free(MG* g) { free(g->times); free(g->v); g->times = g->v = NULL; g->nt = 0; }
+void oracle_free(OM* m) {
+  mg_free(&m->amu); mg_free(&m->amv); mg_free(&m->amp); mg_free(&m->ampr);
+  free(m->spw_times); free(m->spw_xe); free(m->spw_ye); free(m->spw_wu); free(m->spw_wv); free(m->spw_pabs);
+  free(m->spw_prcp);
+  free(m);
+}
 
 #define BIND(field) if (!strcmp(name, #field)) { *(void**)&m->field = ptr; return 0; }
 int oracle_bind(OM* m, const char* name, void* ptr) {
@@ -140,8 +172,9 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(z_zmin) BIND(z_zmax) BIND(z_volmax) BIND(z_dep) BIND(uv_zmin) BIND(uv_zmax) BIND(uv_navg_w) BIND(uv_fnfit)
   BIND(uv_havg) BIND(uv_nrep) BIND(uv_pwet) BIND(uv_havg_zmax) BIND(uv_nrep_zmax)
   BIND(zs) BIND(z_volume) BIND(q) BIND(uv) BIND(q0) BIND(uv0) BIND(zs0) BIND(zsderv) BIND(zsmax) BIND(kfuv)
-  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim) BIND(qext) BIND(storage_volume)
-  BIND(tauwu) BIND(tauwv) BIND(netprcp)
+  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim) BIND(qext) BIND(storage_volume) BIND(qinffield)
+  BIND(tauwu) BIND(tauwv) BIND(netprcp) BIND(z_xz) BIND(z_yz) BIND(patm)
+  BIND(tauwu0) BIND(tauwu1) BIND(tauwv0) BIND(tauwv1) BIND(patm0) BIND(patm1) BIND(prcp0) BIND(prcp1)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
   BIND(nmi_gbp) BIND(index_bdr_gbp) BIND(index_zsi_bdr) BIND(dzs_bdr) BIND(ibuvdir) BIND(fac_bnd_gbp) BIND(t_bnd) BIND(zs_bnd) BIND(zsi_bnd) BIND(zsit_bnd) BIND(zsb) BIND(zsb0) BIND(uvmean) BIND(zst_bnd)
   BIND(nmindsrc) BIND(tsrc) BIND(qsrc) BIND(tprcpt) BIND(tprcpv) BIND(twnd) BIND(wndmag) BIND(wnddir) BIND(cdval) BIND(qtsrc) BIND(x73)
@@ -158,7 +191,10 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
   SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error) SETI(ndrn) SETF(structure_relax) SETI(nstruc)
-  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume)
+  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume) SETI(infiltration)
+  SETI(meteo3d) SETI(patmos) SETI(meteo_wind) SETI(meteo_rain) SETI(spinup_meteo) SETI(amprblock) SETI(usespwprecip)
+  SETF(dtwnd) SETF(gapres) SETF(pavbnd) SETF(spw_merge_frac) SETF(cosrot) SETF(sinrot) SETF(factor_wind)
+  SETF(factor_pres) SETF(factor_prcp) SETF(factor_spw_size) SETF(twindupd) SETF(meteo_t0) SETF(meteo_t1)
   SETF(g) SETF(pi) SETF(huthresh) SETF(huvmin) SETF(advlim) SETF(slopelim) SETF(uvlim) SETF(nuviscfac) SETF(fcorio)
   SETF(theta) SETF(wiggle_factor) SETF(wiggle_threshold) SETF(dtmax) SETF(dtmin) SETF(alpha) SETF(rhoa) SETF(rhow)
   SETF(btfilter) SETF(btrelax) SETF(t0) SETF(t1) SETF(tspinup) SETF(zini) SETF(dx) SETF(dy) SETF(dt) SETF(min_dt)
@@ -174,6 +210,9 @@ double oracle_get(const OM* m, const char* name) {
   if (!strcmp(name, "min_dt")) return m->min_dt;
   if (!strcmp(name, "nt")) return m->nt;
   if (!strcmp(name, "error")) return m->error;
+  if (!strcmp(name, "twindupd")) return m->twindupd;
+  if (!strcmp(name, "meteo_t0")) return m->meteo_t0;
+  if (!strcmp(name, "meteo_t1")) return m->meteo_t1;
   return NAN;
 }
 
@@ -331,6 +370,9 @@ float oracle_compute_fluxes(OM* m, float dt) {
     if (m->wind) {
       if (hwet > 0.25f) frc = frc + phi * (idir == 0 ? m->tauwu[nm] : m->tauwv[nm]);
       else frc = frc + (idir == 0 ? m->tauwu[nm] : m->tauwv[nm]) * hu * 4;
+    }
+    if (m->patmos) {   /* :585-589 */
+      frc = frc + hu * (m->patm[nm] - m->patm[nmu]) * dxuvinv / m->rhow;
     }
     float qfr;
     if (m->kfuv[i] == 0) {
@@ -510,9 +552,11 @@ static float f_modulo(float a, float p) {
   return r;
 }
 
+static void meteo3d_forcing(OM* m, double t);
 void oracle_update_meteo(OM* m, double t, float dt) {
   (void)dt;
-  if (m->wind) {
+  if (m->meteo3d) meteo3d_forcing(m, t);
+  if (m->wind && m->ntwnd > 0) {
     for (int32_t itw = m->itwndlast; itw <= m->ntwnd; ++itw) {
       if ((double)m->twnd[itw - 1] > t) {
         const float twfac = (float)((t - (double)m->twnd[itw - 2]) / (double)(m->twnd[itw - 1] - m->twnd[itw - 2]));
@@ -531,7 +575,7 @@ void oracle_update_meteo(OM* m, double t, float dt) {
       }
     }
   }
-  if (m->precip) {
+  if (m->precip && m->ntprcp > 0) {
     int32_t itp;
     for (itp = m->itprcplast; itp <= m->ntprcp; ++itp)
       if ((double)m->tprcpt[itp - 1] > t) break;
@@ -541,6 +585,302 @@ void oracle_update_meteo(OM* m, double t, float dt) {
     const float ptmp = (m->tprcpv[itp - 2] * (1.0f - twfac) + m->tprcpv[itp - 1] * twfac) / (float)(1000 * 3600);
     for (int32_t nm = 0; nm < m->np; ++nm) m->netprcp[nm] = ptmp;
     m->itprcplast = itp - 1;
+  }
+}
+
+/* ---- gridded and spiderweb meteo: sfincs_meteo.f90 ----------------------------------- */
+static float* fdup(const float* a, size_t n) {
+  float* b = (float*)malloc((n ? n : 1) * sizeof(float));
+  if (n) memcpy(b, a, n * sizeof(float));
+  return b;
+}
+
+/* An amu (q 0), amv (1), amp (2) or ampr (3) grid as read (read_amuv_file). */
+int oracle_set_grid(OM* m, int32_t q, int32_t nt, int32_t nrows, int32_t ncols, float xll, float yll, float dx,
+                    float dy, const float* times, const float* v) {
+  MG* g = q == 0 ? &m->amu : q == 1 ? &m->amv : q == 2 ? &m->amp : q == 3 ? &m->ampr : NULL;
+  if (!g) return -1;
+  mg_free(g);
+  g->nt = nt; g->nrows = nrows; g->ncols = ncols; g->xll = xll; g->yll = yll; g->dx = dx; g->dy = dy;
+  g->times = fdup(times, (size_t)nt);
+  g->v = fdup(v, (size_t)nt * nrows * ncols);
+  return 0;
+}
+
+/* A spiderweb as read (read_spw_file), converted as read_meteo_data does (:57-95, 330-414):
+ * u, v from speed and nautical direction, absolute pressure, the factors. Call after the
+ * scalars are set. */
+int oracle_set_spw(OM* m, int32_t nt, int32_t nrows, int32_t ncols, float radius, const float* times,
+                   const float* xe, const float* ye, const float* vmag, const float* vdir, const float* pdrop,
+                   const float* prcp) {
+  const size_t n = (size_t)nt * nrows * ncols;
+  m->spw_nt = nt; m->spw_nrows = nrows; m->spw_ncols = ncols;
+  m->spw_times = fdup(times, (size_t)nt); m->spw_xe = fdup(xe, (size_t)nt); m->spw_ye = fdup(ye, (size_t)nt);
+  m->spw_wu = (float*)malloc(n * sizeof(float));
+  m->spw_wv = (float*)malloc(n * sizeof(float));
+  m->spw_pabs = (float*)malloc(n * sizeof(float));
+  m->spw_rain = prcp != NULL && m->usespwprecip;
+  m->spw_prcp = m->spw_rain ? fdup(prcp, n) : NULL;
+  for (size_t k = 0; k < n; ++k) {
+    const float ang = m->pi * (270.0f - vdir[k]) / 180;
+    m->spw_wu[k] = vmag[k] * cosf(ang);
+    m->spw_wv[k] = vmag[k] * sinf(ang);
+    m->spw_pabs[k] = m->gapres - pdrop[k];
+  }
+  if (m->factor_wind != 1.0f)
+    for (size_t k = 0; k < n; ++k) { m->spw_wu[k] = m->factor_wind * m->spw_wu[k]; m->spw_wv[k] = m->factor_wind * m->spw_wv[k]; }
+  if (m->factor_pres != 1.0f)
+    for (size_t k = 0; k < n; ++k) m->spw_pabs[k] = m->gapres - m->factor_pres * (m->gapres - m->spw_pabs[k]);
+  if (m->factor_prcp != 1.0f && m->spw_rain)
+    for (size_t k = 0; k < n; ++k) m->spw_prcp[k] = m->factor_prcp * m->spw_prcp[k];
+  m->spw_radius = radius;
+  if (m->factor_spw_size != 1.0f) m->spw_radius = m->factor_spw_size * m->spw_radius;
+  m->dradspw = m->spw_radius / nrows;
+  m->dphispw = 2 * m->pi / ncols;
+  return 0;
+}
+
+/* The factors on the grids (:347-381), after oracle_set_grid and the scalars. */
+void oracle_meteo_factors(OM* m) {
+  if (m->factor_wind != 1.0f) {
+    for (size_t k = 0; k < (size_t)m->amu.nt * m->amu.nrows * m->amu.ncols; ++k) m->amu.v[k] = m->factor_wind * m->amu.v[k];
+    for (size_t k = 0; k < (size_t)m->amv.nt * m->amv.nrows * m->amv.ncols; ++k) m->amv.v[k] = m->factor_wind * m->amv.v[k];
+  }
+  if (m->factor_pres != 1.0f)
+    for (size_t k = 0; k < (size_t)m->amp.nt * m->amp.nrows * m->amp.ncols; ++k)
+      m->amp.v[k] = m->gapres - m->factor_pres * (m->gapres - m->amp.v[k]);
+  if (m->factor_prcp != 1.0f)
+    for (size_t k = 0; k < (size_t)m->ampr.nt * m->ampr.nrows * m->ampr.ncols; ++k) m->ampr.v[k] = m->factor_prcp * m->ampr.v[k];
+}
+
+/* Row iy (1-based, from the north), column ix of a grid: value of frame blend `b`. */
+#define GV(b, g, iy, ix) ((b)[((iy) - 1) * (g)->ncols + ((ix) - 1)])
+
+/* One grid quantity at meteo time mt on every cell (update_amuv_data :821-975,
+ * update_amp_data :996-1106, update_ampr_data :1126-1242): the two file frames around mt
+ * blended (block: the earlier one), then bilinear weights on the cell centre; a cell
+ * outside (the first row and column index 0 included) gets `outside`. kind 0 wind (both
+ * components, to stresses), 1 pressure, 2 rain (to m/s). */
+static void grid_to_cells(OM* m, const MG* g, const MG* g2, int kind, int block, float mt_in, float* o1, float* o2,
+                          float outside) {
+  float mt = mt_in;
+  mt = mt > g->times[0] ? mt : g->times[0];
+  mt = mt < g->times[g->nt - 1] ? mt : g->times[g->nt - 1];
+  int itw0 = 1;
+  for (int k = 1; k <= g->nt; ++k)
+    if (g->times[k - 1] <= mt) itw0 = k;
+  int itw1 = itw0 + 1 < g->nt ? itw0 + 1 : g->nt;
+  mt = mt < g->times[itw1 - 1] ? mt : g->times[itw1 - 1];
+  float den = g->times[itw1 - 1] - g->times[itw0 - 1];
+  if (den < 1.0e-6f) den = 1.0e-6f;
+  float twfac = (mt - g->times[itw0 - 1]) / den;
+  if (block) twfac = 0.0f;
+  const size_t nf = (size_t)g->nrows * g->ncols;
+  float* b1 = (float*)malloc(nf * sizeof(float));
+  float* b2 = g2 ? (float*)malloc(nf * sizeof(float)) : NULL;
+  for (size_t k = 0; k < nf; ++k) {
+    b1[k] = g->v[(size_t)(itw0 - 1) * nf + k] * (1.0f - twfac) + g->v[(size_t)(itw1 - 1) * nf + k] * twfac;
+    if (g2) b2[k] = g2->v[(size_t)(itw0 - 1) * nf + k] * (1.0f - twfac) + g2->v[(size_t)(itw1 - 1) * nf + k] * twfac;
+  }
+  const float yul = g->yll + (g->nrows - 1) * g->dy + 0.5f * g->dy;
+  const float xll = g->xll + 0.5f * g->dx;
+  for (int32_t nm = 0; nm < m->np; ++nm) {
+    const float x = m->z_xz[nm], y = m->z_yz[nm];
+    const int iy = (int)((yul - y) / g->dy) + 1;
+    const float dj1 = (yul - (iy - 1) * g->dy - y) / g->dy;
+    const int ix = (int)((x - xll) / g->dx) + 1;
+    const float di1 = (x - (xll + (ix - 1) * g->dx)) / g->dx;
+    if (iy <= 1 || iy >= g->nrows || ix <= 0 || ix >= g->ncols) {
+      o1[nm] = outside;
+      if (o2) o2[nm] = outside;
+      continue;
+    }
+    const int r[4] = {iy, iy, iy + 1, iy + 1}, c[4] = {ix, ix + 1, ix + 1, ix};
+    const float f[4] = {(1.0f - di1) * (1.0f - dj1), di1 * (1.0f - dj1), di1 * dj1, (1.0f - di1) * dj1};
+    float a = 0.0f, bsum = 0.0f;
+    for (int ip = 0; ip < 4; ++ip) {
+      a = a + f[ip] * GV(b1, g, r[ip], c[ip]);
+      if (b2) bsum = bsum + f[ip] * GV(b2, g, r[ip], c[ip]);
+    }
+    if (kind == 0) {
+      const float vmag = sqrtf(a * a + bsum * bsum);
+      const float cd = m->cdval[(int)(vmag * 10) + 1 - 1];
+      o1[nm] = vmag * (m->cosrot * a + m->sinrot * bsum) * m->rhoa * cd / m->rhow;
+      o2[nm] = vmag * (-m->sinrot * a + m->cosrot * bsum) * m->rhoa * cd / m->rhow;
+    } else if (kind == 1) {
+      o1[nm] = a;
+    } else {
+      o1[nm] = a / (1000 * 3600);
+    }
+  }
+  free(b1);
+  free(b2);
+}
+
+/* update_spiderweb_data (:450-800) at meteo time mt onto the fields tu, tv, pa, pr. */
+static void spw_to_cells(OM* m, float mt, float* tu, float* tv, float* pa, float* pr) {
+  int itw0, itw1;
+  float twfac, tspinup_fac;
+  const int nt = m->spw_nt;
+  if (mt < m->spw_times[0]) {
+    itw0 = itw1 = 1; twfac = 1.0f;
+    tspinup_fac = 1.0f - (m->spw_times[0] - mt) / 21600.0f;
+    if (tspinup_fac < 0.0f) tspinup_fac = 0.0f;
+  } else if (mt > m->spw_times[nt - 1]) {
+    itw0 = itw1 = nt; twfac = 1.0f;
+    tspinup_fac = 1.0f - (mt - m->spw_times[nt - 1]) / 21600.0f;
+    if (tspinup_fac < 0.0f) tspinup_fac = 0.0f;
+  } else {
+    itw0 = 1;
+    for (int k = 1; k <= nt; ++k)
+      if (m->spw_times[k - 1] <= mt) itw0 = k;
+    itw1 = itw0 + 1 < nt ? itw0 + 1 : nt;
+    float den = m->spw_times[itw1 - 1] - m->spw_times[itw0 - 1];
+    if (den < 1.0e-6f) den = 1.0e-6f;
+    twfac = (mt - m->spw_times[itw0 - 1]) / den;
+    tspinup_fac = 1.0f;
+  }
+  const float xe01 = m->spw_xe[itw0 - 1] * (1.0f - twfac) + m->spw_xe[itw1 - 1] * twfac;
+  const float ye01 = m->spw_ye[itw0 - 1] * (1.0f - twfac) + m->spw_ye[itw1 - 1] * twfac;
+  const int nrows = m->spw_nrows, ncols = m->spw_ncols;
+  const size_t nf = (size_t)nrows * ncols;
+  float* wu = (float*)malloc(nf * sizeof(float));
+  float* wv = (float*)malloc(nf * sizeof(float));
+  float* pb = (float*)malloc(nf * sizeof(float));
+  float* rn = (float*)malloc(nf * sizeof(float));
+  for (size_t k = 0; k < nf; ++k) {
+    const size_t a = (size_t)(itw0 - 1) * nf + k, b = (size_t)(itw1 - 1) * nf + k;
+    wu[k] = m->spw_wu[a] * (1.0f - twfac) + m->spw_wu[b] * twfac;
+    wv[k] = m->spw_wv[a] * (1.0f - twfac) + m->spw_wv[b] * twfac;
+    pb[k] = m->spw_pabs[a] * (1.0f - twfac) + m->spw_pabs[b] * twfac;
+    rn[k] = m->spw_rain ? m->spw_prcp[a] * (1.0f - twfac) + m->spw_prcp[b] * twfac : 0.0f;
+  }
+  for (int32_t nm = 0; nm < m->np; ++nm) {
+    const float dxe = m->z_xz[nm] - xe01, dye = m->z_yz[nm] - ye01;
+    const float dst = sqrtf(dxe * dxe + dye * dye);
+    if (dst > m->spw_radius) continue;
+    int id = (int)(dst / m->dradspw);
+    if (id < 1) id = 1;
+    if (id + 1 > nrows) continue;
+    float dj1 = (dst - m->dradspw * id) / m->dradspw;
+    if (dj1 < 0.0f) dj1 = 0.0f;
+    float phi = 0.5f * m->pi - atan2f(dye, dxe);
+    phi = f_modulo(phi, 2 * m->pi);
+    int ip = (int)(phi / m->dphispw) + 1;
+    if (ip > ncols) ip = ncols;
+    const int r[4] = {id, id, id + 1, id + 1};
+    int c[4] = {ip, ip + 1, ip + 1, ip};
+    if (c[1] > ncols) c[1] = 1;
+    if (c[2] > ncols) c[2] = 1;
+    const float di1 = (phi - m->dphispw * (ip - 1)) / m->dphispw;
+    const float f[4] = {(1.0f - di1) * (1.0f - dj1), di1 * (1.0f - dj1), di1 * dj1, (1.0f - di1) * dj1};
+    float wup = 0.0f, wvp = 0.0f, pcp = 0.0f, prp = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      const size_t at = (size_t)(r[k] - 1) * ncols + (c[k] - 1);
+      wup = wup + f[k] * wu[at];
+      wvp = wvp + f[k] * wv[at];
+      if (m->patmos) pcp = pcp + f[k] * pb[at];
+      if (m->precip && m->spw_rain) prp = prp + f[k] * rn[at];
+    }
+    const float vmag = sqrtf(wup * wup + wvp * wvp);
+    const float cd = m->cdval[(int)(vmag * 10) + 1 - 1];
+    float merge;
+    if (dst > m->spw_merge_frac * m->spw_radius) {
+      const float mf = m->spw_merge_frac < 0.999f ? m->spw_merge_frac : 0.999f;
+      merge = (1.0f / (1.0f - mf)) * (m->spw_radius - dst) / m->spw_radius;
+    } else {
+      merge = 1.0f;
+    }
+    merge = merge * tspinup_fac;
+    tu[nm] = (1.0f - merge) * tu[nm] + merge * vmag * (m->cosrot * wup + m->sinrot * wvp) * m->rhoa * cd / m->rhow;
+    tv[nm] = (1.0f - merge) * tv[nm] + merge * vmag * (-m->sinrot * wup + m->cosrot * wvp) * m->rhoa * cd / m->rhow;
+    if (m->patmos) pa[nm] = (1.0f - merge) * pa[nm] + merge * pcp;
+    if (m->precip && m->spw_rain) pr[nm] = (1.0f - merge) * pr[nm] + merge * prp / (1000 * 3600);
+  }
+  free(wu); free(wv); free(pb); free(rn);
+}
+
+/* update_meteo_fields (:1561-1624): the fields at meteo_t0 and meteo_t1. */
+void oracle_meteo_fields(OM* m) {
+  for (int itw = 0; itw < 2; ++itw) {
+    const float mt = itw == 0 ? m->meteo_t0 : m->meteo_t1;
+    float* tu = itw == 0 ? m->tauwu0 : m->tauwu1;
+    float* tv = itw == 0 ? m->tauwv0 : m->tauwv1;
+    float* pa = itw == 0 ? m->patm0 : m->patm1;
+    float* pr = itw == 0 ? m->prcp0 : m->prcp1;
+    if (m->amu.nt > 0) grid_to_cells(m, &m->amu, &m->amv, 0, 0, mt, tu, tv, 0.0f);
+    else for (int32_t nm = 0; nm < m->np; ++nm) { tu[nm] = 0.0f; tv[nm] = 0.0f; }
+    if (m->patmos) {
+      if (m->amp.nt > 0) grid_to_cells(m, &m->amp, NULL, 1, 0, mt, pa, NULL, m->gapres);
+      else for (int32_t nm = 0; nm < m->np; ++nm) pa[nm] = m->gapres;
+    }
+    if (m->meteo_rain) {
+      if (m->ampr.nt > 0) grid_to_cells(m, &m->ampr, NULL, 2, m->amprblock, mt, pr, NULL, 0.0f);
+      else for (int32_t nm = 0; nm < m->np; ++nm) pr[nm] = 0.0f;
+    }
+    if (m->spw_nt > 0) spw_to_cells(m, mt, tu, tv, pa, pr);
+  }
+}
+
+/* sfincs_lib.f90:493-503 with update_meteo_fields: a new window at the first step with
+ * t >= twindupd. */
+void oracle_meteo_window(OM* m) {
+  if (m->meteo3d && m->t >= (double)m->twindupd) {
+    m->meteo_t0 = m->twindupd;
+    m->meteo_t1 = m->twindupd + m->dtwnd;
+    m->twindupd = m->twindupd + m->dtwnd;
+    oracle_meteo_fields(m);
+  }
+}
+
+/* update_meteo_forcing's gridded branch (:1270-1419): every cell between the window's two
+ * fields, negative rain dropped on dry cells, the spin-up. */
+static void meteo3d_forcing(OM* m, double t) {
+  const float twfact = (float)((t - (double)m->meteo_t0) / (double)(m->meteo_t1 - m->meteo_t0));
+  const float onemintwfact = 1.0f - twfact;
+  for (int32_t nm = 0; nm < m->np; ++nm) {
+    const int dry = m->subgrid ? (m->z_volume[nm] <= 0.0) : (m->zs[nm] <= (double)m->zb[nm]);
+    if (m->meteo_wind) {
+      m->tauwu[nm] = m->tauwu0[nm] * onemintwfact + m->tauwu1[nm] * twfact;
+      m->tauwv[nm] = m->tauwv0[nm] * onemintwfact + m->tauwv1[nm] * twfact;
+    }
+    if (m->patmos) m->patm[nm] = m->patm0[nm] * onemintwfact + m->patm1[nm] * twfact;
+    if (m->meteo_rain) {
+      float prcp = m->prcp0[nm] * onemintwfact + m->prcp1[nm] * twfact;
+      if (prcp < 0.0f && dry) prcp = 0.0f;
+      m->netprcp[nm] = prcp;
+    }
+  }
+  if (t < (double)(m->tspinup - 1.0e-3f) && m->spinup_meteo) {
+    const float smfac = (float)((t - (double)m->t0) / (double)(m->tspinup - m->t0));
+    const float oneminsmfac = 1.0f - smfac;
+    for (int32_t nm = 0; nm < m->np; ++nm) {
+      if (m->meteo_wind) { m->tauwu[nm] = m->tauwu[nm] * smfac; m->tauwv[nm] = m->tauwv[nm] * smfac; }
+      if (m->patmos) m->patm[nm] = m->patm[nm] * smfac + m->gapres * oneminsmfac;
+      if (m->meteo_rain) {
+        m->netprcp[nm] = m->netprcp[nm] * smfac;
+        if (m->netprcp[nm] < 0.0f) {
+          const int dry = m->subgrid ? (m->z_volume[nm] <= 0.0) : (m->zs[nm] <= (double)m->zb[nm]);
+          if (dry) m->netprcp[nm] = 0.0f;
+        }
+      }
+    }
+  }
+}
+
+/* ---- infiltration: update_infiltration_map, sfincs_infiltration.f90:632-676 ----------- */
+/* Constant rates ('con', 'c2d'): a cell with water keeps its rate, a dry one (z_volume <= 0
+ * on subgrid meshes, zs <= zb on regular ones) none; the rate leaves the net
+ * precipitation. The cumulative infiltration (:665-670) is output only and not kept. */
+void oracle_update_infiltration(OM* m) {
+  for (int32_t nm = 0; nm < m->np; ++nm) {
+    float qinfmap = m->qinffield[nm];
+    if (m->subgrid) {
+      if (m->z_volume[nm] <= 0.0) qinfmap = 0.0f;
+    } else {
+      if (m->zs[nm] <= (double)m->zb[nm]) qinfmap = 0.0f;
+    }
+    m->netprcp[nm] = m->netprcp[nm] - qinfmap;
   }
 }
 
@@ -583,6 +923,7 @@ static void update_boundary_conditions(OM* m, double t, float dt) {
       } else {
         zst = (double)m->zst_bnd[0];
       }
+      if (m->patmos && m->pavbnd > 1.0f) zst = zst + (m->pavbnd - m->patm[nmb]) / (m->rhow * 9.81f);   /* :820-826 */
       const float zsetup = 0.0f;
       float zig = 0.0f;
       if (m->has_bzi) {   /* :829-848 */
@@ -820,7 +1161,11 @@ int32_t oracle_update(OM* m, double tend, int32_t max_steps) {
     m->dt = m->alpha * m->min_dt;
     const float dtchk = m->alpha * m->min_dt;
     m->t = m->t + (double)m->dt;
-    if (m->wind || m->precip) oracle_update_meteo(m, m->t, m->dt);
+    oracle_meteo_window(m);
+    if (m->wind || m->patmos || m->precip) {
+      oracle_update_meteo(m, m->t, m->dt);
+      if (m->infiltration) oracle_update_infiltration(m);   /* sfincs_lib.f90:534-540 */
+    }
     oracle_update_boundaries(m, m->t, m->dt);
     oracle_update_discharges(m, m->t, m->dt);
     oracle_compute_fluxes(m, m->dt);

@@ -27,7 +27,13 @@ class sfx_params(C.Structure):
         "advection", "advection_scheme", "viscosity", "coriolis", "friction2d", "advection_mask",
         "wiggle_suppression", "store_zsmax", "bndtype", "h73table", "store_velmax", "store_fluxmax",
         "store_twet")] + [("twet_threshold", C.c_float), ("timestep_analysis", C.c_int32), ("use_qext", C.c_int32),
-                                       ("structure_relax", C.c_float)]
+                                       ("structure_relax", C.c_float), ("qinf", C.c_float),
+                                       ("qinf_zmin", C.c_float), ("dtwnd", C.c_float), ("baro", C.c_int32),
+                                       ("pavbnd", C.c_float), ("gapres", C.c_float), ("spinup_meteo", C.c_int32),
+                                       ("amprblock", C.c_int32), ("usespwprecip", C.c_int32),
+                                       ("spw_merge_frac", C.c_float), ("factor_wind", C.c_float),
+                                       ("factor_pres", C.c_float), ("factor_prcp", C.c_float),
+                                       ("factor_spw_size", C.c_float)]
 
 
 class sfx_quadtree_input(C.Structure):
@@ -71,6 +77,11 @@ def _sig(lib):
         "sfx_model_read_subgrid_binary": (I, [P, C.c_char_p]),
         "sfx_model_set_drainage": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_set_thin_dams": (I, [P, i32, P, P, P]),
+        "sfx_model_set_infiltration": (I, [P, P]),
+        "sfx_model_set_meteo_grid": (I, [P, i32, i32, i32, i32, f, f, f, f, P, P]),
+        "sfx_model_set_spiderweb": (I, [P, i32, i32, i32, f, P, P, P, P, P, P, P]),
+        "sfx_model_read_meteo_file": (I, [P, s, s, s]),
+        "sfx_model_find_cells": (I, [P, i32, P, P, P]),
         "sfx_model_set_weirs": (I, [P, i32, P, P, P, P]),
         "sfx_model_sizes": (I, [P, C.POINTER(sfx_sizes)]),
         "sfx_model_get_table": (I, [P, s, P, sz]),
@@ -309,6 +320,53 @@ class Model:
         nv, x, y, par = self._polylines(polylines, 2)
         check(lib().sfx_model_set_weirs(self.h, len(nv), ptr(nv), ptr(x), ptr(y), ptr(par)), "set_weirs")
 
+    METEO_QUANTITY = {"amu": 0, "amv": 1, "amp": 2, "ampr": 3}
+
+    def set_meteo_grid(self, kind, times, values, x_llcorner, y_llcorner, dx, dy):
+        """Gridded meteo (the contents of an amu, amv, amp or ampr file): values[nt, nrows,
+        ncols] with row 0 the northernmost, times in s from tref; m/s, Pa or mm/h."""
+        t = np.ascontiguousarray(times, np.float32)
+        v = np.ascontiguousarray(values, np.float32)
+        nt, nr, nc = v.shape
+        self.meteo_inputs = getattr(self, "meteo_inputs", {})
+        self.meteo_inputs[kind] = dict(times=t, values=v, x_llcorner=float(x_llcorner),
+                                       y_llcorner=float(y_llcorner), dx=float(dx), dy=float(dy))
+        check(lib().sfx_model_set_meteo_grid(self.h, self.METEO_QUANTITY[kind], nt, nr, nc, x_llcorner, y_llcorner,
+                                             dx, dy, ptr(t), ptr(v)), "set_meteo_grid")
+
+    def set_spiderweb(self, times, xe, ye, radius, vmag, vdir, pdrop, rain=None):
+        """A spiderweb (spw file): eye track xe/ye per time, [nt, nrows, ncols] polar fields of
+        wind speed (m/s), nautical direction (deg), pressure drop (Pa) and rain (mm/h or None)."""
+        a = [np.ascontiguousarray(x, np.float32) for x in (times, xe, ye, vmag, vdir, pdrop)]
+        r = None if rain is None else np.ascontiguousarray(rain, np.float32)
+        nt, nr, nc = a[3].shape
+        self.meteo_inputs = getattr(self, "meteo_inputs", {})
+        self.meteo_inputs["spw"] = dict(times=a[0], xe=a[1], ye=a[2], radius=float(radius), vmag=a[3], vdir=a[4],
+                                        pdrop=a[5], rain=r)
+        check(lib().sfx_model_set_spiderweb(self.h, nt, nr, nc, radius, ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]),
+                                            ptr(a[4]), ptr(a[5]), ptr(r) if r is not None else None),
+              "set_spiderweb")
+
+    def find_cells(self, x, y):
+        """The active cell (1-based, 0 outside) holding each point (observation points)."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        nm = np.zeros(x.size, np.int32)
+        check(lib().sfx_model_find_cells(self.h, x.size, ptr(x), ptr(y), ptr(nm)), "find_cells")
+        return nm
+
+    def read_meteo_file(self, kind, path, tref="20000101 000000"):
+        """An ASCII amu / amv / amp / ampr / spw file, read as the reference reads it."""
+        check(lib().sfx_model_read_meteo_file(self.h, kind.encode(), str(path).encode(), tref.encode()),
+              "read_meteo_file")
+
+    def set_infiltration(self, qinf_mmh):
+        """Spatially varying constant infiltration (qinffile): mm/h per active cell. Acts only
+        with rain; params.qinf > 0 (uniform) takes precedence."""
+        a = np.ascontiguousarray(qinf_mmh, np.float32)
+        self._keep_qinf = a
+        check(lib().sfx_model_set_infiltration(self.h, ptr(a)), "set_infiltration")
+
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)
         check(lib().sfx_model_set_rain(self.h, len(t), ptr(t), ptr(v)), "set_rain")
@@ -376,7 +434,7 @@ STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv":
                 "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
                 "uvmean": np.float32, "vmax": np.float32, "qmax": np.float32, "twet": np.float32,
                 "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32, "qext": np.float32,
-                "storage_volume": np.float32, "zb": np.float32}
+                "storage_volume": np.float32, "zb": np.float32, "qtsrc": np.float32}
 
 
 class Instance:
@@ -400,6 +458,8 @@ class Instance:
             return s["npuv"]
         if name == "uvmean":
             return s["nkcuv2"]
+        if name == "qtsrc":
+            return s["nsrc"] + 2 * s["ndrn"]
         raise KeyError(name)
 
     def get(self, name):

@@ -66,6 +66,10 @@ struct Scalars {
   int32_t itbndlast, itsrclast, itwndlast, itprcplast;
   float tauwu, tauwv, netprcp;   // uniform meteo forcing of this step
   float factime, facrel, one_minus_factime;
+  // gridded meteo: the next update time, the current window [meteo_t0, meteo_t1] and its
+  // frame mwin, the step's time weight mtw (and 1 - mtw), the spin-up factor msm (mspin)
+  float twindupd, meteo_t0, meteo_t1, mtw, momtw, msm;
+  int32_t mwin, mspin;
   int32_t ahead;                 // the next step's fold, scalars and series are staged (StepImage)
   int32_t pad_;
 };
@@ -124,11 +128,18 @@ struct DevInst {
   float* tsa_req; float* tsa_sum; int32_t* tsa_wet; int32_t* tsa_lim;
   float* tsa_min;                // the last sweep's minimum time step
   float* qext;                   // external source per cell, m/s (use_qext)
+  const float* qinf;             // infiltration rate per cell, m/s (qinffield; F_INFIL)
+  // gridded meteo fields at the update times, [mnframes][np]: wind stress (F_MWIND),
+  // atmospheric pressure (F_PATM), rain in m/s (F_MPRCP)
+  const float* mtauu; const float* mtauv; const float* mpatm; const float* mprcp;
+  int32_t mnframes;
   // FP64 volume balance of the interior cells (kcs 1). Each step adds to bal[0..3] the
   // volume that entered them through faces whose other side is not interior (boundary,
   // Neumann and inactive cells; summed by one block of the continuity launch), from point
   // sources as applied (the regular branch's bed clamp included), rain and qext; bal[4..5]
-  // receive the interior volume and the storage left when they are summed.
+  // receive the interior volume and the storage left when they are summed; bal[6..6+NINF)
+  // the uniform rain volume of interior cells less what they received (infiltration and
+  // gridded rain), one slot per wave modulo NINF.
   double* bal;
   int32_t nbf;                   // faces read by an interior cell whose other side is not interior
   const int32_t* bf_slot;        // their q slots
@@ -205,6 +216,9 @@ constexpr int NCNT = 6;
 // Continuity counted per tile: cells that stored zs, V, zs0, zsderv, zsmax (a cell stores a
 // field only when its value changes), and steady cells skipped.
 constexpr int NSTORE = 6;
+// Volume balance: six sums, then NINF slots for the infiltrated volume (waves add to slot
+// wave number mod NINF, so their double atomics spread over many addresses).
+constexpr int NINF = 256, NBAL = 6 + NINF;
 
 struct Item { int32_t inst, tile; };
 
@@ -372,6 +386,47 @@ __device__ __forceinline__ void mark_wave(const DevInst& I, int tile, bool chang
   mark_cells(I, nlo, nhi, mlo, mhi);
 }
 
+// ------------------------------------------------------------------------ gridded meteo
+// The step's position in the gridded meteo: element offsets of the frames at the window's
+// start (a) and end (b), the time weight w (omw = 1 - w) and the spin-up factor.
+struct MeteoNow {
+  uint32_t a, b;
+  float w, omw, sm;
+  int spin;
+};
+template <typename SP>
+__device__ __forceinline__ MeteoNow meteo_now(const DevInst& I, SP S) {
+  MeteoNow m;
+  const int win = max((int)S->mwin, 0);
+  m.a = (uint32_t)win * (uint32_t)I.np;
+  m.b = m.a + (uint32_t)I.np;
+  m.w = S->mtw; m.omw = S->momtw; m.sm = S->msm; m.spin = S->mspin;
+  return m;
+}
+// update_meteo_forcing for one cell (sfincs_meteo.f90:1285-1368): a field between its two
+// frames, then the spin-up (wind stress scaled, pressure towards gapres).
+__device__ __forceinline__ float meteo_tau(const float* T, const MeteoNow& m, int c) {
+  float v = G(T)[m.a + (uint32_t)c] * m.omw + G(T)[m.b + (uint32_t)c] * m.w;
+  if (m.spin) v = v * m.sm;
+  return v;
+}
+__device__ __forceinline__ float meteo_patm(const DevInst& I, const MeteoNow& m, int c) {
+  float v = G(I.mpatm)[m.a + (uint32_t)c] * m.omw + G(I.mpatm)[m.b + (uint32_t)c] * m.w;
+  if (m.spin) v = v * m.sm + I.c.gapres * (1.0f - m.sm);
+  return v;
+}
+// The meteo a wet uv-point reads (sfincs_momentum.f90:549-589): the wind stress of its cell
+// nm, uniform (time series) or gridded, and the pressure of its two cells.
+template <typename R, typename SP>
+__device__ __forceinline__ void uv_meteo(const DevInst& I, SP S, int nm, int nmu, sfx::UvIn<R>& in) {
+  const uint32_t f = I.c.flags;
+  if (!(f & (sfx::F_MWIND | sfx::F_PATM))) { in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv; return; }
+  const MeteoNow m = meteo_now(I, S);
+  if (f & sfx::F_MWIND) { in.tauwu = (R)meteo_tau(I.mtauu, m, nm); in.tauwv = (R)meteo_tau(I.mtauv, m, nm); }
+  else { in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv; }
+  if (f & sfx::F_PATM) { in.patm_nm = (R)meteo_patm(I, m, nm); in.patm_nmu = (R)meteo_patm(I, m, nmu); }
+}
+
 // ----------------------------------------------------------------------------- forcing
 // One block per scenario: step scalars, meteo series, boundary points/conditions/fluxes,
 // discharge series (sfincs_lib.f90:379-550 with sfincs_meteo.f90:1423-1558,
@@ -454,7 +509,26 @@ __device__ void step_series(const DevInst& I, StepImage& F, float* qout, int mod
     F.s.one_minus_factime = 1.0f - F.s.factime;
     F.s.facrel = 1.0f - sfx::vmin<float>(dt / c.btrelax, 1.0f);
   }
-  if (wave == 1 && lane == 0 && (c.flags & sfx::F_WIND)) {
+  if (wave == 1 && lane == 0 && (c.flags & sfx::F_METEO3D)) {
+    // gridded meteo: a new window at the first step with t >= twindupd (sfincs_lib.f90:493-503),
+    // the step's time weight and spin-up factor (sfincs_meteo.f90:1270-1273, 1349-1352)
+    if (t >= (double)F.s.twindupd) {
+      F.s.meteo_t0 = F.s.twindupd;
+      F.s.meteo_t1 = F.s.twindupd + c.dtwnd;
+      F.s.twindupd = F.s.twindupd + c.dtwnd;
+      F.s.mwin = min(F.s.mwin + 1, I.mnframes - 2);
+    }
+    const float tw = (float)((t - (double)F.s.meteo_t0) / (double)(F.s.meteo_t1 - F.s.meteo_t0));
+    F.s.mtw = tw;
+    F.s.momtw = 1.0f - tw;
+    F.s.mspin = 0;
+    F.s.msm = 1.0f;
+    if ((c.flags & sfx::F_MSPIN) && t < (double)(c.tspinup - 1.0e-3f)) {
+      F.s.msm = (float)((t - (double)c.t0) / (double)(c.tspinup - c.t0));
+      F.s.mspin = 1;
+    }
+  }
+  if (wave == 1 && lane == 0 && (c.flags & sfx::F_WIND) && I.ntwnd > 0) {
     for (int itw = max(F.s.itwndlast, 2); itw <= I.ntwnd; ++itw) {
       if ((double)G(I.twnd)[itw - 1] > t) {
         const float twfac = (float)((t - (double)G(I.twnd)[itw - 2]) / (double)(G(I.twnd)[itw - 1] - G(I.twnd)[itw - 2]));
@@ -475,7 +549,7 @@ __device__ void step_series(const DevInst& I, StepImage& F, float* qout, int mod
       }
     }
   }
-  if (wave == 2 && lane == 0 && (c.flags & sfx::F_PRECIP)) {
+  if (wave == 2 && lane == 0 && (c.flags & sfx::F_PRECIP) && I.ntprcp > 0) {
     int itp;
     for (itp = F.s.itprcplast; itp <= I.ntprcp; ++itp)
       if ((double)G(I.tprcpt)[itp - 1] > t) break;
@@ -614,6 +688,7 @@ struct BndStep {
   float tbfac;
   double t;
   R factime, facrel, one_minus_factime;
+  MeteoNow m;   // for the pressure correction of water-level boundaries (F_PATM)
 };
 
 // One boundary flux point's inputs: everything update_boundary_fluxes reads from tables and
@@ -675,6 +750,8 @@ __device__ __forceinline__ void bnd_load(const DevInst& I, const BndStep<R>& B, 
       zst = (double)series_at(I.zs_bnd, 0);
       if (I.zsi_bnd) zig = series_at(I.zsi_bnd, 0);
     }
+    if ((c.flags & sfx::F_PATM) && c.pavbnd > 1.0f)   // barometric correction (:820-826)
+      zst = zst + (double)((c.pavbnd - meteo_patm(I, B.m, p.nmb)) / (c.rhow * 9.81f));
     float zb0, zb1;
     if (B.t < (double)(c.tspinup - 1.0e-3f)) {
       const float smfac = (float)(1.0 - (B.t - (double)c.t0) / (double)(c.tspinup - c.t0));
@@ -817,7 +894,8 @@ __global__ void __launch_bounds__(FBLOCK) k_forcing(const DevInst* __restrict__ 
   const auto uv = G((R*)I.uv[curb]);
   const auto uvmean = G((R*)I.uvmean);
   const auto zsmax = G((R*)I.zsmax);
-  const BndStep<R> B{F.itb0, F.itb1, F.tbfac, t, (R)F.s.factime, (R)F.s.facrel, (R)F.s.one_minus_factime};
+  const BndStep<R> B{F.itb0, F.itb1, F.tbfac, t, (R)F.s.factime, (R)F.s.facrel, (R)F.s.one_minus_factime,
+                     meteo_now(I, &F.s)};
   for (int k0 = threadIdx.x; k0 < I.nflux; k0 += 2 * FBLOCK) {
     const int k1 = k0 + FBLOCK;
     BndPt<R> a, b;
@@ -964,10 +1042,10 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, R
   }
   const int ku = G(I.kcuv)[hu ? c : dm], kv = G(I.kcuv)[hv ? np + c : dm];
   const float fu = uv_floor<SUBGRID>(I, hu ? c : dm), fv = uv_floor<SUBGRID>(I, hv ? np + c : dm);
-  const R tauwu = (R)S->tauwu, tauwv = (R)S->tauwv;
   if (hu) {
     if (ku == 1 || ku == 6) {
-      iu.idir = 0; iu.iref = 1; iu.itype = 0; iu.tauwu = tauwu; iu.tauwv = tauwv;
+      iu.idir = 0; iu.iref = 1; iu.itype = 0;
+      uv_meteo<R>(I, S, c, c + nmax, iu);
       int br = 0;
       finish_uv<R, SUBGRID>(I, c, iu, fu, dt, q1, u1, mden, br);
       cnt[br]++;
@@ -984,7 +1062,8 @@ __device__ __forceinline__ void mom_cell(const DevInst& I, int tile, int* cnt, R
   if (hv) {
     const int s = np + c;
     if (kv == 1 || kv == 6) {
-      iv.idir = 1; iv.iref = 1; iv.itype = 0; iv.tauwu = tauwu; iv.tauwv = tauwv;
+      iv.idir = 1; iv.iref = 1; iv.itype = 0;
+      uv_meteo<R>(I, S, c, c + 1, iv);
       int br = 0;
       finish_uv<R, SUBGRID>(I, s, iv, fv, dt, q1, u1, mden, br);
       cnt[br]++;
@@ -1085,7 +1164,7 @@ __device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, in
   }
   // the rest of the stencil of a wet point
   point_stencil<R, SUBGRID>(I, n, m, vplane, kf, q0, u0, in);
-  in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+  uv_meteo<R>(I, S, c, c_nmu, in);
   int br = 0;
   wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mden, br, kf);
   cnt[br]++;
@@ -1231,7 +1310,6 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
   if (m0 >= mmax) return;                              // uniform over the wave
   const R dq = q0[dm], du = u0[dm];
   const int dk = G(I.kcuv)[dm];
-  const R tauwu = (R)S->tauwu, tauwv = (R)S->tauwv;
   Row<R> A, B, C, D;
   strip_raw<R, SUBGRID>(I, q0, u0, n, m0 - 1, colok, A, dq, du, dk);
   strip_raw<R, SUBGRID>(I, q0, u0, n, m0, colok, B, dq, du, dk);
@@ -1258,7 +1336,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           in.qy_nmu = C.vq; in.v_nmu = C.vu;
           in.qy_ndmu = C.l_vq; in.v_ndmu = C.l_vu;
           in.kfuv = G(I.kfuv)[c] & 1; in.mask_adv = G(I.mask_adv)[c]; in.idir = 0; in.iref = 1; in.itype = 0;
-          in.tauwu = tauwu; in.tauwv = tauwv;
+          uv_meteo<R>(I, S, c, c + nmax, in);
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = C.vol; in.zsderv_nm = B.zd; in.zsderv_nmu = C.zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)C.zd; }
           int br = 0;
@@ -1290,7 +1368,7 @@ __device__ __forceinline__ void mom_strip(const DevInst& I, int tile, int* cnt, 
           in.qy_nmu = B.r_uq; in.v_nmu = B.r_uu;
           in.qy_ndmu = A.r_uq; in.v_ndmu = A.r_uu;
           in.kfuv = G(I.kfuv)[s] & 1; in.mask_adv = G(I.mask_adv)[s]; in.idir = 1; in.iref = 1; in.itype = 0;
-          in.tauwu = tauwu; in.tauwv = tauwv;
+          uv_meteo<R>(I, S, c, c + 1, in);
           if (SUBGRID) { in.vol_nm = B.vol; in.vol_nmu = B.r_vol; in.zsderv_nm = B.zd; in.zsderv_nmu = B.r_zd; }
           else { in.zb_nm = (float)B.zd; in.zb_nmu = (float)B.r_zd; }
           int br = 0;
@@ -1371,7 +1449,7 @@ __device__ __forceinline__ void mom_idx(const DevInst& I, int tile, int* cnt, R&
     in.kcuv_nmd = in.kcuv_nmu = 0;
   }
   in.kfuv = kf & 1; in.mask_adv = G(I.mask_adv)[ip];
-  in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv;
+  uv_meteo<R>(I, S, nm, nmu, in);
   if (SUBGRID) {
     in.vol_nm = G(I.vol)[nm]; in.vol_nmu = G(I.vol)[nmu];
     in.zsderv_nm = G((const R*)I.zsderv)[nm]; in.zsderv_nmu = G((const R*)I.zsderv)[nmu];
@@ -1527,10 +1605,38 @@ __global__ void __launch_bounds__(BLOCK) k_cuv(const DevInst* __restrict__ insts
 __device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
 __device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
 
+// The net rain of one cell, `dry` telling whether it held no water at the start of the
+// step (volume not above zero on subgrid meshes, level not above the bed on regular ones;
+// point sources come later, in continuity): gridded rain between its two frames, a
+// negative value dropped on a dry cell, then spun up (update_meteo_forcing,
+// sfincs_meteo.f90:1309-1330, 1370-1390); less the infiltration rate of a wet cell
+// ('con' / 'c2d', update_infiltration_map, sfincs_infiltration.f90:632-663). Without
+// either the step's uniform rain np_u. By how much an interior cell's rain volume differs
+// from the uniform rain's goes to inf.
+template <typename R>
+__device__ __forceinline__ R cell_rain(const DevInst& I, int c, int kcs, int iref, R dt, R np_u, bool dry,
+                                       const MeteoNow& m, double& inf) {
+  const uint32_t f = I.c.flags;
+  if (!(f & (sfx::F_MPRCP | sfx::F_INFIL))) return np_u;
+  R np_c = np_u;
+  if (f & sfx::F_MPRCP) {
+    float p = G(I.mprcp)[m.a + (uint32_t)c] * m.omw + G(I.mprcp)[m.b + (uint32_t)c] * m.w;
+    if (p < 0.0f && dry) p = 0.0f;
+    if (m.spin) {
+      p = p * m.sm;
+      if (p < 0.0f && dry) p = 0.0f;
+    }
+    np_c = (R)p;
+  }
+  if ((f & sfx::F_INFIL) && !dry) np_c = np_c - (R)G(I.qinf)[c];
+  if (kcs == 1) inf = ((double)(np_u * dt) - (double)(np_c * dt)) * (double)I.c.cell_area[iref - 1];
+  return np_c;
+}
+
 // Returns the fields it stored, one bit each in NSTORE order.
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t kraw, R q_md, R q_mu, R q_nd, R q_nu,
-                                              int iref, R dt, R netprcp) {
+                                              int iref, R dt, R netprcp, const MeteoNow& mt, double& inf) {
   unsigned wrote = 0;
   const int kcs = kraw & KCS_MASK;
   double zs = G(I.zs)[c];
@@ -1544,6 +1650,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
   if (SUBGRID) {
     double vol = G(I.vol)[c];
     const double vol_in = vol;   // as loaded: a source alone changes the volume too
+    netprcp = cell_rain<R>(I, c, kcs, iref, dt, netprcp, vol <= 0.0, mt, inf);
     if (kraw & KCS_SOURCE) {   // point sources of this cell, in source order (:311-325)
       double applied = 0.0;
       for (int s = 0; s < I.nsrc + 2 * I.ndrn; ++s)
@@ -1573,6 +1680,8 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
       if (!same_bits(zsd, zsd_in)) { zsdp[c] = zsd; wrote |= 8u; }
     }
   } else {
+    if (I.c.flags & (sfx::F_MPRCP | sfx::F_INFIL))
+      netprcp = cell_rain<R>(I, c, kcs, iref, dt, netprcp, zs <= (double)G(I.zb)[c], mt, inf);
     if (kraw & KCS_SOURCE) {   // (:87-106)
       const double before = zs;
       for (int s = 0; s < I.nsrc + 2 * I.ndrn; ++s)
@@ -1590,20 +1699,24 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
 }
 
 // One cell of continuity with its steady bit. A cell is quiet when it is an ordinary cell
-// (kcs 1, no point source), its four face fluxes are zero and no rain falls: its update
-// then depends on its own state only (not on dt). A quiet cell whose last update, also
+// (kcs 1, no point source), its four face fluxes are zero, no rain falls and it has no
+// infiltration rate, and no gridded rain falls on it: its update then depends on its own
+// state only (not on dt). A quiet cell whose last update, also
 // quiet, changed nothing (KCS_STEADY) would change nothing again, so it is skipped after
 // reading its byte and faces; any other cell is updated and its bit set or cleared.
 // Boundary cells, whose levels forcing sets, are never steady; a host write of cell
 // state clears every bit.
 template <typename R, bool SUBGRID>
 __device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref,
-                                                R dt, R netprcp, bool dry_sky) {
+                                                R dt, R netprcp, bool dry_sky, const MeteoNow& mt, double& inf) {
   const uint8_t kraw = G(I.kcs)[c];
   const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
-                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & (sfx::F_STORE | sfx::F_QEXT));
+                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & (sfx::F_STORE | sfx::F_QEXT)) &&
+                     (!(I.c.flags & sfx::F_INFIL) || G(I.qinf)[c] == 0.0f) &&
+                     (!(I.c.flags & sfx::F_MPRCP) ||
+                      (G(I.mprcp)[mt.a + (uint32_t)c] == 0.0f && G(I.mprcp)[mt.b + (uint32_t)c] == 0.0f));
   if (quiet && (kraw & KCS_STEADY)) return 32u;   // skipped (counted as such)
-  const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp);
+  const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, mt, inf);
   const bool steady = quiet && w == 0u;
   if (steady != ((kraw & KCS_STEADY) != 0))
     G(I.kcs)[c] = (uint8_t)((kraw & ~KCS_STEADY) | (steady ? KCS_STEADY : 0));
@@ -1641,6 +1754,14 @@ __device__ __forceinline__ void store_vars(const DevInst& I, int c, R q_md, R q_
       tw[c] = tw[c] + dt;
     }
   }
+}
+
+// Adds a wave's cells' differences from the uniform rain volume (infiltration, gridded rain)
+// to its balance slot (every lane active).
+__device__ __forceinline__ void add_infiltrated(const DevInst& I, int tile, double v) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  if (threadIdx.x % WAVE == 0 && v != 0.0)
+    atomicAdd(I.bal + 6 + (int)(((uint32_t)tile * (BLOCK / WAVE) + threadIdx.x / WAVE) & (NINF - 1)), v);
 }
 
 // Adds a wave's continuity stores per field to its tile's counters (active lanes only).
@@ -1703,19 +1824,21 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   const R dt = (R)S->dt;
   const R netprcp = (R)S->netprcp;
   const bool dry_sky = !(I.c.flags & sfx::F_PRECIP) || netprcp == R(0);
+  double inf = 0.0;   // uniform rain volume less what this thread's cell receives
+  const MeteoNow mt = meteo_now(I, S);
   if (!indexed) {
     const int nmax = I.nmax, mmax = I.mmax, np = I.np;
     const int tn = tile % I.ntn, tmi = tile / I.ntn;
     const int n = tn * I.tn + (threadIdx.x & (I.tn - 1));
     const int m = tmi * I.tm + (threadIdx.x >> I.tn_sh);
-    if (n >= nmax || m >= mmax) return;
+    if (n < nmax && m < mmax) {
     const int c = m * nmax + n;
     const R q_md = m >= 1 ? q[c - nmax] : R(0);
     const R q_mu = m < mmax - 1 ? q[c] : R(0);
     const R q_nd = n >= 1 ? q[np + c - 1] : R(0);
     const R q_nu = n < nmax - 1 ? q[np + c] : R(0);
-    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky)
-                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky);
+    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky, mt, inf)
+                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky, mt, inf);
     if (I.count) count_stores(I, tile, w);
     if (I.c.flags & sfx::F_STORE) {
       R u_md = R(0), u_mu = R(0), u_nd = R(0), u_nu = R(0);
@@ -1729,13 +1852,14 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
       store_vars<R>(I, c, q_md, q_mu, q_nd, q_nu, u_md, u_mu, u_nd, u_nu, dt);
     }
     if (I.sk < 0) mark_wave(I, tile, (w & 1u) != 0);
+    }
   } else {
     const int c = tile * BLOCK + threadIdx.x;
-    if (c >= I.np) return;
+    if (c < I.np) {
     const R q_md = q[G(I.zmd)[c]], q_mu = q[G(I.zmu)[c]], q_nd = q[G(I.znd)[c]], q_nu = q[G(I.znu)[c]];
     const int iref = G(I.z_iref)[c];
-    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky)
-                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky);
+    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky, mt, inf)
+                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky, mt, inf);
     if (I.count) count_stores(I, tile, w);
     if (I.c.flags & sfx::F_STORE) {
       R u_md = R(0), u_mu = R(0), u_nd = R(0), u_nu = R(0);
@@ -1745,7 +1869,9 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
       }
       store_vars<R>(I, c, q_md, q_mu, q_nd, q_nu, u_md, u_mu, u_nd, u_nu, dt);
     }
+    }
   }
+  if (I.c.flags & (sfx::F_INFIL | sfx::F_MPRCP)) add_infiltrated(I, tile, inf);   // every lane of the block is here
 }
 
 // FP64 volume accumulator: sum over active cells of V (subgrid) or zs*A (regular).
@@ -1927,6 +2053,12 @@ uint32_t build_flags(const sfx_model* M) {
   if (M->subgrid && !M->storage_volume.empty()) f |= sfx::F_STORAGE;
   if (M->wind) f |= sfx::F_WIND;
   if (M->precip) f |= sfx::F_PRECIP;
+  if (M->precip && !M->qinffield.empty()) f |= sfx::F_INFIL;
+  if (M->meteo3d) f |= sfx::F_METEO3D;
+  if (M->meteo3d && M->meteo_wind) f |= sfx::F_MWIND;
+  if (M->meteo3d && M->patmos) f |= sfx::F_PATM;
+  if (M->meteo3d && M->meteo_rain) f |= sfx::F_MPRCP;
+  if (M->meteo3d && p.spinup_meteo) f |= sfx::F_MSPIN;
   if (p.h73table) f |= sfx::F_H73;
   if (M->use_quadtree) f |= sfx::F_QUADTREE;
   if (M->subgrid) f |= sfx::F_SUBGRID;
@@ -1944,6 +2076,7 @@ int fill_consts(const sfx_model* M, Consts& c) {
   c.alpha = p.alpha; c.rhoa = p.rhoa; c.rhow = p.rhow; c.btfilter = p.btfilter; c.btrelax = p.btrelax;
   c.t0 = p.t0; c.t1 = p.t1; c.tspinup = p.t0 + p.tspinup; c.zini = p.zsini; c.dx = M->dxrm[0]; c.dy = M->dyrm[0];
   c.pi = p.pi; c.twet_threshold = p.twet_threshold;
+  c.dtwnd = p.dtwnd; c.gapres = p.gapres; c.pavbnd = p.pavbnd;
   for (int i = 0; i < M->nref; ++i) {
     c.dxrinv[i] = M->dxrinv[i]; c.dyrinv[i] = M->dyrinv[i]; c.dxr2inv[i] = M->dxr2inv[i]; c.dyr2inv[i] = M->dyr2inv[i];
     c.dxrinvc[i] = M->dxrinvc[i]; c.dyrinvc[i] = M->dyrinvc[i]; c.cell_area[i] = M->cell_area[i];
@@ -1982,6 +2115,14 @@ int upload_state(sfx_handle* H, const sfx_model* M) {
     I.tsa_min = H->upload(zero.data(), 1);
   }
   if (f & sfx::F_STORAGE) { auto v = to_r<R>(M->storage_volume); I.storage = H->upload(v.data(), np); }
+  if (f & sfx::F_INFIL) I.qinf = H->upload(M->qinffield.data(), np);
+  if (f & sfx::F_METEO3D) {
+    I.mnframes = M->meteo_nframes;
+    if (f & sfx::F_MWIND) { I.mtauu = H->upload(M->meteo_tauwu.data(), M->meteo_tauwu.size());
+                            I.mtauv = H->upload(M->meteo_tauwv.data(), M->meteo_tauwv.size()); }
+    if (f & sfx::F_PATM) I.mpatm = H->upload(M->meteo_patm.data(), M->meteo_patm.size());
+    if (f & sfx::F_MPRCP) I.mprcp = H->upload(M->meteo_prcp.data(), M->meteo_prcp.size());
+  }
   if (f & sfx::F_QEXT) {   // sfincs_domain.f90:2383-2389
     std::vector<float> z(np, 0.f);
     I.qext = H->upload(z.data(), np);
@@ -2217,6 +2358,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
   s.t = (double)M->p.t0; s.tend = (double)M->p.t1;
   s.dt = 1.0e-6f; s.min_dt = 1.0e-6f;   // sfincs_lib.f90:281-282
   s.itbndlast = 1; s.itsrclast = 1; s.itwndlast = 1; s.itprcplast = 1;
+  s.twindupd = M->p.t0; s.mwin = -1; s.momtw = 1.0f; s.msm = 1.0f;   // sfincs_lib.f90:290
   H->dS = H->upload(&s, 1);
   I.sc = H->dS;
   // tiles
@@ -2275,7 +2417,7 @@ extern "C" int sfx_create_ex(const sfx_model* M, int32_t device, int32_t fp64, s
     I.bf_slot = H->upload(slot.data(), slot.size());
     I.bf_coef = H->upload(coef.data(), coef.size());
     I.rain_area = rain_area;
-    I.bal = H->alloc<double>(6);
+    I.bal = H->alloc<double>(NBAL);
   }
   H->dI = H->upload(&I, 1);
   for (void* p : H->allocs) if (!p) { set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
@@ -2442,6 +2584,12 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
   if (k == "uvmean") {
     const size_t n = H->sz.nkcuv2;
     return H->fp64 ? get_cell_r<double>(H, I.uvmean, dst, nbytes, n) : get_cell_r<float>(H, I.uvmean, dst, nbytes, n);
+  }
+  if (k == "qtsrc") {   // the step's discharges: point sources, then each drainage intake and outfall
+    const size_t n = (size_t)I.nsrc + 2 * (size_t)I.ndrn;
+    if (n == 0 || nbytes != n * 4) { set_error("sfx_get_state: qtsrc needs (nsrc + 2 ndrn) f32 values"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(dst, I.qtsrc, nbytes, hipMemcpyDeviceToHost));
+    return SFX_OK;
   }
   set_error("unknown state field " + k);
   return SFX_ERR_ARG;
@@ -2952,9 +3100,11 @@ extern "C" int sfx_get_balance(sfx_handle* H, double out[6]) {
   hipLaunchKernelGGL(k_balance_volume, dim3(nb), dim3(BLOCK), 0, H->stream, H->dI);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(H->stream));
-  double b[6];
+  double b[NBAL];
   HIPCHK(hipMemcpy(b, H->hI.bal, sizeof(b), hipMemcpyDeviceToHost));
-  out[0] = b[4]; out[1] = b[5]; out[2] = b[0]; out[3] = b[1]; out[4] = b[2]; out[5] = b[3];
+  double infiltrated = 0.0;
+  for (int k = 0; k < NINF; ++k) infiltrated += b[6 + k];
+  out[0] = b[4]; out[1] = b[5]; out[2] = b[0]; out[3] = b[1]; out[4] = b[2] - infiltrated; out[5] = b[3];
   return SFX_OK;
 }
 
@@ -2973,7 +3123,7 @@ extern "C" int sfx_update_zbuvmx(sfx_handle* H) {
 extern "C" int sfx_reset_balance(sfx_handle* H) {
   if (!H) { set_error("null"); return SFX_ERR_ARG; }
   HIPCHK(hipSetDevice(H->device));
-  hipLaunchKernelGGL(k_zero_bal, dim3(1), dim3(64), 0, H->stream, H->hI.bal, 0, 6);
+  hipLaunchKernelGGL(k_zero_bal, dim3(1), dim3(64), 0, H->stream, H->hI.bal, 0, NBAL);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(H->stream));
   return SFX_OK;
@@ -3284,7 +3434,7 @@ extern "C" int sfx_autotune(sfx_handle* H, int32_t trials) {
     snap[f].resize(n);
     if ((rc = sfx_get_state(H, f.c_str(), snap[f].data(), n))) return rc;
   }
-  double bal_keep[6];   // timed sweeps add to the volume balance: restored afterwards
+  double bal_keep[NBAL];   // timed sweeps add to the volume balance: restored afterwards
   HIPCHK(hipMemcpy(bal_keep, H->hI.bal, sizeof(bal_keep), hipMemcpyDeviceToHost));
   Scalars s = s0;
   s.active = 1; s.dt = std::max(s0.dt, 1.0e-3f);

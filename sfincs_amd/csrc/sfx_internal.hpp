@@ -7,6 +7,8 @@
 
 #include "../../include/sfx.h"
 
+struct sfx_model;
+
 namespace sfx {
 
 void set_error(const std::string& msg);
@@ -14,6 +16,10 @@ void set_error(const std::string& msg);
 // reference order; defined in sfx_model.cpp.
 int write_restart_arrays(const char* path, const double* zs, int32_t np, const float* q, int32_t nq,
                          const float* uvmean, int32_t nuvmean);
+
+// The gridded / spiderweb meteo fields of every update time (sfx_meteo.cpp); called by
+// sfx_model_initialize after the Cd table is built.
+int build_meteo(sfx_model* M);
 
 // 1-based reference index -> 0-based storage slot.
 inline int32_t z0(int32_t one_based) { return one_based - 1; }
@@ -114,6 +120,30 @@ struct sfx_model {
   // (kcuv 3) with crest height, coefficient and length relative to the grid spacing
   std::vector<int32_t> thindam_uv_index, structure_uv_index;   // 1-based
   std::vector<float> structure_height, structure_cd, structure_length;
+
+  // gridded meteo (amu/amv/amp/ampr contents) and a spiderweb, as set; and at initialize the
+  // fields of each update time T_k = t0 + k dtwnd (float sums, as twindupd) on the cells,
+  // frame-major [nframes][np]: wind stress, pressure, rain (m/s)
+  struct MeteoGrid {
+    int32_t nt = 0, nrows = 0, ncols = 0;
+    float xll = 0.f, yll = 0.f, dx = 0.f, dy = 0.f;
+    std::vector<float> times, v;   // v[nt][nrows][ncols]
+  };
+  MeteoGrid amu, amv, amp, ampr;
+  struct Spiderweb {
+    int32_t nt = 0, nrows = 0, ncols = 0;
+    float radius = 0.f;
+    bool rain = false;
+    std::vector<float> times, xe, ye, vmag, vdir, pdrop, prcp;
+  };
+  Spiderweb spw;
+  bool meteo3d = false, patmos = false, meteo_wind = false, meteo_rain = false;
+  int32_t meteo_nframes = 0;
+  std::vector<float> meteo_times, meteo_tauwu, meteo_tauwv, meteo_patm, meteo_prcp;
+
+  // infiltration ('con' from params.qinf, or 'c2d' from a per-cell map in mm/h): the rate
+  // per cell in m/s after initialize (empty: no infiltration)
+  std::vector<float> qinf_map_mmh, qinffield;
 
   // meteo series
   int32_t ntprcp = 0, ntwnd = 0;

@@ -40,6 +40,10 @@ extern "C" void sfx_params_default(sfx_params* p) {
   p->store_velmax = 0; p->store_fluxmax = 0; p->store_twet = 0; p->twet_threshold = 0.01f;
   p->timestep_analysis = 0; p->use_qext = 0;
   p->structure_relax = 10.0f;
+  p->qinf = 0.0f; p->qinf_zmin = 0.0f;
+  p->dtwnd = 1800.0f; p->baro = 1; p->pavbnd = 0.0f; p->gapres = 101200.0f; p->spinup_meteo = 0;
+  p->amprblock = 1; p->usespwprecip = 1; p->spw_merge_frac = 0.5f;
+  p->factor_wind = 1.0f; p->factor_pres = 1.0f; p->factor_prcp = 1.0f; p->factor_spw_size = 1.0f;
   p->bndtype = 1; p->h73table = 0;
 }
 
@@ -849,6 +853,23 @@ bool polylines_ok(int32_t npoly, const int32_t* nv, const float* x, const float*
 }
 }  // namespace
 
+// The active cell (1-based, 0 outside the mesh) holding each point, as the reference finds
+// observation points (find_quadtree_cell with index_sfincs_in_quadtree,
+// sfincs_obspoints.f90:95-130).
+extern "C" int sfx_model_find_cells(sfx_model* M, int32_t n, const float* x, const float* y, int32_t* nm) {
+  if (!M || !M->mesh_built || n < 0 || (n > 0 && (!x || !y || !nm))) { set_error("sfx_model_find_cells: bad args"); return SFX_ERR_ARG; }
+  for (int32_t i = 0; i < n; ++i) nm[i] = M->find_cell_xy(x[i], y[i]);
+  return SFX_OK;
+}
+
+// Spatially varying constant infiltration (qinffile, sfincs_infiltration.f90:224-258), in
+// mm/h per active cell; converted to m/s at initialize.
+extern "C" int sfx_model_set_infiltration(sfx_model* M, const float* qinf_mmh) {
+  if (!M || M->initialized || !M->mesh_built || !qinf_mmh) { set_error("sfx_model_set_infiltration: bad args"); return SFX_ERR_ARG; }
+  M->qinf_map_mmh.assign(qinf_mmh, qinf_mmh + M->np);
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_set_thin_dams(sfx_model* M, int32_t npoly, const int32_t* nv, const float* x, const float* y) {
   size_t nvert = 0;
   if (!M || M->initialized || !polylines_ok(npoly, nv, x, y, nvert)) {
@@ -1339,6 +1360,27 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
     }
   }
   apply_structures(M);
+  {
+    const int rc = sfx::build_meteo(M);
+    if (rc) return rc;
+  }
+
+  // initialize_infiltration (sfincs_infiltration.f90:8-258): only with rain (:63); the
+  // uniform rate ('con', qinf > 0, converted by sfincs_input.f90:376) before the map
+  // ('c2d', :224-258), as the reference tries them (:92-104)
+  M->qinffield.clear();
+  if (M->precip && (p.qinf > 0.0f || !M->qinf_map_mmh.empty())) {
+    M->qinffield.assign(np, 0.0f);
+    if (p.qinf > 0.0f) {
+      const float qinf = p.qinf / (float)(3600 * 1000);
+      for (int32_t nm = 0; nm < np; ++nm) {
+        const float floorz = M->subgrid ? M->z_zmin[nm] : M->zb[nm];
+        M->qinffield[nm] = floorz > p.qinf_zmin ? qinf : 0.0f;
+      }
+    } else {
+      for (int32_t nm = 0; nm < np; ++nm) M->qinffield[nm] = M->qinf_map_mmh[nm] / 3600.0f / 1000.0f;
+    }
+  }
   M->initialized = true;
   return SFX_OK;
 }
@@ -1351,6 +1393,29 @@ template <typename T> View V(const std::vector<T>& v) { return View{v.data(), v.
 
 bool lookup(const sfx_model* M, const std::string& k, View& out) {
   if (k == "params") { out = View{&M->p, sizeof(sfx_params)}; return true; }
+  if (k == "cosrot") { out = View{&M->cosrot, sizeof(float)}; return true; }
+  if (k == "sinrot") { out = View{&M->sinrot, sizeof(float)}; return true; }
+  // the meteo inputs as set or read: <kind>_times, <kind>_values and <kind>_geo (x_llcorner,
+  // y_llcorner, dx, dy) for amu, amv, amp, ampr; spw_times, spw_xe, spw_ye, spw_vmag,
+  // spw_vdir, spw_pdrop, spw_prcp, spw_radius
+  {
+    const sfx_model::MeteoGrid* g[4] = {&M->amu, &M->amv, &M->amp, &M->ampr};
+    const char* nm[4] = {"amu", "amv", "amp", "ampr"};
+    for (int i = 0; i < 4; ++i) {
+      const std::string b(nm[i]);
+      if (k == b + "_times") { out = V(g[i]->times); return true; }
+      if (k == b + "_values") { out = V(g[i]->v); return true; }
+      if (k == b + "_geo") { out = View{&g[i]->xll, 4 * sizeof(float)}; return true; }
+    }
+    if (k == "spw_times") { out = V(M->spw.times); return true; }
+    if (k == "spw_xe") { out = V(M->spw.xe); return true; }
+    if (k == "spw_ye") { out = V(M->spw.ye); return true; }
+    if (k == "spw_vmag") { out = V(M->spw.vmag); return true; }
+    if (k == "spw_vdir") { out = V(M->spw.vdir); return true; }
+    if (k == "spw_pdrop") { out = V(M->spw.pdrop); return true; }
+    if (k == "spw_prcp") { out = V(M->spw.prcp); return true; }
+    if (k == "spw_radius") { out = View{&M->spw.radius, sizeof(float)}; return true; }
+  }
 #define T(name) if (k == #name) { out = V(M->name); return true; }
   T(kcs) T(z_flags_iref) T(z_index_z_n) T(z_index_z_m)
   T(z_index_uv_md) T(z_index_uv_mu) T(z_index_uv_nd) T(z_index_uv_nu)
@@ -1361,6 +1426,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(uv_index_v_ndm) T(uv_index_v_nm) T(uv_index_v_ndmu) T(uv_index_v_nmu)
   T(uv_flags_iref) T(uv_flags_type) T(uv_flags_dir) T(kcuv) T(mask_adv)
   T(thindam_uv_index) T(structure_uv_index) T(structure_height) T(structure_cd) T(structure_length)
+  T(qinffield) T(meteo_times) T(meteo_tauwu) T(meteo_tauwv) T(meteo_patm) T(meteo_prcp)
   T(cuv_index_uv) T(cuv_index_uv1) T(cuv_index_uv2) T(zbuv) T(zbuvmx) T(gn2uv)
   T(dxr) T(dyr) T(dxrm) T(dyrm) T(dxrinv) T(dyrinv) T(dxr2inv) T(dyr2inv) T(dxrinvc) T(dyrinvc) T(cell_area) T(nuvisc)
   T(z_zmin) T(z_zmax) T(z_volmax) T(z_dep) T(uv_zmin) T(uv_zmax) T(uv_navg_w) T(uv_fnfit)

@@ -20,7 +20,11 @@ enum : uint32_t {
   F_STORE = F_VELMAX | F_FLUXMAX | F_TWET,
   F_TSA = 1u << 17,  // timestep_analysis (sfincs_timestep_analysis.f90)
   F_QEXT = 1u << 18, // external sources per cell (use_qext, BMI coupling)
-  F_STORAGE = 1u << 19  // storage volume per subgrid cell (use_storage_volume)
+  F_STORAGE = 1u << 19, // storage volume per subgrid cell (use_storage_volume)
+  F_INFIL = 1u << 20,   // constant infiltration per cell, uniform or mapped (sfincs_infiltration.f90)
+  // gridded / spiderweb meteo (sfincs_meteo.f90): update windows, wind stress per cell,
+  // atmospheric pressure (baro), rain per cell, spin-up of the gridded fields
+  F_METEO3D = 1u << 21, F_MWIND = 1u << 22, F_PATM = 1u << 23, F_MPRCP = 1u << 24, F_MSPIN = 1u << 25
 };
 
 constexpr int MAXREF = 8;
@@ -70,6 +74,7 @@ struct Consts {
   float g, huthresh, huvmin, advlim, slopelim, uvlim, nuviscfac, fcorio, theta;
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
   float t0, t1, tspinup, zini, dx, dy, pi, twet_threshold;
+  float dtwnd, gapres, pavbnd;   // meteo update interval, background pressure, boundary pressure
   float dxrinv[MAXREF], dyrinv[MAXREF], dxr2inv[MAXREF], dyr2inv[MAXREF], dxrinvc[MAXREF], dyrinvc[MAXREF];
   float cell_area[MAXREF], dxrm[MAXREF], dyrm[MAXREF], nuvisc[MAXREF];
   uint32_t flags;
@@ -116,6 +121,7 @@ struct UvIn {
   int kcuv_nmd, kcuv_nmu;      // theta smoothing test
   int kfuv, mask_adv, idir, iref, itype;
   R tauwu, tauwv;              // wind stress at cell nm
+  R patm_nm, patm_nmu;         // atmospheric pressure of the two cells (F_PATM)
 };
 
 struct UvTables {              // per-point tables (regular or subgrid)
@@ -263,6 +269,7 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
     if (hwet > R(0.25f)) frc = frc + phi * tw;
     else frc = frc + tw * hu * R(4);
   }
+  if (F & F_PATM) frc = frc + hu * (in.patm_nm - in.patm_nmu) * dxuvinv / (R)c.rhow;   // :585-589
   R qfr;
   if (in.kfuv == 0) {
     qfr = rsqrt_<R>(rabs<R>(dzdx) / (vmax<R>(gnavg2, R(1.0e-5f)) / R(10))) * pow_third<R>(hu, 1, pw);

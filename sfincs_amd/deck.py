@@ -44,6 +44,67 @@ def _write_subgrid(sc, folder):
     shutil.copyfile(sg, os.path.join(folder, "sfincs.sbg"))
 
 
+_QUANT = {"amu": ("x_wind", "m s-1"), "amv": ("y_wind", "m s-1"), "amp": ("air_pressure", "Pa"),
+          "ampr": ("precipitation", "mm/hr")}
+
+
+def _rows(f, a):
+    for row in a:
+        f.write(" ".join(_fmt(v) for v in row) + "\n")
+
+
+def _write_amuv(path, kind, g):
+    """An equidistant-grid meteo file (read_amuv_file, sfincs_spiderweb.f90:135-229): header
+    keys, then per time a TIME line -- here in minutes since 12:00 on the day before tref, so
+    that the reader's date offset is exercised -- and the rows from the north."""
+    v = np.asarray(g["values"], np.float32)
+    nt, nr, nc = v.shape
+    q, u = _QUANT[kind]
+    with open(path, "w") as f:
+        f.write(f"### {kind} file written by sfincs_amd.deck\n")
+        for k, val in (("FileVersion", "1.03"), ("filetype", "meteo_on_equidistant_grid"), ("NODATA_value", "-999"),
+                       ("n_cols", str(nc)), ("n_rows", str(nr)), ("grid_unit", "m"),
+                       ("x_llcorner", _fmt(g["x_llcorner"])), ("y_llcorner", _fmt(g["y_llcorner"])),
+                       ("dx", _fmt(g["dx"])), ("dy", _fmt(g["dy"])), ("n_quantity", "1"), ("quantity1", q),
+                       ("unit1", u)):
+            f.write(f"{k:<17s}=   {val}\n")
+        for it in range(nt):
+            f.write(f"TIME = {_fmt((float(g['times'][it]) + 43200.0) / 60.0)} minutes since "
+                    f"1999-12-31 12:00:00 +00:00\n")
+            _rows(f, v[it])
+
+
+def _write_spw(path, kind, g):
+    """A spiderweb file (read_spw_file, sfincs_spiderweb.f90:8-133): header, then per time a
+    TIME line (hours since tref), the eye (its values within the 12 characters after '='),
+    and the rows of speed, direction, pressure drop and, with rain, precipitation."""
+    vm = np.asarray(g["vmag"], np.float32)
+    nt, nr, nc = vm.shape
+    rain = g.get("rain")
+    nq = 4 if rain is not None else 3
+    with open(path, "w") as f:
+        f.write("### spiderweb written by sfincs_amd.deck\n")
+        for k, val in (("FileVersion", "1.03"), ("filetype", "meteo_on_spiderweb_grid"), ("NODATA_value", "-999"),
+                       ("n_cols", str(nc)), ("n_rows", str(nr)), ("grid_unit", "m"), ("spw_radius", _fmt(g["radius"])),
+                       ("spw_rad_unit", "m"), ("n_quantity", str(nq)), ("quantity1", "wind_speed"),
+                       ("quantity2", "wind_from_direction"), ("quantity3", "p_drop")) + \
+                ((("quantity4", "precipitation"),) if nq == 4 else ()):
+            f.write(f"{k:<17s}=   {val}\n")
+        for it in range(nt):
+            f.write(f"TIME = {_fmt(float(g['times'][it]) / 3600.0)} hours since 2000-01-01 00:00:00 +00:00\n")
+            for key, val in (("x_spw_eye", g["xe"][it]), ("y_spw_eye", g["ye"][it])):
+                txt = _fmt(val)
+                if len(txt) > 11:
+                    raise ValueError(f"deck.write: spiderweb eye {key} = {txt} needs more than 11 characters")
+                f.write(f"{key:<17s}= {txt}\n")
+            f.write(f"{'p_drop_spw_eye':<17s}= 0\n")
+            _rows(f, vm[it])
+            _rows(f, np.asarray(g["vdir"], np.float32)[it])
+            _rows(f, np.asarray(g["pdrop"], np.float32)[it])
+            if nq == 4:
+                _rows(f, np.asarray(rain, np.float32)[it])
+
+
 def write(sc, folder):
     os.makedirs(folder, exist_ok=True)
     p = dict(sc.params)
@@ -69,6 +130,10 @@ def write(sc, folder):
             lines.append(f"timestep_analysis = {int(v)}")
         elif k == "structure_relax":
             lines.append(f"structure_relax = {_fmt(v)}")
+        elif k == "spw_merge_frac":
+            lines.append(f"spwmergefrac = {_fmt(v)}")
+        elif k in ("baro", "spinup_meteo", "amprblock", "usespwprecip"):
+            lines.append(f"{k} = {int(v)}")
         else:
             lines.append(f"{k} = {_fmt(v)}")
     if sc.subgrid is not None:
@@ -153,6 +218,19 @@ def write(sc, folder):
                 f.write(f"line{k + 1:04d}\n{pl.shape[0]} {pl.shape[1]}\n")
                 for row in pl:
                     f.write(" ".join(_fmt(v) for v in row) + "\n")
+    for kind, g in (getattr(sc, "meteo", None) or {}).items():   # ASCII meteo files (sfincs_spiderweb.f90)
+        fname = f"sfincs.{kind}"
+        lines.append(f"{kind}file = {fname}")
+        (_write_spw if kind == "spw" else _write_amuv)(os.path.join(folder, fname), kind, g)
+    if getattr(sc, "qinf_map", None) is not None:   # float32 mm/h per active cell (sfincs_infiltration.f90:245-249)
+        from . import _lib
+        p = _lib.default_params(**sc.params)
+        if sc.kind == "regular":
+            mesh = _lib.Model.regular(p, sc.nmax, sc.mmax, sc.dx, sc.dy, sc.msk)
+        else:
+            mesh = _lib.Model.quadtree(p, sc.qt)
+        lines.append("qinffile = sfincs.qinf")
+        np.ascontiguousarray(sc.qinf_map(mesh), np.float32).tofile(os.path.join(folder, "sfincs.qinf"))
     if sc.src is not None:
         x, y, t, q = sc.src
         lines += ["srcfile = sfincs.src", "disfile = sfincs.dis"]

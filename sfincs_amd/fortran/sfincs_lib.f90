@@ -21,7 +21,12 @@ module sfx_capi
       integer(c_int32_t) :: store_velmax, store_fluxmax, store_twet
       real(c_float) :: twet_threshold
       integer(c_int32_t) :: timestep_analysis, use_qext
-      real(c_float) :: structure_relax
+      real(c_float) :: structure_relax, qinf, qinf_zmin
+      real(c_float) :: dtwnd
+      integer(c_int32_t) :: baro
+      real(c_float) :: pavbnd, gapres
+      integer(c_int32_t) :: spinup_meteo, amprblock, usespwprecip
+      real(c_float) :: spw_merge_frac, factor_wind, factor_pres, factor_prcp, factor_spw_size
    end type
    type, bind(C) :: sfx_quadtree_input
       integer(c_int32_t) :: nr_points, nr_levels
@@ -109,6 +114,10 @@ module sfx_capi
          integer(c_int32_t), value :: npoly
          type(c_ptr), value :: nv, x, y
       end function
+      integer(c_int) function sfx_model_set_infiltration(m, q) bind(C, name='sfx_model_set_infiltration')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m, q
+      end function
       integer(c_int) function sfx_model_set_weirs(m, npoly, nv, x, y, par) bind(C, name='sfx_model_set_weirs')
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: m
@@ -126,6 +135,17 @@ module sfx_capi
          type(c_ptr), value :: m
          integer(c_int32_t), value :: nt
          type(c_ptr), value :: t, mag, dir
+      end function
+      integer(c_int) function sfx_model_find_cells(m, n, x, y, nm) bind(C, name='sfx_model_find_cells')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: n
+         type(c_ptr), value :: x, y, nm
+      end function
+      integer(c_int) function sfx_model_read_meteo_file(m, kind, path, tref) bind(C, name='sfx_model_read_meteo_file')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: kind(*), path(*), tref(*)
       end function
       integer(c_int) function sfx_model_read_subgrid_binary(m, path) bind(C, name='sfx_model_read_subgrid_binary')
          import :: c_int, c_ptr, c_char
@@ -266,6 +286,15 @@ module sfincs_lib
    logical :: out_zvolume = .false., host_subgrid = .false.
    integer :: host_store_zsmax = 0
    real*4 :: host_huthresh = 0.0
+   ! history output at observation points (read_obs_points, sfincs_obspoints.f90:5-135;
+   ! open_his_output / write_his_output, sfincs_output.f90:567-655): zst.txt, prcp.txt
+   ! (with a rain series) and qdrain.txt (drainage structures) every dthisout from t0
+   real*8 :: thisout = 1.0d9, out_dthis = 600.0d0
+   integer :: nthisout = 0, nobs = 0, his_nsrc = 0, his_ndrn = 0
+   integer(c_int32_t), allocatable, target :: obs_nm(:)
+   real(c_float), allocatable, target :: obs_x(:), obs_y(:)
+   real*4, allocatable :: his_tp(:), his_vp(:)
+   logical :: his_qdrain = .false.
 
 contains
 
@@ -464,7 +493,7 @@ contains
    ! ---- sfincs_initialize (sfincs_lib.f90:85-335) --------------------------------------
    ! Reads the input deck into an initialized host model, then moves it to HBM.
    function sfincs_initialize() result(ierr)
-      integer :: ierr
+      integer :: ierr, i
       integer(c_int) :: rc
       ierr = sfincs_read_model()
       if (ierr /= 0) return
@@ -490,6 +519,9 @@ contains
       endif
       t = host_t0
       dt = 1.0e-6
+      call table_size('nmindsrc', 4, i)
+      call table_size('drainage_type', 4, his_ndrn)
+      his_nsrc = i - 2 * his_ndrn
       call output_init()
       call logmsg('---------- Starting simulation -----------')
       ierr = 0
@@ -503,7 +535,11 @@ contains
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout, tdum
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
       character(256) :: bndfile, bzsfile, bzifile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
-      character(256) :: sbgfile, drnfile, thdfile, weirfile
+      character(256) :: sbgfile, drnfile, thdfile, weirfile, qinffile, obsfile
+      character(256) :: metfile(5)
+      character(4), parameter :: metkind(5) = ['amu ', 'amv ', 'amp ', 'ampr', 'spw ']
+      character(8), parameter :: metkey(5) = ['amufile ', 'amvfile ', 'ampfile ', 'amprfile', 'spwfile ']
+      real(c_float) :: rdum
       integer(c_int32_t), allocatable, target :: pl_nv(:)
       real(c_float), allocatable, target :: pl_x(:), pl_y(:), pl_p(:)
       logical :: pl_ok
@@ -607,6 +643,58 @@ contains
       call inp_str('drnfile', drnfile, 'none')
       call inp_str('thdfile', thdfile, 'none'); call inp_str('weirfile', weirfile, 'none')
       call inp_real('structure_relax', p%structure_relax)
+      ! constant infiltration (sfincs_input.f90:78,100,248): qinf in mm/h, converted by the model
+      call inp_real('qinf', p%qinf); call inp_real('qinf_zmin', p%qinf_zmin)
+      call inp_str('qinffile', qinffile, 'none')
+      call inp_str('infiltrationfile', s, 'none')
+      if (s(1:4) /= 'none') then
+         call logmsg('Error   : '//trim(s)//' is a NetCDF infiltration file; this host reads qinf and qinffile only')
+         ierr = -1
+         return
+      endif
+      ! history output (sfincs_input.f90:67, 274, 289)
+      tdum = 600.0
+      call inp_real('dthisout', tdum)
+      out_dthis = tdum
+      call inp_str('obsfile', obsfile, 'none')
+      i = 1; call inp_int('storeqdrain', i); his_qdrain = i /= 0
+      ! gridded and spiderweb meteo (sfincs_input.f90:69,92-113,200-203,236-247): the ASCII files
+      call inp_real('dtwnd', p%dtwnd); i = p%baro; call inp_int('baro', i); p%baro = i
+      call inp_real('pavbnd', p%pavbnd); call inp_real('gapres', p%gapres)
+      i = 0; call inp_int('spinup_meteo', i); p%spinup_meteo = merge(1, 0, i /= 0)
+      i = 1; call inp_int('amprblock', i); p%amprblock = merge(1, 0, i /= 0)
+      i = 1; call inp_int('usespwprecip', i); p%usespwprecip = merge(1, 0, i /= 0)
+      call inp_real('spwmergefrac', p%spw_merge_frac)
+      call inp_real('factor_wind', p%factor_wind); call inp_real('factor_pres', p%factor_pres)
+      call inp_real('factor_prcp', p%factor_prcp); call inp_real('factor_spw_size', p%factor_spw_size)
+      do j = 1, 5
+         call inp_str(trim(metkey(j)), metfile(j), 'none')
+      enddo
+      call inp_str('netamuamvfile', s, 'none')
+      if (s(1:4) == 'none') call inp_str('netampfile', s, 'none')
+      if (s(1:4) == 'none') call inp_str('netamprfile', s, 'none')
+      if (s(1:4) == 'none') call inp_str('netspwfile', s, 'none')
+      if (s(1:4) /= 'none') then
+         call logmsg('Error   : '//trim(s)//' is NetCDF meteo input; this host reads the ASCII amu/amv/amp/ampr '// &
+                     'and spw files')
+         ierr = -1
+         return
+      endif
+      call inp_str('z0lfile', s, 'none')
+      i = 0; call inp_int('crsgeo', i)
+      rdum = -999.0; call inp_real('waveage', rdum)
+      call inp_str('utmzone', line, 'nil')
+      if (s(1:4) /= 'none' .or. i /= 0 .or. rdum > 0.0 .or. line(1:3) /= 'nil') then
+         call logmsg('Error   : z0lfile, crsgeo, waveage and utmzone are not supported by this host')
+         ierr = -1
+         return
+      endif
+      if (qinffile(1:4) /= 'none' .and. qtrfile(1:4) /= 'none') then   ! sfincs_infiltration.f90:181-185
+         call logmsg('Error ! Infiltration input for quadtree mesh model can only be specified using the '// &
+                     'infiltrationfile Netcdf format! !')
+         ierr = -1
+         return
+      endif
 
       if (qtrfile(1:4) /= 'none') then
          call read_quadtree(qtrfile, mskfile, p, sbgfile(1:4) == 'none', rc)
@@ -797,6 +885,69 @@ contains
          endif
          deallocate(tt, v1, drn_type, drn_par)
       endif
+      ! gridded meteo files, read after the mesh (read_meteo_data, sfincs_meteo.f90:5-245)
+      do j = 1, 5
+         if (metfile(j)(1:4) /= 'none') then
+            rc = sfx_model_read_meteo_file(model, trim(metkind(j))//c_null_char, trim(metfile(j))//c_null_char, &
+                                           trim(tref)//c_null_char)
+            if (rc < 0) then
+               call fail(trim(metkey(j)))
+               ierr = -1
+               return
+            endif
+         endif
+      enddo
+      ! observation points: x y ['name'] per line (read_obs_points, sfincs_obspoints.f90:24-81)
+      nobs = 0
+      if (allocated(obs_nm)) deallocate(obs_nm, obs_x, obs_y)
+      if (allocated(his_tp)) deallocate(his_tp, his_vp)
+      if (obsfile(1:4) /= 'none') then
+         open(newunit=u, file=trim(obsfile), status='old', action='read', iostat=ios)
+         if (ios /= 0) then
+            call logmsg('Error   : cannot read observation points file '//trim(obsfile))
+            ierr = -1
+            return
+         endif
+         do
+            read(u, *, iostat=ios) rdum
+            if (ios < 0) exit
+            nobs = nobs + 1
+         enddo
+         rewind(u)
+         allocate(obs_nm(nobs), obs_x(nobs), obs_y(nobs))
+         do i = 1, nobs
+            read(u, '(a)') line
+            read(line, *) obs_x(i), obs_y(i)
+         enddo
+         close(u)
+         rc = sfx_model_find_cells(model, int(nobs, c_int32_t), c_loc(obs_x), c_loc(obs_y), c_loc(obs_nm))
+         do i = 1, nobs
+            if (obs_nm(i) == 0) then
+               write(line, '(a,i0,a)') 'Warning ! Observation point ', i, ' falls outside model domain !'
+               call logmsg(trim(line))
+            endif
+         enddo
+      endif
+      ! spatially varying constant infiltration: float32 mm/h per active cell (sfincs_infiltration.f90:241-249)
+      if (qinffile(1:4) /= 'none') then
+         if (allocated(tmp)) deallocate(tmp)
+         allocate(tmp(host_np))
+         open(newunit=u, file=trim(qinffile), form='unformatted', access='stream', status='old', iostat=ios)
+         if (ios == 0) read(u, iostat=ios) tmp
+         if (ios /= 0) then
+            call logmsg('Error   : cannot read infiltration file '//trim(qinffile))
+            ierr = -1
+            return
+         endif
+         close(u)
+         rc = sfx_model_set_infiltration(model, c_loc(tmp))
+         if (rc < 0) then
+            call fail('infiltration')
+            ierr = -1
+            return
+         endif
+         deallocate(tmp)
+      endif
       ! thin dams and weirs (read_structures, sfincs_structures.f90:8-35): polylines
       if (thdfile(1:4) /= 'none') then
          call read_polylines(thdfile, 0, pl_nv, pl_x, pl_y, pl_p, pl_ok)
@@ -860,6 +1011,10 @@ contains
          close(u)
          rc = sfx_model_set_rain(model, int(nt, c_int32_t), c_loc(tt), c_loc(v1))
          if (rc < 0) call fail('rain')
+         if (allocated(his_tp)) deallocate(his_tp, his_vp)
+         allocate(his_tp(nt), his_vp(nt))
+         his_tp = tt
+         his_vp = v1
          deallocate(tt, v1)
       endif
       if (wndfile(1:4) /= 'none') then
@@ -1053,7 +1208,7 @@ contains
          if (dtrange > 0.0d0) tend = t + dtrange
          rc = 0
          do while (t < tend)
-            tnext = min(tend, tmapout, tmaxout, trstout)
+            tnext = min(tend, tmapout, tmaxout, trstout, thisout)
             if (tnext <= t) then
                rc = sfx_update(handle, 0.0d0, st)      ! an output is due: one step
             else
@@ -1221,6 +1376,15 @@ contains
       else
          tmaxout = 1.0d9
       endif
+      nthisout = 0
+      if (out_dthis > 1.0d-6 .and. nobs > 0) then   ! initialize_output (sfincs_output.f90:59-76)
+         thisout = host_t0
+         call his_delete('zst.txt')
+         call his_delete('prcp.txt')
+         call his_delete('qdrain.txt')
+      else
+         thisout = 1.0d9
+      endif
       if (out_dtrst > 1.0d-6) then
          trstout = out_t0 + out_dtrst
       else if (out_trst > 1.0d-6) then
@@ -1246,16 +1410,17 @@ contains
    subroutine after_steps(st, rc)
       type(sfx_step_stats), intent(in) :: st
       integer(c_int), intent(in) :: rc
-      logical :: wmap, wmax, wrst
+      logical :: wmap, wmax, wrst, whis
       real*8 :: tout
       t = st%t
       dt = st%dt
       if (rc == 1 .or. st%error /= 0) then
          ntmaxout = ntmaxout + 1
          call write_output(t, u_zs /= 0, .true., .false.)
+         if (thisout < 1.0d8) call write_his(t)
          return
       endif
-      wmap = .false.; wmax = .false.; wrst = .false.
+      wmap = .false.; wmax = .false.; wrst = .false.; whis = .false.
       tout = t
       if (t >= tmapout) then
          wmap = .true.
@@ -1278,7 +1443,70 @@ contains
             trstout = 1.0d9
          endif
       endif
+      if (t >= thisout) then
+         whis = .true.
+         nthisout = nthisout + 1
+         tout = max(thisout, t - dt)
+         thisout = thisout + out_dthis
+      endif
       if (wmap .or. wmax .or. wrst) call write_output(tout, wmap, wmax, wrst)
+      if (whis) call write_his(tout)
+   end subroutine
+
+   subroutine his_delete(name)
+      character(*), intent(in) :: name
+      integer :: u, ios
+      open(newunit=u, file=name, iostat=ios)
+      if (ios == 0) close(u, status='delete')
+   end subroutine
+
+   ! write_his_output (sfincs_output.f90:593-655), with the binary-output reference's text
+   ! files: the level at each observation point (-999 outside the mesh), the rain of the
+   ! step's series there in mm/h, the drainage structures' intake discharges
+   subroutine write_his(tout)
+      real*8, intent(in) :: tout
+      real*4, allocatable :: zobs(:), tprcp(:)
+      real(c_float), allocatable, target :: qs(:)
+      real*4 :: twfac, ptmp
+      integer :: i, u, itp, nt
+      integer(c_int) :: r
+      if (nobs == 0) return
+      call host_sync()
+      allocate(zobs(nobs), tprcp(nobs))
+      ptmp = 0.0
+      if (allocated(his_tp)) then   ! update_precipitation_from_timeseries at the step's time
+         nt = size(his_tp)
+         do itp = 1, nt
+            if (his_tp(itp) > t) exit
+         enddo
+         itp = max(min(itp, nt), 2)
+         twfac = real((t - dble(his_tp(itp - 1))) / dble(his_tp(itp) - his_tp(itp - 1)), 4)
+         ptmp = (his_vp(itp - 1) * (1.0 - twfac) + his_vp(itp) * twfac) / (1000 * 3600)
+      endif
+      do i = 1, nobs
+         if (obs_nm(i) > 0) then
+            zobs(i) = real(host_zs(obs_nm(i)), 4)
+            tprcp(i) = ptmp * 1000 * 3600
+         else
+            zobs(i) = -999.0
+            tprcp(i) = -999.0
+         endif
+      enddo
+      open(newunit=u, file='zst.txt', position='append', status='unknown')
+      write(u, '(f12.1,10000f9.3)') tout, zobs
+      close(u)
+      if (allocated(his_tp)) then
+         open(newunit=u, file='prcp.txt', position='append', status='unknown')
+         write(u, '(f12.1,10000f9.3)') tout, tprcp
+         close(u)
+      endif
+      if (his_ndrn > 0 .and. his_qdrain) then
+         allocate(qs(his_nsrc + 2 * his_ndrn))
+         r = sfx_get_state(handle, 'qtsrc'//c_null_char, c_loc(qs), int(4 * size(qs), c_size_t))
+         open(newunit=u, file='qdrain.txt', position='append', status='unknown')
+         write(u, '(f12.1,10000f9.3)') tout, (qs(i), i = his_nsrc + 1, his_nsrc + 2 * his_ndrn, 2)
+         close(u)
+      endif
    end subroutine
 
    ! write_output (sfincs_output.f90:80-258), binary flavour: the fields come from the

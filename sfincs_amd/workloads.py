@@ -45,11 +45,14 @@ class Scenario:
     drn: tuple = None                     # drainage structures (x_in, y_in, x_out, y_out, type, params[n, 6])
     thd: list = None                      # thin dams: polylines, each (n, 2) x, y
     weirs: list = None                    # weirs: polylines, each (n, 4) x, y, crest level, Cd
+    qinf_map: object = None               # callable(model) -> infiltration per cell (mm/h), or None
+    meteo: dict = None                    # gridded meteo: "amu", "amv", "amp", "ampr" grids, "spw" spiderweb
+    rotation: float = 0.0                 # regular mesh rotation (degrees)
 
     def build_model(self):
         p = _lib.default_params(**self.params)
         if self.kind == "regular":
-            m = _lib.Model.regular(p, self.nmax, self.mmax, self.dx, self.dy, self.msk)
+            m = _lib.Model.regular(p, self.nmax, self.mmax, self.dx, self.dy, self.msk, rotation=self.rotation)
         else:
             m = _lib.Model.quadtree(p, self.qt)
         if self.subgrid is not None:
@@ -82,6 +85,13 @@ class Scenario:
             m.set_weirs(self.weirs)
         if self.storage is not None:
             m.set_storage_volume(self.storage(m))
+        if self.qinf_map is not None:
+            m.set_infiltration(self.qinf_map(m))
+        for kind, g in (self.meteo or {}).items():
+            if kind == "spw":
+                m.set_spiderweb(**g)
+            else:
+                m.set_meteo_grid(kind, **g)
         if self.rstfile is not None:
             m.set_restart(self.rstfile)
         m.initialize()
@@ -430,6 +440,98 @@ def with_weirs(sc, polylines):
     """Adds weirs (weirfile): polylines of (x, y, crest level, Cd) vertices."""
     sc.weirs = [np.asarray(p, np.float32).reshape(-1, 4) for p in polylines]
     sc.name += "_weir"
+    return sc
+
+
+def with_infiltration(sc, qinf=None, qinf_zmin=None, qinf_map=None):
+    """Adds constant infiltration: uniform `qinf` mm/h on cells whose bed lies above
+    `qinf_zmin` (sfincs.inp qinf, qinf_zmin), or a map (qinffile): `qinf_map(model)` gives mm/h
+    per active cell. It acts only where the scenario has rain."""
+    if qinf is not None:
+        sc.params = dict(sc.params, qinf=float(qinf))
+    if qinf_zmin is not None:
+        sc.params = dict(sc.params, qinf_zmin=float(qinf_zmin))
+    sc.qinf_map = qinf_map
+    sc.name += "_inf"
+    return sc
+
+
+def domain_extent(sc):
+    """(x0, y0, x1, y1) of a scenario's mesh (regular: from the grid; quadtree: level 1)."""
+    if sc.kind == "regular":
+        return 0.0, 0.0, sc.mmax * sc.dx, sc.nmax * sc.dy
+    q = sc.qt
+    f = 2.0 ** (q["level"].astype(np.float64) - 1)
+    return (float(q["x0"]), float(q["y0"]), float(q["x0"] + np.max(q["m"] * q["dx"] / f)),
+            float(q["y0"] + np.max(q["n"] * q["dy"] / f)))
+
+
+def with_gridded_meteo(sc, nx=14, ny=12, t_step=1800.0, nt=None, wind=True, pressure=True, rain=True,
+                       dp=2500.0, u0=8.0, v0=-3.0, rain_mmh=40.0, seed=11):
+    """Gridded meteo on an equidistant grid one cell wider than the domain on every side (amu,
+    amv, amp and ampr contents): a pressure low of depth dp crossing the domain from south-west
+    to north-east, with a wind circulating around it over a uniform (u0, v0) and rain near its
+    centre, at times 0, t_step, ... covering the run."""
+    x0, y0, x1, y1 = domain_extent(sc)
+    gdx, gdy = (x1 - x0) / (nx - 3), (y1 - y0) / (ny - 3)
+    xll, yll = x0 - 1.5 * gdx, y0 - 1.5 * gdy
+    nt = nt or int(np.ceil(sc.t_end / t_step)) + 2
+    t = (np.arange(nt) * t_step).astype(np.float32)
+    xc = xll + (np.arange(nx) + 0.5) * gdx
+    yc = (yll + (np.arange(ny) + 0.5) * gdy)[::-1]          # row 0 northernmost
+    X, Y = np.meshgrid(xc, yc)
+    L = 0.35 * max(x1 - x0, y1 - y0)
+    r = rng(seed)
+    jitter = r.uniform(-0.5, 0.5, size=(nt, ny, nx))
+    P, U, V, R = (np.zeros((nt, ny, nx), np.float32) for _ in range(4))
+    for k in range(nt):
+        f = k / max(nt - 1, 1)
+        cx, cy = x0 + (x1 - x0) * (0.1 + 0.8 * f), y0 + (y1 - y0) * (0.2 + 0.6 * f)
+        g = np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / L ** 2)
+        P[k] = 101200.0 - dp * g + 20.0 * jitter[k]
+        U[k] = u0 - 15.0 * g * (Y - cy) / L + jitter[k]
+        V[k] = v0 + 15.0 * g * (X - cx) / L - jitter[k]
+        R[k] = rain_mmh * g
+    geo = dict(x_llcorner=float(xll), y_llcorner=float(yll), dx=float(gdx), dy=float(gdy))
+    sc.meteo = dict(sc.meteo or {})
+    if wind:
+        sc.meteo["amu"] = dict(times=t, values=U, **geo)
+        sc.meteo["amv"] = dict(times=t, values=V, **geo)
+    if pressure:
+        sc.meteo["amp"] = dict(times=t, values=P, **geo)
+    if rain:
+        sc.meteo["ampr"] = dict(times=t, values=R, **geo)
+    sc.name += "_met"
+    return sc
+
+
+def with_spiderweb(sc, nrows=16, ncols=24, radius_frac=0.8, vmax=35.0, rmax_frac=0.2, dp=4000.0, rain_mmh=60.0,
+                   t_step=1800.0, nt=None, t_first=0.0):
+    """A spiderweb (spw contents): a tropical cyclone whose eye crosses the domain from west
+    to east, with a Rankine-like wind speed profile peaking vmax at rmax, the wind turning
+    anticlockwise round the eye with 20 degrees of inflow, a pressure drop dp exp(-rmax / r)
+    and rain rain_mmh exp(-r / (2 rmax)); times t_first, t_first + t_step, ..."""
+    x0, y0, x1, y1 = domain_extent(sc)
+    ext = max(x1 - x0, y1 - y0)
+    radius = radius_frac * ext
+    rmax = rmax_frac * ext
+    nt = nt or int(np.ceil(sc.t_end / t_step)) + 2
+    t = (t_first + np.arange(nt) * t_step).astype(np.float32)
+    f = np.arange(nt) / max(nt - 1, 1)
+    xe = np.round(x0 - 0.2 * (x1 - x0) + 1.4 * (x1 - x0) * f, 1).astype(np.float32)
+    ye = np.round(y0 + (y1 - y0) * (0.35 + 0.3 * f), 1).astype(np.float32)
+    rr = (np.arange(1, nrows + 1) * radius / nrows)[:, None]              # row i at radius i * dr
+    bearing = (np.arange(ncols) * 360.0 / ncols)[None, :]
+    speed = np.where(rr < rmax, vmax * rr / rmax, vmax * (rmax / rr) ** 0.6) + 0.0 * bearing
+    vdir = np.mod(bearing + 90.0 + 20.0, 360.0) + 0.0 * rr
+    pdrop = dp * np.exp(-rmax / rr) + 0.0 * bearing
+    rain = rain_mmh * np.exp(-rr / (2 * rmax)) + 0.0 * bearing
+    rep = lambda a: np.repeat(a[None].astype(np.float32), nt, axis=0)
+    decay = np.linspace(1.0, 0.8, nt, dtype=np.float32)[:, None, None]
+    sc.meteo = dict(sc.meteo or {})
+    sc.meteo["spw"] = dict(times=t, xe=xe, ye=ye, radius=float(radius), vmag=rep(speed) * decay, vdir=rep(vdir),
+                           pdrop=rep(pdrop) * decay, rain=rep(rain))
+    sc.name += "_spw"
     return sc
 
 

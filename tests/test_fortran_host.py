@@ -83,7 +83,7 @@ SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax",
 
 @needs_host
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves",
-                                  "subgrid_binary", "drainage", "structures"])
+                                  "subgrid_binary", "drainage", "structures", "infiltration", "meteo"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
@@ -108,6 +108,14 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     elif kind == "structures":
         sc = W.beach_structures(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5),
                                 crest=(-0.3, 0.4))
+    elif kind == "infiltration":     # qinffile map, with qinf and qinf_zmin keys that it overrides
+        sc = W.with_infiltration(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0), qinf_zmin=-3.0,
+                                 qinf_map=lambda m: (m.table("z_xz") / 100.0).astype(np.float32))
+    elif kind == "meteo":            # amu/amv/amp/ampr and spw files, meteo keys
+        sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, dtwnd=450.0, pavbnd=101000.0, spinup_meteo=1,
+                            tspinup=300.0, factor_pres=1.5, spw_merge_frac=0.7)
+        sc.rain = None
+        W.with_spiderweb(W.with_gridded_meteo(sc, t_step=600.0))
     else:
         sc = W.subgrid_case(20, 24, dx=25.0, seed=43, t_end=900.0, nsrc=2, rain_mmh=10.0)
         sc.subgrid = None
@@ -133,6 +141,14 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     if kind == "structures":
         extra = ("thindam_uv_index", "structure_uv_index", "structure_height", "structure_cd", "structure_length")
         assert mp.sizes()["nstruc"] > 10 and mp.sizes()["nthd"] > 10
+    if kind == "infiltration":
+        extra = ("qinffield",)
+        assert np.count_nonzero(mp.table("qinffield")) > 100
+    if kind == "meteo":
+        extra = ("meteo_times", "meteo_tauwu", "meteo_tauwv", "meteo_patm", "meteo_prcp") + tuple(
+            f"{k}_{s}" for k in ("amu", "amv", "amp", "ampr") for s in ("times", "values", "geo")) + (
+            "spw_times", "spw_xe", "spw_ye", "spw_vmag", "spw_vdir", "spw_pdrop", "spw_prcp", "spw_radius")
+        assert len(mp.table("meteo_times")) >= 4 and np.ptp(mp.table("meteo_patm")) > 100.0
     for name in TABLES + extra:
         a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
         assert (a is None) == (b is None), name
@@ -143,7 +159,8 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
 
 @needs_host
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage", "structures"])
+@pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage", "structures",
+                                  "infiltration", "meteo"])
 def test_fortran_host_matches_python_path(kind, tmp_path):
     print(f"case fortran host {kind}", flush=True)
     from sfincs_amd import _lib
@@ -157,6 +174,13 @@ def test_fortran_host_matches_python_path(kind, tmp_path):
     elif kind == "structures":       # thdfile and weirfile
         sc = W.beach_structures(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5),
                                 crest=(-0.3, 0.4))
+    elif kind == "infiltration":     # qinffile
+        sc = W.with_infiltration(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, rain_mmh=30.0),
+                                 qinf_map=lambda m: (m.table("z_xz") / 40.0).astype(np.float32))
+    elif kind == "meteo":            # amu/amv/amp/ampr and spw files
+        sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, dtwnd=450.0, pavbnd=101000.0)
+        sc.rain = None
+        W.with_spiderweb(W.with_gridded_meteo(sc, t_step=600.0))
     else:
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     d = deck.write(sc, str(tmp_path))
@@ -408,3 +432,69 @@ def test_fortran_host_scheduled_output(tmp_path):
             k_max += 1
             inst.set("zsmax", np.full(m.sizes()["np"], -999.0, np.float32))
     assert np.array_equal(np.fromfile(os.path.join(d, "zs_final.dat"), np.float64), inst.get("zs"))
+
+
+def test_deck_reader_refuses_an_infiltration_map_on_a_quadtree_mesh(tmp_path):
+    """As the reference (sfincs_infiltration.f90:181-185): a quadtree mesh takes its
+    infiltration map from the NetCDF infiltrationfile only."""
+    sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
+    sc.rain = (np.array([0.0, 1.0e4], np.float32), np.array([10.0, 10.0], np.float32))
+    W.with_infiltration(sc, qinf_map=lambda m: np.ones(m.sizes()["np"], np.float32))
+    d = deck.write(sc, str(tmp_path))
+    host = ctypes.CDLL(HOSTLIB)
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        assert host.sfincs_host_read_model() != 0
+    finally:
+        os.chdir(cwd)
+
+
+@needs_host
+@pytest.mark.gpu
+def test_fortran_host_writes_history_at_observation_points(tmp_path):
+    """obsfile and dthisout (read_obs_points, sfincs_obspoints.f90:5-135; write_his_output,
+    sfincs_output.f90:593-655; trigger sfincs_lib.f90:470-477): zst.txt holds t and the
+    level at each observation point (-999 for a point outside the mesh) at the first step
+    reaching t0, t0 + dthisout, ...; prcp.txt the rain of the step's series in mm/h;
+    qdrain.txt the drainage structures' intake discharges. Equal, to the files' three
+    decimals, to the Python-driven run stopped at the same times."""
+    print("case fortran host history output", flush=True)
+    from sfincs_amd import _lib
+    sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5, rain_mmh=30.0))
+    d = deck.write(sc, str(tmp_path))
+    pts = [(610.0, 420.0), (1210.0, 1630.0), (2230.0, 1010.0), (9.0e5, 9.0e5)]
+    with open(os.path.join(d, "sfincs.obs"), "w") as f:
+        for k, (x, y) in enumerate(pts):
+            f.write(f"{x} {y} 'gauge{k}'\n")
+    with open(os.path.join(d, "sfincs.inp"), "a") as f:
+        f.write("obsfile = sfincs.obs\ndthisout = 300\n")
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    zst = np.loadtxt(os.path.join(d, "zst.txt"))
+    prcp = np.loadtxt(os.path.join(d, "prcp.txt"))
+    qdr = np.loadtxt(os.path.join(d, "qdrain.txt"))
+    assert zst.shape == (4, 1 + len(pts)) and prcp.shape == zst.shape and qdr.shape[0] == 4
+    m = sc.build_model()
+    nm = m.find_cells([p[0] for p in pts], [p[1] for p in pts])
+    assert nm[-1] == 0 and np.all(nm[:-1] > 0)
+    s = m.sizes()
+    inst = _lib.Instance(m)
+    tp, vp = (np.asarray(a, np.float32) for a in sc.rain)
+    for k in range(4):
+        if k == 0:
+            rc, st = inst.update(0.0)
+        else:
+            rc, st = inst.update_to(300.0 * k)
+        t, dt = st["t"], st["dt"]
+        assert zst[k, 0] == round(max(300.0 * k, t - dt), 1)
+        zs = inst.get("zs").astype(np.float32)
+        want = np.where(nm > 0, zs[np.maximum(nm, 1) - 1], np.float32(-999.0))
+        assert np.max(np.abs(zst[k, 1:] - want)) <= 5.1e-4, (k, zst[k], want)
+        i = int(np.searchsorted(tp, t, side="right"))
+        i = max(min(i, len(tp) - 1), 1)
+        tw = np.float32((t - float(tp[i - 1])) / float(tp[i] - tp[i - 1]))
+        rain = np.float32(np.float32(vp[i - 1] * np.float32(1 - tw) + vp[i] * tw) / np.float32(3.6e6)) * np.float32(3.6e6)
+        assert np.all(np.abs(prcp[k, 1:-1] - rain) <= 5.1e-4) and prcp[k, -1] == -999.0
+        q = inst.get("qtsrc")[s["nsrc"]::2]
+        assert np.max(np.abs(qdr[k, 1:] - q)) <= 5.1e-4, (k, qdr[k], q)

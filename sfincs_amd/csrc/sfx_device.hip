@@ -403,28 +403,37 @@ __device__ __forceinline__ MeteoNow meteo_now(const DevInst& I, SP S) {
   m.w = S->mtw; m.omw = S->momtw; m.sm = S->msm; m.spin = S->mspin;
   return m;
 }
-// update_meteo_forcing for one cell (sfincs_meteo.f90:1285-1368): a field between its two
-// frames, then the spin-up (wind stress scaled, pressure towards gapres).
-__device__ __forceinline__ float meteo_tau(const float* T, const MeteoNow& m, int c) {
-  float v = G(T)[m.a + (uint32_t)c] * m.omw + G(T)[m.b + (uint32_t)c] * m.w;
-  if (m.spin) v = v * m.sm;
-  return v;
-}
+// update_meteo_forcing's pressure for one cell (sfincs_meteo.f90:1303-1307, 1362-1364): the
+// value between its two frames, then the spin-up towards gapres.
 __device__ __forceinline__ float meteo_patm(const DevInst& I, const MeteoNow& m, int c) {
   float v = G(I.mpatm)[m.a + (uint32_t)c] * m.omw + G(I.mpatm)[m.b + (uint32_t)c] * m.w;
   if (m.spin) v = v * m.sm + I.c.gapres * (1.0f - m.sm);
   return v;
 }
 // The meteo a wet uv-point reads (sfincs_momentum.f90:549-589): the wind stress of its cell
-// nm, uniform (time series) or gridded, and the pressure of its two cells.
+// nm, uniform (time series) or gridded, and the pressure difference between its two cells
+// (real*4, as the reference forms it before scaling). Plain scalars rather than MeteoNow:
+// the per-point kernels keep them in registers (64 VGPRs, 8 waves per SIMD).
 template <typename R, typename SP>
 __device__ __forceinline__ void uv_meteo(const DevInst& I, SP S, int nm, int nmu, sfx::UvIn<R>& in) {
   const uint32_t f = I.c.flags;
-  if (!(f & (sfx::F_MWIND | sfx::F_PATM))) { in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv; return; }
-  const MeteoNow m = meteo_now(I, S);
-  if (f & sfx::F_MWIND) { in.tauwu = (R)meteo_tau(I.mtauu, m, nm); in.tauwv = (R)meteo_tau(I.mtauv, m, nm); }
-  else { in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv; }
-  if (f & sfx::F_PATM) { in.patm_nm = (R)meteo_patm(I, m, nm); in.patm_nmu = (R)meteo_patm(I, m, nmu); }
+  if (!(f & sfx::F_MWIND)) { in.tauwu = (R)S->tauwu; in.tauwv = (R)S->tauwv; }
+  if (!(f & (sfx::F_MWIND | sfx::F_PATM))) return;
+  const uint32_t a = (uint32_t)max((int)S->mwin, 0) * (uint32_t)I.np, b = a + (uint32_t)I.np;
+  const float w = S->mtw, omw = S->momtw, sm = S->msm;
+  const bool spin = S->mspin != 0;
+  if (f & sfx::F_MWIND) {
+    float tu = G(I.mtauu)[a + (uint32_t)nm] * omw + G(I.mtauu)[b + (uint32_t)nm] * w;
+    float tv = G(I.mtauv)[a + (uint32_t)nm] * omw + G(I.mtauv)[b + (uint32_t)nm] * w;
+    if (spin) { tu = tu * sm; tv = tv * sm; }
+    in.tauwu = (R)tu; in.tauwv = (R)tv;
+  }
+  if (f & sfx::F_PATM) {
+    float p0 = G(I.mpatm)[a + (uint32_t)nm] * omw + G(I.mpatm)[b + (uint32_t)nm] * w;
+    float p1 = G(I.mpatm)[a + (uint32_t)nmu] * omw + G(I.mpatm)[b + (uint32_t)nmu] * w;
+    if (spin) { p0 = p0 * sm + I.c.gapres * (1.0f - sm); p1 = p1 * sm + I.c.gapres * (1.0f - sm); }
+    in.dpatm = (R)(p0 - p1);
+  }
 }
 
 // ----------------------------------------------------------------------------- forcing
@@ -1163,8 +1172,8 @@ __device__ __forceinline__ bool point_finish(const DevInst& I, GA<Scalars> S, in
     return false;
   }
   // the rest of the stencil of a wet point
-  point_stencil<R, SUBGRID>(I, n, m, vplane, kf, q0, u0, in);
   uv_meteo<R>(I, S, c, c_nmu, in);
+  point_stencil<R, SUBGRID>(I, n, m, vplane, kf, q0, u0, in);
   int br = 0;
   wet_uv<R, SUBGRID>(I, s, in, tb, zsu, dt, q1, u1, mden, br, kf);
   cnt[br]++;
@@ -1634,7 +1643,7 @@ __device__ __forceinline__ R cell_rain(const DevInst& I, int c, int kcs, int ire
 }
 
 // Returns the fields it stored, one bit each in NSTORE order.
-template <typename R, bool SUBGRID>
+template <typename R, bool SUBGRID, bool RX>
 __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t kraw, R q_md, R q_mu, R q_nd, R q_nu,
                                               int iref, R dt, R netprcp, const MeteoNow& mt, double& inf) {
   unsigned wrote = 0;
@@ -1650,7 +1659,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
   if (SUBGRID) {
     double vol = G(I.vol)[c];
     const double vol_in = vol;   // as loaded: a source alone changes the volume too
-    netprcp = cell_rain<R>(I, c, kcs, iref, dt, netprcp, vol <= 0.0, mt, inf);
+    if (RX) netprcp = cell_rain<R>(I, c, kcs, iref, dt, netprcp, vol <= 0.0, mt, inf);
     if (kraw & KCS_SOURCE) {   // point sources of this cell, in source order (:311-325)
       double applied = 0.0;
       for (int s = 0; s < I.nsrc + 2 * I.ndrn; ++s)
@@ -1680,8 +1689,7 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
       if (!same_bits(zsd, zsd_in)) { zsdp[c] = zsd; wrote |= 8u; }
     }
   } else {
-    if (I.c.flags & (sfx::F_MPRCP | sfx::F_INFIL))
-      netprcp = cell_rain<R>(I, c, kcs, iref, dt, netprcp, zs <= (double)G(I.zb)[c], mt, inf);
+    if (RX) netprcp = cell_rain<R>(I, c, kcs, iref, dt, netprcp, zs <= (double)G(I.zb)[c], mt, inf);
     if (kraw & KCS_SOURCE) {   // (:87-106)
       const double before = zs;
       for (int s = 0; s < I.nsrc + 2 * I.ndrn; ++s)
@@ -1706,17 +1714,17 @@ __device__ __forceinline__ unsigned cont_cell(const DevInst& I, int c, uint8_t k
 // reading its byte and faces; any other cell is updated and its bit set or cleared.
 // Boundary cells, whose levels forcing sets, are never steady; a host write of cell
 // state clears every bit.
-template <typename R, bool SUBGRID>
+template <typename R, bool SUBGRID, bool RX>
 __device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md, R q_mu, R q_nd, R q_nu, int iref,
                                                 R dt, R netprcp, bool dry_sky, const MeteoNow& mt, double& inf) {
   const uint8_t kraw = G(I.kcs)[c];
   const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
                      q_nd == R(0) && q_nu == R(0) && !(I.c.flags & (sfx::F_STORE | sfx::F_QEXT)) &&
-                     (!(I.c.flags & sfx::F_INFIL) || G(I.qinf)[c] == 0.0f) &&
-                     (!(I.c.flags & sfx::F_MPRCP) ||
-                      (G(I.mprcp)[mt.a + (uint32_t)c] == 0.0f && G(I.mprcp)[mt.b + (uint32_t)c] == 0.0f));
+                     (!RX || ((!(I.c.flags & sfx::F_INFIL) || G(I.qinf)[c] == 0.0f) &&
+                              (!(I.c.flags & sfx::F_MPRCP) ||
+                               (G(I.mprcp)[mt.a + (uint32_t)c] == 0.0f && G(I.mprcp)[mt.b + (uint32_t)c] == 0.0f))));
   if (quiet && (kraw & KCS_STEADY)) return 32u;   // skipped (counted as such)
-  const unsigned w = cont_cell<R, SUBGRID>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, mt, inf);
+  const unsigned w = cont_cell<R, SUBGRID, RX>(I, c, kraw, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, mt, inf);
   const bool steady = quiet && w == 0u;
   if (steady != ((kraw & KCS_STEADY) != 0))
     G(I.kcs)[c] = (uint8_t)((kraw & ~KCS_STEADY) | (steady ? KCS_STEADY : 0));
@@ -1802,8 +1810,10 @@ __device__ void balance_block(const DevInst& I) {
 // The first `nahead` blocks compute the next step of scenario ahead_ids[b] of ahead_insts
 // (forcing_ahead_block), the next `nbal` blocks the volume balance of scenario
 // ahead_ids[b - nahead] (balance_block); the leading blocks are padded to a multiple of 8,
-// so the cells keep their XCDs.
-template <typename R>
+// so the cells keep their XCDs. RX: some scenario of the launch has infiltration or
+// gridded rain (per-cell rain and its balance slots); the instantiation without it
+// carries none of that code.
+template <typename R, bool RX>
 __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict__ insts, WorkList L, int indexed,
                                                       int nahead, int nbal, const DevInst* __restrict__ ahead_insts,
                                                       const int32_t* __restrict__ ahead_ids) {
@@ -1825,7 +1835,8 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
   const R netprcp = (R)S->netprcp;
   const bool dry_sky = !(I.c.flags & sfx::F_PRECIP) || netprcp == R(0);
   double inf = 0.0;   // uniform rain volume less what this thread's cell receives
-  const MeteoNow mt = meteo_now(I, S);
+  MeteoNow mt{};
+  if (RX) mt = meteo_now(I, S);
   if (!indexed) {
     const int nmax = I.nmax, mmax = I.mmax, np = I.np;
     const int tn = tile % I.ntn, tmi = tile / I.ntn;
@@ -1837,8 +1848,8 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     const R q_mu = m < mmax - 1 ? q[c] : R(0);
     const R q_nd = n >= 1 ? q[np + c - 1] : R(0);
     const R q_nu = n < nmax - 1 ? q[np + c] : R(0);
-    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky, mt, inf)
-                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky, mt, inf);
+    const unsigned w = I.subgrid ? cont_update<R, true, RX>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky, mt, inf)
+                                 : cont_update<R, false, RX>(I, c, q_md, q_mu, q_nd, q_nu, 1, dt, netprcp, dry_sky, mt, inf);
     if (I.count) count_stores(I, tile, w);
     if (I.c.flags & sfx::F_STORE) {
       R u_md = R(0), u_mu = R(0), u_nd = R(0), u_nu = R(0);
@@ -1858,8 +1869,8 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     if (c < I.np) {
     const R q_md = q[G(I.zmd)[c]], q_mu = q[G(I.zmu)[c]], q_nd = q[G(I.znd)[c]], q_nu = q[G(I.znu)[c]];
     const int iref = G(I.z_iref)[c];
-    const unsigned w = I.subgrid ? cont_update<R, true>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky, mt, inf)
-                                 : cont_update<R, false>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky, mt, inf);
+    const unsigned w = I.subgrid ? cont_update<R, true, RX>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky, mt, inf)
+                                 : cont_update<R, false, RX>(I, c, q_md, q_mu, q_nd, q_nu, iref, dt, netprcp, dry_sky, mt, inf);
     if (I.count) count_stores(I, tile, w);
     if (I.c.flags & sfx::F_STORE) {
       R u_md = R(0), u_mu = R(0), u_nd = R(0), u_nu = R(0);
@@ -1871,7 +1882,7 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     }
     }
   }
-  if (I.c.flags & (sfx::F_INFIL | sfx::F_MPRCP)) add_infiltrated(I, tile, inf);   // every lane of the block is here
+  if (RX && (I.c.flags & (sfx::F_INFIL | sfx::F_MPRCP))) add_infiltrated(I, tile, inf);   // every lane of the block is here
 }
 
 // FP64 volume accumulator: sum over active cells of V (subgrid) or zs*A (regular).
@@ -2901,12 +2912,15 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   const int nbal = nm;
   const int lead = (nahead + nbal + 7) / 8 * 8;   // keeps block b of the cells on XCD b mod 8
   if (what & 4) {
+    bool rx = false;   // per-cell rain (infiltration, gridded rain) in some member
+    for (sfx_handle* H : B->members) rx = rx || (H->hI.c.flags & (sfx::F_INFIL | sfx::F_MPRCP)) != 0;
+    auto kc = rx ? k_continuity<R, true> : k_continuity<R, false>;
     if (B->n_cd)
-      hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_cd + lead), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0,
+      hipLaunchKernelGGL(kc, dim3(B->n_cd + lead), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0,
                          nahead, nbal, B->dInsts, B->dIds);
     if (B->n_ci) {
       const int la = B->n_cd ? 0 : nahead, lb = B->n_cd ? 0 : nbal, ll = B->n_cd ? 0 : lead;
-      hipLaunchKernelGGL(k_continuity<R>, dim3(B->n_ci + ll), dim3(BLOCK), 0, st, base_of(B, B->wl_ci), B->wl_ci, 1,
+      hipLaunchKernelGGL(kc, dim3(B->n_ci + ll), dim3(BLOCK), 0, st, base_of(B, B->wl_ci), B->wl_ci, 1,
                          la, lb, B->dInsts, B->dIds);
     }
   }

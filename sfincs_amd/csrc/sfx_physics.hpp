@@ -121,7 +121,7 @@ struct UvIn {
   int kcuv_nmd, kcuv_nmu;      // theta smoothing test
   int kfuv, mask_adv, idir, iref, itype;
   R tauwu, tauwv;              // wind stress at cell nm
-  R patm_nm, patm_nmu;         // atmospheric pressure of the two cells (F_PATM)
+  R dpatm;                     // patm(nm) - patm(nmu), the pressure difference across the point (F_PATM)
 };
 
 struct UvTables {              // per-point tables (regular or subgrid)
@@ -269,7 +269,7 @@ __device__ __forceinline__ UvOut<R> uv_update(const UvIn<R>& in, const UvTables&
     if (hwet > R(0.25f)) frc = frc + phi * tw;
     else frc = frc + tw * hu * R(4);
   }
-  if (F & F_PATM) frc = frc + hu * (in.patm_nm - in.patm_nmu) * dxuvinv / (R)c.rhow;   // :585-589
+  if (F & F_PATM) frc = frc + hu * in.dpatm * dxuvinv / (R)c.rhow;   // :585-589
   R qfr;
   if (in.kfuv == 0) {
     qfr = rsqrt_<R>(rabs<R>(dzdx) / (vmax<R>(gnavg2, R(1.0e-5f)) / R(10))) * pow_third<R>(hu, 1, pw);

@@ -667,8 +667,8 @@ __device__ void drainage_update(const DevInst& I, int k, double t, float dt) {
     G(I.drn_frac)[k] = frac;
     G(I.drn_status)[k] = st;
     const float g = I.c.g;
-    const float h73 = sfx::libm_pow(hgate, 2, (const uint64_t*)I.pw.t13, (const uint64_t*)I.pw.t53,
-                                    (const uint64_t*)I.pw.t73, (const uint64_t*)I.pw.t15);
+    const float h73 = sfx::libm_pow_dev(hgate, 2, (const uint64_t*)I.pw.t13, (const uint64_t*)I.pw.t53,
+                                        (const uint64_t*)I.pw.t73, (const uint64_t*)I.pw.t15);
     qq = (qq0 - g * hgate * dzds * dt) / (1.0f + g * (mng * mng) * dt * fabsf(qq0) / h73);
     qq = qq * wdt * frac;
   }

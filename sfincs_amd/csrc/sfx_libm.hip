@@ -302,9 +302,45 @@ extern "C" int sfx_libm_pow_host(const float* x, int64_t n, int32_t which, float
 namespace {
 __global__ void k_libm_pow(const float* __restrict__ x, int64_t n, int which, sfx::PowTables T, float* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = which >= sfx::NPOW ? sfx::libm_trig(x[i], which - sfx::NPOW, T) : sfx::libm_pow(x[i], which, T);
+  if (i < n)
+    out[i] = which >= sfx::NPOW ? sfx::libm_trig(x[i], which - sfx::NPOW, T)
+                                : sfx::libm_pow_dev(x[i], which, (const uint64_t*)T.t13, (const uint64_t*)T.t53,
+                                                    (const uint64_t*)T.t73, (const uint64_t*)T.t15);
+}
+
+// The device estimate against pow_estimate over float bit patterns [lo, lo + n): counts
+// the inputs where they differ by more than 2^-44 (relative) and keeps the largest
+// difference (as a double's bits, for an atomic max of non-negative values).
+__global__ void k_estimate_check(uint32_t lo, uint32_t n, int which, unsigned long long* bad,
+                                 unsigned long long* worst) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float x = sfx::u2f(lo + i);
+    const double a = sfx::pow_estimate(x, which), b = sfx::pow_estimate_dev(x, which);
+    const double d = fabs(a - b) / a;
+    if (!(d <= 5.684341886080802e-14)) atomicAdd(bad, 1ull);   // 2^-44
+    atomicMax(worst, (unsigned long long)__double_as_longlong(d));
+  }
 }
 }  // namespace
+
+// Device check of the device powf estimate for every float of [2^-40, 2^14) (exponent
+// which 0..2): out[0] = inputs beyond 2^-44 of pow_estimate, out[1] = the largest relative
+// difference's bits.
+extern "C" int sfx_libm_estimate_check(int32_t which, uint64_t out[2]) {
+  if (which < 0 || which > 2 || !out) { sfx::set_error("sfx_libm_estimate_check: which 0..2"); return SFX_ERR_ARG; }
+  unsigned long long* d = nullptr;
+  if (hipMalloc(&d, 2 * sizeof(unsigned long long)) != hipSuccess) { sfx::set_error("hipMalloc failed"); return SFX_ERR_MEMORY; }
+  hipMemset(d, 0, 2 * sizeof(unsigned long long));
+  const uint32_t lo = sfx::f2u(sfx::POW_XLO), n = sfx::f2u(sfx::POW_XHI) - lo;
+  hipLaunchKernelGGL(k_estimate_check, dim3(8192), dim3(256), 0, 0, lo, n, which, d, d + 1);
+  const hipError_t e = hipDeviceSynchronize();
+  unsigned long long h[2] = {0, 0};
+  if (e == hipSuccess) hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) { sfx::set_error(std::string("sfx_libm_estimate_check: ") + hipGetErrorString(e)); return SFX_ERR_HIP; }
+  out[0] = h[0]; out[1] = h[1];
+  return SFX_OK;
+}
 
 // Device evaluation of the emulation, for tests (host buffers in and out).
 extern "C" int sfx_libm_pow_device(const float* x, int64_t n, int32_t which, float* out) {

@@ -109,6 +109,47 @@ SFX_HD inline double pow_estimate(float x, int which) {
   return (double)x * (double)x * y * corr;
 }
 
+// The device's estimate of the same powers: the cube root's polynomial and Newton steps as
+// fused multiply-adds, and ln x for the float exponents' offsets from the hardware log2 of
+// the mantissa (v_log_f32, absolute error near 2^-23 on [1, 2), which the offsets of at most
+// 8e-8 turn into 2^-47 relative), about half the double operations of pow_estimate. Its
+// rounding decisions need it within 2^-40 of the exact power: a device test checks it
+// against pow_estimate (itself verified within 2^-44 of pow for every table input) for
+// every float of [2^-40, 2^14) and each exponent.
+__device__ inline double pow_estimate_dev(float x, int which) {
+  if (which == 3) return (double)x * sqrt((double)x);
+  const uint32_t b = f2u(x);
+  if (b < 0x00800000u) return pow_estimate(x, which);
+  const int e = (int)(b >> 23) - 127;
+  const float mf = u2f((b & 0x7fffffu) | 0x3f800000u);
+  const double m = (double)mf;
+  const int q = e >= 0 ? e / 3 : -((2 - e) / 3);
+  const int r = e - 3 * q;
+  const double a = m * (double)(1 << r);
+  const double cr = r == 0 ? 1.0 : (r == 1 ? 0.79370052598409979 : 0.62996052494743658);
+  double z = fma(fma(fma(fma(0.0285876759712902, m, -0.22121268212268402), m, 0.6954153443691581), m,
+                     -1.1728037038550854), m, 1.6699753569259383);
+  z = z * cr;
+  const double third = 1.0 / 3.0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double w = a * z * z;
+    z = fma(z * third, fma(-w, z, 1.0), z);
+  }
+  union { uint64_t u; double d; } p2;
+  p2.u = (uint64_t)(q + 1023) << 52;
+  const double y = a * z * z * p2.d;
+  const double lnx = ((double)e + (double)__builtin_amdgcn_logf(mf)) * 0.69314718055994530942;
+  const double d13 = (double)(1.0f / 3.0f) - 1.0 / 3.0;
+  const double d53 = (double)(5.0f / 3.0f) - 5.0 / 3.0;
+  const double d73 = (double)(7.0f / 3.0f) - 7.0 / 3.0;
+  const double t = (which == 0 ? d13 : (which == 1 ? d53 : d73)) * lnx;
+  const double corr = fma(t, fma(0.5, t, 1.0), 1.0);
+  if (which == 0) return y * corr;
+  if (which == 1) return (double)x * y * y * corr;
+  return (double)x * (double)x * y * corr;
+}
+
 // True when the double value r lies within 2^-32 (relative) of a midpoint between two
 // adjacent floats, i.e. when its float rounding is not certain. r must be a positive
 // double in the normal float range. On the bits of r = M 2^(E-52) (M the 53-bit
@@ -203,15 +244,24 @@ SFX_HD inline float libm_trig(float x, int which, P tcos, P tsin) {
 }
 
 // powf(x, (float)(1/3)), powf(x, (float)(5/3)), powf(x, (float)(7/3)) or powf(x, 1.5f)
-// as the C library computes it.
+// as the C library computes it, from the estimate r.
 template <typename P>
-SFX_HD inline float libm_pow(float x, int which, P t13, P t53, P t73, P t15) {
-  if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
-  const double r = pow_estimate(x, which);
+SFX_HD inline float libm_pow_from(float x, double r, int which, P t13, P t53, P t73, P t15) {
   const float f = (float)r;
   if (x >= POW_XLO && x < POW_XHI && near_midpoint(r))
     return pow_lookup(which == 0 ? t13 : (which == 1 ? t53 : (which == 2 ? t73 : t15)), f2u(x), f);
   return f;
+}
+template <typename P>
+SFX_HD inline float libm_pow(float x, int which, P t13, P t53, P t73, P t15) {
+  if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
+  return libm_pow_from(x, pow_estimate(x, which), which, t13, t53, t73, t15);
+}
+// The same on the device from pow_estimate_dev (what the kernels call).
+template <typename P>
+__device__ inline float libm_pow_dev(float x, int which, P t13, P t53, P t73, P t15) {
+  if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
+  return libm_pow_from(x, pow_estimate_dev(x, which), which, t13, t53, t73, t15);
 }
 SFX_HD inline float libm_trig(float x, int which, const PowTables& T) {
   return libm_trig(x, which, (const uint64_t*)T.tcos, (const uint64_t*)T.tsin);

@@ -97,7 +97,7 @@ __device__ __forceinline__ PowTablesG pow_tables_g(const PowTables& T) {
 }
 template <typename R> __device__ __forceinline__ R pow_third(R x, int which, const PowTablesG& T);
 template <> __device__ __forceinline__ float pow_third<float>(float x, int which, const PowTablesG& T) {
-  return libm_pow(x, which, T.t13, T.t53, T.t73, T.t15);
+  return libm_pow_dev(x, which, T.t13, T.t53, T.t73, T.t15);
 }
 template <> __device__ __forceinline__ double pow_third<double>(double x, int which, const PowTablesG& T) {
   (void)T;

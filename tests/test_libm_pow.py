@@ -99,3 +99,21 @@ def test_device_emulation_equals_glibc(which):
     got = emulate(x, which, device=True)
     bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
     assert bad.size == 0, (bad.size, x[bad[:5]], got[bad[:5]], want[bad[:5]])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_device_estimate_is_within_its_bound_for_every_table_input(which):
+    """The kernels' powf estimate (fused multiply-adds, hardware log2 of the mantissa) against
+    pow_estimate, which the table build verifies within 2^-46 (2^-44 for 7/3) of pow for every
+    float of [2^-40, 2^14): within 2^-44 of it for every one of those floats, so within 2^-43
+    of the exact power, far inside the 2^-40 the rounding decisions need."""
+    L = _lib.lib()
+    f = L.sfx_libm_estimate_check
+    f.argtypes = [C.c_int32, C.c_void_p]
+    f.restype = C.c_int
+    out = np.zeros(2, np.uint64)
+    _lib.check(f(which, out.ctypes.data), "libm_estimate_check")
+    worst = out[1:2].view(np.float64)[0]
+    print("estimate", which, "beyond 2^-44:", int(out[0]), "largest relative difference", worst, flush=True)
+    assert out[0] == 0 and worst < 2.0 ** -44

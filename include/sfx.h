@@ -150,6 +150,13 @@ int  sfx_model_create_regular(const sfx_params* p, int32_t nmax, int32_t mmax,
                               float x0, float y0, float dx, float dy, float rotation_deg,
                               const int8_t* msk, sfx_model** out);
 int  sfx_model_create_quadtree(const sfx_params* p, const sfx_quadtree_input* qt, sfx_model** out);
+/* The NetCDF quadtree file (a qtrfile whose name holds ".nc"; quadtree_read_file_netcdf,
+ * sfincs_quadtree.F90:295-442): dimension mesh2d_nFaces; variables n, m, level, md, md1, md2,
+ * mu, mu1, mu2, nd, nd1, nd2, nu, nu1, nu2, z, mask; global attributes x0, y0, dx, dy,
+ * rotation (degrees), nmax, mmax, nr_levels. with_bed != 0 also sets the bed level of the
+ * active cells from z (models without subgrid tables; sfincs_domain.f90:1635-1637). Classic
+ * NetCDF formats only, as sfx_model_read_subgrid_netcdf. */
+int  sfx_model_create_quadtree_netcdf(const sfx_params* p, const char* path, int32_t with_bed, sfx_model** out);
 int  sfx_model_sizes(const sfx_model* m, sfx_sizes* s);
 /* Copy a table by its reference name (e.g. "uv_index_z_nm", "kcuv", "mask_adv", "zbuvmx",
  * "z_index_uv_md", "cuv_index_uv1", "dxrinv"); nbytes must equal the table size.
@@ -163,6 +170,15 @@ int  sfx_model_set_subgrid(sfx_model* m, const sfx_subgrid_input* sg);
  * pwet = 1, no fit. Regular and quadtree meshes (the quadtree file's layout over all its
  * points). Call after the mesh is built and before sfx_model_initialize. */
 int  sfx_model_read_subgrid_binary(sfx_model* m, const char* path);
+/* The NetCDF subgrid file (sbgfile that opens as NetCDF; read_subgrid_file_netcdf,
+ * sfincs_subgrid.F90:57-455): dimensions levels, np, npuv; variables z_zmin, z_zmax,
+ * z_volmax (np), z_level (np, levels), uv_zmin, uv_zmax, uv_navg, uv_ffit (npuv), uv_havg,
+ * uv_nrep, uv_pwet (npuv, levels), C order. Regular meshes: records in sfincs order;
+ * quadtree meshes: records over every quadtree point / quadtree uv-point, re-mapped to the
+ * active cells and uv-points. Then as sfx_model_set_subgrid. The classic NetCDF formats
+ * (CDF-1, CDF-2, CDF-5) are read; a NetCDF-4 (HDF5) file is refused (SFX_ERR_IO, the message
+ * names the conversion). Call after the mesh is built and before sfx_model_initialize. */
+int  sfx_model_read_subgrid_netcdf(sfx_model* m, const char* path);
 /* bzs: water level boundaries (bnd/bzs files): zs is nt*nbnd, point fastest. */
 int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const float* y,
                             int32_t nt, const float* t, const float* zs);

@@ -75,6 +75,8 @@ def _sig(lib):
         "sfx_model_create_regular": (I, [C.POINTER(sfx_params), i32, i32, f, f, f, f, f, P, C.POINTER(P)]),
         "sfx_model_create_quadtree": (I, [C.POINTER(sfx_params), C.POINTER(sfx_quadtree_input), C.POINTER(P)]),
         "sfx_model_read_subgrid_binary": (I, [P, C.c_char_p]),
+        "sfx_model_read_subgrid_netcdf": (I, [P, C.c_char_p]),
+        "sfx_model_create_quadtree_netcdf": (I, [C.POINTER(sfx_params), C.c_char_p, i32, C.POINTER(P)]),
         "sfx_model_set_drainage": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_set_thin_dams": (I, [P, i32, P, P, P]),
         "sfx_model_set_infiltration": (I, [P, P]),
@@ -238,6 +240,15 @@ class Model:
         check(lib().sfx_model_create_quadtree(C.byref(params), C.byref(s), C.byref(out)), "create_quadtree")
         return cls(out)
 
+    @classmethod
+    def quadtree_netcdf(cls, params, path, with_bed=True):
+        """NetCDF quadtree file (quadtree_read_file_netcdf), classic formats; with_bed sets
+        the active cells' bed level from its z."""
+        out = C.c_void_p()
+        check(lib().sfx_model_create_quadtree_netcdf(C.byref(params), str(path).encode(), int(bool(with_bed)),
+                                                     C.byref(out)), "create_quadtree_netcdf")
+        return cls(out)
+
     def sizes(self):
         s = sfx_sizes()
         check(lib().sfx_model_sizes(self.h, C.byref(s)), "sizes")
@@ -257,6 +268,10 @@ class Model:
         p = sfx_params()
         check(lib().sfx_model_get_table(self.h, b"params", C.byref(p), C.sizeof(p)), "params")
         return p
+
+    def read_subgrid_netcdf(self, path):
+        """NetCDF subgrid file (read_subgrid_file_netcdf), classic formats."""
+        check(lib().sfx_model_read_subgrid_netcdf(self.h, str(path).encode()), "read_subgrid_netcdf")
 
     def read_subgrid_binary(self, path):
         """Legacy binary subgrid file (read_subgrid_file_original)."""

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsfx.so")
 
-SOURCES = ["sfx_model.cpp", "sfx_meteo.cpp", "sfx_device.hip", "sfx_libm.hip"]
+SOURCES = ["sfx_model.cpp", "sfx_meteo.cpp", "sfx_netcdf.cpp", "sfx_device.hip", "sfx_libm.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -21,6 +21,14 @@ int write_restart_arrays(const char* path, const double* zs, int32_t np, const f
 // sfx_model_initialize after the Cd table is built.
 int build_meteo(sfx_model* M);
 
+// Where the subgrid files of a quadtree mesh keep each active cell and uv-point: the files
+// hold every quadtree point and, per point, its right then its upper uv-points
+// (sfincs_subgrid.F90:156-240, 530-600). zsrc[c] = the file record of active cell c,
+// uvsrc[i] = the file record of active uv-point i, nuvq = the file's uv-point count.
+// Returns false when the active uv-points do not all appear once. Defined in sfx_model.cpp.
+bool quadtree_subgrid_order(const sfx_model* M, std::vector<int32_t>& zsrc, std::vector<int32_t>& uvsrc,
+                            int32_t& nuvq);
+
 // 1-based reference index -> 0-based storage slot.
 inline int32_t z0(int32_t one_based) { return one_based - 1; }
 

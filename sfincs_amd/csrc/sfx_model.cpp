@@ -599,6 +599,34 @@ extern "C" int sfx_model_set_subgrid(sfx_model* M, const sfx_subgrid_input* s) {
   return SFX_OK;
 }
 
+bool sfx::quadtree_subgrid_order(const sfx_model* M, std::vector<int32_t>& zsrc, std::vector<int32_t>& uvsrc,
+                                 int32_t& nuvq) {
+  const int32_t np = M->np, npuv = M->npuv, nzq = (int32_t)M->index_sfincs_in_quadtree.size();
+  zsrc.assign(np, -1);
+  uvsrc.assign(npuv, -1);
+  for (int32_t k = 0; k < nzq; ++k)
+    if (M->index_sfincs_in_quadtree[k] > 0) zsrc[M->index_sfincs_in_quadtree[k] - 1] = k;
+  // one record per neighbour, right then above; the active ones are the sfincs uv-points
+  // in the order met
+  int32_t nq = 0, ns = 0;
+  auto take = [&](int32_t nm, const std::vector<int32_t>& idx) {
+    ++nq;
+    if (nm > 0 && idx[nm - 1] > 0 && ns < npuv) uvsrc[ns++] = nq - 1;
+  };
+  for (int32_t ip = 0; ip < nzq; ++ip) {
+    const int32_t nm = M->index_sfincs_in_quadtree[ip];
+    if (M->qt_mu1[ip] > 0) take(nm, M->z_index_uv_mu1);
+    if (M->qt_mu[ip] >= 1 && M->qt_mu2[ip] > 0) take(nm, M->z_index_uv_mu2);
+    if (M->qt_nu1[ip] > 0) take(nm, M->z_index_uv_nu1);
+    if (M->qt_nu[ip] >= 1 && M->qt_nu2[ip] > 0) take(nm, M->z_index_uv_nu2);
+  }
+  nuvq = nq;
+  if (ns != npuv) return false;
+  for (int32_t c = 0; c < np; ++c)
+    if (zsrc[c] < 0) return false;
+  return true;
+}
+
 // read_subgrid_file_original (sfincs_subgrid.F90:458-903): the legacy binary subgrid file
 // (stream of int32 headers and float32 records, no NetCDF). It holds hrep and navg per
 // level: levels 2..nl come from the file, level 1 of havg is huthresh (which keeps its
@@ -646,23 +674,8 @@ extern "C" int sfx_model_read_subgrid_binary(sfx_model* M, const char* path) {
       set_error(std::string("subgrid file ") + path + ": header does not match the quadtree");
       return SFX_ERR_IO;
     }
-    for (int32_t k = 0; k < nzq; ++k)
-      if (M->index_sfincs_in_quadtree[k] > 0) zsrc[M->index_sfincs_in_quadtree[k] - 1] = k;
-    // the file's uv-points: per quadtree point, right then above, one per neighbour
-    // (sfincs_subgrid.F90:530-600); the active ones map to the sfincs uv-points in order
-    int32_t nq = 0, ns = 0;
-    auto take = [&](int32_t nm, const std::vector<int32_t>& idx) {
-      ++nq;
-      if (nm > 0 && idx[nm - 1] > 0 && ns < npuv) uvsrc[ns++] = nq - 1;
-    };
-    for (int32_t ip = 0; ip < nzq; ++ip) {
-      const int32_t nm = M->index_sfincs_in_quadtree[ip];
-      if (M->qt_mu1[ip] > 0) take(nm, M->z_index_uv_mu1);
-      if (M->qt_mu[ip] >= 1 && M->qt_mu2[ip] > 0) take(nm, M->z_index_uv_mu2);
-      if (M->qt_nu1[ip] > 0) take(nm, M->z_index_uv_nu1);
-      if (M->qt_nu[ip] >= 1 && M->qt_nu2[ip] > 0) take(nm, M->z_index_uv_nu2);
-    }
-    if (nq != nuvq || ns != npuv) {
+    int32_t nq = 0;
+    if (!sfx::quadtree_subgrid_order(M, zsrc, uvsrc, nq) || nq != nuvq) {
       set_error(std::string("subgrid file ") + path + ": uv-points do not match the quadtree");
       return SFX_ERR_IO;
     }

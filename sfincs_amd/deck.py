@@ -3,8 +3,10 @@
 sfincs.inp (key = value), inputformat = bin: indexfile (int32 np, then the column-major
 indices (m-1)*nmax+n of the active cells), mskfile (int8 per index), depfile (float32 per
 index) -- sfincs_domain.f90:244-260, 1670-1676; quadtree meshes as the legacy binary
-qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile; the legacy binary sbgfile
-(sfincs_subgrid.F90:458-903) for subgrid scenarios; ASCII bnd/bzs, src/dis,
+qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile, or (netcdf=True) as the
+NetCDF qtrfile (:295-442); subgrid scenarios as the legacy binary sbgfile
+(sfincs_subgrid.F90:458-903) or (netcdf=True) the NetCDF sbgfile (:57-455), both NetCDF
+files in the classic 64-bit-offset format (scipy writes it); ASCII bnd/bzs, src/dis,
 prc and wnd files (sfincs_boundaries.f90:8-140, sfincs_discharges.f90:36-126,
 sfincs_meteo.f90:247-312). Values are written with 9 significant digits so that
 float32 round-trips exactly.
@@ -28,9 +30,9 @@ def _date(seconds):
 
 def _write_subgrid(sc, folder):
     """The scenario's legacy binary subgrid file as sfincs.sbg. Only scenarios whose tables
-    come from such a file (workloads.with_subgrid_binary) can be written: NetCDF subgrid
-    files need a library this image does not have, and the two formats do not describe the
-    same tables (huthresh, pwet and the fit differ)."""
+    come from such a file (workloads.with_subgrid_binary) can be written this way: the two
+    formats do not describe the same tables (huthresh, pwet and the fit differ), so tables
+    in the NetCDF semantics go to a NetCDF file (write(netcdf=True))."""
     import shutil
     from . import _lib
     p = _lib.default_params(**sc.params)
@@ -42,6 +44,84 @@ def _write_subgrid(sc, folder):
     if not isinstance(sg, str):
         raise ValueError("deck.write: subgrid tables must come from a legacy binary file (with_subgrid_binary)")
     shutil.copyfile(sg, os.path.join(folder, "sfincs.sbg"))
+
+
+def _model_of(sc):
+    from . import _lib
+    p = _lib.default_params(**sc.params)
+    if sc.kind == "regular":
+        return _lib.Model.regular(p, sc.nmax, sc.mmax, sc.dx, sc.dy, sc.msk)
+    return _lib.Model.quadtree(p, sc.qt)
+
+
+def write_quadtree_netcdf(path, qt, z, version=2):
+    """Quadtree arrays (the dict Model.quadtree takes) and the bed level z of every
+    quadtree point as the NetCDF quadtree file: dimension mesh2d_nFaces, variables n, m,
+    level, md/mu/nd/nu (+1, +2), z, mask, global attributes x0, y0, dx, dy, rotation, nmax,
+    mmax, nr_levels (sfincs_quadtree.F90:295-442)."""
+    from scipy.io import netcdf_file
+    npt = len(qt["n"])
+    with netcdf_file(path, "w", version=version) as f:
+        f.createDimension("mesh2d_nFaces", npt)
+        for k in ("n", "m", "md1", "md2", "mu1", "mu2", "nd1", "nd2", "nu1", "nu2"):
+            f.createVariable(k, "i4", ("mesh2d_nFaces",))[:] = np.asarray(qt[k], np.int32)
+        for k in ("level", "md", "mu", "nd", "nu", "mask"):
+            f.createVariable(k, "b", ("mesh2d_nFaces",))[:] = np.asarray(qt[k], np.int8)
+        f.createVariable("z", "f4", ("mesh2d_nFaces",))[:] = np.asarray(z, np.float32)
+        lev = np.asarray(qt["level"], np.int64)
+        f.x0, f.y0 = np.float32(qt["x0"]), np.float32(qt["y0"])
+        f.dx, f.dy = np.float32(qt["dx"]), np.float32(qt["dy"])
+        f.rotation = np.float32(qt.get("rotation", 0.0))
+        f.nmax = np.int32(np.max((np.asarray(qt["n"]) - 1) // 2 ** (lev - 1)) + 1)
+        f.mmax = np.int32(np.max((np.asarray(qt["m"]) - 1) // 2 ** (lev - 1)) + 1)
+        f.nr_levels = np.int32(qt["nr_levels"])
+
+
+def subgrid_file_order(model, qt=None):
+    """The file record of each active cell and uv-point in a NetCDF subgrid file: sfincs
+    order on regular meshes; on quadtree meshes the quadtree point, and per quadtree point
+    its right then upper uv-points (sfincs_subgrid.F90:156-240). Returns (zsrc, uvsrc, np
+    records, npuv records)."""
+    s = model.sizes()
+    if qt is None:
+        return np.arange(s["np"]), np.arange(s["npuv"]), s["np"], s["npuv"]
+    mask = np.asarray(qt["mask"])
+    sidx = np.where(mask > 0, np.cumsum(mask > 0), 0)
+    zsrc = np.nonzero(mask > 0)[0]
+    t = {k: model.table(f"z_index_uv_{k}") for k in ("mu1", "mu2", "nu1", "nu2")}
+    uvsrc, nq = [], 0
+    for ip in range(len(mask)):
+        nm = int(sidx[ip])
+        for side, j in (("mu", "1"), ("mu", "2"), ("nu", "1"), ("nu", "2")):
+            if int(qt[side + j][ip]) > 0 and (j == "1" or int(qt[side][ip]) >= 1):
+                nq += 1
+                if nm > 0 and t[side + j][nm - 1] > 0:
+                    uvsrc.append(nq - 1)
+    return zsrc, np.asarray(uvsrc), len(mask), nq
+
+
+def write_subgrid_netcdf(path, sg, model, qt=None, version=2, fill=0.0):
+    """NetCDF-style tables `sg` (per active cell / uv-point, as Model.set_subgrid takes them)
+    as the NetCDF subgrid file, records in the file order of subgrid_file_order; records of
+    inactive points hold `fill`."""
+    from scipy.io import netcdf_file
+    nl = int(sg["nlevels"])
+    zsrc, uvsrc, npf, nuvf = subgrid_file_order(model, qt)
+    with netcdf_file(path, "w", version=version) as f:
+        f.createDimension("np", npf)
+        f.createDimension("npuv", nuvf)
+        f.createDimension("levels", nl)
+        for k, src, n in (("z_zmin", zsrc, npf), ("z_zmax", zsrc, npf), ("z_volmax", zsrc, npf),
+                          ("uv_zmin", uvsrc, nuvf), ("uv_zmax", uvsrc, nuvf), ("uv_navg", uvsrc, nuvf),
+                          ("uv_ffit", uvsrc, nuvf)):
+            a = np.full(n, fill, np.float32)
+            a[src] = np.asarray(sg[k], np.float32)
+            f.createVariable(k, "f4", ("npuv" if k.startswith("uv") else "np",))[:] = a
+        for k, src, n, d in (("z_level", zsrc, npf, "np"), ("uv_havg", uvsrc, nuvf, "npuv"),
+                             ("uv_nrep", uvsrc, nuvf, "npuv"), ("uv_pwet", uvsrc, nuvf, "npuv")):
+            a = np.full((n, nl), fill, np.float32)
+            a[src] = np.asarray(sg[k], np.float32).reshape(-1, nl)
+            f.createVariable(k, "f4", (d, "levels"))[:] = a
 
 
 _QUANT = {"amu": ("x_wind", "m s-1"), "amv": ("y_wind", "m s-1"), "amp": ("air_pressure", "Pa"),
@@ -105,7 +185,9 @@ def _write_spw(path, kind, g):
                 _rows(f, np.asarray(rain, np.float32)[it])
 
 
-def write(sc, folder):
+def write(sc, folder, netcdf=False):
+    """Writes the deck; netcdf=True writes quadtree meshes and subgrid tables as NetCDF
+    files (qtrfile sfincs_qtr.nc, sbgfile sfincs_subgrid.nc)."""
     os.makedirs(folder, exist_ok=True)
     p = dict(sc.params)
     lines = []
@@ -136,7 +218,15 @@ def write(sc, folder):
             lines.append(f"{k} = {int(v)}")
         else:
             lines.append(f"{k} = {_fmt(v)}")
-    if sc.subgrid is not None:
+    if sc.subgrid is not None and netcdf:
+        mesh = _model_of(sc)
+        sg = sc.subgrid(mesh)
+        if isinstance(sg, str):
+            raise ValueError("deck.write: a legacy binary subgrid file cannot be written as NetCDF")
+        write_subgrid_netcdf(os.path.join(folder, "sfincs_subgrid.nc"), sg, mesh,
+                             qt=None if sc.kind == "regular" else sc.qt)
+        lines.append("sbgfile = sfincs_subgrid.nc")
+    elif sc.subgrid is not None:
         _write_subgrid(sc, folder)
         lines.append("sbgfile = sfincs.sbg")
     if sc.kind == "regular":
@@ -158,6 +248,9 @@ def write(sc, folder):
         if sc.subgrid is None:
             with open(os.path.join(folder, "sfincs.dep"), "wb") as f:
                 f.write(sc.zb_grid[mm, nn].astype(np.float32).tobytes())
+    elif netcdf:
+        lines.append("qtrfile = sfincs_qtr.nc")
+        write_quadtree_netcdf(os.path.join(folder, "sfincs_qtr.nc"), sc.qt, sc.zb_qt)
     else:
         qt = sc.qt
         lines += ["qtrfile = sfincs.qtr", "mskfile = sfincs.msk"]

@@ -6,7 +6,8 @@
 ! include/sfx.h with iso_c_binding. The host reads the reference's input deck:
 !   sfincs.inp (key = value, reference defaults), inputformat bin (indexfile/mskfile/
 !   depfile, stream binary) or asc (mskfile/depfile, nmax rows of mmax values), or a
-!   legacy binary qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile;
+!   legacy binary qtrfile (sfincs_quadtree.F90:210-292) with a binary mskfile, or a NetCDF
+!   qtrfile (:295-442); sbgfile as NetCDF or legacy binary (sfincs_subgrid.F90:20-53);
 !   bndfile/bzsfile, srcfile/disfile, prcfile, wndfile (ASCII time series).
 module sfx_capi
    use iso_c_binding
@@ -151,6 +152,19 @@ module sfx_capi
          import :: c_int, c_ptr, c_char
          type(c_ptr), value :: m
          character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function sfx_model_read_subgrid_netcdf(m, path) bind(C, name='sfx_model_read_subgrid_netcdf')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function sfx_model_create_quadtree_netcdf(p, path, with_bed, out) &
+            bind(C, name='sfx_model_create_quadtree_netcdf')
+         import :: sfx_params, c_int, c_int32_t, c_ptr, c_char
+         type(sfx_params), intent(in) :: p
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int32_t), value :: with_bed
+         type(c_ptr), intent(out) :: out
       end function
       integer(c_int) function sfx_update_zbuvmx(h) bind(C, name='sfx_update_zbuvmx')
          import :: c_int, c_ptr
@@ -696,7 +710,18 @@ contains
          return
       endif
 
-      if (qtrfile(1:4) /= 'none') then
+      if (qtrfile(1:4) /= 'none' .and. index(qtrfile, '.nc') > 0) then
+         ! NetCDF quadtree file, chosen by its name as quadtree_read_file does
+         ! (sfincs_quadtree.F90:85-95); the mask is one of its variables
+         call logmsg('Info    : reading QuadTree netCDF file '//trim(qtrfile))
+         rc = sfx_model_create_quadtree_netcdf(p, trim(qtrfile)//c_null_char, &
+                                               merge(1_c_int32_t, 0_c_int32_t, sbgfile(1:4) == 'none'), model)
+         if (rc < 0) then
+            call fail('quadtree file')
+            ierr = -1
+            return
+         endif
+      elseif (qtrfile(1:4) /= 'none') then
          call read_quadtree(qtrfile, mskfile, p, sbgfile(1:4) == 'none', rc)
          if (rc /= 0) then
             ierr = -1
@@ -774,15 +799,17 @@ contains
             endif
          endif
       endif
-      ! subgrid tables (read_subgrid_file, sfincs_subgrid.F90:20-53): the legacy binary
-      ! format; a NetCDF file needs the NetCDF library, which this host does not link
-      if (sbgfile(1:4) /= 'none') then
-         if (is_netcdf(sbgfile)) then
-            call logmsg('Error   : '//trim(sbgfile)//' is a NetCDF subgrid file; this host reads the binary '// &
-                        'subgrid format only')
+      ! subgrid tables (read_subgrid_file, sfincs_subgrid.F90:20-53): a file that opens as
+      ! NetCDF is read as such (huthresh 0), any other as the legacy binary format
+      if (sbgfile(1:4) /= 'none' .and. is_netcdf(sbgfile)) then
+         call logmsg('Info   : reading sub-grid netCDF file')
+         rc = sfx_model_read_subgrid_netcdf(model, trim(sbgfile)//c_null_char)
+         if (rc < 0) then
+            call fail('subgrid file')
             ierr = -1
             return
          endif
+      elseif (sbgfile(1:4) /= 'none') then
          call logmsg('Info    : reading subgrid file '//trim(sbgfile))
          rc = sfx_model_read_subgrid_binary(model, trim(sbgfile)//c_null_char)
          if (rc < 0) then

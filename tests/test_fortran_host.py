@@ -83,7 +83,8 @@ SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax",
 
 @needs_host
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves",
-                                  "subgrid_binary", "drainage", "structures", "infiltration", "meteo"])
+                                  "subgrid_binary", "drainage", "structures", "infiltration", "meteo",
+                                  "quadtree_netcdf", "subgrid_netcdf"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
@@ -97,11 +98,13 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     elif kind == "store_switches":
         sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, store_velmax=1, store_fluxmax=1, store_twet=1,
                             twet_threshold=0.02, timestep_analysis=1)
-    elif kind == "quadtree":
+    elif kind in ("quadtree", "quadtree_netcdf"):
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     elif kind == "subgrid_binary":
         sc = W.with_subgrid_binary(W.subgrid_case(22, 26, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0),
                                    tmp_path / "gen.sbg")
+    elif kind == "subgrid_netcdf":   # sbgfile as NetCDF
+        sc = W.subgrid_case(22, 26, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0)
     elif kind == "drainage":
         sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5,
                                              structure_relax=7.5))
@@ -122,7 +125,7 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
         sc.zb_grid = W.beach_with_bumps(W.rng(43), 20, 24, 25.0, 25.0)
         sc.wind = (np.array([0.0, 2000.0], np.float32), np.array([12.0, 18.0], np.float32),
                    np.array([200.0, 250.0], np.float32))
-    d = deck.write(sc, str(tmp_path))
+    d = deck.write(sc, str(tmp_path), netcdf=kind.endswith("netcdf"))
     host = ctypes.CDLL(HOSTLIB)
     host.sfincs_host_model.restype = ctypes.c_void_p
     cwd = os.getcwd()
@@ -134,7 +137,7 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     mf = host.sfincs_host_model()
     mp = sc.build_model()
     lib = _lib.lib()
-    extra = SUBGRID_TABLES if kind == "subgrid_binary" else ()
+    extra = SUBGRID_TABLES if kind.startswith("subgrid") else ()
     if kind == "drainage":
         extra = ("drainage_type", "drainage_params", "drainage_distance")
         assert mp.sizes()["ndrn"] == 5 and mp.params().structure_relax == 7.5
@@ -153,14 +156,16 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
         a, b = _table(lib, mf, name), _table(lib, mp.h.value, name)
         assert (a is None) == (b is None), name
         assert a == b, name
-    if kind == "subgrid_binary":
+    if kind.startswith("subgrid"):
         assert mp.sizes()["subgrid"] == 1 and len(_table(lib, mf, "uv_havg")) > 0
+    if kind.endswith("netcdf"):
+        assert any(f.endswith(".nc") for f in os.listdir(d))
 
 
 @needs_host
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage", "structures",
-                                  "infiltration", "meteo"])
+                                  "infiltration", "meteo", "quadtree_netcdf", "subgrid_netcdf"])
 def test_fortran_host_matches_python_path(kind, tmp_path):
     print(f"case fortran host {kind}", flush=True)
     from sfincs_amd import _lib
@@ -181,9 +186,11 @@ def test_fortran_host_matches_python_path(kind, tmp_path):
         sc = W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, dtwnd=450.0, pavbnd=101000.0)
         sc.rain = None
         W.with_spiderweb(W.with_gridded_meteo(sc, t_step=600.0))
-    else:
+    elif kind == "subgrid_netcdf":   # NetCDF sbgfile
+        sc = W.subgrid_case(30, 34, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0)
+    else:                            # legacy binary or NetCDF qtrfile
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
-    d = deck.write(sc, str(tmp_path))
+    d = deck.write(sc, str(tmp_path), netcdf=kind.endswith("netcdf"))
     r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     zs_f = np.fromfile(os.path.join(d, "zs_final.dat"), np.float64)

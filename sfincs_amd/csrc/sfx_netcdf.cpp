@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <new>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -91,9 +92,14 @@ class NcFile {
 
  private:
   bool parse(const std::vector<unsigned char>& h, bool& short_header, std::string& err);
-  int64_t slab_bytes(const NcVar& v) const {   // one record's bytes (all of a fixed variable)
+  // One record's bytes (all of a fixed variable); -1 when that exceeds the file.
+  int64_t slab_bytes(const NcVar& v) const {
     int64_t n = type_size(v.type);
-    for (size_t k = v.record ? 1 : 0; k < v.dimids.size(); ++k) n *= dims_[v.dimids[k]].len;
+    for (size_t k = v.record ? 1 : 0; k < v.dimids.size(); ++k) {
+      const int64_t d = dims_[v.dimids[k]].len;
+      if (d < 0 || (d > 0 && n > fsize_ / d)) return -1;
+      n *= d;
+    }
     return n;
   }
   FILE* f_ = nullptr;
@@ -201,8 +207,12 @@ bool NcFile::parse(const std::vector<unsigned char>& h, bool& short_header, std:
   int nrec = 0;
   for (const NcVar& v : vars_) nrec += v.record;
   recsize_ = 0;
-  for (const NcVar& v : vars_)
-    if (v.record) recsize_ += nrec == 1 ? slab_bytes(v) : ((slab_bytes(v) + 3) & ~3LL);
+  for (const NcVar& v : vars_) {
+    if (!v.record) continue;
+    const int64_t sb = slab_bytes(v);
+    if (sb < 0) { err = "variable " + v.name + " is larger than the file"; return false; }
+    recsize_ += nrec == 1 ? sb : ((sb + 3) & ~3LL);
+  }
   if (streaming) {
     numrecs_ = 0;
     for (const NcVar& v : vars_)
@@ -251,6 +261,10 @@ bool NcFile::values(const NcVar& v, std::vector<double>& out, std::string& err) 
   const int ts = type_size(v.type);
   const int64_t slab = slab_bytes(v);
   const int64_t nslab = v.record ? numrecs_ : 1;
+  if (slab < 0 || nslab < 0 || (slab > 0 && nslab > fsize_ / slab)) {
+    err = path_ + ": variable " + v.name + " is larger than the file";
+    return false;
+  }
   out.resize((size_t)(slab / ts * nslab));
   std::vector<unsigned char> raw((size_t)slab);
   for (int64_t r = 0; r < nslab; ++r) {
@@ -284,8 +298,10 @@ bool get_var(const NcFile& f, const char* name, const std::vector<int64_t>& shap
   if (!f.values(*v, d, err)) return false;
   out.resize(d.size());
   const double lo = (double)std::numeric_limits<T>::lowest(), hi = (double)std::numeric_limits<T>::max();
+  // a float variable read as float passes unconverted (infinities and NaN included)
+  const bool same = std::is_same<T, float>::value && v->type == NC_FLOAT;
   for (size_t k = 0; k < d.size(); ++k) {
-    if (!(d[k] >= lo && d[k] <= hi) && !(std::is_floating_point<T>::value && std::isnan(d[k]))) {
+    if (!same && !(d[k] >= lo && d[k] <= hi) && !(std::is_floating_point<T>::value && std::isnan(d[k]))) {
       err = path + ": variable " + name + " holds a value out of range";
       return false;
     }
@@ -296,10 +312,9 @@ bool get_var(const NcFile& f, const char* name, const std::vector<int64_t>& shap
 
 }  // namespace
 
-extern "C" int sfx_model_create_quadtree_netcdf(const sfx_params* p, const char* path, int32_t with_bed,
-                                                sfx_model** out) {
-  if (!p || !path || !out) { set_error("sfx_model_create_quadtree_netcdf: bad args"); return SFX_ERR_ARG; }
-  *out = nullptr;
+namespace {
+
+int create_quadtree_netcdf(const sfx_params* p, const char* path, int32_t with_bed, sfx_model** out) {
   NcFile f;
   std::string err;
   if (!f.open(path, err)) { set_error(err); return SFX_ERR_IO; }
@@ -345,8 +360,7 @@ extern "C" int sfx_model_create_quadtree_netcdf(const sfx_params* p, const char*
   return rc;
 }
 
-extern "C" int sfx_model_read_subgrid_netcdf(sfx_model* M, const char* path) {
-  if (!M || !path || !M->mesh_built || M->initialized) { set_error("sfx_model_read_subgrid_netcdf: bad state"); return SFX_ERR_STATE; }
+int read_subgrid_netcdf(sfx_model* M, const char* path) {
   NcFile f;
   std::string err;
   if (!f.open(path, err)) { set_error(err); return SFX_ERR_IO; }
@@ -398,4 +412,28 @@ extern "C" int sfx_model_read_subgrid_netcdf(sfx_model* M, const char* path) {
   s.uv_zmin = a_uzmin.data(); s.uv_zmax = a_uzmax.data(); s.uv_navg = a_navg.data(); s.uv_ffit = a_ffit.data();
   s.uv_havg = a_havg.data(); s.uv_nrep = a_nrep.data(); s.uv_pwet = a_pwet.data();
   return sfx_model_set_subgrid(M, &s);
+}
+
+}  // namespace
+
+extern "C" int sfx_model_create_quadtree_netcdf(const sfx_params* p, const char* path, int32_t with_bed,
+                                                sfx_model** out) {
+  if (!p || !path || !out) { set_error("sfx_model_create_quadtree_netcdf: bad args"); return SFX_ERR_ARG; }
+  *out = nullptr;
+  try {
+    return create_quadtree_netcdf(p, path, with_bed, out);
+  } catch (const std::bad_alloc&) {
+    set_error(std::string(path) + ": out of host memory");
+    return SFX_ERR_MEMORY;
+  }
+}
+
+extern "C" int sfx_model_read_subgrid_netcdf(sfx_model* M, const char* path) {
+  if (!M || !path || !M->mesh_built || M->initialized) { set_error("sfx_model_read_subgrid_netcdf: bad state"); return SFX_ERR_STATE; }
+  try {
+    return read_subgrid_netcdf(M, path);
+  } catch (const std::bad_alloc&) {
+    set_error(std::string(path) + ": out of host memory");
+    return SFX_ERR_MEMORY;
+  }
 }

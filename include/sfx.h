@@ -101,6 +101,7 @@ typedef struct sfx_params {
   float spw_merge_frac;      /* 0.5 (spwmergefrac): the spiderweb fades into the background from
                               * this fraction of its radius outwards */
   float factor_wind, factor_pres, factor_prcp, factor_spw_size;   /* 1.0 each */
+  float sfacinf;         /* initial abstraction of the curve-number method, fraction of S (0.2) */
 } sfx_params;
 
 void sfx_params_default(sfx_params* p);
@@ -179,6 +180,14 @@ int  sfx_model_read_subgrid_binary(sfx_model* m, const char* path);
  * (CDF-1, CDF-2, CDF-5) are read; a NetCDF-4 (HDF5) file is refused (SFX_ERR_IO, the message
  * names the conversion). Call after the mesh is built and before sfx_model_initialize. */
 int  sfx_model_read_subgrid_netcdf(sfx_model* m, const char* path);
+/* Curve-number infiltration, method A (scsfile; initialize_infiltration 'cna',
+ * sfincs_infiltration.f90:106-112,260-293; update_infiltration_map :678-722): the maximum
+ * retention S per active cell in inches (converted to m by 0.0254). Each step every cell's
+ * cumulative rain P grows by its rain times dt; while P <= sfacinf S all rain infiltrates,
+ * after that the cumulative infiltration follows P - (P - sfacinf S)^2 / (P + (1 - sfacinf) S),
+ * wet or dry, no recovery. Needs rain; a uniform qinf > 0 or an infiltration map takes
+ * precedence (:92-112). Call before sfx_model_initialize. */
+int  sfx_model_set_curve_number(sfx_model* m, const float* scs_inch);
 /* bzs: water level boundaries (bnd/bzs files): zs is nt*nbnd, point fastest. */
 int  sfx_model_set_boundary(sfx_model* m, int32_t nbnd, const float* x, const float* y,
                             int32_t nt, const float* t, const float* zs);

@@ -69,7 +69,7 @@ def olib():
         _olib.oracle_compute_fluxes.restype = C.c_float
         _olib.oracle_compute_water_levels.argtypes = [C.c_void_p, C.c_double, C.c_float]
         _olib.oracle_compute_fluxes_over_structures.argtypes = [C.c_void_p]
-        _olib.oracle_update_infiltration.argtypes = [C.c_void_p]
+        _olib.oracle_update_infiltration.argtypes = [C.c_void_p, C.c_float]
         F, I32, P = C.c_float, C.c_int32, C.c_void_p
         _olib.oracle_set_grid.argtypes = [P, I32, I32, I32, I32, F, F, F, F, P, P]
         _olib.oracle_set_spw.argtypes = [P, I32, I32, I32, F, P, P, P, P, P, P, P]
@@ -150,6 +150,10 @@ class OracleRun:
             self._bind(n, a if a.size else np.zeros(1, dt))
         qi = model.table("qinffield")
         self._bind("qinffield", qi.copy() if qi.size else np.zeros(1, np.float32))
+        sm = model.table("scs_m")
+        self._bind("scs_m", sm.copy() if sm.size else np.zeros(1, np.float32))
+        self._bind("cumprcp", np.zeros(max(1, s["np"]), np.float32))
+        self._bind("cuminf", np.zeros(max(1, s["np"]), np.float32))
         p = model.params_dict if hasattr(model, "params_dict") else None
         self.params = p
         self._bind("x73", x73_table())
@@ -185,7 +189,8 @@ class OracleRun:
                     store_velmax=p.store_velmax, store_fluxmax=p.store_fluxmax, store_twet=p.store_twet,
                     twet_threshold=p.twet_threshold, timestep_analysis=p.timestep_analysis, use_qext=p.use_qext,
                     use_storage_volume=int(bool(s["subgrid"]) and len(tb("storage_volume")) > 0),
-                    infiltration=int(len(tb("qinffield")) > 0),
+                    infiltration=int(len(tb("qinffield")) > 0 or len(tb("scs_m")) > 0),
+                    cna=int(len(tb("scs_m")) > 0), sfacinf=p.sfacinf,
                     wind=int(len(tb("twnd")) > 0), precip=int(len(tb("tprcpt")) > 0), h73table=p.h73table,
                     ngbnd=s["ngbnd"], nkcuv2=s["nkcuv2"], nbnd=s["nbnd"], ntbnd=len(tb("t_bnd")),
                     has_bzi=int(len(tb("zsi_bnd")) > 0),
@@ -246,7 +251,7 @@ class OracleRun:
         if self.vals["wind"] or self.vals["precip"] or self.vals["patmos"]:
             self.L.oracle_update_meteo(self.m, t, dt)
             if self.vals["infiltration"]:
-                self.L.oracle_update_infiltration(self.m)
+                self.L.oracle_update_infiltration(self.m, dt)
         self.L.oracle_update_boundaries(self.m, t, dt)
         self.L.oracle_update_discharges(self.m, t, dt)
 

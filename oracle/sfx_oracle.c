@@ -11,7 +11,8 @@
  *                                runs right after it
  *   oracle_compute_water_levels  sfincs_continuity.f90:5-720 (with compute_store_variables)
  *   oracle_update_meteo          sfincs_meteo.f90:1246-1558 (time-series branch)
- *   oracle_update_infiltration   sfincs_infiltration.f90:632-676 ('con' and 'c2d')
+ *   oracle_update_infiltration   sfincs_infiltration.f90:632-676 ('con' and 'c2d'), 678-722
+ *                                ('cna', curve number A)
  *   oracle_update_boundaries     sfincs_boundaries.f90:682-1175
  *   oracle_update_discharges     sfincs_discharges.f90:325-655 (point sources and drainage
  *                                structures)
@@ -47,7 +48,11 @@
  *   oracle_update_infiltration   a flat basin's level falls by the rate less the rain to
  *                                the float step (regular and subgrid); dry cells keep their
  *                                rain; the removed volume against the rate times the wet
- *                                area (test_infiltration.py)
+ *                                area (test_infiltration.py); 'cna': the cumulative
+ *                                infiltration against the SCS runoff relation
+ *                                P - (P - Ia)^2 / (P - Ia + S) at every step, all rain
+ *                                infiltrating below Ia = sfacinf S, wet and dry cells alike
+ *                                (test_curve_number.py)
  *   oracle_update                step counts and model time against the GPU and the
  *                                reference loop (test_oracle_vs_ref.py); the instability stop
  *                                by closed form (test_forcing_pins.py)
@@ -100,6 +105,10 @@ typedef struct OM {
   int32_t *tsa_wet, *tsa_lim;
   int8_t *kfuv;
   float *tauwu, *tauwv, *netprcp;   /* np each */
+  int32_t cna;                       /* curve-number infiltration, method A */
+  float sfacinf;
+  const float *scs_m;                /* S per cell, m ('cna') */
+  float *cumprcp, *cuminf;           /* cumulative rain and infiltration per cell, m ('cna') */
   /* boundaries */
   const int32_t *nmindbnd, *index_kcuv2, *nmbkcuv2, *nmikcuv2, *ibkcuv2, *ind1_bnd_gbp, *ind2_bnd_gbp, *nmi_gbp;
   const int8_t *ibuvdir;
@@ -172,7 +181,7 @@ int oracle_bind(OM* m, const char* name, void* ptr) {
   BIND(z_zmin) BIND(z_zmax) BIND(z_volmax) BIND(z_dep) BIND(uv_zmin) BIND(uv_zmax) BIND(uv_navg_w) BIND(uv_fnfit)
   BIND(uv_havg) BIND(uv_nrep) BIND(uv_pwet) BIND(uv_havg_zmax) BIND(uv_nrep_zmax)
   BIND(zs) BIND(z_volume) BIND(q) BIND(uv) BIND(q0) BIND(uv0) BIND(zs0) BIND(zsderv) BIND(zsmax) BIND(kfuv)
-  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim) BIND(qext) BIND(storage_volume) BIND(qinffield)
+  BIND(vmax) BIND(qmax) BIND(twet) BIND(tsa_req) BIND(tsa_sum) BIND(tsa_wet) BIND(tsa_lim) BIND(qext) BIND(storage_volume) BIND(qinffield) BIND(scs_m) BIND(cumprcp) BIND(cuminf)
   BIND(tauwu) BIND(tauwv) BIND(netprcp) BIND(z_xz) BIND(z_yz) BIND(patm)
   BIND(tauwu0) BIND(tauwu1) BIND(tauwv0) BIND(tauwv1) BIND(patm0) BIND(patm1) BIND(prcp0) BIND(prcp1)
   BIND(nmindbnd) BIND(index_kcuv2) BIND(nmbkcuv2) BIND(nmikcuv2) BIND(ibkcuv2) BIND(ind1_bnd_gbp) BIND(ind2_bnd_gbp)
@@ -191,7 +200,7 @@ int oracle_set(OM* m, const char* name, double v) {
   SETI(advection_scheme) SETI(viscosity) SETI(coriolis) SETI(friction2d) SETI(thetasmoothing) SETI(wiggle_suppression)
   SETI(store_zsmax) SETI(wind) SETI(precip) SETI(ngbnd) SETI(nkcuv2) SETI(nbnd) SETI(ntbnd) SETI(nsrc) SETI(ntsrc)
   SETI(ntprcp) SETI(ntwnd) SETI(h73table) SETI(nt) SETI(error) SETI(ndrn) SETF(structure_relax) SETI(nstruc)
-  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume) SETI(infiltration)
+  SETI(store_velmax) SETI(store_fluxmax) SETI(store_twet) SETF(twet_threshold) SETI(has_bzi) SETI(timestep_analysis) SETI(use_qext) SETI(use_storage_volume) SETI(infiltration) SETI(cna) SETF(sfacinf)
   SETI(meteo3d) SETI(patmos) SETI(meteo_wind) SETI(meteo_rain) SETI(spinup_meteo) SETI(amprblock) SETI(usespwprecip)
   SETF(dtwnd) SETF(gapres) SETF(pavbnd) SETF(spw_merge_frac) SETF(cosrot) SETF(sinrot) SETF(factor_wind)
   SETF(factor_pres) SETF(factor_prcp) SETF(factor_spw_size) SETF(twindupd) SETF(meteo_t0) SETF(meteo_t1)
@@ -872,7 +881,28 @@ static void meteo3d_forcing(OM* m, double t) {
 /* Constant rates ('con', 'c2d'): a cell with water keeps its rate, a dry one (z_volume <= 0
  * on subgrid meshes, zs <= zb on regular ones) none; the rate leaves the net
  * precipitation. The cumulative infiltration (:665-670) is output only and not kept. */
-void oracle_update_infiltration(OM* m) {
+void oracle_update_infiltration(OM* m, float dt) {
+  if (m->cna) {
+    /* 'cna' (:678-722): every cell, wet or dry; the cumulative rain grows first, as
+     * update_meteo_forcing adds it (sfincs_meteo.f90:1334-1336,1547-1549); netprcp holds
+     * the cell's rain here */
+    for (int32_t nm = 0; nm < m->np; ++nm) {
+      const float prcp = m->netprcp[nm], S = m->scs_m[nm];
+      m->cumprcp[nm] = m->cumprcp[nm] + prcp * dt;
+      float qinfmap;
+      if (m->cumprcp[nm] > m->sfacinf * S) {
+        const float e = m->cumprcp[nm] - m->sfacinf * S;
+        const float Qq = e * e / (m->cumprcp[nm] + (1.0f - m->sfacinf) * S);
+        const float I = m->cumprcp[nm] - Qq;
+        qinfmap = (I - m->cuminf[nm]) / dt;
+      } else {
+        qinfmap = prcp;
+      }
+      m->netprcp[nm] = m->netprcp[nm] - qinfmap;
+      m->cuminf[nm] = m->cuminf[nm] + qinfmap * dt;
+    }
+    return;
+  }
   for (int32_t nm = 0; nm < m->np; ++nm) {
     float qinfmap = m->qinffield[nm];
     if (m->subgrid) {
@@ -1164,7 +1194,7 @@ int32_t oracle_update(OM* m, double tend, int32_t max_steps) {
     oracle_meteo_window(m);
     if (m->wind || m->patmos || m->precip) {
       oracle_update_meteo(m, m->t, m->dt);
-      if (m->infiltration) oracle_update_infiltration(m);   /* sfincs_lib.f90:534-540 */
+      if (m->infiltration) oracle_update_infiltration(m, m->dt);   /* sfincs_lib.f90:534-540 */
     }
     oracle_update_boundaries(m, m->t, m->dt);
     oracle_update_discharges(m, m->t, m->dt);

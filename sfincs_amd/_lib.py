@@ -33,7 +33,7 @@ class sfx_params(C.Structure):
                                        ("amprblock", C.c_int32), ("usespwprecip", C.c_int32),
                                        ("spw_merge_frac", C.c_float), ("factor_wind", C.c_float),
                                        ("factor_pres", C.c_float), ("factor_prcp", C.c_float),
-                                       ("factor_spw_size", C.c_float)]
+                                       ("factor_spw_size", C.c_float), ("sfacinf", C.c_float)]
 
 
 class sfx_quadtree_input(C.Structure):
@@ -80,6 +80,7 @@ def _sig(lib):
         "sfx_model_set_drainage": (I, [P, i32, P, P, P, P, P, P]),
         "sfx_model_set_thin_dams": (I, [P, i32, P, P, P]),
         "sfx_model_set_infiltration": (I, [P, P]),
+        "sfx_model_set_curve_number": (I, [P, P]),
         "sfx_model_set_meteo_grid": (I, [P, i32, i32, i32, i32, f, f, f, f, P, P]),
         "sfx_model_set_spiderweb": (I, [P, i32, i32, i32, f, P, P, P, P, P, P, P]),
         "sfx_model_read_meteo_file": (I, [P, s, s, s]),
@@ -382,6 +383,14 @@ class Model:
         self._keep_qinf = a
         check(lib().sfx_model_set_infiltration(self.h, ptr(a)), "set_infiltration")
 
+    def set_curve_number(self, scs_inch):
+        """Curve-number infiltration, method A (scsfile): maximum retention S per active
+        cell in inches. Acts only with rain; qinf > 0 or an infiltration map takes
+        precedence."""
+        a = np.ascontiguousarray(scs_inch, np.float32)
+        self._keep_scs = a
+        check(lib().sfx_model_set_curve_number(self.h, ptr(a)), "set_curve_number")
+
     def set_rain(self, t, mmh):
         t = np.ascontiguousarray(t, np.float32); v = np.ascontiguousarray(mmh, np.float32)
         check(lib().sfx_model_set_rain(self.h, len(t), ptr(t), ptr(v)), "set_rain")
@@ -449,7 +458,8 @@ STATE_FIELDS = {"zs": np.float64, "z_volume": np.float64, "q": np.float32, "uv":
                 "kfuv": np.int8, "zs0": np.float32, "zsderv": np.float32, "zsmax": np.float32,
                 "uvmean": np.float32, "vmax": np.float32, "qmax": np.float32, "twet": np.float32,
                 "tsa_sum": np.float32, "tsa_wet": np.int32, "tsa_lim": np.int32, "qext": np.float32,
-                "storage_volume": np.float32, "zb": np.float32, "qtsrc": np.float32}
+                "storage_volume": np.float32, "zb": np.float32, "qtsrc": np.float32, "cumprcp": np.float32,
+                "cuminf": np.float32}
 
 
 class Instance:
@@ -465,7 +475,7 @@ class Instance:
     def _n(self, name):
         s = self.sizes
         if name in ("zs", "z_volume", "zs0", "zsderv", "zsmax", "vmax", "qmax", "twet", "qext", "storage_volume",
-                    "zb"):
+                    "zb", "cumprcp", "cuminf"):
             return s["np"]
         if name in ("q", "uv"):
             return s["npuv"] + s["ncuv"] + 1

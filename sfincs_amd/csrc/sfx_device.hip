@@ -128,7 +128,8 @@ struct DevInst {
   float* tsa_req; float* tsa_sum; int32_t* tsa_wet; int32_t* tsa_lim;
   float* tsa_min;                // the last sweep's minimum time step
   float* qext;                   // external source per cell, m/s (use_qext)
-  const float* qinf;             // infiltration rate per cell, m/s (qinffield; F_INFIL)
+  const float* qinf;             // infiltration rate per cell, m/s (qinffield; F_INFIL), or S in m (F_CNA)
+  float* cumprcp; float* cuminf; // cumulative rain and infiltration per cell, m (F_CNA)
   // gridded meteo fields at the update times, [mnframes][np]: wind stress (F_MWIND),
   // atmospheric pressure (F_PATM), rain in m/s (F_MPRCP)
   const float* mtauu; const float* mtauv; const float* mpatm; const float* mprcp;
@@ -1626,7 +1627,7 @@ template <typename R>
 __device__ __forceinline__ R cell_rain(const DevInst& I, int c, int kcs, int iref, R dt, R np_u, bool dry,
                                        const MeteoNow& m, double& inf) {
   const uint32_t f = I.c.flags;
-  if (!(f & (sfx::F_MPRCP | sfx::F_INFIL))) return np_u;
+  if (!(f & (sfx::F_MPRCP | sfx::F_INFIL | sfx::F_CNA))) return np_u;
   R np_c = np_u;
   if (f & sfx::F_MPRCP) {
     float p = G(I.mprcp)[m.a + (uint32_t)c] * m.omw + G(I.mprcp)[m.b + (uint32_t)c] * m.w;
@@ -1638,6 +1639,22 @@ __device__ __forceinline__ R cell_rain(const DevInst& I, int c, int kcs, int ire
     np_c = (R)p;
   }
   if ((f & sfx::F_INFIL) && !dry) np_c = np_c - (R)G(I.qinf)[c];
+  if (f & sfx::F_CNA) {   // update_infiltration_map 'cna' (sfincs_infiltration.f90:678-722), wet or dry
+    const float prcp = (float)np_c, S = G(I.qinf)[c], sf = I.c.sfacinf, dtf = (float)dt;
+    const float P = G(I.cumprcp)[c] + prcp * dtf;   // update_meteo_forcing (sfincs_meteo.f90:1547-1549)
+    const float cinf = G(I.cuminf)[c];
+    float qinfmap;
+    if (P > sf * S) {
+      const float e = P - sf * S;
+      const float Qq = e * e / (P + (1.0f - sf) * S);
+      qinfmap = ((P - Qq) - cinf) / dtf;
+    } else {
+      qinfmap = prcp;
+    }
+    np_c = (R)(prcp - qinfmap);
+    G(I.cumprcp)[c] = P;
+    G(I.cuminf)[c] = cinf + qinfmap * dtf;
+  }
   if (kcs == 1) inf = ((double)(np_u * dt) - (double)(np_c * dt)) * (double)I.c.cell_area[iref - 1];
   return np_c;
 }
@@ -1719,7 +1736,7 @@ __device__ __forceinline__ unsigned cont_update(const DevInst& I, int c, R q_md,
                                                 R dt, R netprcp, bool dry_sky, const MeteoNow& mt, double& inf) {
   const uint8_t kraw = G(I.kcs)[c];
   const bool quiet = (kraw & (KCS_MASK | KCS_SOURCE)) == 1 && dry_sky && q_md == R(0) && q_mu == R(0) &&
-                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & (sfx::F_STORE | sfx::F_QEXT)) &&
+                     q_nd == R(0) && q_nu == R(0) && !(I.c.flags & (sfx::F_STORE | sfx::F_QEXT | sfx::F_CNA)) &&
                      (!RX || ((!(I.c.flags & sfx::F_INFIL) || G(I.qinf)[c] == 0.0f) &&
                               (!(I.c.flags & sfx::F_MPRCP) ||
                                (G(I.mprcp)[mt.a + (uint32_t)c] == 0.0f && G(I.mprcp)[mt.b + (uint32_t)c] == 0.0f))));
@@ -1882,7 +1899,7 @@ __global__ void __launch_bounds__(BLOCK) k_continuity(const DevInst* __restrict_
     }
     }
   }
-  if (RX && (I.c.flags & (sfx::F_INFIL | sfx::F_MPRCP))) add_infiltrated(I, tile, inf);   // every lane of the block is here
+  if (RX && (I.c.flags & (sfx::F_INFIL | sfx::F_MPRCP | sfx::F_CNA))) add_infiltrated(I, tile, inf);   // every lane of the block is here
 }
 
 // FP64 volume accumulator: sum over active cells of V (subgrid) or zs*A (regular).
@@ -2065,6 +2082,7 @@ uint32_t build_flags(const sfx_model* M) {
   if (M->wind) f |= sfx::F_WIND;
   if (M->precip) f |= sfx::F_PRECIP;
   if (M->precip && !M->qinffield.empty()) f |= sfx::F_INFIL;
+  if (M->precip && !M->scs_m.empty()) f |= sfx::F_CNA;
   if (M->meteo3d) f |= sfx::F_METEO3D;
   if (M->meteo3d && M->meteo_wind) f |= sfx::F_MWIND;
   if (M->meteo3d && M->patmos) f |= sfx::F_PATM;
@@ -2087,7 +2105,7 @@ int fill_consts(const sfx_model* M, Consts& c) {
   c.alpha = p.alpha; c.rhoa = p.rhoa; c.rhow = p.rhow; c.btfilter = p.btfilter; c.btrelax = p.btrelax;
   c.t0 = p.t0; c.t1 = p.t1; c.tspinup = p.t0 + p.tspinup; c.zini = p.zsini; c.dx = M->dxrm[0]; c.dy = M->dyrm[0];
   c.pi = p.pi; c.twet_threshold = p.twet_threshold;
-  c.dtwnd = p.dtwnd; c.gapres = p.gapres; c.pavbnd = p.pavbnd;
+  c.dtwnd = p.dtwnd; c.gapres = p.gapres; c.pavbnd = p.pavbnd; c.sfacinf = p.sfacinf;
   for (int i = 0; i < M->nref; ++i) {
     c.dxrinv[i] = M->dxrinv[i]; c.dyrinv[i] = M->dyrinv[i]; c.dxr2inv[i] = M->dxr2inv[i]; c.dyr2inv[i] = M->dyr2inv[i];
     c.dxrinvc[i] = M->dxrinvc[i]; c.dyrinvc[i] = M->dyrinvc[i]; c.cell_area[i] = M->cell_area[i];
@@ -2127,6 +2145,12 @@ int upload_state(sfx_handle* H, const sfx_model* M) {
   }
   if (f & sfx::F_STORAGE) { auto v = to_r<R>(M->storage_volume); I.storage = H->upload(v.data(), np); }
   if (f & sfx::F_INFIL) I.qinf = H->upload(M->qinffield.data(), np);
+  if (f & sfx::F_CNA) I.qinf = H->upload(M->scs_m.data(), np);
+  if (f & sfx::F_CNA) {
+    std::vector<float> z(np, 0.f);
+    I.cumprcp = H->upload(z.data(), np);
+    I.cuminf = H->upload(z.data(), np);
+  }
   if (f & sfx::F_METEO3D) {
     I.mnframes = M->meteo_nframes;
     if (f & sfx::F_MWIND) { I.mtauu = H->upload(M->meteo_tauwu.data(), M->meteo_tauwu.size());
@@ -2596,6 +2620,12 @@ extern "C" int sfx_get_state(sfx_handle* H, const char* name, void* dst, size_t 
     const size_t n = H->sz.nkcuv2;
     return H->fp64 ? get_cell_r<double>(H, I.uvmean, dst, nbytes, n) : get_cell_r<float>(H, I.uvmean, dst, nbytes, n);
   }
+  if (k == "cumprcp" || k == "cuminf") {   // curve-number cumulative rain and infiltration, f32 per cell
+    const float* src = k == "cumprcp" ? I.cumprcp : I.cuminf;
+    if (!src || nbytes != (size_t)np * 4) { set_error("sfx_get_state: " + k + " needs curve-number infiltration and np f32"); return SFX_ERR_ARG; }
+    HIPCHK(hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost));
+    return SFX_OK;
+  }
   if (k == "qtsrc") {   // the step's discharges: point sources, then each drainage intake and outfall
     const size_t n = (size_t)I.nsrc + 2 * (size_t)I.ndrn;
     if (n == 0 || nbytes != n * 4) { set_error("sfx_get_state: qtsrc needs (nsrc + 2 ndrn) f32 values"); return SFX_ERR_ARG; }
@@ -2913,7 +2943,7 @@ void launch_phases(sfx_batch* B, int what, int forcing_mode) {
   const int lead = (nahead + nbal + 7) / 8 * 8;   // keeps block b of the cells on XCD b mod 8
   if (what & 4) {
     bool rx = false;   // per-cell rain (infiltration, gridded rain) in some member
-    for (sfx_handle* H : B->members) rx = rx || (H->hI.c.flags & (sfx::F_INFIL | sfx::F_MPRCP)) != 0;
+    for (sfx_handle* H : B->members) rx = rx || (H->hI.c.flags & (sfx::F_INFIL | sfx::F_MPRCP | sfx::F_CNA)) != 0;
     auto kc = rx ? k_continuity<R, true> : k_continuity<R, false>;
     if (B->n_cd)
       hipLaunchKernelGGL(kc, dim3(B->n_cd + lead), dim3(BLOCK), 0, st, base_of(B, B->wl_cd), B->wl_cd, 0,

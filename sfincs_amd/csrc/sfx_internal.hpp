@@ -151,6 +151,8 @@ struct sfx_model {
 
   // infiltration ('con' from params.qinf, or 'c2d' from a per-cell map in mm/h): the rate
   // per cell in m/s after initialize (empty: no infiltration)
+  std::vector<float> scs_inch;                  // curve-number retention S per cell, inches ('cna')
+  std::vector<float> scs_m;                     // S in m when curve-number infiltration is on ('cna')
   std::vector<float> qinf_map_mmh, qinffield;
 
   // meteo series

@@ -44,6 +44,7 @@ extern "C" void sfx_params_default(sfx_params* p) {
   p->dtwnd = 1800.0f; p->baro = 1; p->pavbnd = 0.0f; p->gapres = 101200.0f; p->spinup_meteo = 0;
   p->amprblock = 1; p->usespwprecip = 1; p->spw_merge_frac = 0.5f;
   p->factor_wind = 1.0f; p->factor_pres = 1.0f; p->factor_prcp = 1.0f; p->factor_spw_size = 1.0f;
+  p->sfacinf = 0.2f;   // sfincs_input.f90:102
   p->bndtype = 1; p->h73table = 0;
 }
 
@@ -883,6 +884,12 @@ extern "C" int sfx_model_set_infiltration(sfx_model* M, const float* qinf_mmh) {
   return SFX_OK;
 }
 
+extern "C" int sfx_model_set_curve_number(sfx_model* M, const float* scs_inch) {
+  if (!M || M->initialized || !M->mesh_built || !scs_inch) { set_error("sfx_model_set_curve_number: bad args"); return SFX_ERR_ARG; }
+  M->scs_inch.assign(scs_inch, scs_inch + M->np);
+  return SFX_OK;
+}
+
 extern "C" int sfx_model_set_thin_dams(sfx_model* M, int32_t npoly, const int32_t* nv, const float* x, const float* y) {
   size_t nvert = 0;
   if (!M || M->initialized || !polylines_ok(npoly, nv, x, y, nvert)) {
@@ -1382,7 +1389,12 @@ extern "C" int sfx_model_initialize(sfx_model* M) {
   // uniform rate ('con', qinf > 0, converted by sfincs_input.f90:376) before the map
   // ('c2d', :224-258), as the reference tries them (:92-104)
   M->qinffield.clear();
-  if (M->precip && (p.qinf > 0.0f || !M->qinf_map_mmh.empty())) {
+  M->scs_m.clear();
+  if (M->precip && p.qinf <= 0.0f && M->qinf_map_mmh.empty() && !M->scs_inch.empty()) {
+    // 'cna' (:260-293): S in m (the reference keeps it in qinffield)
+    M->scs_m.assign(np, 0.0f);
+    for (int32_t nm = 0; nm < np; ++nm) M->scs_m[nm] = M->scs_inch[nm] * 0.0254f;
+  } else if (M->precip && (p.qinf > 0.0f || !M->qinf_map_mmh.empty())) {
     M->qinffield.assign(np, 0.0f);
     if (p.qinf > 0.0f) {
       const float qinf = p.qinf / (float)(3600 * 1000);
@@ -1439,7 +1451,7 @@ bool lookup(const sfx_model* M, const std::string& k, View& out) {
   T(uv_index_v_ndm) T(uv_index_v_nm) T(uv_index_v_ndmu) T(uv_index_v_nmu)
   T(uv_flags_iref) T(uv_flags_type) T(uv_flags_dir) T(kcuv) T(mask_adv)
   T(thindam_uv_index) T(structure_uv_index) T(structure_height) T(structure_cd) T(structure_length)
-  T(qinffield) T(meteo_times) T(meteo_tauwu) T(meteo_tauwv) T(meteo_patm) T(meteo_prcp)
+  T(qinffield) T(scs_m) T(meteo_times) T(meteo_tauwu) T(meteo_tauwv) T(meteo_patm) T(meteo_prcp)
   T(cuv_index_uv) T(cuv_index_uv1) T(cuv_index_uv2) T(zbuv) T(zbuvmx) T(gn2uv)
   T(dxr) T(dyr) T(dxrm) T(dyrm) T(dxrinv) T(dyrinv) T(dxr2inv) T(dyr2inv) T(dxrinvc) T(dyrinvc) T(cell_area) T(nuvisc)
   T(z_zmin) T(z_zmax) T(z_volmax) T(z_dep) T(uv_zmin) T(uv_zmax) T(uv_navg_w) T(uv_fnfit)

@@ -24,7 +24,8 @@ enum : uint32_t {
   F_INFIL = 1u << 20,   // constant infiltration per cell, uniform or mapped (sfincs_infiltration.f90)
   // gridded / spiderweb meteo (sfincs_meteo.f90): update windows, wind stress per cell,
   // atmospheric pressure (baro), rain per cell, spin-up of the gridded fields
-  F_METEO3D = 1u << 21, F_MWIND = 1u << 22, F_PATM = 1u << 23, F_MPRCP = 1u << 24, F_MSPIN = 1u << 25
+  F_METEO3D = 1u << 21, F_MWIND = 1u << 22, F_PATM = 1u << 23, F_MPRCP = 1u << 24, F_MSPIN = 1u << 25,
+  F_CNA = 1u << 26,     // curve-number infiltration, method A (qinffield holds S; cumulative rain and infiltration)
 };
 
 constexpr int MAXREF = 8;
@@ -75,6 +76,7 @@ struct Consts {
   float wiggle_factor, wiggle_threshold, dtmax, dtmin, alpha, rhoa, rhow, btfilter, btrelax;
   float t0, t1, tspinup, zini, dx, dy, pi, twet_threshold;
   float dtwnd, gapres, pavbnd;   // meteo update interval, background pressure, boundary pressure
+  float sfacinf;                 // curve-number initial abstraction, fraction of S
   float dxrinv[MAXREF], dyrinv[MAXREF], dxr2inv[MAXREF], dyr2inv[MAXREF], dxrinvc[MAXREF], dyrinvc[MAXREF];
   float cell_area[MAXREF], dxrm[MAXREF], dyrm[MAXREF], nuvisc[MAXREF];
   uint32_t flags;

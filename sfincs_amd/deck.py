@@ -324,6 +324,9 @@ def write(sc, folder, netcdf=False):
             mesh = _lib.Model.quadtree(p, sc.qt)
         lines.append("qinffile = sfincs.qinf")
         np.ascontiguousarray(sc.qinf_map(mesh), np.float32).tofile(os.path.join(folder, "sfincs.qinf"))
+    if getattr(sc, "scs_map", None) is not None:    # float32 inches per active cell (sfincs_infiltration.f90:284-286)
+        lines.append("scsfile = sfincs.scs")
+        np.ascontiguousarray(sc.scs_map(_model_of(sc)), np.float32).tofile(os.path.join(folder, "sfincs.scs"))
     if sc.src is not None:
         x, y, t, q = sc.src
         lines += ["srcfile = sfincs.src", "disfile = sfincs.dis"]

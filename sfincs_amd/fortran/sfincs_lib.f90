@@ -28,6 +28,7 @@ module sfx_capi
       real(c_float) :: pavbnd, gapres
       integer(c_int32_t) :: spinup_meteo, amprblock, usespwprecip
       real(c_float) :: spw_merge_frac, factor_wind, factor_pres, factor_prcp, factor_spw_size
+      real(c_float) :: sfacinf
    end type
    type, bind(C) :: sfx_quadtree_input
       integer(c_int32_t) :: nr_points, nr_levels
@@ -118,6 +119,10 @@ module sfx_capi
       integer(c_int) function sfx_model_set_infiltration(m, q) bind(C, name='sfx_model_set_infiltration')
          import :: c_int, c_ptr
          type(c_ptr), value :: m, q
+      end function
+      integer(c_int) function sfx_model_set_curve_number(m, s) bind(C, name='sfx_model_set_curve_number')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m, s
       end function
       integer(c_int) function sfx_model_set_weirs(m, npoly, nv, x, y, par) bind(C, name='sfx_model_set_weirs')
          import :: c_int, c_ptr, c_int32_t
@@ -549,7 +554,7 @@ contains
       real(c_float) :: x0, y0, dx, dy, rot, dtmaxout, tdum
       character(256) :: tref, tstart, tstop, inputformat, depfile, mskfile, indexfile, qtrfile
       character(256) :: bndfile, bzsfile, bzifile, srcfile, disfile, prcfile, wndfile, rstfile, bdrfile, s
-      character(256) :: sbgfile, drnfile, thdfile, weirfile, qinffile, obsfile
+      character(256) :: sbgfile, drnfile, thdfile, weirfile, qinffile, obsfile, scsfile
       character(256) :: metfile(5)
       character(4), parameter :: metkind(5) = ['amu ', 'amv ', 'amp ', 'ampr', 'spw ']
       character(8), parameter :: metkey(5) = ['amufile ', 'amvfile ', 'ampfile ', 'amprfile', 'spwfile ']
@@ -660,6 +665,8 @@ contains
       ! constant infiltration (sfincs_input.f90:78,100,248): qinf in mm/h, converted by the model
       call inp_real('qinf', p%qinf); call inp_real('qinf_zmin', p%qinf_zmin)
       call inp_str('qinffile', qinffile, 'none')
+      ! curve-number infiltration, method A (sfincs_input.f90:102; sfincs_infiltration.f90:106-112)
+      call inp_str('scsfile', scsfile, 'none'); call inp_real('sfacinf', p%sfacinf)
       call inp_str('infiltrationfile', s, 'none')
       if (s(1:4) /= 'none') then
          call logmsg('Error   : '//trim(s)//' is a NetCDF infiltration file; this host reads qinf and qinffile only')
@@ -703,7 +710,10 @@ contains
          ierr = -1
          return
       endif
-      if (qinffile(1:4) /= 'none' .and. qtrfile(1:4) /= 'none') then   ! sfincs_infiltration.f90:181-185
+      ! a map type on a quadtree mesh (any type but 'con', which a qinf > 0 selects first;
+      ! sfincs_infiltration.f90:92-133,172-186)
+      if ((qinffile(1:4) /= 'none' .or. scsfile(1:4) /= 'none') .and. qtrfile(1:4) /= 'none' .and. &
+          p%qinf <= 0.0) then
          call logmsg('Error ! Infiltration input for quadtree mesh model can only be specified using the '// &
                      'infiltrationfile Netcdf format! !')
          ierr = -1
@@ -970,6 +980,26 @@ contains
          rc = sfx_model_set_infiltration(model, c_loc(tmp))
          if (rc < 0) then
             call fail('infiltration')
+            ierr = -1
+            return
+         endif
+         deallocate(tmp)
+      endif
+      ! curve-number retention S, float32 inches per active cell (sfincs_infiltration.f90:277-286)
+      if (scsfile(1:4) /= 'none') then
+         if (allocated(tmp)) deallocate(tmp)
+         allocate(tmp(host_np))
+         open(newunit=u, file=trim(scsfile), form='unformatted', access='stream', status='old', iostat=ios)
+         if (ios == 0) read(u, iostat=ios) tmp
+         if (ios /= 0) then
+            call logmsg('Error   : cannot read scs file '//trim(scsfile))
+            ierr = -1
+            return
+         endif
+         close(u)
+         rc = sfx_model_set_curve_number(model, c_loc(tmp))
+         if (rc < 0) then
+            call fail('curve number')
             ierr = -1
             return
          endif

@@ -46,6 +46,7 @@ class Scenario:
     thd: list = None                      # thin dams: polylines, each (n, 2) x, y
     weirs: list = None                    # weirs: polylines, each (n, 4) x, y, crest level, Cd
     qinf_map: object = None               # callable(model) -> infiltration per cell (mm/h), or None
+    scs_map: object = None                # callable(model) -> curve-number retention S per cell (inches)
     meteo: dict = None                    # gridded meteo: "amu", "amv", "amp", "ampr" grids, "spw" spiderweb
     rotation: float = 0.0                 # regular mesh rotation (degrees)
 
@@ -87,6 +88,8 @@ class Scenario:
             m.set_storage_volume(self.storage(m))
         if self.qinf_map is not None:
             m.set_infiltration(self.qinf_map(m))
+        if self.scs_map is not None:
+            m.set_curve_number(self.scs_map(m))
         for kind, g in (self.meteo or {}).items():
             if kind == "spw":
                 m.set_spiderweb(**g)
@@ -453,6 +456,17 @@ def with_infiltration(sc, qinf=None, qinf_zmin=None, qinf_map=None):
         sc.params = dict(sc.params, qinf_zmin=float(qinf_zmin))
     sc.qinf_map = qinf_map
     sc.name += "_inf"
+    return sc
+
+
+def with_curve_number(sc, scs_map, sfacinf=None):
+    """Adds curve-number infiltration, method A (scsfile): `scs_map(model)` gives the maximum
+    retention S in inches per active cell; `sfacinf` the initial abstraction fraction. It
+    acts only where the scenario has rain."""
+    if sfacinf is not None:
+        sc.params = dict(sc.params, sfacinf=float(sfacinf))
+    sc.scs_map = scs_map
+    sc.name += "_cn"
     return sc
 
 

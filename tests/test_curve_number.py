@@ -1,0 +1,108 @@
+"""Curve-number infiltration, method A ('cna': scsfile; initialize_infiltration
+sfincs_infiltration.f90:106-112,260-293, update_infiltration_map :678-722).
+
+The reference's infiltration module reads NetCDF maps, so it is not compiled into
+oracle/_ref. Its 'cna' branch is pinned here by the SCS runoff relation it implements:
+with cumulative rain P and initial abstraction Ia = sfacinf S, all rain infiltrates while
+P <= Ia, after which the cumulative infiltration is P - (P - Ia)^2 / (P - Ia + S); the
+cumulative rain is the rain rate times the elapsed time; wet and dry cells alike; the
+retention map converts from inches by 0.0254. The GPU then runs bit for bit with the
+oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import Q_TOL, ZS_TOL, compare_state, gpu_for, oracle_for
+from sfincs_amd import workloads as W
+
+F = np.float32
+
+
+def _basin(S_inch, rain=40.0, t1=3600.0, sfacinf=None):
+    nmax, mmax = 8, 10
+    zb = np.full((mmax, nmax), -0.5, F)
+    zb[: mmax // 2] = 0.4                        # a dry half
+    sc = W.Scenario(name="cn_basin", params=dict(t0=0.0, t1=t1, zsini=0.0), nmax=nmax, mmax=mmax, dx=20.0, dy=20.0,
+                    msk=np.ones(nmax * mmax, np.int8), zb_grid=zb, t_end=t1,
+                    rain=(np.array([0.0, 1.0e5], F), np.array([rain, rain], F)))
+    return W.with_curve_number(sc, lambda m: np.resize(np.asarray(S_inch, F), m.sizes()["np"]), sfacinf=sfacinf)
+
+
+def test_retention_converts_from_inches_and_needs_rain():
+    sc = _basin([0.0, 0.5, 1.0, 2.0, 4.0])
+    m = sc.build_model()
+    assert np.array_equal(m.table("scs_m"), np.resize(np.array([0.0, 0.5, 1.0, 2.0, 4.0], F), m.sizes()["np"]) * F(0.0254))
+    assert m.table("qinffield").size == 0
+    sc.rain = None
+    assert sc.build_model().table("scs_m").size == 0
+    sc = W.with_infiltration(_basin([1.0]), qinf=5.0, qinf_zmin=-100.0)      # 'con' first (:92-112)
+    m = sc.build_model()
+    assert m.table("scs_m").size == 0 and m.table("qinffield").size > 0
+
+
+@pytest.mark.parametrize("sfacinf", [None, 0.05])
+def test_cumulative_infiltration_follows_the_scs_relation(sfacinf):
+    S_inch = np.array([0.0, 0.02, 0.05, 0.2, 1.0, 3.0, 20.0], F)
+    sc = _basin(S_inch, rain=40.0, sfacinf=sfacinf)
+    m = sc.build_model()
+    orc = O.OracleRun(m).configure(sc.params, m)
+    S = m.table("scs_m").astype(np.float64)
+    lam = 0.2 if sfacinf is None else sfacinf
+    R = 40.0 / 3.6e6
+    crossed = np.zeros(S.size, bool)
+    below = np.zeros(S.size, bool)
+    for step in range(400):
+        orc.update(1e30, 1)
+        t = orc.get("t")
+        P = orc.state("cumprcp").astype(np.float64)
+        assert np.allclose(P, R * t, rtol=2e-5, atol=0), step
+        Ia = lam * S
+        expect = np.where(P > Ia, P - (P - Ia) ** 2 / np.maximum(P - Ia + S, 1e-30), P)
+        got = orc.state("cuminf").astype(np.float64)
+        assert np.all(np.abs(got - expect) <= 2e-5 * P + 1e-9), (step, np.max(np.abs(got - expect)))
+        crossed |= P > Ia
+        below |= P <= Ia
+    assert crossed.any() and below.any()      # both branches taken
+    # runoff: the net rain of a cell past Ia is positive and below the rain
+    net = orc.state("netprcp")
+    assert np.all(net >= -1e-12) and np.all(net <= R * (1 + 1e-6))
+
+
+# ------------------------------------------------------------------------------ GPU
+def cn_case(kind):
+    if kind == "regular":
+        sc = W.regular_case(32, 40, dx=50.0, seed=95, t_end=1800.0, tide_amp=1.0, rain_mmh=60.0)
+    else:
+        sc = W.subgrid_case(26, 30, dx=25.0, seed=96, t_end=1800.0, nsrc=1, rain_mmh=60.0)
+    return W.with_curve_number(sc, lambda m: (0.02 + 10.0 * np.abs(np.sin(m.table("z_xz") / 150.0))).astype(F))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+def test_curve_number_matches_oracle(kind):
+    print("case curve number", kind, flush=True)
+    sc = cn_case(kind)
+    m, orc = oracle_for(sc)
+    inst = gpu_for(m)
+    b0 = inst.balance()
+    rc, st = inst.update(-1000.0)
+    orc.update(sc.t_end)
+    d = compare_state(inst, orc)
+    print(sc.name, st["nt"], int(orc.get("nt")), d, flush=True)
+    assert rc == 0 and st["nt"] == int(orc.get("nt"))
+    assert d["zs"] <= ZS_TOL and d["q"] <= Q_TOL, d
+    for f in ("zs", "q", "uv", "zsmax", "cumprcp", "cuminf") + (("z_volume",) if m.sizes()["subgrid"] else ()):
+        assert np.array_equal(inst.get(f), orc.state(f)), f
+    P = inst.get("cumprcp")
+    assert np.any(inst.get("cuminf") < P * 0.99) and np.any(inst.get("cuminf") >= P * 0.999)
+    b1 = inst.balance()
+    dv = b1["volume"] - b0["volume"] + (b0["storage"] - b1["storage"])
+    inflow = sum(b1[k] - b0[k] for k in ("boundary", "sources", "rain", "qext"))
+    kcs = m.table("kcs")
+    if m.sizes()["subgrid"]:
+        water = float(np.sum(inst.get("z_volume")[kcs == 1]))
+    else:
+        area = m.table("cell_area")[m.table("z_flags_iref") - 1].astype(np.float64)
+        water = float(np.sum((np.maximum(inst.get("zs") - m.table("zb"), 0.0) * area)[kcs == 1]))
+    assert abs(dv - inflow) <= 1e-6 * water, (dv, inflow, water)

@@ -84,7 +84,7 @@ SUBGRID_TABLES = ("z_zmin", "z_zmax", "z_volmax", "z_dep", "uv_zmin", "uv_zmax",
 @needs_host
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "wind_sources", "store_switches", "river", "ig_waves",
                                   "subgrid_binary", "drainage", "structures", "infiltration", "meteo",
-                                  "quadtree_netcdf", "subgrid_netcdf"])
+                                  "quadtree_netcdf", "subgrid_netcdf", "curve_number"])
 def test_deck_reader_builds_the_same_model(kind, tmp_path):
     """The Fortran deck reader (no GPU) builds, table for table and byte for byte, the
     model that the Python builder makes from the same scenario (parameters included)."""
@@ -105,6 +105,9 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
                                    tmp_path / "gen.sbg")
     elif kind == "subgrid_netcdf":   # sbgfile as NetCDF
         sc = W.subgrid_case(22, 26, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0)
+    elif kind == "curve_number":     # scsfile and sfacinf
+        sc = W.with_curve_number(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, rain_mmh=20.0),
+                                 lambda m: (m.table("z_xz") / 3000.0).astype(np.float32), sfacinf=0.1)
     elif kind == "drainage":
         sc = W.beach_drainage(W.regular_case(40, 48, dx=50.0, seed=71, t_end=900.0, tide_amp=2.5,
                                              structure_relax=7.5))
@@ -144,6 +147,9 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
     if kind == "structures":
         extra = ("thindam_uv_index", "structure_uv_index", "structure_height", "structure_cd", "structure_length")
         assert mp.sizes()["nstruc"] > 10 and mp.sizes()["nthd"] > 10
+    if kind == "curve_number":
+        extra = ("scs_m",)
+        assert np.count_nonzero(mp.table("scs_m")) > 100 and mp.params().sfacinf == np.float32(0.1)
     if kind == "infiltration":
         extra = ("qinffield",)
         assert np.count_nonzero(mp.table("qinffield")) > 100

@@ -73,13 +73,16 @@ def test_cumulative_infiltration_follows_the_scs_relation(sfacinf):
 def cn_case(kind):
     if kind == "regular":
         sc = W.regular_case(32, 40, dx=50.0, seed=95, t_end=1800.0, tide_amp=1.0, rain_mmh=60.0)
+    elif kind == "quadtree":
+        sc = W.quadtree_case(10, 14, nlev=3, dx0=100.0, seed=97, t_end=1800.0, surge_amp=2.0)
+        sc.rain = (np.array([0.0, 600.0, 1.0e5], F), np.array([80.0, 20.0, 20.0], F))
     else:
         sc = W.subgrid_case(26, 30, dx=25.0, seed=96, t_end=1800.0, nsrc=1, rain_mmh=60.0)
     return W.with_curve_number(sc, lambda m: (0.02 + 10.0 * np.abs(np.sin(m.table("z_xz") / 150.0))).astype(F))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["regular", "subgrid"])
+@pytest.mark.parametrize("kind", ["regular", "subgrid", "quadtree"])
 def test_curve_number_matches_oracle(kind):
     print("case curve number", kind, flush=True)
     sc = cn_case(kind)

@@ -171,7 +171,7 @@ def test_deck_reader_builds_the_same_model(kind, tmp_path):
 @needs_host
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["regular", "quadtree", "subgrid_binary", "drainage", "structures",
-                                  "infiltration", "meteo", "quadtree_netcdf", "subgrid_netcdf"])
+                                  "infiltration", "meteo", "quadtree_netcdf", "subgrid_netcdf", "curve_number"])
 def test_fortran_host_matches_python_path(kind, tmp_path):
     print(f"case fortran host {kind}", flush=True)
     from sfincs_amd import _lib
@@ -194,6 +194,9 @@ def test_fortran_host_matches_python_path(kind, tmp_path):
         W.with_spiderweb(W.with_gridded_meteo(sc, t_step=600.0))
     elif kind == "subgrid_netcdf":   # NetCDF sbgfile
         sc = W.subgrid_case(30, 34, dx=25.0, seed=40, t_end=900.0, nsrc=2, rain_mmh=10.0)
+    elif kind == "curve_number":     # scsfile
+        sc = W.with_curve_number(W.regular_case(28, 36, dx=50.0, seed=41, t_end=900.0, rain_mmh=40.0),
+                                 lambda m: (m.table("z_xz") / 300.0).astype(np.float32))
     else:                            # legacy binary or NetCDF qtrfile
         sc = W.quadtree_case(8, 10, nlev=3, dx0=100.0, seed=42, t_end=900.0)
     d = deck.write(sc, str(tmp_path), netcdf=kind.endswith("netcdf"))
